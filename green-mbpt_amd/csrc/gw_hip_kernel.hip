@@ -1,0 +1,518 @@
+// Host orchestration of the GW self-energy on one MI355X, and the C ABI (include/gwhip.h).
+#include "gw_hip_kernel.hpp"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <memory>
+
+#include "../../include/gwhip.h"
+#include "gw_kernels.hpp"
+#include "zgemm_mfma.hpp"
+
+namespace gwhip {
+
+namespace {
+
+int k_shift(int k, int q, const std::array<int64_t, 3>& mesh, int sign) {
+  const int n1 = (int)mesh[1], n2 = (int)mesh[2];
+  int a[3] = {k / (n1 * n2), (k / n2) % n1, k % n2};
+  int b[3] = {q / (n1 * n2), (q / n2) % n1, q % n2};
+  int c[3];
+  for (int i = 0; i < 3; ++i) {
+    const int n = (int)mesh[i];
+    c[i] = ((a[i] + sign * b[i]) % n + n) % n;
+  }
+  return (c[0] * n1 + c[1]) * n2 + c[2];
+}
+
+int query_max_lds(int device) {
+  int v = 65536;
+  int a = 0;
+  if (hipDeviceGetAttribute(&a, hipDeviceAttributeMaxSharedMemoryPerBlock, device) == hipSuccess && a > v) v = a;
+#if HIP_VERSION_MAJOR >= 6
+  int b = 0;
+  if (hipDeviceGetAttribute(&b, hipDeviceAttributeSharedMemPerBlockOptin, device) == hipSuccess && b > v) v = b;
+#endif
+  (void)hipGetLastError();
+  return v;
+}
+
+template <typename K>
+void allow_lds(K kernel, size_t bytes) {
+  if (bytes > 65536) GWHIP_CHECK(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+}
+
+}  // namespace
+
+template <typename T>
+T* gw_hip_kernel::dalloc(size_t count) {
+  void* p = nullptr;
+  const size_t bytes = count * sizeof(T);
+  hipError_t e = hipMalloc(&p, bytes);
+  if (e != hipSuccess) {
+    (void)hipGetLastError();
+    throw status_error(GWHIP_ERR_OOM, "hipMalloc of " + std::to_string(bytes) + " bytes failed: " + hipGetErrorString(e));
+  }
+  bytes_ += bytes;
+  return static_cast<T*>(p);
+}
+
+gw_hip_kernel::gw_hip_kernel(const Dims& d, int device) : d_(d), device_(device) {
+  if (d_.nk <= 0 || d_.nao <= 0 || d_.nso <= 0 || d_.ns <= 0 || d_.NQ <= 0 || d_.nts <= 0 || d_.nw_b <= 0)
+    throw status_error(GWHIP_ERR_ARG, "all dimensions must be positive");
+  if (d_.mesh[0] * d_.mesh[1] * d_.mesh[2] != d_.nk) throw status_error(GWHIP_ERR_ARG, "mesh does not multiply to nk");
+  if (d_.nso != d_.nao && d_.nso != 2 * d_.nao) throw status_error(GWHIP_ERR_ARG, "nso must be nao or 2*nao");
+  if (d_.ns != 1 && d_.ns != 2) throw status_error(GWHIP_ERR_ARG, "ns must be 1 or 2");
+  d_.X2C = d_.nso != d_.nao;
+  if (d_.X2C && d_.ns != 1) throw status_error(GWHIP_ERR_ARG, "X2C requires ns == 1");
+  if (d_.nts % 2 != 0) throw status_error(GWHIP_ERR_WRONG_GRID, "Number of tau points should be even!!!");
+  d_.pns = d_.nso / d_.nao;
+  d_.pseudo_ns = d_.X2C ? 4 : d_.ns;
+  d_.prefactor = (d_.ns == 2 || d_.X2C) ? 1.0 : 2.0;
+  if (d_.nao > 45) throw status_error(GWHIP_ERR_UNSUPPORTED, "nao > 45 not supported by the epilogue kernels");
+
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) {
+    (void)hipGetLastError();
+    throw status_error(GWHIP_ERR_DEVICE, "no HIP device " + std::to_string(device));
+  }
+  GWHIP_CHECK(hipSetDevice(device_));
+  GWHIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  max_lds_ = query_max_lds(device_);
+
+  const int64_t nk = d_.nk, NQ = d_.NQ, nao2 = d_.nao * d_.nao, NQ2 = NQ * NQ;
+  Vq_ = dalloc<double2>((size_t)nk * nk * NQ * nao2);
+  P_ = dalloc<double2>((size_t)nk * d_.nts * NQ2);
+  Tbt_fold_ = dalloc<double2>((size_t)d_.nw_b * (d_.nts / 2));
+  Ttb_ = dalloc<double2>((size_t)d_.nts * d_.nw_b);
+  kplus_tab_ = dalloc<int>((size_t)nk * nk);
+  kminus_tab_ = dalloc<int>((size_t)nk * nk);
+  flag_ = dalloc<int>(1);
+  build_maps();
+
+  // chunk sizes for the transient buffers from the memory left on the device
+  size_t free_b = 0, total_b = 0;
+  GWHIP_CHECK(hipMemGetInfo(&free_b, &total_b));
+  const double budget = 0.7 * (double)free_b;
+  const double per_q_dyson = 3.0 * d_.nw_b * NQ2 * 16.0;
+  const double per_t_scr = (double)nk * NQ * nao2 * 16.0;
+  const double dyson_share = std::min(0.3 * budget, per_q_dyson * nk);
+  qchunk_ = std::max<int64_t>(1, std::min<int64_t>(nk, (int64_t)(dyson_share / per_q_dyson)));
+  const double scr_budget = budget - qchunk_ * per_q_dyson;
+  int64_t tmax = std::max<int64_t>(1, (int64_t)(scr_budget / per_t_scr));
+  tchunk_sigma_ = std::min<int64_t>(d_.nts, tmax);
+  tchunk_p0_ = std::min<int64_t>(d_.nts / 2, tmax);
+  Pw_ = dalloc<double2>((size_t)qchunk_ * d_.nw_b * NQ2);
+  X_ = dalloc<double2>((size_t)qchunk_ * d_.nw_b * NQ2);
+  Y_ = dalloc<double2>((size_t)qchunk_ * d_.nw_b * NQ2);
+  scratch_ = dalloc<double2>((size_t)std::max(tchunk_sigma_, tchunk_p0_) * nk * NQ * nao2);
+}
+
+gw_hip_kernel::~gw_hip_kernel() {
+  (void)hipSetDevice(device_);
+  if (stream_) (void)hipStreamSynchronize(stream_);
+  for (void* p : {(void*)Vq_, (void*)P_, (void*)Pw_, (void*)X_, (void*)Y_, (void*)scratch_, (void*)Tbt_fold_,
+                  (void*)Ttb_, (void*)G_dev_, (void*)S_dev_, (void*)kplus_tab_, (void*)kminus_tab_, (void*)flag_})
+    if (p) (void)hipFree(p);
+  if (stream_) (void)hipStreamDestroy(stream_);
+}
+
+void gw_hip_kernel::build_maps() {
+  const int nk = (int)d_.nk;
+  std::vector<int> kp((size_t)nk * nk), kpq((size_t)nk * nk), kmq((size_t)nk * nk);
+  neg_q_.assign(nk, 0);
+  for (int q = 0; q < nk; ++q) {
+    neg_q_[q] = k_shift(0, q, d_.mesh, -1);
+    for (int k = 0; k < nk; ++k) {
+      kp[(size_t)k * nk + q] = k_shift(k, q, d_.mesh, +1);   // [k][q] for the V expansion
+      kpq[(size_t)q * nk + k] = k_shift(k, q, d_.mesh, +1);  // [q][k]: k + q
+      kmq[(size_t)q * nk + k] = k_shift(k, q, d_.mesh, -1);  // [q][k]: k - q
+    }
+  }
+  kplus_host_ = kp;
+  // kplus_tab_ holds [q][k] (k + q); the [k][q] form is only needed transiently
+  GWHIP_CHECK(hipMemcpy(kplus_tab_, kpq.data(), kpq.size() * sizeof(int), hipMemcpyHostToDevice));
+  GWHIP_CHECK(hipMemcpy(kminus_tab_, kmq.data(), kmq.size() * sizeof(int), hipMemcpyHostToDevice));
+}
+
+void gw_hip_kernel::set_grid(const double2* T_bt, const double2* T_tb) {
+  if (!T_bt || !T_tb) throw status_error(GWHIP_ERR_ARG, "null transform matrix");
+  const int64_t nts = d_.nts, half = nts / 2, nwb = d_.nw_b;
+  // fold P(beta - tau) = P(tau): T'[n][t] = T[n][t] + T[n][nts-1-t], t < nts/2
+  std::vector<double2> fold((size_t)nwb * half);
+  for (int64_t n = 0; n < nwb; ++n)
+    for (int64_t t = 0; t < half; ++t) {
+      const double2 a = T_bt[n * nts + t], b = T_bt[n * nts + nts - 1 - t];
+      fold[n * half + t] = make_double2(a.x + b.x, a.y + b.y);
+    }
+  GWHIP_CHECK(hipSetDevice(device_));
+  GWHIP_CHECK(hipMemcpy(Tbt_fold_, fold.data(), fold.size() * sizeof(double2), hipMemcpyHostToDevice));
+  GWHIP_CHECK(hipMemcpy(Ttb_, T_tb, (size_t)nts * nwb * sizeof(double2), hipMemcpyHostToDevice));
+  grid_set_ = true;
+}
+
+void gw_hip_kernel::upload_V(const double2* V_packed) {
+  if (!V_packed) throw status_error(GWHIP_ERR_ARG, "null V");
+  GWHIP_CHECK(hipSetDevice(device_));
+  const int64_t nk = d_.nk, NQ = d_.NQ, nao2 = d_.nao * d_.nao;
+  const size_t npk = (size_t)nk * (nk + 1) / 2 * NQ * nao2;
+  double2* tmp = nullptr;
+  int* tab = nullptr;
+  if (hipMalloc(&tmp, npk * sizeof(double2)) != hipSuccess) {
+    (void)hipGetLastError();
+    throw status_error(GWHIP_ERR_OOM, "no room for the packed V staging buffer");
+  }
+  hipError_t e = hipMalloc(&tab, kplus_host_.size() * sizeof(int));
+  if (e == hipSuccess) e = hipMemcpy(tmp, V_packed, npk * sizeof(double2), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(tab, kplus_host_.data(), kplus_host_.size() * sizeof(int), hipMemcpyHostToDevice);
+  if (e == hipSuccess) {
+    const int64_t total = nk * nk * NQ * nao2;
+    hipLaunchKernelGGL(expand_v_kernel, dim3((unsigned)ceil_div(total, 256)), dim3(256), 0, stream_, tmp, Vq_, tab,
+                       (int)nk, (int)NQ, (int)d_.nao, total);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+  }
+  (void)hipFree(tmp);
+  if (tab) (void)hipFree(tab);
+  GWHIP_CHECK(e);
+  v_set_ = true;
+}
+
+void gw_hip_kernel::generate_V(uint64_t seed, double sigma) {
+  GWHIP_CHECK(hipSetDevice(device_));
+  const int64_t nk = d_.nk, NQ = d_.NQ, nao2 = d_.nao * d_.nao;
+  int* tab = nullptr;
+  GWHIP_CHECK(hipMalloc(&tab, kplus_host_.size() * sizeof(int)));
+  hipError_t e = hipMemcpy(tab, kplus_host_.data(), kplus_host_.size() * sizeof(int), hipMemcpyHostToDevice);
+  // key = splitmix64(seed ^ splitmix64(stream)), stream id of V = 0x5611 (gwhip.inputs.STREAM_V)
+  auto sm = [](uint64_t x) {
+    uint64_t z = x + 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    return z ^ (z >> 31);
+  };
+  const uint64_t key = sm(seed ^ sm(0x5611ull));
+  if (e == hipSuccess) {
+    const int64_t total = nk * nk * NQ * nao2;
+    hipLaunchKernelGGL(generate_v_kernel, dim3((unsigned)ceil_div(total, 256)), dim3(256), 0, stream_, Vq_, tab,
+                       (int)nk, (int)NQ, (int)d_.nao, total, key, sigma);
+    e = hipGetLastError();
+    if (e == hipSuccess) e = hipStreamSynchronize(stream_);
+  }
+  (void)hipFree(tab);
+  GWHIP_CHECK(e);
+  v_set_ = true;
+}
+
+template <typename F>
+void gw_hip_kernel::timed(int which, F&& launch) {
+  hipEvent_t a, b;
+  GWHIP_CHECK(hipEventCreate(&a));
+  GWHIP_CHECK(hipEventCreate(&b));
+  GWHIP_CHECK(hipEventRecord(a, stream_));
+  launch();
+  GWHIP_CHECK(hipEventRecord(b, stream_));
+  GWHIP_CHECK(hipEventSynchronize(b));
+  float ms = 0.f;
+  GWHIP_CHECK(hipEventElapsedTime(&ms, a, b));
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  kstat_[which].launches += 1;
+  kstat_[which].ms += ms;
+}
+
+void gw_hip_kernel::stage_p0(const double2* G) {
+  const int64_t nk = d_.nk, NQ = d_.NQ, nao2 = d_.nao * d_.nao, NQ2 = NQ * NQ, half = d_.nts / 2;
+  DevDims dd{(int)d_.nts, (int)d_.ns, (int)nk, (int)d_.nso, (int)d_.nao, (int)NQ, (int)d_.pns, (int)d_.pseudo_ns};
+  const int64_t lds_unit = nao2 * (int64_t)sizeof(double2);
+  int QC = (int)std::min<int64_t>(max_lds_ / lds_unit / 2 - 1, (int64_t)Z_MAXO * 256 / nao2);
+  QC = std::max(1, std::min<int>(QC, (int)NQ));
+  const size_t lds = (size_t)(2 + 2 * QC) * lds_unit;
+  if ((int64_t)lds > max_lds_) throw status_error(GWHIP_ERR_UNSUPPORTED, "nao too large for the P0 prep kernel");
+  allow_lds(z_prep_kernel, lds);
+
+  for (int64_t q = 0; q < nk; ++q) {
+    const double2* Vq = Vq_ + q * nk * NQ * nao2;
+    for (int64_t t0 = 0; t0 < half; t0 += tchunk_p0_) {
+      const int64_t tc = std::min(tchunk_p0_, half - t0);
+      timed(K_P0_PREP, [&] {
+        hipLaunchKernelGGL(z_prep_kernel, dim3((unsigned)ceil_div(NQ, QC), (unsigned)nk, (unsigned)tc), dim3(256), lds,
+                           stream_, dd, G, Vq, scratch_, kplus_tab_ + q * nk, (int)t0, QC);
+        GWHIP_CHECK(hipGetLastError());
+      });
+      ZGemmArgs a{};
+      a.M = (int)NQ; a.N = (int)NQ; a.K = (int)(nk * nao2); a.batch = (int)tc;
+      a.kin = (int)nao2; a.nin = (int)NQ;
+      a.A = scratch_; a.sAb = nk * NQ * nao2; a.sAm = nao2; a.sAko = NQ * nao2; a.sAki = 1;
+      a.B = Vq; a.sBb = 0; a.sBko = NQ * nao2; a.sBki = 1; a.sBno = 0; a.sBni = nao2;
+      a.C = P_ + (q * d_.nts + t0) * NQ2; a.sCb = NQ2; a.sCm = NQ; a.sCno = 0; a.sCni = 1;
+      a.alpha = make_double2(-d_.prefactor / (double)nk, 0.0);
+      a.accumulate = 0; a.conjA = 0; a.conjB = 1;
+      timed(K_P0_GEMM, [&] { zgemm_f64(a, true, true, stream_); });
+    }
+    // Hermitise the half grid of this q (src/gw_cpu_kernel.cpp:217-225)
+    hipLaunchKernelGGL(hermitize_kernel, dim3((unsigned)ceil_div(NQ2, 256), (unsigned)half), dim3(256), 0, stream_,
+                       P_ + q * d_.nts * NQ2, (int)NQ, NQ2);
+    GWHIP_CHECK(hipGetLastError());
+  }
+}
+
+void gw_hip_kernel::stage_dyson() {
+  const int64_t nk = d_.nk, NQ = d_.NQ, NQ2 = NQ * NQ, half = d_.nts / 2, nwb = d_.nw_b, nts = d_.nts;
+  int nb = GJ_NB;
+  size_t gj_lds = (size_t)(GJ_NB * GJ_NB + NQ * nb) * sizeof(double2);
+  if ((int64_t)gj_lds > max_lds_) {
+    nb = 8;
+    gj_lds = (size_t)(GJ_NB * GJ_NB + NQ * nb) * sizeof(double2);
+  }
+  if ((int64_t)gj_lds > max_lds_) throw status_error(GWHIP_ERR_UNSUPPORTED, "NQ too large for the Dyson kernel");
+  allow_lds(gj_inverse_kernel, gj_lds);
+  const int gj_threads = (int)std::min<int64_t>(1024, std::max<int64_t>(256, ceil_div(NQ, 64) * 64));
+
+  hipEvent_t e0, e1, e2, e3;
+  GWHIP_CHECK(hipEventCreate(&e0)); GWHIP_CHECK(hipEventCreate(&e1));
+  GWHIP_CHECK(hipEventCreate(&e2)); GWHIP_CHECK(hipEventCreate(&e3));
+  float tw = 0.f, bse = 0.f, wt = 0.f;
+  for (int64_t q0 = 0; q0 < nk; q0 += qchunk_) {
+    const int64_t qc = std::min(qchunk_, nk - q0);
+    GWHIP_CHECK(hipEventRecord(e0, stream_));
+    {  // P0(tau_f) -> P0(i Omega_b) with the folded matrix
+      ZGemmArgs a{};
+      a.M = (int)nwb; a.N = (int)NQ2; a.K = (int)half; a.batch = (int)qc; a.kin = (int)half; a.nin = (int)NQ2;
+      a.A = Tbt_fold_; a.sAb = 0; a.sAm = half; a.sAko = 0; a.sAki = 1;
+      a.B = P_ + q0 * nts * NQ2; a.sBb = nts * NQ2; a.sBko = 0; a.sBki = NQ2; a.sBno = 0; a.sBni = 1;
+      a.C = Pw_; a.sCb = nwb * NQ2; a.sCm = NQ2; a.sCno = 0; a.sCni = 1;
+      a.alpha = make_double2(1.0, 0.0);
+      zgemm_f64(a, true, false, stream_);
+    }
+    GWHIP_CHECK(hipEventRecord(e1, stream_));
+    const int64_t nmat = qc * nwb;
+    hipLaunchKernelGGL(identity_minus_kernel, dim3((unsigned)ceil_div(nmat * NQ2, 256)), dim3(256), 0, stream_, X_, Pw_,
+                       (int)NQ, nmat);
+    GWHIP_CHECK(hipGetLastError());
+    timed(K_DYSON_INV, [&] {
+      hipLaunchKernelGGL(gj_inverse_kernel, dim3((unsigned)nmat), dim3(gj_threads), gj_lds, stream_, X_, (int)NQ, NQ2, nb);
+      GWHIP_CHECK(hipGetLastError());
+    });
+    hipLaunchKernelGGL(hermitize_kernel, dim3((unsigned)ceil_div(NQ2, 256), (unsigned)nmat), dim3(256), 0, stream_, X_,
+                       (int)NQ, NQ2);
+    GWHIP_CHECK(hipGetLastError());
+    {  // P-tilde(i Omega) = X P0(i Omega)
+      ZGemmArgs a{};
+      a.M = (int)NQ; a.N = (int)NQ; a.K = (int)NQ; a.batch = (int)nmat; a.kin = (int)NQ; a.nin = (int)NQ;
+      a.A = X_; a.sAb = NQ2; a.sAm = NQ; a.sAko = 0; a.sAki = 1;
+      a.B = Pw_; a.sBb = NQ2; a.sBko = 0; a.sBki = NQ; a.sBno = 0; a.sBni = 1;
+      a.C = Y_; a.sCb = NQ2; a.sCm = NQ; a.sCno = 0; a.sCni = 1;
+      a.alpha = make_double2(1.0, 0.0);
+      zgemm_f64(a, true, false, stream_);
+    }
+    GWHIP_CHECK(hipEventRecord(e2, stream_));
+    {  // P-tilde(i Omega_b) -> P-tilde(tau_f), all nts
+      ZGemmArgs a{};
+      a.M = (int)nts; a.N = (int)NQ2; a.K = (int)nwb; a.batch = (int)qc; a.kin = (int)nwb; a.nin = (int)NQ2;
+      a.A = Ttb_; a.sAb = 0; a.sAm = nwb; a.sAko = 0; a.sAki = 1;
+      a.B = Y_; a.sBb = nwb * NQ2; a.sBko = 0; a.sBki = NQ2; a.sBno = 0; a.sBni = 1;
+      a.C = P_ + q0 * nts * NQ2; a.sCb = nts * NQ2; a.sCm = NQ2; a.sCno = 0; a.sCni = 1;
+      a.alpha = make_double2(1.0, 0.0);
+      zgemm_f64(a, true, false, stream_);
+    }
+    GWHIP_CHECK(hipEventRecord(e3, stream_));
+    GWHIP_CHECK(hipEventSynchronize(e3));
+    float x;
+    GWHIP_CHECK(hipEventElapsedTime(&x, e0, e1)); tw += x;
+    GWHIP_CHECK(hipEventElapsedTime(&x, e1, e2)); bse += x;
+    GWHIP_CHECK(hipEventElapsedTime(&x, e2, e3)); wt += x;
+  }
+  region_ms_[R_TW] += tw;
+  region_ms_[R_BSE] += bse;
+  region_ms_[R_WT] += wt;
+  (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); (void)hipEventDestroy(e2); (void)hipEventDestroy(e3);
+}
+
+void gw_hip_kernel::stage_sigma(const double2* G, double2* Sigma) {
+  const int64_t nk = d_.nk, NQ = d_.NQ, nao2 = d_.nao * d_.nao, NQ2 = NQ * NQ, nts = d_.nts;
+  DevDims dd{(int)nts, (int)d_.ns, (int)nk, (int)d_.nso, (int)d_.nao, (int)NQ, (int)d_.pns, (int)d_.pseudo_ns};
+  const int64_t lds_unit = nao2 * (int64_t)sizeof(double2);
+  int QC = (int)((max_lds_ / lds_unit - 1) / 3);
+  QC = std::max(1, std::min<int>(QC, (int)NQ));
+  const size_t lds = (size_t)(1 + 3 * QC) * lds_unit;
+  if ((int64_t)lds > max_lds_) throw status_error(GWHIP_ERR_UNSUPPORTED, "nao too large for the Sigma epilogue kernel");
+  allow_lds(sigma_epilogue_kernel, lds);
+  const double scale = -1.0 / (double)nk;
+
+  for (int64_t q = 0; q < nk; ++q) {
+    const double2* Vneg = Vq_ + (int64_t)neg_q_[q] * nk * NQ * nao2;  // V(k, k - q)
+    for (int64_t t0 = 0; t0 < nts; t0 += tchunk_sigma_) {
+      const int64_t tc = std::min(tchunk_sigma_, nts - t0);
+      ZGemmArgs a{};
+      a.M = (int)NQ; a.N = (int)(nk * nao2); a.K = (int)NQ; a.batch = (int)tc; a.kin = (int)NQ; a.nin = (int)nao2;
+      a.A = P_ + (q * nts + t0) * NQ2; a.sAb = NQ2; a.sAm = NQ; a.sAko = 0; a.sAki = 1;
+      a.B = Vneg; a.sBb = 0; a.sBko = 0; a.sBki = nao2; a.sBno = NQ * nao2; a.sBni = 1;
+      a.C = scratch_; a.sCb = nk * NQ * nao2; a.sCm = nao2; a.sCno = NQ * nao2; a.sCni = 1;
+      a.alpha = make_double2(1.0, 0.0);
+      a.accumulate = 0; a.conjA = 0; a.conjB = 1;
+      timed(K_SIGMA_GEMM, [&] { zgemm_f64(a, true, false, stream_); });
+      timed(K_SIGMA_EPI, [&] {
+        hipLaunchKernelGGL(sigma_epilogue_kernel, dim3((unsigned)nk, (unsigned)tc), dim3(256), lds, stream_, dd, G, Vneg,
+                           scratch_, Sigma, kminus_tab_ + q * nk, (int)t0, QC, scale);
+        GWHIP_CHECK(hipGetLastError());
+      });
+    }
+  }
+}
+
+void gw_hip_kernel::solve_device(const double2* G, double2* Sigma, int precision) {
+  if (!grid_set_) throw status_error(GWHIP_ERR_STATE, "transform grid not set (gwhip_set_grid)");
+  if (!v_set_) throw status_error(GWHIP_ERR_STATE, "integrals not set (gwhip_upload_V / gwhip_generate_V)");
+  if (!G || !Sigma) throw status_error(GWHIP_ERR_ARG, "null G or Sigma");
+  if (precision != GWHIP_PREC_FP64) throw status_error(GWHIP_ERR_UNSUPPORTED, "only FP64 precision is implemented");
+  GWHIP_CHECK(hipSetDevice(device_));
+  for (double& r : region_ms_) r = 0.0;
+  for (auto& k : kstat_) k = KernelStat{};
+  const size_t nsig = (size_t)d_.nts * d_.ns * d_.nk * d_.nso * d_.nso;
+
+  hipEvent_t t0, t1, t2, t3;
+  GWHIP_CHECK(hipEventCreate(&t0)); GWHIP_CHECK(hipEventCreate(&t1));
+  GWHIP_CHECK(hipEventCreate(&t2)); GWHIP_CHECK(hipEventCreate(&t3));
+  GWHIP_CHECK(hipEventRecord(t0, stream_));
+  GWHIP_CHECK(hipMemsetAsync(Sigma, 0, nsig * sizeof(double2), stream_));
+  stage_p0(G);
+  GWHIP_CHECK(hipEventRecord(t1, stream_));
+  stage_dyson();
+  GWHIP_CHECK(hipEventRecord(t2, stream_));
+  stage_sigma(G, Sigma);
+  GWHIP_CHECK(hipEventRecord(t3, stream_));
+  GWHIP_CHECK(hipMemsetAsync(flag_, 0, sizeof(int), stream_));
+  hipLaunchKernelGGL(nonfinite_kernel, dim3(1024), dim3(256), 0, stream_, Sigma, (int64_t)nsig, flag_);
+  GWHIP_CHECK(hipGetLastError());
+  GWHIP_CHECK(hipStreamSynchronize(stream_));
+  float x;
+  GWHIP_CHECK(hipEventElapsedTime(&x, t0, t1)); region_ms_[R_P0] = x;
+  GWHIP_CHECK(hipEventElapsedTime(&x, t2, t3)); region_ms_[R_SIGMA] = x;
+  GWHIP_CHECK(hipEventElapsedTime(&x, t0, t3)); region_ms_[R_TOTAL] = x;
+  (void)hipEventDestroy(t0); (void)hipEventDestroy(t1); (void)hipEventDestroy(t2); (void)hipEventDestroy(t3);
+  int flag = 0;
+  GWHIP_CHECK(hipMemcpy(&flag, flag_, sizeof(int), hipMemcpyDeviceToHost));
+  if (flag) throw status_error(GWHIP_ERR_NONFINITE, "Sigma contains non-finite values");
+}
+
+void gw_hip_kernel::solve_host(const double2* G, double2* Sigma, int precision) {
+  if (!G || !Sigma) throw status_error(GWHIP_ERR_ARG, "null G or Sigma");
+  GWHIP_CHECK(hipSetDevice(device_));
+  const size_t nsig = (size_t)d_.nts * d_.ns * d_.nk * d_.nso * d_.nso;
+  if (!G_dev_) G_dev_ = dalloc<double2>(nsig);
+  if (!S_dev_) S_dev_ = dalloc<double2>(nsig);
+  GWHIP_CHECK(hipMemcpy(G_dev_, G, nsig * sizeof(double2), hipMemcpyHostToDevice));
+  solve_device(G_dev_, S_dev_, precision);
+  GWHIP_CHECK(hipMemcpy(Sigma, S_dev_, nsig * sizeof(double2), hipMemcpyDeviceToHost));
+}
+
+void gw_hip_kernel::copy_P_tilde(double2* host) const {
+  GWHIP_CHECK(hipSetDevice(device_));
+  const size_t n = (size_t)d_.nk * d_.nts * d_.NQ * d_.NQ;
+  GWHIP_CHECK(hipMemcpy(host, P_, n * sizeof(double2), hipMemcpyDeviceToHost));
+}
+
+}  // namespace gwhip
+
+// ------------------------------------------------------------------------------ C ABI
+struct gwhip_handle {
+  std::unique_ptr<gwhip::gw_hip_kernel> k;
+  std::string err;
+};
+
+namespace {
+thread_local std::string g_create_error;
+
+template <typename F>
+int guarded(gwhip_handle* h, F&& f) {
+  try {
+    f();
+    return GWHIP_OK;
+  } catch (const gwhip::status_error& e) {
+    if (h) h->err = e.what();
+    return e.status;
+  } catch (const gwhip::hip_error& e) {
+    if (h) h->err = e.what();
+    return e.code == hipErrorOutOfMemory ? GWHIP_ERR_OOM : GWHIP_ERR_DEVICE;
+  } catch (const std::exception& e) {
+    if (h) h->err = e.what();
+    return GWHIP_ERR_DEVICE;
+  }
+}
+}  // namespace
+
+extern "C" {
+
+int gwhip_create(const gwhip_dims* dims, int device, gwhip_handle** out) {
+  if (!dims || !out) {
+    g_create_error = "null argument";
+    return GWHIP_ERR_ARG;
+  }
+  *out = nullptr;
+  gwhip::Dims d;
+  d.nk = dims->nk; d.nao = dims->nao; d.nso = dims->nso; d.ns = dims->ns; d.NQ = dims->NQ;
+  d.nts = dims->nts; d.nw_b = dims->nw_b; d.beta = dims->beta;
+  d.mesh = {dims->mesh[0], dims->mesh[1], dims->mesh[2]};
+  auto h = std::make_unique<gwhip_handle>();
+  int st = guarded(h.get(), [&] { h->k = std::make_unique<gwhip::gw_hip_kernel>(d, device); });
+  if (st != GWHIP_OK) {
+    g_create_error = h->err;
+    return st;
+  }
+  *out = h.release();
+  return GWHIP_OK;
+}
+
+void gwhip_destroy(gwhip_handle* h) { delete h; }
+
+const char* gwhip_last_error(const gwhip_handle* h) { return h ? h->err.c_str() : g_create_error.c_str(); }
+
+int gwhip_set_grid(gwhip_handle* h, const double* T_bt, const double* T_tb) {
+  if (!h) return GWHIP_ERR_ARG;
+  return guarded(h, [&] { h->k->set_grid((const double2*)T_bt, (const double2*)T_tb); });
+}
+
+int gwhip_upload_V(gwhip_handle* h, const double* V) {
+  if (!h) return GWHIP_ERR_ARG;
+  return guarded(h, [&] { h->k->upload_V((const double2*)V); });
+}
+
+int gwhip_generate_V(gwhip_handle* h, uint64_t seed, double sigma) {
+  if (!h) return GWHIP_ERR_ARG;
+  return guarded(h, [&] { h->k->generate_V(seed, sigma); });
+}
+
+int gwhip_eval_sigma(gwhip_handle* h, const double* G, double* Sigma, int precision) {
+  if (!h) return GWHIP_ERR_ARG;
+  return guarded(h, [&] { h->k->solve_host((const double2*)G, (double2*)Sigma, precision); });
+}
+
+int gwhip_eval_sigma_device(gwhip_handle* h, const void* G, void* Sigma, int precision) {
+  if (!h) return GWHIP_ERR_ARG;
+  return guarded(h, [&] { h->k->solve_device((const double2*)G, (double2*)Sigma, precision); });
+}
+
+int gwhip_get_P_tilde(gwhip_handle* h, double* P) {
+  if (!h || !P) return GWHIP_ERR_ARG;
+  return guarded(h, [&] { h->k->copy_P_tilde((double2*)P); });
+}
+
+int gwhip_timings(const gwhip_handle* h, double* ms, int n) {
+  if (!h || !ms) return 0;
+  const int m = std::min(n, (int)gwhip::gw_hip_kernel::R_COUNT);
+  for (int i = 0; i < m; ++i) ms[i] = h->k->region_ms()[i];
+  return m;
+}
+
+int gwhip_kernel_stats(const gwhip_handle* h, int which, int64_t* launches, double* total_ms) {
+  if (!h || !launches || !total_ms || which < 0 || which >= gwhip::gw_hip_kernel::K_COUNT) return GWHIP_ERR_ARG;
+  *launches = h->k->kstat(which).launches;
+  *total_ms = h->k->kstat(which).ms;
+  return GWHIP_OK;
+}
+
+size_t gwhip_device_bytes(const gwhip_handle* h) { return h ? h->k->device_bytes() : 0; }
+
+}  // extern "C"

@@ -1,0 +1,96 @@
+// gw_hip_kernel: the MI355X counterpart of green::mbpt::kernels::gw_cpu_kernel
+// (src/green/mbpt/kernels.h:36-178, src/gw_cpu_kernel.cpp:31-373).
+//
+// Device-resident data: V for every ordered k pair, P0/P-tilde(tau, q) for every q, G and
+// Sigma.  One evaluation runs three stages over all q on one HIP stream:
+//   1. P0(tau, q) = -(pref/nk) sum_k Z_k(tau) V_k^H   for the half tau grid  (MFMA GEMM)
+//      with Z_k = sum_s Gbar_s V_k G_s                                      (prep kernel)
+//   2. Hermitise; P0(i Omega) = T_bt' P0(tau);  P-tilde = (I - P0)^-1 P0;  P-tilde(tau) = T_tb P-tilde(i Omega)
+//   3. Sigma(tau, k) += -(1/nk) sum_{Q,n} (V G_s)[Q,i,n] R_k[Q,j,n],  R = P-tilde(tau, q) conj(V)
+// where T_bt' folds P0(beta - tau) = P0(tau) into the tau -> Omega matrix.
+#pragma once
+
+#include <array>
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+
+namespace gwhip {
+
+struct Dims {
+  int64_t nk = 0, nao = 0, nso = 0, ns = 0, NQ = 0, nts = 0, nw_b = 0;
+  std::array<int64_t, 3> mesh{0, 0, 0};
+  double beta = 0.0;
+  // derived
+  int64_t pns = 1, pseudo_ns = 1;
+  bool X2C = false;
+  double prefactor = 2.0;
+};
+
+struct KernelStat {
+  int64_t launches = 0;
+  double ms = 0.0;
+};
+
+class gw_hip_kernel {
+public:
+  enum Region { R_P0 = 0, R_TW = 1, R_BSE = 2, R_WT = 3, R_SIGMA = 4, R_TOTAL = 5, R_COUNT = 6 };
+  enum KStat { K_P0_GEMM = 0, K_SIGMA_GEMM = 1, K_P0_PREP = 2, K_SIGMA_EPI = 3, K_DYSON_INV = 4, K_COUNT = 5 };
+
+  gw_hip_kernel(const Dims& d, int device);
+  ~gw_hip_kernel();
+  gw_hip_kernel(const gw_hip_kernel&) = delete;
+  gw_hip_kernel& operator=(const gw_hip_kernel&) = delete;
+
+  void set_grid(const double2* T_bt, const double2* T_tb);
+  void upload_V(const double2* V_packed);
+  void generate_V(uint64_t seed, double sigma);
+  // G_dev, Sigma_dev: device pointers [nts][ns][nk][nso][nso]
+  void solve_device(const double2* G_dev, double2* Sigma_dev, int precision);
+  void solve_host(const double2* G, double2* Sigma, int precision);
+  void copy_P_tilde(double2* host) const;
+
+  const double* region_ms() const { return region_ms_; }
+  const KernelStat& kstat(int which) const { return kstat_[which]; }
+  size_t device_bytes() const { return bytes_; }
+
+private:
+  Dims d_;
+  int device_ = 0;
+  hipStream_t stream_ = nullptr;
+  size_t bytes_ = 0;
+  int max_lds_ = 65536;
+  bool grid_set_ = false, v_set_ = false;
+
+  // device buffers
+  double2* Vq_ = nullptr;      // [nq][nk][NQ][nao^2]
+  double2* P_ = nullptr;       // [nq][nts][NQ^2]
+  double2* Pw_ = nullptr;      // [qc][nw_b][NQ^2]
+  double2* X_ = nullptr;       // [qc][nw_b][NQ^2]
+  double2* Y_ = nullptr;       // [qc][nw_b][NQ^2]
+  double2* scratch_ = nullptr; // Z or R: [tc][nk][NQ][nao^2]
+  double2* Tbt_fold_ = nullptr;  // [nw_b][nts/2]
+  double2* Ttb_ = nullptr;       // [nts][nw_b]
+  double2* G_dev_ = nullptr;   // host-path staging
+  double2* S_dev_ = nullptr;
+  int* kplus_tab_ = nullptr;   // [nk][nq] : k + q
+  int* kminus_tab_ = nullptr;  // [nq][nk] : k - q  (row q)
+  int* flag_ = nullptr;
+  int64_t qchunk_ = 1, tchunk_p0_ = 1, tchunk_sigma_ = 1;
+  std::vector<int> kplus_host_, neg_q_;
+
+  double region_ms_[R_COUNT] = {0};
+  KernelStat kstat_[K_COUNT];
+
+  template <typename T>
+  T* dalloc(size_t count);
+  void build_maps();
+  void stage_p0(const double2* G);
+  void stage_dyson();
+  void stage_sigma(const double2* G, double2* Sigma);
+  template <typename F>
+  void timed(int kstat, F&& launch);
+};
+
+}  // namespace gwhip

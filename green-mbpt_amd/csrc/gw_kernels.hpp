@@ -1,0 +1,398 @@
+// GW-specific device kernels (FP64).  Index conventions (all row-major, complex = double2):
+//   G, Sigma : [nts][ns][nk][nso][nso]
+//   Vq       : [nq][nk][NQ][nao][nao]   Vq[q][k] = V^Q(k, k+q)   (every ordered k pair resident)
+//   P        : [nq][nts][NQ][NQ]         P0 on the first nts/2 tau, then P-tilde on all nts
+//   Z        : [tc][nk][NQ][nao][nao]    Z = sum_s Gbar_s V G_s   (P0 left operand)
+//   R        : [tc][nk][NQ][nao][nao]    R = P-tilde conj(V)       (Sigma right operand)
+// Spin blocks follow the reference (src/gw_cpu_kernel.cpp:166-186, 328-351): block s of
+// pseudo_ns has is = s/(pns^2), a = (s/pns)%pns, b = s%pns.
+#pragma once
+
+#include "common.hpp"
+
+namespace gwhip {
+
+struct DevDims {
+  int nts, ns, nk, nso, nao, NQ, pns, pseudo_ns;
+};
+
+__device__ __forceinline__ int64_t g_index(const DevDims& d, int t, int is, int k) {
+  return (((int64_t)t * d.ns + is) * d.nk + k) * d.nso * d.nso;
+}
+
+// ---------------------------------------------------------------------------------------
+// P0 left operand: Z[tz][k][Q][t][n] = sum_s sum_{p,m} Gb_s[t][p] V[Q][p][m] Gq_s[m][n],
+// Gb_s = G(beta - tau, k) block (b, a), Gq_s = G(tau, k + q) block (a, b)
+// (src/gw_cpu_kernel.cpp:170-187 with X1 = Gb V, X2 = conj(V) Gq^T regrouped so that the
+// big contraction runs against V itself).  grid: (ceil(NQ/QC), nk, tcount); LDS holds
+// Gb, Gq and QC slices of V and of W = V Gq.
+constexpr int Z_MAXO = 16;
+
+__global__ __launch_bounds__(256) void z_prep_kernel(DevDims d, const double2* __restrict__ G,
+                                                     const double2* __restrict__ Vq,  // this q
+                                                     double2* __restrict__ Z, const int* __restrict__ kplus,
+                                                     int t0, int QC) {
+  extern __shared__ double2 lds[];
+  const int nao = d.nao, nao2 = nao * nao;
+  double2* Gb = lds;
+  double2* Gq = Gb + nao2;
+  double2* Vc = Gq + nao2;
+  double2* W = Vc + QC * nao2;
+  const int tz = blockIdx.z, tau = t0 + tz, tt = d.nts - 1 - tau;
+  const int k = blockIdx.y, kq = kplus[k];
+  const int Q0 = blockIdx.x * QC;
+  const int nq = min(QC, d.NQ - Q0);
+  const int nout = nq * nao2;
+  const int tid = threadIdx.x;
+
+  for (int e = tid; e < nout; e += blockDim.x) Vc[e] = Vq[((int64_t)k * d.NQ + Q0) * nao2 + e];
+  double2 acc[Z_MAXO];
+#pragma unroll
+  for (int i = 0; i < Z_MAXO; ++i) acc[i] = make_double2(0.0, 0.0);
+
+  for (int s = 0; s < d.pseudo_ns; ++s) {
+    const int is = s / (d.pns * d.pns), a = (s / d.pns) % d.pns, b = s % d.pns;
+    const double2* Gbar = G + g_index(d, tt, is, k);
+    const double2* Gk = G + g_index(d, tau, is, kq);
+    __syncthreads();
+    for (int e = tid; e < nao2; e += blockDim.x) {
+      const int r = e / nao, c = e - (e / nao) * nao;
+      Gb[e] = Gbar[(int64_t)(b * nao + r) * d.nso + a * nao + c];
+      Gq[e] = Gk[(int64_t)(a * nao + r) * d.nso + b * nao + c];
+    }
+    __syncthreads();
+    // W[Qc][p][n] = sum_m V[Qc][p][m] Gq[m][n]
+    for (int e = tid; e < nout; e += blockDim.x) {
+      const int qc = e / nao2, rem = e - qc * nao2;
+      const int p = rem / nao, n = rem - p * nao;
+      const double2* vr = Vc + qc * nao2 + p * nao;
+      double2 w = make_double2(0.0, 0.0);
+      for (int m = 0; m < nao; ++m) w = cfma(vr[m], Gq[m * nao + n], w);
+      W[e] = w;
+    }
+    __syncthreads();
+    // acc[Qc][t][n] += sum_p Gb[t][p] W[Qc][p][n]
+#pragma unroll
+    for (int i = 0; i < Z_MAXO; ++i) {
+      const int e = tid + i * blockDim.x;
+      if (e < nout) {
+        const int qc = e / nao2, rem = e - qc * nao2;
+        const int t = rem / nao, n = rem - t * nao;
+        const double2* gr = Gb + t * nao;
+        const double2* wc = W + qc * nao2 + n;
+        double2 z = acc[i];
+        for (int p = 0; p < nao; ++p) z = cfma(gr[p], wc[p * nao], z);
+        acc[i] = z;
+      }
+    }
+  }
+  double2* Zout = Z + (((int64_t)tz * d.nk + k) * d.NQ + Q0) * nao2;
+#pragma unroll
+  for (int i = 0; i < Z_MAXO; ++i) {
+    const int e = tid + i * blockDim.x;
+    if (e < nout) Zout[e] = acc[i];
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// Sigma epilogue: Sigma(tau, is, k)[a,b block](i, j) += scale * sum_Q sum_n T_s[Q][i][n] R[Q][j][n],
+// T_s[Q] = V(k, k-q)[Q] G(tau, k-q)[a,b block]  (src/gw_cpu_kernel.cpp:359-373 with
+// Y2 = Y1^T P computed by the big GEMM on the other side: R = P conj(V)).
+// grid: (nk, tcount); LDS: Gs, QC slices of V, R and T.
+constexpr int SE_MAXO = 8;
+
+__global__ __launch_bounds__(256) void sigma_epilogue_kernel(DevDims d, const double2* __restrict__ G,
+                                                             const double2* __restrict__ Vq,  // V(k, k-q)
+                                                             const double2* __restrict__ R,
+                                                             double2* __restrict__ Sigma,
+                                                             const int* __restrict__ kminus, int t0, int QC,
+                                                             double scale) {
+  extern __shared__ double2 lds[];
+  const int nao = d.nao, nao2 = nao * nao;
+  double2* Gs = lds;
+  double2* Vc = Gs + nao2;
+  double2* Rc = Vc + QC * nao2;
+  double2* T = Rc + QC * nao2;
+  const int k = blockIdx.x, tz = blockIdx.y, tau = t0 + tz;
+  const int kmq = kminus[k];
+  const int tid = threadIdx.x;
+  const double2* Vk = Vq + (int64_t)k * d.NQ * nao2;
+  const double2* Rk = R + ((int64_t)tz * d.nk + k) * d.NQ * nao2;
+
+  for (int s = 0; s < d.pseudo_ns; ++s) {
+    const int is = s / (d.pns * d.pns), a = (s / d.pns) % d.pns, b = s % d.pns;
+    const double2* Gk = G + g_index(d, tau, is, kmq);
+    __syncthreads();
+    for (int e = tid; e < nao2; e += blockDim.x) {
+      const int r = e / nao, c = e - (e / nao) * nao;
+      Gs[e] = Gk[(int64_t)(a * nao + r) * d.nso + b * nao + c];
+    }
+    double2 acc[SE_MAXO];
+#pragma unroll
+    for (int i = 0; i < SE_MAXO; ++i) acc[i] = make_double2(0.0, 0.0);
+
+    for (int Q0 = 0; Q0 < d.NQ; Q0 += QC) {
+      const int nq = min(QC, d.NQ - Q0);
+      __syncthreads();
+      for (int e = tid; e < nq * nao2; e += blockDim.x) {
+        Vc[e] = Vk[(int64_t)Q0 * nao2 + e];
+        Rc[e] = Rk[(int64_t)Q0 * nao2 + e];
+      }
+      __syncthreads();
+      // T[Qc][i][n] = sum_m V[Qc][i][m] Gs[m][n]
+      for (int e = tid; e < nq * nao2; e += blockDim.x) {
+        const int qc = e / nao2, rem = e - qc * nao2;
+        const int i = rem / nao, n = rem - i * nao;
+        const double2* vr = Vc + qc * nao2 + i * nao;
+        double2 w = make_double2(0.0, 0.0);
+        for (int m = 0; m < nao; ++m) w = cfma(vr[m], Gs[m * nao + n], w);
+        T[e] = w;
+      }
+      __syncthreads();
+      // acc(i, j) += sum_{Qc, n} T[Qc][i][n] R[Qc][j][n]
+#pragma unroll
+      for (int o = 0; o < SE_MAXO; ++o) {
+        const int e = tid + o * blockDim.x;
+        if (e < nao2) {
+          const int i = e / nao, j = e - (e / nao) * nao;
+          double2 z = acc[o];
+          for (int qc = 0; qc < nq; ++qc) {
+            const double2* tr = T + qc * nao2 + i * nao;
+            const double2* rr = Rc + qc * nao2 + j * nao;
+            for (int n = 0; n < nao; ++n) z = cfma(tr[n], rr[n], z);
+          }
+          acc[o] = z;
+        }
+      }
+    }
+    double2* S = Sigma + g_index(d, tau, is, k);
+#pragma unroll
+    for (int o = 0; o < SE_MAXO; ++o) {
+      const int e = tid + o * blockDim.x;
+      if (e < nao2) {
+        const int i = e / nao, j = e - (e / nao) * nao;
+        double2* dst = S + (int64_t)(a * nao + i) * d.nso + b * nao + j;
+        double2 v = *dst;
+        v.x += scale * acc[o].x;
+        v.y += scale * acc[o].y;
+        *dst = v;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// X = (X + X^H)/2 for a batch of N x N matrices (common_defs.h:104-115).  One thread per
+// (i <= j) pair.
+__global__ void hermitize_kernel(double2* X, int N, int64_t stride) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)N * N) return;
+  const int i = (int)(e / N), j = (int)(e - (e / N) * N);
+  if (i > j) return;
+  double2* M = X + (int64_t)blockIdx.y * stride;
+  const double2 u = M[(int64_t)i * N + j], l = M[(int64_t)j * N + i];
+  const double2 h = make_double2(0.5 * (u.x + l.x), 0.5 * (u.y - l.y));
+  M[(int64_t)i * N + j] = h;
+  M[(int64_t)j * N + i] = make_double2(h.x, -h.y);
+}
+
+// P0 half-grid symmetrisation for all q: Hermitise P[q][t] for t < nts/2
+// (src/gw_cpu_kernel.cpp:217-225; the /nk and the prefactor are applied by the GEMM).
+// The mirror P0(beta - tau) = P0(tau) is folded into the tau -> Omega matrix instead.
+
+// Dyson operand: X[b] = I - Pw[b]
+__global__ void identity_minus_kernel(double2* X, const double2* Pw, int N, int64_t batch) {
+  const int64_t n2 = (int64_t)N * N;
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= n2 * batch) return;
+  const int64_t e = gid % n2;
+  const int i = (int)(e / N), j = (int)(e - (e / N) * N);
+  const double2 p = Pw[gid];
+  X[gid] = make_double2((i == j ? 1.0 : 0.0) - p.x, -p.y);
+}
+
+// ---------------------------------------------------------------------------------------
+// Batched in-place inverse of Hermitian positive-definite N x N matrices by blocked
+// Gauss-Jordan sweeps without pivoting (the reference uses Eigen's LDLT on I - P0,
+// src/gw_cpu_kernel.cpp:243-252; I - P0(i Omega) has eigenvalues >= 1 since P0 is negative
+// semidefinite, so every pivot is positive).  One workgroup per matrix; per block column J:
+//   Dinv = A_JJ^-1 (small sweep in LDS);  R = Dinv A_JI;  C = A_IJ (old);
+//   A_II -= C R;  A_IJ = -C Dinv;  A_JI = R;  A_JJ = Dinv.
+// Threads own columns; the column panel C lives in LDS ([N][nb]) and the row panel in registers.
+constexpr int GJ_NB = 16;
+
+__global__ void gj_inverse_kernel(double2* A, int N, int64_t stride, int nb) {
+  extern __shared__ double2 lds[];
+  double2* D = lds;               // [GJ_NB][GJ_NB]
+  double2* Cp = D + GJ_NB * GJ_NB;  // [N][nb]
+  double2* M = A + (int64_t)blockIdx.x * stride;
+  const int tid = threadIdx.x, nth = blockDim.x;
+
+  for (int J0 = 0; J0 < N; J0 += nb) {
+    const int jb = min(nb, N - J0);
+    __syncthreads();
+    for (int e = tid; e < jb * jb; e += nth) {
+      const int r = e / jb, c = e - (e / jb) * jb;
+      D[r * GJ_NB + c] = M[(int64_t)(J0 + r) * N + J0 + c];
+    }
+    for (int e = tid; e < N * jb; e += nth) {
+      const int i = e / jb, l = e - (e / jb) * jb;
+      Cp[i * nb + l] = M[(int64_t)i * N + J0 + l];
+    }
+    __syncthreads();
+    // small in-place sweep of D (jb x jb); the launcher guarantees nth >= GJ_NB*GJ_NB so
+    // each element has its own thread: read old values, barrier, write.
+    for (int piv = 0; piv < jb; ++piv) {
+      const bool act = tid < jb * jb;
+      double2 v = make_double2(0.0, 0.0);
+      int r = 0, c = 0;
+      if (act) {
+        r = tid / jb;
+        c = tid - r * jb;
+        const double2 pv = D[piv * GJ_NB + piv];
+        const double den = pv.x * pv.x + pv.y * pv.y;
+        const double2 pinv = make_double2(pv.x / den, -pv.y / den);
+        if (r == piv && c == piv) {
+          v = pinv;
+        } else if (r == piv) {
+          v = cmul(D[piv * GJ_NB + c], pinv);
+        } else if (c == piv) {
+          v = cmul(D[r * GJ_NB + piv], pinv);
+          v.x = -v.x;
+          v.y = -v.y;
+        } else {
+          const double2 t = cmul(cmul(D[r * GJ_NB + piv], D[piv * GJ_NB + c]), pinv);
+          const double2 o = D[r * GJ_NB + c];
+          v = make_double2(o.x - t.x, o.y - t.y);
+        }
+      }
+      __syncthreads();
+      if (act) D[r * GJ_NB + c] = v;
+      __syncthreads();
+    }
+    // column sweep
+    for (int j = tid; j < N; j += nth) {
+      double2 rv[GJ_NB];
+      const bool inJ = (j >= J0) && (j < J0 + jb);
+      if (inJ) {
+#pragma unroll
+        for (int l = 0; l < GJ_NB; ++l) rv[l] = (l < jb) ? D[l * GJ_NB + (j - J0)] : make_double2(0.0, 0.0);
+      } else {
+        double2 col[GJ_NB];
+#pragma unroll
+        for (int l = 0; l < GJ_NB; ++l) col[l] = (l < jb) ? M[(int64_t)(J0 + l) * N + j] : make_double2(0.0, 0.0);
+#pragma unroll
+        for (int l = 0; l < GJ_NB; ++l) {
+          double2 acc = make_double2(0.0, 0.0);
+          if (l < jb)
+            for (int lp = 0; lp < jb; ++lp) acc = cfma(D[l * GJ_NB + lp], col[lp], acc);
+          rv[l] = acc;
+        }
+      }
+#pragma unroll
+      for (int l = 0; l < GJ_NB; ++l)
+        if (l < jb) M[(int64_t)(J0 + l) * N + j] = rv[l];
+      for (int i = 0; i < N; ++i) {
+        if (i >= J0 && i < J0 + jb) continue;
+        double2 v = inJ ? make_double2(0.0, 0.0) : M[(int64_t)i * N + j];
+        const double2* c = Cp + i * nb;
+#pragma unroll
+        for (int l = 0; l < GJ_NB; ++l)
+          if (l < jb) {
+            const double2 t = cmul(c[l], rv[l]);
+            v.x -= t.x;
+            v.y -= t.y;
+          }
+        M[(int64_t)i * N + j] = v;
+      }
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------
+// V expansion: Vq[q][k][Q][i][j] = V^Q_ij(k, k+q) from the packed pair store
+// (df_integral_t.h:68,169-176: k1 >= k2 stored, k1 < k2 = per-Q adjoint).
+__global__ void expand_v_kernel(const double2* __restrict__ Vp, double2* __restrict__ Vq,
+                                const int* __restrict__ kplus_tab,  // [nk][nq]: k + q
+                                int nk, int NQ, int nao, int64_t total) {
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= total) return;
+  const int nao2 = nao * nao;
+  int64_t r = gid;
+  const int j = (int)(r % nao); r /= nao;
+  const int i = (int)(r % nao); r /= nao;
+  const int Q = (int)(r % NQ); r /= NQ;
+  const int k = (int)(r % nk); r /= nk;
+  const int q = (int)r;
+  const int k2 = kplus_tab[k * nk + q];
+  double2 v;
+  if (k >= k2) {
+    const int64_t p = (int64_t)k * (k + 1) / 2 + k2;
+    v = Vp[(p * NQ + Q) * nao2 + i * nao + j];
+  } else {
+    const int64_t p = (int64_t)k2 * (k2 + 1) / 2 + k;
+    v = cconj(Vp[(p * NQ + Q) * nao2 + j * nao + i]);
+  }
+  Vq[gid] = v;
+}
+
+// Counter-based normal variates (SplitMix64 + Box-Muller), identical counters to
+// gwhip.inputs.normal_pairs.
+__device__ __forceinline__ uint64_t splitmix64(uint64_t x) {
+  uint64_t z = x + 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+__device__ __forceinline__ double u01(uint64_t key, uint64_t idx) {
+  const uint64_t u = splitmix64(key + idx);
+  return ((double)(u >> 11) + 0.5) * 0x1.0p-53;
+}
+__device__ __forceinline__ double2 normal_pair(uint64_t key, uint64_t idx, double sigma) {
+  const double u1 = u01(key, 2 * idx), u2 = u01(key, 2 * idx + 1);
+  const double r = sqrt(-2.0 * log(u1));
+  double s, c;
+  sincospi(2.0 * u2, &s, &c);
+  return make_double2(sigma * r * c, sigma * r * s);
+}
+
+__global__ void generate_v_kernel(double2* __restrict__ Vq, const int* __restrict__ kplus_tab, int nk, int NQ,
+                                  int nao, int64_t total, uint64_t key, double sigma) {
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= total) return;
+  int64_t r = gid;
+  const int j = (int)(r % nao); r /= nao;
+  const int i = (int)(r % nao); r /= nao;
+  const int Q = (int)(r % NQ); r /= NQ;
+  const int k = (int)(r % nk); r /= nk;
+  const int q = (int)r;
+  const int k2 = kplus_tab[k * nk + q];
+  auto raw = [&](int64_t p, int ii, int jj) {
+    return normal_pair(key, ((p * NQ + Q) * nao + ii) * nao + jj, sigma);
+  };
+  double2 v;
+  if (k == k2) {
+    const int64_t p = (int64_t)k * (k + 1) / 2 + k;
+    const double2 w1 = raw(p, i, j), w2 = raw(p, j, i);
+    v = make_double2(0.5 * (w1.x + w2.x), 0.5 * (w1.y - w2.y));
+  } else if (k > k2) {
+    v = raw((int64_t)k * (k + 1) / 2 + k2, i, j);
+  } else {
+    v = cconj(raw((int64_t)k2 * (k2 + 1) / 2 + k, j, i));
+  }
+  Vq[gid] = v;
+}
+
+// Non-finite scan: flag[0] = 1 if any component is NaN/Inf.
+__global__ void nonfinite_kernel(const double2* X, int64_t n, int* flag) {
+  int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t step = (int64_t)gridDim.x * blockDim.x;
+  bool bad = false;
+  for (; gid < n; gid += step) {
+    const double2 v = X[gid];
+    bad |= !isfinite(v.x) || !isfinite(v.y);
+  }
+  if (bad) atomicOr(flag, 1);
+}
+
+}  // namespace gwhip

@@ -1,0 +1,173 @@
+"""ctypes binding of libgwhip.so (include/gwhip.h) and a thin Python kernel object.
+
+There is no CPU fallback: if the HIP library is missing or no GPU is visible, the calls
+raise.  The object mirrors the reference's kernel life cycle
+(``gw_kernel_factory::get_kernel`` -> kernel + callback, ``src/green/mbpt/kernel_factory.h:85-91``;
+``solve(G, Sigma)``, ``src/gw_cpu_kernel.cpp:31-72``).
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import re
+
+import numpy as np
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libgwhip.so")
+HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__)))),
+                      "include", "gwhip.h")
+
+OK, ERR_ARG, ERR_WRONG_GRID, ERR_STATE, ERR_OOM, ERR_DEVICE, ERR_NONFINITE, ERR_UNSUPPORTED = range(8)
+PREC_FP64, PREC_SPLIT = 0, 1
+REGIONS = ("P0_zgemm", "P0(t) -> P0(w)", "GW-BSE", "P(w) -> P(t)", "Selfenergy_zgemm", "total")
+KSTATS = ("P0 GEMM", "Sigma GEMM", "P0 prep", "Sigma epilogue", "Dyson inverse")
+
+
+class GwhipError(RuntimeError):
+    def __init__(self, status: int, msg: str):
+        super().__init__(f"gwhip status {status}: {msg}")
+        self.status = status
+
+
+class WrongGrid(GwhipError):
+    """Mirror of ``mbpt_wrong_grid`` (``src/green/mbpt/except.h:33-36``)."""
+
+
+class Dims(ctypes.Structure):
+    _fields_ = [("nk", ctypes.c_int64), ("mesh", ctypes.c_int64 * 3), ("nao", ctypes.c_int64),
+                ("nso", ctypes.c_int64), ("ns", ctypes.c_int64), ("NQ", ctypes.c_int64),
+                ("nts", ctypes.c_int64), ("nw_b", ctypes.c_int64), ("beta", ctypes.c_double)]
+
+
+_lib = None
+
+
+def load() -> ctypes.CDLL:
+    """Load libgwhip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is missing: build it with `python green-mbpt_amd/build.py` "
+                           "(there is no CPU fallback)")
+    lib = ctypes.CDLL(LIB_PATH)
+    P = ctypes.c_void_p
+    lib.gwhip_create.argtypes = [ctypes.POINTER(Dims), ctypes.c_int, ctypes.POINTER(P)]
+    lib.gwhip_create.restype = ctypes.c_int
+    lib.gwhip_destroy.argtypes = [P]
+    lib.gwhip_destroy.restype = None
+    lib.gwhip_last_error.argtypes = [P]
+    lib.gwhip_last_error.restype = ctypes.c_char_p
+    lib.gwhip_set_grid.argtypes = [P, P, P]
+    lib.gwhip_upload_V.argtypes = [P, P]
+    lib.gwhip_generate_V.argtypes = [P, ctypes.c_uint64, ctypes.c_double]
+    lib.gwhip_eval_sigma.argtypes = [P, P, P, ctypes.c_int]
+    lib.gwhip_eval_sigma_device.argtypes = [P, P, P, ctypes.c_int]
+    lib.gwhip_get_P_tilde.argtypes = [P, P]
+    lib.gwhip_timings.argtypes = [P, ctypes.POINTER(ctypes.c_double), ctypes.c_int]
+    lib.gwhip_kernel_stats.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int64),
+                                       ctypes.POINTER(ctypes.c_double)]
+    lib.gwhip_device_bytes.argtypes = [P]
+    lib.gwhip_device_bytes.restype = ctypes.c_size_t
+    for f in ("gwhip_set_grid", "gwhip_upload_V", "gwhip_generate_V", "gwhip_eval_sigma",
+              "gwhip_eval_sigma_device", "gwhip_get_P_tilde", "gwhip_timings", "gwhip_kernel_stats"):
+        getattr(lib, f).restype = ctypes.c_int
+    _lib = lib
+    return lib
+
+
+def header_functions() -> list[str]:
+    """Names of every function the C header declares."""
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[\w\s\*]+?\b(gwhip_\w+)\s*\(", text, re.M)))
+
+
+def _ptr(a: np.ndarray) -> ctypes.c_void_p:
+    return ctypes.c_void_p(a.ctypes.data)
+
+
+class GwHipKernel:
+    """One device-resident GW kernel instance (dims fixed at construction)."""
+
+    def __init__(self, nk, nao, nso, ns, NQ, nts, nw_b, beta, mesh=None, device=0):
+        self.lib = load()
+        if mesh is None:
+            n = round(nk ** (1.0 / 3.0))
+            mesh = (n, n, n) if n ** 3 == nk else (nk, 1, 1)
+        self.dims = dict(nk=nk, nao=nao, nso=nso, ns=ns, NQ=NQ, nts=nts, nw_b=nw_b, beta=beta, mesh=tuple(mesh))
+        d = Dims(nk, (ctypes.c_int64 * 3)(*mesh), nao, nso, ns, NQ, nts, nw_b, beta)
+        h = ctypes.c_void_p()
+        st = self.lib.gwhip_create(ctypes.byref(d), device, ctypes.byref(h))
+        if st != OK:
+            msg = self.lib.gwhip_last_error(None).decode()
+            raise (WrongGrid if st == ERR_WRONG_GRID else GwhipError)(st, msg)
+        self.h = h
+
+    def _check(self, st: int):
+        if st != OK:
+            msg = self.lib.gwhip_last_error(self.h).decode()
+            raise (WrongGrid if st == ERR_WRONG_GRID else GwhipError)(st, msg)
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.gwhip_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_grid(self, T_bt: np.ndarray, T_tb: np.ndarray):
+        a = np.ascontiguousarray(T_bt, dtype=np.complex128)
+        b = np.ascontiguousarray(T_tb, dtype=np.complex128)
+        assert a.shape == (self.dims["nw_b"], self.dims["nts"]) and b.shape == (self.dims["nts"], self.dims["nw_b"])
+        self._check(self.lib.gwhip_set_grid(self.h, _ptr(a), _ptr(b)))
+
+    def upload_V(self, Vp: np.ndarray):
+        nk, NQ, nao = self.dims["nk"], self.dims["NQ"], self.dims["nao"]
+        v = np.ascontiguousarray(Vp, dtype=np.complex128)
+        assert v.shape == (nk * (nk + 1) // 2, NQ, nao, nao), v.shape
+        self._check(self.lib.gwhip_upload_V(self.h, _ptr(v)))
+
+    def generate_V(self, seed: int, sigma: float = 0.1):
+        self._check(self.lib.gwhip_generate_V(self.h, ctypes.c_uint64(seed), sigma))
+
+    def sigma_shape(self):
+        d = self.dims
+        return (d["nts"], d["ns"], d["nk"], d["nso"], d["nso"])
+
+    def solve(self, G: np.ndarray, precision: int = PREC_FP64) -> np.ndarray:
+        g = np.ascontiguousarray(G, dtype=np.complex128)
+        assert g.shape == self.sigma_shape(), (g.shape, self.sigma_shape())
+        S = np.empty_like(g)
+        self._check(self.lib.gwhip_eval_sigma(self.h, _ptr(g), _ptr(S), precision))
+        return S
+
+    def solve_device(self, G_ptr: int, S_ptr: int, precision: int = PREC_FP64):
+        self._check(self.lib.gwhip_eval_sigma_device(self.h, ctypes.c_void_p(G_ptr), ctypes.c_void_p(S_ptr),
+                                                     precision))
+
+    def P_tilde(self) -> np.ndarray:
+        d = self.dims
+        P = np.empty((d["nk"], d["nts"], d["NQ"], d["NQ"]), dtype=np.complex128)
+        self._check(self.lib.gwhip_get_P_tilde(self.h, _ptr(P)))
+        return P
+
+    def timings(self) -> dict:
+        buf = (ctypes.c_double * len(REGIONS))()
+        n = self.lib.gwhip_timings(self.h, buf, len(REGIONS))
+        return {REGIONS[i]: buf[i] for i in range(n)}
+
+    def kernel_stats(self) -> dict:
+        out = {}
+        for i, name in enumerate(KSTATS):
+            n = ctypes.c_int64()
+            ms = ctypes.c_double()
+            self._check(self.lib.gwhip_kernel_stats(self.h, i, ctypes.byref(n), ctypes.byref(ms)))
+            out[name] = (n.value, ms.value)
+        return out
+
+    def device_bytes(self) -> int:
+        return int(self.lib.gwhip_device_bytes(self.h))

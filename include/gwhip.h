@@ -1,0 +1,125 @@
+/*
+ * gwhip -- MI355X (gfx950) GW self-energy kernel, C ABI.
+ *
+ * Drop-in for green-mbpt's GW hot path.  The reference's kernel-selection interface
+ * (src/green/mbpt/kernel_factory.h:28-31,80-106) hands a custom kernel the solver's
+ * dimensions, the imaginary-time transformer, the Brillouin-zone maps and the overlap,
+ * and later calls  solve(G, Sigma)  (src/gw_cpu_kernel.cpp:31-72) with row-major
+ * [nts, ns, ink, nso, nso] complex128 tensors.  This ABI exposes the same steps with
+ * plain pointers and sizes; the C++ adapter in green-mbpt_amd/adapter/ maps the
+ * reference's C++ types onto it.
+ *
+ * Complex numbers are interleaved (re, im) doubles.  All host buffers are owned by
+ * the caller.  A handle owns one HIP stream and its device memory; calls on one handle
+ * are not thread-safe.  Every function returns a status code (GWHIP_OK on success);
+ * gwhip_last_error() gives the message of the last failure on a handle.
+ */
+#ifndef GWHIP_H
+#define GWHIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct gwhip_handle gwhip_handle;
+
+/* Status codes; the exception each mirrors is in src/green/mbpt/except.h:28-56. */
+enum gwhip_status {
+  GWHIP_OK = 0,
+  GWHIP_ERR_ARG = 1,         /* bad dimensions or null pointer                            */
+  GWHIP_ERR_WRONG_GRID = 2,  /* odd nts: mbpt_wrong_grid (src/gw_cpu_kernel.cpp:79-81)   */
+  GWHIP_ERR_STATE = 3,       /* grid or integrals not set before eval                    */
+  GWHIP_ERR_OOM = 4,         /* device allocation failed                                 */
+  GWHIP_ERR_DEVICE = 5,      /* HIP runtime error or no usable device                    */
+  GWHIP_ERR_NONFINITE = 6,   /* Sigma contains NaN/Inf                                   */
+  GWHIP_ERR_UNSUPPORTED = 7  /* request outside what this build implements               */
+};
+
+/* Arithmetic of the P0 and Sigma contractions (the Dyson step, transforms and
+ * accumulation across k/q are FP64 in every mode). */
+enum gwhip_precision {
+  GWHIP_PREC_FP64 = 0,       /* complex128 throughout (reference default, P_sp = Sigma_sp = false) */
+  GWHIP_PREC_SPLIT = 1       /* split-precision contractions, FP64 accumulation             */
+};
+
+/* Problem dimensions; names follow gw_cpu_kernel's members (src/green/mbpt/kernels.h:58-83).
+ * Only the full k mesh without symmetry is supported (ink = nq = inq = nk). */
+typedef struct gwhip_dims {
+  int64_t nk;        /* k points in the full Brillouin zone                          */
+  int64_t mesh[3];   /* Monkhorst-Pack mesh, mesh[0]*mesh[1]*mesh[2] == nk;
+                        k = (i0*mesh[1] + i1)*mesh[2] + i2, k+q componentwise mod mesh */
+  int64_t nao;       /* atomic orbitals                                              */
+  int64_t nso;       /* spin orbitals: nao, or 2*nao for X2C                         */
+  int64_t ns;        /* spins (1 or 2; 1 for X2C)                                    */
+  int64_t NQ;        /* auxiliary (density-fitting) basis size                       */
+  int64_t nts;       /* fermionic tau nodes including 0 and beta (must be even)      */
+  int64_t nw_b;      /* bosonic Matsubara frequencies                                */
+  double beta;       /* inverse temperature                                          */
+} gwhip_dims;
+
+/* Create a kernel instance on HIP device `device`.
+ * Replaces: gw_cpu_kernel::gw_cpu_kernel (src/green/mbpt/kernels.h:42-53) as returned by
+ * gw_kernel_factory::get_kernel / CUSTOM_GW_KERNEL_CALL (src/green/mbpt/kernel_factory.h:28-31,85-91). */
+int gwhip_create(const gwhip_dims* dims, int device, gwhip_handle** out);
+
+/* Release device memory and the stream.  Replaces the kernel's shared_ptr<void> deleter
+ * (src/green/mbpt/kernel_factory.h:89). */
+void gwhip_destroy(gwhip_handle* h);
+
+/* Message of the last failed call on h (static string for h == NULL). */
+const char* gwhip_last_error(const gwhip_handle* h);
+
+/* Set the bosonic transforms: T_bt [nw_b][nts] maps P(tau_f) -> P(i Omega_b) and
+ * T_tb [nts][nw_b] maps back; both complex.
+ * Replaces: transformer_t::tau_f_to_w_b / w_b_to_tau_f as used at src/gw_cpu_kernel.cpp:234,264. */
+int gwhip_set_grid(gwhip_handle* h, const double* T_bt, const double* T_tb);
+
+/* Upload V^Q_ij(k1,k2) in the packed pair store of df_integral_t: [nk(nk+1)/2][NQ][nao][nao]
+ * complex, pair index k1(k1+1)/2 + k2 for k1 >= k2; (k1 < k2) blocks are the per-Q adjoints.
+ * V stays resident on the device across eval calls.
+ * Replaces: df_integral_t::read_integrals + symmetrize (src/green/mbpt/df_integral_t.h:64-92,162-187). */
+int gwhip_upload_V(gwhip_handle* h, const double* V_packed);
+
+/* Fill the resident V on the device from the seeded counter-based generator
+ * (SplitMix64 + Box-Muller, N(0, sigma^2) per real component, k1==k2 blocks Hermitised),
+ * without a host copy.  Same counters as the Python generator gwhip.inputs.generate_V.
+ * No reference counterpart: synthetic-input path for benchmarks. */
+int gwhip_generate_V(gwhip_handle* h, uint64_t seed, double sigma);
+
+/* Sigma(tau, s, k) from G(tau, s, k); host buffers [nts][ns][nk][nso][nso] complex.
+ * Sigma is overwritten (the reference zeroes it first, src/gw_cpu_kernel.cpp:39-42).
+ * Replaces: gw_cpu_kernel::solve (src/gw_cpu_kernel.cpp:31-72), reached via
+ * gw_solver::solve (src/gw_solver.cpp:30-32). */
+int gwhip_eval_sigma(gwhip_handle* h, const double* G, double* Sigma, int precision);
+
+/* As gwhip_eval_sigma, with G and Sigma already in device memory (same layout).
+ * Synchronous with respect to the host. */
+int gwhip_eval_sigma_device(gwhip_handle* h, const void* G_dev, void* Sigma_dev, int precision);
+
+/* Copy P-tilde(tau, q) of the last eval to the host: [nk][nts][NQ][NQ] complex.
+ * Exposes the intermediate the reference keeps in P0_tilde_s after eval_P_tilde
+ * (src/gw_cpu_kernel.cpp:227-279); used by tests. */
+int gwhip_get_P_tilde(gwhip_handle* h, double* P);
+
+/* Timings of the last eval, in milliseconds, for the region names of the reference's
+ * statistics object (src/gw_cpu_kernel.cpp:43-68,205,233-274,363):
+ *   0 "P0_zgemm" (P0 contraction), 1 "P0(t) -> P0(w)", 2 "GW-BSE" (Dyson),
+ *   3 "P(w) -> P(t)", 4 "Selfenergy_zgemm" (Sigma contraction), 5 "total".
+ * Measured with HIP events on the handle's stream.  Returns the number written. */
+int gwhip_timings(const gwhip_handle* h, double* ms, int n);
+
+/* Per-launch timing of one kernel family over the last eval:
+ * which = 0 P0 big GEMM, 1 Sigma big GEMM, 2 P0 prep (G.V.G), 3 Sigma epilogue, 4 Dyson inverse.
+ * Writes launch count and total milliseconds (HIP events around each launch). */
+int gwhip_kernel_stats(const gwhip_handle* h, int which, int64_t* launches, double* total_ms);
+
+/* Device bytes held by the handle. */
+size_t gwhip_device_bytes(const gwhip_handle* h);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* GWHIP_H */

@@ -1,0 +1,54 @@
+"""C-ABI checks that need no GPU: the library loads, exports every declared symbol, and
+rejects bad input with the documented status codes before touching a device."""
+import ctypes
+
+import pytest
+
+from gwhip import lib as gl
+
+
+def test_library_exports_every_header_symbol():
+    lib = gl.load()
+    names = gl.header_functions()
+    assert len(names) >= 12
+    for n in names:
+        assert hasattr(lib, n), n
+
+
+def _create(**kw):
+    lib = gl.load()
+    base = dict(nk=1, mesh=(1, 1, 1), nao=2, nso=2, ns=1, NQ=3, nts=4, nw_b=3, beta=10.0)
+    base.update(kw)
+    d = gl.Dims(base["nk"], (ctypes.c_int64 * 3)(*base["mesh"]), base["nao"], base["nso"], base["ns"],
+                base["NQ"], base["nts"], base["nw_b"], base["beta"])
+    h = ctypes.c_void_p()
+    st = lib.gwhip_create(ctypes.byref(d), 0, ctypes.byref(h))
+    if st == gl.OK:
+        lib.gwhip_destroy(h)
+    return st, lib.gwhip_last_error(None).decode()
+
+
+def test_odd_nts_is_wrong_grid():
+    st, msg = _create(nts=5)
+    assert st == gl.ERR_WRONG_GRID and "even" in msg
+
+
+@pytest.mark.parametrize("bad", [dict(nk=0), dict(mesh=(2, 1, 1)), dict(nso=3), dict(ns=3),
+                                 dict(ns=2, nso=4), dict(NQ=-1)])
+def test_bad_dims_are_rejected(bad):
+    st, _ = _create(**bad)
+    assert st == gl.ERR_ARG
+
+
+def test_null_arguments():
+    lib = gl.load()
+    assert lib.gwhip_create(None, 0, None) == gl.ERR_ARG
+    assert lib.gwhip_set_grid(None, None, None) == gl.ERR_ARG
+    assert lib.gwhip_eval_sigma(None, None, None, 0) == gl.ERR_ARG
+    assert lib.gwhip_device_bytes(None) == 0
+
+
+def test_valid_dims_reach_the_device_check():
+    """Without a GPU the valid request fails only at the device step (no silent fallback)."""
+    st, _ = _create()
+    assert st in (gl.OK, gl.ERR_DEVICE)

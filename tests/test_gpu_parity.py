@@ -1,0 +1,127 @@
+"""HIP path vs the oracle, through the C ABI (needs an MI355X).
+
+Bar: FP64 Sigma within 1e-10 relative (max-abs difference over max |Sigma|) of the oracle
+on the same seeded inputs; the reference's own fixtures within its 1e-6 abs tolerance.
+"""
+import numpy as np
+import pytest
+
+from gwhip import lib as gl
+from helpers import CONFIG0, load_fixture, rel_err, synthetic
+from oracle import gw_oracle as go
+
+pytestmark = pytest.mark.gpu
+
+TOL_FP64 = 1e-10
+
+
+def _kernel(d, T):
+    return gl.GwHipKernel(d["nk"], d["nao"], d["nso"], d["ns"], d["NQ"], T.nts, T.nw_b, d["beta"], mesh=d["mesh"])
+
+
+def _gpu(G, V, T, d):
+    k = _kernel(d, T)
+    try:
+        k.set_grid(T.T_bt, T.T_tb)
+        k.upload_V(V)
+        S = k.solve(G)
+        P = k.P_tilde()
+    finally:
+        k.close()
+    return S, P
+
+
+def _oracle(G, V, T, d):
+    return go.gw_sigma(G, V, T.T_bt, T.T_tb, d["nk"], d["nao"], d["nso"], d["ns"], d["NQ"], mesh=d["mesh"],
+                       return_P=True)
+
+
+def test_config0_parity():
+    """2x2x2 mesh, nao 8, naux 40, ns 1, beta 50, IR 1e3, seed 0."""
+    c = CONFIG0
+    G, V, T, d = synthetic(c["nk"], c["nao"], c["NQ"], ns=c["ns"], beta=c["beta"], lam=c["lam"], mesh=c["mesh"])
+    S, P = _gpu(G, V, T, d)
+    S_ref, P_ref = _oracle(G, V, T, d)
+    assert rel_err(P, P_ref) < TOL_FP64
+    assert rel_err(S, S_ref) < TOL_FP64
+
+
+@pytest.mark.parametrize("name", ["gw_x2c", "seet_dc"])
+def test_reference_fixtures(name):
+    G, V, T, d, S_fix = load_fixture(name)
+    S, _ = _gpu(G, V, T, d)
+    S_ref, _ = _oracle(G, V, T, d)
+    assert rel_err(S, S_ref) < TOL_FP64
+    assert np.abs(S - S_fix).max() < 1e-6
+
+
+@pytest.mark.parametrize("case", [
+    dict(nk=3, mesh=(3, 1, 1), nao=3, NQ=5, ns=2),             # ragged tiles everywhere, unrestricted
+    dict(nk=2, mesh=(2, 1, 1), nao=3, NQ=7, ns=1, nso=6),      # X2C spin blocks, k > 1
+    dict(nk=1, mesh=(1, 1, 1), nao=1, NQ=1, ns=1),             # scalar
+    dict(nk=4, mesh=(2, 2, 1), nao=5, NQ=70, ns=1),            # NQ spans two GEMM tiles
+])
+def test_small_cases(case):
+    nso = case.get("nso", case["nao"])
+    G, V, T, d = synthetic(case["nk"], case["nao"], case["NQ"], ns=case["ns"], beta=20.0, lam=1e3,
+                           mesh=case["mesh"], seed=7, nso=nso, sigma=0.3)
+    S, P = _gpu(G, V, T, d)
+    S_ref, P_ref = _oracle(G, V, T, d)
+    assert rel_err(P, P_ref) < TOL_FP64
+    assert rel_err(S, S_ref) < TOL_FP64
+
+
+def test_device_generator_matches_host_generator():
+    G, V, T, d = synthetic(8, 4, 12, beta=50.0, lam=1e3, mesh=(2, 2, 2), seed=11)
+    k = _kernel(d, T)
+    try:
+        k.set_grid(T.T_bt, T.T_tb)
+        k.upload_V(V)
+        S_up = k.solve(G)
+        k.generate_V(11, 0.1)
+        S_gen = k.solve(G)
+    finally:
+        k.close()
+    assert rel_err(S_gen, S_up) < 1e-12
+
+
+def test_repeatable_bitwise():
+    G, V, T, d = synthetic(8, 6, 20, beta=50.0, lam=1e3, mesh=(2, 2, 2), seed=2)
+    k = _kernel(d, T)
+    try:
+        k.set_grid(T.T_bt, T.T_tb)
+        k.upload_V(V)
+        S1 = k.solve(G)
+        S2 = k.solve(G)
+    finally:
+        k.close()
+    assert np.array_equal(S1, S2)
+
+
+def test_state_errors():
+    G, V, T, d = synthetic(1, 2, 3, beta=20.0, lam=1e3, mesh=(1, 1, 1), seed=1)
+    k = _kernel(d, T)
+    try:
+        with pytest.raises(gl.GwhipError) as e:
+            k.solve(G)
+        assert e.value.status == gl.ERR_STATE
+        k.set_grid(T.T_bt, T.T_tb)
+        with pytest.raises(gl.GwhipError) as e:
+            k.solve(G)
+        assert e.value.status == gl.ERR_STATE
+    finally:
+        k.close()
+
+
+def test_nonfinite_input_is_reported():
+    G, V, T, d = synthetic(1, 2, 3, beta=20.0, lam=1e3, mesh=(1, 1, 1), seed=1)
+    G[3, 0, 0, 0, 0] = np.nan
+    k = _kernel(d, T)
+    try:
+        k.set_grid(T.T_bt, T.T_tb)
+        k.upload_V(V)
+        with pytest.raises(gl.GwhipError) as e:
+            k.solve(G)
+        assert e.value.status == gl.ERR_NONFINITE
+    finally:
+        k.close()
