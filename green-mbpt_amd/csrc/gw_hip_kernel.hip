@@ -408,6 +408,43 @@ void gw_hip_kernel::solve_host(const double2* G, double2* Sigma, int precision) 
   GWHIP_CHECK(hipMemcpy(Sigma, S_dev_, nsig * sizeof(double2), hipMemcpyDeviceToHost));
 }
 
+void gw_hip_kernel::run_stages_host(const double2* G, int last_stage) {
+  if (!grid_set_ || !v_set_) throw status_error(GWHIP_ERR_STATE, "grid or integrals not set");
+  if (!G) throw status_error(GWHIP_ERR_ARG, "null G");
+  GWHIP_CHECK(hipSetDevice(device_));
+  const size_t nsig = (size_t)d_.nts * d_.ns * d_.nk * d_.nso * d_.nso;
+  if (!G_dev_) G_dev_ = dalloc<double2>(nsig);
+  GWHIP_CHECK(hipMemcpy(G_dev_, G, nsig * sizeof(double2), hipMemcpyHostToDevice));
+  GWHIP_CHECK(hipMemsetAsync(P_, 0, (size_t)d_.nk * d_.nts * d_.NQ * d_.NQ * sizeof(double2), stream_));
+  stage_p0(G_dev_);
+  if (last_stage >= 2) stage_dyson();
+  GWHIP_CHECK(hipStreamSynchronize(stream_));
+}
+
+void debug_zgemm(int M, int N, int K, const double2* A, const double2* B, double2* C, int conjA, int conjB) {
+  if (M <= 0 || N <= 0 || K <= 0 || !A || !B || !C) throw status_error(GWHIP_ERR_ARG, "bad debug_zgemm arguments");
+  GWHIP_CHECK(hipSetDevice(0));
+  double2 *dA = nullptr, *dB = nullptr, *dC = nullptr;
+  GWHIP_CHECK(hipMalloc(&dA, (size_t)M * K * sizeof(double2)));
+  GWHIP_CHECK(hipMalloc(&dB, (size_t)K * N * sizeof(double2)));
+  GWHIP_CHECK(hipMalloc(&dC, (size_t)M * N * sizeof(double2)));
+  GWHIP_CHECK(hipMemcpy(dA, A, (size_t)M * K * sizeof(double2), hipMemcpyHostToDevice));
+  GWHIP_CHECK(hipMemcpy(dB, B, (size_t)K * N * sizeof(double2), hipMemcpyHostToDevice));
+  ZGemmArgs a{};
+  a.M = M; a.N = N; a.K = K; a.batch = 1; a.kin = K; a.nin = N;
+  a.A = dA; a.sAm = K; a.sAki = 1;
+  a.B = dB; a.sBki = N; a.sBni = 1;
+  a.C = dC; a.sCm = N; a.sCni = 1;
+  a.alpha = make_double2(1.0, 0.0);
+  a.conjA = conjA; a.conjB = conjB;
+  zgemm_f64(a, true, false, nullptr);
+  GWHIP_CHECK(hipDeviceSynchronize());
+  GWHIP_CHECK(hipMemcpy(C, dC, (size_t)M * N * sizeof(double2), hipMemcpyDeviceToHost));
+  (void)hipFree(dA);
+  (void)hipFree(dB);
+  (void)hipFree(dC);
+}
+
 void gw_hip_kernel::copy_P_tilde(double2* host) const {
   GWHIP_CHECK(hipSetDevice(device_));
   const size_t n = (size_t)d_.nk * d_.nts * d_.NQ * d_.NQ;
@@ -514,5 +551,17 @@ int gwhip_kernel_stats(const gwhip_handle* h, int which, int64_t* launches, doub
 }
 
 size_t gwhip_device_bytes(const gwhip_handle* h) { return h ? h->k->device_bytes() : 0; }
+
+int gwhip_debug_zgemm(int M, int N, int K, const double* A, const double* B, double* C, int conjA, int conjB) {
+  gwhip_handle tmp;
+  return guarded(&tmp, [&] {
+    gwhip::debug_zgemm(M, N, K, (const double2*)A, (const double2*)B, (double2*)C, conjA, conjB);
+  });
+}
+
+int gwhip_debug_stage(gwhip_handle* h, const double* G, int last_stage) {
+  if (!h) return GWHIP_ERR_ARG;
+  return guarded(h, [&] { h->k->run_stages_host((const double2*)G, last_stage); });
+}
 
 }  // extern "C"

@@ -49,6 +49,8 @@ public:
   // G_dev, Sigma_dev: device pointers [nts][ns][nk][nso][nso]
   void solve_device(const double2* G_dev, double2* Sigma_dev, int precision);
   void solve_host(const double2* G, double2* Sigma, int precision);
+  // test hook: stop after stage 1 (P0) or 2 (P-tilde); stage 3 runs Sigma as usual
+  void run_stages_host(const double2* G, int last_stage);
   void copy_P_tilde(double2* host) const;
 
   const double* region_ms() const { return region_ms_; }

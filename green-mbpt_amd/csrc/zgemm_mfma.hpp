@@ -159,7 +159,8 @@ __global__ __launch_bounds__(256) void zgemm_f64_kernel(ZGemmArgs p) {
     }
   }
 
-  // epilogue: lane holds rows 4*(lane/16)+r, column lane%16 of each 16x16 block
+  // epilogue: register r of lane l holds row (l/16) + 4*r, column l%16 of each 16x16 block
+  // (v_mfma_f64_16x16x4f64 result layout, checked by the identity-probe test)
 #pragma unroll
   for (int x = 0; x < 2; ++x)
 #pragma unroll
@@ -170,7 +171,7 @@ __global__ __launch_bounds__(256) void zgemm_f64_kernel(ZGemmArgs p) {
       const int64_t coff = (int64_t)no * p.sCno + (int64_t)(n - no * p.nin) * p.sCni;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int m = m0 + wm + x * 16 + 4 * lk + r;
+        const int m = m0 + wm + x * 16 + lk + 4 * r;
         if (m >= p.M) continue;
         const double cr = acc_r[x][y][r], ci = acc_i[x][y][r];
         double2 v = make_double2(p.alpha.x * cr - p.alpha.y * ci, p.alpha.x * ci + p.alpha.y * cr);

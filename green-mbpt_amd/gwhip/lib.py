@@ -69,7 +69,9 @@ def load() -> ctypes.CDLL:
                                        ctypes.POINTER(ctypes.c_double)]
     lib.gwhip_device_bytes.argtypes = [P]
     lib.gwhip_device_bytes.restype = ctypes.c_size_t
-    for f in ("gwhip_set_grid", "gwhip_upload_V", "gwhip_generate_V", "gwhip_eval_sigma",
+    lib.gwhip_debug_zgemm.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, P, ctypes.c_int, ctypes.c_int]
+    lib.gwhip_debug_stage.argtypes = [P, P, ctypes.c_int]
+    for f in ("gwhip_debug_zgemm", "gwhip_debug_stage", "gwhip_set_grid", "gwhip_upload_V", "gwhip_generate_V", "gwhip_eval_sigma",
               "gwhip_eval_sigma_device", "gwhip_get_P_tilde", "gwhip_timings", "gwhip_kernel_stats"):
         getattr(lib, f).restype = ctypes.c_int
     _lib = lib
@@ -80,6 +82,21 @@ def header_functions() -> list[str]:
     """Names of every function the C header declares."""
     text = open(HEADER).read()
     return sorted(set(re.findall(r"^\s*(?:const\s+)?[\w\s\*]+?\b(gwhip_\w+)\s*\(", text, re.M)))
+
+
+def debug_zgemm(A: np.ndarray, B: np.ndarray, conjA: bool = False, conjB: bool = False) -> np.ndarray:
+    """The library's FP64 MFMA GEMM on host arrays (test hook)."""
+    lib = load()
+    a = np.ascontiguousarray(A, dtype=np.complex128)
+    b = np.ascontiguousarray(B, dtype=np.complex128)
+    M, K = a.shape
+    K2, N = b.shape
+    assert K == K2
+    c = np.empty((M, N), dtype=np.complex128)
+    st = lib.gwhip_debug_zgemm(M, N, K, a.ctypes.data, b.ctypes.data, c.ctypes.data, int(conjA), int(conjB))
+    if st != OK:
+        raise GwhipError(st, "debug_zgemm failed")
+    return c
 
 
 def _ptr(a: np.ndarray) -> ctypes.c_void_p:
@@ -148,6 +165,10 @@ class GwHipKernel:
     def solve_device(self, G_ptr: int, S_ptr: int, precision: int = PREC_FP64):
         self._check(self.lib.gwhip_eval_sigma_device(self.h, ctypes.c_void_p(G_ptr), ctypes.c_void_p(S_ptr),
                                                      precision))
+
+    def run_stages(self, G: np.ndarray, last_stage: int):
+        g = np.ascontiguousarray(G, dtype=np.complex128)
+        self._check(self.lib.gwhip_debug_stage(self.h, _ptr(g), last_stage))
 
     def P_tilde(self) -> np.ndarray:
         d = self.dims

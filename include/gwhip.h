@@ -119,6 +119,17 @@ int gwhip_kernel_stats(const gwhip_handle* h, int which, int64_t* launches, doub
 /* Device bytes held by the handle. */
 size_t gwhip_device_bytes(const gwhip_handle* h);
 
+/* ---- test hooks (no reference counterpart; used by the parity tests) ---- */
+
+/* C = opA(A) opB(B) with the library's FP64 MFMA GEMM on device 0; row-major host
+ * arrays A [M][K], B [K][N], C [M][N] complex; op = conjugate when the flag is set. */
+int gwhip_debug_zgemm(int M, int N, int K, const double* A, const double* B, double* C, int conjA, int conjB);
+
+/* Run an evaluation only up to stage `last_stage` (1: P0 on the half tau grid after
+ * Hermitisation, 2: P-tilde on all tau) and leave the result where gwhip_get_P_tilde
+ * reads it.  Sigma is not produced. */
+int gwhip_debug_stage(gwhip_handle* h, const double* G, int last_stage);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,77 @@
+"""Unit parity of the HIP building blocks (needs an MI355X): the FP64 MFMA GEMM and the
+intermediates P0(tau, q) and P-tilde(tau, q) against the oracle."""
+import numpy as np
+import pytest
+
+from gwhip import lib as gl
+from helpers import CONFIG0, rel_err, synthetic
+from oracle import gw_oracle as go
+
+pytestmark = pytest.mark.gpu
+
+
+def test_zgemm_identity_probe():
+    """A = I makes C = B; a wrong MFMA lane/register mapping shows up as a permutation."""
+    n = 16
+    A = np.eye(n, dtype=complex)
+    B = (np.arange(n)[:, None] * 100 + np.arange(n)[None, :]).astype(complex)
+    C = gl.debug_zgemm(A, B)
+    if not np.array_equal(C, B):
+        bad = np.argwhere(C != B)
+        raise AssertionError(f"layout mismatch at {len(bad)} entries; first: C{tuple(bad[0])}="
+                             f"{C[tuple(bad[0])]} expected {B[tuple(bad[0])]}")
+
+
+@pytest.mark.parametrize("shape", [(1, 1, 1), (16, 16, 4), (64, 64, 16), (70, 130, 37), (300, 257, 41)])
+@pytest.mark.parametrize("conj", [(False, False), (True, False), (False, True)])
+def test_zgemm_random(shape, conj):
+    M, N, K = shape
+    rng = np.random.default_rng(M * 1000 + N + K)
+    A = rng.standard_normal((M, K)) + 1j * rng.standard_normal((M, K))
+    B = rng.standard_normal((K, N)) + 1j * rng.standard_normal((K, N))
+    C = gl.debug_zgemm(A, B, *conj)
+    Aop = np.conj(A) if conj[0] else A
+    Bop = np.conj(B) if conj[1] else B
+    ref = Aop @ Bop
+    assert np.abs(C - ref).max() < 1e-13 * max(1.0, np.abs(ref).max()) * np.sqrt(K)
+
+
+def _setup(G, V, T, d):
+    k = gl.GwHipKernel(d["nk"], d["nao"], d["nso"], d["ns"], d["NQ"], T.nts, T.nw_b, d["beta"], mesh=d["mesh"])
+    k.set_grid(T.T_bt, T.T_tb)
+    k.upload_V(V)
+    return k
+
+
+def _oracle_P0(G, V, d):
+    nts = G.shape[0]
+    mesh = go.mesh_shape(d["nk"], d["mesh"])
+    out = []
+    for q in range(d["nk"]):
+        P0 = np.zeros((nts, d["NQ"], d["NQ"]), complex)
+        go.eval_P0_tilde(P0, G, V, q, d["nk"], d["nao"], d["nso"], d["ns"], d["NQ"], d["nso"] != d["nao"], mesh)
+        go.symmetrize_P0(P0, d["nk"])
+        out.append(P0)
+    return np.stack(out)
+
+
+@pytest.mark.parametrize("case", ["config0", "ragged"])
+def test_stage_P0_and_P_tilde(case):
+    if case == "config0":
+        c = CONFIG0
+        G, V, T, d = synthetic(c["nk"], c["nao"], c["NQ"], beta=c["beta"], lam=c["lam"], mesh=c["mesh"])
+    else:
+        G, V, T, d = synthetic(3, 3, 5, ns=2, beta=20.0, lam=1e3, mesh=(3, 1, 1), seed=4, sigma=0.3)
+    half = T.nts // 2
+    P0_ref = _oracle_P0(G, V, d)
+    k = _setup(G, V, T, d)
+    try:
+        k.run_stages(G, 1)
+        P0 = k.P_tilde()
+        k.run_stages(G, 2)
+        Pt = k.P_tilde()
+    finally:
+        k.close()
+    assert rel_err(P0[:, :half], P0_ref[:, :half]) < 1e-12
+    Pt_ref = np.stack([go.eval_P_tilde(P0_ref[q], T.T_bt, T.T_tb) for q in range(d["nk"])])
+    assert rel_err(Pt, Pt_ref) < 1e-10
