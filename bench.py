@@ -22,6 +22,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "GW Σ-evals/s @nk64 nao26 naux300; split-prec vs FP64 at rel err ≤1e-8"
 PEAK_FP64_TFLOPS = 78.6
+PEAK_INT8_TOPS = 5000.0
 
 WORKLOAD = dict(nk=64, mesh=(4, 4, 4), nao=26, NQ=300, ns=1, beta=100.0, lam=1e4, seed=0, sigma=0.1)
 
@@ -82,7 +83,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--precision", choices=["fp64", "split"], default="fp64")
+    ap.add_argument("--precision", choices=["fp64", "split"], default="split")
+    ap.add_argument("--slices", type=int, default=5, help="int8 slices per operand in split precision")
+    ap.add_argument("--no-check", action="store_true", help="skip the FP64 reference evaluation")
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -108,6 +111,8 @@ def main():
     G_d = torch.from_numpy(G.view(np.float64)).to(dev)
     S_d = torch.empty_like(G_d)
     prec = gl.PREC_FP64 if args.precision == "fp64" else gl.PREC_SPLIT
+    if prec == gl.PREC_SPLIT:
+        k.set_split_slices(args.slices)
     torch.cuda.synchronize()
     for i in range(args.warmup):
         k.solve_device(G_d.data_ptr(), S_d.data_ptr(), prec)
@@ -126,27 +131,52 @@ def main():
     ms_per_step = elapsed / args.steps * 1e3
     regions = k.timings()
     fl = algorithmic_flops(w["nk"], w["nao"], w["NQ"], T.nts, T.nw_b)
+    S_host = S_d.cpu().numpy().view(np.complex128).copy()
+    herm = float(np.abs(S_host - np.conj(np.swapaxes(S_host, -1, -2))).max() / np.abs(S_host).max())
 
-    # dominant kernel: the Sigma contraction GEMM (R = P~ conj(V)), one launch per q
+    # dominant kernel: the Sigma contraction GEMM (R = P~ conj(V)), one launch per q (all 100 tau)
     n_l, ms_l = stats["Sigma GEMM"]
     avg_s = ms_l / max(n_l, 1) * 1e-3
-    per_launch = 8.0 * w["NQ"] * w["NQ"] * w["nk"] * w["nao"] ** 2 * T.nts   # nao^2 NQ^2 part, all k and tau of one q
-    achieved = per_launch / avg_s / 1e12
-    S_host = S_d.cpu().numpy().view(np.complex128)
-    herm = float(np.abs(S_host - np.conj(np.swapaxes(S_host, -1, -2))).max() / np.abs(S_host).max())
+    per_launch = 8.0 * w["NQ"] * w["NQ"] * w["nk"] * w["nao"] ** 2 * T.nts   # complex MACs x 8, all k and tau of one q
+    if prec == gl.PREC_FP64:
+        roof = {"bound": "mfma", "achieved": per_launch / avg_s / 1e12, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
+                "kernel": "zgemm_f64_kernel<64,64,true,false> (Sigma contraction R = P~ conj(V), FP64 MFMA)"}
+    else:
+        ns = args.slices
+        kpad = -(-w["NQ"] // 64) * 64
+        # int8 MFMA operations actually issued: 4 real products per complex product, ns(ns+1)/2 slice pairs
+        ops = 2.0 * 4 * (ns * (ns + 1) // 2) * w["NQ"] * kpad * w["nk"] * w["nao"] ** 2 * T.nts
+        roof = {"bound": "mfma", "achieved": ops / avg_s / 1e12, "peak": PEAK_INT8_TOPS, "unit": "TOPS",
+                "kernel": f"zgemm_ozaki_kernel<{ns}> (Sigma contraction, int8 slices)",
+                "algorithmic_fp64_tflops": per_launch / avg_s / 1e12}
+    roof["frac"] = roof["achieved"] / roof["peak"]
+    roof.update({"traffic": None, "launches": n_l, "avg_launch_ms": avg_s * 1e3})
+
+    check = {}
+    if prec == gl.PREC_SPLIT and not args.no_check:
+        from oracle import gw_oracle as go
+        t1 = time.perf_counter()
+        k.solve_device(G_d.data_ptr(), S_d.data_ptr(), gl.PREC_FP64)
+        torch.cuda.synchronize()
+        fp64_ms = (time.perf_counter() - t1) * 1e3
+        S_ref = S_d.cpu().numpy().view(np.complex128)
+        err = float(np.abs(S_host - S_ref).max() / np.abs(S_ref).max())
+        E_ref = go.gm_energy(G, S_ref, T.T_ft, T.T_tf, w["nk"], w["ns"])
+        E_split = go.gm_energy(G, S_host, T.T_ft, T.T_tf, w["nk"], w["ns"])
+        check = {"fp64_ms_per_step": fp64_ms, "speedup_vs_fp64": fp64_ms / ms_per_step,
+                 "max_rel_sigma_err_vs_fp64": err, "gm_energy_fp64": E_ref, "gm_energy_split": E_split,
+                 "gm_energy_rel_err": abs(E_split - E_ref) / abs(E_ref), "slices": args.slices}
 
     out = {
         "metric": METRIC, "value": 1e3 / ms_per_step, "unit": "Σ-evals/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_per_step, "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f64" if prec == gl.PREC_FP64 else "split-bf16/f64",
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64" if prec == gl.PREC_FP64 else "int8-slices/i32/f64",
         "data": "synthetic (seed 0: Haar-random Fock eigenbasis, V ~ N(0, 0.1^2) generated on device)",
         "config": {"workload": "GW Sigma(tau,k), 4x4x4 k mesh, nao 26, naux 300, ns 1, beta 100, IR 1e4",
                    "nk": w["nk"], "nao": w["nao"], "naux": w["NQ"], "nts": T.nts, "nw_b": T.nw_b,
                    "precision": args.precision, "parallelism": "single GPU"},
-        "roofline": {"bound": "mfma", "achieved": achieved, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP64_TFLOPS, "traffic": None,
-                     "kernel": "zgemm_f64_kernel<true,false> (Sigma contraction R = P~ conj(V))",
-                     "launches": n_l, "avg_launch_ms": avg_s * 1e3},
+        "roofline": roof,
+        "precision_check": check,
         "algorithmic_tflops_per_step": fl["total"] / 1e12,
         "whole_step_tflops": fl["total"] / (ms_per_step * 1e-3) / 1e12,
         "regions_ms": regions,

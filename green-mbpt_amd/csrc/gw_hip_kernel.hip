@@ -8,7 +8,9 @@
 
 #include "../../include/gwhip.h"
 #include "gw_kernels.hpp"
+#include "gw_mfma_kernels.hpp"
 #include "zgemm_mfma.hpp"
+#include "zgemm_ozaki.hpp"
 
 namespace gwhip {
 
@@ -70,7 +72,7 @@ gw_hip_kernel::gw_hip_kernel(const Dims& d, int device) : d_(d), device_(device)
   d_.pns = d_.nso / d_.nao;
   d_.pseudo_ns = d_.X2C ? 4 : d_.ns;
   d_.prefactor = (d_.ns == 2 || d_.X2C) ? 1.0 : 2.0;
-  if (d_.nao > 45) throw status_error(GWHIP_ERR_UNSUPPORTED, "nao > 45 not supported by the epilogue kernels");
+  if (d_.nao > 48) throw status_error(GWHIP_ERR_UNSUPPORTED, "nao > 48 not supported by the orbital contractions");
 
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) {
@@ -83,6 +85,7 @@ gw_hip_kernel::gw_hip_kernel(const Dims& d, int device) : d_(d), device_(device)
 
   const int64_t nk = d_.nk, NQ = d_.NQ, nao2 = d_.nao * d_.nao, NQ2 = NQ * NQ;
   Vq_ = dalloc<double2>((size_t)nk * nk * NQ * nao2);
+  Vt_ = dalloc<double2>((size_t)nk * nk * NQ * nao2);
   P_ = dalloc<double2>((size_t)nk * d_.nts * NQ2);
   Tbt_fold_ = dalloc<double2>((size_t)d_.nw_b * (d_.nts / 2));
   Ttb_ = dalloc<double2>((size_t)d_.nts * d_.nw_b);
@@ -112,28 +115,51 @@ gw_hip_kernel::gw_hip_kernel(const Dims& d, int device) : d_(d), device_(device)
 gw_hip_kernel::~gw_hip_kernel() {
   (void)hipSetDevice(device_);
   if (stream_) (void)hipStreamSynchronize(stream_);
-  for (void* p : {(void*)Vq_, (void*)P_, (void*)Pw_, (void*)X_, (void*)Y_, (void*)scratch_, (void*)Tbt_fold_,
+  for (void* p : {(void*)Vq_, (void*)Vt_, (void*)P_, (void*)Pw_, (void*)X_, (void*)Y_, (void*)scratch_, (void*)Tbt_fold_,
                   (void*)Ttb_, (void*)G_dev_, (void*)S_dev_, (void*)kplus_tab_, (void*)kminus_tab_, (void*)flag_})
     if (p) (void)hipFree(p);
+  free_slices();
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
 void gw_hip_kernel::build_maps() {
   const int nk = (int)d_.nk;
-  std::vector<int> kp((size_t)nk * nk), kpq((size_t)nk * nk), kmq((size_t)nk * nk);
-  neg_q_.assign(nk, 0);
-  for (int q = 0; q < nk; ++q) {
-    neg_q_[q] = k_shift(0, q, d_.mesh, -1);
+  std::vector<int> kpq((size_t)nk * nk), kmq((size_t)nk * nk);
+  for (int q = 0; q < nk; ++q)
     for (int k = 0; k < nk; ++k) {
-      kp[(size_t)k * nk + q] = k_shift(k, q, d_.mesh, +1);   // [k][q] for the V expansion
       kpq[(size_t)q * nk + k] = k_shift(k, q, d_.mesh, +1);  // [q][k]: k + q
       kmq[(size_t)q * nk + k] = k_shift(k, q, d_.mesh, -1);  // [q][k]: k - q
     }
-  }
-  kplus_host_ = kp;
-  // kplus_tab_ holds [q][k] (k + q); the [k][q] form is only needed transiently
+  upload_maps(kpq, kmq);
+}
+
+void gw_hip_kernel::upload_maps(const std::vector<int>& kpq, const std::vector<int>& kmq) {
+  const int nk = (int)d_.nk;
+  for (int v : kpq)
+    if (v < 0 || v >= nk) throw status_error(GWHIP_ERR_ARG, "k + q map entry out of range");
+  for (int v : kmq)
+    if (v < 0 || v >= nk) throw status_error(GWHIP_ERR_ARG, "k - q map entry out of range");
+  // V(k, k - q) lives in the store of the q' with k + q' = k - q for every k
+  neg_q_.assign(nk, -1);
+  for (int q = 0; q < nk; ++q)
+    for (int qp = 0; qp < nk && neg_q_[q] < 0; ++qp)
+      if (std::equal(kpq.begin() + (size_t)qp * nk, kpq.begin() + (size_t)(qp + 1) * nk, kmq.begin() + (size_t)q * nk))
+        neg_q_[q] = qp;
+  for (int q = 0; q < nk; ++q)
+    if (neg_q_[q] < 0) throw status_error(GWHIP_ERR_ARG, "k - q map has no matching k + q' row");
+  kplus_host_.assign((size_t)nk * nk, 0);  // [k][q] for the V expansion
+  for (int q = 0; q < nk; ++q)
+    for (int k = 0; k < nk; ++k) kplus_host_[(size_t)k * nk + q] = kpq[(size_t)q * nk + k];
   GWHIP_CHECK(hipMemcpy(kplus_tab_, kpq.data(), kpq.size() * sizeof(int), hipMemcpyHostToDevice));
   GWHIP_CHECK(hipMemcpy(kminus_tab_, kmq.data(), kmq.size() * sizeof(int), hipMemcpyHostToDevice));
+}
+
+void gw_hip_kernel::set_kq_maps(const int32_t* kplus, const int32_t* kminus) {
+  if (!kplus || !kminus) throw status_error(GWHIP_ERR_ARG, "null k-q map");
+  GWHIP_CHECK(hipSetDevice(device_));
+  const size_t n = (size_t)d_.nk * d_.nk;
+  upload_maps(std::vector<int>(kplus, kplus + n), std::vector<int>(kminus, kminus + n));
+  v_set_ = false;  // the V store is indexed by the maps
 }
 
 void gw_hip_kernel::set_grid(const double2* T_bt, const double2* T_tb) {
@@ -176,7 +202,71 @@ void gw_hip_kernel::upload_V(const double2* V_packed) {
   (void)hipFree(tmp);
   if (tab) (void)hipFree(tab);
   GWHIP_CHECK(e);
+  make_transposed_V();
   v_set_ = true;
+}
+
+void gw_hip_kernel::free_slices() {
+  for (void* p : {(void*)Vs_p0_, (void*)eVs_p0_, (void*)Vs_s_, (void*)eVs_s_, (void*)Zs_, (void*)eZs_, (void*)Ps_,
+                  (void*)ePs_})
+    if (p) (void)hipFree(p);
+  Vs_p0_ = Vs_s_ = Zs_ = Ps_ = nullptr;
+  eVs_p0_ = eVs_s_ = eZs_ = ePs_ = nullptr;
+  slices_ns_ = 0;
+}
+
+void gw_hip_kernel::set_split_slices(int ns) {
+  if (ns < 3 || ns > 5) throw status_error(GWHIP_ERR_ARG, "split slices must be 3, 4 or 5");
+  if (ns != split_ns_) {
+    GWHIP_CHECK(hipSetDevice(device_));
+    GWHIP_CHECK(hipStreamSynchronize(stream_));
+    free_slices();
+  }
+  split_ns_ = ns;
+}
+
+// Slice stores of conj(V) (built once per V and slice count).
+void gw_hip_kernel::ensure_slices() {
+  if (slices_ns_ == split_ns_ && Vs_p0_) return;
+  free_slices();
+  const int ns = split_ns_;
+  const int64_t nk = d_.nk, NQ = d_.NQ, nao2 = d_.nao * d_.nao;
+  kc_p0_ = ceil_div(nao2, 64) * 64;
+  kc_s_ = ceil_div(NQ, 64) * 64;
+  Vs_p0_ = dalloc<int8_t>((size_t)nk * NQ * nk * 2 * ns * kc_p0_);
+  eVs_p0_ = dalloc<int>((size_t)nk * NQ * nk);
+  Vs_s_ = dalloc<int8_t>((size_t)nk * nk * nao2 * 2 * ns * kc_s_);
+  eVs_s_ = dalloc<int>((size_t)nk * nk * nao2);
+  Zs_ = dalloc<int8_t>((size_t)tchunk_p0_ * NQ * nk * 3 * ns * kc_p0_);
+  eZs_ = dalloc<int>((size_t)tchunk_p0_ * NQ * nk);
+  Ps_ = dalloc<int8_t>((size_t)tchunk_sigma_ * NQ * 3 * ns * kc_s_);
+  ePs_ = dalloc<int>((size_t)tchunk_sigma_ * NQ);
+  {  // P0 operand: rows (q, Q'), chunks k, elements (t, n) of V(k, k+q)
+    SliceArgs a{};
+    a.src = Vq_; a.nrows = nk * NQ; a.row_div = NQ; a.s_rhi = nk * NQ * nao2; a.s_rlo = nao2;
+    a.nchunk = (int)nk; a.s_c = NQ * nao2; a.klen = (int)nao2; a.s_k = 1; a.kc = (int)kc_p0_;
+    a.conj = 1; a.nplanes = 2; a.dst = Vs_p0_; a.exps = eVs_p0_;
+    slice_rows(a, ns, stream_);
+  }
+  {  // Sigma operand: rows (q, k, (n, j)) of the transposed store, elements Q'
+    SliceArgs a{};
+    a.src = Vt_; a.nrows = nk * nk * nao2; a.row_div = nao2; a.s_rhi = NQ * nao2; a.s_rlo = 1;
+    a.nchunk = 1; a.s_c = 0; a.klen = (int)NQ; a.s_k = nao2; a.kc = (int)kc_s_;
+    a.conj = 1; a.nplanes = 2; a.dst = Vs_s_; a.exps = eVs_s_;
+    slice_rows(a, ns, stream_);
+  }
+  GWHIP_CHECK(hipStreamSynchronize(stream_));
+  slices_ns_ = ns;
+}
+
+void gw_hip_kernel::make_transposed_V() {
+  free_slices();
+  const int64_t nmat = d_.nk * d_.nk * d_.NQ;
+  const int64_t total = nmat * d_.nao * d_.nao;
+  hipLaunchKernelGGL(transpose_v_kernel, dim3((unsigned)ceil_div(total, 256)), dim3(256), 0, stream_, Vq_, Vt_,
+                     (int)d_.nao, nmat);
+  GWHIP_CHECK(hipGetLastError());
+  GWHIP_CHECK(hipStreamSynchronize(stream_));
 }
 
 void gw_hip_kernel::generate_V(uint64_t seed, double sigma) {
@@ -202,6 +292,7 @@ void gw_hip_kernel::generate_V(uint64_t seed, double sigma) {
   }
   (void)hipFree(tab);
   GWHIP_CHECK(e);
+  make_transposed_V();
   v_set_ = true;
 }
 
@@ -222,25 +313,63 @@ void gw_hip_kernel::timed(int which, F&& launch) {
   kstat_[which].ms += ms;
 }
 
-void gw_hip_kernel::stage_p0(const double2* G) {
+namespace {
+template <int NB>
+void launch_z_prep(dim3 grid, size_t lds, hipStream_t st, const DevDims& dd, const double2* G, const double2* Vt,
+                   double2* Z, const int* kplus, int t0, int QPW) {
+  allow_lds(z_prep_mfma_kernel<NB>, lds);
+  hipLaunchKernelGGL(z_prep_mfma_kernel<NB>, grid, dim3(256), lds, st, dd, G, Vt, Z, kplus, t0, QPW);
+  GWHIP_CHECK(hipGetLastError());
+}
+template <int NB>
+void launch_sigma_epi(dim3 grid, size_t lds, hipStream_t st, const DevDims& dd, const double2* G, const double2* Vt,
+                      const double2* R, double2* S, const int* kminus, int t0, double scale) {
+  allow_lds(sigma_epi_mfma_kernel<NB>, lds);
+  hipLaunchKernelGGL(sigma_epi_mfma_kernel<NB>, grid, dim3(256), lds, st, dd, G, Vt, R, S, kminus, t0, scale);
+  GWHIP_CHECK(hipGetLastError());
+}
+}  // namespace
+
+void gw_hip_kernel::stage_p0(const double2* G, bool split) {
   const int64_t nk = d_.nk, NQ = d_.NQ, nao2 = d_.nao * d_.nao, NQ2 = NQ * NQ, half = d_.nts / 2;
   DevDims dd{(int)d_.nts, (int)d_.ns, (int)nk, (int)d_.nso, (int)d_.nao, (int)NQ, (int)d_.pns, (int)d_.pseudo_ns};
-  const int64_t lds_unit = nao2 * (int64_t)sizeof(double2);
-  int QC = (int)std::min<int64_t>(max_lds_ / lds_unit / 2 - 1, (int64_t)Z_MAXO * 256 / nao2);
-  QC = std::max(1, std::min<int>(QC, (int)NQ));
-  const size_t lds = (size_t)(2 + 2 * QC) * lds_unit;
-  if ((int64_t)lds > max_lds_) throw status_error(GWHIP_ERR_UNSUPPORTED, "nao too large for the P0 prep kernel");
-  allow_lds(z_prep_kernel, lds);
+  const int NB = (int)ceil_div(d_.nao, 16);
+  const size_t lds = (size_t)4 * (16 * NB) * (16 * NB + 1) * sizeof(double);
+  const int QPW = 4;
+  const unsigned gx = (unsigned)ceil_div(NQ, 4 * QPW);
 
   for (int64_t q = 0; q < nk; ++q) {
     const double2* Vq = Vq_ + q * nk * NQ * nao2;
+    const double2* Vtq = Vt_ + q * nk * NQ * nao2;
     for (int64_t t0 = 0; t0 < half; t0 += tchunk_p0_) {
       const int64_t tc = std::min(tchunk_p0_, half - t0);
       timed(K_P0_PREP, [&] {
-        hipLaunchKernelGGL(z_prep_kernel, dim3((unsigned)ceil_div(NQ, QC), (unsigned)nk, (unsigned)tc), dim3(256), lds,
-                           stream_, dd, G, Vq, scratch_, kplus_tab_ + q * nk, (int)t0, QC);
-        GWHIP_CHECK(hipGetLastError());
+        const dim3 grid(gx, (unsigned)nk, (unsigned)tc);
+        const int* kp = kplus_tab_ + q * nk;
+        if (NB == 1) launch_z_prep<1>(grid, lds, stream_, dd, G, Vtq, scratch_, kp, (int)t0, QPW);
+        else if (NB == 2) launch_z_prep<2>(grid, lds, stream_, dd, G, Vtq, scratch_, kp, (int)t0, QPW);
+        else launch_z_prep<3>(grid, lds, stream_, dd, G, Vtq, scratch_, kp, (int)t0, QPW);
       });
+      if (split) {
+        const int ns = split_ns_;
+        SliceArgs sa{};
+        sa.src = scratch_; sa.nrows = tc * NQ; sa.row_div = NQ; sa.s_rhi = nk * NQ * nao2; sa.s_rlo = nao2;
+        sa.nchunk = (int)nk; sa.s_c = NQ * nao2; sa.klen = (int)nao2; sa.s_k = 1; sa.kc = (int)kc_p0_;
+        sa.conj = 0; sa.nplanes = 3; sa.dst = Zs_; sa.exps = eZs_;
+        slice_rows(sa, ns, stream_);
+        OzGemmArgs o{};
+        o.M = (int)NQ; o.N = (int)NQ; o.batch = (int)tc; o.kc = (int)kc_p0_; o.nchunk = (int)nk; o.nin = (int)NQ;
+        o.A = Zs_; o.sAm = nk * 3 * ns * kc_p0_; o.sAb = NQ * o.sAm; o.sAc = 3 * ns * kc_p0_;
+        o.eA = eZs_; o.seAb = NQ * nk; o.seAm = nk;
+        o.B = Vs_p0_ + q * NQ * nk * 2 * ns * kc_p0_; o.sBb = 0; o.sBno = 0; o.sBni = nk * 2 * ns * kc_p0_;
+        o.sBc = 2 * ns * kc_p0_;
+        o.eB = eVs_p0_ + q * NQ * nk; o.seBb = 0; o.seBno = 0; o.seBni = nk;
+        o.C = P_ + (q * d_.nts + t0) * NQ2; o.sCb = NQ2; o.sCm = NQ; o.sCno = 0; o.sCni = 1;
+        o.alpha = make_double2(-d_.prefactor / (double)nk, 0.0);
+        o.accumulate = 0;
+        timed(K_P0_GEMM, [&] { zgemm_ozaki(o, ns, stream_); });
+        continue;
+      }
       ZGemmArgs a{};
       a.M = (int)NQ; a.N = (int)NQ; a.K = (int)(nk * nao2); a.batch = (int)tc;
       a.kin = (int)nao2; a.nin = (int)NQ;
@@ -330,21 +459,37 @@ void gw_hip_kernel::stage_dyson() {
   (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); (void)hipEventDestroy(e2); (void)hipEventDestroy(e3);
 }
 
-void gw_hip_kernel::stage_sigma(const double2* G, double2* Sigma) {
+void gw_hip_kernel::stage_sigma(const double2* G, double2* Sigma, bool split) {
   const int64_t nk = d_.nk, NQ = d_.NQ, nao2 = d_.nao * d_.nao, NQ2 = NQ * NQ, nts = d_.nts;
   DevDims dd{(int)nts, (int)d_.ns, (int)nk, (int)d_.nso, (int)d_.nao, (int)NQ, (int)d_.pns, (int)d_.pseudo_ns};
-  const int64_t lds_unit = nao2 * (int64_t)sizeof(double2);
-  int QC = (int)((max_lds_ / lds_unit - 1) / 3);
-  QC = std::max(1, std::min<int>(QC, (int)NQ));
-  const size_t lds = (size_t)(1 + 3 * QC) * lds_unit;
-  if ((int64_t)lds > max_lds_) throw status_error(GWHIP_ERR_UNSUPPORTED, "nao too large for the Sigma epilogue kernel");
-  allow_lds(sigma_epilogue_kernel, lds);
+  const int NB = (int)ceil_div(d_.nao, 16);
+  const size_t lds = ((size_t)2 * (16 * NB) * (16 * NB + 1) + (size_t)NB * NB * 8 * 64) * sizeof(double);
   const double scale = -1.0 / (double)nk;
 
   for (int64_t q = 0; q < nk; ++q) {
-    const double2* Vneg = Vq_ + (int64_t)neg_q_[q] * nk * NQ * nao2;  // V(k, k - q)
+    const double2* Vneg = Vt_ + (int64_t)neg_q_[q] * nk * NQ * nao2;  // V(k, k - q), per-Q transposed
     for (int64_t t0 = 0; t0 < nts; t0 += tchunk_sigma_) {
       const int64_t tc = std::min(tchunk_sigma_, nts - t0);
+      if (split) {
+        const int ns = split_ns_;
+        const int qn = neg_q_[q];
+        SliceArgs sa{};
+        sa.src = P_ + (q * nts + t0) * NQ2; sa.nrows = tc * NQ; sa.row_div = tc * NQ; sa.s_rhi = 0; sa.s_rlo = NQ;
+        sa.nchunk = 1; sa.s_c = 0; sa.klen = (int)NQ; sa.s_k = 1; sa.kc = (int)kc_s_;
+        sa.conj = 0; sa.nplanes = 3; sa.dst = Ps_; sa.exps = ePs_;
+        slice_rows(sa, ns, stream_);
+        OzGemmArgs o{};
+        o.M = (int)NQ; o.N = (int)(nk * nao2); o.batch = (int)tc; o.kc = (int)kc_s_; o.nchunk = 1; o.nin = (int)nao2;
+        o.A = Ps_; o.sAm = 3 * ns * kc_s_; o.sAb = NQ * o.sAm; o.sAc = 0;
+        o.eA = ePs_; o.seAb = NQ; o.seAm = 1;
+        o.B = Vs_s_ + (int64_t)qn * nk * nao2 * 2 * ns * kc_s_; o.sBb = 0; o.sBno = nao2 * 2 * ns * kc_s_;
+        o.sBni = 2 * ns * kc_s_; o.sBc = 0;
+        o.eB = eVs_s_ + (int64_t)qn * nk * nao2; o.seBb = 0; o.seBno = nao2; o.seBni = 1;
+        o.C = scratch_; o.sCb = nk * NQ * nao2; o.sCm = nao2; o.sCno = NQ * nao2; o.sCni = 1;
+        o.alpha = make_double2(1.0, 0.0);
+        o.accumulate = 0;
+        timed(K_SIGMA_GEMM, [&] { zgemm_ozaki(o, ns, stream_); });
+      } else {
       ZGemmArgs a{};
       a.M = (int)NQ; a.N = (int)(nk * nao2); a.K = (int)NQ; a.batch = (int)tc; a.kin = (int)NQ; a.nin = (int)nao2;
       a.A = P_ + (q * nts + t0) * NQ2; a.sAb = NQ2; a.sAm = NQ; a.sAko = 0; a.sAki = 1;
@@ -353,10 +498,13 @@ void gw_hip_kernel::stage_sigma(const double2* G, double2* Sigma) {
       a.alpha = make_double2(1.0, 0.0);
       a.accumulate = 0; a.conjA = 0; a.conjB = 1;
       timed(K_SIGMA_GEMM, [&] { zgemm_f64(a, true, false, stream_); });
+      }
       timed(K_SIGMA_EPI, [&] {
-        hipLaunchKernelGGL(sigma_epilogue_kernel, dim3((unsigned)nk, (unsigned)tc), dim3(256), lds, stream_, dd, G, Vneg,
-                           scratch_, Sigma, kminus_tab_ + q * nk, (int)t0, QC, scale);
-        GWHIP_CHECK(hipGetLastError());
+        const dim3 grid((unsigned)nk, (unsigned)tc);
+        const int* km = kminus_tab_ + q * nk;
+        if (NB == 1) launch_sigma_epi<1>(grid, lds, stream_, dd, G, Vneg, scratch_, Sigma, km, (int)t0, scale);
+        else if (NB == 2) launch_sigma_epi<2>(grid, lds, stream_, dd, G, Vneg, scratch_, Sigma, km, (int)t0, scale);
+        else launch_sigma_epi<3>(grid, lds, stream_, dd, G, Vneg, scratch_, Sigma, km, (int)t0, scale);
       });
     }
   }
@@ -366,8 +514,10 @@ void gw_hip_kernel::solve_device(const double2* G, double2* Sigma, int precision
   if (!grid_set_) throw status_error(GWHIP_ERR_STATE, "transform grid not set (gwhip_set_grid)");
   if (!v_set_) throw status_error(GWHIP_ERR_STATE, "integrals not set (gwhip_upload_V / gwhip_generate_V)");
   if (!G || !Sigma) throw status_error(GWHIP_ERR_ARG, "null G or Sigma");
-  if (precision != GWHIP_PREC_FP64) throw status_error(GWHIP_ERR_UNSUPPORTED, "only FP64 precision is implemented");
+  if (precision != GWHIP_PREC_FP64 && precision != GWHIP_PREC_SPLIT) throw status_error(GWHIP_ERR_ARG, "unknown precision");
+  const bool split = precision == GWHIP_PREC_SPLIT;
   GWHIP_CHECK(hipSetDevice(device_));
+  if (split) ensure_slices();
   for (double& r : region_ms_) r = 0.0;
   for (auto& k : kstat_) k = KernelStat{};
   const size_t nsig = (size_t)d_.nts * d_.ns * d_.nk * d_.nso * d_.nso;
@@ -377,11 +527,11 @@ void gw_hip_kernel::solve_device(const double2* G, double2* Sigma, int precision
   GWHIP_CHECK(hipEventCreate(&t2)); GWHIP_CHECK(hipEventCreate(&t3));
   GWHIP_CHECK(hipEventRecord(t0, stream_));
   GWHIP_CHECK(hipMemsetAsync(Sigma, 0, nsig * sizeof(double2), stream_));
-  stage_p0(G);
+  stage_p0(G, split);
   GWHIP_CHECK(hipEventRecord(t1, stream_));
   stage_dyson();
   GWHIP_CHECK(hipEventRecord(t2, stream_));
-  stage_sigma(G, Sigma);
+  stage_sigma(G, Sigma, split);
   GWHIP_CHECK(hipEventRecord(t3, stream_));
   GWHIP_CHECK(hipMemsetAsync(flag_, 0, sizeof(int), stream_));
   hipLaunchKernelGGL(nonfinite_kernel, dim3(1024), dim3(256), 0, stream_, Sigma, (int64_t)nsig, flag_);
@@ -416,12 +566,13 @@ void gw_hip_kernel::run_stages_host(const double2* G, int last_stage) {
   if (!G_dev_) G_dev_ = dalloc<double2>(nsig);
   GWHIP_CHECK(hipMemcpy(G_dev_, G, nsig * sizeof(double2), hipMemcpyHostToDevice));
   GWHIP_CHECK(hipMemsetAsync(P_, 0, (size_t)d_.nk * d_.nts * d_.NQ * d_.NQ * sizeof(double2), stream_));
-  stage_p0(G_dev_);
+  stage_p0(G_dev_, false);
   if (last_stage >= 2) stage_dyson();
   GWHIP_CHECK(hipStreamSynchronize(stream_));
 }
 
-void debug_zgemm(int M, int N, int K, const double2* A, const double2* B, double2* C, int conjA, int conjB) {
+void debug_zgemm(int M, int N, int K, const double2* A, const double2* B, double2* C, int conjA, int conjB,
+                 int shape) {
   if (M <= 0 || N <= 0 || K <= 0 || !A || !B || !C) throw status_error(GWHIP_ERR_ARG, "bad debug_zgemm arguments");
   GWHIP_CHECK(hipSetDevice(0));
   double2 *dA = nullptr, *dB = nullptr, *dC = nullptr;
@@ -437,12 +588,222 @@ void debug_zgemm(int M, int N, int K, const double2* A, const double2* B, double
   a.C = dC; a.sCm = N; a.sCni = 1;
   a.alpha = make_double2(1.0, 0.0);
   a.conjA = conjA; a.conjB = conjB;
-  zgemm_f64(a, true, false, nullptr);
+  zgemm_f64(a, true, false, nullptr, shape);
   GWHIP_CHECK(hipDeviceSynchronize());
   GWHIP_CHECK(hipMemcpy(C, dC, (size_t)M * N * sizeof(double2), hipMemcpyDeviceToHost));
   (void)hipFree(dA);
   (void)hipFree(dB);
   (void)hipFree(dC);
+}
+
+__global__ void fill_hash_kernel(double2* X, int64_t n, uint64_t salt) {
+  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (gid >= n) return;
+  X[gid] = make_double2(u01(salt, 2 * gid) - 0.5, u01(salt, 2 * gid + 1) - 0.5);
+}
+
+template <int CH>
+__global__ __launch_bounds__(256) void mfma_peak_kernel(double* out, int iters) {
+  v4d acc[CH];
+#pragma unroll
+  for (int j = 0; j < CH; ++j) acc[j] = v4d{0.0, 0.0, 0.0, 0.0};
+  const double a = 1e-3 * threadIdx.x, b = 1.0 + 1e-6 * blockIdx.x;
+  for (int i = 0; i < iters; ++i) {
+#pragma unroll
+    for (int j = 0; j < CH; ++j) acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[j], 0, 0, 0);
+  }
+  double s = 0.0;
+#pragma unroll
+  for (int j = 0; j < CH; ++j) s += acc[j][0] + acc[j][1] + acc[j][2] + acc[j][3];
+  out[(int64_t)blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
+// FP64 vector FMA probe (mode 1) and MFMA + vector FMA in the same wave (mode 2).
+template <int MODE>
+__global__ __launch_bounds__(256) void valu_peak_kernel(double* out, int iters) {
+  double x[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) x[j] = 1e-3 * (threadIdx.x + j);
+  v4d acc[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) acc[j] = v4d{0.0, 0.0, 0.0, 0.0};
+  const double a = 1.0 - 1e-9 * blockIdx.x, b = 1e-7;
+  for (int i = 0; i < iters; ++i) {
+    if (MODE == 2) {
+#pragma unroll
+      for (int j = 0; j < 4; ++j) acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[j], 0, 0, 0);
+    }
+#pragma unroll
+    for (int r = 0; r < 4; ++r)
+#pragma unroll
+      for (int j = 0; j < 8; ++j) x[j] = fma(x[j], a, b);
+  }
+  double s = 0.0;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) s += x[j];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) s += acc[j][0];
+  out[(int64_t)blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
+typedef int v4i_t __attribute__((ext_vector_type(4)));
+typedef float v4f_t __attribute__((ext_vector_type(4)));
+typedef __bf16 v8bf_t __attribute__((ext_vector_type(8)));
+
+// Low-precision MFMA probes: MODE 0 = v_mfma_i32_16x16x64_i8, MODE 1 = v_mfma_f32_16x16x32_bf16;
+// eight independent accumulator chains per wave.
+template <int MODE>
+__global__ __launch_bounds__(256) void lowp_peak_kernel(double* out, int iters) {
+  v4i_t ai = {(int)threadIdx.x, 3, 5, 7}, bi = {11, (int)blockIdx.x, 13, 17};
+  v8bf_t ab, bb;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) { ab[j] = (__bf16)(0.001f * (threadIdx.x + j)); bb[j] = (__bf16)(0.5f); }
+  v4i_t ci[8];
+  v4f_t cf[8];
+#pragma unroll
+  for (int j = 0; j < 8; ++j) { ci[j] = v4i_t{0, 0, 0, 0}; cf[j] = v4f_t{0.f, 0.f, 0.f, 0.f}; }
+  for (int i = 0; i < iters; ++i) {
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+      if (MODE == 0) ci[j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ai, bi, ci[j], 0, 0, 0);
+      else cf[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ab, bb, cf[j], 0, 0, 0);
+    }
+  }
+  double s = 0.0;
+#pragma unroll
+  for (int j = 0; j < 8; ++j) s += (double)ci[j][0] + (double)cf[j][0];
+  out[(int64_t)blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
+double debug_mfma_peak(int chains, int wps, int iters) {
+  if (chains < 1 || (chains > 8 && chains != 10 && chains != 11) || chains == 7 || wps < 1 || wps > 8 || iters < 1) throw status_error(GWHIP_ERR_ARG, "bad mfma probe args");
+  GWHIP_CHECK(hipSetDevice(0));
+  hipDeviceProp_t prop;
+  GWHIP_CHECK(hipGetDeviceProperties(&prop, 0));
+  const int nblocks = prop.multiProcessorCount * wps;  // 4 waves per block -> one per SIMD
+  double* out = nullptr;
+  GWHIP_CHECK(hipMalloc(&out, (size_t)nblocks * 256 * sizeof(double)));
+  auto launch = [&] {
+    switch (chains) {
+      case 10: hipLaunchKernelGGL(lowp_peak_kernel<0>, dim3(nblocks), dim3(256), 0, nullptr, out, iters); break;
+      case 11: hipLaunchKernelGGL(lowp_peak_kernel<1>, dim3(nblocks), dim3(256), 0, nullptr, out, iters); break;
+      case 5: hipLaunchKernelGGL(valu_peak_kernel<1>, dim3(nblocks), dim3(256), 0, nullptr, out, iters); break;
+      case 6: hipLaunchKernelGGL(valu_peak_kernel<2>, dim3(nblocks), dim3(256), 0, nullptr, out, iters); break;
+      case 1: hipLaunchKernelGGL(mfma_peak_kernel<1>, dim3(nblocks), dim3(256), 0, nullptr, out, iters); break;
+      case 2: hipLaunchKernelGGL(mfma_peak_kernel<2>, dim3(nblocks), dim3(256), 0, nullptr, out, iters); break;
+      case 4: hipLaunchKernelGGL(mfma_peak_kernel<4>, dim3(nblocks), dim3(256), 0, nullptr, out, iters); break;
+      default: hipLaunchKernelGGL(mfma_peak_kernel<8>, dim3(nblocks), dim3(256), 0, nullptr, out, iters); break;
+    }
+  };
+  launch();
+  GWHIP_CHECK(hipDeviceSynchronize());
+  hipEvent_t e0, e1;
+  GWHIP_CHECK(hipEventCreate(&e0));
+  GWHIP_CHECK(hipEventCreate(&e1));
+  GWHIP_CHECK(hipEventRecord(e0, nullptr));
+  launch();
+  GWHIP_CHECK(hipEventRecord(e1, nullptr));
+  GWHIP_CHECK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  GWHIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(out);
+  double per_wave_iter;
+  if (chains == 10) per_wave_iter = 8 * 16.0 * 16 * 64 * 2;
+  else if (chains == 11) per_wave_iter = 8 * 16.0 * 16 * 32 * 2;
+  else if (chains == 5) per_wave_iter = 32.0 * 64 * 2;                   // 32 FMA per lane
+  else if (chains == 6) per_wave_iter = 32.0 * 64 * 2 + 4 * 2048.0;  // + 4 MFMA
+  else per_wave_iter = (chains == 3 ? 4 : (chains > 4 ? 8 : chains)) * 2048.0;
+  const double flops = (double)nblocks * 4 * iters * per_wave_iter;
+  return flops / (ms * 1e-3) / 1e12;
+}
+
+double debug_gemm_bench(int which, int NQ, int nk, int nao, int batch, int shape, int reps) {
+  if (NQ <= 0 || nk <= 0 || nao <= 0 || batch <= 0 || reps <= 0 || shape < 0 || shape > 2)
+    throw status_error(GWHIP_ERR_ARG, "bad debug_gemm_bench arguments");
+  GWHIP_CHECK(hipSetDevice(0));
+  const int64_t nao2 = (int64_t)nao * nao, NQ2 = (int64_t)NQ * NQ, vsz = (int64_t)nk * NQ * nao2;
+  ZGemmArgs a{};
+  int64_t na, nb, nc;
+  bool akc = true, bkc = false;
+  if (which == 0) {  // Sigma: R[tau] = P[tau] conj(V)
+    na = batch * NQ2; nb = vsz; nc = batch * vsz;
+    a.M = NQ; a.N = (int)(nk * nao2); a.K = NQ; a.batch = batch; a.kin = NQ; a.nin = (int)nao2;
+    a.sAb = NQ2; a.sAm = NQ; a.sAko = 0; a.sAki = 1;
+    a.sBb = 0; a.sBko = 0; a.sBki = nao2; a.sBno = NQ * nao2; a.sBni = 1;
+    a.sCb = vsz; a.sCm = nao2; a.sCno = NQ * nao2; a.sCni = 1;
+  } else {           // P0: P[tau] = Z[tau] conj(V)^T
+    na = batch * vsz; nb = vsz; nc = batch * NQ2;
+    a.M = NQ; a.N = NQ; a.K = (int)(nk * nao2); a.batch = batch; a.kin = (int)nao2; a.nin = NQ;
+    a.sAb = vsz; a.sAm = nao2; a.sAko = NQ * nao2; a.sAki = 1;
+    a.sBb = 0; a.sBko = NQ * nao2; a.sBki = 1; a.sBno = 0; a.sBni = nao2;
+    a.sCb = NQ2; a.sCm = NQ; a.sCno = 0; a.sCni = 1;
+    bkc = true;
+  }
+  double2 *dA = nullptr, *dB = nullptr, *dC = nullptr;
+  GWHIP_CHECK(hipMalloc(&dA, na * sizeof(double2)));
+  GWHIP_CHECK(hipMalloc(&dB, nb * sizeof(double2)));
+  GWHIP_CHECK(hipMalloc(&dC, nc * sizeof(double2)));
+  hipLaunchKernelGGL(fill_hash_kernel, dim3((unsigned)ceil_div(na, 256)), dim3(256), 0, nullptr, dA, na, 11ull);
+  hipLaunchKernelGGL(fill_hash_kernel, dim3((unsigned)ceil_div(nb, 256)), dim3(256), 0, nullptr, dB, nb, 12ull);
+  GWHIP_CHECK(hipGetLastError());
+  a.A = dA; a.B = dB; a.C = dC;
+  a.alpha = make_double2(1.0, 0.0);
+  a.conjB = 1;
+  zgemm_f64(a, akc, bkc, nullptr, shape);
+  GWHIP_CHECK(hipDeviceSynchronize());
+  hipEvent_t e0, e1;
+  GWHIP_CHECK(hipEventCreate(&e0));
+  GWHIP_CHECK(hipEventCreate(&e1));
+  GWHIP_CHECK(hipEventRecord(e0, nullptr));
+  for (int r = 0; r < reps; ++r) zgemm_f64(a, akc, bkc, nullptr, shape);
+  GWHIP_CHECK(hipEventRecord(e1, nullptr));
+  GWHIP_CHECK(hipEventSynchronize(e1));
+  float ms = 0.f;
+  GWHIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(dA);
+  (void)hipFree(dB);
+  (void)hipFree(dC);
+  return ms / reps;
+}
+
+void debug_zgemm_split(int M, int N, int K, const double2* A, const double2* B, double2* C, int ns) {
+  if (M <= 0 || N <= 0 || K <= 0 || !A || !B || !C) throw status_error(GWHIP_ERR_ARG, "bad debug_zgemm_split arguments");
+  GWHIP_CHECK(hipSetDevice(0));
+  const int kc = (int)ceil_div(K, 64) * 64;
+  double2 *dA = nullptr, *dB = nullptr, *dC = nullptr;
+  int8_t *sA = nullptr, *sB = nullptr;
+  int *eA = nullptr, *eB = nullptr;
+  GWHIP_CHECK(hipMalloc(&dA, (size_t)M * K * sizeof(double2)));
+  GWHIP_CHECK(hipMalloc(&dB, (size_t)K * N * sizeof(double2)));
+  GWHIP_CHECK(hipMalloc(&dC, (size_t)M * N * sizeof(double2)));
+  GWHIP_CHECK(hipMalloc(&sA, (size_t)M * 3 * ns * kc));
+  GWHIP_CHECK(hipMalloc(&sB, (size_t)N * 2 * ns * kc));
+  GWHIP_CHECK(hipMalloc(&eA, (size_t)M * sizeof(int)));
+  GWHIP_CHECK(hipMalloc(&eB, (size_t)N * sizeof(int)));
+  GWHIP_CHECK(hipMemcpy(dA, A, (size_t)M * K * sizeof(double2), hipMemcpyHostToDevice));
+  GWHIP_CHECK(hipMemcpy(dB, B, (size_t)K * N * sizeof(double2), hipMemcpyHostToDevice));
+  SliceArgs a{};
+  a.src = dA; a.nrows = M; a.row_div = M; a.s_rhi = 0; a.s_rlo = K; a.nchunk = 1; a.s_c = 0; a.klen = K; a.s_k = 1;
+  a.kc = kc; a.conj = 0; a.nplanes = 3; a.dst = sA; a.exps = eA;
+  slice_rows(a, ns, nullptr);
+  SliceArgs bsl{};
+  bsl.src = dB; bsl.nrows = N; bsl.row_div = N; bsl.s_rhi = 0; bsl.s_rlo = 1; bsl.nchunk = 1; bsl.s_c = 0; bsl.klen = K;
+  bsl.s_k = N; bsl.kc = kc; bsl.conj = 0; bsl.nplanes = 2; bsl.dst = sB; bsl.exps = eB;
+  slice_rows(bsl, ns, nullptr);
+  OzGemmArgs o{};
+  o.M = M; o.N = N; o.batch = 1; o.kc = kc; o.nchunk = 1; o.nin = N;
+  o.A = sA; o.sAm = 3 * ns * kc; o.eA = eA; o.seAm = 1;
+  o.B = sB; o.sBni = 2 * ns * kc; o.eB = eB; o.seBni = 1;
+  o.C = dC; o.sCm = N; o.sCni = 1;
+  o.alpha = make_double2(1.0, 0.0);
+  zgemm_ozaki(o, ns, nullptr);
+  GWHIP_CHECK(hipDeviceSynchronize());
+  GWHIP_CHECK(hipMemcpy(C, dC, (size_t)M * N * sizeof(double2), hipMemcpyDeviceToHost));
+  for (void* p : {(void*)dA, (void*)dB, (void*)dC, (void*)sA, (void*)sB, (void*)eA, (void*)eB}) (void)hipFree(p);
 }
 
 void gw_hip_kernel::copy_P_tilde(double2* host) const {
@@ -511,6 +872,16 @@ int gwhip_set_grid(gwhip_handle* h, const double* T_bt, const double* T_tb) {
   return guarded(h, [&] { h->k->set_grid((const double2*)T_bt, (const double2*)T_tb); });
 }
 
+int gwhip_set_kq_maps(gwhip_handle* h, const int32_t* kplus, const int32_t* kminus) {
+  if (!h) return GWHIP_ERR_ARG;
+  return guarded(h, [&] { h->k->set_kq_maps(kplus, kminus); });
+}
+
+int gwhip_set_split_slices(gwhip_handle* h, int slices) {
+  if (!h) return GWHIP_ERR_ARG;
+  return guarded(h, [&] { h->k->set_split_slices(slices); });
+}
+
 int gwhip_upload_V(gwhip_handle* h, const double* V) {
   if (!h) return GWHIP_ERR_ARG;
   return guarded(h, [&] { h->k->upload_V((const double2*)V); });
@@ -552,10 +923,30 @@ int gwhip_kernel_stats(const gwhip_handle* h, int which, int64_t* launches, doub
 
 size_t gwhip_device_bytes(const gwhip_handle* h) { return h ? h->k->device_bytes() : 0; }
 
-int gwhip_debug_zgemm(int M, int N, int K, const double* A, const double* B, double* C, int conjA, int conjB) {
+int gwhip_debug_zgemm(int M, int N, int K, const double* A, const double* B, double* C, int conjA, int conjB,
+                      int shape) {
   gwhip_handle tmp;
   return guarded(&tmp, [&] {
-    gwhip::debug_zgemm(M, N, K, (const double2*)A, (const double2*)B, (double2*)C, conjA, conjB);
+    gwhip::debug_zgemm(M, N, K, (const double2*)A, (const double2*)B, (double2*)C, conjA, conjB, shape);
+  });
+}
+
+int gwhip_debug_gemm_bench(int which, int NQ, int nk, int nao, int batch, int shape, int reps, double* ms) {
+  gwhip_handle tmp;
+  if (!ms) return GWHIP_ERR_ARG;
+  return guarded(&tmp, [&] { *ms = gwhip::debug_gemm_bench(which, NQ, nk, nao, batch, shape, reps); });
+}
+
+int gwhip_debug_mfma_peak(int chains, int waves_per_simd, int iters, double* tflops) {
+  gwhip_handle tmp;
+  if (!tflops) return GWHIP_ERR_ARG;
+  return guarded(&tmp, [&] { *tflops = gwhip::debug_mfma_peak(chains, waves_per_simd, iters); });
+}
+
+int gwhip_debug_zgemm_split(int M, int N, int K, const double* A, const double* B, double* C, int slices) {
+  gwhip_handle tmp;
+  return guarded(&tmp, [&] {
+    gwhip::debug_zgemm_split(M, N, K, (const double2*)A, (const double2*)B, (double2*)C, slices);
   });
 }
 

@@ -44,6 +44,8 @@ public:
   gw_hip_kernel& operator=(const gw_hip_kernel&) = delete;
 
   void set_grid(const double2* T_bt, const double2* T_tb);
+  void set_kq_maps(const int32_t* kplus, const int32_t* kminus);
+  void set_split_slices(int ns);
   void upload_V(const double2* V_packed);
   void generate_V(uint64_t seed, double sigma);
   // G_dev, Sigma_dev: device pointers [nts][ns][nk][nso][nso]
@@ -66,7 +68,8 @@ private:
   bool grid_set_ = false, v_set_ = false;
 
   // device buffers
-  double2* Vq_ = nullptr;      // [nq][nk][NQ][nao^2]
+  double2* Vq_ = nullptr;      // [nq][nk][NQ][nao^2]   V^Q_ab(k, k+q)
+  double2* Vt_ = nullptr;      // [nq][nk][NQ][nao^2]   V^Q_ba(k, k+q)
   double2* P_ = nullptr;       // [nq][nts][NQ^2]
   double2* Pw_ = nullptr;      // [qc][nw_b][NQ^2]
   double2* X_ = nullptr;       // [qc][nw_b][NQ^2]
@@ -79,6 +82,18 @@ private:
   int* kplus_tab_ = nullptr;   // [nk][nq] : k + q
   int* kminus_tab_ = nullptr;  // [nq][nk] : k - q  (row q)
   int* flag_ = nullptr;
+  // split precision: int8 slice stores of conj(V) for the P0 and Sigma contractions, and the
+  // per-evaluation slices of Z and P-tilde (zgemm_ozaki.hpp)
+  int split_ns_ = 4, slices_ns_ = 0;
+  int64_t kc_p0_ = 0, kc_s_ = 0;
+  int8_t* Vs_p0_ = nullptr;   // [nq][NQ][nk][2][ns][kc_p0]
+  int* eVs_p0_ = nullptr;     // [nq][NQ][nk]
+  int8_t* Vs_s_ = nullptr;    // [nq][nk][nao^2][2][ns][kc_s]
+  int* eVs_s_ = nullptr;      // [nq][nk][nao^2]
+  int8_t* Zs_ = nullptr;      // [tc][NQ][nk][3][ns][kc_p0]
+  int* eZs_ = nullptr;
+  int8_t* Ps_ = nullptr;      // [tc][NQ][3][ns][kc_s]
+  int* ePs_ = nullptr;
   int64_t qchunk_ = 1, tchunk_p0_ = 1, tchunk_sigma_ = 1;
   std::vector<int> kplus_host_, neg_q_;
 
@@ -88,9 +103,13 @@ private:
   template <typename T>
   T* dalloc(size_t count);
   void build_maps();
-  void stage_p0(const double2* G);
+  void upload_maps(const std::vector<int>& kpq, const std::vector<int>& kmq);
+  void make_transposed_V();
+  void ensure_slices();
+  void free_slices();
+  void stage_p0(const double2* G, bool split);
   void stage_dyson();
-  void stage_sigma(const double2* G, double2* Sigma);
+  void stage_sigma(const double2* G, double2* Sigma, bool split);
   template <typename F>
   void timed(int kstat, F&& launch);
 };

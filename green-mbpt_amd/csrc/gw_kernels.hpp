@@ -2,8 +2,9 @@
 //   G, Sigma : [nts][ns][nk][nso][nso]
 //   Vq       : [nq][nk][NQ][nao][nao]   Vq[q][k] = V^Q(k, k+q)   (every ordered k pair resident)
 //   P        : [nq][nts][NQ][NQ]         P0 on the first nts/2 tau, then P-tilde on all nts
+//   Vt       : as Vq, each Q block transposed (Vt[q][k][Q][b][a] = Vq[q][k][Q][a][b])
 //   Z        : [tc][nk][NQ][nao][nao]    Z = sum_s Gbar_s V G_s   (P0 left operand)
-//   R        : [tc][nk][NQ][nao][nao]    R = P-tilde conj(V)       (Sigma right operand)
+//   R        : [tc][nk][NQ][nao][nao]    R = P-tilde conj(V), each Q block transposed
 // Spin blocks follow the reference (src/gw_cpu_kernel.cpp:166-186, 328-351): block s of
 // pseudo_ns has is = s/(pns^2), a = (s/pns)%pns, b = s%pns.
 #pragma once
@@ -18,167 +19,6 @@ struct DevDims {
 
 __device__ __forceinline__ int64_t g_index(const DevDims& d, int t, int is, int k) {
   return (((int64_t)t * d.ns + is) * d.nk + k) * d.nso * d.nso;
-}
-
-// ---------------------------------------------------------------------------------------
-// P0 left operand: Z[tz][k][Q][t][n] = sum_s sum_{p,m} Gb_s[t][p] V[Q][p][m] Gq_s[m][n],
-// Gb_s = G(beta - tau, k) block (b, a), Gq_s = G(tau, k + q) block (a, b)
-// (src/gw_cpu_kernel.cpp:170-187 with X1 = Gb V, X2 = conj(V) Gq^T regrouped so that the
-// big contraction runs against V itself).  grid: (ceil(NQ/QC), nk, tcount); LDS holds
-// Gb, Gq and QC slices of V and of W = V Gq.
-constexpr int Z_MAXO = 16;
-
-__global__ __launch_bounds__(256) void z_prep_kernel(DevDims d, const double2* __restrict__ G,
-                                                     const double2* __restrict__ Vq,  // this q
-                                                     double2* __restrict__ Z, const int* __restrict__ kplus,
-                                                     int t0, int QC) {
-  extern __shared__ double2 lds[];
-  const int nao = d.nao, nao2 = nao * nao;
-  double2* Gb = lds;
-  double2* Gq = Gb + nao2;
-  double2* Vc = Gq + nao2;
-  double2* W = Vc + QC * nao2;
-  const int tz = blockIdx.z, tau = t0 + tz, tt = d.nts - 1 - tau;
-  const int k = blockIdx.y, kq = kplus[k];
-  const int Q0 = blockIdx.x * QC;
-  const int nq = min(QC, d.NQ - Q0);
-  const int nout = nq * nao2;
-  const int tid = threadIdx.x;
-
-  for (int e = tid; e < nout; e += blockDim.x) Vc[e] = Vq[((int64_t)k * d.NQ + Q0) * nao2 + e];
-  double2 acc[Z_MAXO];
-#pragma unroll
-  for (int i = 0; i < Z_MAXO; ++i) acc[i] = make_double2(0.0, 0.0);
-
-  for (int s = 0; s < d.pseudo_ns; ++s) {
-    const int is = s / (d.pns * d.pns), a = (s / d.pns) % d.pns, b = s % d.pns;
-    const double2* Gbar = G + g_index(d, tt, is, k);
-    const double2* Gk = G + g_index(d, tau, is, kq);
-    __syncthreads();
-    for (int e = tid; e < nao2; e += blockDim.x) {
-      const int r = e / nao, c = e - (e / nao) * nao;
-      Gb[e] = Gbar[(int64_t)(b * nao + r) * d.nso + a * nao + c];
-      Gq[e] = Gk[(int64_t)(a * nao + r) * d.nso + b * nao + c];
-    }
-    __syncthreads();
-    // W[Qc][p][n] = sum_m V[Qc][p][m] Gq[m][n]
-    for (int e = tid; e < nout; e += blockDim.x) {
-      const int qc = e / nao2, rem = e - qc * nao2;
-      const int p = rem / nao, n = rem - p * nao;
-      const double2* vr = Vc + qc * nao2 + p * nao;
-      double2 w = make_double2(0.0, 0.0);
-      for (int m = 0; m < nao; ++m) w = cfma(vr[m], Gq[m * nao + n], w);
-      W[e] = w;
-    }
-    __syncthreads();
-    // acc[Qc][t][n] += sum_p Gb[t][p] W[Qc][p][n]
-#pragma unroll
-    for (int i = 0; i < Z_MAXO; ++i) {
-      const int e = tid + i * blockDim.x;
-      if (e < nout) {
-        const int qc = e / nao2, rem = e - qc * nao2;
-        const int t = rem / nao, n = rem - t * nao;
-        const double2* gr = Gb + t * nao;
-        const double2* wc = W + qc * nao2 + n;
-        double2 z = acc[i];
-        for (int p = 0; p < nao; ++p) z = cfma(gr[p], wc[p * nao], z);
-        acc[i] = z;
-      }
-    }
-  }
-  double2* Zout = Z + (((int64_t)tz * d.nk + k) * d.NQ + Q0) * nao2;
-#pragma unroll
-  for (int i = 0; i < Z_MAXO; ++i) {
-    const int e = tid + i * blockDim.x;
-    if (e < nout) Zout[e] = acc[i];
-  }
-}
-
-// ---------------------------------------------------------------------------------------
-// Sigma epilogue: Sigma(tau, is, k)[a,b block](i, j) += scale * sum_Q sum_n T_s[Q][i][n] R[Q][j][n],
-// T_s[Q] = V(k, k-q)[Q] G(tau, k-q)[a,b block]  (src/gw_cpu_kernel.cpp:359-373 with
-// Y2 = Y1^T P computed by the big GEMM on the other side: R = P conj(V)).
-// grid: (nk, tcount); LDS: Gs, QC slices of V, R and T.
-constexpr int SE_MAXO = 8;
-
-__global__ __launch_bounds__(256) void sigma_epilogue_kernel(DevDims d, const double2* __restrict__ G,
-                                                             const double2* __restrict__ Vq,  // V(k, k-q)
-                                                             const double2* __restrict__ R,
-                                                             double2* __restrict__ Sigma,
-                                                             const int* __restrict__ kminus, int t0, int QC,
-                                                             double scale) {
-  extern __shared__ double2 lds[];
-  const int nao = d.nao, nao2 = nao * nao;
-  double2* Gs = lds;
-  double2* Vc = Gs + nao2;
-  double2* Rc = Vc + QC * nao2;
-  double2* T = Rc + QC * nao2;
-  const int k = blockIdx.x, tz = blockIdx.y, tau = t0 + tz;
-  const int kmq = kminus[k];
-  const int tid = threadIdx.x;
-  const double2* Vk = Vq + (int64_t)k * d.NQ * nao2;
-  const double2* Rk = R + ((int64_t)tz * d.nk + k) * d.NQ * nao2;
-
-  for (int s = 0; s < d.pseudo_ns; ++s) {
-    const int is = s / (d.pns * d.pns), a = (s / d.pns) % d.pns, b = s % d.pns;
-    const double2* Gk = G + g_index(d, tau, is, kmq);
-    __syncthreads();
-    for (int e = tid; e < nao2; e += blockDim.x) {
-      const int r = e / nao, c = e - (e / nao) * nao;
-      Gs[e] = Gk[(int64_t)(a * nao + r) * d.nso + b * nao + c];
-    }
-    double2 acc[SE_MAXO];
-#pragma unroll
-    for (int i = 0; i < SE_MAXO; ++i) acc[i] = make_double2(0.0, 0.0);
-
-    for (int Q0 = 0; Q0 < d.NQ; Q0 += QC) {
-      const int nq = min(QC, d.NQ - Q0);
-      __syncthreads();
-      for (int e = tid; e < nq * nao2; e += blockDim.x) {
-        Vc[e] = Vk[(int64_t)Q0 * nao2 + e];
-        Rc[e] = Rk[(int64_t)Q0 * nao2 + e];
-      }
-      __syncthreads();
-      // T[Qc][i][n] = sum_m V[Qc][i][m] Gs[m][n]
-      for (int e = tid; e < nq * nao2; e += blockDim.x) {
-        const int qc = e / nao2, rem = e - qc * nao2;
-        const int i = rem / nao, n = rem - i * nao;
-        const double2* vr = Vc + qc * nao2 + i * nao;
-        double2 w = make_double2(0.0, 0.0);
-        for (int m = 0; m < nao; ++m) w = cfma(vr[m], Gs[m * nao + n], w);
-        T[e] = w;
-      }
-      __syncthreads();
-      // acc(i, j) += sum_{Qc, n} T[Qc][i][n] R[Qc][j][n]
-#pragma unroll
-      for (int o = 0; o < SE_MAXO; ++o) {
-        const int e = tid + o * blockDim.x;
-        if (e < nao2) {
-          const int i = e / nao, j = e - (e / nao) * nao;
-          double2 z = acc[o];
-          for (int qc = 0; qc < nq; ++qc) {
-            const double2* tr = T + qc * nao2 + i * nao;
-            const double2* rr = Rc + qc * nao2 + j * nao;
-            for (int n = 0; n < nao; ++n) z = cfma(tr[n], rr[n], z);
-          }
-          acc[o] = z;
-        }
-      }
-    }
-    double2* S = Sigma + g_index(d, tau, is, k);
-#pragma unroll
-    for (int o = 0; o < SE_MAXO; ++o) {
-      const int e = tid + o * blockDim.x;
-      if (e < nao2) {
-        const int i = e / nao, j = e - (e / nao) * nao;
-        double2* dst = S + (int64_t)(a * nao + i) * d.nso + b * nao + j;
-        double2 v = *dst;
-        v.x += scale * acc[o].x;
-        v.y += scale * acc[o].y;
-        *dst = v;
-      }
-    }
-  }
 }
 
 // ---------------------------------------------------------------------------------------

@@ -11,10 +11,12 @@
 // which lets one kernel run the P0 contraction, the Sigma contraction, the IR transforms
 // and the Dyson products straight out of their resident layouts.
 //
-// Tiling: 256 threads = 4 waves of 64; block tile 64x64 complex, K tile 16; each wave owns a
-// 32x32 complex sub-tile = 2x2 MFMA blocks x (re, im) accumulators.  Operands are staged
-// through LDS as separate real / imaginary planes (k-major), global loads for the next K tile
-// are issued before the MFMAs of the current one.  A complex product is four real MFMAs.
+// Tiling: 256 threads = 4 waves in a 2x2 grid; block tile BM x BN complex, K tile 16; each
+// wave owns a (BM/2) x (BN/2) sub-tile of 16x16 MFMA blocks with separate real/imaginary
+// accumulators.  Operands are staged through LDS as real/imaginary planes (k-major); the
+// global loads of the next K tile are in flight during the MFMAs of the current one.  Each
+// thread walks its operand addresses incrementally along K, so the hot loop carries no
+// integer division.  A complex product is four real MFMAs.
 #pragma once
 
 #include "common.hpp"
@@ -34,18 +36,39 @@ struct ZGemmArgs {
   int accumulate;  // 1: C += alpha*AB, 0: C = alpha*AB
   int conjA, conjB;
   int tiles_m, tiles_n;
-  int64_t nblocks;      // batch * tiles_m * tiles_n
-  int64_t per_xcd;      // ceil(nblocks / 8)
+  int64_t nblocks;   // batch * tiles_m * tiles_n
+  int64_t per_xcd;   // ceil(nblocks / 8)
 };
 
-constexpr int ZG_BM = 64, ZG_BN = 64, ZG_BK = 16, ZG_PAD = 2;
+constexpr int ZG_BK = 16, ZG_PAD = 2;
 
-// A_KC: A is contiguous along k (thread loads run along k); otherwise along m.
-// B_KC: B is contiguous along k; otherwise along n.
-template <bool A_KC, bool B_KC>
+// Running position (ko, ki) of one operand element along K with its address offset.
+struct KWalk {
+  int ki;
+  int64_t off;
+  __device__ __forceinline__ void init(int k, int kin, int64_t sko, int64_t ski) {
+    const int ko = k / kin;
+    ki = k - ko * kin;
+    off = (int64_t)ko * sko + (int64_t)ki * ski;
+  }
+  __device__ __forceinline__ void advance(int kin, int64_t sko, int64_t ski) {
+    ki += ZG_BK;
+    off += (int64_t)ZG_BK * ski;
+    while (ki >= kin) {
+      ki -= kin;
+      off += sko - (int64_t)kin * ski;
+    }
+  }
+};
+
+template <int BM, int BN, bool A_KC, bool B_KC>
 __global__ __launch_bounds__(256) void zgemm_f64_kernel(ZGemmArgs p) {
-  __shared__ double As_r[ZG_BK][ZG_BM + ZG_PAD], As_i[ZG_BK][ZG_BM + ZG_PAD];
-  __shared__ double Bs_r[ZG_BK][ZG_BN + ZG_PAD], Bs_i[ZG_BK][ZG_BN + ZG_PAD];
+  constexpr int BK = ZG_BK;
+  constexpr int WM = BM / 2, WN = BN / 2;        // per-wave tile
+  constexpr int MB = WM / 16, NB = WN / 16;      // 16x16 blocks per wave
+  constexpr int EA = BM * BK / 256, EB = BN * BK / 256;  // elements per thread
+  __shared__ double As_r[BK][BM + ZG_PAD], As_i[BK][BM + ZG_PAD];
+  __shared__ double Bs_r[BK][BN + ZG_PAD], Bs_i[BK][BN + ZG_PAD];
 
   // XCD-aware tile order: consecutive hardware workgroup ids go to different XCDs, so give
   // each XCD a contiguous range of logical tiles (tiles of one batch entry share operands).
@@ -56,7 +79,7 @@ __global__ __launch_bounds__(256) void zgemm_f64_kernel(ZGemmArgs p) {
   const int64_t b = logical / tiles;
   const int rem = (int)(logical - b * tiles);
   const int tm = rem / p.tiles_n, tn = rem - (rem / p.tiles_n) * p.tiles_n;
-  const int m0 = tm * ZG_BM, n0 = tn * ZG_BN;
+  const int m0 = tm * BM, n0 = tn * BN;
 
   const double2* __restrict__ A = p.A + b * p.sAb;
   const double2* __restrict__ B = p.B + b * p.sBb;
@@ -66,105 +89,139 @@ __global__ __launch_bounds__(256) void zgemm_f64_kernel(ZGemmArgs p) {
   const int lane = tid & 63, wave = tid >> 6;
   const double sa = p.conjA ? -1.0 : 1.0, sb = p.conjB ? -1.0 : 1.0;
 
-  // load coordinates (4 elements per thread per operand)
-  int a_m[4], a_k[4], b_k[4], b_n[4];
+  // thread -> tile coordinates of its EA (EB) elements
+  int a_m[EA], a_k[EA], b_k[EB], b_n[EB];
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
-    if (A_KC) { a_m[j] = tid >> 2; a_k[j] = (tid & 3) * 4 + j; }
-    else      { a_k[j] = tid >> 4; a_m[j] = (tid & 15) * 4 + j; }
-    if (B_KC) { b_n[j] = tid >> 2; b_k[j] = (tid & 3) * 4 + j; }
-    else      { b_k[j] = tid >> 4; b_n[j] = (tid & 15) * 4 + j; }
+  for (int j = 0; j < EA; ++j) {
+    if (A_KC) {  // runs of 4 consecutive k
+      const int id = tid + (j / 4) * 256;
+      a_m[j] = id / (BK / 4);
+      a_k[j] = (id % (BK / 4)) * 4 + (j % 4);
+    } else {     // runs of 4 consecutive m
+      const int id = tid + (j / 4) * 256;
+      a_k[j] = id / (BM / 4);
+      a_m[j] = (id % (BM / 4)) * 4 + (j % 4);
+    }
   }
-  // per-thread column offsets of B (n is fixed per thread across K tiles)
-  int64_t b_noff[4];
-  bool b_nok[4];
 #pragma unroll
-  for (int j = 0; j < 4; ++j) {
+  for (int j = 0; j < EB; ++j) {
+    if (B_KC) {
+      const int id = tid + (j / 4) * 256;
+      b_n[j] = id / (BK / 4);
+      b_k[j] = (id % (BK / 4)) * 4 + (j % 4);
+    } else {
+      const int id = tid + (j / 4) * 256;
+      b_k[j] = id / (BN / 4);
+      b_n[j] = (id % (BN / 4)) * 4 + (j % 4);
+    }
+  }
+  // fixed row / column parts of the addresses
+  int64_t a_moff[EA], b_noff[EB];
+  bool a_ok[EA], b_ok[EB];
+  KWalk ka[EA], kb[EB];
+#pragma unroll
+  for (int j = 0; j < EA; ++j) {
+    const int m = m0 + a_m[j];
+    a_ok[j] = m < p.M;
+    a_moff[j] = (int64_t)(a_ok[j] ? m : 0) * p.sAm;
+    ka[j].init(a_k[j], p.kin, p.sAko, p.sAki);
+  }
+#pragma unroll
+  for (int j = 0; j < EB; ++j) {
     const int n = n0 + b_n[j];
-    b_nok[j] = n < p.N;
-    const int no = n / p.nin;
-    b_noff[j] = (int64_t)no * p.sBno + (int64_t)(n - no * p.nin) * p.sBni;
+    b_ok[j] = n < p.N;
+    const int nn = b_ok[j] ? n : 0;
+    const int no = nn / p.nin;
+    b_noff[j] = (int64_t)no * p.sBno + (int64_t)(nn - no * p.nin) * p.sBni;
+    kb[j].init(b_k[j], p.kin, p.sBko, p.sBki);
   }
 
-  double2 ra[4], rb[4];
+  double2 ra[EA], rb[EB];
   auto load_tiles = [&](int k0) {
+    const bool full = k0 + BK <= p.K;
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int m = m0 + a_m[j], k = k0 + a_k[j];
-      double2 v = make_double2(0.0, 0.0);
-      if (m < p.M && k < p.K) {
-        const int ko = k / p.kin;
-        v = A[(int64_t)m * p.sAm + (int64_t)ko * p.sAko + (int64_t)(k - ko * p.kin) * p.sAki];
-      }
-      ra[j] = v;
+    for (int j = 0; j < EA; ++j) {
+      const bool ok = a_ok[j] && (full || k0 + a_k[j] < p.K);
+      ra[j] = ok ? A[a_moff[j] + ka[j].off] : make_double2(0.0, 0.0);
+      ka[j].advance(p.kin, p.sAko, p.sAki);
     }
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int k = k0 + b_k[j];
-      double2 v = make_double2(0.0, 0.0);
-      if (b_nok[j] && k < p.K) {
-        const int ko = k / p.kin;
-        v = B[(int64_t)ko * p.sBko + (int64_t)(k - ko * p.kin) * p.sBki + b_noff[j]];
-      }
-      rb[j] = v;
+    for (int j = 0; j < EB; ++j) {
+      const bool ok = b_ok[j] && (full || k0 + b_k[j] < p.K);
+      rb[j] = ok ? B[kb[j].off + b_noff[j]] : make_double2(0.0, 0.0);
+      kb[j].advance(p.kin, p.sBko, p.sBki);
     }
   };
   auto store_tiles = [&]() {
 #pragma unroll
-    for (int j = 0; j < 4; ++j) {
+    for (int j = 0; j < EA; ++j) {
       As_r[a_k[j]][a_m[j]] = ra[j].x;
       As_i[a_k[j]][a_m[j]] = sa * ra[j].y;
+    }
+#pragma unroll
+    for (int j = 0; j < EB; ++j) {
       Bs_r[b_k[j]][b_n[j]] = rb[j].x;
       Bs_i[b_k[j]][b_n[j]] = sb * rb[j].y;
     }
   };
 
-  v4d acc_r[2][2], acc_i[2][2];
+  v4d acc_r[MB][NB], acc_i[MB][NB];
 #pragma unroll
-  for (int x = 0; x < 2; ++x)
+  for (int x = 0; x < MB; ++x)
 #pragma unroll
-    for (int y = 0; y < 2; ++y) {
+    for (int y = 0; y < NB; ++y) {
       acc_r[x][y] = v4d{0.0, 0.0, 0.0, 0.0};
       acc_i[x][y] = v4d{0.0, 0.0, 0.0, 0.0};
     }
-  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const int wm = (wave >> 1) * WM, wn = (wave & 1) * WN;
   const int lrow = lane & 15, lk = lane >> 4;
 
   load_tiles(0);
-  for (int k0 = 0; k0 < p.K; k0 += ZG_BK) {
+  for (int k0 = 0; k0 < p.K; k0 += BK) {
     __syncthreads();
     store_tiles();
     __syncthreads();
-    if (k0 + ZG_BK < p.K) load_tiles(k0 + ZG_BK);
+    if (k0 + BK < p.K) load_tiles(k0 + BK);
 #pragma unroll
-    for (int ks = 0; ks < ZG_BK; ks += 4) {
+    for (int ks = 0; ks < BK; ks += 4) {
       const int kr = ks + lk;
-      double ar[2], ai[2], br[2], bi[2];
+      double ar[MB], ai[MB], br[NB], bi[NB];
 #pragma unroll
-      for (int x = 0; x < 2; ++x) {
+      for (int x = 0; x < MB; ++x) {
         ar[x] = As_r[kr][wm + x * 16 + lrow];
         ai[x] = As_i[kr][wm + x * 16 + lrow];
-        br[x] = Bs_r[kr][wn + x * 16 + lrow];
-        bi[x] = Bs_i[kr][wn + x * 16 + lrow];
       }
 #pragma unroll
-      for (int x = 0; x < 2; ++x)
+      for (int y = 0; y < NB; ++y) {
+        br[y] = Bs_r[kr][wn + y * 16 + lrow];
+        bi[y] = Bs_i[kr][wn + y * 16 + lrow];
+      }
+      // independent products first, the dependent second halves after
 #pragma unroll
-        for (int y = 0; y < 2; ++y) {
+      for (int x = 0; x < MB; ++x)
+#pragma unroll
+        for (int y = 0; y < NB; ++y) {
           acc_r[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[x], br[y], acc_r[x][y], 0, 0, 0);
-          acc_r[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai[x], bi[y], acc_r[x][y], 0, 0, 0);
           acc_i[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ar[x], bi[y], acc_i[x][y], 0, 0, 0);
+        }
+#pragma unroll
+      for (int x = 0; x < MB; ++x) {
+        const double nai = -ai[x];
+#pragma unroll
+        for (int y = 0; y < NB; ++y) {
+          acc_r[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(nai, bi[y], acc_r[x][y], 0, 0, 0);
           acc_i[x][y] = __builtin_amdgcn_mfma_f64_16x16x4f64(ai[x], br[y], acc_i[x][y], 0, 0, 0);
         }
+      }
     }
   }
 
   // epilogue: register r of lane l holds row (l/16) + 4*r, column l%16 of each 16x16 block
   // (v_mfma_f64_16x16x4f64 result layout, checked by the identity-probe test)
 #pragma unroll
-  for (int x = 0; x < 2; ++x)
+  for (int x = 0; x < MB; ++x)
 #pragma unroll
-    for (int y = 0; y < 2; ++y) {
+    for (int y = 0; y < NB; ++y) {
       const int n = n0 + wn + y * 16 + lrow;
       if (n >= p.N) continue;
       const int no = n / p.nin;
@@ -186,25 +243,38 @@ __global__ __launch_bounds__(256) void zgemm_f64_kernel(ZGemmArgs p) {
     }
 }
 
+// Tile shapes: 0 = 64x64, 1 = 128x64, 2 = 64x128.
+template <int BM, int BN>
+inline void zgemm_launch_shape(const ZGemmArgs& p, bool a_kc, bool b_kc, dim3 g, hipStream_t stream) {
+  dim3 blk(256);
+  if (a_kc && b_kc)
+    hipLaunchKernelGGL((zgemm_f64_kernel<BM, BN, true, true>), g, blk, 0, stream, p);
+  else if (a_kc && !b_kc)
+    hipLaunchKernelGGL((zgemm_f64_kernel<BM, BN, true, false>), g, blk, 0, stream, p);
+  else if (!a_kc && b_kc)
+    hipLaunchKernelGGL((zgemm_f64_kernel<BM, BN, false, true>), g, blk, 0, stream, p);
+  else
+    hipLaunchKernelGGL((zgemm_f64_kernel<BM, BN, false, false>), g, blk, 0, stream, p);
+}
+
 // Host-side launcher.  Fills tile counts and the XCD remap, checks sizes.
-inline void zgemm_f64(ZGemmArgs p, bool a_kc, bool b_kc, hipStream_t stream) {
+inline void zgemm_f64(ZGemmArgs p, bool a_kc, bool b_kc, hipStream_t stream, int shape = 0) {
   if (p.M <= 0 || p.N <= 0 || p.batch <= 0) return;
   if (p.kin <= 0 || p.nin <= 0) throw std::runtime_error("zgemm_f64: kin/nin must be positive");
-  p.tiles_m = (int)ceil_div(p.M, ZG_BM);
-  p.tiles_n = (int)ceil_div(p.N, ZG_BN);
+  const int BM = shape == 1 ? 128 : 64, BN = shape == 2 ? 128 : 64;
+  p.tiles_m = (int)ceil_div(p.M, BM);
+  p.tiles_n = (int)ceil_div(p.N, BN);
   p.nblocks = (int64_t)p.batch * p.tiles_m * p.tiles_n;
   p.per_xcd = ceil_div(p.nblocks, 8);
   const int64_t grid = p.per_xcd * 8;
   if (grid > 0x7fffffffLL) throw std::runtime_error("zgemm_f64: grid too large");
-  dim3 g((unsigned)grid), blk(256);
-  if (a_kc && b_kc)
-    hipLaunchKernelGGL((zgemm_f64_kernel<true, true>), g, blk, 0, stream, p);
-  else if (a_kc && !b_kc)
-    hipLaunchKernelGGL((zgemm_f64_kernel<true, false>), g, blk, 0, stream, p);
-  else if (!a_kc && b_kc)
-    hipLaunchKernelGGL((zgemm_f64_kernel<false, true>), g, blk, 0, stream, p);
+  const dim3 g((unsigned)grid);
+  if (shape == 1)
+    zgemm_launch_shape<128, 64>(p, a_kc, b_kc, g, stream);
+  else if (shape == 2)
+    zgemm_launch_shape<64, 128>(p, a_kc, b_kc, g, stream);
   else
-    hipLaunchKernelGGL((zgemm_f64_kernel<false, false>), g, blk, 0, stream, p);
+    zgemm_launch_shape<64, 64>(p, a_kc, b_kc, g, stream);
   GWHIP_CHECK(hipGetLastError());
 }
 

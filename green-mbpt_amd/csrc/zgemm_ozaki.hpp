@@ -1,0 +1,348 @@
+// Split-precision complex GEMM: FP64 operands cut into int8 slices, products on the int8
+// matrix cores (v_mfma_i32_16x16x64_i8) with exact int32 accumulation, FP64 accumulation
+// across K chunks.  This is the fixed-point slicing of Ozaki et al. (Numer. Algorithms 59,
+// 2012; Ootomo, Ozaki & Yokota, IJHPCA 2024 for integer matrix units):
+//
+//   x = 2^e * sum_{s<NS} x_s 2^{-7(s+1)} + r,   x_s in [-127, 127],  |r| < 2^{e - 7 NS}
+//
+// with one power-of-two exponent e per (row, K chunk) of A and per (column, K chunk) of B,
+// so every slice product is an integer and sum_k x_s y_t is exact in int32 while
+// 2^14 * (chunk length) * (number of slice pairs) < 2^31.  Slice pairs with s + t = d share
+// the weight 2^{-7(d+2)}; pairs with s + t >= NS are dropped.
+//
+// Slice store (int8), for each row (A) or column (B):
+//   [chunk][plane][slice][kc]   planes of A: Re, Im, -Im;  planes of B: Re, Im
+// (B is stored already conjugated when the contraction needs conj(B)), so
+//   Re C = Re A Re B + (-Im A) Im B,   Im C = Re A Im B + Im A Re B.
+#pragma once
+
+#include "common.hpp"
+
+namespace gwhip {
+
+typedef int v4i __attribute__((ext_vector_type(4)));
+
+// ------------------------------------------------------------------------------ slicing
+// Row r = r_hi * row_div + r_lo of a strided FP64 complex source; element kk of chunk c at
+//   src + r_hi*s_rhi + r_lo*s_rlo + c*s_c + kk*s_k,   kk < klen (zero up to kc).
+struct SliceArgs {
+  const double2* src;
+  int64_t nrows;
+  int64_t row_div, s_rhi, s_rlo;
+  int nchunk;
+  int64_t s_c;
+  int klen;
+  int64_t s_k;
+  int kc;        // padded chunk length (multiple of 64)
+  int conj;      // slice conj(x)
+  int nplanes;   // 2: Re, Im   3: Re, Im, -Im
+  int8_t* dst;   // [row][chunk][plane][slice][kc]
+  int* exps;     // [row][chunk]
+};
+
+__device__ __forceinline__ double wave_max(double v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v = fmax(v, __shfl_xor(v, o, 64));
+  return v;
+}
+
+// One wave per (row, chunk); each lane slices 4 consecutive elements per step.
+template <int NS>
+__global__ __launch_bounds__(256) void slice_rows_kernel(SliceArgs a) {
+  const int lane = threadIdx.x & 63;
+  const int64_t task = (int64_t)blockIdx.x * 4 + (threadIdx.x >> 6);
+  if (task >= a.nrows * a.nchunk) return;
+  const int64_t row = task / a.nchunk;
+  const int c = (int)(task - row * a.nchunk);
+  const int64_t rhi = row / a.row_div, rlo = row - rhi * a.row_div;
+  const double2* s = a.src + rhi * a.s_rhi + rlo * a.s_rlo + (int64_t)c * a.s_c;
+  double m = 0.0;
+  for (int kk = lane; kk < a.klen; kk += 64) {
+    const double2 v = s[(int64_t)kk * a.s_k];
+    m = fmax(m, fmax(fabs(v.x), fabs(v.y)));
+  }
+  m = wave_max(m);
+  int e = 0;
+  if (m > 0.0) (void)frexp(m, &e);  // m < 2^e
+  if (lane == 0) a.exps[task] = e;
+  const double sc = ldexp(1.0, -e);
+  const double sgn = a.conj ? -1.0 : 1.0;
+  int8_t* out = a.dst + task * (int64_t)a.nplanes * NS * a.kc;
+  for (int base = 0; base < a.kc; base += 256) {
+    const int k0 = base + 4 * lane;
+    if (k0 >= a.kc) continue;
+    int wr[NS], wi[NS];
+#pragma unroll
+    for (int t = 0; t < NS; ++t) { wr[t] = 0; wi[t] = 0; }
+#pragma unroll
+    for (int j = 0; j < 4; ++j) {
+      const int kk = k0 + j;
+      double2 v = make_double2(0.0, 0.0);
+      if (kk < a.klen) v = s[(int64_t)kk * a.s_k];
+      double tr = v.x * sc, ti = sgn * v.y * sc;  // |t| < 1
+#pragma unroll
+      for (int t = 0; t < NS; ++t) {
+        tr *= 128.0;
+        ti *= 128.0;
+        const double qr = trunc(tr), qi = trunc(ti);
+        tr -= qr;
+        ti -= qi;
+        wr[t] |= ((int)qr & 0xff) << (8 * j);
+        wi[t] |= ((int)qi & 0xff) << (8 * j);
+      }
+    }
+#pragma unroll
+    for (int t = 0; t < NS; ++t) {
+      *reinterpret_cast<int*>(out + (int64_t)(0 * NS + t) * a.kc + k0) = wr[t];
+      *reinterpret_cast<int*>(out + (int64_t)(1 * NS + t) * a.kc + k0) = wi[t];
+      if (a.nplanes == 3) {
+        int nw = 0;
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+          const int8_t bq = (int8_t)((wi[t] >> (8 * j)) & 0xff);
+          nw |= ((int)(int8_t)(-bq) & 0xff) << (8 * j);
+        }
+        *reinterpret_cast<int*>(out + (int64_t)(2 * NS + t) * a.kc + k0) = nw;
+      }
+    }
+  }
+}
+
+// ------------------------------------------------------------------------------ GEMM
+struct OzGemmArgs {
+  int M, N, batch;
+  int kc, nchunk;        // K = nchunk * kc (int8 elements per plane-slice)
+  int nin;               // N split for B / C columns
+  // A(row m, chunk c): slices at A + b*sAb + m*sAm + c*sAc, exponent eA[b*seAb + m*seAm + c]
+  const int8_t* A;
+  int64_t sAb, sAm, sAc;
+  const int* eA;
+  int64_t seAb, seAm;
+  // B(col n, chunk c): slices at B + b*sBb + (n/nin)*sBno + (n%nin)*sBni + c*sBc,
+  //                    exponent eB[b*seBb + (n/nin)*seBno + (n%nin)*seBni + c]
+  const int8_t* B;
+  int64_t sBb, sBno, sBni, sBc;
+  const int* eB;
+  int64_t seBb, seBno, seBni;
+  double2* C;
+  int64_t sCb, sCm, sCno, sCni;
+  double2 alpha;
+  int accumulate;
+  int tiles_m, tiles_n;
+  int64_t nblocks, per_xcd;
+};
+
+// Output layout of v_mfma_i32_16x16x64_i8: register r of lane l holds row
+// OZ_ROW(l/16, r), column l%16 (checked by the identity-probe test).
+#define OZ_ROW(g, r) (4 * (g) + (r))
+
+constexpr int OZ_BM = 64, OZ_BN = 64, OZ_KS = 64, OZ_LDROW = 80;  // bytes per LDS row (64 + pad)
+
+template <int NS>
+__global__ __launch_bounds__(256) void zgemm_ozaki_kernel(OzGemmArgs p) {
+  constexpr int PA = 3, PB = 2;
+  extern __shared__ __attribute__((aligned(16))) int8_t oz_lds[];
+  int8_t* As = oz_lds;                                  // [PA*NS][64 rows][OZ_LDROW]
+  int8_t* Bs = As + PA * NS * OZ_BM * OZ_LDROW;         // [PB*NS][64 cols][OZ_LDROW]
+  int* eAs = reinterpret_cast<int*>(Bs + PB * NS * OZ_BN * OZ_LDROW);  // [64]
+  int* eBs = eAs + OZ_BM;                                               // [64]
+
+  const int64_t hw = blockIdx.x;
+  const int64_t logical = (hw % 8) * p.per_xcd + hw / 8;
+  if (logical >= p.nblocks) return;
+  const int tiles = p.tiles_m * p.tiles_n;
+  const int64_t b = logical / tiles;
+  const int rem = (int)(logical - b * tiles);
+  const int tm = rem / p.tiles_n, tn = rem - (rem / p.tiles_n) * p.tiles_n;
+  const int m0 = tm * OZ_BM, n0 = tn * OZ_BN;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lrow = lane & 15, lg = lane >> 4;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+
+  // staging: each thread moves 16-byte pieces; a row of one plane-slice K step is 64 bytes
+  // = 4 pieces, so one pass of 256 threads covers 64 rows x 1 plane-slice.
+  const int srow = tid >> 2, spiece = tid & 3;
+  const int8_t* Arow = nullptr;
+  const int8_t* Brow = nullptr;
+  {
+    const int m = m0 + srow;
+    if (m < p.M) Arow = p.A + b * p.sAb + (int64_t)m * p.sAm;
+    const int n = n0 + srow;
+    if (n < p.N) {
+      const int no = n / p.nin;
+      Brow = p.B + b * p.sBb + (int64_t)no * p.sBno + (int64_t)(n - no * p.nin) * p.sBni;
+    }
+  }
+
+  double cr[2][2][4], ci[2][2][4];
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) { cr[x][y][r] = 0.0; ci[x][y][r] = 0.0; }
+
+  for (int c = 0; c < p.nchunk; ++c) {
+    v4i ar[NS][2][2], ai[NS][2][2];  // [d][x][y] int32 accumulators (Re, Im)
+#pragma unroll
+    for (int d = 0; d < NS; ++d)
+#pragma unroll
+      for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) { ar[d][x][y] = v4i{0, 0, 0, 0}; ai[d][x][y] = v4i{0, 0, 0, 0}; }
+
+    for (int ks = 0; ks < p.kc; ks += OZ_KS) {
+      __syncthreads();
+#pragma unroll
+      for (int ps = 0; ps < PA * NS; ++ps) {
+        v4i v = v4i{0, 0, 0, 0};
+        if (Arow) v = *reinterpret_cast<const v4i*>(Arow + (int64_t)c * p.sAc + (int64_t)ps * p.kc + ks + 16 * spiece);
+        *reinterpret_cast<v4i*>(As + (ps * OZ_BM + srow) * OZ_LDROW + 16 * spiece) = v;
+      }
+#pragma unroll
+      for (int ps = 0; ps < PB * NS; ++ps) {
+        v4i v = v4i{0, 0, 0, 0};
+        if (Brow) v = *reinterpret_cast<const v4i*>(Brow + (int64_t)c * p.sBc + (int64_t)ps * p.kc + ks + 16 * spiece);
+        *reinterpret_cast<v4i*>(Bs + (ps * OZ_BN + srow) * OZ_LDROW + 16 * spiece) = v;
+      }
+      __syncthreads();
+#pragma unroll
+      for (int s = 0; s < NS; ++s) {
+        v4i fa_r[2], fa_i[2], fa_n[2];
+#pragma unroll
+        for (int x = 0; x < 2; ++x) {
+          const int row = wm + 16 * x + lrow;
+          fa_r[x] = *reinterpret_cast<const v4i*>(As + ((0 * NS + s) * OZ_BM + row) * OZ_LDROW + 16 * lg);
+          fa_i[x] = *reinterpret_cast<const v4i*>(As + ((1 * NS + s) * OZ_BM + row) * OZ_LDROW + 16 * lg);
+          fa_n[x] = *reinterpret_cast<const v4i*>(As + ((2 * NS + s) * OZ_BM + row) * OZ_LDROW + 16 * lg);
+        }
+#pragma unroll
+        for (int t = 0; t + s < NS; ++t) {
+          v4i fb_r[2], fb_i[2];
+#pragma unroll
+          for (int y = 0; y < 2; ++y) {
+            const int col = wn + 16 * y + lrow;
+            fb_r[y] = *reinterpret_cast<const v4i*>(Bs + ((0 * NS + t) * OZ_BN + col) * OZ_LDROW + 16 * lg);
+            fb_i[y] = *reinterpret_cast<const v4i*>(Bs + ((1 * NS + t) * OZ_BN + col) * OZ_LDROW + 16 * lg);
+          }
+#pragma unroll
+          for (int x = 0; x < 2; ++x)
+#pragma unroll
+            for (int y = 0; y < 2; ++y) {
+              ar[s + t][x][y] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa_r[x], fb_r[y], ar[s + t][x][y], 0, 0, 0);
+              ai[s + t][x][y] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa_r[x], fb_i[y], ai[s + t][x][y], 0, 0, 0);
+              ar[s + t][x][y] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa_n[x], fb_i[y], ar[s + t][x][y], 0, 0, 0);
+              ai[s + t][x][y] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa_i[x], fb_r[y], ai[s + t][x][y], 0, 0, 0);
+            }
+        }
+      }
+    }
+    // chunk end: int32 -> FP64 with the (row, column, chunk) power-of-two scale
+    __syncthreads();
+    if (tid < OZ_BM) {
+      const int m = m0 + tid;
+      eAs[tid] = (m < p.M) ? p.eA[b * p.seAb + (int64_t)m * p.seAm + c] : 0;
+    } else if (tid < OZ_BM + OZ_BN) {
+      const int n = n0 + tid - OZ_BM;
+      int e = 0;
+      if (n < p.N) {
+        const int no = n / p.nin;
+        e = p.eB[b * p.seBb + (int64_t)no * p.seBno + (int64_t)(n - no * p.nin) * p.seBni + c];
+      }
+      eBs[tid - OZ_BM] = e;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+      for (int y = 0; y < 2; ++y)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = wm + 16 * x + OZ_ROW(lg, r), col = wn + 16 * y + lrow;
+          double sc = ldexp(1.0, eAs[row] + eBs[col] - 14);
+          double sr = 0.0, si = 0.0;
+#pragma unroll
+          for (int d = 0; d < NS; ++d) {
+            sr = fma((double)ar[d][x][y][r], sc, sr);
+            si = fma((double)ai[d][x][y][r], sc, si);
+            sc *= 0.0078125;  // 2^-7
+          }
+          cr[x][y][r] += sr;
+          ci[x][y][r] += si;
+        }
+  }
+
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y) {
+      const int n = n0 + wn + 16 * y + lrow;
+      if (n >= p.N) continue;
+      const int no = n / p.nin;
+      const int64_t coff = (int64_t)no * p.sCno + (int64_t)(n - no * p.nin) * p.sCni;
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = m0 + wm + 16 * x + OZ_ROW(lg, r);
+        if (m >= p.M) continue;
+        const double vr = cr[x][y][r], vi = ci[x][y][r];
+        double2 v = make_double2(p.alpha.x * vr - p.alpha.y * vi, p.alpha.x * vi + p.alpha.y * vr);
+        double2* dst = p.C + b * p.sCb + (int64_t)m * p.sCm + coff;
+        if (p.accumulate) {
+          const double2 o = *dst;
+          v.x += o.x;
+          v.y += o.y;
+        }
+        *dst = v;
+      }
+    }
+}
+
+inline size_t ozaki_lds_bytes(int ns) {
+  return (size_t)(3 + 2) * ns * OZ_BM * OZ_LDROW + (OZ_BM + OZ_BN) * sizeof(int);
+}
+
+template <typename K>
+inline void oz_allow_lds(K kernel, size_t bytes) {
+  if (bytes > 65536) GWHIP_CHECK(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
+}
+
+inline void zgemm_ozaki(OzGemmArgs p, int ns, hipStream_t stream) {
+  if (p.M <= 0 || p.N <= 0 || p.batch <= 0) return;
+  if (p.kc % OZ_KS != 0 || p.nchunk <= 0 || p.nin <= 0) throw std::runtime_error("zgemm_ozaki: bad chunking");
+  // exact int32 accumulation: ns pairs at most share one accumulator, 4 products each ... bounded
+  if ((double)p.kc * 2.0 * 127.0 * 127.0 * ns > 2147483647.0) throw std::runtime_error("zgemm_ozaki: chunk too long");
+  p.tiles_m = (int)ceil_div(p.M, OZ_BM);
+  p.tiles_n = (int)ceil_div(p.N, OZ_BN);
+  p.nblocks = (int64_t)p.batch * p.tiles_m * p.tiles_n;
+  p.per_xcd = ceil_div(p.nblocks, 8);
+  const int64_t grid = p.per_xcd * 8;
+  if (grid > 0x7fffffffLL) throw std::runtime_error("zgemm_ozaki: grid too large");
+  const size_t lds = ozaki_lds_bytes(ns);
+  if (ns == 4) {
+    oz_allow_lds(zgemm_ozaki_kernel<4>, lds);
+    hipLaunchKernelGGL(zgemm_ozaki_kernel<4>, dim3((unsigned)grid), dim3(256), lds, stream, p);
+  } else if (ns == 5) {
+    oz_allow_lds(zgemm_ozaki_kernel<5>, lds);
+    hipLaunchKernelGGL(zgemm_ozaki_kernel<5>, dim3((unsigned)grid), dim3(256), lds, stream, p);
+  } else if (ns == 3) {
+    oz_allow_lds(zgemm_ozaki_kernel<3>, lds);
+    hipLaunchKernelGGL(zgemm_ozaki_kernel<3>, dim3((unsigned)grid), dim3(256), lds, stream, p);
+  } else {
+    throw std::runtime_error("zgemm_ozaki: slices must be 3, 4 or 5");
+  }
+  GWHIP_CHECK(hipGetLastError());
+}
+
+inline void slice_rows(const SliceArgs& a, int ns, hipStream_t stream) {
+  const int64_t tasks = a.nrows * a.nchunk;
+  if (tasks <= 0) return;
+  if (a.kc % 64 != 0 || a.klen > a.kc) throw std::runtime_error("slice_rows: bad chunk length");
+  const dim3 g((unsigned)ceil_div(tasks, 4));
+  if (ns == 4) hipLaunchKernelGGL(slice_rows_kernel<4>, g, dim3(256), 0, stream, a);
+  else if (ns == 5) hipLaunchKernelGGL(slice_rows_kernel<5>, g, dim3(256), 0, stream, a);
+  else if (ns == 3) hipLaunchKernelGGL(slice_rows_kernel<3>, g, dim3(256), 0, stream, a);
+  else throw std::runtime_error("slice_rows: slices must be 3, 4 or 5");
+  GWHIP_CHECK(hipGetLastError());
+}
+
+}  // namespace gwhip

@@ -60,6 +60,8 @@ def load() -> ctypes.CDLL:
     lib.gwhip_last_error.restype = ctypes.c_char_p
     lib.gwhip_set_grid.argtypes = [P, P, P]
     lib.gwhip_upload_V.argtypes = [P, P]
+    lib.gwhip_set_kq_maps.argtypes = [P, P, P]
+    lib.gwhip_set_split_slices.argtypes = [P, ctypes.c_int]
     lib.gwhip_generate_V.argtypes = [P, ctypes.c_uint64, ctypes.c_double]
     lib.gwhip_eval_sigma.argtypes = [P, P, P, ctypes.c_int]
     lib.gwhip_eval_sigma_device.argtypes = [P, P, P, ctypes.c_int]
@@ -69,9 +71,13 @@ def load() -> ctypes.CDLL:
                                        ctypes.POINTER(ctypes.c_double)]
     lib.gwhip_device_bytes.argtypes = [P]
     lib.gwhip_device_bytes.restype = ctypes.c_size_t
-    lib.gwhip_debug_zgemm.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, P, ctypes.c_int, ctypes.c_int]
+    lib.gwhip_debug_zgemm.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, P, ctypes.c_int, ctypes.c_int,
+                                      ctypes.c_int]
     lib.gwhip_debug_stage.argtypes = [P, P, ctypes.c_int]
-    for f in ("gwhip_debug_zgemm", "gwhip_debug_stage", "gwhip_set_grid", "gwhip_upload_V", "gwhip_generate_V", "gwhip_eval_sigma",
+    lib.gwhip_debug_zgemm_split.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, P, ctypes.c_int]
+    lib.gwhip_debug_gemm_bench.argtypes = [ctypes.c_int] * 7 + [ctypes.POINTER(ctypes.c_double)]
+    lib.gwhip_debug_mfma_peak.argtypes = [ctypes.c_int] * 3 + [ctypes.POINTER(ctypes.c_double)]
+    for f in ("gwhip_debug_zgemm_split", "gwhip_set_split_slices", "gwhip_set_kq_maps", "gwhip_debug_mfma_peak", "gwhip_debug_gemm_bench", "gwhip_debug_zgemm", "gwhip_debug_stage", "gwhip_set_grid", "gwhip_upload_V", "gwhip_generate_V", "gwhip_eval_sigma",
               "gwhip_eval_sigma_device", "gwhip_get_P_tilde", "gwhip_timings", "gwhip_kernel_stats"):
         getattr(lib, f).restype = ctypes.c_int
     _lib = lib
@@ -84,7 +90,7 @@ def header_functions() -> list[str]:
     return sorted(set(re.findall(r"^\s*(?:const\s+)?[\w\s\*]+?\b(gwhip_\w+)\s*\(", text, re.M)))
 
 
-def debug_zgemm(A: np.ndarray, B: np.ndarray, conjA: bool = False, conjB: bool = False) -> np.ndarray:
+def debug_zgemm(A: np.ndarray, B: np.ndarray, conjA: bool = False, conjB: bool = False, shape: int = 0) -> np.ndarray:
     """The library's FP64 MFMA GEMM on host arrays (test hook)."""
     lib = load()
     a = np.ascontiguousarray(A, dtype=np.complex128)
@@ -93,10 +99,33 @@ def debug_zgemm(A: np.ndarray, B: np.ndarray, conjA: bool = False, conjB: bool =
     K2, N = b.shape
     assert K == K2
     c = np.empty((M, N), dtype=np.complex128)
-    st = lib.gwhip_debug_zgemm(M, N, K, a.ctypes.data, b.ctypes.data, c.ctypes.data, int(conjA), int(conjB))
+    st = lib.gwhip_debug_zgemm(M, N, K, a.ctypes.data, b.ctypes.data, c.ctypes.data, int(conjA), int(conjB), shape)
     if st != OK:
         raise GwhipError(st, "debug_zgemm failed")
     return c
+
+
+def debug_zgemm_split(A: np.ndarray, B: np.ndarray, slices: int = 4) -> np.ndarray:
+    """The split-precision (int8 slice) GEMM on host arrays (test hook)."""
+    lib = load()
+    a = np.ascontiguousarray(A, dtype=np.complex128)
+    b = np.ascontiguousarray(B, dtype=np.complex128)
+    M, K = a.shape
+    N = b.shape[1]
+    c = np.empty((M, N), dtype=np.complex128)
+    st = lib.gwhip_debug_zgemm_split(M, N, K, a.ctypes.data, b.ctypes.data, c.ctypes.data, slices)
+    if st != OK:
+        raise GwhipError(st, "debug_zgemm_split failed")
+    return c
+
+
+def debug_gemm_bench(which: int, NQ: int, nk: int, nao: int, batch: int, shape: int = 0, reps: int = 3) -> float:
+    """Mean ms per launch of the FP64 GEMM in the Sigma (0) or P0 (1) layout (test hook)."""
+    ms = ctypes.c_double()
+    st = load().gwhip_debug_gemm_bench(which, NQ, nk, nao, batch, shape, reps, ctypes.byref(ms))
+    if st != OK:
+        raise GwhipError(st, "debug_gemm_bench failed")
+    return ms.value
 
 
 def _ptr(a: np.ndarray) -> ctypes.c_void_p:
@@ -141,6 +170,17 @@ class GwHipKernel:
         b = np.ascontiguousarray(T_tb, dtype=np.complex128)
         assert a.shape == (self.dims["nw_b"], self.dims["nts"]) and b.shape == (self.dims["nts"], self.dims["nw_b"])
         self._check(self.lib.gwhip_set_grid(self.h, _ptr(a), _ptr(b)))
+
+    def set_kq_maps(self, kplus: np.ndarray, kminus: np.ndarray):
+        """Explicit k + q / k - q tables [nq][nk] (instead of the Monkhorst-Pack mesh maps)."""
+        a = np.ascontiguousarray(kplus, dtype=np.int32)
+        b = np.ascontiguousarray(kminus, dtype=np.int32)
+        nk = self.dims["nk"]
+        assert a.shape == (nk, nk) and b.shape == (nk, nk)
+        self._check(self.lib.gwhip_set_kq_maps(self.h, _ptr(a), _ptr(b)))
+
+    def set_split_slices(self, ns: int):
+        self._check(self.lib.gwhip_set_split_slices(self.h, ns))
 
     def upload_V(self, Vp: np.ndarray):
         nk, NQ, nao = self.dims["nk"], self.dims["NQ"], self.dims["nao"]

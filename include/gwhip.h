@@ -72,6 +72,12 @@ void gwhip_destroy(gwhip_handle* h);
 /* Message of the last failed call on h (static string for h == NULL). */
 const char* gwhip_last_error(const gwhip_handle* h);
 
+/* Replace the k +/- q maps derived from `mesh` by explicit tables [nq][nk]: kplus[q][k] =
+ * k1_from_k2q(k, q) and kminus[q][k] = k2_from_k1q(k, q) (green-symmetry's k_q_map, used at
+ * src/gw_cpu_kernel.cpp:90,117).  Every q must have a partner q' with kplus[q'] == kminus[q];
+ * call before uploading or generating V. */
+int gwhip_set_kq_maps(gwhip_handle* h, const int32_t* kplus, const int32_t* kminus);
+
 /* Set the bosonic transforms: T_bt [nw_b][nts] maps P(tau_f) -> P(i Omega_b) and
  * T_tb [nts][nw_b] maps back; both complex.
  * Replaces: transformer_t::tau_f_to_w_b / w_b_to_tau_f as used at src/gw_cpu_kernel.cpp:234,264. */
@@ -88,6 +94,12 @@ int gwhip_upload_V(gwhip_handle* h, const double* V_packed);
  * without a host copy.  Same counters as the Python generator gwhip.inputs.generate_V.
  * No reference counterpart: synthetic-input path for benchmarks. */
 int gwhip_generate_V(gwhip_handle* h, uint64_t seed, double sigma);
+
+/* Number of int8 slices per operand in GWHIP_PREC_SPLIT (3, 4 or 5; default 4): the P0 and
+ * Sigma contractions keep 7*slices bits of each operand relative to its row/column maximum
+ * per K chunk and accumulate slice products exactly (int32), then in FP64.
+ * No reference counterpart (the reference's P_sp / Sigma_sp use complex<float>). */
+int gwhip_set_split_slices(gwhip_handle* h, int slices);
 
 /* Sigma(tau, s, k) from G(tau, s, k); host buffers [nts][ns][nk][nso][nso] complex.
  * Sigma is overwritten (the reference zeroes it first, src/gw_cpu_kernel.cpp:39-42).
@@ -122,13 +134,32 @@ size_t gwhip_device_bytes(const gwhip_handle* h);
 /* ---- test hooks (no reference counterpart; used by the parity tests) ---- */
 
 /* C = opA(A) opB(B) with the library's FP64 MFMA GEMM on device 0; row-major host
- * arrays A [M][K], B [K][N], C [M][N] complex; op = conjugate when the flag is set. */
-int gwhip_debug_zgemm(int M, int N, int K, const double* A, const double* B, double* C, int conjA, int conjB);
+ * arrays A [M][K], B [K][N], C [M][N] complex; op = conjugate when the flag is set;
+ * shape selects the tile (0: 64x64, 1: 128x64, 2: 64x128). */
+int gwhip_debug_zgemm(int M, int N, int K, const double* A, const double* B, double* C, int conjA, int conjB,
+                      int shape);
+
+/* C = A B with the split-precision (int8-slice) GEMM on device 0, `slices` slices per operand,
+ * one K chunk; row-major host arrays as in gwhip_debug_zgemm. */
+int gwhip_debug_zgemm_split(int M, int N, int K, const double* A, const double* B, double* C, int slices);
 
 /* Run an evaluation only up to stage `last_stage` (1: P0 on the half tau grid after
  * Hermitisation, 2: P-tilde on all tau) and leave the result where gwhip_get_P_tilde
  * reads it.  Sigma is not produced. */
 int gwhip_debug_stage(gwhip_handle* h, const double* G, int last_stage);
+
+/* Time the FP64 GEMM on device 0 with device-resident operands laid out as one q of the
+ * Sigma contraction (which = 0: M = NQ, N = nk*nao^2, K = NQ, `batch` tau) or of the P0
+ * contraction (which = 1: M = N = NQ, K = nk*nao^2, `batch` tau); `shape` selects the tile
+ * (0: 64x64, 1: 128x64, 2: 64x128).  Writes the mean milliseconds per launch over `reps`. */
+int gwhip_debug_gemm_bench(int which, int NQ, int nk, int nao, int batch, int shape, int reps, double* ms);
+
+/* Register-only FP64 MFMA throughput probe on device 0: `chains` independent accumulators
+ * per wave (1, 2, 4 or 8; 5 = FP64 vector FMA only, 6 = four MFMA chains plus vector FMAs in
+ * the same wave, 10 = int8 16x16x64, 11 = bf16 16x16x32), `waves_per_simd` waves on every SIMD,
+ * `iters` rounds.  Writes the achieved
+ * TFLOP/s (2048 flop per v_mfma_f64_16x16x4f64, 2 per FMA lane). */
+int gwhip_debug_mfma_peak(int chains, int waves_per_simd, int iters, double* tflops);
 
 #ifdef __cplusplus
 }

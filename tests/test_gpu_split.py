@@ -1,0 +1,119 @@
+"""Split-precision path (int8 slices, exact int32 accumulation, FP64 accumulation across
+chunks) -- needs an MI355X.
+
+Bars: the slice GEMM reproduces exactly representable products bit for bit and random
+products within the slicing bound; the split Sigma is within 1e-8 relative (max-abs
+difference over max |Sigma|) of the FP64 oracle.
+"""
+import numpy as np
+import pytest
+
+from gwhip import lib as gl
+from helpers import CONFIG0, load_fixture, rel_err, synthetic
+from oracle import gw_oracle as go
+
+pytestmark = pytest.mark.gpu
+
+TOL_SPLIT = 1e-8
+
+
+def test_split_gemm_identity_probe():
+    """A = I and small integers in B are exact in every slice count: checks the int8 MFMA layout."""
+    n = 16
+    A = np.eye(n, dtype=complex)
+    B = (np.arange(n)[:, None] * 100 + np.arange(n)[None, :]).astype(complex) * (1 - 2j)
+    C = gl.debug_zgemm_split(A, B, 4)
+    if not np.array_equal(C, B):
+        bad = np.argwhere(C != B)
+        raise AssertionError(f"layout mismatch at {len(bad)} entries; first: C{tuple(bad[0])}="
+                             f"{C[tuple(bad[0])]} expected {B[tuple(bad[0])]}")
+
+
+@pytest.mark.parametrize("shape", [(1, 1, 1), (70, 130, 300), (300, 257, 700)])
+@pytest.mark.parametrize("ns", [3, 4, 5])
+def test_split_gemm_random(shape, ns):
+    M, N, K = shape
+    rng = np.random.default_rng(M + N + K + ns)
+    A = rng.standard_normal((M, K)) + 1j * rng.standard_normal((M, K))
+    B = rng.standard_normal((K, N)) + 1j * rng.standard_normal((K, N))
+    C = gl.debug_zgemm_split(A, B, ns)
+    ref = A @ B
+    # per-element bound: each operand keeps 7*ns bits relative to its row / column maximum
+    bound = 8.0 * K * 2.0 ** (-7 * ns) * np.abs(A).max() * np.abs(B).max()
+    assert np.abs(C - ref).max() < bound
+
+
+def _kernel(d, T):
+    return gl.GwHipKernel(d["nk"], d["nao"], d["nso"], d["ns"], d["NQ"], T.nts, T.nw_b, d["beta"], mesh=d["mesh"])
+
+
+def _split_vs_oracle(G, V, T, d, ns=4):
+    k = _kernel(d, T)
+    try:
+        k.set_grid(T.T_bt, T.T_tb)
+        k.upload_V(V)
+        k.set_split_slices(ns)
+        S_split = k.solve(G, gl.PREC_SPLIT)
+        S_fp64 = k.solve(G, gl.PREC_FP64)
+    finally:
+        k.close()
+    S_ref = go.gw_sigma(G, V, T.T_bt, T.T_tb, d["nk"], d["nao"], d["nso"], d["ns"], d["NQ"], mesh=d["mesh"])
+    return S_split, S_fp64, S_ref
+
+
+@pytest.mark.parametrize("ns", [4, 5])
+def test_split_sigma_config0(ns):
+    c = CONFIG0
+    G, V, T, d = synthetic(c["nk"], c["nao"], c["NQ"], beta=c["beta"], lam=c["lam"], mesh=c["mesh"])
+    S_split, S_fp64, S_ref = _split_vs_oracle(G, V, T, d, ns)
+    assert rel_err(S_fp64, S_ref) < 1e-10
+    err = rel_err(S_split, S_ref)
+    print(f"split ns={ns}: rel err vs oracle {err:.3e}")
+    assert err < TOL_SPLIT
+    # Galitskii-Migdal energy from both
+    E_ref = go.gm_energy(G, S_ref, T.T_ft, T.T_tf, d["nk"], d["ns"])
+    E_split = go.gm_energy(G, S_split, T.T_ft, T.T_tf, d["nk"], d["ns"])
+    assert abs(E_split - E_ref) < 1e-8 * abs(E_ref)
+
+
+@pytest.mark.parametrize("case", [
+    dict(nk=3, mesh=(3, 1, 1), nao=3, NQ=5, ns=2),
+    dict(nk=2, mesh=(2, 1, 1), nao=3, NQ=70, ns=1, nso=6),
+])
+def test_split_small_cases(case):
+    nso = case.get("nso", case["nao"])
+    G, V, T, d = synthetic(case["nk"], case["nao"], case["NQ"], ns=case["ns"], beta=20.0, lam=1e3,
+                           mesh=case["mesh"], seed=9, nso=nso, sigma=0.3)
+    S_split, _, S_ref = _split_vs_oracle(G, V, T, d)
+    assert rel_err(S_split, S_ref) < TOL_SPLIT
+
+
+def test_split_reference_fixture():
+    G, V, T, d, S_fix = load_fixture("gw_x2c")
+    S_split, _, S_ref = _split_vs_oracle(G, V, T, d)
+    assert rel_err(S_split, S_ref) < TOL_SPLIT
+    assert np.abs(S_split - S_fix).max() < 1e-6
+
+
+def test_explicit_kq_maps_match_mesh_maps():
+    mesh = (2, 2, 1)
+    G, V, T, d = synthetic(4, 3, 6, beta=20.0, lam=1e3, mesh=mesh, seed=3, sigma=0.3)
+    nk = 4
+    kp = np.array([[go.k_add(k, q, mesh, +1) for k in range(nk)] for q in range(nk)], dtype=np.int32)
+    km = np.array([[go.k_add(k, q, mesh, -1) for k in range(nk)] for q in range(nk)], dtype=np.int32)
+    k1 = _kernel(d, T)
+    k2 = _kernel(d, T)
+    try:
+        for k in (k1, k2):
+            k.set_grid(T.T_bt, T.T_tb)
+        k2.set_kq_maps(kp, km)
+        for k in (k1, k2):
+            k.upload_V(V)
+        S1, S2 = k1.solve(G), k2.solve(G)
+        with pytest.raises(gl.GwhipError) as e:
+            k2.set_kq_maps(kp + 10, km)
+        assert e.value.status == gl.ERR_ARG
+    finally:
+        k1.close()
+        k2.close()
+    assert np.array_equal(S1, S2)

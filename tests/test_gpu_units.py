@@ -24,12 +24,13 @@ def test_zgemm_identity_probe():
 
 @pytest.mark.parametrize("shape", [(1, 1, 1), (16, 16, 4), (64, 64, 16), (70, 130, 37), (300, 257, 41)])
 @pytest.mark.parametrize("conj", [(False, False), (True, False), (False, True)])
-def test_zgemm_random(shape, conj):
+@pytest.mark.parametrize("tile", [0, 1, 2])
+def test_zgemm_random(shape, conj, tile):
     M, N, K = shape
     rng = np.random.default_rng(M * 1000 + N + K)
     A = rng.standard_normal((M, K)) + 1j * rng.standard_normal((M, K))
     B = rng.standard_normal((K, N)) + 1j * rng.standard_normal((K, N))
-    C = gl.debug_zgemm(A, B, *conj)
+    C = gl.debug_zgemm(A, B, *conj, shape=tile)
     Aop = np.conj(A) if conj[0] else A
     Bop = np.conj(B) if conj[1] else B
     ref = Aop @ Bop
