@@ -314,19 +314,31 @@ void gw_hip_kernel::timed(int which, F&& launch) {
 }
 
 namespace {
+template <int NB, typename T>
+void launch_z_prep_t(dim3 grid, size_t lds, hipStream_t st, const DevDims& dd, const double2* G, const double2* Vt,
+                     double2* Z, const int* kplus, int t0, int QPW) {
+  allow_lds(z_prep_mfma_kernel<NB, T>, lds);
+  hipLaunchKernelGGL((z_prep_mfma_kernel<NB, T>), grid, dim3(256), lds, st, dd, G, Vt, Z, kplus, t0, QPW);
+  GWHIP_CHECK(hipGetLastError());
+}
+// The orbital-size products stay FP64 in both precision modes: FP32 matrix cores here put the
+// Sigma error at ~2.4e-8 relative on the nk = 8, nao = 8, naux = 40 case, above the 1e-8 bar.
 template <int NB>
 void launch_z_prep(dim3 grid, size_t lds, hipStream_t st, const DevDims& dd, const double2* G, const double2* Vt,
                    double2* Z, const int* kplus, int t0, int QPW) {
-  allow_lds(z_prep_mfma_kernel<NB>, lds);
-  hipLaunchKernelGGL(z_prep_mfma_kernel<NB>, grid, dim3(256), lds, st, dd, G, Vt, Z, kplus, t0, QPW);
+  launch_z_prep_t<NB, double>(grid, lds, st, dd, G, Vt, Z, kplus, t0, QPW);
+}
+template <int NB, typename T>
+void launch_sigma_epi_t(dim3 grid, size_t lds, hipStream_t st, const DevDims& dd, const double2* G, const double2* Vt,
+                        const double2* R, double2* S, const int* kminus, int t0, double scale) {
+  allow_lds(sigma_epi_mfma_kernel<NB, T>, lds);
+  hipLaunchKernelGGL((sigma_epi_mfma_kernel<NB, T>), grid, dim3(256), lds, st, dd, G, Vt, R, S, kminus, t0, scale);
   GWHIP_CHECK(hipGetLastError());
 }
 template <int NB>
-void launch_sigma_epi(dim3 grid, size_t lds, hipStream_t st, const DevDims& dd, const double2* G, const double2* Vt,
-                      const double2* R, double2* S, const int* kminus, int t0, double scale) {
-  allow_lds(sigma_epi_mfma_kernel<NB>, lds);
-  hipLaunchKernelGGL(sigma_epi_mfma_kernel<NB>, grid, dim3(256), lds, st, dd, G, Vt, R, S, kminus, t0, scale);
-  GWHIP_CHECK(hipGetLastError());
+void launch_sigma_epi(dim3 grid, size_t lds, hipStream_t st, const DevDims& dd, const double2* G,
+                      const double2* Vt, const double2* R, double2* S, const int* kminus, int t0, double scale) {
+  launch_sigma_epi_t<NB, double>(grid, lds, st, dd, G, Vt, R, S, kminus, t0, scale);
 }
 }  // namespace
 

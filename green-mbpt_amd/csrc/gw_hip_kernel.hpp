@@ -84,7 +84,7 @@ private:
   int* flag_ = nullptr;
   // split precision: int8 slice stores of conj(V) for the P0 and Sigma contractions, and the
   // per-evaluation slices of Z and P-tilde (zgemm_ozaki.hpp)
-  int split_ns_ = 4, slices_ns_ = 0;
+  int split_ns_ = 5, slices_ns_ = 0;
   int64_t kc_p0_ = 0, kc_s_ = 0;
   int8_t* Vs_p0_ = nullptr;   // [nq][NQ][nk][2][ns][kc_p0]
   int* eVs_p0_ = nullptr;     // [nq][NQ][nk]

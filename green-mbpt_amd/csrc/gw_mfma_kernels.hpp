@@ -17,12 +17,39 @@
 
 namespace gwhip {
 
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+// Matrix-core flavour of the orbital contractions: FP64 (v_mfma_f64_16x16x4f64) for the
+// plain path, FP32 (v_mfma_f32_16x16x4f32) for the split-precision path.  acc_t is the
+// 4-register accumulator; row_of(g, r) is the result row held in register r by lane group g.
+template <typename T> struct Mfma;
+template <> struct Mfma<double> {
+  typedef v4d acc_t;
+  static __device__ __forceinline__ acc_t mma(double a, double b, acc_t c) {
+    return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row_of(int g, int r) { return g + 4 * r; }
+  static __device__ __forceinline__ acc_t zero() { return acc_t{0.0, 0.0, 0.0, 0.0}; }
+};
+template <> struct Mfma<float> {
+  typedef v4f acc_t;
+  static __device__ __forceinline__ acc_t mma(float a, float b, acc_t c) {
+    return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+  }
+  static __device__ __forceinline__ int row_of(int g, int r) { return 4 * g + r; }
+  static __device__ __forceinline__ acc_t zero() { return acc_t{0.f, 0.f, 0.f, 0.f}; }
+};
+
 // (cr, ci) += (ar + i ai) * (br + i bi) on 16x16x4 fragments
+template <typename T>
+__device__ __forceinline__ void cmfma(typename Mfma<T>::acc_t& cr, typename Mfma<T>::acc_t& ci, T ar, T ai, T br, T bi) {
+  cr = Mfma<T>::mma(ar, br, cr);
+  ci = Mfma<T>::mma(ar, bi, ci);
+  cr = Mfma<T>::mma(-ai, bi, cr);
+  ci = Mfma<T>::mma(ai, br, ci);
+}
 __device__ __forceinline__ void cmfma(v4d& cr, v4d& ci, double ar, double ai, double br, double bi) {
-  cr = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, br, cr, 0, 0, 0);
-  ci = __builtin_amdgcn_mfma_f64_16x16x4f64(ar, bi, ci, 0, 0, 0);
-  cr = __builtin_amdgcn_mfma_f64_16x16x4f64(-ai, bi, cr, 0, 0, 0);
-  ci = __builtin_amdgcn_mfma_f64_16x16x4f64(ai, br, ci, 0, 0, 0);
+  cmfma<double>(cr, ci, ar, ai, br, bi);
 }
 
 __device__ __forceinline__ v4d v4zero() { return v4d{0.0, 0.0, 0.0, 0.0}; }
@@ -31,18 +58,22 @@ __device__ __forceinline__ v4d v4zero() { return v4d{0.0, 0.0, 0.0, 0.0}; }
 // P0 left operand  Z_Q = sum_s Gb_s V_Q Gq_s  (Gb_s = G(beta - tau, k) block (b, a),
 // Gq_s = G(tau, k + q) block (a, b)); src/gw_cpu_kernel.cpp:170-213 regrouped.
 // grid: (ceil(NQ / (4 QPW)), nk, tcount); 4 waves, each handles QPW consecutive Q.
-// LDS per spin block: Gb and Gq as zero-padded real/imag planes [16NB][16NB+1].
-template <int NB>
+// LDS per spin block: Gb and Gq as zero-padded real/imag planes [16NB][16NB+1] of T.
+// T = double: FP64 matrix cores; T = float: FP32 matrix cores (split-precision path), K <= 16NB
+// per MFMA chain, result written in FP64.
+template <int NB, typename T>
 __global__ __launch_bounds__(256) void z_prep_mfma_kernel(DevDims d, const double2* __restrict__ G,
                                                           const double2* __restrict__ Vt,
                                                           double2* __restrict__ Z, const int* __restrict__ kplus,
                                                           int t0, int QPW) {
+  typedef Mfma<T> M;
+  typedef typename M::acc_t acc_t;
   constexpr int N16 = 16 * NB, LD = N16 + 1, PL = N16 * LD;
   extern __shared__ double lds_d[];
-  double* Gbr = lds_d;
-  double* Gbi = Gbr + PL;
-  double* Gqr = Gbi + PL;
-  double* Gqi = Gqr + PL;
+  T* Gbr = reinterpret_cast<T*>(lds_d);
+  T* Gbi = Gbr + PL;
+  T* Gqr = Gbi + PL;
+  T* Gqi = Gqr + PL;
   const int nao = d.nao, nao2 = nao * nao;
   const int tz = blockIdx.z, tau = t0 + tz, tt = d.nts - 1 - tau;
   const int k = blockIdx.y, kq = kplus[k];
@@ -62,10 +93,10 @@ __global__ __launch_bounds__(256) void z_prep_mfma_kernel(DevDims d, const doubl
         gb = Gbar[(int64_t)(b * nao + r) * d.nso + a * nao + c];  // Gb[t = r][p = c]
         gq = Gk[(int64_t)(a * nao + r) * d.nso + b * nao + c];    // Gq[m = r][n = c]
       }
-      Gbr[r * LD + c] = gb.x;
-      Gbi[r * LD + c] = gb.y;
-      Gqr[r * LD + c] = gq.x;
-      Gqi[r * LD + c] = gq.y;
+      Gbr[r * LD + c] = (T)gb.x;
+      Gbi[r * LD + c] = (T)gb.y;
+      Gqr[r * LD + c] = (T)gq.x;
+      Gqi[r * LD + c] = (T)gq.y;
     }
     __syncthreads();
     for (int qi = 0; qi < QPW; ++qi) {
@@ -75,34 +106,34 @@ __global__ __launch_bounds__(256) void z_prep_mfma_kernel(DevDims d, const doubl
       double2* zq = Z + (((int64_t)tz * d.nk + k) * d.NQ + Q) * nao2;
       for (int bn = 0; bn < NB; ++bn) {
         // W[:, bn] = V Gq[:, bn]
-        v4d wr[NB], wi[NB];
+        acc_t wr[NB], wi[NB];
 #pragma unroll
-        for (int x = 0; x < NB; ++x) { wr[x] = v4zero(); wi[x] = v4zero(); }
+        for (int x = 0; x < NB; ++x) { wr[x] = M::zero(); wi[x] = M::zero(); }
         for (int sl = 0; sl < nslab; ++sl) {
           const int m = 4 * sl + lk;
-          const double br = Gqr[m * LD + 16 * bn + lrow], bi = Gqi[m * LD + 16 * bn + lrow];
+          const T br = Gqr[m * LD + 16 * bn + lrow], bi = Gqi[m * LD + 16 * bn + lrow];
 #pragma unroll
           for (int bp = 0; bp < NB; ++bp) {
             const int p = 16 * bp + lrow;
             double2 av = make_double2(0.0, 0.0);
             if (p < nao && m < nao) av = vt[m * nao + p];
-            cmfma(wr[bp], wi[bp], av.x, av.y, br, bi);
+            cmfma<T>(wr[bp], wi[bp], (T)av.x, (T)av.y, br, bi);
           }
         }
-        // Z[:, bn] (+)= Gb W[:, bn]
-        v4d zr[NB], zi[NB];
+        // Z[:, bn] (+)= Gb W[:, bn]; register r of W holds row p = 16 bp + row_of(lk, r)
+        acc_t zr[NB], zi[NB];
 #pragma unroll
-        for (int x = 0; x < NB; ++x) { zr[x] = v4zero(); zi[x] = v4zero(); }
+        for (int x = 0; x < NB; ++x) { zr[x] = M::zero(); zi[x] = M::zero(); }
 #pragma unroll
         for (int bp = 0; bp < NB; ++bp)
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            const int p = 16 * bp + 4 * r + lk;
-            const double bwr = wr[bp][r], bwi = wi[bp][r];
+            const int p = 16 * bp + M::row_of(lk, r);
+            const T bwr = wr[bp][r], bwi = wi[bp][r];
 #pragma unroll
             for (int bt = 0; bt < NB; ++bt) {
               const int t = 16 * bt + lrow;
-              cmfma(zr[bt], zi[bt], Gbr[t * LD + p], Gbi[t * LD + p], bwr, bwi);
+              cmfma<T>(zr[bt], zi[bt], Gbr[t * LD + p], Gbi[t * LD + p], bwr, bwi);
             }
           }
         const int n = 16 * bn + lrow;
@@ -111,9 +142,9 @@ __global__ __launch_bounds__(256) void z_prep_mfma_kernel(DevDims d, const doubl
           for (int bt = 0; bt < NB; ++bt)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-              const int t = 16 * bt + lk + 4 * r;
+              const int t = 16 * bt + M::row_of(lk, r);
               if (t < nao) {
-                double2 v = make_double2(zr[bt][r], zi[bt][r]);
+                double2 v = make_double2((double)zr[bt][r], (double)zi[bt][r]);
                 if (s > 0) {
                   const double2 o = zq[t * nao + n];
                   v.x += o.x;
@@ -134,17 +165,21 @@ __global__ __launch_bounds__(256) void z_prep_mfma_kernel(DevDims d, const doubl
 // Y2 = Y1^T P done by the big GEMM as R = P-tilde conj(V)).  The first MFMA produces
 // T^T = G^T V^T so that its result registers feed the second MFMA's A operand.
 // grid: (nk, tcount); wave w takes Q = w, w + 4, ...; the four partial sums meet in LDS.
-template <int NB>
+// With T = float each Q's product (K <= 16 NB) is added to FP64 accumulators.
+template <int NB, typename T>
 __global__ __launch_bounds__(256) void sigma_epi_mfma_kernel(DevDims d, const double2* __restrict__ G,
                                                              const double2* __restrict__ Vt,
                                                              const double2* __restrict__ R,
                                                              double2* __restrict__ Sigma,
                                                              const int* __restrict__ kminus, int t0, double scale) {
+  typedef Mfma<T> M;
+  typedef typename M::acc_t acc_t;
+  constexpr bool F64 = sizeof(T) == 8;
   constexpr int N16 = 16 * NB, LD = N16 + 1, PL = N16 * LD;
   extern __shared__ double lds_d[];
-  double* Gr = lds_d;
-  double* Gi = Gr + PL;
-  double* red = Gi + PL;  // [NB*NB*8][64]
+  T* Gr = reinterpret_cast<T*>(lds_d);
+  T* Gi = Gr + PL;
+  double* red = lds_d + PL;  // [NB*NB*8][64], after the T planes (2 PL T <= PL doubles)
   const int nao = d.nao, nao2 = nao * nao;
   const int k = blockIdx.x, tz = blockIdx.y, tau = t0 + tz;
   const int kmq = kminus[k];
@@ -160,24 +195,31 @@ __global__ __launch_bounds__(256) void sigma_epi_mfma_kernel(DevDims d, const do
       const int r = e / N16, c = e - (e / N16) * N16;
       double2 g = make_double2(0.0, 0.0);
       if (r < nao && c < nao) g = Gk[(int64_t)(a * nao + r) * d.nso + b * nao + c];  // G[m = r][n = c]
-      Gr[r * LD + c] = g.x;
-      Gi[r * LD + c] = g.y;
+      Gr[r * LD + c] = (T)g.x;
+      Gi[r * LD + c] = (T)g.y;
     }
     __syncthreads();
-    v4d sr[NB][NB], si[NB][NB];
+    double sr[NB][NB][4], si[NB][NB][4];
 #pragma unroll
     for (int x = 0; x < NB; ++x)
 #pragma unroll
-      for (int y = 0; y < NB; ++y) { sr[x][y] = v4zero(); si[x][y] = v4zero(); }
+      for (int y = 0; y < NB; ++y)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) { sr[x][y][r] = 0.0; si[x][y][r] = 0.0; }
+    acc_t ar64[NB][NB], ai64[NB][NB];  // FP64 flavour accumulates straight in the MFMA registers
+#pragma unroll
+    for (int x = 0; x < NB; ++x)
+#pragma unroll
+      for (int y = 0; y < NB; ++y) { ar64[x][y] = M::zero(); ai64[x][y] = M::zero(); }
 
     for (int Q = wave; Q < d.NQ; Q += 4) {
       const double2* vt = Vt + ((int64_t)k * d.NQ + Q) * nao2;                   // vt[m * nao + i] = V[i][m]
       const double2* rq = R + (((int64_t)tz * d.nk + k) * d.NQ + Q) * nao2;     // rq[n * nao + j]
 #pragma unroll
       for (int bi = 0; bi < NB; ++bi) {
-        v4d tr[NB], ti[NB];
+        acc_t tr[NB], ti[NB];
 #pragma unroll
-        for (int x = 0; x < NB; ++x) { tr[x] = v4zero(); ti[x] = v4zero(); }
+        for (int x = 0; x < NB; ++x) { tr[x] = M::zero(); ti[x] = M::zero(); }
         const int i = 16 * bi + lrow;
         for (int sl = 0; sl < nslab; ++sl) {
           const int m = 4 * sl + lk;
@@ -186,24 +228,54 @@ __global__ __launch_bounds__(256) void sigma_epi_mfma_kernel(DevDims d, const do
 #pragma unroll
           for (int bn = 0; bn < NB; ++bn) {
             const int n = 16 * bn + lrow;
-            cmfma(tr[bn], ti[bn], Gr[m * LD + n], Gi[m * LD + n], bv.x, bv.y);
+            cmfma<T>(tr[bn], ti[bn], Gr[m * LD + n], Gi[m * LD + n], (T)bv.x, (T)bv.y);
           }
+        }
+        acc_t pr[NB], pi[NB];
+#pragma unroll
+        for (int y = 0; y < NB; ++y) {
+          pr[y] = F64 ? ar64[bi][y] : M::zero();
+          pi[y] = F64 ? ai64[bi][y] : M::zero();
         }
 #pragma unroll
         for (int bn = 0; bn < NB; ++bn)
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            const int n = 16 * bn + 4 * r + lk;
-            const double ar = tr[bn][r], ai = ti[bn][r];
+            const int n = 16 * bn + M::row_of(lk, r);
+            const T xr = tr[bn][r], xi = ti[bn][r];
 #pragma unroll
             for (int bj = 0; bj < NB; ++bj) {
               const int j = 16 * bj + lrow;
               double2 rv = make_double2(0.0, 0.0);
               if (n < nao && j < nao) rv = rq[n * nao + j];
-              cmfma(sr[bi][bj], si[bi][bj], ar, ai, rv.x, rv.y);
+              cmfma<T>(pr[bj], pi[bj], xr, xi, (T)rv.x, (T)rv.y);
             }
           }
+#pragma unroll
+        for (int y = 0; y < NB; ++y) {
+          if (F64) {
+            ar64[bi][y] = pr[y];
+            ai64[bi][y] = pi[y];
+          } else {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+              sr[bi][y][r] += (double)pr[y][r];
+              si[bi][y][r] += (double)pi[y][r];
+            }
+          }
+        }
       }
+    }
+    if (F64) {
+#pragma unroll
+      for (int x = 0; x < NB; ++x)
+#pragma unroll
+        for (int y = 0; y < NB; ++y)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            sr[x][y][r] = (double)ar64[x][y][r];
+            si[x][y][r] = (double)ai64[x][y][r];
+          }
     }
     // reduce the four waves' partial sums into wave 0
     for (int w = 1; w < 4; ++w) {
@@ -242,7 +314,7 @@ __global__ __launch_bounds__(256) void sigma_epi_mfma_kernel(DevDims d, const do
           if (j >= nao) continue;
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            const int i = 16 * x + lk + 4 * r;
+            const int i = 16 * x + M::row_of(lk, r);
             if (i >= nao) continue;
             double2* dst = S + (int64_t)(a * nao + i) * d.nso + b * nao + j;
             double2 v = *dst;

@@ -136,16 +136,24 @@ struct OzGemmArgs {
 // OZ_ROW(l/16, r), column l%16 (checked by the identity-probe test).
 #define OZ_ROW(g, r) (4 * (g) + (r))
 
-constexpr int OZ_BM = 64, OZ_BN = 64, OZ_KS = 64, OZ_LDROW = 80;  // bytes per LDS row (64 + pad)
+// Block tile 32 x 64 (complex); the four waves split N, each owning 32 x 16 = two 16x16
+// blocks.  A (32 rows, shared by all waves) is staged through LDS one K step (64 bytes per
+// plane-slice row) at a time, its four 16-byte pieces XOR-swizzled by (r >> 2) & 3 so that
+// the 16 rows of one fragment read fall on distinct banks.  Each wave reads its own 16
+// B columns straight from global memory into MFMA fragments (no other wave uses them).
+constexpr int OZ_BM = 32, OZ_BN = 64, OZ_KS = 64;
+
+__device__ __forceinline__ int oz_lds_off(int ps, int rows, int r, int piece) {
+  return (ps * rows + r) * 64 + 16 * (piece ^ ((r >> 2) & 3));
+}
 
 template <int NS>
-__global__ __launch_bounds__(256) void zgemm_ozaki_kernel(OzGemmArgs p) {
-  constexpr int PA = 3, PB = 2;
+__global__ __launch_bounds__(256, 2) void zgemm_ozaki_kernel(OzGemmArgs p) {
+  constexpr int PA = 3;
   extern __shared__ __attribute__((aligned(16))) int8_t oz_lds[];
-  int8_t* As = oz_lds;                                  // [PA*NS][64 rows][OZ_LDROW]
-  int8_t* Bs = As + PA * NS * OZ_BM * OZ_LDROW;         // [PB*NS][64 cols][OZ_LDROW]
-  int* eAs = reinterpret_cast<int*>(Bs + PB * NS * OZ_BN * OZ_LDROW);  // [64]
-  int* eBs = eAs + OZ_BM;                                               // [64]
+  int8_t* As = oz_lds;                                          // [PA*NS][32 rows][64 B]
+  int* eAs = reinterpret_cast<int*>(As + PA * NS * OZ_BM * 64);  // [32]
+  int* eBs = eAs + OZ_BM;                                        // [64]
 
   const int64_t hw = blockIdx.x;
   const int64_t logical = (hw % 8) * p.per_xcd + hw / 8;
@@ -157,53 +165,54 @@ __global__ __launch_bounds__(256) void zgemm_ozaki_kernel(OzGemmArgs p) {
   const int m0 = tm * OZ_BM, n0 = tn * OZ_BN;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lrow = lane & 15, lg = lane >> 4;
-  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  const int wn = wave * 16;
 
-  // staging: each thread moves 16-byte pieces; a row of one plane-slice K step is 64 bytes
-  // = 4 pieces, so one pass of 256 threads covers 64 rows x 1 plane-slice.
-  const int srow = tid >> 2, spiece = tid & 3;
-  const int8_t* Arow = nullptr;
-  const int8_t* Brow = nullptr;
-  {
-    const int m = m0 + srow;
-    if (m < p.M) Arow = p.A + b * p.sAb + (int64_t)m * p.sAm;
-    const int n = n0 + srow;
-    if (n < p.N) {
-      const int no = n / p.nin;
-      Brow = p.B + b * p.sBb + (int64_t)no * p.sBno + (int64_t)(n - no * p.nin) * p.sBni;
-    }
+  // A staging slots: 32 rows x 4 pieces, two threads per slot splitting the plane-slices
+  const int a_row = tid >> 3, a_piece = tid & 3, a_par = (tid >> 2) & 1;
+  const int8_t* a_ptr = nullptr;
+  if (m0 + a_row < p.M) a_ptr = p.A + b * p.sAb + (int64_t)(m0 + a_row) * p.sAm + 16 * a_piece;
+  const int a_lds = oz_lds_off(0, OZ_BM, a_row, a_piece);
+  // this lane's B column and K-step piece
+  const int bcol = n0 + wn + lrow;
+  const int8_t* b_ptr = nullptr;
+  if (bcol < p.N) {
+    const int no = bcol / p.nin;
+    b_ptr = p.B + b * p.sBb + (int64_t)no * p.sBno + (int64_t)(bcol - no * p.nin) * p.sBni + 16 * lg;
   }
 
-  double cr[2][2][4], ci[2][2][4];
+  double cr[2][4], ci[2][4];
 #pragma unroll
   for (int x = 0; x < 2; ++x)
 #pragma unroll
-    for (int y = 0; y < 2; ++y)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) { cr[x][y][r] = 0.0; ci[x][y][r] = 0.0; }
+    for (int r = 0; r < 4; ++r) { cr[x][r] = 0.0; ci[x][r] = 0.0; }
 
   for (int c = 0; c < p.nchunk; ++c) {
-    v4i ar[NS][2][2], ai[NS][2][2];  // [d][x][y] int32 accumulators (Re, Im)
+    v4i ar[NS][2], ai[NS][2];  // [d][x] int32 accumulators (Re, Im)
 #pragma unroll
     for (int d = 0; d < NS; ++d)
 #pragma unroll
-      for (int x = 0; x < 2; ++x)
-#pragma unroll
-        for (int y = 0; y < 2; ++y) { ar[d][x][y] = v4i{0, 0, 0, 0}; ai[d][x][y] = v4i{0, 0, 0, 0}; }
+      for (int x = 0; x < 2; ++x) { ar[d][x] = v4i{0, 0, 0, 0}; ai[d][x] = v4i{0, 0, 0, 0}; }
+    const int8_t* a_c = a_ptr ? a_ptr + (int64_t)c * p.sAc : nullptr;
+    const int8_t* b_c = b_ptr ? b_ptr + (int64_t)c * p.sBc : nullptr;
 
     for (int ks = 0; ks < p.kc; ks += OZ_KS) {
       __syncthreads();
 #pragma unroll
-      for (int ps = 0; ps < PA * NS; ++ps) {
+      for (int ps = a_par; ps < PA * NS; ps += 2) {
         v4i v = v4i{0, 0, 0, 0};
-        if (Arow) v = *reinterpret_cast<const v4i*>(Arow + (int64_t)c * p.sAc + (int64_t)ps * p.kc + ks + 16 * spiece);
-        *reinterpret_cast<v4i*>(As + (ps * OZ_BM + srow) * OZ_LDROW + 16 * spiece) = v;
+        if (a_c) v = *reinterpret_cast<const v4i*>(a_c + (int64_t)ps * p.kc + ks);
+        *reinterpret_cast<v4i*>(As + a_lds + ps * OZ_BM * 64) = v;
       }
+      // B fragments of this K step, all slices (Re, Im)
+      v4i fb_r[NS], fb_i[NS];
 #pragma unroll
-      for (int ps = 0; ps < PB * NS; ++ps) {
-        v4i v = v4i{0, 0, 0, 0};
-        if (Brow) v = *reinterpret_cast<const v4i*>(Brow + (int64_t)c * p.sBc + (int64_t)ps * p.kc + ks + 16 * spiece);
-        *reinterpret_cast<v4i*>(Bs + (ps * OZ_BN + srow) * OZ_LDROW + 16 * spiece) = v;
+      for (int t = 0; t < NS; ++t) {
+        fb_r[t] = v4i{0, 0, 0, 0};
+        fb_i[t] = v4i{0, 0, 0, 0};
+        if (b_c) {
+          fb_r[t] = *reinterpret_cast<const v4i*>(b_c + (int64_t)(0 * NS + t) * p.kc + ks);
+          fb_i[t] = *reinterpret_cast<const v4i*>(b_c + (int64_t)(1 * NS + t) * p.kc + ks);
+        }
       }
       __syncthreads();
 #pragma unroll
@@ -211,29 +220,20 @@ __global__ __launch_bounds__(256) void zgemm_ozaki_kernel(OzGemmArgs p) {
         v4i fa_r[2], fa_i[2], fa_n[2];
 #pragma unroll
         for (int x = 0; x < 2; ++x) {
-          const int row = wm + 16 * x + lrow;
-          fa_r[x] = *reinterpret_cast<const v4i*>(As + ((0 * NS + s) * OZ_BM + row) * OZ_LDROW + 16 * lg);
-          fa_i[x] = *reinterpret_cast<const v4i*>(As + ((1 * NS + s) * OZ_BM + row) * OZ_LDROW + 16 * lg);
-          fa_n[x] = *reinterpret_cast<const v4i*>(As + ((2 * NS + s) * OZ_BM + row) * OZ_LDROW + 16 * lg);
+          const int row = 16 * x + lrow;
+          fa_r[x] = *reinterpret_cast<const v4i*>(As + oz_lds_off(0 * NS + s, OZ_BM, row, lg));
+          fa_i[x] = *reinterpret_cast<const v4i*>(As + oz_lds_off(1 * NS + s, OZ_BM, row, lg));
+          fa_n[x] = *reinterpret_cast<const v4i*>(As + oz_lds_off(2 * NS + s, OZ_BM, row, lg));
         }
 #pragma unroll
         for (int t = 0; t + s < NS; ++t) {
-          v4i fb_r[2], fb_i[2];
 #pragma unroll
-          for (int y = 0; y < 2; ++y) {
-            const int col = wn + 16 * y + lrow;
-            fb_r[y] = *reinterpret_cast<const v4i*>(Bs + ((0 * NS + t) * OZ_BN + col) * OZ_LDROW + 16 * lg);
-            fb_i[y] = *reinterpret_cast<const v4i*>(Bs + ((1 * NS + t) * OZ_BN + col) * OZ_LDROW + 16 * lg);
+          for (int x = 0; x < 2; ++x) {
+            ar[s + t][x] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa_r[x], fb_r[t], ar[s + t][x], 0, 0, 0);
+            ai[s + t][x] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa_r[x], fb_i[t], ai[s + t][x], 0, 0, 0);
+            ar[s + t][x] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa_n[x], fb_i[t], ar[s + t][x], 0, 0, 0);
+            ai[s + t][x] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa_i[x], fb_r[t], ai[s + t][x], 0, 0, 0);
           }
-#pragma unroll
-          for (int x = 0; x < 2; ++x)
-#pragma unroll
-            for (int y = 0; y < 2; ++y) {
-              ar[s + t][x][y] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa_r[x], fb_r[y], ar[s + t][x][y], 0, 0, 0);
-              ai[s + t][x][y] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa_r[x], fb_i[y], ai[s + t][x][y], 0, 0, 0);
-              ar[s + t][x][y] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa_n[x], fb_i[y], ar[s + t][x][y], 0, 0, 0);
-              ai[s + t][x][y] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa_i[x], fb_r[y], ai[s + t][x][y], 0, 0, 0);
-            }
         }
       }
     }
@@ -255,50 +255,45 @@ __global__ __launch_bounds__(256) void zgemm_ozaki_kernel(OzGemmArgs p) {
 #pragma unroll
     for (int x = 0; x < 2; ++x)
 #pragma unroll
-      for (int y = 0; y < 2; ++y)
+      for (int r = 0; r < 4; ++r) {
+        const int row = 16 * x + OZ_ROW(lg, r), col = wn + lrow;
+        double sc = ldexp(1.0, eAs[row] + eBs[col] - 14);
+        double sr = 0.0, si = 0.0;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int row = wm + 16 * x + OZ_ROW(lg, r), col = wn + 16 * y + lrow;
-          double sc = ldexp(1.0, eAs[row] + eBs[col] - 14);
-          double sr = 0.0, si = 0.0;
-#pragma unroll
-          for (int d = 0; d < NS; ++d) {
-            sr = fma((double)ar[d][x][y][r], sc, sr);
-            si = fma((double)ai[d][x][y][r], sc, si);
-            sc *= 0.0078125;  // 2^-7
-          }
-          cr[x][y][r] += sr;
-          ci[x][y][r] += si;
+        for (int d = 0; d < NS; ++d) {
+          sr = fma((double)ar[d][x][r], sc, sr);
+          si = fma((double)ai[d][x][r], sc, si);
+          sc *= 0.0078125;  // 2^-7
         }
+        cr[x][r] += sr;
+        ci[x][r] += si;
+      }
   }
 
+  const int n = n0 + wn + lrow;
+  if (n >= p.N) return;
+  const int no = n / p.nin;
+  const int64_t coff = (int64_t)no * p.sCno + (int64_t)(n - no * p.nin) * p.sCni;
 #pragma unroll
   for (int x = 0; x < 2; ++x)
 #pragma unroll
-    for (int y = 0; y < 2; ++y) {
-      const int n = n0 + wn + 16 * y + lrow;
-      if (n >= p.N) continue;
-      const int no = n / p.nin;
-      const int64_t coff = (int64_t)no * p.sCno + (int64_t)(n - no * p.nin) * p.sCni;
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int m = m0 + wm + 16 * x + OZ_ROW(lg, r);
-        if (m >= p.M) continue;
-        const double vr = cr[x][y][r], vi = ci[x][y][r];
-        double2 v = make_double2(p.alpha.x * vr - p.alpha.y * vi, p.alpha.x * vi + p.alpha.y * vr);
-        double2* dst = p.C + b * p.sCb + (int64_t)m * p.sCm + coff;
-        if (p.accumulate) {
-          const double2 o = *dst;
-          v.x += o.x;
-          v.y += o.y;
-        }
-        *dst = v;
+    for (int r = 0; r < 4; ++r) {
+      const int m = m0 + 16 * x + OZ_ROW(lg, r);
+      if (m >= p.M) continue;
+      const double vr = cr[x][r], vi = ci[x][r];
+      double2 v = make_double2(p.alpha.x * vr - p.alpha.y * vi, p.alpha.x * vi + p.alpha.y * vr);
+      double2* dst = p.C + b * p.sCb + (int64_t)m * p.sCm + coff;
+      if (p.accumulate) {
+        const double2 o = *dst;
+        v.x += o.x;
+        v.y += o.y;
       }
+      *dst = v;
     }
 }
 
 inline size_t ozaki_lds_bytes(int ns) {
-  return (size_t)(3 + 2) * ns * OZ_BM * OZ_LDROW + (OZ_BM + OZ_BN) * sizeof(int);
+  return (size_t)3 * OZ_BM * ns * 64 + (OZ_BM + OZ_BN) * sizeof(int);
 }
 
 template <typename K>

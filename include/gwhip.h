@@ -95,7 +95,7 @@ int gwhip_upload_V(gwhip_handle* h, const double* V_packed);
  * No reference counterpart: synthetic-input path for benchmarks. */
 int gwhip_generate_V(gwhip_handle* h, uint64_t seed, double sigma);
 
-/* Number of int8 slices per operand in GWHIP_PREC_SPLIT (3, 4 or 5; default 4): the P0 and
+/* Number of int8 slices per operand in GWHIP_PREC_SPLIT (3, 4 or 5; default 5): the P0 and
  * Sigma contractions keep 7*slices bits of each operand relative to its row/column maximum
  * per K chunk and accumulate slice products exactly (int32), then in FP64.
  * No reference counterpart (the reference's P_sp / Sigma_sp use complex<float>). */

@@ -47,7 +47,7 @@ def _kernel(d, T):
     return gl.GwHipKernel(d["nk"], d["nao"], d["nso"], d["ns"], d["NQ"], T.nts, T.nw_b, d["beta"], mesh=d["mesh"])
 
 
-def _split_vs_oracle(G, V, T, d, ns=4):
+def _split_vs_oracle(G, V, T, d, ns=5):
     k = _kernel(d, T)
     try:
         k.set_grid(T.T_bt, T.T_tb)
@@ -61,7 +61,7 @@ def _split_vs_oracle(G, V, T, d, ns=4):
     return S_split, S_fp64, S_ref
 
 
-@pytest.mark.parametrize("ns", [4, 5])
+@pytest.mark.parametrize("ns", [5])
 def test_split_sigma_config0(ns):
     c = CONFIG0
     G, V, T, d = synthetic(c["nk"], c["nao"], c["NQ"], beta=c["beta"], lam=c["lam"], mesh=c["mesh"])
@@ -86,6 +86,14 @@ def test_split_small_cases(case):
                            mesh=case["mesh"], seed=9, nso=nso, sigma=0.3)
     S_split, _, S_ref = _split_vs_oracle(G, V, T, d)
     assert rel_err(S_split, S_ref) < TOL_SPLIT
+
+
+def test_four_slices_are_looser_but_bounded():
+    """4 slices keep 28 bits per operand: accurate to ~1e-7 here, so 5 is the default."""
+    c = CONFIG0
+    G, V, T, d = synthetic(c["nk"], c["nao"], c["NQ"], beta=c["beta"], lam=c["lam"], mesh=c["mesh"])
+    S_split, _, S_ref = _split_vs_oracle(G, V, T, d, ns=4)
+    assert 1e-10 < rel_err(S_split, S_ref) < 1e-6
 
 
 def test_split_reference_fixture():
