@@ -87,11 +87,16 @@ def main():
     ap.add_argument("--slices", type=int, default=5, help="int8 slices per operand in split precision")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 reference evaluation")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--vjp", action="store_true",
+                    help="time the Sigma[G] vector-Jacobian product on the nk=27 nao=26 naux=250 system instead")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
               "sharding over q with one Sigma all-reduce is described in DESIGN.md, not run.", file=sys.stderr)
         sys.exit(2)
+
+    if args.vjp:
+        return bench_vjp(args)
 
     import numpy as np
     import torch
@@ -187,6 +192,75 @@ def main():
         out["cpu_baseline"] = cpu_baseline(w, T)
     k.close()
     print(json.dumps(out), flush=True)
+
+
+VJP_WORKLOAD = dict(nk=27, mesh=(3, 3, 3), nao=26, NQ=250, ns=1, beta=100.0, lam=1e4, seed=0, sigma=0.1)
+
+
+def bench_vjp(args):
+    """VJP of Sigma[G] with a Hermitian-per-tau N(0,1) cotangent, P-tilde recomputed vs stored."""
+    import numpy as np
+    import torch
+
+    from gwhip import inputs, irgrid
+    from gwhip import lib as gl
+
+    w = VJP_WORKLOAD
+    T = irgrid.Transformer.from_grid(irgrid.load_grid(w["lam"]), w["beta"])
+    G = inputs.generate_G(w["seed"], w["nk"], w["nao"], w["ns"], T.tau, w["beta"])
+    rng = np.random.default_rng(1)
+    Sb = rng.standard_normal(G.shape) + 1j * rng.standard_normal(G.shape)
+    Sb = np.ascontiguousarray(0.5 * (Sb + np.conj(np.swapaxes(Sb, -1, -2))))
+    k = gl.GwHipKernel(w["nk"], w["nao"], w["nao"], w["ns"], w["NQ"], T.nts, T.nw_b, w["beta"], mesh=w["mesh"])
+    k.set_grid(T.T_bt, T.T_tb)
+    k.generate_V(w["seed"], w["sigma"])
+    dev = torch.device("cuda:0")
+    G_d = torch.from_numpy(G.view(np.float64)).to(dev)
+    Sb_d = torch.from_numpy(Sb.view(np.float64)).to(dev)
+    S_d = torch.empty_like(G_d)
+    Gb_d = torch.empty_like(G_d)
+    torch.cuda.synchronize()
+    out = {}
+    base_bytes = k.device_bytes()
+    for mode, name in ((gl.VJP_RECOMPUTE, "recompute"), (gl.VJP_STORED, "stored")):
+        k.set_vjp_store(mode == gl.VJP_STORED)
+        k.solve_device(G_d.data_ptr(), S_d.data_ptr(), gl.PREC_FP64)
+        for _ in range(args.warmup):
+            k.vjp_device(G_d.data_ptr(), Sb_d.data_ptr(), Gb_d.data_ptr(), mode)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            k.vjp_device(G_d.data_ptr(), Sb_d.data_ptr(), Gb_d.data_ptr(), mode)
+            print(f"[bench] vjp {name} step {i} done", file=sys.stderr, flush=True)
+        torch.cuda.synchronize()
+        out[name] = {"ms_per_vjp": (time.perf_counter() - t0) / args.steps * 1e3,
+                     "device_gb": k.device_bytes() / 1e9}
+    t0 = time.perf_counter()
+    k.solve_device(G_d.data_ptr(), S_d.data_ptr(), gl.PREC_FP64)
+    torch.cuda.synchronize()
+    fwd_ms = (time.perf_counter() - t0) * 1e3
+    # one central finite difference along a Hermitian-per-tau direction, step 1e-6
+    Gbar = Gb_d.cpu().numpy().view(np.complex128)
+    dG = rng.standard_normal(G.shape) + 1j * rng.standard_normal(G.shape)
+    dG = 0.5 * (dG + np.conj(np.swapaxes(dG, -1, -2)))
+    h = 1e-6
+    k.set_vjp_store(False)
+    Sp = k.solve(G + h * dG)
+    Sm = k.solve(G - h * dG)
+    fd = float(np.real(np.vdot(Sb, (Sp - Sm) / (2 * h))))
+    an = float(np.real(np.vdot(Gbar, dG)))
+    k.close()
+    ms = out["stored"]["ms_per_vjp"]
+    print(json.dumps({
+        "metric": "Sigma[G] VJPs/s @nk27 nao26 naux250 (FP64, P-tilde stored)", "value": 1e3 / ms, "unit": "VJP/s",
+        "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seed 0 G and V, Hermitian-per-tau N(0,1) cotangent)",
+        "config": {"workload": "VJP of GW Sigma(tau,k) wrt G, 3x3x3 k mesh, nao 26, naux 250, ns 1, beta 100, IR 1e4",
+                   "nts": T.nts, "nw_b": T.nw_b, "parallelism": "single GPU"},
+        "modes": out, "forward_ms": fwd_ms, "base_device_gb": base_bytes / 1e9,
+        "fd_check": {"step": h, "fd": fd, "vjp": an, "rel_diff": abs(fd - an) / abs(an)},
+    }), flush=True)
 
 
 if __name__ == "__main__":

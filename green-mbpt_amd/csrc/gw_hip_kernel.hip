@@ -9,6 +9,7 @@
 #include "../../include/gwhip.h"
 #include "gw_kernels.hpp"
 #include "gw_mfma_kernels.hpp"
+#include "gw_vjp_kernels.hpp"
 #include "zgemm_mfma.hpp"
 #include "zgemm_ozaki.hpp"
 
@@ -119,6 +120,7 @@ gw_hip_kernel::~gw_hip_kernel() {
                   (void*)Ttb_, (void*)G_dev_, (void*)S_dev_, (void*)kplus_tab_, (void*)kminus_tab_, (void*)flag_})
     if (p) (void)hipFree(p);
   free_slices();
+  free_vjp_buffers();
   if (stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -160,6 +162,7 @@ void gw_hip_kernel::set_kq_maps(const int32_t* kplus, const int32_t* kminus) {
   const size_t n = (size_t)d_.nk * d_.nk;
   upload_maps(std::vector<int>(kplus, kplus + n), std::vector<int>(kminus, kminus + n));
   v_set_ = false;  // the V store is indexed by the maps
+  stored_valid_ = false;
 }
 
 void gw_hip_kernel::set_grid(const double2* T_bt, const double2* T_tb) {
@@ -176,6 +179,7 @@ void gw_hip_kernel::set_grid(const double2* T_bt, const double2* T_tb) {
   GWHIP_CHECK(hipMemcpy(Tbt_fold_, fold.data(), fold.size() * sizeof(double2), hipMemcpyHostToDevice));
   GWHIP_CHECK(hipMemcpy(Ttb_, T_tb, (size_t)nts * nwb * sizeof(double2), hipMemcpyHostToDevice));
   grid_set_ = true;
+  stored_valid_ = false;
 }
 
 void gw_hip_kernel::upload_V(const double2* V_packed) {
@@ -204,6 +208,7 @@ void gw_hip_kernel::upload_V(const double2* V_packed) {
   GWHIP_CHECK(e);
   make_transposed_V();
   v_set_ = true;
+  stored_valid_ = false;
 }
 
 void gw_hip_kernel::free_slices() {
@@ -294,6 +299,7 @@ void gw_hip_kernel::generate_V(uint64_t seed, double sigma) {
   GWHIP_CHECK(e);
   make_transposed_V();
   v_set_ = true;
+  stored_valid_ = false;
 }
 
 template <typename F>
@@ -343,14 +349,18 @@ void launch_sigma_epi(dim3 grid, size_t lds, hipStream_t st, const DevDims& dd, 
 }  // namespace
 
 void gw_hip_kernel::stage_p0(const double2* G, bool split) {
+  for (int64_t q = 0; q < d_.nk; ++q) p0_one_q(q, G, split, P_ + q * d_.nts * d_.NQ * d_.NQ);
+}
+
+// P0(tau, q) for the half tau grid into Pq[t][Q][Q'] (t < nts/2), Hermitised.
+void gw_hip_kernel::p0_one_q(int64_t q, const double2* G, bool split, double2* Pq) {
   const int64_t nk = d_.nk, NQ = d_.NQ, nao2 = d_.nao * d_.nao, NQ2 = NQ * NQ, half = d_.nts / 2;
   DevDims dd{(int)d_.nts, (int)d_.ns, (int)nk, (int)d_.nso, (int)d_.nao, (int)NQ, (int)d_.pns, (int)d_.pseudo_ns};
   const int NB = (int)ceil_div(d_.nao, 16);
   const size_t lds = (size_t)4 * (16 * NB) * (16 * NB + 1) * sizeof(double);
   const int QPW = 4;
   const unsigned gx = (unsigned)ceil_div(NQ, 4 * QPW);
-
-  for (int64_t q = 0; q < nk; ++q) {
+  {
     const double2* Vq = Vq_ + q * nk * NQ * nao2;
     const double2* Vtq = Vt_ + q * nk * NQ * nao2;
     for (int64_t t0 = 0; t0 < half; t0 += tchunk_p0_) {
@@ -376,7 +386,7 @@ void gw_hip_kernel::stage_p0(const double2* G, bool split) {
         o.B = Vs_p0_ + q * NQ * nk * 2 * ns * kc_p0_; o.sBb = 0; o.sBno = 0; o.sBni = nk * 2 * ns * kc_p0_;
         o.sBc = 2 * ns * kc_p0_;
         o.eB = eVs_p0_ + q * NQ * nk; o.seBb = 0; o.seBno = 0; o.seBni = nk;
-        o.C = P_ + (q * d_.nts + t0) * NQ2; o.sCb = NQ2; o.sCm = NQ; o.sCno = 0; o.sCni = 1;
+        o.C = Pq + t0 * NQ2; o.sCb = NQ2; o.sCm = NQ; o.sCno = 0; o.sCni = 1;
         o.alpha = make_double2(-d_.prefactor / (double)nk, 0.0);
         o.accumulate = 0;
         timed(K_P0_GEMM, [&] { zgemm_ozaki(o, ns, stream_); });
@@ -387,20 +397,21 @@ void gw_hip_kernel::stage_p0(const double2* G, bool split) {
       a.kin = (int)nao2; a.nin = (int)NQ;
       a.A = scratch_; a.sAb = nk * NQ * nao2; a.sAm = nao2; a.sAko = NQ * nao2; a.sAki = 1;
       a.B = Vq; a.sBb = 0; a.sBko = NQ * nao2; a.sBki = 1; a.sBno = 0; a.sBni = nao2;
-      a.C = P_ + (q * d_.nts + t0) * NQ2; a.sCb = NQ2; a.sCm = NQ; a.sCno = 0; a.sCni = 1;
+      a.C = Pq + t0 * NQ2; a.sCb = NQ2; a.sCm = NQ; a.sCno = 0; a.sCni = 1;
       a.alpha = make_double2(-d_.prefactor / (double)nk, 0.0);
       a.accumulate = 0; a.conjA = 0; a.conjB = 1;
       timed(K_P0_GEMM, [&] { zgemm_f64(a, true, true, stream_); });
     }
     // Hermitise the half grid of this q (src/gw_cpu_kernel.cpp:217-225)
     hipLaunchKernelGGL(hermitize_kernel, dim3((unsigned)ceil_div(NQ2, 256), (unsigned)half), dim3(256), 0, stream_,
-                       P_ + q * d_.nts * NQ2, (int)NQ, NQ2);
+                       Pq, (int)NQ, NQ2);
     GWHIP_CHECK(hipGetLastError());
   }
 }
 
-void gw_hip_kernel::stage_dyson() {
-  const int64_t nk = d_.nk, NQ = d_.NQ, NQ2 = NQ * NQ, half = d_.nts / 2, nwb = d_.nw_b, nts = d_.nts;
+// In-place inverse of nmat Hermitian positive definite NQ x NQ matrices (I - P0(i Omega)).
+void gw_hip_kernel::gj_inverse(double2* X, int64_t nmat) {
+  const int64_t NQ = d_.NQ, NQ2 = NQ * NQ;
   int nb = GJ_NB;
   size_t gj_lds = (size_t)(GJ_NB * GJ_NB + NQ * nb) * sizeof(double2);
   if ((int64_t)gj_lds > max_lds_) {
@@ -410,6 +421,12 @@ void gw_hip_kernel::stage_dyson() {
   if ((int64_t)gj_lds > max_lds_) throw status_error(GWHIP_ERR_UNSUPPORTED, "NQ too large for the Dyson kernel");
   allow_lds(gj_inverse_kernel, gj_lds);
   const int gj_threads = (int)std::min<int64_t>(1024, std::max<int64_t>(256, ceil_div(NQ, 64) * 64));
+  hipLaunchKernelGGL(gj_inverse_kernel, dim3((unsigned)nmat), dim3(gj_threads), gj_lds, stream_, X, (int)NQ, NQ2, nb);
+  GWHIP_CHECK(hipGetLastError());
+}
+
+void gw_hip_kernel::stage_dyson() {
+  const int64_t nk = d_.nk, NQ = d_.NQ, NQ2 = NQ * NQ, half = d_.nts / 2, nwb = d_.nw_b, nts = d_.nts;
 
   hipEvent_t e0, e1, e2, e3;
   GWHIP_CHECK(hipEventCreate(&e0)); GWHIP_CHECK(hipEventCreate(&e1));
@@ -432,10 +449,7 @@ void gw_hip_kernel::stage_dyson() {
     hipLaunchKernelGGL(identity_minus_kernel, dim3((unsigned)ceil_div(nmat * NQ2, 256)), dim3(256), 0, stream_, X_, Pw_,
                        (int)NQ, nmat);
     GWHIP_CHECK(hipGetLastError());
-    timed(K_DYSON_INV, [&] {
-      hipLaunchKernelGGL(gj_inverse_kernel, dim3((unsigned)nmat), dim3(gj_threads), gj_lds, stream_, X_, (int)NQ, NQ2, nb);
-      GWHIP_CHECK(hipGetLastError());
-    });
+    timed(K_DYSON_INV, [&] { gj_inverse(X_, nmat); });
     hipLaunchKernelGGL(hermitize_kernel, dim3((unsigned)ceil_div(NQ2, 256), (unsigned)nmat), dim3(256), 0, stream_, X_,
                        (int)NQ, NQ2);
     GWHIP_CHECK(hipGetLastError());
@@ -539,7 +553,13 @@ void gw_hip_kernel::solve_device(const double2* G, double2* Sigma, int precision
   GWHIP_CHECK(hipEventCreate(&t2)); GWHIP_CHECK(hipEventCreate(&t3));
   GWHIP_CHECK(hipEventRecord(t0, stream_));
   GWHIP_CHECK(hipMemsetAsync(Sigma, 0, nsig * sizeof(double2), stream_));
+  stored_valid_ = false;
   stage_p0(G, split);
+  if (keep_p0_) {  // half-grid P0 of every q, for the stored-mode VJP
+    const size_t row = (size_t)(d_.nts / 2) * d_.NQ * d_.NQ * sizeof(double2);
+    GWHIP_CHECK(hipMemcpy2DAsync(P0keep_, row, P_, (size_t)d_.nts * d_.NQ * d_.NQ * sizeof(double2), row, (size_t)d_.nk,
+                                 hipMemcpyDeviceToDevice, stream_));
+  }
   GWHIP_CHECK(hipEventRecord(t1, stream_));
   stage_dyson();
   GWHIP_CHECK(hipEventRecord(t2, stream_));
@@ -557,6 +577,7 @@ void gw_hip_kernel::solve_device(const double2* G, double2* Sigma, int precision
   int flag = 0;
   GWHIP_CHECK(hipMemcpy(&flag, flag_, sizeof(int), hipMemcpyDeviceToHost));
   if (flag) throw status_error(GWHIP_ERR_NONFINITE, "Sigma contains non-finite values");
+  stored_valid_ = keep_p0_;
 }
 
 void gw_hip_kernel::solve_host(const double2* G, double2* Sigma, int precision) {
@@ -577,11 +598,18 @@ void gw_hip_kernel::run_stages_host(const double2* G, int last_stage) {
   const size_t nsig = (size_t)d_.nts * d_.ns * d_.nk * d_.nso * d_.nso;
   if (!G_dev_) G_dev_ = dalloc<double2>(nsig);
   GWHIP_CHECK(hipMemcpy(G_dev_, G, nsig * sizeof(double2), hipMemcpyHostToDevice));
+  stored_valid_ = false;
   GWHIP_CHECK(hipMemsetAsync(P_, 0, (size_t)d_.nk * d_.nts * d_.NQ * d_.NQ * sizeof(double2), stream_));
   stage_p0(G_dev_, false);
   if (last_stage >= 2) stage_dyson();
   GWHIP_CHECK(hipStreamSynchronize(stream_));
 }
+
+}  // namespace gwhip
+
+#include "gw_vjp_host.hpp"
+
+namespace gwhip {
 
 void debug_zgemm(int M, int N, int K, const double2* A, const double2* B, double2* C, int conjA, int conjB,
                  int shape) {
@@ -913,6 +941,23 @@ int gwhip_eval_sigma_device(gwhip_handle* h, const void* G, void* Sigma, int pre
   if (!h) return GWHIP_ERR_ARG;
   return guarded(h, [&] { h->k->solve_device((const double2*)G, (double2*)Sigma, precision); });
 }
+
+int gwhip_set_vjp_store(gwhip_handle* h, int keep) {
+  if (!h) return GWHIP_ERR_ARG;
+  return guarded(h, [&] { h->k->set_vjp_store(keep != 0); });
+}
+
+int gwhip_vjp(gwhip_handle* h, const double* G, const double* Sigma_bar, double* G_bar, int mode) {
+  if (!h) return GWHIP_ERR_ARG;
+  return guarded(h, [&] { h->k->vjp_host((const double2*)G, (const double2*)Sigma_bar, (double2*)G_bar, mode); });
+}
+
+int gwhip_vjp_device(gwhip_handle* h, const void* G, const void* Sigma_bar, void* G_bar, int mode) {
+  if (!h) return GWHIP_ERR_ARG;
+  return guarded(h, [&] { h->k->vjp_device((const double2*)G, (const double2*)Sigma_bar, (double2*)G_bar, mode); });
+}
+
+double gwhip_vjp_ms(const gwhip_handle* h) { return h ? h->k->vjp_ms() : 0.0; }
 
 int gwhip_get_P_tilde(gwhip_handle* h, double* P) {
   if (!h || !P) return GWHIP_ERR_ARG;

@@ -54,6 +54,11 @@ public:
   // test hook: stop after stage 1 (P0) or 2 (P-tilde); stage 3 runs Sigma as usual
   void run_stages_host(const double2* G, int last_stage);
   void copy_P_tilde(double2* host) const;
+  // vector-Jacobian product of Sigma[G]: G_bar for the cotangent Sigma_bar (device pointers)
+  void set_vjp_store(bool keep);
+  void vjp_device(const double2* G_dev, const double2* Sbar_dev, double2* Gbar_dev, int mode);
+  void vjp_host(const double2* G, const double2* Sbar, double2* Gbar, int mode);
+  double vjp_ms() const { return vjp_ms_; }
 
   const double* region_ms() const { return region_ms_; }
   const KernelStat& kstat(int which) const { return kstat_[which]; }
@@ -95,6 +100,15 @@ private:
   int8_t* Ps_ = nullptr;      // [tc][NQ][3][ns][kc_s]
   int* ePs_ = nullptr;
   int64_t qchunk_ = 1, tchunk_p0_ = 1, tchunk_sigma_ = 1;
+  // VJP: per-q buffers, a second orbital scratch, and the kept half-grid P0 of the last
+  // forward pass (stored mode)
+  double2 *vP0h_ = nullptr, *vP0w_ = nullptr, *vBinv_ = nullptr, *vAinv_ = nullptr;
+  double2 *vM1_ = nullptr, *vM2_ = nullptr, *vM3_ = nullptr, *vPt_ = nullptr, *vPtbar_ = nullptr, *vPbar_ = nullptr;
+  double2* vW_ = nullptr;      // [tc][nk][NQ][nao^2]
+  double2* P0keep_ = nullptr;  // [nq][nts/2][NQ^2]
+  int64_t tchunk_vjp_ = 1;
+  bool keep_p0_ = false, stored_valid_ = false;
+  double vjp_ms_ = 0.0;
   std::vector<int> kplus_host_, neg_q_;
 
   double region_ms_[R_COUNT] = {0};
@@ -107,7 +121,11 @@ private:
   void make_transposed_V();
   void ensure_slices();
   void free_slices();
+  void ensure_vjp_buffers();
+  void free_vjp_buffers();
   void stage_p0(const double2* G, bool split);
+  void p0_one_q(int64_t q, const double2* G, bool split, double2* Pq);
+  void gj_inverse(double2* X, int64_t nmat);
   void stage_dyson();
   void stage_sigma(const double2* G, double2* Sigma, bool split);
   template <typename F>

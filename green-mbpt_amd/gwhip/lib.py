@@ -19,6 +19,7 @@ HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.dirname(os.path.ab
 
 OK, ERR_ARG, ERR_WRONG_GRID, ERR_STATE, ERR_OOM, ERR_DEVICE, ERR_NONFINITE, ERR_UNSUPPORTED = range(8)
 PREC_FP64, PREC_SPLIT = 0, 1
+VJP_RECOMPUTE, VJP_STORED = 0, 1
 REGIONS = ("P0_zgemm", "P0(t) -> P0(w)", "GW-BSE", "P(w) -> P(t)", "Selfenergy_zgemm", "total")
 KSTATS = ("P0 GEMM", "Sigma GEMM", "P0 prep", "Sigma epilogue", "Dyson inverse")
 
@@ -77,8 +78,15 @@ def load() -> ctypes.CDLL:
     lib.gwhip_debug_zgemm_split.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, P, ctypes.c_int]
     lib.gwhip_debug_gemm_bench.argtypes = [ctypes.c_int] * 7 + [ctypes.POINTER(ctypes.c_double)]
     lib.gwhip_debug_mfma_peak.argtypes = [ctypes.c_int] * 3 + [ctypes.POINTER(ctypes.c_double)]
-    for f in ("gwhip_debug_zgemm_split", "gwhip_set_split_slices", "gwhip_set_kq_maps", "gwhip_debug_mfma_peak", "gwhip_debug_gemm_bench", "gwhip_debug_zgemm", "gwhip_debug_stage", "gwhip_set_grid", "gwhip_upload_V", "gwhip_generate_V", "gwhip_eval_sigma",
-              "gwhip_eval_sigma_device", "gwhip_get_P_tilde", "gwhip_timings", "gwhip_kernel_stats"):
+    lib.gwhip_set_vjp_store.argtypes = [P, ctypes.c_int]
+    lib.gwhip_vjp.argtypes = [P, P, P, P, ctypes.c_int]
+    lib.gwhip_vjp_device.argtypes = [P, P, P, P, ctypes.c_int]
+    lib.gwhip_vjp_ms.argtypes = [P]
+    lib.gwhip_vjp_ms.restype = ctypes.c_double
+    for f in ("gwhip_debug_zgemm_split", "gwhip_set_split_slices", "gwhip_set_kq_maps", "gwhip_debug_mfma_peak",
+              "gwhip_debug_gemm_bench", "gwhip_debug_zgemm", "gwhip_debug_stage", "gwhip_set_grid", "gwhip_upload_V",
+              "gwhip_generate_V", "gwhip_eval_sigma", "gwhip_eval_sigma_device", "gwhip_get_P_tilde", "gwhip_timings",
+              "gwhip_kernel_stats", "gwhip_set_vjp_store", "gwhip_vjp", "gwhip_vjp_device"):
         getattr(lib, f).restype = ctypes.c_int
     _lib = lib
     return lib
@@ -205,6 +213,25 @@ class GwHipKernel:
     def solve_device(self, G_ptr: int, S_ptr: int, precision: int = PREC_FP64):
         self._check(self.lib.gwhip_eval_sigma_device(self.h, ctypes.c_void_p(G_ptr), ctypes.c_void_p(S_ptr),
                                                      precision))
+
+    def set_vjp_store(self, keep: bool):
+        self._check(self.lib.gwhip_set_vjp_store(self.h, 1 if keep else 0))
+
+    def vjp(self, G: np.ndarray, Sbar: np.ndarray, mode: int = VJP_RECOMPUTE) -> np.ndarray:
+        """G_bar with Re<Sbar, dSigma> = Re<G_bar, dG> (host arrays)."""
+        g = np.ascontiguousarray(G, dtype=np.complex128)
+        sb = np.ascontiguousarray(Sbar, dtype=np.complex128)
+        assert g.shape == self.sigma_shape() and sb.shape == g.shape
+        out = np.empty_like(g)
+        self._check(self.lib.gwhip_vjp(self.h, _ptr(g), _ptr(sb), _ptr(out), mode))
+        return out
+
+    def vjp_device(self, G_ptr: int, Sbar_ptr: int, Gbar_ptr: int, mode: int = VJP_RECOMPUTE):
+        self._check(self.lib.gwhip_vjp_device(self.h, ctypes.c_void_p(G_ptr), ctypes.c_void_p(Sbar_ptr),
+                                              ctypes.c_void_p(Gbar_ptr), mode))
+
+    def vjp_ms(self) -> float:
+        return float(self.lib.gwhip_vjp_ms(self.h))
 
     def run_stages(self, G: np.ndarray, last_stage: int):
         g = np.ascontiguousarray(G, dtype=np.complex128)

@@ -45,6 +45,13 @@ enum gwhip_precision {
   GWHIP_PREC_SPLIT = 1       /* split-precision contractions, FP64 accumulation             */
 };
 
+/* Where the vector-Jacobian product takes P-tilde(tau, q) and P0(tau, q) from. */
+enum gwhip_vjp_mode {
+  GWHIP_VJP_RECOMPUTE = 0,   /* recompute them per q inside the VJP (no state kept)           */
+  GWHIP_VJP_STORED = 1       /* use those of the last gwhip_eval_sigma*, which must have run
+                                with gwhip_set_vjp_store(h, 1) on the same G                 */
+};
+
 /* Problem dimensions; names follow gw_cpu_kernel's members (src/green/mbpt/kernels.h:58-83).
  * Only the full k mesh without symmetry is supported (ink = nq = inq = nk). */
 typedef struct gwhip_dims {
@@ -110,6 +117,25 @@ int gwhip_eval_sigma(gwhip_handle* h, const double* G, double* Sigma, int precis
 /* As gwhip_eval_sigma, with G and Sigma already in device memory (same layout).
  * Synchronous with respect to the host. */
 int gwhip_eval_sigma_device(gwhip_handle* h, const void* G_dev, void* Sigma_dev, int precision);
+
+/* Keep (keep = 1) or stop keeping (0) the half-grid P0(tau, q) of every q after each
+ * evaluation, so that GWHIP_VJP_STORED can reuse P0 and P-tilde instead of recomputing them.
+ * Costs nk * nts/2 * NQ^2 complex of device memory while enabled.
+ * No reference counterpart (the reference has no derivative of Sigma). */
+int gwhip_set_vjp_store(gwhip_handle* h, int keep);
+
+/* Vector-Jacobian product of Sigma[G] (the map gw_cpu_kernel::solve computes,
+ * src/gw_cpu_kernel.cpp:31-373) at G for the cotangent Sigma_bar: writes G_bar with
+ * Re<Sigma_bar, dSigma> = Re<G_bar, dG> for every dG (<a,b> = sum conj(a) b; the convention of
+ * torch.autograd for complex tensors).  Host buffers, all [nts][ns][nk][nso][nso] complex.
+ * FP64 throughout.  `mode` is a gwhip_vjp_mode.  No reference counterpart. */
+int gwhip_vjp(gwhip_handle* h, const double* G, const double* Sigma_bar, double* G_bar, int mode);
+
+/* As gwhip_vjp with device buffers.  Synchronous with respect to the host. */
+int gwhip_vjp_device(gwhip_handle* h, const void* G_dev, const void* Sigma_bar_dev, void* G_bar_dev, int mode);
+
+/* Milliseconds of the last VJP (HIP events on the handle's stream). */
+double gwhip_vjp_ms(const gwhip_handle* h);
 
 /* Copy P-tilde(tau, q) of the last eval to the host: [nk][nts][NQ][NQ] complex.
  * Exposes the intermediate the reference keeps in P0_tilde_s after eval_P_tilde
