@@ -89,6 +89,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--vjp", action="store_true",
                     help="time the Sigma[G] vector-Jacobian product on the nk=27 nao=26 naux=250 system instead")
+    ap.add_argument("--batch", action="store_true",
+                    help="time the 8 uneven ns=2 systems evaluated as one captured graph instead")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
@@ -97,6 +99,8 @@ def main():
 
     if args.vjp:
         return bench_vjp(args)
+    if args.batch:
+        return bench_batch(args)
 
     import numpy as np
     import torch
@@ -260,6 +264,78 @@ def bench_vjp(args):
                    "nts": T.nts, "nw_b": T.nw_b, "parallelism": "single GPU"},
         "modes": out, "forward_ms": fwd_ms, "base_device_gb": base_bytes / 1e9,
         "fd_check": {"step": h, "fd": fd, "vjp": an, "rel_diff": abs(fd - an) / abs(an)},
+    }), flush=True)
+
+
+BATCH_SYSTEMS = [(8, 8, 60), (8, 14, 120), (27, 18, 160), (27, 26, 250), (27, 40, 400), (64, 12, 100),
+                 (64, 26, 300), (64, 32, 350)]
+
+
+def bench_batch(args):
+    """8 independent ns=2 systems (spin-split Fock +-0.05 Ha), beta 100, IR 1e4, one graph per eval."""
+    import numpy as np
+    import torch
+
+    from gwhip import inputs, irgrid
+    from gwhip import lib as gl
+    from oracle import gw_oracle as go
+
+    beta, lam = 100.0, 1e4
+    T = irgrid.Transformer.from_grid(irgrid.load_grid(lam), beta)
+    dims, Gs = [], []
+    for i, (nk, nao, NQ) in enumerate(BATCH_SYSTEMS):
+        n = round(nk ** (1 / 3))
+        dims.append(dict(nk=nk, nao=nao, nso=nao, ns=2, NQ=NQ, nts=T.nts, nw_b=T.nw_b, beta=beta, mesh=(n, n, n)))
+        Gs.append(inputs.generate_G(i, nk, nao, 2, T.tau, beta, spin_split=0.05))
+    b = gl.Batch(dims)
+    for i, s in enumerate(b.systems):
+        s.set_grid(T.T_bt, T.T_tb)
+        s.generate_V(i, 0.1)
+    dev = torch.device("cuda:0")
+    G_d = [torch.from_numpy(G.view(np.float64)).to(dev) for G in Gs]
+    S_d = [torch.empty_like(g) for g in G_d]
+    gp, sp = [g.data_ptr() for g in G_d], [x.data_ptr() for x in S_d]
+    torch.cuda.synchronize()
+    flops = sum(algorithmic_flops(nk, nao, NQ, T.nts, T.nw_b, pseudo_ns=2)["total"] for nk, nao, NQ in BATCH_SYSTEMS)
+    res, host = {}, {}
+    for prec, name in ((gl.PREC_SPLIT, "split"), (gl.PREC_FP64, "fp64")):
+        t0 = time.perf_counter()
+        b.autotune(gp, prec)
+        tune_s = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        for _ in range(max(args.warmup, 1)):  # the first call captures the graph
+            b.eval_device(gp, sp, prec)
+        first_s = (time.perf_counter() - t0) / max(args.warmup, 1)
+        print(f"[bench] batch {name} warm", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        for i in range(args.steps):
+            b.eval_device(gp, sp, prec)
+            print(f"[bench] batch {name} step {i} done", file=sys.stderr, flush=True)
+        ms = (time.perf_counter() - t0) / args.steps * 1e3
+        st = b.stats()
+        res[name] = {"ms_per_batch_eval": ms, "graph_launch_ms": st["last_ms"], "graph_nodes": st["graph_nodes"],
+                     "workspace_gb": st["workspace_bytes"] / 1e9, "autotune_s": tune_s, "first_call_s": first_s,
+                     "tuning": [s.tuning() for s in b.systems], "algorithmic_tflops": flops / (ms * 1e-3) / 1e12}
+        host[name] = [x.cpu().numpy().view(np.complex128).copy() for x in S_d]
+    per_sys = []
+    for i, (nk, nao, NQ) in enumerate(BATCH_SYSTEMS):
+        Sf, Ss = host["fp64"][i], host["split"][i]
+        Ef = go.gm_energy(Gs[i], Sf, T.T_ft, T.T_tf, nk, 2)
+        Es = go.gm_energy(Gs[i], Ss, T.T_ft, T.T_tf, nk, 2)
+        per_sys.append({"nk": nk, "nao": nao, "naux": NQ,
+                        "max_rel_sigma_err_split_vs_fp64": float(np.abs(Ss - Sf).max() / np.abs(Sf).max()),
+                        "gm_energy_fp64": Ef, "gm_energy_rel_err": abs(Es - Ef) / abs(Ef)})
+    b.close()
+    ms = res["split"]["ms_per_batch_eval"]
+    print(json.dumps({
+        "metric": "GW Sigma batch-evals/s, 8 uneven ns=2 systems in one captured graph (split precision)",
+        "value": 1e3 / ms, "unit": "batch-evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": ms, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "int8-slices/i32/f64", "data": "synthetic (seeds 0-7: spin-split Fock +-0.05 Ha, V ~ N(0, 0.1^2))",
+        "config": {"workload": "8 GW systems ns=2, (nk,nao,naux) " + str(BATCH_SYSTEMS) + ", beta 100, IR 1e4",
+                   "nts": T.nts, "nw_b": T.nw_b, "parallelism": "single GPU"},
+        "modes": res, "systems": per_sys, "algorithmic_flops_per_batch_eval": flops,
+        "speedup_split_vs_fp64": res["fp64"]["ms_per_batch_eval"] / ms,
     }), flush=True)
 
 

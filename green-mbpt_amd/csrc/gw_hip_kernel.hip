@@ -61,7 +61,7 @@ T* gw_hip_kernel::dalloc(size_t count) {
   return static_cast<T*>(p);
 }
 
-gw_hip_kernel::gw_hip_kernel(const Dims& d, int device) : d_(d), device_(device) {
+gw_hip_kernel::gw_hip_kernel(const Dims& d, int device, hipStream_t shared_stream) : d_(d), device_(device) {
   if (d_.nk <= 0 || d_.nao <= 0 || d_.nso <= 0 || d_.ns <= 0 || d_.NQ <= 0 || d_.nts <= 0 || d_.nw_b <= 0)
     throw status_error(GWHIP_ERR_ARG, "all dimensions must be positive");
   if (d_.mesh[0] * d_.mesh[1] * d_.mesh[2] != d_.nk) throw status_error(GWHIP_ERR_ARG, "mesh does not multiply to nk");
@@ -81,7 +81,13 @@ gw_hip_kernel::gw_hip_kernel(const Dims& d, int device) : d_(d), device_(device)
     throw status_error(GWHIP_ERR_DEVICE, "no HIP device " + std::to_string(device));
   }
   GWHIP_CHECK(hipSetDevice(device_));
-  GWHIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  if (shared_stream) {
+    stream_ = shared_stream;
+    own_stream_ = false;
+    own_ws_ = false;
+  } else {
+    GWHIP_CHECK(hipStreamCreateWithFlags(&stream_, hipStreamNonBlocking));
+  }
   max_lds_ = query_max_lds(device_);
 
   const int64_t nk = d_.nk, NQ = d_.NQ, nao2 = d_.nao * d_.nao, NQ2 = NQ * NQ;
@@ -94,6 +100,7 @@ gw_hip_kernel::gw_hip_kernel(const Dims& d, int device) : d_(d), device_(device)
   kminus_tab_ = dalloc<int>((size_t)nk * nk);
   flag_ = dalloc<int>(1);
   build_maps();
+  if (!own_ws_) return;  // a batch binds a shared workspace later
 
   // chunk sizes for the transient buffers from the memory left on the device
   size_t free_b = 0, total_b = 0;
@@ -116,12 +123,15 @@ gw_hip_kernel::gw_hip_kernel(const Dims& d, int device) : d_(d), device_(device)
 gw_hip_kernel::~gw_hip_kernel() {
   (void)hipSetDevice(device_);
   if (stream_) (void)hipStreamSynchronize(stream_);
-  for (void* p : {(void*)Vq_, (void*)Vt_, (void*)P_, (void*)Pw_, (void*)X_, (void*)Y_, (void*)scratch_, (void*)Tbt_fold_,
-                  (void*)Ttb_, (void*)G_dev_, (void*)S_dev_, (void*)kplus_tab_, (void*)kminus_tab_, (void*)flag_})
+  for (void* p : {(void*)Vq_, (void*)Vt_, (void*)P_, (void*)Tbt_fold_, (void*)Ttb_, (void*)G_dev_, (void*)S_dev_,
+                  (void*)kplus_tab_, (void*)kminus_tab_, (void*)flag_})
     if (p) (void)hipFree(p);
+  if (own_ws_)
+    for (void* p : {(void*)Pw_, (void*)X_, (void*)Y_, (void*)scratch_})
+      if (p) (void)hipFree(p);
   free_slices();
   free_vjp_buffers();
-  if (stream_) (void)hipStreamDestroy(stream_);
+  if (stream_ && own_stream_) (void)hipStreamDestroy(stream_);
 }
 
 void gw_hip_kernel::build_maps() {
@@ -212,12 +222,19 @@ void gw_hip_kernel::upload_V(const double2* V_packed) {
 }
 
 void gw_hip_kernel::free_slices() {
-  for (void* p : {(void*)Vs_p0_, (void*)eVs_p0_, (void*)Vs_s_, (void*)eVs_s_, (void*)Zs_, (void*)eZs_, (void*)Ps_,
-                  (void*)ePs_})
+  for (void* p : {(void*)Vs_p0_, (void*)eVs_p0_, (void*)Vs_s_, (void*)eVs_s_})
     if (p) (void)hipFree(p);
-  Vs_p0_ = Vs_s_ = Zs_ = Ps_ = nullptr;
-  eVs_p0_ = eVs_s_ = eZs_ = ePs_ = nullptr;
+  if (own_ws_)
+    for (void* p : {(void*)Zs_, (void*)eZs_})
+      if (p) (void)hipFree(p);
+  Vs_p0_ = Vs_s_ = nullptr;
+  eVs_p0_ = eVs_s_ = nullptr;
+  if (own_ws_) {
+    Zs_ = Ps_ = nullptr;
+    eZs_ = ePs_ = nullptr;
+  }
   slices_ns_ = 0;
+  ++generation_;
 }
 
 void gw_hip_kernel::set_split_slices(int ns) {
@@ -228,6 +245,50 @@ void gw_hip_kernel::set_split_slices(int ns) {
     free_slices();
   }
   split_ns_ = ns;
+}
+
+size_t gw_hip_kernel::scratch_per_t() const { return (size_t)d_.nk * d_.NQ * d_.nao * d_.nao; }
+size_t gw_hip_kernel::dyson_per_q() const { return (size_t)3 * d_.nw_b * d_.NQ * d_.NQ; }
+size_t gw_hip_kernel::slice_bytes_per_t() const {
+  const size_t kcp = (size_t)ceil_div(d_.nao * d_.nao, 64) * 64, kcs = (size_t)ceil_div(d_.NQ, 64) * 64;
+  return std::max((size_t)d_.NQ * d_.nk * 3 * split_ns_ * kcp, (size_t)d_.NQ * 3 * split_ns_ * kcs);
+}
+size_t gw_hip_kernel::exps_per_t() const { return (size_t)d_.NQ * d_.nk; }
+
+void gw_hip_kernel::bind_workspace(const Workspace& ws) {
+  if (own_ws_) throw status_error(GWHIP_ERR_STATE, "this kernel owns its workspace");
+  int64_t tmax = (int64_t)(ws.scratch_elems / scratch_per_t());
+  const int64_t qmax = (int64_t)(ws.dyson_elems / dyson_per_q());
+  if (ws.slices) {
+    tmax = std::min<int64_t>(tmax, (int64_t)(ws.slice_bytes / slice_bytes_per_t()));
+    tmax = std::min<int64_t>(tmax, (int64_t)(ws.exp_elems / exps_per_t()));
+  }
+  if (tmax < 1 || qmax < 1) throw status_error(GWHIP_ERR_OOM, "shared workspace too small for this system");
+  tchunk_sigma_ = std::min<int64_t>(d_.nts, tmax);
+  tchunk_p0_ = std::min<int64_t>(d_.nts / 2, tmax);
+  qchunk_ = std::min<int64_t>(d_.nk, qmax);
+  const size_t nmat = (size_t)qchunk_ * d_.nw_b * d_.NQ * d_.NQ;
+  scratch_ = ws.scratch;
+  Pw_ = ws.dyson;
+  X_ = Pw_ + nmat;
+  Y_ = X_ + nmat;
+  Zs_ = Ps_ = ws.slices;
+  eZs_ = ePs_ = ws.exps;
+  ++generation_;
+}
+
+void gw_hip_kernel::check_ready(int precision) const {
+  if (!grid_set_) throw status_error(GWHIP_ERR_STATE, "transform grid not set (gwhip_set_grid)");
+  if (!v_set_) throw status_error(GWHIP_ERR_STATE, "integrals not set (gwhip_upload_V / gwhip_generate_V)");
+  if (precision != GWHIP_PREC_FP64 && precision != GWHIP_PREC_SPLIT) throw status_error(GWHIP_ERR_ARG, "unknown precision");
+  if (!scratch_) throw status_error(GWHIP_ERR_STATE, "no workspace bound");
+  if (precision == GWHIP_PREC_SPLIT && !own_ws_ && (!split_ready() || !Zs_))
+    throw status_error(GWHIP_ERR_STATE, "split-precision stores or workspace not prepared");
+}
+
+void gw_hip_kernel::prepare_split() {
+  GWHIP_CHECK(hipSetDevice(device_));
+  ensure_slices();
 }
 
 // Slice stores of conj(V) (built once per V and slice count).
@@ -242,10 +303,11 @@ void gw_hip_kernel::ensure_slices() {
   eVs_p0_ = dalloc<int>((size_t)nk * NQ * nk);
   Vs_s_ = dalloc<int8_t>((size_t)nk * nk * nao2 * 2 * ns * kc_s_);
   eVs_s_ = dalloc<int>((size_t)nk * nk * nao2);
-  Zs_ = dalloc<int8_t>((size_t)tchunk_p0_ * NQ * nk * 3 * ns * kc_p0_);
-  eZs_ = dalloc<int>((size_t)tchunk_p0_ * NQ * nk);
-  Ps_ = dalloc<int8_t>((size_t)tchunk_sigma_ * NQ * 3 * ns * kc_s_);
-  ePs_ = dalloc<int>((size_t)tchunk_sigma_ * NQ);
+  if (own_ws_) {  // Z slices (P0 stage) and P-tilde slices (Sigma stage) share one buffer
+    Zs_ = Ps_ = dalloc<int8_t>(std::max((size_t)tchunk_p0_ * NQ * nk * 3 * ns * kc_p0_,
+                                        (size_t)tchunk_sigma_ * NQ * 3 * ns * kc_s_));
+    eZs_ = ePs_ = dalloc<int>(std::max((size_t)tchunk_p0_ * NQ * nk, (size_t)tchunk_sigma_ * NQ));
+  }
   {  // P0 operand: rows (q, Q'), chunks k, elements (t, n) of V(k, k+q)
     SliceArgs a{};
     a.src = Vq_; a.nrows = nk * NQ; a.row_div = NQ; a.s_rhi = nk * NQ * nao2; a.s_rlo = nao2;
@@ -262,6 +324,7 @@ void gw_hip_kernel::ensure_slices() {
   }
   GWHIP_CHECK(hipStreamSynchronize(stream_));
   slices_ns_ = ns;
+  ++generation_;
 }
 
 void gw_hip_kernel::make_transposed_V() {
@@ -304,6 +367,10 @@ void gw_hip_kernel::generate_V(uint64_t seed, double sigma) {
 
 template <typename F>
 void gw_hip_kernel::timed(int which, F&& launch) {
+  if (!timing_) {  // capture-safe: no events, no host synchronisation
+    launch();
+    return;
+  }
   hipEvent_t a, b;
   GWHIP_CHECK(hipEventCreate(&a));
   GWHIP_CHECK(hipEventCreate(&b));
@@ -348,6 +415,53 @@ void launch_sigma_epi(dim3 grid, size_t lds, hipStream_t st, const DevDims& dd, 
 }
 }  // namespace
 
+void gw_hip_kernel::zero(double2* X, int64_t n) {
+  hipLaunchKernelGGL(zero_kernel, dim3((unsigned)std::min<int64_t>(4096, std::max<int64_t>(1, ceil_div(n, 256)))),
+                     dim3(256), 0, stream_, X, n);
+  GWHIP_CHECK(hipGetLastError());
+}
+
+// Z_Q = sum_s Gb_s V_Q Gq_s for q and tau in [t0, t0 + tc) into scratch_.
+void gw_hip_kernel::p0_prep(int64_t q, const double2* G, int64_t t0, int64_t tc, int qpw) {
+  const int64_t nk = d_.nk, NQ = d_.NQ, nao2 = d_.nao * d_.nao;
+  DevDims dd{(int)d_.nts, (int)d_.ns, (int)nk, (int)d_.nso, (int)d_.nao, (int)NQ, (int)d_.pns, (int)d_.pseudo_ns};
+  const int NB = (int)ceil_div(d_.nao, 16);
+  const size_t lds = (size_t)4 * (16 * NB) * (16 * NB + 1) * sizeof(double);
+  const double2* Vtq = Vt_ + q * nk * NQ * nao2;
+  const dim3 grid((unsigned)ceil_div(NQ, 4 * qpw), (unsigned)nk, (unsigned)tc);
+  const int* kp = kplus_tab_ + q * nk;
+  if (NB == 1) launch_z_prep<1>(grid, lds, stream_, dd, G, Vtq, scratch_, kp, (int)t0, qpw);
+  else if (NB == 2) launch_z_prep<2>(grid, lds, stream_, dd, G, Vtq, scratch_, kp, (int)t0, qpw);
+  else launch_z_prep<3>(grid, lds, stream_, dd, G, Vtq, scratch_, kp, (int)t0, qpw);
+}
+
+// P0(t0 .. t0+tc) = -(pref/nk) Z conj(V)^T, FP64 (tile `shape`).
+void gw_hip_kernel::p0_gemm(int64_t q, int64_t tc, double2* Pt0, int shape) {
+  const int64_t nk = d_.nk, NQ = d_.NQ, nao2 = d_.nao * d_.nao, NQ2 = NQ * NQ;
+  ZGemmArgs a{};
+  a.M = (int)NQ; a.N = (int)NQ; a.K = (int)(nk * nao2); a.batch = (int)tc;
+  a.kin = (int)nao2; a.nin = (int)NQ;
+  a.A = scratch_; a.sAb = nk * NQ * nao2; a.sAm = nao2; a.sAko = NQ * nao2; a.sAki = 1;
+  a.B = Vq_ + q * nk * NQ * nao2; a.sBb = 0; a.sBko = NQ * nao2; a.sBki = 1; a.sBno = 0; a.sBni = nao2;
+  a.C = Pt0; a.sCb = NQ2; a.sCm = NQ; a.sCno = 0; a.sCni = 1;
+  a.alpha = make_double2(-d_.prefactor / (double)nk, 0.0);
+  a.accumulate = 0; a.conjA = 0; a.conjB = 1;
+  zgemm_f64(a, true, true, stream_, shape);
+}
+
+// R = P-tilde(t0 .. t0+tc) conj(V(k, k - q)) into scratch_, FP64 (tile `shape`).
+void gw_hip_kernel::sigma_gemm(int64_t q, int64_t t0, int64_t tc, int shape) {
+  const int64_t nk = d_.nk, NQ = d_.NQ, nao2 = d_.nao * d_.nao, NQ2 = NQ * NQ, nts = d_.nts;
+  ZGemmArgs a{};
+  a.M = (int)NQ; a.N = (int)(nk * nao2); a.K = (int)NQ; a.batch = (int)tc; a.kin = (int)NQ; a.nin = (int)nao2;
+  a.A = P_ + (q * nts + t0) * NQ2; a.sAb = NQ2; a.sAm = NQ; a.sAko = 0; a.sAki = 1;
+  a.B = Vt_ + (int64_t)neg_q_[q] * nk * NQ * nao2; a.sBb = 0; a.sBko = 0; a.sBki = nao2; a.sBno = NQ * nao2; a.sBni = 1;
+  a.C = scratch_; a.sCb = nk * NQ * nao2; a.sCm = nao2; a.sCno = NQ * nao2; a.sCni = 1;
+  a.alpha = make_double2(1.0, 0.0);
+  a.accumulate = 0; a.conjA = 0; a.conjB = 1;
+  zgemm_f64(a, true, false, stream_, shape);
+}
+
 void gw_hip_kernel::stage_p0(const double2* G, bool split) {
   for (int64_t q = 0; q < d_.nk; ++q) p0_one_q(q, G, split, P_ + q * d_.nts * d_.NQ * d_.NQ);
 }
@@ -355,58 +469,35 @@ void gw_hip_kernel::stage_p0(const double2* G, bool split) {
 // P0(tau, q) for the half tau grid into Pq[t][Q][Q'] (t < nts/2), Hermitised.
 void gw_hip_kernel::p0_one_q(int64_t q, const double2* G, bool split, double2* Pq) {
   const int64_t nk = d_.nk, NQ = d_.NQ, nao2 = d_.nao * d_.nao, NQ2 = NQ * NQ, half = d_.nts / 2;
-  DevDims dd{(int)d_.nts, (int)d_.ns, (int)nk, (int)d_.nso, (int)d_.nao, (int)NQ, (int)d_.pns, (int)d_.pseudo_ns};
-  const int NB = (int)ceil_div(d_.nao, 16);
-  const size_t lds = (size_t)4 * (16 * NB) * (16 * NB + 1) * sizeof(double);
-  const int QPW = 4;
-  const unsigned gx = (unsigned)ceil_div(NQ, 4 * QPW);
-  {
-    const double2* Vq = Vq_ + q * nk * NQ * nao2;
-    const double2* Vtq = Vt_ + q * nk * NQ * nao2;
-    for (int64_t t0 = 0; t0 < half; t0 += tchunk_p0_) {
-      const int64_t tc = std::min(tchunk_p0_, half - t0);
-      timed(K_P0_PREP, [&] {
-        const dim3 grid(gx, (unsigned)nk, (unsigned)tc);
-        const int* kp = kplus_tab_ + q * nk;
-        if (NB == 1) launch_z_prep<1>(grid, lds, stream_, dd, G, Vtq, scratch_, kp, (int)t0, QPW);
-        else if (NB == 2) launch_z_prep<2>(grid, lds, stream_, dd, G, Vtq, scratch_, kp, (int)t0, QPW);
-        else launch_z_prep<3>(grid, lds, stream_, dd, G, Vtq, scratch_, kp, (int)t0, QPW);
-      });
-      if (split) {
-        const int ns = split_ns_;
-        SliceArgs sa{};
-        sa.src = scratch_; sa.nrows = tc * NQ; sa.row_div = NQ; sa.s_rhi = nk * NQ * nao2; sa.s_rlo = nao2;
-        sa.nchunk = (int)nk; sa.s_c = NQ * nao2; sa.klen = (int)nao2; sa.s_k = 1; sa.kc = (int)kc_p0_;
-        sa.conj = 0; sa.nplanes = 3; sa.dst = Zs_; sa.exps = eZs_;
-        slice_rows(sa, ns, stream_);
-        OzGemmArgs o{};
-        o.M = (int)NQ; o.N = (int)NQ; o.batch = (int)tc; o.kc = (int)kc_p0_; o.nchunk = (int)nk; o.nin = (int)NQ;
-        o.A = Zs_; o.sAm = nk * 3 * ns * kc_p0_; o.sAb = NQ * o.sAm; o.sAc = 3 * ns * kc_p0_;
-        o.eA = eZs_; o.seAb = NQ * nk; o.seAm = nk;
-        o.B = Vs_p0_ + q * NQ * nk * 2 * ns * kc_p0_; o.sBb = 0; o.sBno = 0; o.sBni = nk * 2 * ns * kc_p0_;
-        o.sBc = 2 * ns * kc_p0_;
-        o.eB = eVs_p0_ + q * NQ * nk; o.seBb = 0; o.seBno = 0; o.seBni = nk;
-        o.C = Pq + t0 * NQ2; o.sCb = NQ2; o.sCm = NQ; o.sCno = 0; o.sCni = 1;
-        o.alpha = make_double2(-d_.prefactor / (double)nk, 0.0);
-        o.accumulate = 0;
-        timed(K_P0_GEMM, [&] { zgemm_ozaki(o, ns, stream_); });
-        continue;
-      }
-      ZGemmArgs a{};
-      a.M = (int)NQ; a.N = (int)NQ; a.K = (int)(nk * nao2); a.batch = (int)tc;
-      a.kin = (int)nao2; a.nin = (int)NQ;
-      a.A = scratch_; a.sAb = nk * NQ * nao2; a.sAm = nao2; a.sAko = NQ * nao2; a.sAki = 1;
-      a.B = Vq; a.sBb = 0; a.sBko = NQ * nao2; a.sBki = 1; a.sBno = 0; a.sBni = nao2;
-      a.C = Pq + t0 * NQ2; a.sCb = NQ2; a.sCm = NQ; a.sCno = 0; a.sCni = 1;
-      a.alpha = make_double2(-d_.prefactor / (double)nk, 0.0);
-      a.accumulate = 0; a.conjA = 0; a.conjB = 1;
-      timed(K_P0_GEMM, [&] { zgemm_f64(a, true, true, stream_); });
+  for (int64_t t0 = 0; t0 < half; t0 += tchunk_p0_) {
+    const int64_t tc = std::min(tchunk_p0_, half - t0);
+    timed(K_P0_PREP, [&] { p0_prep(q, G, t0, tc, tune_.qpw); });
+    if (split) {
+      const int ns = split_ns_;
+      SliceArgs sa{};
+      sa.src = scratch_; sa.nrows = tc * NQ; sa.row_div = NQ; sa.s_rhi = nk * NQ * nao2; sa.s_rlo = nao2;
+      sa.nchunk = (int)nk; sa.s_c = NQ * nao2; sa.klen = (int)nao2; sa.s_k = 1; sa.kc = (int)kc_p0_;
+      sa.conj = 0; sa.nplanes = 3; sa.dst = Zs_; sa.exps = eZs_;
+      slice_rows(sa, ns, stream_);
+      OzGemmArgs o{};
+      o.M = (int)NQ; o.N = (int)NQ; o.batch = (int)tc; o.kc = (int)kc_p0_; o.nchunk = (int)nk; o.nin = (int)NQ;
+      o.A = Zs_; o.sAm = nk * 3 * ns * kc_p0_; o.sAb = NQ * o.sAm; o.sAc = 3 * ns * kc_p0_;
+      o.eA = eZs_; o.seAb = NQ * nk; o.seAm = nk;
+      o.B = Vs_p0_ + q * NQ * nk * 2 * ns * kc_p0_; o.sBb = 0; o.sBno = 0; o.sBni = nk * 2 * ns * kc_p0_;
+      o.sBc = 2 * ns * kc_p0_;
+      o.eB = eVs_p0_ + q * NQ * nk; o.seBb = 0; o.seBno = 0; o.seBni = nk;
+      o.C = Pq + t0 * NQ2; o.sCb = NQ2; o.sCm = NQ; o.sCno = 0; o.sCni = 1;
+      o.alpha = make_double2(-d_.prefactor / (double)nk, 0.0);
+      o.accumulate = 0;
+      timed(K_P0_GEMM, [&] { zgemm_ozaki(o, ns, stream_); });
+    } else {
+      timed(K_P0_GEMM, [&] { p0_gemm(q, tc, Pq + t0 * NQ2, tune_.p0_shape); });
     }
-    // Hermitise the half grid of this q (src/gw_cpu_kernel.cpp:217-225)
-    hipLaunchKernelGGL(hermitize_kernel, dim3((unsigned)ceil_div(NQ2, 256), (unsigned)half), dim3(256), 0, stream_,
-                       Pq, (int)NQ, NQ2);
-    GWHIP_CHECK(hipGetLastError());
   }
+  // Hermitise the half grid of this q (src/gw_cpu_kernel.cpp:217-225)
+  hipLaunchKernelGGL(hermitize_kernel, dim3((unsigned)ceil_div(NQ2, 256), (unsigned)half), dim3(256), 0, stream_, Pq,
+                     (int)NQ, NQ2);
+  GWHIP_CHECK(hipGetLastError());
 }
 
 // In-place inverse of nmat Hermitian positive definite NQ x NQ matrices (I - P0(i Omega)).
@@ -428,13 +519,16 @@ void gw_hip_kernel::gj_inverse(double2* X, int64_t nmat) {
 void gw_hip_kernel::stage_dyson() {
   const int64_t nk = d_.nk, NQ = d_.NQ, NQ2 = NQ * NQ, half = d_.nts / 2, nwb = d_.nw_b, nts = d_.nts;
 
-  hipEvent_t e0, e1, e2, e3;
-  GWHIP_CHECK(hipEventCreate(&e0)); GWHIP_CHECK(hipEventCreate(&e1));
-  GWHIP_CHECK(hipEventCreate(&e2)); GWHIP_CHECK(hipEventCreate(&e3));
+  hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
+  if (timing_) {
+    GWHIP_CHECK(hipEventCreate(&e0)); GWHIP_CHECK(hipEventCreate(&e1));
+    GWHIP_CHECK(hipEventCreate(&e2)); GWHIP_CHECK(hipEventCreate(&e3));
+  }
+  auto mark = [&](hipEvent_t e) { if (timing_) GWHIP_CHECK(hipEventRecord(e, stream_)); };
   float tw = 0.f, bse = 0.f, wt = 0.f;
   for (int64_t q0 = 0; q0 < nk; q0 += qchunk_) {
     const int64_t qc = std::min(qchunk_, nk - q0);
-    GWHIP_CHECK(hipEventRecord(e0, stream_));
+    mark(e0);
     {  // P0(tau_f) -> P0(i Omega_b) with the folded matrix
       ZGemmArgs a{};
       a.M = (int)nwb; a.N = (int)NQ2; a.K = (int)half; a.batch = (int)qc; a.kin = (int)half; a.nin = (int)NQ2;
@@ -444,7 +538,7 @@ void gw_hip_kernel::stage_dyson() {
       a.alpha = make_double2(1.0, 0.0);
       zgemm_f64(a, true, false, stream_);
     }
-    GWHIP_CHECK(hipEventRecord(e1, stream_));
+    mark(e1);
     const int64_t nmat = qc * nwb;
     hipLaunchKernelGGL(identity_minus_kernel, dim3((unsigned)ceil_div(nmat * NQ2, 256)), dim3(256), 0, stream_, X_, Pw_,
                        (int)NQ, nmat);
@@ -462,7 +556,7 @@ void gw_hip_kernel::stage_dyson() {
       a.alpha = make_double2(1.0, 0.0);
       zgemm_f64(a, true, false, stream_);
     }
-    GWHIP_CHECK(hipEventRecord(e2, stream_));
+    mark(e2);
     {  // P-tilde(i Omega_b) -> P-tilde(tau_f), all nts
       ZGemmArgs a{};
       a.M = (int)nts; a.N = (int)NQ2; a.K = (int)nwb; a.batch = (int)qc; a.kin = (int)nwb; a.nin = (int)NQ2;
@@ -472,13 +566,16 @@ void gw_hip_kernel::stage_dyson() {
       a.alpha = make_double2(1.0, 0.0);
       zgemm_f64(a, true, false, stream_);
     }
-    GWHIP_CHECK(hipEventRecord(e3, stream_));
-    GWHIP_CHECK(hipEventSynchronize(e3));
-    float x;
-    GWHIP_CHECK(hipEventElapsedTime(&x, e0, e1)); tw += x;
-    GWHIP_CHECK(hipEventElapsedTime(&x, e1, e2)); bse += x;
-    GWHIP_CHECK(hipEventElapsedTime(&x, e2, e3)); wt += x;
+    mark(e3);
+    if (timing_) {
+      GWHIP_CHECK(hipEventSynchronize(e3));
+      float x;
+      GWHIP_CHECK(hipEventElapsedTime(&x, e0, e1)); tw += x;
+      GWHIP_CHECK(hipEventElapsedTime(&x, e1, e2)); bse += x;
+      GWHIP_CHECK(hipEventElapsedTime(&x, e2, e3)); wt += x;
+    }
   }
+  if (!timing_) return;
   region_ms_[R_TW] += tw;
   region_ms_[R_BSE] += bse;
   region_ms_[R_WT] += wt;
@@ -516,14 +613,7 @@ void gw_hip_kernel::stage_sigma(const double2* G, double2* Sigma, bool split) {
         o.accumulate = 0;
         timed(K_SIGMA_GEMM, [&] { zgemm_ozaki(o, ns, stream_); });
       } else {
-      ZGemmArgs a{};
-      a.M = (int)NQ; a.N = (int)(nk * nao2); a.K = (int)NQ; a.batch = (int)tc; a.kin = (int)NQ; a.nin = (int)nao2;
-      a.A = P_ + (q * nts + t0) * NQ2; a.sAb = NQ2; a.sAm = NQ; a.sAko = 0; a.sAki = 1;
-      a.B = Vneg; a.sBb = 0; a.sBko = 0; a.sBki = nao2; a.sBno = NQ * nao2; a.sBni = 1;
-      a.C = scratch_; a.sCb = nk * NQ * nao2; a.sCm = nao2; a.sCno = NQ * nao2; a.sCni = 1;
-      a.alpha = make_double2(1.0, 0.0);
-      a.accumulate = 0; a.conjA = 0; a.conjB = 1;
-      timed(K_SIGMA_GEMM, [&] { zgemm_f64(a, true, false, stream_); });
+        timed(K_SIGMA_GEMM, [&] { sigma_gemm(q, t0, tc, tune_.s_shape); });
       }
       timed(K_SIGMA_EPI, [&] {
         const dim3 grid((unsigned)nk, (unsigned)tc);
@@ -536,23 +626,35 @@ void gw_hip_kernel::stage_sigma(const double2* G, double2* Sigma, bool split) {
   }
 }
 
+// One evaluation on the stream, kernel nodes only (Sigma zeroed by a fill kernel).
+void gw_hip_kernel::enqueue_eval(const double2* G, double2* Sigma, bool split, int* nonfinite_flag) {
+  const int64_t nsig = d_.nts * d_.ns * d_.nk * d_.nso * d_.nso;
+  zero(Sigma, nsig);
+  stage_p0(G, split);
+  stage_dyson();
+  stage_sigma(G, Sigma, split);
+  if (nonfinite_flag) {
+    hipLaunchKernelGGL(nonfinite_kernel, dim3(1024), dim3(256), 0, stream_, Sigma, nsig, nonfinite_flag);
+    GWHIP_CHECK(hipGetLastError());
+  }
+}
+
 void gw_hip_kernel::solve_device(const double2* G, double2* Sigma, int precision) {
-  if (!grid_set_) throw status_error(GWHIP_ERR_STATE, "transform grid not set (gwhip_set_grid)");
-  if (!v_set_) throw status_error(GWHIP_ERR_STATE, "integrals not set (gwhip_upload_V / gwhip_generate_V)");
+  check_ready(precision);
   if (!G || !Sigma) throw status_error(GWHIP_ERR_ARG, "null G or Sigma");
-  if (precision != GWHIP_PREC_FP64 && precision != GWHIP_PREC_SPLIT) throw status_error(GWHIP_ERR_ARG, "unknown precision");
   const bool split = precision == GWHIP_PREC_SPLIT;
   GWHIP_CHECK(hipSetDevice(device_));
   if (split) ensure_slices();
+  timing_ = true;
   for (double& r : region_ms_) r = 0.0;
   for (auto& k : kstat_) k = KernelStat{};
-  const size_t nsig = (size_t)d_.nts * d_.ns * d_.nk * d_.nso * d_.nso;
+  const int64_t nsig = d_.nts * d_.ns * d_.nk * d_.nso * d_.nso;
 
   hipEvent_t t0, t1, t2, t3;
   GWHIP_CHECK(hipEventCreate(&t0)); GWHIP_CHECK(hipEventCreate(&t1));
   GWHIP_CHECK(hipEventCreate(&t2)); GWHIP_CHECK(hipEventCreate(&t3));
   GWHIP_CHECK(hipEventRecord(t0, stream_));
-  GWHIP_CHECK(hipMemsetAsync(Sigma, 0, nsig * sizeof(double2), stream_));
+  zero(Sigma, nsig);
   stored_valid_ = false;
   stage_p0(G, split);
   if (keep_p0_) {  // half-grid P0 of every q, for the stored-mode VJP
@@ -565,8 +667,8 @@ void gw_hip_kernel::solve_device(const double2* G, double2* Sigma, int precision
   GWHIP_CHECK(hipEventRecord(t2, stream_));
   stage_sigma(G, Sigma, split);
   GWHIP_CHECK(hipEventRecord(t3, stream_));
-  GWHIP_CHECK(hipMemsetAsync(flag_, 0, sizeof(int), stream_));
-  hipLaunchKernelGGL(nonfinite_kernel, dim3(1024), dim3(256), 0, stream_, Sigma, (int64_t)nsig, flag_);
+  hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, stream_, flag_, (int64_t)1);
+  hipLaunchKernelGGL(nonfinite_kernel, dim3(1024), dim3(256), 0, stream_, Sigma, nsig, flag_);
   GWHIP_CHECK(hipGetLastError());
   GWHIP_CHECK(hipStreamSynchronize(stream_));
   float x;
@@ -578,6 +680,55 @@ void gw_hip_kernel::solve_device(const double2* G, double2* Sigma, int precision
   GWHIP_CHECK(hipMemcpy(&flag, flag_, sizeof(int), hipMemcpyDeviceToHost));
   if (flag) throw status_error(GWHIP_ERR_NONFINITE, "Sigma contains non-finite values");
   stored_valid_ = keep_p0_;
+}
+
+// Time the launch-parameter candidates on this system's own operands (q = 0, first tau chunk)
+// and keep the fastest.  Overwrites P0/P-tilde of q = 0 and the scratch (no results kept).
+void gw_hip_kernel::autotune(const double2* G, int precision) {
+  check_ready(precision);
+  if (!G) throw status_error(GWHIP_ERR_ARG, "null G");
+  GWHIP_CHECK(hipSetDevice(device_));
+  const bool split = precision == GWHIP_PREC_SPLIT;
+  if (split) ensure_slices();
+  const int64_t half = d_.nts / 2, tcp = std::min(tchunk_p0_, half), tcs = std::min(tchunk_sigma_, d_.nts);
+  hipEvent_t a, b;
+  GWHIP_CHECK(hipEventCreate(&a));
+  GWHIP_CHECK(hipEventCreate(&b));
+  auto best_of = [&](int ncand, const int* cand, auto&& run) {
+    run(cand[0]);  // warm-up (module load, LDS attribute)
+    GWHIP_CHECK(hipStreamSynchronize(stream_));
+    int best = cand[0];
+    float best_ms = 1e30f;
+    for (int i = 0; i < ncand; ++i) {
+      float tot = 0.f;
+      for (int r = 0; r < 2; ++r) {
+        GWHIP_CHECK(hipEventRecord(a, stream_));
+        run(cand[i]);
+        GWHIP_CHECK(hipEventRecord(b, stream_));
+        GWHIP_CHECK(hipEventSynchronize(b));
+        float ms = 0.f;
+        GWHIP_CHECK(hipEventElapsedTime(&ms, a, b));
+        tot += ms;
+      }
+      if (tot < best_ms) {
+        best_ms = tot;
+        best = cand[i];
+      }
+    }
+    return best;
+  };
+  const int qpws[4] = {1, 2, 4, 8};
+  tune_.qpw = best_of(4, qpws, [&](int c) { p0_prep(0, G, 0, tcp, c); });
+  if (!split) {
+    const int shapes[3] = {0, 1, 2};
+    p0_prep(0, G, 0, tcp, tune_.qpw);
+    tune_.p0_shape = best_of(3, shapes, [&](int c) { p0_gemm(0, tcp, P_, c); });
+    tune_.s_shape = best_of(3, shapes, [&](int c) { sigma_gemm(0, 0, tcs, c); });
+  }
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  stored_valid_ = false;
+  ++generation_;
 }
 
 void gw_hip_kernel::solve_host(const double2* G, double2* Sigma, int precision) {
@@ -608,6 +759,7 @@ void gw_hip_kernel::run_stages_host(const double2* G, int last_stage) {
 }  // namespace gwhip
 
 #include "gw_vjp_host.hpp"
+#include "gw_batch.hpp"
 
 namespace gwhip {
 
@@ -856,12 +1008,29 @@ void gw_hip_kernel::copy_P_tilde(double2* host) const {
 
 // ------------------------------------------------------------------------------ C ABI
 struct gwhip_handle {
-  std::unique_ptr<gwhip::gw_hip_kernel> k;
+  std::unique_ptr<gwhip::gw_hip_kernel> owned;
+  gwhip::gw_hip_kernel* k = nullptr;
+  bool borrowed = false;  // a system of a batch: the batch owns it
+  std::string err;
+};
+
+struct gwhip_batch {
+  std::unique_ptr<gwhip::gw_hip_batch> b;
+  std::vector<std::unique_ptr<gwhip_handle>> systems;
   std::string err;
 };
 
 namespace {
 thread_local std::string g_create_error;
+
+gwhip::Dims to_dims(const gwhip_dims* dims) {
+  gwhip::Dims d;
+  d.nk = dims->nk; d.nao = dims->nao; d.nso = dims->nso; d.ns = dims->ns; d.NQ = dims->NQ;
+  d.nts = dims->nts; d.nw_b = dims->nw_b; d.beta = dims->beta;
+  d.mesh = {dims->mesh[0], dims->mesh[1], dims->mesh[2]};
+  return d;
+}
+
 
 template <typename F>
 int guarded(gwhip_handle* h, F&& f) {
@@ -879,6 +1048,14 @@ int guarded(gwhip_handle* h, F&& f) {
     return GWHIP_ERR_DEVICE;
   }
 }
+
+template <typename F>
+int guarded_b(gwhip_batch* b, F&& f) {
+  gwhip_handle tmp;
+  const int st = guarded(&tmp, f);
+  if (st != GWHIP_OK && b) b->err = tmp.err;
+  return st;
+}
 }  // namespace
 
 extern "C" {
@@ -894,7 +1071,10 @@ int gwhip_create(const gwhip_dims* dims, int device, gwhip_handle** out) {
   d.nts = dims->nts; d.nw_b = dims->nw_b; d.beta = dims->beta;
   d.mesh = {dims->mesh[0], dims->mesh[1], dims->mesh[2]};
   auto h = std::make_unique<gwhip_handle>();
-  int st = guarded(h.get(), [&] { h->k = std::make_unique<gwhip::gw_hip_kernel>(d, device); });
+  int st = guarded(h.get(), [&] {
+    h->owned = std::make_unique<gwhip::gw_hip_kernel>(d, device);
+    h->k = h->owned.get();
+  });
   if (st != GWHIP_OK) {
     g_create_error = h->err;
     return st;
@@ -903,7 +1083,9 @@ int gwhip_create(const gwhip_dims* dims, int device, gwhip_handle** out) {
   return GWHIP_OK;
 }
 
-void gwhip_destroy(gwhip_handle* h) { delete h; }
+void gwhip_destroy(gwhip_handle* h) {
+  if (h && !h->borrowed) delete h;
+}
 
 const char* gwhip_last_error(const gwhip_handle* h) { return h ? h->err.c_str() : g_create_error.c_str(); }
 
@@ -940,6 +1122,85 @@ int gwhip_eval_sigma(gwhip_handle* h, const double* G, double* Sigma, int precis
 int gwhip_eval_sigma_device(gwhip_handle* h, const void* G, void* Sigma, int precision) {
   if (!h) return GWHIP_ERR_ARG;
   return guarded(h, [&] { h->k->solve_device((const double2*)G, (double2*)Sigma, precision); });
+}
+
+int gwhip_autotune(gwhip_handle* h, const void* G_dev, int precision) {
+  if (!h) return GWHIP_ERR_ARG;
+  return guarded(h, [&] { h->k->autotune((const double2*)G_dev, precision); });
+}
+
+int gwhip_get_tuning(const gwhip_handle* h, int* p0_tile, int* sigma_tile, int* aux_per_wave) {
+  if (!h || !p0_tile || !sigma_tile || !aux_per_wave) return GWHIP_ERR_ARG;
+  const gwhip::Tuning& t = h->k->tuning();
+  *p0_tile = t.p0_shape;
+  *sigma_tile = t.s_shape;
+  *aux_per_wave = t.qpw;
+  return GWHIP_OK;
+}
+
+int gwhip_set_tuning(gwhip_handle* h, int p0_tile, int sigma_tile, int aux_per_wave) {
+  if (!h || p0_tile < 0 || p0_tile > 2 || sigma_tile < 0 || sigma_tile > 2 ||
+      (aux_per_wave != 1 && aux_per_wave != 2 && aux_per_wave != 4 && aux_per_wave != 8))
+    return GWHIP_ERR_ARG;
+  gwhip::Tuning t;
+  t.p0_shape = p0_tile;
+  t.s_shape = sigma_tile;
+  t.qpw = aux_per_wave;
+  h->k->set_tuning(t);
+  return GWHIP_OK;
+}
+
+int gwhip_batch_create(int nsys, const gwhip_dims* dims, int device, gwhip_batch** out) {
+  if (!dims || !out || nsys <= 0) {
+    g_create_error = "null argument or empty batch";
+    return GWHIP_ERR_ARG;
+  }
+  *out = nullptr;
+  auto b = std::make_unique<gwhip_batch>();
+  std::vector<gwhip::Dims> d;
+  for (int i = 0; i < nsys; ++i) d.push_back(to_dims(&dims[i]));
+  int st = guarded_b(b.get(), [&] { b->b = std::make_unique<gwhip::gw_hip_batch>(nsys, d.data(), device); });
+  if (st != GWHIP_OK) {
+    g_create_error = b->err;
+    return st;
+  }
+  for (int i = 0; i < nsys; ++i) {
+    auto h = std::make_unique<gwhip_handle>();
+    h->k = &b->b->system(i);
+    h->borrowed = true;
+    b->systems.push_back(std::move(h));
+  }
+  *out = b.release();
+  return GWHIP_OK;
+}
+
+void gwhip_batch_destroy(gwhip_batch* b) { delete b; }
+
+const char* gwhip_batch_last_error(const gwhip_batch* b) { return b ? b->err.c_str() : g_create_error.c_str(); }
+
+gwhip_handle* gwhip_batch_system(gwhip_batch* b, int i) {
+  if (!b || i < 0 || i >= (int)b->systems.size()) return nullptr;
+  return b->systems[i].get();
+}
+
+int gwhip_batch_autotune(gwhip_batch* b, const void* const* G_dev, int precision) {
+  if (!b) return GWHIP_ERR_ARG;
+  return guarded_b(b, [&] { b->b->autotune((const double2* const*)G_dev, precision); });
+}
+
+int gwhip_batch_eval_device(gwhip_batch* b, const void* const* G_dev, void* const* Sigma_dev, int precision) {
+  if (!b) return GWHIP_ERR_ARG;
+  return guarded_b(b, [&] { b->b->eval((const double2* const*)G_dev, (double2* const*)Sigma_dev, precision); });
+}
+
+int gwhip_batch_stats(const gwhip_batch* b, double* last_ms, int64_t* graph_nodes, int64_t* captures,
+                      size_t* workspace_bytes) {
+  if (!b) return GWHIP_ERR_ARG;
+  if (last_ms) *last_ms = b->b->last_ms();
+  if (graph_nodes) *graph_nodes = b->b->last_nodes();
+  if (captures) *captures = b->b->captures();
+  if (workspace_bytes) *workspace_bytes = b->b->workspace_bytes();
+  return GWHIP_OK;
 }
 
 int gwhip_set_vjp_store(gwhip_handle* h, int keep) {

@@ -33,12 +33,34 @@ struct KernelStat {
   double ms = 0.0;
 };
 
+// Launch parameters picked by the autotuner (defaults = the best measured at nk = 64, nao = 26,
+// naux = 300): FP64 GEMM tile of the P0 and Sigma contractions (0: 64x64, 1: 128x64,
+// 2: 64x128) and the number of aux indices per wave of the P0 prep kernel.
+struct Tuning {
+  int p0_shape = 0, s_shape = 0, qpw = 4;
+};
+
+// Transient device memory of one evaluation.  A stand-alone kernel owns its own; the kernels
+// of a batch share one, sized for the largest system, and run one after another.
+struct Workspace {
+  double2* scratch = nullptr;  // Z or R, [tc][nk][NQ][nao^2]
+  size_t scratch_elems = 0;
+  double2* dyson = nullptr;    // Pw, X, Y, each [qc][nw_b][NQ^2]
+  size_t dyson_elems = 0;
+  int8_t* slices = nullptr;    // int8 slices of Z (P0 stage) or P-tilde (Sigma stage)
+  size_t slice_bytes = 0;
+  int* exps = nullptr;         // their exponents
+  size_t exp_elems = 0;
+};
+
 class gw_hip_kernel {
 public:
   enum Region { R_P0 = 0, R_TW = 1, R_BSE = 2, R_WT = 3, R_SIGMA = 4, R_TOTAL = 5, R_COUNT = 6 };
   enum KStat { K_P0_GEMM = 0, K_SIGMA_GEMM = 1, K_P0_PREP = 2, K_SIGMA_EPI = 3, K_DYSON_INV = 4, K_COUNT = 5 };
 
-  gw_hip_kernel(const Dims& d, int device);
+  // shared_stream == nullptr: own stream and workspace.  Otherwise the kernel runs on the given
+  // stream and waits for bind_workspace() before it can evaluate.
+  gw_hip_kernel(const Dims& d, int device, hipStream_t shared_stream = nullptr);
   ~gw_hip_kernel();
   gw_hip_kernel(const gw_hip_kernel&) = delete;
   gw_hip_kernel& operator=(const gw_hip_kernel&) = delete;
@@ -60,6 +82,30 @@ public:
   void vjp_host(const double2* G, const double2* Sbar, double2* Gbar, int mode);
   double vjp_ms() const { return vjp_ms_; }
 
+  // batch support: workspace needs per tau point / per q, binding, and capture-safe enqueueing
+  size_t scratch_per_t() const;
+  size_t dyson_per_q() const;
+  size_t slice_bytes_per_t() const;
+  size_t exps_per_t() const;
+  void bind_workspace(const Workspace& ws);
+  void prepare_split();  // persistent slice stores for GWHIP_PREC_SPLIT
+  bool split_ready() const { return slices_ns_ == split_ns_ && Vs_p0_ != nullptr; }
+  // Enqueue one evaluation on the stream: no host synchronisation, no memset nodes, no
+  // allocation (safe under stream capture once prepare_split / bind_workspace are done).
+  void enqueue_eval(const double2* G_dev, double2* Sigma_dev, bool split, int* nonfinite_flag);
+  void check_ready(int precision) const;
+  void autotune(const double2* G_dev, int precision);
+  const Tuning& tuning() const { return tune_; }
+  void set_tuning(const Tuning& t) {
+    tune_ = t;
+    ++generation_;
+  }
+  void set_timing(bool on) { timing_ = on; }
+  bool has_inputs() const { return grid_set_ && v_set_; }
+  uint64_t generation() const { return generation_; }
+  hipStream_t stream() const { return stream_; }
+  int device() const { return device_; }
+  const Dims& dims() const { return d_; }
   const double* region_ms() const { return region_ms_; }
   const KernelStat& kstat(int which) const { return kstat_[which]; }
   size_t device_bytes() const { return bytes_; }
@@ -68,6 +114,9 @@ private:
   Dims d_;
   int device_ = 0;
   hipStream_t stream_ = nullptr;
+  bool own_stream_ = true, own_ws_ = true, timing_ = true;
+  Tuning tune_;
+  uint64_t generation_ = 0;  // bumped whenever buffers or launch parameters change
   size_t bytes_ = 0;
   int max_lds_ = 65536;
   bool grid_set_ = false, v_set_ = false;
@@ -95,9 +144,9 @@ private:
   int* eVs_p0_ = nullptr;     // [nq][NQ][nk]
   int8_t* Vs_s_ = nullptr;    // [nq][nk][nao^2][2][ns][kc_s]
   int* eVs_s_ = nullptr;      // [nq][nk][nao^2]
-  int8_t* Zs_ = nullptr;      // [tc][NQ][nk][3][ns][kc_p0]
+  int8_t* Zs_ = nullptr;      // [tc][NQ][nk][3][ns][kc_p0]  (P0 stage)
   int* eZs_ = nullptr;
-  int8_t* Ps_ = nullptr;      // [tc][NQ][3][ns][kc_s]
+  int8_t* Ps_ = nullptr;      // [tc][NQ][3][ns][kc_s]       (Sigma stage; same memory as Zs_)
   int* ePs_ = nullptr;
   int64_t qchunk_ = 1, tchunk_p0_ = 1, tchunk_sigma_ = 1;
   // VJP: per-q buffers, a second orbital scratch, and the kept half-grid P0 of the last
@@ -123,6 +172,10 @@ private:
   void free_slices();
   void ensure_vjp_buffers();
   void free_vjp_buffers();
+  void zero(double2* X, int64_t n);
+  void p0_prep(int64_t q, const double2* G, int64_t t0, int64_t tc, int qpw);
+  void p0_gemm(int64_t q, int64_t tc, double2* Pt0, int shape);
+  void sigma_gemm(int64_t q, int64_t t0, int64_t tc, int shape);
   void stage_p0(const double2* G, bool split);
   void p0_one_q(int64_t q, const double2* G, bool split, double2* Pq);
   void gj_inverse(double2* X, int64_t nmat);

@@ -224,6 +224,17 @@ __global__ void generate_v_kernel(double2* __restrict__ Vq, const int* __restric
 }
 
 // Non-finite scan: flag[0] = 1 if any component is NaN/Inf.
+// Fill kernels (used instead of memset so that captured graphs hold kernel nodes only).
+__global__ void zero_kernel(double2* X, int64_t n) {
+  const int64_t step = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) X[i] = make_double2(0.0, 0.0);
+}
+
+__global__ void zero_int_kernel(int* X, int64_t n) {
+  const int64_t step = (int64_t)gridDim.x * blockDim.x;
+  for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) X[i] = 0;
+}
+
 __global__ void nonfinite_kernel(const double2* X, int64_t n, int* flag) {
   int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const int64_t step = (int64_t)gridDim.x * blockDim.x;

@@ -78,6 +78,21 @@ def load() -> ctypes.CDLL:
     lib.gwhip_debug_zgemm_split.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, P, ctypes.c_int]
     lib.gwhip_debug_gemm_bench.argtypes = [ctypes.c_int] * 7 + [ctypes.POINTER(ctypes.c_double)]
     lib.gwhip_debug_mfma_peak.argtypes = [ctypes.c_int] * 3 + [ctypes.POINTER(ctypes.c_double)]
+    lib.gwhip_autotune.argtypes = [P, P, ctypes.c_int]
+    IP = ctypes.POINTER(ctypes.c_int)
+    lib.gwhip_get_tuning.argtypes = [P, IP, IP, IP]
+    lib.gwhip_set_tuning.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    lib.gwhip_batch_create.argtypes = [ctypes.c_int, ctypes.POINTER(Dims), ctypes.c_int, ctypes.POINTER(P)]
+    lib.gwhip_batch_destroy.argtypes = [P]
+    lib.gwhip_batch_destroy.restype = None
+    lib.gwhip_batch_last_error.argtypes = [P]
+    lib.gwhip_batch_last_error.restype = ctypes.c_char_p
+    lib.gwhip_batch_system.argtypes = [P, ctypes.c_int]
+    lib.gwhip_batch_system.restype = P
+    lib.gwhip_batch_autotune.argtypes = [P, ctypes.POINTER(P), ctypes.c_int]
+    lib.gwhip_batch_eval_device.argtypes = [P, ctypes.POINTER(P), ctypes.POINTER(P), ctypes.c_int]
+    lib.gwhip_batch_stats.argtypes = [P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64),
+                                      ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_size_t)]
     lib.gwhip_set_vjp_store.argtypes = [P, ctypes.c_int]
     lib.gwhip_vjp.argtypes = [P, P, P, P, ctypes.c_int]
     lib.gwhip_vjp_device.argtypes = [P, P, P, P, ctypes.c_int]
@@ -86,7 +101,9 @@ def load() -> ctypes.CDLL:
     for f in ("gwhip_debug_zgemm_split", "gwhip_set_split_slices", "gwhip_set_kq_maps", "gwhip_debug_mfma_peak",
               "gwhip_debug_gemm_bench", "gwhip_debug_zgemm", "gwhip_debug_stage", "gwhip_set_grid", "gwhip_upload_V",
               "gwhip_generate_V", "gwhip_eval_sigma", "gwhip_eval_sigma_device", "gwhip_get_P_tilde", "gwhip_timings",
-              "gwhip_kernel_stats", "gwhip_set_vjp_store", "gwhip_vjp", "gwhip_vjp_device"):
+              "gwhip_kernel_stats", "gwhip_set_vjp_store", "gwhip_vjp", "gwhip_vjp_device", "gwhip_autotune",
+              "gwhip_get_tuning", "gwhip_set_tuning", "gwhip_batch_create", "gwhip_batch_autotune",
+              "gwhip_batch_eval_device", "gwhip_batch_stats"):
         getattr(lib, f).restype = ctypes.c_int
     _lib = lib
     return lib
@@ -140,15 +157,24 @@ def _ptr(a: np.ndarray) -> ctypes.c_void_p:
     return ctypes.c_void_p(a.ctypes.data)
 
 
+def default_mesh(nk, mesh=None):
+    if mesh is not None:
+        return tuple(mesh)
+    n = round(nk ** (1.0 / 3.0))
+    return (n, n, n) if n ** 3 == nk else (nk, 1, 1)
+
+
 class GwHipKernel:
     """One device-resident GW kernel instance (dims fixed at construction)."""
 
-    def __init__(self, nk, nao, nso, ns, NQ, nts, nw_b, beta, mesh=None, device=0):
+    def __init__(self, nk, nao, nso, ns, NQ, nts, nw_b, beta, mesh=None, device=0, _borrowed=None):
         self.lib = load()
-        if mesh is None:
-            n = round(nk ** (1.0 / 3.0))
-            mesh = (n, n, n) if n ** 3 == nk else (nk, 1, 1)
+        mesh = default_mesh(nk, mesh)
         self.dims = dict(nk=nk, nao=nao, nso=nso, ns=ns, NQ=NQ, nts=nts, nw_b=nw_b, beta=beta, mesh=tuple(mesh))
+        if _borrowed is not None:  # a system of a Batch: the batch owns the handle
+            self.h, self._owner = _borrowed, False
+            return
+        self._owner = True
         d = Dims(nk, (ctypes.c_int64 * 3)(*mesh), nao, nso, ns, NQ, nts, nw_b, beta)
         h = ctypes.c_void_p()
         st = self.lib.gwhip_create(ctypes.byref(d), device, ctypes.byref(h))
@@ -164,7 +190,8 @@ class GwHipKernel:
 
     def close(self):
         if getattr(self, "h", None):
-            self.lib.gwhip_destroy(self.h)
+            if getattr(self, "_owner", True):
+                self.lib.gwhip_destroy(self.h)
             self.h = None
 
     def __del__(self):
@@ -230,6 +257,17 @@ class GwHipKernel:
         self._check(self.lib.gwhip_vjp_device(self.h, ctypes.c_void_p(G_ptr), ctypes.c_void_p(Sbar_ptr),
                                               ctypes.c_void_p(Gbar_ptr), mode))
 
+    def autotune(self, G_ptr: int, precision: int = PREC_FP64):
+        self._check(self.lib.gwhip_autotune(self.h, ctypes.c_void_p(G_ptr), precision))
+
+    def tuning(self) -> dict:
+        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        self._check(self.lib.gwhip_get_tuning(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
+        return dict(p0_tile=a.value, sigma_tile=b.value, aux_per_wave=c.value)
+
+    def set_tuning(self, p0_tile: int, sigma_tile: int, aux_per_wave: int):
+        self._check(self.lib.gwhip_set_tuning(self.h, p0_tile, sigma_tile, aux_per_wave))
+
     def vjp_ms(self) -> float:
         return float(self.lib.gwhip_vjp_ms(self.h))
 
@@ -259,3 +297,61 @@ class GwHipKernel:
 
     def device_bytes(self) -> int:
         return int(self.lib.gwhip_device_bytes(self.h))
+
+
+class Batch:
+    """Several systems evaluated by one captured hipGraph (gwhip_batch_*)."""
+
+    def __init__(self, systems: list[dict], device: int = 0):
+        """systems: dicts with nk, nao, nso, ns, NQ, nts, nw_b, beta and optional mesh."""
+        self.lib = load()
+        n = len(systems)
+        arr = (Dims * n)()
+        for i, s in enumerate(systems):
+            mesh = default_mesh(s["nk"], s.get("mesh"))
+            arr[i] = Dims(s["nk"], (ctypes.c_int64 * 3)(*mesh), s["nao"], s["nso"], s["ns"], s["NQ"], s["nts"],
+                          s["nw_b"], s["beta"])
+        b = ctypes.c_void_p()
+        st = self.lib.gwhip_batch_create(n, arr, device, ctypes.byref(b))
+        if st != OK:
+            raise GwhipError(st, self.lib.gwhip_batch_last_error(None).decode())
+        self.b = b
+        self.systems = []
+        for i, s in enumerate(systems):
+            h = self.lib.gwhip_batch_system(self.b, i)
+            self.systems.append(GwHipKernel(s["nk"], s["nao"], s["nso"], s["ns"], s["NQ"], s["nts"], s["nw_b"],
+                                            s["beta"], mesh=s.get("mesh"), _borrowed=ctypes.c_void_p(h)))
+
+    def _check(self, st: int):
+        if st != OK:
+            raise GwhipError(st, self.lib.gwhip_batch_last_error(self.b).decode())
+
+    @staticmethod
+    def _ptrs(ptrs):
+        return (ctypes.c_void_p * len(ptrs))(*[ctypes.c_void_p(p) for p in ptrs])
+
+    def autotune(self, G_ptrs, precision: int = PREC_FP64):
+        self._check(self.lib.gwhip_batch_autotune(self.b, self._ptrs(G_ptrs), precision))
+
+    def eval_device(self, G_ptrs, S_ptrs, precision: int = PREC_FP64):
+        assert len(G_ptrs) == len(self.systems) == len(S_ptrs)
+        self._check(self.lib.gwhip_batch_eval_device(self.b, self._ptrs(G_ptrs), self._ptrs(S_ptrs), precision))
+
+    def stats(self) -> dict:
+        ms, nodes, caps, ws = ctypes.c_double(), ctypes.c_int64(), ctypes.c_int64(), ctypes.c_size_t()
+        self._check(self.lib.gwhip_batch_stats(self.b, ctypes.byref(ms), ctypes.byref(nodes), ctypes.byref(caps),
+                                               ctypes.byref(ws)))
+        return dict(last_ms=ms.value, graph_nodes=nodes.value, captures=caps.value, workspace_bytes=ws.value)
+
+    def close(self):
+        if getattr(self, "b", None):
+            for s in self.systems:
+                s.h = None
+            self.lib.gwhip_batch_destroy(self.b)
+            self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

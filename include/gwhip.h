@@ -118,6 +118,37 @@ int gwhip_eval_sigma(gwhip_handle* h, const double* G, double* Sigma, int precis
  * Synchronous with respect to the host. */
 int gwhip_eval_sigma_device(gwhip_handle* h, const void* G_dev, void* Sigma_dev, int precision);
 
+/* ---- launch-parameter autotuning ------------------------------------------------------
+ * Time the candidate launch parameters of this system on its own operands (G in device memory)
+ * and keep the fastest: the FP64 GEMM tiles of the P0 and Sigma contractions (0: 64x64,
+ * 1: 128x64, 2: 64x128) and the aux indices per wave of the P0 orbital kernel (1, 2, 4, 8).
+ * Results do not change, only speed.  No reference counterpart. */
+int gwhip_autotune(gwhip_handle* h, const void* G_dev, int precision);
+int gwhip_get_tuning(const gwhip_handle* h, int* p0_tile, int* sigma_tile, int* aux_per_wave);
+int gwhip_set_tuning(gwhip_handle* h, int p0_tile, int sigma_tile, int aux_per_wave);
+
+/* ---- several systems in one graph -------------------------------------------------------
+ * A batch holds nsys independent systems (each its own dims, grid, V and G), one stream and one
+ * transient workspace sized for the largest system.  gwhip_batch_eval_device evaluates Sigma of
+ * every system; the first call for a given precision and set of G / Sigma pointers captures the
+ * whole multi-system evaluation into a hipGraph, later calls replay it with one hipGraphLaunch.
+ * Each system is configured through the handle gwhip_batch_system returns (gwhip_set_grid,
+ * gwhip_upload_V / gwhip_generate_V, gwhip_set_kq_maps, gwhip_set_split_slices, gwhip_vjp...);
+ * those handles belong to the batch (gwhip_destroy ignores them).  The reference evaluates one
+ * system per gw_cpu_kernel (src/gw_cpu_kernel.cpp:31-72); this has no counterpart there. */
+typedef struct gwhip_batch gwhip_batch;
+int gwhip_batch_create(int nsys, const gwhip_dims* dims, int device, gwhip_batch** out);
+void gwhip_batch_destroy(gwhip_batch* b);
+const char* gwhip_batch_last_error(const gwhip_batch* b);
+gwhip_handle* gwhip_batch_system(gwhip_batch* b, int i);
+/* Autotune every system (G_dev[i] device pointers); drops captured graphs. */
+int gwhip_batch_autotune(gwhip_batch* b, const void* const* G_dev, int precision);
+/* Sigma_dev[i] = Sigma[G_dev[i]] for every system, device pointers, synchronous. */
+int gwhip_batch_eval_device(gwhip_batch* b, const void* const* G_dev, void* const* Sigma_dev, int precision);
+/* Milliseconds of the last graph launch, its node count, graphs captured so far, workspace bytes. */
+int gwhip_batch_stats(const gwhip_batch* b, double* last_ms, int64_t* graph_nodes, int64_t* captures,
+                      size_t* workspace_bytes);
+
 /* Keep (keep = 1) or stop keeping (0) the half-grid P0(tau, q) of every q after each
  * evaluation, so that GWHIP_VJP_STORED can reuse P0 and P-tilde instead of recomputing them.
  * Costs nk * nts/2 * NQ^2 complex of device memory while enabled.
