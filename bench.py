@@ -91,12 +91,16 @@ def main():
                     help="time the Sigma[G] vector-Jacobian product on the nk=27 nao=26 naux=250 system instead")
     ap.add_argument("--batch", action="store_true",
                     help="time the 8 uneven ns=2 systems evaluated as one captured graph instead")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a plain 2 GiB device-to-device copy (reference point for HBM byte counters)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
               "sharding over q with one Sigma all-reduce is described in DESIGN.md, not run.", file=sys.stderr)
         sys.exit(2)
 
+    if args.copy_calibration:
+        return copy_calibration()
     if args.vjp:
         return bench_vjp(args)
     if args.batch:
@@ -123,6 +127,10 @@ def main():
     if prec == gl.PREC_SPLIT:
         k.set_split_slices(args.slices)
     torch.cuda.synchronize()
+    t_tune = time.perf_counter()
+    k.autotune(G_d.data_ptr(), prec)
+    t_tune = time.perf_counter() - t_tune
+    print(f"[bench] autotuned in {t_tune:.2f} s: {k.tuning()}", file=sys.stderr, flush=True)
     for i in range(args.warmup):
         k.solve_device(G_d.data_ptr(), S_d.data_ptr(), prec)
         print(f"[bench] warmup {i} done", file=sys.stderr, flush=True)
@@ -191,6 +199,7 @@ def main():
         "regions_ms": regions,
         "kernel_ms": {k_: v_[1] / args.steps for k_, v_ in stats.items()},
         "sigma_hermiticity": herm,
+        "tuning": k.tuning(), "autotune_s": t_tune,
     }
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(w, T)
@@ -265,6 +274,23 @@ def bench_vjp(args):
         "modes": out, "forward_ms": fwd_ms, "base_device_gb": base_bytes / 1e9,
         "fd_check": {"step": h, "fd": fd, "vjp": an, "rel_diff": abs(fd - an) / abs(an)},
     }), flush=True)
+
+
+def copy_calibration():
+    """Three launches of torch's copy kernel over 2 GiB: 2 GiB read + 2 GiB written each."""
+    import torch
+
+    n = 1 << 28
+    a = torch.ones(n, dtype=torch.float64, device="cuda:0")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / 3
+    print(json.dumps({"copy_bytes_read": 8 * n, "copy_bytes_written": 8 * n, "ms_per_copy": dt * 1e3,
+                      "GB_per_s": 16 * n / dt / 1e9}), flush=True)
 
 
 BATCH_SYSTEMS = [(8, 8, 60), (8, 14, 120), (27, 18, 160), (27, 26, 250), (27, 40, 400), (64, 12, 100),

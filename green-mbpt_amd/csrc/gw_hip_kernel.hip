@@ -462,6 +462,63 @@ void gw_hip_kernel::sigma_gemm(int64_t q, int64_t t0, int64_t tc, int shape) {
   zgemm_f64(a, true, false, stream_, shape);
 }
 
+// Split precision, P0: int8 slices of Z (rows (tau, Q), chunks k) into Zs_.
+void gw_hip_kernel::p0_slice(int64_t tc) {
+  const int64_t nk = d_.nk, NQ = d_.NQ, nao2 = d_.nao * d_.nao;
+  SliceArgs sa{};
+  sa.src = scratch_; sa.nrows = tc * NQ; sa.row_div = NQ; sa.s_rhi = nk * NQ * nao2; sa.s_rlo = nao2;
+  sa.nchunk = (int)nk; sa.s_c = NQ * nao2; sa.klen = (int)nao2; sa.s_k = 1; sa.kc = (int)kc_p0_;
+  sa.conj = 0; sa.nplanes = 3; sa.dst = Zs_; sa.exps = eZs_;
+  slice_rows(sa, split_ns_, stream_);
+}
+
+// Split precision, P0 = -(pref/nk) Z conj(V)^T from the slices (row-tile group `group`).
+void gw_hip_kernel::p0_gemm_split(int64_t q, int64_t tc, double2* Pt0, int group) {
+  const int64_t nk = d_.nk, NQ = d_.NQ, NQ2 = NQ * NQ;
+  const int ns = split_ns_;
+  OzGemmArgs o{};
+  o.M = (int)NQ; o.N = (int)NQ; o.batch = (int)tc; o.kc = (int)kc_p0_; o.nchunk = (int)nk; o.nin = (int)NQ;
+  o.A = Zs_; o.sAm = nk * 3 * ns * kc_p0_; o.sAb = NQ * o.sAm; o.sAc = 3 * ns * kc_p0_;
+  o.eA = eZs_; o.seAb = NQ * nk; o.seAm = nk;
+  o.B = Vs_p0_ + q * NQ * nk * 2 * ns * kc_p0_; o.sBb = 0; o.sBno = 0; o.sBni = nk * 2 * ns * kc_p0_;
+  o.sBc = 2 * ns * kc_p0_;
+  o.eB = eVs_p0_ + q * NQ * nk; o.seBb = 0; o.seBno = 0; o.seBni = nk;
+  o.C = Pt0; o.sCb = NQ2; o.sCm = NQ; o.sCno = 0; o.sCni = 1;
+  o.alpha = make_double2(-d_.prefactor / (double)nk, 0.0);
+  o.accumulate = 0;
+  o.group_m = group;
+  zgemm_ozaki(o, ns, stream_);
+}
+
+// Split precision, Sigma: int8 slices of P-tilde(q, t0 .. t0+tc) into Ps_.
+void gw_hip_kernel::sigma_slice(int64_t q, int64_t t0, int64_t tc) {
+  const int64_t NQ = d_.NQ, NQ2 = NQ * NQ, nts = d_.nts;
+  SliceArgs sa{};
+  sa.src = P_ + (q * nts + t0) * NQ2; sa.nrows = tc * NQ; sa.row_div = tc * NQ; sa.s_rhi = 0; sa.s_rlo = NQ;
+  sa.nchunk = 1; sa.s_c = 0; sa.klen = (int)NQ; sa.s_k = 1; sa.kc = (int)kc_s_;
+  sa.conj = 0; sa.nplanes = 3; sa.dst = Ps_; sa.exps = ePs_;
+  slice_rows(sa, split_ns_, stream_);
+}
+
+// Split precision, R = P-tilde conj(V(k, k - q)) into scratch_ from the slices.
+void gw_hip_kernel::sigma_gemm_split(int64_t q, int64_t tc, int group) {
+  const int64_t nk = d_.nk, NQ = d_.NQ, nao2 = d_.nao * d_.nao;
+  const int ns = split_ns_;
+  const int qn = neg_q_[q];
+  OzGemmArgs o{};
+  o.M = (int)NQ; o.N = (int)(nk * nao2); o.batch = (int)tc; o.kc = (int)kc_s_; o.nchunk = 1; o.nin = (int)nao2;
+  o.A = Ps_; o.sAm = 3 * ns * kc_s_; o.sAb = NQ * o.sAm; o.sAc = 0;
+  o.eA = ePs_; o.seAb = NQ; o.seAm = 1;
+  o.B = Vs_s_ + (int64_t)qn * nk * nao2 * 2 * ns * kc_s_; o.sBb = 0; o.sBno = nao2 * 2 * ns * kc_s_;
+  o.sBni = 2 * ns * kc_s_; o.sBc = 0;
+  o.eB = eVs_s_ + (int64_t)qn * nk * nao2; o.seBb = 0; o.seBno = nao2; o.seBni = 1;
+  o.C = scratch_; o.sCb = nk * NQ * nao2; o.sCm = nao2; o.sCno = NQ * nao2; o.sCni = 1;
+  o.alpha = make_double2(1.0, 0.0);
+  o.accumulate = 0;
+  o.group_m = group;
+  zgemm_ozaki(o, ns, stream_);
+}
+
 void gw_hip_kernel::stage_p0(const double2* G, bool split) {
   for (int64_t q = 0; q < d_.nk; ++q) p0_one_q(q, G, split, P_ + q * d_.nts * d_.NQ * d_.NQ);
 }
@@ -473,23 +530,8 @@ void gw_hip_kernel::p0_one_q(int64_t q, const double2* G, bool split, double2* P
     const int64_t tc = std::min(tchunk_p0_, half - t0);
     timed(K_P0_PREP, [&] { p0_prep(q, G, t0, tc, tune_.qpw); });
     if (split) {
-      const int ns = split_ns_;
-      SliceArgs sa{};
-      sa.src = scratch_; sa.nrows = tc * NQ; sa.row_div = NQ; sa.s_rhi = nk * NQ * nao2; sa.s_rlo = nao2;
-      sa.nchunk = (int)nk; sa.s_c = NQ * nao2; sa.klen = (int)nao2; sa.s_k = 1; sa.kc = (int)kc_p0_;
-      sa.conj = 0; sa.nplanes = 3; sa.dst = Zs_; sa.exps = eZs_;
-      slice_rows(sa, ns, stream_);
-      OzGemmArgs o{};
-      o.M = (int)NQ; o.N = (int)NQ; o.batch = (int)tc; o.kc = (int)kc_p0_; o.nchunk = (int)nk; o.nin = (int)NQ;
-      o.A = Zs_; o.sAm = nk * 3 * ns * kc_p0_; o.sAb = NQ * o.sAm; o.sAc = 3 * ns * kc_p0_;
-      o.eA = eZs_; o.seAb = NQ * nk; o.seAm = nk;
-      o.B = Vs_p0_ + q * NQ * nk * 2 * ns * kc_p0_; o.sBb = 0; o.sBno = 0; o.sBni = nk * 2 * ns * kc_p0_;
-      o.sBc = 2 * ns * kc_p0_;
-      o.eB = eVs_p0_ + q * NQ * nk; o.seBb = 0; o.seBno = 0; o.seBni = nk;
-      o.C = Pq + t0 * NQ2; o.sCb = NQ2; o.sCm = NQ; o.sCno = 0; o.sCni = 1;
-      o.alpha = make_double2(-d_.prefactor / (double)nk, 0.0);
-      o.accumulate = 0;
-      timed(K_P0_GEMM, [&] { zgemm_ozaki(o, ns, stream_); });
+      p0_slice(tc);
+      timed(K_P0_GEMM, [&] { p0_gemm_split(q, tc, Pq + t0 * NQ2, tune_.p0_group); });
     } else {
       timed(K_P0_GEMM, [&] { p0_gemm(q, tc, Pq + t0 * NQ2, tune_.p0_shape); });
     }
@@ -594,24 +636,8 @@ void gw_hip_kernel::stage_sigma(const double2* G, double2* Sigma, bool split) {
     for (int64_t t0 = 0; t0 < nts; t0 += tchunk_sigma_) {
       const int64_t tc = std::min(tchunk_sigma_, nts - t0);
       if (split) {
-        const int ns = split_ns_;
-        const int qn = neg_q_[q];
-        SliceArgs sa{};
-        sa.src = P_ + (q * nts + t0) * NQ2; sa.nrows = tc * NQ; sa.row_div = tc * NQ; sa.s_rhi = 0; sa.s_rlo = NQ;
-        sa.nchunk = 1; sa.s_c = 0; sa.klen = (int)NQ; sa.s_k = 1; sa.kc = (int)kc_s_;
-        sa.conj = 0; sa.nplanes = 3; sa.dst = Ps_; sa.exps = ePs_;
-        slice_rows(sa, ns, stream_);
-        OzGemmArgs o{};
-        o.M = (int)NQ; o.N = (int)(nk * nao2); o.batch = (int)tc; o.kc = (int)kc_s_; o.nchunk = 1; o.nin = (int)nao2;
-        o.A = Ps_; o.sAm = 3 * ns * kc_s_; o.sAb = NQ * o.sAm; o.sAc = 0;
-        o.eA = ePs_; o.seAb = NQ; o.seAm = 1;
-        o.B = Vs_s_ + (int64_t)qn * nk * nao2 * 2 * ns * kc_s_; o.sBb = 0; o.sBno = nao2 * 2 * ns * kc_s_;
-        o.sBni = 2 * ns * kc_s_; o.sBc = 0;
-        o.eB = eVs_s_ + (int64_t)qn * nk * nao2; o.seBb = 0; o.seBno = nao2; o.seBni = 1;
-        o.C = scratch_; o.sCb = nk * NQ * nao2; o.sCm = nao2; o.sCno = NQ * nao2; o.sCni = 1;
-        o.alpha = make_double2(1.0, 0.0);
-        o.accumulate = 0;
-        timed(K_SIGMA_GEMM, [&] { zgemm_ozaki(o, ns, stream_); });
+        sigma_slice(q, t0, tc);
+        timed(K_SIGMA_GEMM, [&] { sigma_gemm_split(q, tc, tune_.s_group); });
       } else {
         timed(K_SIGMA_GEMM, [&] { sigma_gemm(q, t0, tc, tune_.s_shape); });
       }
@@ -724,6 +750,13 @@ void gw_hip_kernel::autotune(const double2* G, int precision) {
     p0_prep(0, G, 0, tcp, tune_.qpw);
     tune_.p0_shape = best_of(3, shapes, [&](int c) { p0_gemm(0, tcp, P_, c); });
     tune_.s_shape = best_of(3, shapes, [&](int c) { sigma_gemm(0, 0, tcs, c); });
+  } else {
+    const int groups[7] = {1, 2, 4, 8, 16, 32, 64};
+    p0_prep(0, G, 0, tcp, tune_.qpw);
+    p0_slice(tcp);
+    tune_.p0_group = best_of(7, groups, [&](int c) { p0_gemm_split(0, tcp, P_, c); });
+    sigma_slice(0, 0, tcs);
+    tune_.s_group = best_of(7, groups, [&](int c) { sigma_gemm_split(0, tcs, c); });
   }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
@@ -1129,23 +1162,29 @@ int gwhip_autotune(gwhip_handle* h, const void* G_dev, int precision) {
   return guarded(h, [&] { h->k->autotune((const double2*)G_dev, precision); });
 }
 
-int gwhip_get_tuning(const gwhip_handle* h, int* p0_tile, int* sigma_tile, int* aux_per_wave) {
-  if (!h || !p0_tile || !sigma_tile || !aux_per_wave) return GWHIP_ERR_ARG;
+int gwhip_get_tuning(const gwhip_handle* h, int* p0_tile, int* sigma_tile, int* aux_per_wave, int* p0_group,
+                     int* sigma_group) {
+  if (!h || !p0_tile || !sigma_tile || !aux_per_wave || !p0_group || !sigma_group) return GWHIP_ERR_ARG;
   const gwhip::Tuning& t = h->k->tuning();
   *p0_tile = t.p0_shape;
   *sigma_tile = t.s_shape;
   *aux_per_wave = t.qpw;
+  *p0_group = t.p0_group;
+  *sigma_group = t.s_group;
   return GWHIP_OK;
 }
 
-int gwhip_set_tuning(gwhip_handle* h, int p0_tile, int sigma_tile, int aux_per_wave) {
+int gwhip_set_tuning(gwhip_handle* h, int p0_tile, int sigma_tile, int aux_per_wave, int p0_group, int sigma_group) {
   if (!h || p0_tile < 0 || p0_tile > 2 || sigma_tile < 0 || sigma_tile > 2 ||
-      (aux_per_wave != 1 && aux_per_wave != 2 && aux_per_wave != 4 && aux_per_wave != 8))
+      (aux_per_wave != 1 && aux_per_wave != 2 && aux_per_wave != 4 && aux_per_wave != 8) || p0_group < 1 ||
+      sigma_group < 1)
     return GWHIP_ERR_ARG;
   gwhip::Tuning t;
   t.p0_shape = p0_tile;
   t.s_shape = sigma_tile;
   t.qpw = aux_per_wave;
+  t.p0_group = p0_group;
+  t.s_group = sigma_group;
   h->k->set_tuning(t);
   return GWHIP_OK;
 }

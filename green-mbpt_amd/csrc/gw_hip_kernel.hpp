@@ -35,9 +35,11 @@ struct KernelStat {
 
 // Launch parameters picked by the autotuner (defaults = the best measured at nk = 64, nao = 26,
 // naux = 300): FP64 GEMM tile of the P0 and Sigma contractions (0: 64x64, 1: 128x64,
-// 2: 64x128) and the number of aux indices per wave of the P0 prep kernel.
+// 2: 64x128), the number of aux indices per wave of the P0 prep kernel, and the row-tile group
+// of the split-precision GEMMs' L2-aware tile order.
 struct Tuning {
   int p0_shape = 0, s_shape = 0, qpw = 4;
+  int p0_group = 1, s_group = 16;  // split precision: row tiles per L2 group of the int8 GEMMs
 };
 
 // Transient device memory of one evaluation.  A stand-alone kernel owns its own; the kernels
@@ -176,6 +178,10 @@ private:
   void p0_prep(int64_t q, const double2* G, int64_t t0, int64_t tc, int qpw);
   void p0_gemm(int64_t q, int64_t tc, double2* Pt0, int shape);
   void sigma_gemm(int64_t q, int64_t t0, int64_t tc, int shape);
+  void p0_slice(int64_t tc);
+  void p0_gemm_split(int64_t q, int64_t tc, double2* Pt0, int group);
+  void sigma_slice(int64_t q, int64_t t0, int64_t tc);
+  void sigma_gemm_split(int64_t q, int64_t tc, int group);
   void stage_p0(const double2* G, bool split);
   void p0_one_q(int64_t q, const double2* G, bool split, double2* Pq);
   void gj_inverse(double2* X, int64_t nmat);

@@ -16,6 +16,8 @@
 //   Re C = Re A Re B + (-Im A) Im B,   Im C = Re A Im B + Im A Re B.
 #pragma once
 
+#include <algorithm>
+
 #include "common.hpp"
 
 namespace gwhip {
@@ -130,6 +132,7 @@ struct OzGemmArgs {
   int accumulate;
   int tiles_m, tiles_n;
   int64_t nblocks, per_xcd;
+  int group_m;  // row tiles (over batch x M) swept together across all column tiles; 0 = heuristic
 };
 
 // Output layout of v_mfma_i32_16x16x64_i8: register r of lane l holds row
@@ -155,13 +158,22 @@ __global__ __launch_bounds__(256, 2) void zgemm_ozaki_kernel(OzGemmArgs p) {
   int* eAs = reinterpret_cast<int*>(As + PA * NS * OZ_BM * 64);  // [32]
   int* eBs = eAs + OZ_BM;                                        // [64]
 
+  // Each XCD takes a contiguous range of logical tiles; within it, groups of group_m row tiles
+  // (rows run over batch x M) sweep every column tile, row fastest, so a group's A panels stay in
+  // that XCD's L2 while the B panels stream through it once per group.
   const int64_t hw = blockIdx.x;
   const int64_t logical = (hw % 8) * p.per_xcd + hw / 8;
   if (logical >= p.nblocks) return;
-  const int tiles = p.tiles_m * p.tiles_n;
-  const int64_t b = logical / tiles;
-  const int rem = (int)(logical - b * tiles);
-  const int tm = rem / p.tiles_n, tn = rem - (rem / p.tiles_n) * p.tiles_n;
+  const int64_t rows = (int64_t)p.batch * p.tiles_m;
+  const int64_t in_group = (int64_t)p.group_m * p.tiles_n;
+  const int64_t grp = logical / in_group;
+  const int64_t first = grp * p.group_m;
+  const int64_t gsize = rows - first < p.group_m ? rows - first : p.group_m;
+  const int64_t idx = logical - grp * in_group;
+  const int64_t rt = first + idx % gsize;
+  const int tn = (int)(idx / gsize);
+  const int64_t b = rt / p.tiles_m;
+  const int tm = (int)(rt - b * p.tiles_m);
   const int m0 = tm * OZ_BM, n0 = tn * OZ_BN;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lrow = lane & 15, lg = lane >> 4;
@@ -292,6 +304,10 @@ __global__ __launch_bounds__(256, 2) void zgemm_ozaki_kernel(OzGemmArgs p) {
     }
 }
 
+// About 64 blocks (the ones resident on one XCD) share a group when there are few column
+// tiles; with many column tiles 16 row tiles of A (~2.5 MB at K = 320, 5 slices) stay in L2.
+inline int ozaki_default_group(int tiles_n) { return tiles_n >= 64 ? 16 : std::max(1, 64 / tiles_n); }
+
 inline size_t ozaki_lds_bytes(int ns) {
   return (size_t)3 * OZ_BM * ns * 64 + (OZ_BM + OZ_BN) * sizeof(int);
 }
@@ -309,6 +325,8 @@ inline void zgemm_ozaki(OzGemmArgs p, int ns, hipStream_t stream) {
   p.tiles_m = (int)ceil_div(p.M, OZ_BM);
   p.tiles_n = (int)ceil_div(p.N, OZ_BN);
   p.nblocks = (int64_t)p.batch * p.tiles_m * p.tiles_n;
+  if (p.group_m <= 0) p.group_m = ozaki_default_group(p.tiles_n);
+  p.group_m = (int)std::min<int64_t>(p.group_m, (int64_t)p.batch * p.tiles_m);
   p.per_xcd = ceil_div(p.nblocks, 8);
   const int64_t grid = p.per_xcd * 8;
   if (grid > 0x7fffffffLL) throw std::runtime_error("zgemm_ozaki: grid too large");

@@ -80,8 +80,8 @@ def load() -> ctypes.CDLL:
     lib.gwhip_debug_mfma_peak.argtypes = [ctypes.c_int] * 3 + [ctypes.POINTER(ctypes.c_double)]
     lib.gwhip_autotune.argtypes = [P, P, ctypes.c_int]
     IP = ctypes.POINTER(ctypes.c_int)
-    lib.gwhip_get_tuning.argtypes = [P, IP, IP, IP]
-    lib.gwhip_set_tuning.argtypes = [P, ctypes.c_int, ctypes.c_int, ctypes.c_int]
+    lib.gwhip_get_tuning.argtypes = [P, IP, IP, IP, IP, IP]
+    lib.gwhip_set_tuning.argtypes = [P] + [ctypes.c_int] * 5
     lib.gwhip_batch_create.argtypes = [ctypes.c_int, ctypes.POINTER(Dims), ctypes.c_int, ctypes.POINTER(P)]
     lib.gwhip_batch_destroy.argtypes = [P]
     lib.gwhip_batch_destroy.restype = None
@@ -261,12 +261,13 @@ class GwHipKernel:
         self._check(self.lib.gwhip_autotune(self.h, ctypes.c_void_p(G_ptr), precision))
 
     def tuning(self) -> dict:
-        a, b, c = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-        self._check(self.lib.gwhip_get_tuning(self.h, ctypes.byref(a), ctypes.byref(b), ctypes.byref(c)))
-        return dict(p0_tile=a.value, sigma_tile=b.value, aux_per_wave=c.value)
+        v = [ctypes.c_int() for _ in range(5)]
+        self._check(self.lib.gwhip_get_tuning(self.h, *[ctypes.byref(x) for x in v]))
+        return dict(p0_tile=v[0].value, sigma_tile=v[1].value, aux_per_wave=v[2].value, p0_group=v[3].value,
+                    sigma_group=v[4].value)
 
-    def set_tuning(self, p0_tile: int, sigma_tile: int, aux_per_wave: int):
-        self._check(self.lib.gwhip_set_tuning(self.h, p0_tile, sigma_tile, aux_per_wave))
+    def set_tuning(self, p0_tile: int, sigma_tile: int, aux_per_wave: int, p0_group: int = 1, sigma_group: int = 16):
+        self._check(self.lib.gwhip_set_tuning(self.h, p0_tile, sigma_tile, aux_per_wave, p0_group, sigma_group))
 
     def vjp_ms(self) -> float:
         return float(self.lib.gwhip_vjp_ms(self.h))

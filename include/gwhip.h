@@ -121,11 +121,14 @@ int gwhip_eval_sigma_device(gwhip_handle* h, const void* G_dev, void* Sigma_dev,
 /* ---- launch-parameter autotuning ------------------------------------------------------
  * Time the candidate launch parameters of this system on its own operands (G in device memory)
  * and keep the fastest: the FP64 GEMM tiles of the P0 and Sigma contractions (0: 64x64,
- * 1: 128x64, 2: 64x128) and the aux indices per wave of the P0 orbital kernel (1, 2, 4, 8).
- * Results do not change, only speed.  No reference counterpart. */
+ * 1: 128x64, 2: 64x128), the aux indices per wave of the P0 orbital kernel (1, 2, 4, 8) and, for
+ * GWHIP_PREC_SPLIT, the number of row tiles that sweep the column tiles together in the int8
+ * GEMMs' L2-aware tile order (1 .. 64).  Results do not change, only speed.
+ * No reference counterpart. */
 int gwhip_autotune(gwhip_handle* h, const void* G_dev, int precision);
-int gwhip_get_tuning(const gwhip_handle* h, int* p0_tile, int* sigma_tile, int* aux_per_wave);
-int gwhip_set_tuning(gwhip_handle* h, int p0_tile, int sigma_tile, int aux_per_wave);
+int gwhip_get_tuning(const gwhip_handle* h, int* p0_tile, int* sigma_tile, int* aux_per_wave, int* p0_group,
+                     int* sigma_group);
+int gwhip_set_tuning(gwhip_handle* h, int p0_tile, int sigma_tile, int aux_per_wave, int p0_group, int sigma_group);
 
 /* ---- several systems in one graph -------------------------------------------------------
  * A batch holds nsys independent systems (each its own dims, grid, V and G), one stream and one

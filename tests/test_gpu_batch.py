@@ -111,7 +111,11 @@ def test_recapture_rules_and_autotune():
         for s in b.systems:
             t = s.tuning()
             assert t["p0_tile"] in (0, 1, 2) and t["sigma_tile"] in (0, 1, 2) and t["aux_per_wave"] in (1, 2, 4, 8)
-        b.systems[1].set_tuning(1, 2, 1)
+        b.autotune(gp, gl.PREC_SPLIT)
+        for s in b.systems:
+            t = s.tuning()
+            assert t["p0_group"] in (1, 2, 4, 8, 16, 32, 64) and t["sigma_group"] in (1, 2, 4, 8, 16, 32, 64)
+        b.systems[1].set_tuning(1, 2, 1, 4, 8)
         b.eval_device(gp, [s.data_ptr() for s in S1])
         assert b.stats()["captures"] == 3
         for a, s in zip(first, S1):
