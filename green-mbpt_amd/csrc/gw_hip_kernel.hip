@@ -131,6 +131,10 @@ gw_hip_kernel::~gw_hip_kernel() {
       if (p) (void)hipFree(p);
   free_slices();
   free_vjp_buffers();
+  for (auto& pd : pending_) {
+    (void)hipEventDestroy(pd.a);
+    (void)hipEventDestroy(pd.b);
+  }
   if (stream_ && own_stream_) (void)hipStreamDestroy(stream_);
 }
 
@@ -367,23 +371,32 @@ void gw_hip_kernel::generate_V(uint64_t seed, double sigma) {
 
 template <typename F>
 void gw_hip_kernel::timed(int which, F&& launch) {
-  if (!timing_) {  // capture-safe: no events, no host synchronisation
+  if (!timing_) {  // capture-safe: no events
     launch();
     return;
   }
-  hipEvent_t a, b;
-  GWHIP_CHECK(hipEventCreate(&a));
-  GWHIP_CHECK(hipEventCreate(&b));
-  GWHIP_CHECK(hipEventRecord(a, stream_));
+  Pending pd{which, nullptr, nullptr};
+  GWHIP_CHECK(hipEventCreate(&pd.a));
+  GWHIP_CHECK(hipEventCreate(&pd.b));
+  GWHIP_CHECK(hipEventRecord(pd.a, stream_));
   launch();
-  GWHIP_CHECK(hipEventRecord(b, stream_));
-  GWHIP_CHECK(hipEventSynchronize(b));
-  float ms = 0.f;
-  GWHIP_CHECK(hipEventElapsedTime(&ms, a, b));
-  (void)hipEventDestroy(a);
-  (void)hipEventDestroy(b);
-  kstat_[which].launches += 1;
-  kstat_[which].ms += ms;
+  GWHIP_CHECK(hipEventRecord(pd.b, stream_));
+  pending_.push_back(pd);
+}
+
+// Accumulate the per-launch timings recorded since the last call (the stream must have drained).
+void gw_hip_kernel::resolve_timings() {
+  for (auto& pd : pending_) {
+    float ms = 0.f;
+    if (hipEventElapsedTime(&ms, pd.a, pd.b) == hipSuccess) {
+      kstat_[pd.which].launches += 1;
+      kstat_[pd.which].ms += ms;
+    }
+    (void)hipEventDestroy(pd.a);
+    (void)hipEventDestroy(pd.b);
+  }
+  (void)hipGetLastError();
+  pending_.clear();
 }
 
 namespace {
@@ -473,7 +486,7 @@ void gw_hip_kernel::p0_slice(int64_t tc) {
 }
 
 // Split precision, P0 = -(pref/nk) Z conj(V)^T from the slices (row-tile group `group`).
-void gw_hip_kernel::p0_gemm_split(int64_t q, int64_t tc, double2* Pt0, int group) {
+void gw_hip_kernel::p0_gemm_split(int64_t q, int64_t tc, double2* Pt0, int group, int tile) {
   const int64_t nk = d_.nk, NQ = d_.NQ, NQ2 = NQ * NQ;
   const int ns = split_ns_;
   OzGemmArgs o{};
@@ -487,6 +500,7 @@ void gw_hip_kernel::p0_gemm_split(int64_t q, int64_t tc, double2* Pt0, int group
   o.alpha = make_double2(-d_.prefactor / (double)nk, 0.0);
   o.accumulate = 0;
   o.group_m = group;
+  o.tile = tile;
   zgemm_ozaki(o, ns, stream_);
 }
 
@@ -501,7 +515,7 @@ void gw_hip_kernel::sigma_slice(int64_t q, int64_t t0, int64_t tc) {
 }
 
 // Split precision, R = P-tilde conj(V(k, k - q)) into scratch_ from the slices.
-void gw_hip_kernel::sigma_gemm_split(int64_t q, int64_t tc, int group) {
+void gw_hip_kernel::sigma_gemm_split(int64_t q, int64_t tc, int group, int tile) {
   const int64_t nk = d_.nk, NQ = d_.NQ, nao2 = d_.nao * d_.nao;
   const int ns = split_ns_;
   const int qn = neg_q_[q];
@@ -516,6 +530,7 @@ void gw_hip_kernel::sigma_gemm_split(int64_t q, int64_t tc, int group) {
   o.alpha = make_double2(1.0, 0.0);
   o.accumulate = 0;
   o.group_m = group;
+  o.tile = tile;
   zgemm_ozaki(o, ns, stream_);
 }
 
@@ -531,7 +546,7 @@ void gw_hip_kernel::p0_one_q(int64_t q, const double2* G, bool split, double2* P
     timed(K_P0_PREP, [&] { p0_prep(q, G, t0, tc, tune_.qpw); });
     if (split) {
       p0_slice(tc);
-      timed(K_P0_GEMM, [&] { p0_gemm_split(q, tc, Pq + t0 * NQ2, tune_.p0_group); });
+      timed(K_P0_GEMM, [&] { p0_gemm_split(q, tc, Pq + t0 * NQ2, tune_.p0_group, tune_.p0_oz_tile); });
     } else {
       timed(K_P0_GEMM, [&] { p0_gemm(q, tc, Pq + t0 * NQ2, tune_.p0_shape); });
     }
@@ -637,7 +652,7 @@ void gw_hip_kernel::stage_sigma(const double2* G, double2* Sigma, bool split) {
       const int64_t tc = std::min(tchunk_sigma_, nts - t0);
       if (split) {
         sigma_slice(q, t0, tc);
-        timed(K_SIGMA_GEMM, [&] { sigma_gemm_split(q, tc, tune_.s_group); });
+        timed(K_SIGMA_GEMM, [&] { sigma_gemm_split(q, tc, tune_.s_group, tune_.s_oz_tile); });
       } else {
         timed(K_SIGMA_GEMM, [&] { sigma_gemm(q, t0, tc, tune_.s_shape); });
       }
@@ -672,6 +687,7 @@ void gw_hip_kernel::solve_device(const double2* G, double2* Sigma, int precision
   GWHIP_CHECK(hipSetDevice(device_));
   if (split) ensure_slices();
   timing_ = true;
+  resolve_timings();
   for (double& r : region_ms_) r = 0.0;
   for (auto& k : kstat_) k = KernelStat{};
   const int64_t nsig = d_.nts * d_.ns * d_.nk * d_.nso * d_.nso;
@@ -697,6 +713,7 @@ void gw_hip_kernel::solve_device(const double2* G, double2* Sigma, int precision
   hipLaunchKernelGGL(nonfinite_kernel, dim3(1024), dim3(256), 0, stream_, Sigma, nsig, flag_);
   GWHIP_CHECK(hipGetLastError());
   GWHIP_CHECK(hipStreamSynchronize(stream_));
+  resolve_timings();
   float x;
   GWHIP_CHECK(hipEventElapsedTime(&x, t0, t1)); region_ms_[R_P0] = x;
   GWHIP_CHECK(hipEventElapsedTime(&x, t2, t3)); region_ms_[R_SIGMA] = x;
@@ -751,12 +768,17 @@ void gw_hip_kernel::autotune(const double2* G, int precision) {
     tune_.p0_shape = best_of(3, shapes, [&](int c) { p0_gemm(0, tcp, P_, c); });
     tune_.s_shape = best_of(3, shapes, [&](int c) { sigma_gemm(0, 0, tcs, c); });
   } else {
-    const int groups[7] = {1, 2, 4, 8, 16, 32, 64};
+    // candidates (block tile, row-tile group) encoded as 100 * tile + group
+    const int cands[14] = {1, 2, 4, 8, 16, 32, 64, 101, 102, 104, 108, 116, 132, 164};
     p0_prep(0, G, 0, tcp, tune_.qpw);
     p0_slice(tcp);
-    tune_.p0_group = best_of(7, groups, [&](int c) { p0_gemm_split(0, tcp, P_, c); });
+    const int bp = best_of(14, cands, [&](int c) { p0_gemm_split(0, tcp, P_, c % 100, c / 100); });
+    tune_.p0_oz_tile = bp / 100;
+    tune_.p0_group = bp % 100;
     sigma_slice(0, 0, tcs);
-    tune_.s_group = best_of(7, groups, [&](int c) { sigma_gemm_split(0, tcs, c); });
+    const int bs = best_of(14, cands, [&](int c) { sigma_gemm_split(0, tcs, c % 100, c / 100); });
+    tune_.s_oz_tile = bs / 100;
+    tune_.s_group = bs % 100;
   }
   (void)hipEventDestroy(a);
   (void)hipEventDestroy(b);
@@ -787,6 +809,7 @@ void gw_hip_kernel::run_stages_host(const double2* G, int last_stage) {
   stage_p0(G_dev_, false);
   if (last_stage >= 2) stage_dyson();
   GWHIP_CHECK(hipStreamSynchronize(stream_));
+  resolve_timings();
 }
 
 }  // namespace gwhip
@@ -1162,29 +1185,16 @@ int gwhip_autotune(gwhip_handle* h, const void* G_dev, int precision) {
   return guarded(h, [&] { h->k->autotune((const double2*)G_dev, precision); });
 }
 
-int gwhip_get_tuning(const gwhip_handle* h, int* p0_tile, int* sigma_tile, int* aux_per_wave, int* p0_group,
-                     int* sigma_group) {
-  if (!h || !p0_tile || !sigma_tile || !aux_per_wave || !p0_group || !sigma_group) return GWHIP_ERR_ARG;
-  const gwhip::Tuning& t = h->k->tuning();
-  *p0_tile = t.p0_shape;
-  *sigma_tile = t.s_shape;
-  *aux_per_wave = t.qpw;
-  *p0_group = t.p0_group;
-  *sigma_group = t.s_group;
+int gwhip_get_tuning(const gwhip_handle* h, int* values, int n) {
+  if (!h || !values || n < gwhip::Tuning::count) return GWHIP_ERR_ARG;
+  h->k->tuning().to_array(values);
   return GWHIP_OK;
 }
 
-int gwhip_set_tuning(gwhip_handle* h, int p0_tile, int sigma_tile, int aux_per_wave, int p0_group, int sigma_group) {
-  if (!h || p0_tile < 0 || p0_tile > 2 || sigma_tile < 0 || sigma_tile > 2 ||
-      (aux_per_wave != 1 && aux_per_wave != 2 && aux_per_wave != 4 && aux_per_wave != 8) || p0_group < 1 ||
-      sigma_group < 1)
-    return GWHIP_ERR_ARG;
+int gwhip_set_tuning(gwhip_handle* h, const int* values, int n) {
+  if (!h || !values || n < gwhip::Tuning::count) return GWHIP_ERR_ARG;
   gwhip::Tuning t;
-  t.p0_shape = p0_tile;
-  t.s_shape = sigma_tile;
-  t.qpw = aux_per_wave;
-  t.p0_group = p0_group;
-  t.s_group = sigma_group;
+  if (!t.from_array(values)) return GWHIP_ERR_ARG;
   h->k->set_tuning(t);
   return GWHIP_OK;
 }

@@ -40,6 +40,19 @@ struct KernelStat {
 struct Tuning {
   int p0_shape = 0, s_shape = 0, qpw = 4;
   int p0_group = 1, s_group = 16;  // split precision: row tiles per L2 group of the int8 GEMMs
+  int p0_oz_tile = 0, s_oz_tile = 0;  // split precision: int8 GEMM block tile (0: 32x64, 1: 64x64)
+  static constexpr int count = 7;
+  void to_array(int* v) const {
+    const int a[count] = {p0_shape, s_shape, qpw, p0_group, s_group, p0_oz_tile, s_oz_tile};
+    for (int i = 0; i < count; ++i) v[i] = a[i];
+  }
+  bool from_array(const int* v) {
+    if (v[0] < 0 || v[0] > 2 || v[1] < 0 || v[1] > 2 || (v[2] != 1 && v[2] != 2 && v[2] != 4 && v[2] != 8) ||
+        v[3] < 1 || v[4] < 1 || v[5] < 0 || v[5] > 1 || v[6] < 0 || v[6] > 1)
+      return false;
+    p0_shape = v[0]; s_shape = v[1]; qpw = v[2]; p0_group = v[3]; s_group = v[4]; p0_oz_tile = v[5]; s_oz_tile = v[6];
+    return true;
+  }
 };
 
 // Transient device memory of one evaluation.  A stand-alone kernel owns its own; the kernels
@@ -117,6 +130,11 @@ private:
   int device_ = 0;
   hipStream_t stream_ = nullptr;
   bool own_stream_ = true, own_ws_ = true, timing_ = true;
+  struct Pending {
+    int which;
+    hipEvent_t a, b;
+  };
+  std::vector<Pending> pending_;  // kernel timings read once the stream has drained
   Tuning tune_;
   uint64_t generation_ = 0;  // bumped whenever buffers or launch parameters change
   size_t bytes_ = 0;
@@ -179,9 +197,9 @@ private:
   void p0_gemm(int64_t q, int64_t tc, double2* Pt0, int shape);
   void sigma_gemm(int64_t q, int64_t t0, int64_t tc, int shape);
   void p0_slice(int64_t tc);
-  void p0_gemm_split(int64_t q, int64_t tc, double2* Pt0, int group);
+  void p0_gemm_split(int64_t q, int64_t tc, double2* Pt0, int group, int tile);
   void sigma_slice(int64_t q, int64_t t0, int64_t tc);
-  void sigma_gemm_split(int64_t q, int64_t tc, int group);
+  void sigma_gemm_split(int64_t q, int64_t tc, int group, int tile);
   void stage_p0(const double2* G, bool split);
   void p0_one_q(int64_t q, const double2* G, bool split, double2* Pq);
   void gj_inverse(double2* X, int64_t nmat);
@@ -189,6 +207,7 @@ private:
   void stage_sigma(const double2* G, double2* Sigma, bool split);
   template <typename F>
   void timed(int kstat, F&& launch);
+  void resolve_timings();
 };
 
 }  // namespace gwhip

@@ -294,6 +294,7 @@ void gw_hip_kernel::vjp_device(const double2* G, const double2* Sbar, double2* G
   hipLaunchKernelGGL(nonfinite_kernel, dim3(1024), dim3(256), 0, stream_, Gbar, (int64_t)nsig, flag_);
   GWHIP_CHECK(hipGetLastError());
   GWHIP_CHECK(hipStreamSynchronize(stream_));
+  resolve_timings();
   float ms = 0.f;
   GWHIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
   vjp_ms_ = ms;

@@ -133,6 +133,7 @@ struct OzGemmArgs {
   int tiles_m, tiles_n;
   int64_t nblocks, per_xcd;
   int group_m;  // row tiles (over batch x M) swept together across all column tiles; 0 = heuristic
+  int tile;     // 0: 32 x 64 block tile, two blocks per CU; 1: 64 x 64 with register prefetch, one per CU
 };
 
 // Output layout of v_mfma_i32_16x16x64_i8: register r of lane l holds row
@@ -308,6 +309,185 @@ __global__ __launch_bounds__(256, 2) void zgemm_ozaki_kernel(OzGemmArgs p) {
 // tiles; with many column tiles 16 row tiles of A (~2.5 MB at K = 320, 5 slices) stay in L2.
 inline int ozaki_default_group(int tiles_n) { return tiles_n >= 64 ? 16 : std::max(1, 64 / tiles_n); }
 
+// 64 x 64 block tile, 2 x 2 waves of 32 x 32, one block per CU.  A (15 plane-slices x 64 rows)
+// and B (10 plane-slices x 64 columns) of one K step sit in LDS; the next K step is loaded into
+// registers while the current one is multiplied (240 MFMAs per wave per K step hide the L2
+// latency without a second resident block).  The int32 accumulators are meant for AGPRs.
+constexpr int OZ4_BM = 64, OZ4_BN = 64;
+
+__device__ __forceinline__ void oz64_store(const OzGemmArgs& p, int64_t b, int m, int n, double vr, double vi) {
+  if (m >= p.M || n >= p.N) return;
+  const int no = n / p.nin;
+  double2 v = make_double2(p.alpha.x * vr - p.alpha.y * vi, p.alpha.x * vi + p.alpha.y * vr);
+  double2* dst = p.C + b * p.sCb + (int64_t)m * p.sCm + (int64_t)no * p.sCno + (int64_t)(n - no * p.nin) * p.sCni;
+  if (p.accumulate) {
+    const double2 o = *dst;
+    v.x += o.x;
+    v.y += o.y;
+  }
+  *dst = v;
+}
+
+template <int NS, bool MULTI>
+__global__ __launch_bounds__(256, 1) void zgemm_ozaki64_kernel(OzGemmArgs p) {
+  constexpr int PA = 3, PB = 2;
+  extern __shared__ __attribute__((aligned(16))) int8_t oz_lds[];
+  int8_t* As = oz_lds;                                         // [PA*NS][64 rows][64 B]
+  int8_t* Bs = As + PA * NS * OZ4_BM * 64;                       // [PB*NS][64 cols][64 B]
+  int* eAs = reinterpret_cast<int*>(Bs + PB * NS * OZ4_BN * 64);  // [64]
+  int* eBs = eAs + OZ4_BM;                                       // [64]
+
+  const int64_t hw = blockIdx.x;
+  const int64_t logical = (hw % 8) * p.per_xcd + hw / 8;
+  if (logical >= p.nblocks) return;
+  const int64_t rows = (int64_t)p.batch * p.tiles_m;
+  const int64_t in_group = (int64_t)p.group_m * p.tiles_n;
+  const int64_t grp = logical / in_group;
+  const int64_t first = grp * p.group_m;
+  const int64_t gsize = rows - first < p.group_m ? rows - first : p.group_m;
+  const int64_t idx = logical - grp * in_group;
+  const int64_t rt = first + idx % gsize;
+  const int tn = (int)(idx / gsize);
+  const int64_t b = rt / p.tiles_m;
+  const int tm = (int)(rt - b * p.tiles_m);
+  const int m0 = tm * OZ4_BM, n0 = tn * OZ4_BN;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lrow = lane & 15, lg = lane >> 4;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+
+  // staging: thread -> (row / column, 16-byte piece); all plane-slices of that piece
+  const int s_row = tid >> 2, s_piece = tid & 3;
+  const int8_t* a_ptr = nullptr;
+  if (m0 + s_row < p.M) a_ptr = p.A + b * p.sAb + (int64_t)(m0 + s_row) * p.sAm + 16 * s_piece;
+  const int8_t* b_ptr = nullptr;
+  if (n0 + s_row < p.N) {
+    const int col = n0 + s_row, no = col / p.nin;
+    b_ptr = p.B + b * p.sBb + (int64_t)no * p.sBno + (int64_t)(col - no * p.nin) * p.sBni + 16 * s_piece;
+  }
+  const int a_lds = oz_lds_off(0, OZ4_BM, s_row, s_piece);
+  const int b_lds = oz_lds_off(0, OZ4_BN, s_row, s_piece);
+
+  constexpr int NC = MULTI ? 2 : 1;  // FP64 accumulators across chunks only when there are several
+  double cr[NC][2][4], ci[NC][2][4];
+#pragma unroll
+  for (int x = 0; x < NC; ++x)
+#pragma unroll
+    for (int y = 0; y < 2; ++y)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) { cr[x][y][r] = 0.0; ci[x][y][r] = 0.0; }
+
+  v4i ra[PA * NS], rb[PB * NS];
+  auto load = [&](int c, int ks) {
+#pragma unroll
+    for (int ps = 0; ps < PA * NS; ++ps)
+      ra[ps] = a_ptr ? *reinterpret_cast<const v4i*>(a_ptr + (int64_t)c * p.sAc + (int64_t)ps * p.kc + ks) : v4i{0, 0, 0, 0};
+#pragma unroll
+    for (int ps = 0; ps < PB * NS; ++ps)
+      rb[ps] = b_ptr ? *reinterpret_cast<const v4i*>(b_ptr + (int64_t)c * p.sBc + (int64_t)ps * p.kc + ks) : v4i{0, 0, 0, 0};
+  };
+  auto store = [&]() {
+#pragma unroll
+    for (int ps = 0; ps < PA * NS; ++ps) *reinterpret_cast<v4i*>(As + a_lds + ps * OZ4_BM * 64) = ra[ps];
+#pragma unroll
+    for (int ps = 0; ps < PB * NS; ++ps) *reinterpret_cast<v4i*>(Bs + b_lds + ps * OZ4_BN * 64) = rb[ps];
+  };
+
+  const int steps = p.kc / OZ_KS;
+  load(0, 0);
+  for (int c = 0; c < p.nchunk; ++c) {
+    v4i ar[NS][2][2], ai[NS][2][2];
+#pragma unroll
+    for (int d = 0; d < NS; ++d)
+#pragma unroll
+      for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int y = 0; y < 2; ++y) { ar[d][x][y] = v4i{0, 0, 0, 0}; ai[d][x][y] = v4i{0, 0, 0, 0}; }
+    for (int st = 0; st < steps; ++st) {
+      __syncthreads();
+      store();
+      __syncthreads();
+      // prefetch the next K step (next chunk after the last step of this one)
+      if (st + 1 < steps) load(c, (st + 1) * OZ_KS);
+      else if (c + 1 < p.nchunk) load(c + 1, 0);
+#pragma unroll
+      for (int s = 0; s < NS; ++s) {
+        v4i fa_r[2], fa_i[2], fa_n[2];
+#pragma unroll
+        for (int x = 0; x < 2; ++x) {
+          const int row = wm + 16 * x + lrow;
+          fa_r[x] = *reinterpret_cast<const v4i*>(As + oz_lds_off(0 * NS + s, OZ4_BM, row, lg));
+          fa_i[x] = *reinterpret_cast<const v4i*>(As + oz_lds_off(1 * NS + s, OZ4_BM, row, lg));
+          fa_n[x] = *reinterpret_cast<const v4i*>(As + oz_lds_off(2 * NS + s, OZ4_BM, row, lg));
+        }
+#pragma unroll
+        for (int t = 0; t + s < NS; ++t) {
+#pragma unroll
+          for (int y = 0; y < 2; ++y) {
+            const int col = wn + 16 * y + lrow;
+            const v4i fb_r = *reinterpret_cast<const v4i*>(Bs + oz_lds_off(0 * NS + t, OZ4_BN, col, lg));
+            const v4i fb_i = *reinterpret_cast<const v4i*>(Bs + oz_lds_off(1 * NS + t, OZ4_BN, col, lg));
+#pragma unroll
+            for (int x = 0; x < 2; ++x) {
+              ar[s + t][x][y] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa_r[x], fb_r, ar[s + t][x][y], 0, 0, 0);
+              ai[s + t][x][y] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa_r[x], fb_i, ai[s + t][x][y], 0, 0, 0);
+              ar[s + t][x][y] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa_n[x], fb_i, ar[s + t][x][y], 0, 0, 0);
+              ai[s + t][x][y] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa_i[x], fb_r, ai[s + t][x][y], 0, 0, 0);
+            }
+          }
+        }
+        // keep the scheduler from hoisting every slice's fragment loads (register pressure)
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+    // chunk end: int32 -> FP64 with the (row, column, chunk) power-of-two scale
+    if (tid < OZ4_BM) {
+      const int m = m0 + tid;
+      eAs[tid] = (m < p.M) ? p.eA[b * p.seAb + (int64_t)m * p.seAm + c] : 0;
+    } else if (tid < OZ4_BM + OZ4_BN) {
+      const int n = n0 + tid - OZ4_BM;
+      int e = 0;
+      if (n < p.N) {
+        const int no = n / p.nin;
+        e = p.eB[b * p.seBb + (int64_t)no * p.seBno + (int64_t)(n - no * p.nin) * p.seBni + c];
+      }
+      eBs[tid - OZ4_BM] = e;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+      for (int y = 0; y < 2; ++y)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int row = wm + 16 * x + OZ_ROW(lg, r), col = wn + 16 * y + lrow;
+          double sc = ldexp(1.0, eAs[row] + eBs[col] - 14);
+          double sr = 0.0, si = 0.0;
+#pragma unroll
+          for (int d = 0; d < NS; ++d) {
+            sr = fma((double)ar[d][x][y][r], sc, sr);
+            si = fma((double)ai[d][x][y][r], sc, si);
+            sc *= 0.0078125;  // 2^-7
+          }
+          if (MULTI) {
+            cr[x % NC][y][r] += sr;  // NC == 2 here
+            ci[x % NC][y][r] += si;
+          } else {
+            oz64_store(p, b, m0 + row, n0 + col, sr, si);
+          }
+        }
+    __syncthreads();  // eAs / eBs are rewritten by the next chunk
+  }
+  if (MULTI) {
+#pragma unroll
+    for (int x = 0; x < NC; ++x)
+#pragma unroll
+      for (int y = 0; y < 2; ++y)
+#pragma unroll
+        for (int r = 0; r < 4; ++r)
+          oz64_store(p, b, m0 + wm + 16 * x + OZ_ROW(lg, r), n0 + wn + 16 * y + lrow, cr[x][y][r], ci[x][y][r]);
+  }
+}
+
 inline size_t ozaki_lds_bytes(int ns) {
   return (size_t)3 * OZ_BM * ns * 64 + (OZ_BM + OZ_BN) * sizeof(int);
 }
@@ -317,32 +497,46 @@ inline void oz_allow_lds(K kernel, size_t bytes) {
   if (bytes > 65536) GWHIP_CHECK(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
 }
 
+inline size_t ozaki64_lds_bytes(int ns) {
+  return (size_t)(3 * OZ4_BM + 2 * OZ4_BN) * ns * 64 + (OZ4_BM + OZ4_BN) * sizeof(int);
+}
+
+template <int NS>
+inline void ozaki_launch(const OzGemmArgs& p, int64_t grid, hipStream_t stream) {
+  if (p.tile == 1 && p.nchunk > 1) {
+    const size_t lds = ozaki64_lds_bytes(NS);
+    oz_allow_lds(zgemm_ozaki64_kernel<NS, true>, lds);
+    hipLaunchKernelGGL((zgemm_ozaki64_kernel<NS, true>), dim3((unsigned)grid), dim3(256), lds, stream, p);
+  } else if (p.tile == 1) {
+    const size_t lds = ozaki64_lds_bytes(NS);
+    oz_allow_lds(zgemm_ozaki64_kernel<NS, false>, lds);
+    hipLaunchKernelGGL((zgemm_ozaki64_kernel<NS, false>), dim3((unsigned)grid), dim3(256), lds, stream, p);
+  } else {
+    const size_t lds = ozaki_lds_bytes(NS);
+    oz_allow_lds(zgemm_ozaki_kernel<NS>, lds);
+    hipLaunchKernelGGL(zgemm_ozaki_kernel<NS>, dim3((unsigned)grid), dim3(256), lds, stream, p);
+  }
+}
+
 inline void zgemm_ozaki(OzGemmArgs p, int ns, hipStream_t stream) {
   if (p.M <= 0 || p.N <= 0 || p.batch <= 0) return;
   if (p.kc % OZ_KS != 0 || p.nchunk <= 0 || p.nin <= 0) throw std::runtime_error("zgemm_ozaki: bad chunking");
+  if (p.tile != 0 && p.tile != 1) throw std::runtime_error("zgemm_ozaki: tile must be 0 or 1");
   // exact int32 accumulation: ns pairs at most share one accumulator, 4 products each ... bounded
   if ((double)p.kc * 2.0 * 127.0 * 127.0 * ns > 2147483647.0) throw std::runtime_error("zgemm_ozaki: chunk too long");
-  p.tiles_m = (int)ceil_div(p.M, OZ_BM);
-  p.tiles_n = (int)ceil_div(p.N, OZ_BN);
+  const int BM = p.tile == 1 ? OZ4_BM : OZ_BM, BN = p.tile == 1 ? OZ4_BN : OZ_BN;
+  p.tiles_m = (int)ceil_div(p.M, BM);
+  p.tiles_n = (int)ceil_div(p.N, BN);
   p.nblocks = (int64_t)p.batch * p.tiles_m * p.tiles_n;
   if (p.group_m <= 0) p.group_m = ozaki_default_group(p.tiles_n);
   p.group_m = (int)std::min<int64_t>(p.group_m, (int64_t)p.batch * p.tiles_m);
   p.per_xcd = ceil_div(p.nblocks, 8);
   const int64_t grid = p.per_xcd * 8;
   if (grid > 0x7fffffffLL) throw std::runtime_error("zgemm_ozaki: grid too large");
-  const size_t lds = ozaki_lds_bytes(ns);
-  if (ns == 4) {
-    oz_allow_lds(zgemm_ozaki_kernel<4>, lds);
-    hipLaunchKernelGGL(zgemm_ozaki_kernel<4>, dim3((unsigned)grid), dim3(256), lds, stream, p);
-  } else if (ns == 5) {
-    oz_allow_lds(zgemm_ozaki_kernel<5>, lds);
-    hipLaunchKernelGGL(zgemm_ozaki_kernel<5>, dim3((unsigned)grid), dim3(256), lds, stream, p);
-  } else if (ns == 3) {
-    oz_allow_lds(zgemm_ozaki_kernel<3>, lds);
-    hipLaunchKernelGGL(zgemm_ozaki_kernel<3>, dim3((unsigned)grid), dim3(256), lds, stream, p);
-  } else {
-    throw std::runtime_error("zgemm_ozaki: slices must be 3, 4 or 5");
-  }
+  if (ns == 4) ozaki_launch<4>(p, grid, stream);
+  else if (ns == 5) ozaki_launch<5>(p, grid, stream);
+  else if (ns == 3) ozaki_launch<3>(p, grid, stream);
+  else throw std::runtime_error("zgemm_ozaki: slices must be 3, 4 or 5");
   GWHIP_CHECK(hipGetLastError());
 }
 

@@ -20,6 +20,7 @@ HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.dirname(os.path.ab
 OK, ERR_ARG, ERR_WRONG_GRID, ERR_STATE, ERR_OOM, ERR_DEVICE, ERR_NONFINITE, ERR_UNSUPPORTED = range(8)
 PREC_FP64, PREC_SPLIT = 0, 1
 VJP_RECOMPUTE, VJP_STORED = 0, 1
+TUNING_KEYS = ("p0_tile", "sigma_tile", "aux_per_wave", "p0_group", "sigma_group", "p0_split_tile", "sigma_split_tile")
 REGIONS = ("P0_zgemm", "P0(t) -> P0(w)", "GW-BSE", "P(w) -> P(t)", "Selfenergy_zgemm", "total")
 KSTATS = ("P0 GEMM", "Sigma GEMM", "P0 prep", "Sigma epilogue", "Dyson inverse")
 
@@ -80,8 +81,8 @@ def load() -> ctypes.CDLL:
     lib.gwhip_debug_mfma_peak.argtypes = [ctypes.c_int] * 3 + [ctypes.POINTER(ctypes.c_double)]
     lib.gwhip_autotune.argtypes = [P, P, ctypes.c_int]
     IP = ctypes.POINTER(ctypes.c_int)
-    lib.gwhip_get_tuning.argtypes = [P, IP, IP, IP, IP, IP]
-    lib.gwhip_set_tuning.argtypes = [P] + [ctypes.c_int] * 5
+    lib.gwhip_get_tuning.argtypes = [P, IP, ctypes.c_int]
+    lib.gwhip_set_tuning.argtypes = [P, IP, ctypes.c_int]
     lib.gwhip_batch_create.argtypes = [ctypes.c_int, ctypes.POINTER(Dims), ctypes.c_int, ctypes.POINTER(P)]
     lib.gwhip_batch_destroy.argtypes = [P]
     lib.gwhip_batch_destroy.restype = None
@@ -261,13 +262,19 @@ class GwHipKernel:
         self._check(self.lib.gwhip_autotune(self.h, ctypes.c_void_p(G_ptr), precision))
 
     def tuning(self) -> dict:
-        v = [ctypes.c_int() for _ in range(5)]
-        self._check(self.lib.gwhip_get_tuning(self.h, *[ctypes.byref(x) for x in v]))
-        return dict(p0_tile=v[0].value, sigma_tile=v[1].value, aux_per_wave=v[2].value, p0_group=v[3].value,
-                    sigma_group=v[4].value)
+        v = (ctypes.c_int * len(TUNING_KEYS))()
+        self._check(self.lib.gwhip_get_tuning(self.h, v, len(TUNING_KEYS)))
+        return dict(zip(TUNING_KEYS, list(v)))
 
-    def set_tuning(self, p0_tile: int, sigma_tile: int, aux_per_wave: int, p0_group: int = 1, sigma_group: int = 16):
-        self._check(self.lib.gwhip_set_tuning(self.h, p0_tile, sigma_tile, aux_per_wave, p0_group, sigma_group))
+    def set_tuning(self, **kw):
+        """Change any of TUNING_KEYS (the others keep their current values)."""
+        t = self.tuning()
+        unknown = set(kw) - set(TUNING_KEYS)
+        if unknown:
+            raise KeyError(sorted(unknown))
+        t.update(kw)
+        v = (ctypes.c_int * len(TUNING_KEYS))(*[t[k] for k in TUNING_KEYS])
+        self._check(self.lib.gwhip_set_tuning(self.h, v, len(TUNING_KEYS)))
 
     def vjp_ms(self) -> float:
         return float(self.lib.gwhip_vjp_ms(self.h))

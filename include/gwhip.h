@@ -122,13 +122,16 @@ int gwhip_eval_sigma_device(gwhip_handle* h, const void* G_dev, void* Sigma_dev,
  * Time the candidate launch parameters of this system on its own operands (G in device memory)
  * and keep the fastest: the FP64 GEMM tiles of the P0 and Sigma contractions (0: 64x64,
  * 1: 128x64, 2: 64x128), the aux indices per wave of the P0 orbital kernel (1, 2, 4, 8) and, for
- * GWHIP_PREC_SPLIT, the number of row tiles that sweep the column tiles together in the int8
- * GEMMs' L2-aware tile order (1 .. 64).  Results do not change, only speed.
+ * GWHIP_PREC_SPLIT, the block tile of the int8 GEMMs and the number of row tiles that sweep the
+ * column tiles together in their L2-aware tile order (1 .. 64).  Results do not change, only speed.
  * No reference counterpart. */
 int gwhip_autotune(gwhip_handle* h, const void* G_dev, int precision);
-int gwhip_get_tuning(const gwhip_handle* h, int* p0_tile, int* sigma_tile, int* aux_per_wave, int* p0_group,
-                     int* sigma_group);
-int gwhip_set_tuning(gwhip_handle* h, int p0_tile, int sigma_tile, int aux_per_wave, int p0_group, int sigma_group);
+/* The tuning as an array of GWHIP_TUNING_COUNT ints: [0] FP64 P0 GEMM tile, [1] FP64 Sigma GEMM
+ * tile, [2] aux indices per wave, [3] int8 P0 GEMM row-tile group, [4] int8 Sigma GEMM row-tile
+ * group, [5] int8 P0 GEMM block tile (0: 32x64, 1: 64x64), [6] int8 Sigma GEMM block tile. */
+#define GWHIP_TUNING_COUNT 7
+int gwhip_get_tuning(const gwhip_handle* h, int* values, int n);
+int gwhip_set_tuning(gwhip_handle* h, const int* values, int n);
 
 /* ---- several systems in one graph -------------------------------------------------------
  * A batch holds nsys independent systems (each its own dims, grid, V and G), one stream and one

@@ -96,6 +96,26 @@ def test_four_slices_are_looser_but_bounded():
     assert 1e-10 < rel_err(S_split, S_ref) < 1e-6
 
 
+@pytest.mark.parametrize("p0_group,s_group", [(1, 16), (12, 1), (64, 64)])
+def test_split_tiles_and_orders_give_identical_sigma(p0_group, s_group):
+    """Both int8 block tiles and any L2 tile order produce bit-identical Sigma (exact int32 sums)."""
+    c = CONFIG0
+    G, V, T, d = synthetic(c["nk"], c["nao"], c["NQ"], beta=c["beta"], lam=c["lam"], mesh=c["mesh"])
+    k = _kernel(d, T)
+    try:
+        k.set_grid(T.T_bt, T.T_tb)
+        k.upload_V(V)
+        ref = k.solve(G, gl.PREC_SPLIT)
+        out = []
+        for tile in (0, 1):
+            k.set_tuning(p0_split_tile=tile, sigma_split_tile=tile, p0_group=p0_group, sigma_group=s_group)
+            out.append(k.solve(G, gl.PREC_SPLIT))
+    finally:
+        k.close()
+    for o in out:
+        assert np.array_equal(o, ref)
+
+
 def test_split_reference_fixture():
     G, V, T, d, S_fix = load_fixture("gw_x2c")
     S_split, _, S_ref = _split_vs_oracle(G, V, T, d)
