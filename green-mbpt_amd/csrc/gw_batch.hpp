@@ -67,6 +67,8 @@ public:
     for (int i = 0; i < size(); ++i) {
       key.push_back(G[i]);
       key.push_back(S[i]);
+      // a graph holds the P0 tile set of each system's G (upper tiles only when G is Hermitian)
+      key.push_back(systems_[i]->detect_hermitian_G(G[i]) ? (const void*)1 : nullptr);
     }
     const uint64_t gen = gen_sum();
     GraphEntry* e = nullptr;

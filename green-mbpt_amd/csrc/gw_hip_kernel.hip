@@ -99,6 +99,7 @@ gw_hip_kernel::gw_hip_kernel(const Dims& d, int device, hipStream_t shared_strea
   kplus_tab_ = dalloc<int>((size_t)nk * nk);
   kminus_tab_ = dalloc<int>((size_t)nk * nk);
   flag_ = dalloc<int>(1);
+  hcheck_ = dalloc<unsigned long long>(2);
   build_maps();
   if (!own_ws_) return;  // a batch binds a shared workspace later
 
@@ -124,7 +125,7 @@ gw_hip_kernel::~gw_hip_kernel() {
   (void)hipSetDevice(device_);
   if (stream_) (void)hipStreamSynchronize(stream_);
   for (void* p : {(void*)Vq_, (void*)Vt_, (void*)P_, (void*)Tbt_fold_, (void*)Ttb_, (void*)G_dev_, (void*)S_dev_,
-                  (void*)kplus_tab_, (void*)kminus_tab_, (void*)flag_})
+                  (void*)kplus_tab_, (void*)kminus_tab_, (void*)flag_, (void*)hcheck_})
     if (p) (void)hipFree(p);
   if (own_ws_)
     for (void* p : {(void*)Pw_, (void*)X_, (void*)Y_, (void*)scratch_})
@@ -255,7 +256,7 @@ size_t gw_hip_kernel::scratch_per_t() const { return (size_t)d_.nk * d_.NQ * d_.
 size_t gw_hip_kernel::dyson_per_q() const { return (size_t)3 * d_.nw_b * d_.NQ * d_.NQ; }
 size_t gw_hip_kernel::slice_bytes_per_t() const {
   const size_t kcp = (size_t)ceil_div(d_.nao * d_.nao, 64) * 64, kcs = (size_t)ceil_div(d_.NQ, 64) * 64;
-  return std::max((size_t)d_.NQ * d_.nk * 3 * split_ns_ * kcp, (size_t)d_.NQ * 3 * split_ns_ * kcs);
+  return std::max((size_t)d_.NQ * d_.nk * 2 * split_ns_ * kcp, (size_t)d_.NQ * 2 * split_ns_ * kcs);
 }
 size_t gw_hip_kernel::exps_per_t() const { return (size_t)d_.NQ * d_.nk; }
 
@@ -279,6 +280,22 @@ void gw_hip_kernel::bind_workspace(const Workspace& ws) {
   Zs_ = Ps_ = ws.slices;
   eZs_ = ePs_ = ws.exps;
   ++generation_;
+}
+
+bool gw_hip_kernel::detect_hermitian_G(const double2* G) {
+  GWHIP_CHECK(hipSetDevice(device_));
+  hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, stream_, reinterpret_cast<int*>(hcheck_), (int64_t)4);
+  const int64_t nmat = d_.nts * d_.ns * d_.nk;
+  hipLaunchKernelGGL(herm_check_kernel, dim3(1024), dim3(256), 0, stream_, G, nmat, (int)d_.nso, hcheck_);
+  GWHIP_CHECK(hipGetLastError());
+  unsigned long long v[2] = {0, 0};
+  GWHIP_CHECK(hipMemcpyAsync(v, hcheck_, sizeof(v), hipMemcpyDeviceToHost, stream_));
+  GWHIP_CHECK(hipStreamSynchronize(stream_));
+  double dmax, amax;
+  std::memcpy(&dmax, &v[0], sizeof(double));
+  std::memcpy(&amax, &v[1], sizeof(double));
+  g_herm_ = std::isfinite(dmax) && dmax <= 1e-12 * amax;
+  return g_herm_;
 }
 
 void gw_hip_kernel::check_ready(int precision) const {
@@ -308,8 +325,8 @@ void gw_hip_kernel::ensure_slices() {
   Vs_s_ = dalloc<int8_t>((size_t)nk * nk * nao2 * 2 * ns * kc_s_);
   eVs_s_ = dalloc<int>((size_t)nk * nk * nao2);
   if (own_ws_) {  // Z slices (P0 stage) and P-tilde slices (Sigma stage) share one buffer
-    Zs_ = Ps_ = dalloc<int8_t>(std::max((size_t)tchunk_p0_ * NQ * nk * 3 * ns * kc_p0_,
-                                        (size_t)tchunk_sigma_ * NQ * 3 * ns * kc_s_));
+    Zs_ = Ps_ = dalloc<int8_t>(std::max((size_t)tchunk_p0_ * NQ * nk * 2 * ns * kc_p0_,
+                                        (size_t)tchunk_sigma_ * NQ * 2 * ns * kc_s_));
     eZs_ = ePs_ = dalloc<int>(std::max((size_t)tchunk_p0_ * NQ * nk, (size_t)tchunk_sigma_ * NQ));
   }
   {  // P0 operand: rows (q, Q'), chunks k, elements (t, n) of V(k, k+q)
@@ -414,17 +431,33 @@ void launch_z_prep(dim3 grid, size_t lds, hipStream_t st, const DevDims& dd, con
                    double2* Z, const int* kplus, int t0, int QPW) {
   launch_z_prep_t<NB, double>(grid, lds, st, dd, G, Vt, Z, kplus, t0, QPW);
 }
+template <int NB, int NS>
+void launch_z_prep_slice_t(dim3 grid, size_t lds, hipStream_t st, const DevDims& dd, const double2* G, const double2* Vt,
+                           int8_t* dst, int* exps, int kc, const int* kplus, int t0, int QPW) {
+  allow_lds(z_prep_slice_kernel<NB, NS>, lds);
+  hipLaunchKernelGGL((z_prep_slice_kernel<NB, NS>), grid, dim3(256), lds, st, dd, G, Vt, dst, exps, kc, kplus, t0, QPW);
+  GWHIP_CHECK(hipGetLastError());
+}
+template <int NB>
+void launch_z_prep_slice(int ns, dim3 grid, size_t lds, hipStream_t st, const DevDims& dd, const double2* G,
+                         const double2* Vt, int8_t* dst, int* exps, int kc, const int* kplus, int t0, int QPW) {
+  if (ns == 5) launch_z_prep_slice_t<NB, 5>(grid, lds, st, dd, G, Vt, dst, exps, kc, kplus, t0, QPW);
+  else if (ns == 4) launch_z_prep_slice_t<NB, 4>(grid, lds, st, dd, G, Vt, dst, exps, kc, kplus, t0, QPW);
+  else launch_z_prep_slice_t<NB, 3>(grid, lds, st, dd, G, Vt, dst, exps, kc, kplus, t0, QPW);
+}
 template <int NB, typename T>
 void launch_sigma_epi_t(dim3 grid, size_t lds, hipStream_t st, const DevDims& dd, const double2* G, const double2* Vt,
-                        const double2* R, double2* S, const int* kminus, int t0, double scale) {
+                        const double2* R, double2* S, const int* kminus, int t0, double scale, int upper) {
   allow_lds(sigma_epi_mfma_kernel<NB, T>, lds);
-  hipLaunchKernelGGL((sigma_epi_mfma_kernel<NB, T>), grid, dim3(256), lds, st, dd, G, Vt, R, S, kminus, t0, scale);
+  hipLaunchKernelGGL((sigma_epi_mfma_kernel<NB, T>), grid, dim3(256), lds, st, dd, G, Vt, R, S, kminus, t0, scale,
+                     upper);
   GWHIP_CHECK(hipGetLastError());
 }
 template <int NB>
 void launch_sigma_epi(dim3 grid, size_t lds, hipStream_t st, const DevDims& dd, const double2* G,
-                      const double2* Vt, const double2* R, double2* S, const int* kminus, int t0, double scale) {
-  launch_sigma_epi_t<NB, double>(grid, lds, st, dd, G, Vt, R, S, kminus, t0, scale);
+                      const double2* Vt, const double2* R, double2* S, const int* kminus, int t0, double scale,
+                      int upper) {
+  launch_sigma_epi_t<NB, double>(grid, lds, st, dd, G, Vt, R, S, kminus, t0, scale, upper);
 }
 }  // namespace
 
@@ -448,6 +481,21 @@ void gw_hip_kernel::p0_prep(int64_t q, const double2* G, int64_t t0, int64_t tc,
   else launch_z_prep<3>(grid, lds, stream_, dd, G, Vtq, scratch_, kp, (int)t0, qpw);
 }
 
+// Split precision, one spin block: Z slices for q and tau in [t0, t0 + tc) straight into Zs_.
+void gw_hip_kernel::p0_prep_slice(int64_t q, const double2* G, int64_t t0, int64_t tc, int qpw) {
+  const int64_t nk = d_.nk, NQ = d_.NQ, nao2 = d_.nao * d_.nao;
+  DevDims dd{(int)d_.nts, (int)d_.ns, (int)nk, (int)d_.nso, (int)d_.nao, (int)NQ, (int)d_.pns, (int)d_.pseudo_ns};
+  const int NB = (int)ceil_div(d_.nao, 16);
+  const size_t lds = (size_t)4 * (16 * NB) * (16 * NB + 1) * sizeof(double);
+  const double2* Vtq = Vt_ + q * nk * NQ * nao2;
+  const dim3 grid((unsigned)ceil_div(NQ, 4 * qpw), (unsigned)nk, (unsigned)tc);
+  const int* kp = kplus_tab_ + q * nk;
+  const int ns = split_ns_;
+  if (NB == 1) launch_z_prep_slice<1>(ns, grid, lds, stream_, dd, G, Vtq, Zs_, eZs_, (int)kc_p0_, kp, (int)t0, qpw);
+  else if (NB == 2) launch_z_prep_slice<2>(ns, grid, lds, stream_, dd, G, Vtq, Zs_, eZs_, (int)kc_p0_, kp, (int)t0, qpw);
+  else launch_z_prep_slice<3>(ns, grid, lds, stream_, dd, G, Vtq, Zs_, eZs_, (int)kc_p0_, kp, (int)t0, qpw);
+}
+
 // P0(t0 .. t0+tc) = -(pref/nk) Z conj(V)^T, FP64 (tile `shape`).
 void gw_hip_kernel::p0_gemm(int64_t q, int64_t tc, double2* Pt0, int shape) {
   const int64_t nk = d_.nk, NQ = d_.NQ, nao2 = d_.nao * d_.nao, NQ2 = NQ * NQ;
@@ -459,6 +507,7 @@ void gw_hip_kernel::p0_gemm(int64_t q, int64_t tc, double2* Pt0, int shape) {
   a.C = Pt0; a.sCb = NQ2; a.sCm = NQ; a.sCno = 0; a.sCni = 1;
   a.alpha = make_double2(-d_.prefactor / (double)nk, 0.0);
   a.accumulate = 0; a.conjA = 0; a.conjB = 1;
+  a.upper = g_herm_ ? 1 : 0;
   zgemm_f64(a, true, true, stream_, shape);
 }
 
@@ -481,7 +530,7 @@ void gw_hip_kernel::p0_slice(int64_t tc) {
   SliceArgs sa{};
   sa.src = scratch_; sa.nrows = tc * NQ; sa.row_div = NQ; sa.s_rhi = nk * NQ * nao2; sa.s_rlo = nao2;
   sa.nchunk = (int)nk; sa.s_c = NQ * nao2; sa.klen = (int)nao2; sa.s_k = 1; sa.kc = (int)kc_p0_;
-  sa.conj = 0; sa.nplanes = 3; sa.dst = Zs_; sa.exps = eZs_;
+  sa.conj = 0; sa.nplanes = 2; sa.dst = Zs_; sa.exps = eZs_;
   slice_rows(sa, split_ns_, stream_);
 }
 
@@ -491,7 +540,7 @@ void gw_hip_kernel::p0_gemm_split(int64_t q, int64_t tc, double2* Pt0, int group
   const int ns = split_ns_;
   OzGemmArgs o{};
   o.M = (int)NQ; o.N = (int)NQ; o.batch = (int)tc; o.kc = (int)kc_p0_; o.nchunk = (int)nk; o.nin = (int)NQ;
-  o.A = Zs_; o.sAm = nk * 3 * ns * kc_p0_; o.sAb = NQ * o.sAm; o.sAc = 3 * ns * kc_p0_;
+  o.A = Zs_; o.sAm = nk * 2 * ns * kc_p0_; o.sAb = NQ * o.sAm; o.sAc = 2 * ns * kc_p0_;
   o.eA = eZs_; o.seAb = NQ * nk; o.seAm = nk;
   o.B = Vs_p0_ + q * NQ * nk * 2 * ns * kc_p0_; o.sBb = 0; o.sBno = 0; o.sBni = nk * 2 * ns * kc_p0_;
   o.sBc = 2 * ns * kc_p0_;
@@ -501,6 +550,7 @@ void gw_hip_kernel::p0_gemm_split(int64_t q, int64_t tc, double2* Pt0, int group
   o.accumulate = 0;
   o.group_m = group;
   o.tile = tile;
+  o.upper = g_herm_ ? 1 : 0;
   zgemm_ozaki(o, ns, stream_);
 }
 
@@ -510,7 +560,7 @@ void gw_hip_kernel::sigma_slice(int64_t q, int64_t t0, int64_t tc) {
   SliceArgs sa{};
   sa.src = P_ + (q * nts + t0) * NQ2; sa.nrows = tc * NQ; sa.row_div = tc * NQ; sa.s_rhi = 0; sa.s_rlo = NQ;
   sa.nchunk = 1; sa.s_c = 0; sa.klen = (int)NQ; sa.s_k = 1; sa.kc = (int)kc_s_;
-  sa.conj = 0; sa.nplanes = 3; sa.dst = Ps_; sa.exps = ePs_;
+  sa.conj = 0; sa.nplanes = 2; sa.dst = Ps_; sa.exps = ePs_;
   slice_rows(sa, split_ns_, stream_);
 }
 
@@ -521,7 +571,7 @@ void gw_hip_kernel::sigma_gemm_split(int64_t q, int64_t tc, int group, int tile)
   const int qn = neg_q_[q];
   OzGemmArgs o{};
   o.M = (int)NQ; o.N = (int)(nk * nao2); o.batch = (int)tc; o.kc = (int)kc_s_; o.nchunk = 1; o.nin = (int)nao2;
-  o.A = Ps_; o.sAm = 3 * ns * kc_s_; o.sAb = NQ * o.sAm; o.sAc = 0;
+  o.A = Ps_; o.sAm = 2 * ns * kc_s_; o.sAb = NQ * o.sAm; o.sAc = 0;
   o.eA = ePs_; o.seAb = NQ; o.seAm = 1;
   o.B = Vs_s_ + (int64_t)qn * nk * nao2 * 2 * ns * kc_s_; o.sBb = 0; o.sBno = nao2 * 2 * ns * kc_s_;
   o.sBni = 2 * ns * kc_s_; o.sBc = 0;
@@ -543,17 +593,26 @@ void gw_hip_kernel::p0_one_q(int64_t q, const double2* G, bool split, double2* P
   const int64_t nk = d_.nk, NQ = d_.NQ, nao2 = d_.nao * d_.nao, NQ2 = NQ * NQ, half = d_.nts / 2;
   for (int64_t t0 = 0; t0 < half; t0 += tchunk_p0_) {
     const int64_t tc = std::min(tchunk_p0_, half - t0);
-    timed(K_P0_PREP, [&] { p0_prep(q, G, t0, tc, tune_.qpw); });
+    if (split && d_.pseudo_ns == 1 && tune_.p0_fused) {
+      timed(K_P0_PREP, [&] { p0_prep_slice(q, G, t0, tc, tune_.qpw); });
+    } else {
+      timed(K_P0_PREP, [&] { p0_prep(q, G, t0, tc, tune_.qpw); });
+      if (split) p0_slice(tc);
+    }
     if (split) {
-      p0_slice(tc);
       timed(K_P0_GEMM, [&] { p0_gemm_split(q, tc, Pq + t0 * NQ2, tune_.p0_group, tune_.p0_oz_tile); });
     } else {
       timed(K_P0_GEMM, [&] { p0_gemm(q, tc, Pq + t0 * NQ2, tune_.p0_shape); });
     }
   }
-  // Hermitise the half grid of this q (src/gw_cpu_kernel.cpp:217-225)
-  hipLaunchKernelGGL(hermitize_kernel, dim3((unsigned)ceil_div(NQ2, 256), (unsigned)half), dim3(256), 0, stream_, Pq,
-                     (int)NQ, NQ2);
+  // Hermitise the half grid of this q (src/gw_cpu_kernel.cpp:217-225); for Hermitian G the
+  // product is Hermitian and only its upper tiles were computed
+  if (g_herm_)
+    hipLaunchKernelGGL(mirror_upper_kernel, dim3((unsigned)ceil_div(NQ2, 256), (unsigned)half), dim3(256), 0, stream_,
+                       Pq, (int)NQ, NQ2);
+  else
+    hipLaunchKernelGGL(hermitize_kernel, dim3((unsigned)ceil_div(NQ2, 256), (unsigned)half), dim3(256), 0, stream_, Pq,
+                       (int)NQ, NQ2);
   GWHIP_CHECK(hipGetLastError());
 }
 
@@ -659,11 +718,27 @@ void gw_hip_kernel::stage_sigma(const double2* G, double2* Sigma, bool split) {
       timed(K_SIGMA_EPI, [&] {
         const dim3 grid((unsigned)nk, (unsigned)tc);
         const int* km = kminus_tab_ + q * nk;
-        if (NB == 1) launch_sigma_epi<1>(grid, lds, stream_, dd, G, Vneg, scratch_, Sigma, km, (int)t0, scale);
-        else if (NB == 2) launch_sigma_epi<2>(grid, lds, stream_, dd, G, Vneg, scratch_, Sigma, km, (int)t0, scale);
-        else launch_sigma_epi<3>(grid, lds, stream_, dd, G, Vneg, scratch_, Sigma, km, (int)t0, scale);
+        const int up = sigma_upper() ? 1 : 0;
+        if (NB == 1) launch_sigma_epi<1>(grid, lds, stream_, dd, G, Vneg, scratch_, Sigma, km, (int)t0, scale, up);
+        else if (NB == 2) launch_sigma_epi<2>(grid, lds, stream_, dd, G, Vneg, scratch_, Sigma, km, (int)t0, scale, up);
+        else launch_sigma_epi<3>(grid, lds, stream_, dd, G, Vneg, scratch_, Sigma, km, (int)t0, scale, up);
       });
     }
+  }
+}
+
+// Sigma of Hermitian G is Hermitian; with one spin block per (tau, s, k) only its upper 16 x 16
+// blocks were accumulated, so fill the rest from them.
+bool gw_hip_kernel::sigma_upper() const { return g_herm_ && d_.pns == 1; }
+
+void gw_hip_kernel::finish_sigma(double2* Sigma) {
+  if (!sigma_upper()) return;
+  const int64_t n = d_.nso, nmat = d_.nts * d_.ns * d_.nk;
+  for (int64_t m0 = 0; m0 < nmat; m0 += 65535) {  // grid.y limit
+    const int64_t mc = std::min<int64_t>(65535, nmat - m0);
+    hipLaunchKernelGGL(mirror_upper_kernel, dim3((unsigned)ceil_div(n * n, 256), (unsigned)mc), dim3(256), 0, stream_,
+                       Sigma + m0 * n * n, (int)n, n * n);
+    GWHIP_CHECK(hipGetLastError());
   }
 }
 
@@ -674,6 +749,7 @@ void gw_hip_kernel::enqueue_eval(const double2* G, double2* Sigma, bool split, i
   stage_p0(G, split);
   stage_dyson();
   stage_sigma(G, Sigma, split);
+  finish_sigma(Sigma);
   if (nonfinite_flag) {
     hipLaunchKernelGGL(nonfinite_kernel, dim3(1024), dim3(256), 0, stream_, Sigma, nsig, nonfinite_flag);
     GWHIP_CHECK(hipGetLastError());
@@ -688,6 +764,7 @@ void gw_hip_kernel::solve_device(const double2* G, double2* Sigma, int precision
   if (split) ensure_slices();
   timing_ = true;
   resolve_timings();
+  detect_hermitian_G(G);
   for (double& r : region_ms_) r = 0.0;
   for (auto& k : kstat_) k = KernelStat{};
   const int64_t nsig = d_.nts * d_.ns * d_.nk * d_.nso * d_.nso;
@@ -708,6 +785,7 @@ void gw_hip_kernel::solve_device(const double2* G, double2* Sigma, int precision
   stage_dyson();
   GWHIP_CHECK(hipEventRecord(t2, stream_));
   stage_sigma(G, Sigma, split);
+  finish_sigma(Sigma);
   GWHIP_CHECK(hipEventRecord(t3, stream_));
   hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, stream_, flag_, (int64_t)1);
   hipLaunchKernelGGL(nonfinite_kernel, dim3(1024), dim3(256), 0, stream_, Sigma, nsig, flag_);
@@ -733,6 +811,7 @@ void gw_hip_kernel::autotune(const double2* G, int precision) {
   GWHIP_CHECK(hipSetDevice(device_));
   const bool split = precision == GWHIP_PREC_SPLIT;
   if (split) ensure_slices();
+  detect_hermitian_G(G);
   const int64_t half = d_.nts / 2, tcp = std::min(tchunk_p0_, half), tcs = std::min(tchunk_sigma_, d_.nts);
   hipEvent_t a, b;
   GWHIP_CHECK(hipEventCreate(&a));
@@ -768,6 +847,17 @@ void gw_hip_kernel::autotune(const double2* G, int precision) {
     tune_.p0_shape = best_of(3, shapes, [&](int c) { p0_gemm(0, tcp, P_, c); });
     tune_.s_shape = best_of(3, shapes, [&](int c) { sigma_gemm(0, 0, tcs, c); });
   } else {
+    if (d_.pseudo_ns == 1) {  // fused prep + slicing vs the two kernels
+      const int fz[2] = {0, 1};
+      tune_.p0_fused = best_of(2, fz, [&](int c) {
+        if (c) {
+          p0_prep_slice(0, G, 0, tcp, tune_.qpw);
+        } else {
+          p0_prep(0, G, 0, tcp, tune_.qpw);
+          p0_slice(tcp);
+        }
+      });
+    }
     // candidates (block tile, row-tile group) encoded as 100 * tile + group
     const int cands[14] = {1, 2, 4, 8, 16, 32, 64, 101, 102, 104, 108, 116, 132, 164};
     p0_prep(0, G, 0, tcp, tune_.qpw);
@@ -805,6 +895,7 @@ void gw_hip_kernel::run_stages_host(const double2* G, int last_stage) {
   if (!G_dev_) G_dev_ = dalloc<double2>(nsig);
   GWHIP_CHECK(hipMemcpy(G_dev_, G, nsig * sizeof(double2), hipMemcpyHostToDevice));
   stored_valid_ = false;
+  detect_hermitian_G(G_dev_);
   GWHIP_CHECK(hipMemsetAsync(P_, 0, (size_t)d_.nk * d_.nts * d_.NQ * d_.NQ * sizeof(double2), stream_));
   stage_p0(G_dev_, false);
   if (last_stage >= 2) stage_dyson();
@@ -1028,7 +1119,7 @@ void debug_zgemm_split(int M, int N, int K, const double2* A, const double2* B, 
   GWHIP_CHECK(hipMalloc(&dA, (size_t)M * K * sizeof(double2)));
   GWHIP_CHECK(hipMalloc(&dB, (size_t)K * N * sizeof(double2)));
   GWHIP_CHECK(hipMalloc(&dC, (size_t)M * N * sizeof(double2)));
-  GWHIP_CHECK(hipMalloc(&sA, (size_t)M * 3 * ns * kc));
+  GWHIP_CHECK(hipMalloc(&sA, (size_t)M * 2 * ns * kc));
   GWHIP_CHECK(hipMalloc(&sB, (size_t)N * 2 * ns * kc));
   GWHIP_CHECK(hipMalloc(&eA, (size_t)M * sizeof(int)));
   GWHIP_CHECK(hipMalloc(&eB, (size_t)N * sizeof(int)));
@@ -1036,7 +1127,7 @@ void debug_zgemm_split(int M, int N, int K, const double2* A, const double2* B, 
   GWHIP_CHECK(hipMemcpy(dB, B, (size_t)K * N * sizeof(double2), hipMemcpyHostToDevice));
   SliceArgs a{};
   a.src = dA; a.nrows = M; a.row_div = M; a.s_rhi = 0; a.s_rlo = K; a.nchunk = 1; a.s_c = 0; a.klen = K; a.s_k = 1;
-  a.kc = kc; a.conj = 0; a.nplanes = 3; a.dst = sA; a.exps = eA;
+  a.kc = kc; a.conj = 0; a.nplanes = 2; a.dst = sA; a.exps = eA;
   slice_rows(a, ns, nullptr);
   SliceArgs bsl{};
   bsl.src = dB; bsl.nrows = N; bsl.row_div = N; bsl.s_rhi = 0; bsl.s_rlo = 1; bsl.nchunk = 1; bsl.s_c = 0; bsl.klen = K;
@@ -1044,7 +1135,7 @@ void debug_zgemm_split(int M, int N, int K, const double2* A, const double2* B, 
   slice_rows(bsl, ns, nullptr);
   OzGemmArgs o{};
   o.M = M; o.N = N; o.batch = 1; o.kc = kc; o.nchunk = 1; o.nin = N;
-  o.A = sA; o.sAm = 3 * ns * kc; o.eA = eA; o.seAm = 1;
+  o.A = sA; o.sAm = 2 * ns * kc; o.eA = eA; o.seAm = 1;
   o.B = sB; o.sBni = 2 * ns * kc; o.eB = eB; o.seBni = 1;
   o.C = dC; o.sCm = N; o.sCni = 1;
   o.alpha = make_double2(1.0, 0.0);

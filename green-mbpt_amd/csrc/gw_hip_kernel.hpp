@@ -35,22 +35,25 @@ struct KernelStat {
 
 // Launch parameters picked by the autotuner (defaults = the best measured at nk = 64, nao = 26,
 // naux = 300): FP64 GEMM tile of the P0 and Sigma contractions (0: 64x64, 1: 128x64,
-// 2: 64x128), the number of aux indices per wave of the P0 prep kernel, and the row-tile group
-// of the split-precision GEMMs' L2-aware tile order.
+// 2: 64x128), the number of aux indices per wave of the P0 prep kernel, and for split precision
+// the int8 GEMM block tile, the row-tile group of its L2-aware tile order and whether the P0
+// orbital kernel writes the int8 slices itself.
 struct Tuning {
   int p0_shape = 0, s_shape = 0, qpw = 4;
   int p0_group = 1, s_group = 16;  // split precision: row tiles per L2 group of the int8 GEMMs
   int p0_oz_tile = 0, s_oz_tile = 0;  // split precision: int8 GEMM block tile (0: 32x64, 1: 64x64)
-  static constexpr int count = 7;
+  int p0_fused = 1;  // split precision, one spin block: P0 orbital kernel writes int8 slices directly
+  static constexpr int count = 8;
   void to_array(int* v) const {
-    const int a[count] = {p0_shape, s_shape, qpw, p0_group, s_group, p0_oz_tile, s_oz_tile};
+    const int a[count] = {p0_shape, s_shape, qpw, p0_group, s_group, p0_oz_tile, s_oz_tile, p0_fused};
     for (int i = 0; i < count; ++i) v[i] = a[i];
   }
   bool from_array(const int* v) {
     if (v[0] < 0 || v[0] > 2 || v[1] < 0 || v[1] > 2 || (v[2] != 1 && v[2] != 2 && v[2] != 4 && v[2] != 8) ||
-        v[3] < 1 || v[4] < 1 || v[5] < 0 || v[5] > 1 || v[6] < 0 || v[6] > 1)
+        v[3] < 1 || v[4] < 1 || v[5] < 0 || v[5] > 1 || v[6] < 0 || v[6] > 1 || v[7] < 0 || v[7] > 1)
       return false;
     p0_shape = v[0]; s_shape = v[1]; qpw = v[2]; p0_group = v[3]; s_group = v[4]; p0_oz_tile = v[5]; s_oz_tile = v[6];
+    p0_fused = v[7];
     return true;
   }
 };
@@ -109,6 +112,9 @@ public:
   // allocation (safe under stream capture once prepare_split / bind_workspace are done).
   void enqueue_eval(const double2* G_dev, double2* Sigma_dev, bool split, int* nonfinite_flag);
   void check_ready(int precision) const;
+  // Whether G is Hermitian per (tau, spin, k) to 1e-12 relative; when it is, only the upper block
+  // tiles of P0 are computed and mirrored.  Runs on the stream and synchronises.
+  bool detect_hermitian_G(const double2* G_dev);
   void autotune(const double2* G_dev, int precision);
   const Tuning& tuning() const { return tune_; }
   void set_tuning(const Tuning& t) {
@@ -137,6 +143,8 @@ private:
   std::vector<Pending> pending_;  // kernel timings read once the stream has drained
   Tuning tune_;
   uint64_t generation_ = 0;  // bumped whenever buffers or launch parameters change
+  bool g_herm_ = false;      // G of the evaluation being enqueued is Hermitian
+  unsigned long long* hcheck_ = nullptr;
   size_t bytes_ = 0;
   int max_lds_ = 65536;
   bool grid_set_ = false, v_set_ = false;
@@ -164,9 +172,9 @@ private:
   int* eVs_p0_ = nullptr;     // [nq][NQ][nk]
   int8_t* Vs_s_ = nullptr;    // [nq][nk][nao^2][2][ns][kc_s]
   int* eVs_s_ = nullptr;      // [nq][nk][nao^2]
-  int8_t* Zs_ = nullptr;      // [tc][NQ][nk][3][ns][kc_p0]  (P0 stage)
+  int8_t* Zs_ = nullptr;      // [tc][NQ][nk][2][ns][kc_p0]  (P0 stage)
   int* eZs_ = nullptr;
-  int8_t* Ps_ = nullptr;      // [tc][NQ][3][ns][kc_s]       (Sigma stage; same memory as Zs_)
+  int8_t* Ps_ = nullptr;      // [tc][NQ][2][ns][kc_s]       (Sigma stage; same memory as Zs_)
   int* ePs_ = nullptr;
   int64_t qchunk_ = 1, tchunk_p0_ = 1, tchunk_sigma_ = 1;
   // VJP: per-q buffers, a second orbital scratch, and the kept half-grid P0 of the last
@@ -194,6 +202,7 @@ private:
   void free_vjp_buffers();
   void zero(double2* X, int64_t n);
   void p0_prep(int64_t q, const double2* G, int64_t t0, int64_t tc, int qpw);
+  void p0_prep_slice(int64_t q, const double2* G, int64_t t0, int64_t tc, int qpw);
   void p0_gemm(int64_t q, int64_t tc, double2* Pt0, int shape);
   void sigma_gemm(int64_t q, int64_t t0, int64_t tc, int shape);
   void p0_slice(int64_t tc);
@@ -205,6 +214,8 @@ private:
   void gj_inverse(double2* X, int64_t nmat);
   void stage_dyson();
   void stage_sigma(const double2* G, double2* Sigma, bool split);
+  bool sigma_upper() const;
+  void finish_sigma(double2* Sigma);
   template <typename F>
   void timed(int kstat, F&& launch);
   void resolve_timings();

@@ -36,6 +36,46 @@ __global__ void hermitize_kernel(double2* X, int N, int64_t stride) {
   M[(int64_t)j * N + i] = make_double2(h.x, -h.y);
 }
 
+// Upper triangle -> Hermitian matrix: X[i][j] = conj(X[j][i]) below the diagonal, real diagonal.
+// Used when only the upper tiles of a Hermitian product were computed (the result equals
+// (X + X^H)/2 up to rounding when X is Hermitian, which it is for Hermitian G).
+__global__ void mirror_upper_kernel(double2* X, int N, int64_t stride) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)N * N) return;
+  const int i = (int)(e / N), j = (int)(e - (e / N) * N);
+  if (i < j) return;
+  double2* M = X + (int64_t)blockIdx.y * stride;
+  if (i == j) {
+    M[(int64_t)i * N + i].y = 0.0;
+  } else {
+    const double2 u = M[(int64_t)j * N + i];
+    M[(int64_t)i * N + j] = make_double2(u.x, -u.y);
+  }
+}
+
+// max_(i,j) |X[i][j] - conj(X[j][i])| and max |X| over nmat N x N matrices, as the bit patterns
+// of non-negative doubles (ordered like the values) combined with atomicMax.
+__global__ void herm_check_kernel(const double2* X, int64_t nmat, int N, unsigned long long* out) {
+  const int64_t n2 = (int64_t)N * N, total = nmat * n2;
+  const int64_t step = (int64_t)gridDim.x * blockDim.x;
+  double dmax = 0.0, amax = 0.0;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < total; e += step) {
+    const int64_t m = e / n2, r = e - m * n2;
+    const int i = (int)(r / N), j = (int)(r - (r / N) * N);
+    const double2 a = X[e], b = X[m * n2 + (int64_t)j * N + i];
+    dmax = fmax(dmax, fmax(fabs(a.x - b.x), fabs(a.y + b.y)));
+    amax = fmax(amax, fmax(fabs(a.x), fabs(a.y)));
+  }
+  for (int o = 32; o > 0; o >>= 1) {
+    dmax = fmax(dmax, __shfl_xor(dmax, o, 64));
+    amax = fmax(amax, __shfl_xor(amax, o, 64));
+  }
+  if ((threadIdx.x & 63) == 0) {
+    atomicMax(out + 0, (unsigned long long)__double_as_longlong(dmax));
+    atomicMax(out + 1, (unsigned long long)__double_as_longlong(amax));
+  }
+}
+
 // P0 half-grid symmetrisation for all q: Hermitise P[q][t] for t < nts/2
 // (src/gw_cpu_kernel.cpp:217-225; the /nk and the prefactor are applied by the GEMM).
 // The mirror P0(beta - tau) = P0(tau) is folded into the tau -> Omega matrix instead.

@@ -128,6 +128,7 @@ __global__ __launch_bounds__(256) void z_prep_mfma_kernel(DevDims d, const doubl
         for (int bp = 0; bp < NB; ++bp)
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
+            if (sizeof(T) == 8 && 16 * bp + 4 * r >= nao) continue;  // K slab wholly in the padding
             const int p = 16 * bp + M::row_of(lk, r);
             const T bwr = wr[bp][r], bwi = wi[bp][r];
 #pragma unroll
@@ -160,7 +161,134 @@ __global__ __launch_bounds__(256) void z_prep_mfma_kernel(DevDims d, const doubl
 }
 
 // ---------------------------------------------------------------------------------------
+// Split-precision P0 operand straight into int8 slices (one spin block, pseudo_ns == 1): the
+// wave that forms Z_Q(tau, k) = Gb V_Q Gq in registers also owns the whole slice row-chunk
+// (row (tau, Q), chunk k, nao^2 elements), so it finds the chunk exponent with a wave reduction
+// and writes the NS slices of Re and Im itself -- the same arithmetic as slice_rows_kernel, so
+// the slices are bit-identical to slicing the FP64 Z, without Z ever reaching HBM.
+// Slice layout: dst[((tz*NQ + Q)*nk + k)][plane][slice][kc], exps[(tz*NQ + Q)*nk + k].
+template <int NB, int NS>
+__global__ __launch_bounds__(256) void z_prep_slice_kernel(DevDims d, const double2* __restrict__ G,
+                                                           const double2* __restrict__ Vt, int8_t* __restrict__ dst,
+                                                           int* __restrict__ exps, int kc,
+                                                           const int* __restrict__ kplus, int t0, int QPW) {
+  constexpr int N16 = 16 * NB, LD = N16 + 1, PL = N16 * LD;
+  extern __shared__ double lds_d[];
+  double* Gbr = lds_d;
+  double* Gbi = Gbr + PL;
+  double* Gqr = Gbi + PL;
+  double* Gqi = Gqr + PL;
+  const int nao = d.nao, nao2 = nao * nao;
+  const int tz = blockIdx.z, tau = t0 + tz, tt = d.nts - 1 - tau;
+  const int k = blockIdx.y, kq = kplus[k];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lrow = lane & 15, lk = lane >> 4;
+  const int nslab = (nao + 3) / 4;
+  {
+    const double2* Gbar = G + g_index(d, tt, 0, k);
+    const double2* Gk = G + g_index(d, tau, 0, kq);
+    for (int e = tid; e < N16 * N16; e += blockDim.x) {
+      const int r = e / N16, c = e - (e / N16) * N16;
+      double2 gb = make_double2(0.0, 0.0), gq = make_double2(0.0, 0.0);
+      if (r < nao && c < nao) {
+        gb = Gbar[(int64_t)r * d.nso + c];
+        gq = Gk[(int64_t)r * d.nso + c];
+      }
+      Gbr[r * LD + c] = gb.x;
+      Gbi[r * LD + c] = gb.y;
+      Gqr[r * LD + c] = gq.x;
+      Gqi[r * LD + c] = gq.y;
+    }
+  }
+  __syncthreads();
+  for (int qi = 0; qi < QPW; ++qi) {
+    const int Q = (blockIdx.x * 4 + wave) * QPW + qi;
+    if (Q >= d.NQ) break;
+    const double2* vt = Vt + ((int64_t)k * d.NQ + Q) * nao2;  // vt[m * nao + p] = V[p][m]
+    v4d zr[NB][NB], zi[NB][NB];                                 // [bn][bt]
+#pragma unroll
+    for (int bn = 0; bn < NB; ++bn) {
+      v4d wr[NB], wi[NB];
+#pragma unroll
+      for (int x = 0; x < NB; ++x) { wr[x] = v4zero(); wi[x] = v4zero(); }
+      for (int sl = 0; sl < nslab; ++sl) {
+        const int m = 4 * sl + lk;
+        const double br = Gqr[m * LD + 16 * bn + lrow], bi = Gqi[m * LD + 16 * bn + lrow];
+#pragma unroll
+        for (int bp = 0; bp < NB; ++bp) {
+          const int p = 16 * bp + lrow;
+          double2 av = make_double2(0.0, 0.0);
+          if (p < nao && m < nao) av = vt[m * nao + p];
+          cmfma(wr[bp], wi[bp], av.x, av.y, br, bi);
+        }
+      }
+#pragma unroll
+      for (int x = 0; x < NB; ++x) { zr[bn][x] = v4zero(); zi[bn][x] = v4zero(); }
+#pragma unroll
+      for (int bp = 0; bp < NB; ++bp)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          if (16 * bp + 4 * r >= nao) continue;  // K slab wholly in the padding
+          const int p = 16 * bp + lk + 4 * r;
+#pragma unroll
+          for (int bt = 0; bt < NB; ++bt) {
+            const int t = 16 * bt + lrow;
+            cmfma(zr[bn][bt], zi[bn][bt], Gbr[t * LD + p], Gbi[t * LD + p], wr[bp][r], wi[bp][r]);
+          }
+        }
+    }
+    // chunk exponent: max |Re|, |Im| over the nao x nao tile (padding entries are exact zeros)
+    double mx = 0.0;
+#pragma unroll
+    for (int bn = 0; bn < NB; ++bn)
+#pragma unroll
+      for (int bt = 0; bt < NB; ++bt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) mx = fmax(mx, fmax(fabs(zr[bn][bt][r]), fabs(zi[bn][bt][r])));
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) mx = fmax(mx, __shfl_xor(mx, o, 64));
+    int e = 0;
+    if (mx > 0.0) (void)frexp(mx, &e);
+    const int64_t task = ((int64_t)tz * d.NQ + Q) * d.nk + k;
+    if (lane == 0) exps[task] = e;
+    const double sc = ldexp(1.0, -e);
+    int8_t* out = dst + task * (int64_t)2 * NS * kc;
+#pragma unroll
+    for (int bn = 0; bn < NB; ++bn) {
+      const int n = 16 * bn + lrow;
+#pragma unroll
+      for (int bt = 0; bt < NB; ++bt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int t = 16 * bt + lk + 4 * r;
+          if (n >= nao || t >= nao) continue;
+          const int kk = t * nao + n;
+          double vr = zr[bn][bt][r] * sc, vi = zi[bn][bt][r] * sc;
+#pragma unroll
+          for (int sl = 0; sl < NS; ++sl) {
+            vr *= 128.0;
+            vi *= 128.0;
+            const double qr = trunc(vr), qi2 = trunc(vi);
+            vr -= qr;
+            vi -= qi2;
+            out[(int64_t)(0 * NS + sl) * kc + kk] = (int8_t)(int)qr;
+            out[(int64_t)(1 * NS + sl) * kc + kk] = (int8_t)(int)qi2;
+          }
+        }
+    }
+    // zero the chunk padding [nao^2, kc) of every plane-slice
+    const int padn = kc - nao2;
+    for (int i = lane; i < 2 * NS * padn; i += 64) {
+      const int ps = i / padn, kk = nao2 + (i - ps * padn);
+      out[(int64_t)ps * kc + kk] = 0;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------------------
 // Sigma(tau, is, k)[a,b](i, j) += scale * sum_Q sum_n T_Q[i][n] R_Q[n][j],  T_Q = V_Q G_s
+// (upper != 0: 16 x 16 blocks below the diagonal are skipped; Sigma is Hermitian for Hermitian
+// G and the caller mirrors it once all q are in)
 // with V = V(k, k - q), G_s = G(tau, k - q) block (a, b)  (src/gw_cpu_kernel.cpp:359-373,
 // Y2 = Y1^T P done by the big GEMM as R = P-tilde conj(V)).  The first MFMA produces
 // T^T = G^T V^T so that its result registers feed the second MFMA's A operand.
@@ -171,7 +299,8 @@ __global__ __launch_bounds__(256) void sigma_epi_mfma_kernel(DevDims d, const do
                                                              const double2* __restrict__ Vt,
                                                              const double2* __restrict__ R,
                                                              double2* __restrict__ Sigma,
-                                                             const int* __restrict__ kminus, int t0, double scale) {
+                                                             const int* __restrict__ kminus, int t0, double scale,
+                                                             int upper) {
   typedef Mfma<T> M;
   typedef typename M::acc_t acc_t;
   constexpr bool F64 = sizeof(T) == 8;
@@ -241,10 +370,12 @@ __global__ __launch_bounds__(256) void sigma_epi_mfma_kernel(DevDims d, const do
         for (int bn = 0; bn < NB; ++bn)
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
+            if (sizeof(T) == 8 && 16 * bn + 4 * r >= nao) continue;  // K slab wholly in the padding
             const int n = 16 * bn + M::row_of(lk, r);
             const T xr = tr[bn][r], xi = ti[bn][r];
 #pragma unroll
             for (int bj = 0; bj < NB; ++bj) {
+              if (upper && bj < bi) continue;  // block below the diagonal: mirrored afterwards
               const int j = 16 * bj + lrow;
               double2 rv = make_double2(0.0, 0.0);
               if (n < nao && j < nao) rv = rq[n * nao + j];

@@ -149,6 +149,7 @@ void gw_hip_kernel::vjp_device(const double2* G, const double2* Sbar, double2* G
     throw status_error(GWHIP_ERR_STATE, "no stored forward pass (gwhip_set_vjp_store(h, 1), then gwhip_eval_sigma)");
   GWHIP_CHECK(hipSetDevice(device_));
   ensure_vjp_buffers();
+  detect_hermitian_G(G);
 
   const int64_t nk = d_.nk, NQ = d_.NQ, nao = d_.nao, nso = d_.nso, nao2 = nao * nao, NQ2 = NQ * NQ;
   const int64_t nts = d_.nts, half = nts / 2, nwb = d_.nw_b, nso2 = nso * nso;

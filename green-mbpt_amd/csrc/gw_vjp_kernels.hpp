@@ -121,6 +121,7 @@ __global__ __launch_bounds__(256) void orb_sandwich_kernel(OrbArgs a, OrbOp L, O
           for (int bp = 0; bp < NB; ++bp)
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
+              if (16 * bp + 4 * r >= nao) continue;  // K slab wholly in the padding
               const int p = 16 * bp + lk + 4 * r;
 #pragma unroll
               for (int bt = 0; bt < NB; ++bt) {
@@ -211,6 +212,7 @@ __global__ __launch_bounds__(256) void orb_reduce_kernel(OrbArgs a, OrbOp A, Orb
         for (int bn = 0; bn < NB; ++bn)
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
+            if (16 * bn + 4 * r >= nao) continue;  // K slab wholly in the padding
             const int n = 16 * bn + lk + 4 * r;
 #pragma unroll
             for (int bj = 0; bj < NB; ++bj) {

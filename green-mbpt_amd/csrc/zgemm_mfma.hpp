@@ -35,6 +35,7 @@ struct ZGemmArgs {
   double2 alpha;
   int accumulate;  // 1: C += alpha*AB, 0: C = alpha*AB
   int conjA, conjB;
+  int upper;       // 1: skip block tiles that lie wholly below the diagonal (Hermitian products)
   int tiles_m, tiles_n;
   int64_t nblocks;   // batch * tiles_m * tiles_n
   int64_t per_xcd;   // ceil(nblocks / 8)
@@ -80,6 +81,7 @@ __global__ __launch_bounds__(256) void zgemm_f64_kernel(ZGemmArgs p) {
   const int rem = (int)(logical - b * tiles);
   const int tm = rem / p.tiles_n, tn = rem - (rem / p.tiles_n) * p.tiles_n;
   const int m0 = tm * BM, n0 = tn * BN;
+  if (p.upper && m0 >= n0 + BN) return;
 
   const double2* __restrict__ A = p.A + b * p.sAb;
   const double2* __restrict__ B = p.B + b * p.sBb;

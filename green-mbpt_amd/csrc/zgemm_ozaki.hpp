@@ -11,9 +11,10 @@
 // the weight 2^{-7(d+2)}; pairs with s + t >= NS are dropped.
 //
 // Slice store (int8), for each row (A) or column (B):
-//   [chunk][plane][slice][kc]   planes of A: Re, Im, -Im;  planes of B: Re, Im
+//   [chunk][plane][slice][kc]   planes Re, Im
 // (B is stored already conjugated when the contraction needs conj(B)), so
-//   Re C = Re A Re B + (-Im A) Im B,   Im C = Re A Im B + Im A Re B.
+//   Re C = Re A Re B + (-Im A) Im B,   Im C = Re A Im B + Im A Re B,
+// with -Im A negated in registers (slices lie in [-127, 127], so negation is exact).
 #pragma once
 
 #include <algorithm>
@@ -37,7 +38,7 @@ struct SliceArgs {
   int64_t s_k;
   int kc;        // padded chunk length (multiple of 64)
   int conj;      // slice conj(x)
-  int nplanes;   // 2: Re, Im   3: Re, Im, -Im
+  int nplanes;   // 2: Re, Im   (3: also -Im; not used by the GEMMs)
   int8_t* dst;   // [row][chunk][plane][slice][kc]
   int* exps;     // [row][chunk]
 };
@@ -111,6 +112,18 @@ __global__ __launch_bounds__(256) void slice_rows_kernel(SliceArgs a) {
 }
 
 // ------------------------------------------------------------------------------ GEMM
+// Byte-wise two's complement negation of 16 int8 values in [-127, 127]:
+// -x = ~x + 1 per byte, the +1 kept inside each byte by adding it to the low seven bits.
+__device__ __forceinline__ v4i neg_i8x16(v4i w) {
+  v4i r;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) {
+    const unsigned n = ~(unsigned)w[i];
+    r[i] = (int)(((n & 0x7F7F7F7Fu) + 0x01010101u) ^ (n & 0x80808080u));
+  }
+  return r;
+}
+
 struct OzGemmArgs {
   int M, N, batch;
   int kc, nchunk;        // K = nchunk * kc (int8 elements per plane-slice)
@@ -134,6 +147,7 @@ struct OzGemmArgs {
   int64_t nblocks, per_xcd;
   int group_m;  // row tiles (over batch x M) swept together across all column tiles; 0 = heuristic
   int tile;     // 0: 32 x 64 block tile, two blocks per CU; 1: 64 x 64 with register prefetch, one per CU
+  int upper;    // 1: skip block tiles that lie wholly below the diagonal (Hermitian products)
 };
 
 // Output layout of v_mfma_i32_16x16x64_i8: register r of lane l holds row
@@ -153,7 +167,7 @@ __device__ __forceinline__ int oz_lds_off(int ps, int rows, int r, int piece) {
 
 template <int NS>
 __global__ __launch_bounds__(256, 2) void zgemm_ozaki_kernel(OzGemmArgs p) {
-  constexpr int PA = 3;
+  constexpr int PA = 2;
   extern __shared__ __attribute__((aligned(16))) int8_t oz_lds[];
   int8_t* As = oz_lds;                                          // [PA*NS][32 rows][64 B]
   int* eAs = reinterpret_cast<int*>(As + PA * NS * OZ_BM * 64);  // [32]
@@ -176,6 +190,7 @@ __global__ __launch_bounds__(256, 2) void zgemm_ozaki_kernel(OzGemmArgs p) {
   const int64_t b = rt / p.tiles_m;
   const int tm = (int)(rt - b * p.tiles_m);
   const int m0 = tm * OZ_BM, n0 = tn * OZ_BN;
+  if (p.upper && m0 >= n0 + OZ_BN) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lrow = lane & 15, lg = lane >> 4;
   const int wn = wave * 16;
@@ -236,7 +251,7 @@ __global__ __launch_bounds__(256, 2) void zgemm_ozaki_kernel(OzGemmArgs p) {
           const int row = 16 * x + lrow;
           fa_r[x] = *reinterpret_cast<const v4i*>(As + oz_lds_off(0 * NS + s, OZ_BM, row, lg));
           fa_i[x] = *reinterpret_cast<const v4i*>(As + oz_lds_off(1 * NS + s, OZ_BM, row, lg));
-          fa_n[x] = *reinterpret_cast<const v4i*>(As + oz_lds_off(2 * NS + s, OZ_BM, row, lg));
+          fa_n[x] = neg_i8x16(fa_i[x]);
         }
 #pragma unroll
         for (int t = 0; t + s < NS; ++t) {
@@ -330,7 +345,7 @@ __device__ __forceinline__ void oz64_store(const OzGemmArgs& p, int64_t b, int m
 
 template <int NS, bool MULTI>
 __global__ __launch_bounds__(256, 1) void zgemm_ozaki64_kernel(OzGemmArgs p) {
-  constexpr int PA = 3, PB = 2;
+  constexpr int PA = 2, PB = 2;
   extern __shared__ __attribute__((aligned(16))) int8_t oz_lds[];
   int8_t* As = oz_lds;                                         // [PA*NS][64 rows][64 B]
   int8_t* Bs = As + PA * NS * OZ4_BM * 64;                       // [PB*NS][64 cols][64 B]
@@ -351,6 +366,7 @@ __global__ __launch_bounds__(256, 1) void zgemm_ozaki64_kernel(OzGemmArgs p) {
   const int64_t b = rt / p.tiles_m;
   const int tm = (int)(rt - b * p.tiles_m);
   const int m0 = tm * OZ4_BM, n0 = tn * OZ4_BN;
+  if (p.upper && m0 >= n0 + OZ4_BN) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lrow = lane & 15, lg = lane >> 4;
   const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
@@ -417,7 +433,7 @@ __global__ __launch_bounds__(256, 1) void zgemm_ozaki64_kernel(OzGemmArgs p) {
           const int row = wm + 16 * x + lrow;
           fa_r[x] = *reinterpret_cast<const v4i*>(As + oz_lds_off(0 * NS + s, OZ4_BM, row, lg));
           fa_i[x] = *reinterpret_cast<const v4i*>(As + oz_lds_off(1 * NS + s, OZ4_BM, row, lg));
-          fa_n[x] = *reinterpret_cast<const v4i*>(As + oz_lds_off(2 * NS + s, OZ4_BM, row, lg));
+          fa_n[x] = neg_i8x16(fa_i[x]);
         }
 #pragma unroll
         for (int t = 0; t + s < NS; ++t) {
@@ -489,7 +505,7 @@ __global__ __launch_bounds__(256, 1) void zgemm_ozaki64_kernel(OzGemmArgs p) {
 }
 
 inline size_t ozaki_lds_bytes(int ns) {
-  return (size_t)3 * OZ_BM * ns * 64 + (OZ_BM + OZ_BN) * sizeof(int);
+  return (size_t)2 * OZ_BM * ns * 64 + (OZ_BM + OZ_BN) * sizeof(int);
 }
 
 template <typename K>
@@ -498,7 +514,7 @@ inline void oz_allow_lds(K kernel, size_t bytes) {
 }
 
 inline size_t ozaki64_lds_bytes(int ns) {
-  return (size_t)(3 * OZ4_BM + 2 * OZ4_BN) * ns * 64 + (OZ4_BM + OZ4_BN) * sizeof(int);
+  return (size_t)(2 * OZ4_BM + 2 * OZ4_BN) * ns * 64 + (OZ4_BM + OZ4_BN) * sizeof(int);
 }
 
 template <int NS>

@@ -20,7 +20,8 @@ HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.dirname(os.path.ab
 OK, ERR_ARG, ERR_WRONG_GRID, ERR_STATE, ERR_OOM, ERR_DEVICE, ERR_NONFINITE, ERR_UNSUPPORTED = range(8)
 PREC_FP64, PREC_SPLIT = 0, 1
 VJP_RECOMPUTE, VJP_STORED = 0, 1
-TUNING_KEYS = ("p0_tile", "sigma_tile", "aux_per_wave", "p0_group", "sigma_group", "p0_split_tile", "sigma_split_tile")
+TUNING_KEYS = ("p0_tile", "sigma_tile", "aux_per_wave", "p0_group", "sigma_group", "p0_split_tile", "sigma_split_tile",
+               "p0_fused_slices")
 REGIONS = ("P0_zgemm", "P0(t) -> P0(w)", "GW-BSE", "P(w) -> P(t)", "Selfenergy_zgemm", "total")
 KSTATS = ("P0 GEMM", "Sigma GEMM", "P0 prep", "Sigma epilogue", "Dyson inverse")
 

@@ -71,6 +71,27 @@ def test_small_cases(case):
     assert rel_err(S, S_ref) < TOL_FP64
 
 
+@pytest.mark.parametrize("hermitian", [True, False])
+def test_hermitian_and_general_G(hermitian):
+    """Hermitian G: only the upper P0 tiles are computed and mirrored.  A G with an
+    anti-Hermitian part takes the full products and the reference's (X + X^H)/2."""
+    G, V, T, d = synthetic(4, 5, 70, beta=20.0, lam=1e3, mesh=(2, 2, 1), seed=12, sigma=0.3)
+    if not hermitian:
+        rng = np.random.default_rng(5)
+        G = G + 1e-3 * (rng.standard_normal(G.shape) + 1j * rng.standard_normal(G.shape))
+    k = _kernel(d, T)
+    try:
+        k.set_grid(T.T_bt, T.T_tb)
+        k.upload_V(V)
+        S = k.solve(G)
+        S_split = k.solve(G, gl.PREC_SPLIT)
+    finally:
+        k.close()
+    S_ref, _ = _oracle(G, V, T, d)
+    assert rel_err(S, S_ref) < TOL_FP64
+    assert rel_err(S_split, S_ref) < 1e-8
+
+
 def test_device_generator_matches_host_generator():
     G, V, T, d = synthetic(8, 4, 12, beta=50.0, lam=1e3, mesh=(2, 2, 2), seed=11)
     k = _kernel(d, T)

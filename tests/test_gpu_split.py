@@ -98,7 +98,8 @@ def test_four_slices_are_looser_but_bounded():
 
 @pytest.mark.parametrize("p0_group,s_group", [(1, 16), (12, 1), (64, 64)])
 def test_split_tiles_and_orders_give_identical_sigma(p0_group, s_group):
-    """Both int8 block tiles and any L2 tile order produce bit-identical Sigma (exact int32 sums)."""
+    """Both int8 block tiles, any L2 tile order and the fused or separate slicing of Z produce
+    bit-identical Sigma (exact int32 sums, identical slices)."""
     c = CONFIG0
     G, V, T, d = synthetic(c["nk"], c["nao"], c["NQ"], beta=c["beta"], lam=c["lam"], mesh=c["mesh"])
     k = _kernel(d, T)
@@ -108,8 +109,10 @@ def test_split_tiles_and_orders_give_identical_sigma(p0_group, s_group):
         ref = k.solve(G, gl.PREC_SPLIT)
         out = []
         for tile in (0, 1):
-            k.set_tuning(p0_split_tile=tile, sigma_split_tile=tile, p0_group=p0_group, sigma_group=s_group)
-            out.append(k.solve(G, gl.PREC_SPLIT))
+            for fused in (0, 1):
+                k.set_tuning(p0_split_tile=tile, sigma_split_tile=tile, p0_group=p0_group, sigma_group=s_group,
+                             p0_fused_slices=fused)
+                out.append(k.solve(G, gl.PREC_SPLIT))
     finally:
         k.close()
     for o in out:
