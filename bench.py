@@ -26,6 +26,14 @@ PEAK_INT8_TOPS = 5000.0
 
 WORKLOAD = dict(nk=64, mesh=(4, 4, 4), nao=26, NQ=300, ns=1, beta=100.0, lam=1e4, seed=0, sigma=0.1)
 
+# HBM bytes per launch of the Sigma-contraction GEMM on this workload, from two rocprofv3 --pmc
+# runs (FETCH_SIZE, WRITE_SIZE in KiB per dispatch) of this command with the same build; each
+# scaled by what the counter reads for a plain 2 GiB device copy (bench.py --copy-calibration):
+# FETCH_SIZE reads 0.5x the bytes the copy reads, WRITE_SIZE 1.0x the bytes it writes.
+# Source: profiles/gw_sigma_nk64_nao26_naux300_hbm_traffic.md
+PMC_PER_LAUNCH_KIB = {"split": (8.6671e6, 2.0407e7), "fp64": (5.1556e7, 2.0376e7)}
+PMC_CALIBRATION = (0.5, 1.0)
+
 
 def algorithmic_flops(nk, nao, NQ, nts, nw_b, pseudo_ns=1):
     """Reference-order flop count of one Sigma evaluation (8 flops per complex MAC)."""
@@ -91,6 +99,8 @@ def main():
                     help="time the Sigma[G] vector-Jacobian product on the nk=27 nao=26 naux=250 system instead")
     ap.add_argument("--batch", action="store_true",
                     help="time the 8 uneven ns=2 systems evaluated as one captured graph instead")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="keep the default launch parameters (the autotuner's choice for this workload)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 2 GiB device-to-device copy (reference point for HBM byte counters)")
     args = ap.parse_args()
@@ -128,7 +138,8 @@ def main():
         k.set_split_slices(args.slices)
     torch.cuda.synchronize()
     t_tune = time.perf_counter()
-    k.autotune(G_d.data_ptr(), prec)
+    if not args.no_autotune:
+        k.autotune(G_d.data_ptr(), prec)
     t_tune = time.perf_counter() - t_tune
     print(f"[bench] autotuned in {t_tune:.2f} s: {k.tuning()}", file=sys.stderr, flush=True)
     for i in range(args.warmup):
@@ -167,7 +178,10 @@ def main():
                 "kernel": f"zgemm_ozaki_kernel<{ns}> (Sigma contraction, int8 slices)",
                 "algorithmic_fp64_tflops": per_launch / avg_s / 1e12}
     roof["frac"] = roof["achieved"] / roof["peak"]
-    roof.update({"traffic": None, "launches": n_l, "avg_launch_ms": avg_s * 1e3})
+    fetch_kib, write_kib = PMC_PER_LAUNCH_KIB[args.precision]
+    traffic = 1024.0 * (fetch_kib / PMC_CALIBRATION[0] + write_kib / PMC_CALIBRATION[1])
+    roof.update({"traffic": traffic, "traffic_unit": "bytes/launch", "traffic_GBps": traffic / avg_s / 1e9,
+                 "launches": n_l, "avg_launch_ms": avg_s * 1e3})
 
     check = {}
     if prec == gl.PREC_SPLIT and not args.no_check:

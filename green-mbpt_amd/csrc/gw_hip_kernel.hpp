@@ -40,7 +40,7 @@ struct KernelStat {
 // orbital kernel writes the int8 slices itself.
 struct Tuning {
   int p0_shape = 0, s_shape = 0, qpw = 4;
-  int p0_group = 1, s_group = 16;  // split precision: row tiles per L2 group of the int8 GEMMs
+  int p0_group = 1, s_group = 8;  // split precision: row tiles per L2 group of the int8 GEMMs
   int p0_oz_tile = 0, s_oz_tile = 0;  // split precision: int8 GEMM block tile (0: 32x64, 1: 64x64)
   int p0_fused = 1;  // split precision, one spin block: P0 orbital kernel writes int8 slices directly
   static constexpr int count = 8;
