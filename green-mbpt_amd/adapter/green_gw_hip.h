@@ -41,15 +41,63 @@ namespace green::mbpt::hip {
     p.define<bool>("hip_split_precision", "Split-precision P0/Sigma contractions in the HIP GW kernel", false);
   }
 
+  namespace detail {
+    /** The GW kernel's handle, so that the HF kernel reuses its resident integrals (node rank 0). */
+    inline std::weak_ptr<gwhip_handle>& resident_handle() {
+      static std::weak_ptr<gwhip_handle> h;
+      return h;
+    }
+
+    inline void check(int st, const gwhip_handle* h) {
+      if (st == GWHIP_OK) return;
+      const std::string msg = gwhip_last_error(h);
+      if (st == GWHIP_ERR_WRONG_GRID) throw mbpt_wrong_grid(msg);
+      throw mbpt_kernel_error("HIP GW kernel: " + msg);
+    }
+
+    inline void require_full_mesh_one_node(const bz_utils_t& bz) {
+      if (bz.ink() != bz.nk() || bz.inq() != bz.nq())
+        throw mbpt_kernel_error("HIP GW kernel: symmetry-reduced k/q meshes are not supported");
+      if (utils::context().internode_size != 1)
+        throw mbpt_kernel_error("HIP GW kernel: one node (q is not split across nodes)");
+    }
+
+    // k + q and k - q exactly as the CPU kernels use them (gw_cpu_kernel.cpp:90,117)
+    inline void set_maps(gwhip_handle* h, const bz_utils_t& bz) {
+      const size_t nk = bz.nk();
+      std::vector<int32_t> kp(nk * nk), km(nk * nk);
+      for (size_t q = 0; q < nk; ++q)
+        for (size_t k = 0; k < nk; ++k) {
+          kp[q * nk + k] = bz.k_q_map().k1_from_k2q(k, q);
+          km[q * nk + k] = bz.k_q_map().k2_from_k1q(k, q);
+        }
+      check(gwhip_set_kq_maps(h, kp.data(), km.data()), h);
+    }
+
+    // Read every stored k pair through the reference's own reader (df_integral_t.h:64-187)
+    // into the packed layout the device expands.
+    inline void upload_integrals(gwhip_handle* h, const bz_utils_t& bz, const std::string& path, size_t nao,
+                                 size_t NQ) {
+      const size_t nk = bz.nk(), npair = nk * (nk + 1) / 2;
+      std::vector<std::complex<double>> Vp(npair * NQ * nao * nao);
+      df_integral_t reader(path, nao, NQ, bz, utils::mpi_context(MPI_COMM_SELF));
+      tensor<std::complex<double>, 3> v(NQ, nao, nao);
+      for (size_t k1 = 0; k1 < nk; ++k1)
+        for (size_t k2 = 0; k2 <= k1; ++k2) {
+          reader.read_integrals(k1, k2);
+          reader.symmetrize(v, k1, k2);
+          std::copy(v.data(), v.data() + v.size(), Vp.data() + (k1 * (k1 + 1) / 2 + k2) * NQ * nao * nao);
+        }
+      check(gwhip_upload_V(h, reinterpret_cast<const double*>(Vp.data())), h);
+    }
+  }  // namespace detail
+
   /** Owns one libgwhip handle with V resident on the device across SC iterations. */
   class gw_hip_kernel {
   public:
     gw_hip_kernel(bool X2C, const params::params& p, size_t nao, size_t nso, size_t ns, size_t NQ,
                   const grids::transformer_t& ft, const bz_utils_t& bz) : _ft(ft), _bz(bz) {
-      if (bz.ink() != bz.nk() || bz.inq() != bz.nq())
-        throw mbpt_kernel_error("HIP GW kernel: symmetry-reduced k/q meshes are not supported");
-      if (utils::context().internode_size != 1)
-        throw mbpt_kernel_error("HIP GW kernel: one node (q is not split across nodes)");
+      detail::require_full_mesh_one_node(bz);
       _nts  = ft.sd().repn_fermi().nts();
       _nw_b = ft.sd().repn_bose().nw();
       gwhip_dims d{};
@@ -63,9 +111,10 @@ namespace green::mbpt::hip {
         gwhip_handle* h = nullptr;
         check(gwhip_create(&d, p["hip_device"], &h), nullptr);
         _h.reset(h, gwhip_destroy);
-        set_maps();
+        detail::set_maps(h, bz);
         set_transforms();
-        upload_integrals(p["dfintegral_file"], nao, NQ);
+        detail::upload_integrals(h, bz, p["dfintegral_file"], nao, NQ);
+        detail::resident_handle() = _h;
       }
     }
 
@@ -88,24 +137,7 @@ namespace green::mbpt::hip {
     size_t _nts = 0, _nw_b = 0;
     bool   _split = false;
 
-    static void check(int st, const gwhip_handle* h) {
-      if (st == GWHIP_OK) return;
-      const std::string msg = gwhip_last_error(h);
-      if (st == GWHIP_ERR_WRONG_GRID) throw mbpt_wrong_grid(msg);
-      throw mbpt_kernel_error("HIP GW kernel: " + msg);
-    }
-
-    // k + q and k - q exactly as the CPU kernel uses them (gw_cpu_kernel.cpp:90,117)
-    void set_maps() {
-      const size_t nk = _bz.nk();
-      std::vector<int32_t> kp(nk * nk), km(nk * nk);
-      for (size_t q = 0; q < nk; ++q)
-        for (size_t k = 0; k < nk; ++k) {
-          kp[q * nk + k] = _bz.k_q_map().k1_from_k2q(k, q);
-          km[q * nk + k] = _bz.k_q_map().k2_from_k1q(k, q);
-        }
-      check(gwhip_set_kq_maps(_h.get(), kp.data(), km.data()), _h.get());
-    }
+    static void check(int st, const gwhip_handle* h) { detail::check(st, h); }
 
     // The transforms are linear maps per aux element: probe them with unit vectors so the
     // device applies exactly green-grids' tau_f_to_w_b / w_b_to_tau_f (gw_cpu_kernel.cpp:234,264).
@@ -125,22 +157,6 @@ namespace green::mbpt::hip {
                            reinterpret_cast<const double*>(T_tb.data())),
             _h.get());
     }
-
-    // Read every stored k pair through the reference's own reader (df_integral_t.h:64-187)
-    // into the packed layout the device expands.
-    void upload_integrals(const std::string& path, size_t nao, size_t NQ) {
-      const size_t nk = _bz.nk(), npair = nk * (nk + 1) / 2;
-      std::vector<std::complex<double>> Vp(npair * NQ * nao * nao);
-      df_integral_t reader(path, nao, NQ, _bz, utils::mpi_context(MPI_COMM_SELF));
-      tensor<std::complex<double>, 3> v(NQ, nao, nao);
-      for (size_t k1 = 0; k1 < nk; ++k1)
-        for (size_t k2 = 0; k2 <= k1; ++k2) {
-          reader.read_integrals(k1, k2);
-          reader.symmetrize(v, k1, k2);
-          std::copy(v.data(), v.data() + v.size(), Vp.data() + (k1 * (k1 + 1) / 2 + k2) * NQ * nao * nao);
-        }
-      check(gwhip_upload_V(_h.get(), reinterpret_cast<const double*>(Vp.data())), _h.get());
-    }
   };
 
   /** GW factory hook (src/green/mbpt/kernel_factory.h:28-31). */
@@ -152,23 +168,67 @@ namespace green::mbpt::hip {
     return std::tuple{std::static_pointer_cast<void>(kernel), callback};
   }
 
-  /** HF factory hook (kernel_factory.h:33-36): delegates to the reference CPU kernels. */
-  inline std::tuple<std::shared_ptr<void>, std::function<ztensor<4>(const ztensor<4>&)>> custom_hf_kernel(
-      bool X2C, const params::params& p, size_t nao, size_t nso, size_t ns, size_t NQ, double madelung,
-      const bz_utils_t& bz_utils, const ztensor<4>& S_k) {
-    using x_type = ztensor<4>;
-    if (X2C) {
-      std::shared_ptr<void> kernel(new kernels::hf_x2c_cpu_kernel(p, nao, nso, ns, NQ, madelung, bz_utils, S_k));
-      std::function callback = [kernel](const x_type& dm) -> x_type {
-        return static_cast<kernels::hf_x2c_cpu_kernel*>(kernel.get())->solve(dm);
-      };
-      return std::tuple{kernel, callback};
+  /** Hartree-Fock on the device: the counterpart of hf_scalar_cpu_kernel / hf_x2c_cpu_kernel::solve
+   *  (src/hf_cpu_kernels.cpp:24-279).  Uses the GW kernel's handle (V already resident) when one
+   *  exists at the first solve, else its own; the result is broadcast to every rank, as the CPU
+   *  kernels' allreduce leaves it (hf_cpu_kernels.cpp:140). */
+  class hf_hip_kernel {
+  public:
+    hf_hip_kernel(const params::params& p, size_t nao, size_t nso, size_t ns, size_t NQ, double madelung,
+                  const bz_utils_t& bz, const ztensor<4>& S_k) :
+        _bz(bz), _S(S_k), _nao(nao), _nso(nso), _ns(ns), _NQ(NQ), _madelung(madelung),
+        _device(p["hip_device"]), _path(p["dfintegral_file"].as<std::string>()), _beta(p["BETA"]) {
+      detail::require_full_mesh_one_node(bz);
     }
-    std::shared_ptr<void> kernel(new kernels::hf_scalar_cpu_kernel(p, nao, nso, ns, NQ, madelung, bz_utils, S_k));
-    std::function callback = [kernel](const x_type& dm) -> x_type {
-      return static_cast<kernels::hf_scalar_cpu_kernel*>(kernel.get())->solve(dm);
-    };
-    return std::tuple{kernel, callback};
+
+    ztensor<4> solve(const ztensor<4>& dm) {
+      ztensor<4> F(_ns, _bz.nk(), _nso, _nso);
+      if (utils::context().global_rank == 0) {
+        gwhip_handle* h = handle();
+        detail::check(gwhip_hf(h, reinterpret_cast<const double*>(dm.data()), reinterpret_cast<const double*>(_S.data()),
+                               _madelung, reinterpret_cast<double*>(F.data())),
+                      h);
+      }
+      MPI_Bcast(F.data(), 2 * static_cast<int>(F.size()), MPI_DOUBLE, 0, utils::context().global);
+      return F;
+    }
+
+  private:
+    const bz_utils_t&             _bz;
+    ztensor<4>                    _S;
+    size_t                        _nao, _nso, _ns, _NQ;
+    double                        _madelung;
+    int                           _device;
+    std::string                   _path;
+    double                        _beta;
+    std::shared_ptr<gwhip_handle> _h;
+
+    gwhip_handle* handle() {
+      if (_h) return _h.get();
+      _h = detail::resident_handle().lock();
+      if (_h) return _h.get();
+      // no GW kernel: a handle of its own; the HF entry point reads no tau grid, so a minimal one
+      gwhip_dims d{};
+      d.nk = _bz.nk();
+      d.mesh[0] = _bz.nk(); d.mesh[1] = 1; d.mesh[2] = 1;
+      d.nao = _nao; d.nso = _nso; d.ns = _ns; d.NQ = _NQ;
+      d.nts = 2; d.nw_b = 1; d.beta = _beta;
+      gwhip_handle* h = nullptr;
+      detail::check(gwhip_create(&d, _device, &h), nullptr);
+      _h.reset(h, gwhip_destroy);
+      detail::set_maps(h, _bz);
+      detail::upload_integrals(h, _bz, _path, _nao, _NQ);
+      return h;
+    }
+  };
+
+  /** HF factory hook (kernel_factory.h:33-36). */
+  inline std::tuple<std::shared_ptr<void>, std::function<ztensor<4>(const ztensor<4>&)>> custom_hf_kernel(
+      bool /*X2C*/, const params::params& p, size_t nao, size_t nso, size_t ns, size_t NQ, double madelung,
+      const bz_utils_t& bz_utils, const ztensor<4>& S_k) {
+    auto kernel = std::make_shared<hf_hip_kernel>(p, nao, nso, ns, NQ, madelung, bz_utils, S_k);
+    std::function<ztensor<4>(const ztensor<4>&)> callback = [kernel](const ztensor<4>& dm) { return kernel->solve(dm); };
+    return std::tuple{std::static_pointer_cast<void>(kernel), callback};
   }
 
 }  // namespace green::mbpt::hip

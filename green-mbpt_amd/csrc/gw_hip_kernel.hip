@@ -125,7 +125,7 @@ gw_hip_kernel::~gw_hip_kernel() {
   (void)hipSetDevice(device_);
   if (stream_) (void)hipStreamSynchronize(stream_);
   for (void* p : {(void*)Vq_, (void*)Vt_, (void*)P_, (void*)Tbt_fold_, (void*)Ttb_, (void*)G_dev_, (void*)S_dev_,
-                  (void*)kplus_tab_, (void*)kminus_tab_, (void*)flag_, (void*)hcheck_})
+                  (void*)kplus_tab_, (void*)kminus_tab_, (void*)flag_, (void*)hcheck_, (void*)hf_work_, (void*)hf_x_})
     if (p) (void)hipFree(p);
   if (own_ws_)
     for (void* p : {(void*)Pw_, (void*)X_, (void*)Y_, (void*)scratch_})
@@ -906,6 +906,7 @@ void gw_hip_kernel::run_stages_host(const double2* G, int last_stage) {
 }  // namespace gwhip
 
 #include "gw_vjp_host.hpp"
+#include "gw_hf_host.hpp"
 #include "gw_batch.hpp"
 
 namespace gwhip {
@@ -1359,6 +1360,25 @@ int gwhip_vjp_device(gwhip_handle* h, const void* G, const void* Sigma_bar, void
 }
 
 double gwhip_vjp_ms(const gwhip_handle* h) { return h ? h->k->vjp_ms() : 0.0; }
+
+int gwhip_hf(gwhip_handle* h, const double* dm, const double* S, double madelung, double* F) {
+  if (!h) return GWHIP_ERR_ARG;
+  return guarded(h, [&] { h->k->hf_host((const double2*)dm, (const double2*)S, madelung, (double2*)F); });
+}
+
+int gwhip_hf_device(gwhip_handle* h, const void* dm, const void* S, double madelung, void* F) {
+  if (!h) return GWHIP_ERR_ARG;
+  return guarded(h, [&] { h->k->hf_device((const double2*)dm, (const double2*)S, madelung, (double2*)F); });
+}
+
+double gwhip_hf_ms(const gwhip_handle* h) { return h ? h->k->hf_ms() : 0.0; }
+
+int gwhip_hf_exchange_stats(const gwhip_handle* h, double* ms, int64_t* launches) {
+  if (!h) return GWHIP_ERR_ARG;
+  if (ms) *ms = h->k->hf_exchange_ms();
+  if (launches) *launches = h->k->hf_exchange_launches();
+  return GWHIP_OK;
+}
 
 int gwhip_get_P_tilde(gwhip_handle* h, double* P) {
   if (!h || !P) return GWHIP_ERR_ARG;

@@ -99,6 +99,13 @@ public:
   void vjp_device(const double2* G_dev, const double2* Sbar_dev, double2* Gbar_dev, int mode);
   void vjp_host(const double2* G, const double2* Sbar, double2* Gbar, int mode);
   double vjp_ms() const { return vjp_ms_; }
+  // Hartree-Fock Fock contribution from the resident integrals (device / host pointers):
+  // dm, S, F [ns][nk][nso][nso]
+  void hf_device(const double2* dm, const double2* S, double madelung, double2* F);
+  void hf_host(const double2* dm, const double2* S, double madelung, double2* F);
+  double hf_ms() const { return hf_ms_; }
+  double hf_exchange_ms() const { return hf_exchange_ms_; }
+  int64_t hf_exchange_launches() const { return hf_exchange_launches_; }
 
   // batch support: workspace needs per tau point / per q, binding, and capture-safe enqueueing
   size_t scratch_per_t() const;
@@ -186,6 +193,10 @@ private:
   int64_t tchunk_vjp_ = 1;
   bool keep_p0_ = false, stored_valid_ = false;
   double vjp_ms_ = 0.0;
+  double2* hf_work_ = nullptr;  // Hartree-Fock: J[Q], then S dm products [ns][nk][nso^2]
+  double2* hf_x_ = nullptr;     // Hartree-Fock exchange per (q, k): [blocks][nq][nk][nao^2]
+  double hf_ms_ = 0.0, hf_exchange_ms_ = 0.0;
+  int64_t hf_exchange_launches_ = 0;
   std::vector<int> kplus_host_, neg_q_;
 
   double region_ms_[R_COUNT] = {0};

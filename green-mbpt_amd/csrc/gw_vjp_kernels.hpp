@@ -40,10 +40,12 @@ struct OrbArgs {
   const int* kplus;   // [nk] row of the k + q table for this q
   const int* kminus;  // [nk] row of the k - q table for this q
   double scale;
+  int64_t kmap_st = 0;  // k +- q table row stride per blockIdx.y (0: one q for the whole launch)
 };
 
-__device__ __forceinline__ int orb_kidx(const OrbArgs& a, int kmap, int k) {
-  return kmap == 1 ? a.kplus[k] : (kmap == 2 ? a.kminus[k] : k);
+__device__ __forceinline__ int orb_kidx(const OrbArgs& a, int kmap, int k, int tz) {
+  const int64_t r = (int64_t)tz * a.kmap_st;
+  return kmap == 1 ? a.kplus[r + k] : (kmap == 2 ? a.kminus[r + k] : k);
 }
 
 __device__ __forceinline__ double2 orb_load(const double2* base, const OrbOp& op, int r, int c) {
@@ -56,7 +58,7 @@ __device__ __forceinline__ double2 orb_load(const double2* base, const OrbOp& op
 template <int NB>
 __device__ __forceinline__ void orb_stage(const OrbArgs& a, const OrbOp& op, int tz, int k, double* Xr, double* Xi) {
   constexpr int N16 = 16 * NB, LD = N16 + 1;
-  const double2* base = op.p + (int64_t)tz * op.s_t + (int64_t)orb_kidx(a, op.kmap, k) * op.s_k;
+  const double2* base = op.p + (int64_t)tz * op.s_t + (int64_t)orb_kidx(a, op.kmap, k, tz) * op.s_k;
   for (int e = threadIdx.x; e < N16 * N16; e += blockDim.x) {
     const int r = e / N16, c = e - (e / N16) * N16;
     double2 v = make_double2(0.0, 0.0);
@@ -85,8 +87,8 @@ __global__ __launch_bounds__(256) void orb_sandwich_kernel(OrbArgs a, OrbOp L, O
   if (HAS_L) orb_stage<NB>(a, L, tz, k, Lr, Li);
   if (HAS_R) orb_stage<NB>(a, R, tz, k, Rr, Ri);
   __syncthreads();
-  const double2* vb0 = V.p + (int64_t)tz * V.s_t + (int64_t)orb_kidx(a, V.kmap, k) * V.s_k;
-  double2* xb0 = X.p + (int64_t)tz * X.s_t + (int64_t)orb_kidx(a, X.kmap, k) * X.s_k;
+  const double2* vb0 = V.p + (int64_t)tz * V.s_t + (int64_t)orb_kidx(a, V.kmap, k, tz) * V.s_k;
+  double2* xb0 = X.p + (int64_t)tz * X.s_t + (int64_t)orb_kidx(a, X.kmap, k, tz) * X.s_k;
   const double sv = V.conj ? -1.0 : 1.0;
 
   for (int qi = 0; qi < a.QPW; ++qi) {
@@ -177,8 +179,8 @@ __global__ __launch_bounds__(256) void orb_reduce_kernel(OrbArgs a, OrbOp A, Orb
 
   if (HAS_M) orb_stage<NB>(a, M, tz, k, Mr, Mi);
   __syncthreads();
-  const double2* ab0 = A.p + (int64_t)tz * A.s_t + (int64_t)orb_kidx(a, A.kmap, k) * A.s_k;
-  const double2* cb0 = C.p + (int64_t)tz * C.s_t + (int64_t)orb_kidx(a, C.kmap, k) * C.s_k;
+  const double2* ab0 = A.p + (int64_t)tz * A.s_t + (int64_t)orb_kidx(a, A.kmap, k, tz) * A.s_k;
+  const double2* cb0 = C.p + (int64_t)tz * C.s_t + (int64_t)orb_kidx(a, C.kmap, k, tz) * C.s_k;
   const double sa = A.conj ? -1.0 : 1.0, sc = C.conj ? -1.0 : 1.0;
 
   v4d acr[NB][NB], aci[NB][NB];
@@ -266,7 +268,7 @@ __global__ __launch_bounds__(256) void orb_reduce_kernel(OrbArgs a, OrbOp A, Orb
     }
   }
   if (wave == 0) {
-    double2* ob = O.p + (int64_t)tz * O.s_t + (int64_t)orb_kidx(a, O.kmap, k) * O.s_k;
+    double2* ob = O.p + (int64_t)tz * O.s_t + (int64_t)orb_kidx(a, O.kmap, k, tz) * O.s_k;
 #pragma unroll
     for (int x = 0; x < NB; ++x)
 #pragma unroll

@@ -100,6 +100,11 @@ def load() -> ctypes.CDLL:
     lib.gwhip_vjp_device.argtypes = [P, P, P, P, ctypes.c_int]
     lib.gwhip_vjp_ms.argtypes = [P]
     lib.gwhip_vjp_ms.restype = ctypes.c_double
+    lib.gwhip_hf.argtypes = [P, P, P, ctypes.c_double, P]
+    lib.gwhip_hf_device.argtypes = [P, P, P, ctypes.c_double, P]
+    lib.gwhip_hf_ms.argtypes = [P]
+    lib.gwhip_hf_ms.restype = ctypes.c_double
+    lib.gwhip_hf_exchange_stats.argtypes = [P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]
     for f in ("gwhip_debug_zgemm_split", "gwhip_set_split_slices", "gwhip_set_kq_maps", "gwhip_debug_mfma_peak",
               "gwhip_debug_gemm_bench", "gwhip_debug_zgemm", "gwhip_debug_stage", "gwhip_set_grid", "gwhip_upload_V",
               "gwhip_generate_V", "gwhip_eval_sigma", "gwhip_eval_sigma_device", "gwhip_get_P_tilde", "gwhip_timings",
@@ -279,6 +284,33 @@ class GwHipKernel:
 
     def vjp_ms(self) -> float:
         return float(self.lib.gwhip_vjp_ms(self.h))
+
+    def hf_shape(self):
+        d = self.dims
+        return (d["ns"], d["nk"], d["nso"], d["nso"])
+
+    def hf(self, dm: np.ndarray, S: np.ndarray, madelung: float) -> np.ndarray:
+        """Hartree-Fock Fock contribution [ns][nk][nso][nso] (host arrays), the counterpart of
+        hf_scalar_cpu_kernel / hf_x2c_cpu_kernel::solve (src/hf_cpu_kernels.cpp:24-279)."""
+        d = np.ascontiguousarray(dm, dtype=np.complex128)
+        s = np.ascontiguousarray(S, dtype=np.complex128)
+        assert d.shape == self.hf_shape() and s.shape == d.shape, (d.shape, s.shape, self.hf_shape())
+        F = np.empty_like(d)
+        self._check(self.lib.gwhip_hf(self.h, _ptr(d), _ptr(s), float(madelung), _ptr(F)))
+        return F
+
+    def hf_device(self, dm_ptr: int, S_ptr: int, madelung: float, F_ptr: int):
+        self._check(self.lib.gwhip_hf_device(self.h, ctypes.c_void_p(dm_ptr), ctypes.c_void_p(S_ptr),
+                                             float(madelung), ctypes.c_void_p(F_ptr)))
+
+    def hf_ms(self) -> float:
+        return float(self.lib.gwhip_hf_ms(self.h))
+
+    def hf_exchange_stats(self):
+        """(milliseconds, launches) of the exchange-term kernel in the last hf call."""
+        ms, n = ctypes.c_double(), ctypes.c_int64()
+        self._check(self.lib.gwhip_hf_exchange_stats(self.h, ctypes.byref(ms), ctypes.byref(n)))
+        return ms.value, n.value
 
     def run_stages(self, G: np.ndarray, last_stage: int):
         g = np.ascontiguousarray(G, dtype=np.complex128)

@@ -175,6 +175,27 @@ int gwhip_vjp_device(gwhip_handle* h, const void* G_dev, const void* Sigma_bar_d
 /* Milliseconds of the last VJP (HIP events on the handle's stream). */
 double gwhip_vjp_ms(const gwhip_handle* h);
 
+/* Hartree-Fock Fock contribution from the handle's resident integrals (replaces
+ * hf_scalar_cpu_kernel::solve and hf_x2c_cpu_kernel::solve, src/hf_cpu_kernels.cpp:24-146 and
+ * :148-279, reached through the factory's HF hook, src/green/mbpt/kernel_factory.h:33-36):
+ *   F_s(k) = sum_Q J[Q] V(k,k)_Q - pref/nk sum_{k'} sum_Q V(k,k')_Q dm_s(k') V(k,k')_Q^H
+ *            - pref * madelung * S_s(k) dm_s(k) S_s(k),   J[Q] = 1/nk sum_{k',s} tr(V(k',k')_Q dm_s(k'))
+ * pref = 1/2 for ns = 1 and 1 for ns = 2.  X2C (nso = 2 nao, ns = 1): spin blocks aa, bb, ab with
+ * pref 1, the direct term from dm_aa + dm_bb into aa and bb, the Madelung term with S_aa on both
+ * sides, F_ba = F_ab^H.  dm is the density matrix the reference forms (spin prefactor times
+ * G(beta), src/hf_solver.cpp:9-16).  Host buffers dm, S, F: [ns][nk][nso][nso] complex. FP64. */
+int gwhip_hf(gwhip_handle* h, const double* dm, const double* S, double madelung, double* F);
+
+/* As gwhip_hf with device buffers.  Synchronous with respect to the host. */
+int gwhip_hf_device(gwhip_handle* h, const void* dm_dev, const void* S_dev, double madelung, void* F_dev);
+
+/* Milliseconds of the last gwhip_hf[_device] (HIP events on the handle's stream). */
+double gwhip_hf_ms(const gwhip_handle* h);
+
+/* Of the last gwhip_hf[_device]: total milliseconds of its exchange-term launches (HIP events
+ * around each) and their number (ns, or 3 spin blocks for X2C). */
+int gwhip_hf_exchange_stats(const gwhip_handle* h, double* ms, int64_t* launches);
+
 /* Copy P-tilde(tau, q) of the last eval to the host: [nk][nts][NQ][NQ] complex.
  * Exposes the intermediate the reference keeps in P0_tilde_s after eval_P_tilde
  * (src/gw_cpu_kernel.cpp:227-279); used by tests. */
