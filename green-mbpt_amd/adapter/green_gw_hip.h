@@ -10,7 +10,9 @@
  * `mbpt.exe --kernel HIP` selects it.
  *
  * This header needs green-mbpt's own headers (params, grids, symmetry, utils, ndarray) and is
- * compiled only inside a green-mbpt build.  Everything device-side sits behind include/gwhip.h.
+ * compiled only inside a green-mbpt build.  Everything device-side sits behind include/gwhip.h;
+ * the integral files and symmetry tables are read by libgwhip_h5 (include/gwhip_h5.h), so link
+ * both gwhip and gwhip_h5.
  */
 #ifndef GREEN_MBPT_GW_HIP_H
 #define GREEN_MBPT_GW_HIP_H
@@ -29,6 +31,7 @@
 #include <vector>
 
 #include "gwhip.h"
+#include "gwhip_h5.h"
 
 namespace green::mbpt::hip {
 
@@ -42,22 +45,26 @@ namespace green::mbpt::hip {
   }
 
   namespace detail {
-    /** The GW kernel's handle, so that the HF kernel reuses its resident integrals (node rank 0). */
-    inline std::weak_ptr<gwhip_handle>& resident_handle() {
-      static std::weak_ptr<gwhip_handle> h;
-      return h;
+    /** The GW kernel's handle and the integral directory it holds, so that the HF kernel reuses the
+     *  resident integrals when it reads the same directory (node rank 0). */
+    struct resident_t {
+      std::weak_ptr<gwhip_handle> handle;
+      std::string                 path;
+    };
+    inline resident_t& resident() {
+      static resident_t r;
+      return r;
     }
 
     inline void check(int st, const gwhip_handle* h) {
       if (st == GWHIP_OK) return;
-      const std::string msg = gwhip_last_error(h);
+      const std::string msg = h ? gwhip_last_error(h) : gwhip_h5_last_error();
       if (st == GWHIP_ERR_WRONG_GRID) throw mbpt_wrong_grid(msg);
+      if (st == GWHIP_ERR_OUTDATED_INPUT) throw mbpt_outdated_input(msg);
       throw mbpt_kernel_error("HIP GW kernel: " + msg);
     }
 
-    inline void require_full_mesh_one_node(const bz_utils_t& bz) {
-      if (bz.ink() != bz.nk() || bz.inq() != bz.nq())
-        throw mbpt_kernel_error("HIP GW kernel: symmetry-reduced k/q meshes are not supported");
+    inline void require_one_node() {
       if (utils::context().internode_size != 1)
         throw mbpt_kernel_error("HIP GW kernel: one node (q is not split across nodes)");
     }
@@ -74,21 +81,13 @@ namespace green::mbpt::hip {
       check(gwhip_set_kq_maps(h, kp.data(), km.data()), h);
     }
 
-    // Read every stored k pair through the reference's own reader (df_integral_t.h:64-187)
-    // into the packed layout the device expands.
-    inline void upload_integrals(gwhip_handle* h, const bz_utils_t& bz, const std::string& path, size_t nao,
-                                 size_t NQ) {
-      const size_t nk = bz.nk(), npair = nk * (nk + 1) / 2;
-      std::vector<std::complex<double>> Vp(npair * NQ * nao * nao);
-      df_integral_t reader(path, nao, NQ, bz, utils::mpi_context(MPI_COMM_SELF));
-      tensor<std::complex<double>, 3> v(NQ, nao, nao);
-      for (size_t k1 = 0; k1 < nk; ++k1)
-        for (size_t k2 = 0; k2 <= k1; ++k2) {
-          reader.read_integrals(k1, k2);
-          reader.symmetrize(v, k1, k2);
-          std::copy(v.data(), v.data() + v.size(), Vp.data() + (k1 * (k1 + 1) / 2 + k2) * NQ * nao * nao);
-        }
-      check(gwhip_upload_V(h, reinterpret_cast<const double*>(Vp.data())), h);
+    // Integrals (every stored chunk, expanded to all k pairs, df_integral_t.h:40-187) and, on a
+    // symmetry-reduced mesh, the irreducible-wedge tables of the input file; then the reference's
+    // own k +- q maps.
+    inline void load_files(gwhip_handle* h, const bz_utils_t& bz, const std::string& df_path,
+                           const std::string& input_file) {
+      check(gwhip_h5_load(h, df_path.c_str(), input_file.c_str(), bz.ink() != bz.nk() ? 1 : 0), nullptr);
+      set_maps(h, bz);
     }
   }  // namespace detail
 
@@ -97,7 +96,8 @@ namespace green::mbpt::hip {
   public:
     gw_hip_kernel(bool X2C, const params::params& p, size_t nao, size_t nso, size_t ns, size_t NQ,
                   const grids::transformer_t& ft, const bz_utils_t& bz) : _ft(ft), _bz(bz) {
-      detail::require_full_mesh_one_node(bz);
+      detail::require_one_node();
+      _ibz  = bz.ink() != bz.nk();
       _nts  = ft.sd().repn_fermi().nts();
       _nw_b = ft.sd().repn_bose().nw();
       gwhip_dims d{};
@@ -111,10 +111,10 @@ namespace green::mbpt::hip {
         gwhip_handle* h = nullptr;
         check(gwhip_create(&d, p["hip_device"], &h), nullptr);
         _h.reset(h, gwhip_destroy);
-        detail::set_maps(h, bz);
         set_transforms();
-        detail::upload_integrals(h, bz, p["dfintegral_file"], nao, NQ);
-        detail::resident_handle() = _h;
+        const std::string path = p["dfintegral_file"];
+        detail::load_files(h, bz, path, p["input_file"]);
+        detail::resident() = {_h, path};
       }
     }
 
@@ -125,7 +125,8 @@ namespace green::mbpt::hip {
       if (utils::context().node_rank == 0) {
         const auto* G = reinterpret_cast<const double*>(g.object().data());
         auto*       S = reinterpret_cast<double*>(sigma_tau.object().data());
-        check(gwhip_eval_sigma(_h.get(), G, S, _split ? GWHIP_PREC_SPLIT : GWHIP_PREC_FP64), _h.get());
+        const int prec = _split ? GWHIP_PREC_SPLIT : GWHIP_PREC_FP64;
+        check(_ibz ? gwhip_eval_sigma_ibz(_h.get(), G, S, prec) : gwhip_eval_sigma(_h.get(), G, S, prec), _h.get());
       }
       sigma_tau.fence();
     }
@@ -135,7 +136,7 @@ namespace green::mbpt::hip {
     const bz_utils_t&           _bz;
     std::shared_ptr<gwhip_handle> _h;
     size_t _nts = 0, _nw_b = 0;
-    bool   _split = false;
+    bool   _split = false, _ibz = false;
 
     static void check(int st, const gwhip_handle* h) { detail::check(st, h); }
 
@@ -170,24 +171,28 @@ namespace green::mbpt::hip {
 
   /** Hartree-Fock on the device: the counterpart of hf_scalar_cpu_kernel / hf_x2c_cpu_kernel::solve
    *  (src/hf_cpu_kernels.cpp:24-279).  Uses the GW kernel's handle (V already resident) when one
-   *  exists at the first solve, else its own; the result is broadcast to every rank, as the CPU
-   *  kernels' allreduce leaves it (hf_cpu_kernels.cpp:140). */
+   *  exists at the first solve and reads the same integral directory (dfintegral_hf_file ==
+   *  dfintegral_file), else its own; the result is broadcast to every rank, as the CPU kernels'
+   *  allreduce leaves it (hf_cpu_kernels.cpp:140). */
   class hf_hip_kernel {
   public:
     hf_hip_kernel(const params::params& p, size_t nao, size_t nso, size_t ns, size_t NQ, double madelung,
                   const bz_utils_t& bz, const ztensor<4>& S_k) :
         _bz(bz), _S(S_k), _nao(nao), _nso(nso), _ns(ns), _NQ(NQ), _madelung(madelung),
-        _device(p["hip_device"]), _path(p["dfintegral_file"].as<std::string>()), _beta(p["BETA"]) {
-      detail::require_full_mesh_one_node(bz);
+        _device(p["hip_device"]), _path(p["dfintegral_hf_file"].as<std::string>()),
+        _input(p["input_file"].as<std::string>()), _beta(p["BETA"]) {
+      detail::require_one_node();
     }
 
+    /** dm, S and the result on the irreducible k points: [ns][ink][nso][nso]. */
     ztensor<4> solve(const ztensor<4>& dm) {
-      ztensor<4> F(_ns, _bz.nk(), _nso, _nso);
+      ztensor<4> F(_ns, _bz.ink(), _nso, _nso);
       if (utils::context().global_rank == 0) {
-        gwhip_handle* h = handle();
-        detail::check(gwhip_hf(h, reinterpret_cast<const double*>(dm.data()), reinterpret_cast<const double*>(_S.data()),
-                               _madelung, reinterpret_cast<double*>(F.data())),
-                      h);
+        gwhip_handle* h  = handle();
+        const auto*   d  = reinterpret_cast<const double*>(dm.data());
+        const auto*   S  = reinterpret_cast<const double*>(_S.data());
+        auto*         Fp = reinterpret_cast<double*>(F.data());
+        detail::check(_bz.ink() != _bz.nk() ? gwhip_hf_ibz(h, d, S, _madelung, Fp) : gwhip_hf(h, d, S, _madelung, Fp), h);
       }
       MPI_Bcast(F.data(), 2 * static_cast<int>(F.size()), MPI_DOUBLE, 0, utils::context().global);
       return F;
@@ -199,15 +204,15 @@ namespace green::mbpt::hip {
     size_t                        _nao, _nso, _ns, _NQ;
     double                        _madelung;
     int                           _device;
-    std::string                   _path;
+    std::string                   _path, _input;
     double                        _beta;
     std::shared_ptr<gwhip_handle> _h;
 
     gwhip_handle* handle() {
       if (_h) return _h.get();
-      _h = detail::resident_handle().lock();
+      if (detail::resident().path == _path) _h = detail::resident().handle.lock();
       if (_h) return _h.get();
-      // no GW kernel: a handle of its own; the HF entry point reads no tau grid, so a minimal one
+      // no GW kernel on the same integrals: a handle of its own; HF reads no tau grid, so a minimal one
       gwhip_dims d{};
       d.nk = _bz.nk();
       d.mesh[0] = _bz.nk(); d.mesh[1] = 1; d.mesh[2] = 1;
@@ -216,8 +221,7 @@ namespace green::mbpt::hip {
       gwhip_handle* h = nullptr;
       detail::check(gwhip_create(&d, _device, &h), nullptr);
       _h.reset(h, gwhip_destroy);
-      detail::set_maps(h, _bz);
-      detail::upload_integrals(h, _bz, _path, _nao, _NQ);
+      detail::load_files(h, _bz, _path, _input);
       return h;
     }
   };

@@ -125,7 +125,7 @@ gw_hip_kernel::~gw_hip_kernel() {
   (void)hipSetDevice(device_);
   if (stream_) (void)hipStreamSynchronize(stream_);
   for (void* p : {(void*)Vq_, (void*)Vt_, (void*)P_, (void*)Tbt_fold_, (void*)Ttb_, (void*)G_dev_, (void*)S_dev_,
-                  (void*)kplus_tab_, (void*)kminus_tab_, (void*)flag_, (void*)hcheck_, (void*)hf_work_, (void*)hf_x_})
+                  (void*)kplus_tab_, (void*)kminus_tab_, (void*)flag_, (void*)hcheck_, (void*)hf_work_, (void*)hf_x_, (void*)ksym_idx_, (void*)ksym_U_, (void*)hf_sym_})
     if (p) (void)hipFree(p);
   if (own_ws_)
     for (void* p : {(void*)Pw_, (void*)X_, (void*)Y_, (void*)scratch_})
@@ -907,6 +907,7 @@ void gw_hip_kernel::run_stages_host(const double2* G, int last_stage) {
 
 #include "gw_vjp_host.hpp"
 #include "gw_hf_host.hpp"
+#include "gw_symmetry.hpp"
 #include "gw_batch.hpp"
 
 namespace gwhip {
@@ -1372,6 +1373,43 @@ int gwhip_hf_device(gwhip_handle* h, const void* dm, const void* S, double madel
 }
 
 double gwhip_hf_ms(const gwhip_handle* h) { return h ? h->k->hf_ms() : 0.0; }
+
+int gwhip_set_k_symmetry(gwhip_handle* h, int64_t ink, const int64_t* bz2ibz, const int64_t* ibz2bz, const double* U,
+                         const int64_t* tr_conj) {
+  if (!h) return GWHIP_ERR_ARG;
+  return guarded(h, [&] { h->k->set_k_symmetry(ink, bz2ibz, ibz2bz, (const double2*)U, tr_conj); });
+}
+
+int gwhip_eval_sigma_ibz(gwhip_handle* h, const double* G_ibz, double* Sigma_ibz, int precision) {
+  if (!h) return GWHIP_ERR_ARG;
+  return guarded(h, [&] { h->k->solve_ibz_host((const double2*)G_ibz, (double2*)Sigma_ibz, precision); });
+}
+
+int gwhip_eval_sigma_ibz_device(gwhip_handle* h, const void* G_ibz, void* Sigma_ibz, int precision) {
+  if (!h) return GWHIP_ERR_ARG;
+  return guarded(h, [&] { h->k->solve_ibz_device((const double2*)G_ibz, (double2*)Sigma_ibz, precision); });
+}
+
+int gwhip_hf_ibz(gwhip_handle* h, const double* dm_ibz, const double* S_ibz, double madelung, double* F_ibz) {
+  if (!h) return GWHIP_ERR_ARG;
+  return guarded(h, [&] { h->k->hf_ibz_host((const double2*)dm_ibz, (const double2*)S_ibz, madelung, (double2*)F_ibz); });
+}
+
+int gwhip_hf_ibz_device(gwhip_handle* h, const void* dm_ibz, const void* S_ibz, double madelung, void* F_ibz) {
+  if (!h) return GWHIP_ERR_ARG;
+  return guarded(h,
+                 [&] { h->k->hf_ibz_device((const double2*)dm_ibz, (const double2*)S_ibz, madelung, (double2*)F_ibz); });
+}
+
+int gwhip_get_dims(const gwhip_handle* h, gwhip_dims* out) {
+  if (!h || !out) return GWHIP_ERR_ARG;
+  const gwhip::Dims& d = h->k->dims();
+  out->nk = d.nk;
+  for (int i = 0; i < 3; ++i) out->mesh[i] = d.mesh[i];
+  out->nao = d.nao; out->nso = d.nso; out->ns = d.ns; out->NQ = d.NQ;
+  out->nts = d.nts; out->nw_b = d.nw_b; out->beta = d.beta;
+  return GWHIP_OK;
+}
 
 int gwhip_hf_exchange_stats(const gwhip_handle* h, double* ms, int64_t* launches) {
   if (!h) return GWHIP_ERR_ARG;

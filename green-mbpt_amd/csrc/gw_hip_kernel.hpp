@@ -104,6 +104,15 @@ public:
   void hf_device(const double2* dm, const double2* S, double madelung, double2* F);
   void hf_host(const double2* dm, const double2* S, double madelung, double2* F);
   double hf_ms() const { return hf_ms_; }
+  // Irreducible-wedge inputs/outputs: G_ibz, Sigma_ibz [nts][ns][ink][nso][nso]; dm, S, F
+  // [ns][ink][nso][nso].  set_k_symmetry takes green-symmetry's tables (U: [nk][nso][nso]).
+  void set_k_symmetry(int64_t ink, const int64_t* bz2ibz, const int64_t* ibz2bz, const double2* U,
+                      const int64_t* tr_conj);
+  int64_t ink() const { return ink_; }
+  void solve_ibz_device(const double2* G_ibz, double2* Sigma_ibz, int precision);
+  void solve_ibz_host(const double2* G_ibz, double2* Sigma_ibz, int precision);
+  void hf_ibz_device(const double2* dm_ibz, const double2* S_ibz, double madelung, double2* F_ibz);
+  void hf_ibz_host(const double2* dm_ibz, const double2* S_ibz, double madelung, double2* F_ibz);
   double hf_exchange_ms() const { return hf_exchange_ms_; }
   int64_t hf_exchange_launches() const { return hf_exchange_launches_; }
 
@@ -196,6 +205,12 @@ private:
   double2* hf_work_ = nullptr;  // Hartree-Fock: J[Q], then S dm products [ns][nk][nso^2]
   double2* hf_x_ = nullptr;     // Hartree-Fock exchange per (q, k): [blocks][nq][nk][nao^2]
   double hf_ms_ = 0.0, hf_exchange_ms_ = 0.0;
+  // k symmetry: [IBZ ordinal | representative flag | time reversal | ibz2bz] ints, U per k, and
+  // the full-mesh dm, S, F and scratch of the irreducible HF entry
+  int64_t ink_ = 0;
+  int* ksym_idx_ = nullptr;
+  double2* ksym_U_ = nullptr;
+  double2* hf_sym_ = nullptr;
   int64_t hf_exchange_launches_ = 0;
   std::vector<int> kplus_host_, neg_q_;
 
@@ -212,6 +227,8 @@ private:
   void ensure_vjp_buffers();
   void free_vjp_buffers();
   void zero(double2* X, int64_t n);
+  void unfold(const double2* X, double2* T, double2* Y, int64_t nb);
+  void gather_ibz(const double2* Full, double2* Ibz, int64_t nb);
   void p0_prep(int64_t q, const double2* G, int64_t t0, int64_t tc, int qpw);
   void p0_prep_slice(int64_t q, const double2* G, int64_t t0, int64_t tc, int qpw);
   void p0_gemm(int64_t q, int64_t tc, double2* Pt0, int shape);

@@ -14,10 +14,14 @@ import re
 import numpy as np
 
 LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libgwhip.so")
+H5_LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libgwhip_h5.so")
 HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__)))),
                       "include", "gwhip.h")
+H5_HEADER = os.path.join(os.path.dirname(HEADER), "gwhip_h5.h")
+HDF5_ROOT = os.environ.get("HDF5_ROOT", "/opt/conda")
 
-OK, ERR_ARG, ERR_WRONG_GRID, ERR_STATE, ERR_OOM, ERR_DEVICE, ERR_NONFINITE, ERR_UNSUPPORTED = range(8)
+(OK, ERR_ARG, ERR_WRONG_GRID, ERR_STATE, ERR_OOM, ERR_DEVICE, ERR_NONFINITE, ERR_UNSUPPORTED, ERR_OUTDATED_INPUT,
+ ERR_IO) = range(10)
 PREC_FP64, PREC_SPLIT = 0, 1
 VJP_RECOMPUTE, VJP_STORED = 0, 1
 TUNING_KEYS = ("p0_tile", "sigma_tile", "aux_per_wave", "p0_group", "sigma_group", "p0_split_tile", "sigma_split_tile",
@@ -34,6 +38,18 @@ class GwhipError(RuntimeError):
 
 class WrongGrid(GwhipError):
     """Mirror of ``mbpt_wrong_grid`` (``src/green/mbpt/except.h:33-36``)."""
+
+
+class OutdatedInput(GwhipError):
+    """Mirror of ``mbpt_outdated_input`` (integral files older than format 1.0.0)."""
+
+
+def _raise(st: int, msg: str):
+    if st == ERR_WRONG_GRID:
+        raise WrongGrid(st, msg)
+    if st == ERR_OUTDATED_INPUT:
+        raise OutdatedInput(st, msg)
+    raise GwhipError(st, msg)
 
 
 class Dims(ctypes.Structure):
@@ -105,21 +121,106 @@ def load() -> ctypes.CDLL:
     lib.gwhip_hf_ms.argtypes = [P]
     lib.gwhip_hf_ms.restype = ctypes.c_double
     lib.gwhip_hf_exchange_stats.argtypes = [P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]
+    lib.gwhip_set_k_symmetry.argtypes = [P, ctypes.c_int64, P, P, P, P]
+    lib.gwhip_eval_sigma_ibz.argtypes = [P, P, P, ctypes.c_int]
+    lib.gwhip_eval_sigma_ibz_device.argtypes = [P, P, P, ctypes.c_int]
+    lib.gwhip_hf_ibz.argtypes = [P, P, P, ctypes.c_double, P]
+    lib.gwhip_hf_ibz_device.argtypes = [P, P, P, ctypes.c_double, P]
+    lib.gwhip_get_dims.argtypes = [P, ctypes.POINTER(Dims)]
     for f in ("gwhip_debug_zgemm_split", "gwhip_set_split_slices", "gwhip_set_kq_maps", "gwhip_debug_mfma_peak",
               "gwhip_debug_gemm_bench", "gwhip_debug_zgemm", "gwhip_debug_stage", "gwhip_set_grid", "gwhip_upload_V",
               "gwhip_generate_V", "gwhip_eval_sigma", "gwhip_eval_sigma_device", "gwhip_get_P_tilde", "gwhip_timings",
               "gwhip_kernel_stats", "gwhip_set_vjp_store", "gwhip_vjp", "gwhip_vjp_device", "gwhip_autotune",
               "gwhip_get_tuning", "gwhip_set_tuning", "gwhip_batch_create", "gwhip_batch_autotune",
-              "gwhip_batch_eval_device", "gwhip_batch_stats"):
+              "gwhip_batch_eval_device", "gwhip_batch_stats", "gwhip_set_k_symmetry", "gwhip_eval_sigma_ibz",
+              "gwhip_eval_sigma_ibz_device", "gwhip_hf_ibz", "gwhip_hf_ibz_device", "gwhip_get_dims"):
         getattr(lib, f).restype = ctypes.c_int
     _lib = lib
     return lib
 
 
-def header_functions() -> list[str]:
-    """Names of every function the C header declares."""
-    text = open(HEADER).read()
+def header_functions(header: str = HEADER) -> list[str]:
+    """Names of every function a C header declares."""
+    text = open(header).read()
     return sorted(set(re.findall(r"^\s*(?:const\s+)?[\w\s\*]+?\b(gwhip_\w+)\s*\(", text, re.M)))
+
+
+_h5 = None
+
+
+def load_h5() -> ctypes.CDLL:
+    """Load libgwhip_h5.so, the HDF5 integral reader (raises if it has not been built).  The image's
+    libhdf5 is loaded first by its full path, so the reader finds it whatever the run path says."""
+    global _h5
+    if _h5 is not None:
+        return _h5
+    if not os.path.exists(H5_LIB_PATH):
+        raise RuntimeError(f"{H5_LIB_PATH} is missing: build it with `python green-mbpt_amd/build.py`")
+    load()
+    libdir = os.path.join(HDF5_ROOT, "lib")
+    for name in sorted(os.listdir(libdir)) if os.path.isdir(libdir) else []:
+        if name.startswith("libhdf5.so.") and name.count(".") == 2:
+            ctypes.CDLL(os.path.join(libdir, name), mode=ctypes.RTLD_GLOBAL)
+            break
+    h5 = ctypes.CDLL(H5_LIB_PATH)
+    P, I64P = ctypes.c_void_p, ctypes.POINTER(ctypes.c_int64)
+    h5.gwhip_h5_check_version.argtypes = [ctypes.c_char_p]
+    h5.gwhip_h5_read_meta.argtypes = [ctypes.c_char_p, I64P]
+    h5.gwhip_h5_read_packed.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int64, ctypes.c_int64,
+                                        ctypes.c_int64, P]
+    h5.gwhip_h5_read_k_symmetry.argtypes = [ctypes.c_char_p, ctypes.c_int64, ctypes.c_int64, I64P, P, P, P, P, P]
+    h5.gwhip_h5_kq_maps.argtypes = [P, ctypes.c_int64, P, P]
+    h5.gwhip_h5_load.argtypes = [P, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
+    h5.gwhip_h5_last_error.restype = ctypes.c_char_p
+    _h5 = h5
+    return h5
+
+
+def _h5check(st: int):
+    if st != OK:
+        _raise(st, load_h5().gwhip_h5_last_error().decode())
+
+
+def h5_check_version(version: str) -> int:
+    """1 if version >= 1.0.0, 0 if older, -1 if malformed (CheckVersion)."""
+    return int(load_h5().gwhip_h5_check_version(version.encode()))
+
+
+def h5_read_meta(df_path: str) -> int:
+    """chunk_size of an integral directory after the version check."""
+    cs = ctypes.c_int64()
+    _h5check(load_h5().gwhip_h5_read_meta(df_path.encode(), ctypes.byref(cs)))
+    return cs.value
+
+
+def h5_read_packed(df_path: str, input_h5: str, nk: int, nao: int, NQ: int) -> np.ndarray:
+    """Every k pair k1 >= k2, packed [nk(nk+1)/2, NQ, nao, nao] complex128."""
+    out = np.empty((nk * (nk + 1) // 2, NQ, nao, nao), np.complex128)
+    _h5check(load_h5().gwhip_h5_read_packed(df_path.encode(), input_h5.encode(), nk, nao, NQ, _ptr(out)))
+    return out
+
+
+def h5_read_k_symmetry(input_h5: str, nk: int, nso: int) -> dict:
+    """/symmetry/k tables: ink, bz2ibz, ibz2bz, U, tr_conj, mesh_scaled."""
+    ink = ctypes.c_int64()
+    b = np.empty(nk, np.int64)
+    i = np.empty(nk, np.int64)
+    t = np.empty(nk, np.int64)
+    U = np.empty((nk, nso, nso), np.complex128)
+    m = np.empty((nk, 3), np.float64)
+    _h5check(load_h5().gwhip_h5_read_k_symmetry(input_h5.encode(), nk, nso, ctypes.byref(ink), _ptr(b), _ptr(i),
+                                                _ptr(U), _ptr(t), _ptr(m)))
+    return dict(ink=ink.value, bz2ibz=b, ibz2bz=i[:ink.value].copy(), U=U, tr_conj=t, mesh_scaled=m)
+
+
+def h5_kq_maps(mesh_scaled: np.ndarray):
+    """(kplus[q][k], kminus[q][k]) from fractional k coordinates."""
+    x = np.ascontiguousarray(mesh_scaled, np.float64)
+    nk = x.shape[0]
+    kp = np.empty((nk, nk), np.int32)
+    km = np.empty((nk, nk), np.int32)
+    _h5check(load_h5().gwhip_h5_kq_maps(_ptr(x), nk, _ptr(kp), _ptr(km)))
+    return kp, km
 
 
 def debug_zgemm(A: np.ndarray, B: np.ndarray, conjA: bool = False, conjB: bool = False, shape: int = 0) -> np.ndarray:
@@ -192,8 +293,7 @@ class GwHipKernel:
 
     def _check(self, st: int):
         if st != OK:
-            msg = self.lib.gwhip_last_error(self.h).decode()
-            raise (WrongGrid if st == ERR_WRONG_GRID else GwhipError)(st, msg)
+            _raise(st, self.lib.gwhip_last_error(self.h).decode())
 
     def close(self):
         if getattr(self, "h", None):
@@ -284,6 +384,51 @@ class GwHipKernel:
 
     def vjp_ms(self) -> float:
         return float(self.lib.gwhip_vjp_ms(self.h))
+
+    def set_k_symmetry(self, ink: int, bz2ibz, ibz2bz, U, tr_conj):
+        """Irreducible-wedge tables (input.h5:/symmetry/k); U: [nk, nso, nso] complex."""
+        d = self.dims
+        b = np.ascontiguousarray(bz2ibz, np.int64)
+        i = np.ascontiguousarray(ibz2bz, np.int64)
+        u = np.ascontiguousarray(U, np.complex128)
+        t = np.ascontiguousarray(tr_conj, np.int64)
+        assert b.shape == (d["nk"],) and i.shape == (ink,) and t.shape == (d["nk"],)
+        assert u.shape == (d["nk"], d["nso"], d["nso"])
+        self._check(self.lib.gwhip_set_k_symmetry(self.h, ink, _ptr(b), _ptr(i), _ptr(u), _ptr(t)))
+        self.dims["ink"] = ink
+
+    def load_files(self, df_path: str, input_h5: str, with_symmetry: bool = True):
+        """k +- q maps, integrals and (optionally) k symmetry from green-mbpt's files."""
+        st = load_h5().gwhip_h5_load(self.h, df_path.encode(), input_h5.encode(), 1 if with_symmetry else 0)
+        _h5check(st)
+        if with_symmetry:
+            self.dims["ink"] = h5_read_k_symmetry(input_h5, self.dims["nk"], self.dims["nso"])["ink"]
+
+    def ibz_shape(self, with_tau: bool = True):
+        d = self.dims
+        s = (d["ns"], d["ink"], d["nso"], d["nso"])
+        return ((d["nts"],) + s) if with_tau else s
+
+    def solve_ibz(self, G_ibz: np.ndarray, precision: int = PREC_FP64) -> np.ndarray:
+        """Sigma on the irreducible k points from G on them (host arrays)."""
+        g = np.ascontiguousarray(G_ibz, dtype=np.complex128)
+        assert g.shape == self.ibz_shape(), (g.shape, self.ibz_shape())
+        S = np.empty_like(g)
+        self._check(self.lib.gwhip_eval_sigma_ibz(self.h, _ptr(g), _ptr(S), precision))
+        return S
+
+    def solve_ibz_device(self, G_ptr: int, S_ptr: int, precision: int = PREC_FP64):
+        self._check(self.lib.gwhip_eval_sigma_ibz_device(self.h, ctypes.c_void_p(G_ptr), ctypes.c_void_p(S_ptr),
+                                                         precision))
+
+    def hf_ibz(self, dm: np.ndarray, S: np.ndarray, madelung: float) -> np.ndarray:
+        """Fock contribution on the irreducible k points (host arrays [ns, ink, nso, nso])."""
+        d = np.ascontiguousarray(dm, dtype=np.complex128)
+        s = np.ascontiguousarray(S, dtype=np.complex128)
+        assert d.shape == self.ibz_shape(False) and s.shape == d.shape, (d.shape, s.shape)
+        F = np.empty_like(d)
+        self._check(self.lib.gwhip_hf_ibz(self.h, _ptr(d), _ptr(s), float(madelung), _ptr(F)))
+        return F
 
     def hf_shape(self):
         d = self.dims

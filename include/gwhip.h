@@ -35,7 +35,10 @@ enum gwhip_status {
   GWHIP_ERR_OOM = 4,         /* device allocation failed                                 */
   GWHIP_ERR_DEVICE = 5,      /* HIP runtime error or no usable device                    */
   GWHIP_ERR_NONFINITE = 6,   /* Sigma contains NaN/Inf                                   */
-  GWHIP_ERR_UNSUPPORTED = 7  /* request outside what this build implements               */
+  GWHIP_ERR_UNSUPPORTED = 7, /* request outside what this build implements               */
+  GWHIP_ERR_OUTDATED_INPUT = 8, /* integral files older than format 1.0.0: mbpt_outdated_input
+                                   (src/green/mbpt/df_integral_t.h:42-51, except.h)          */
+  GWHIP_ERR_IO = 9           /* a file or dataset is missing or has the wrong shape        */
 };
 
 /* Arithmetic of the P0 and Sigma contractions (the Dyson step, transforms and
@@ -53,7 +56,8 @@ enum gwhip_vjp_mode {
 };
 
 /* Problem dimensions; names follow gw_cpu_kernel's members (src/green/mbpt/kernels.h:58-83).
- * Only the full k mesh without symmetry is supported (ink = nq = inq = nk). */
+ * The kernels run on the full k mesh (nq = nk); irreducible-wedge inputs (ink < nk) are unfolded
+ * to it by the *_ibz entry points after gwhip_set_k_symmetry. */
 typedef struct gwhip_dims {
   int64_t nk;        /* k points in the full Brillouin zone                          */
   int64_t mesh[3];   /* Monkhorst-Pack mesh, mesh[0]*mesh[1]*mesh[2] == nk;
@@ -191,6 +195,33 @@ int gwhip_hf_device(gwhip_handle* h, const void* dm_dev, const void* S_dev, doub
 
 /* Milliseconds of the last gwhip_hf[_device] (HIP events on the handle's stream). */
 double gwhip_hf_ms(const gwhip_handle* h);
+
+/* Irreducible-wedge tables of green-symmetry's brillouin_zone_utils (input.h5:/symmetry/k), so
+ * that G, Sigma, dm, S and F may be passed on the ink irreducible k points as the reference does
+ * (G, Sigma: [nts][ns][ink][nso][nso]; dm, S, F: [ns][ink][nso][nso]):
+ *   bz2ibz [nk]  full-mesh index of each k's representative;  ibz2bz [ink]  the representatives
+ *   U [nk][nso][nso] complex  k_sym_transform_ao;  tr_conj [nk]  1 where time reversal applies.
+ * An irreducible object is carried to k as X(k) = U_k op(X(ik)) U_k^H (op = conj when
+ * tr_conj[k]), the identity at a representative: green-symmetry's value_AO as used at
+ * src/gw_cpu_kernel.cpp:177-178,336 and src/hf_cpu_kernels.cpp:107. */
+int gwhip_set_k_symmetry(gwhip_handle* h, int64_t ink, const int64_t* bz2ibz, const int64_t* ibz2bz, const double* U,
+                         const int64_t* tr_conj);
+
+/* Sigma on the irreducible k points from G on the irreducible k points: gw_cpu_kernel::solve with
+ * symmetry (src/gw_cpu_kernel.cpp:31-134, the q-star loop at :113-130).  G is unfolded on the
+ * device, the full-mesh kernel runs, and the representatives' Sigma is returned.  Host buffers. */
+int gwhip_eval_sigma_ibz(gwhip_handle* h, const double* G_ibz, double* Sigma_ibz, int precision);
+int gwhip_eval_sigma_ibz_device(gwhip_handle* h, const void* G_ibz_dev, void* Sigma_ibz_dev, int precision);
+
+/* Hartree-Fock on the irreducible k points (hf_scalar_cpu_kernel::solve with symmetry,
+ * src/hf_cpu_kernels.cpp:24-146: dm is unfolded with value_AO, F returned at the
+ * representatives).  dm, S, F: [ns][ink][nso][nso]. */
+int gwhip_hf_ibz(gwhip_handle* h, const double* dm_ibz, const double* S_ibz, double madelung, double* F_ibz);
+int gwhip_hf_ibz_device(gwhip_handle* h, const void* dm_ibz_dev, const void* S_ibz_dev, double madelung,
+                        void* F_ibz_dev);
+
+/* The dimensions the handle was created with. */
+int gwhip_get_dims(const gwhip_handle* h, gwhip_dims* out);
 
 /* Of the last gwhip_hf[_device]: total milliseconds of its exchange-term launches (HIP events
  * around each) and their number (ns, or 3 spin blocks for X2C). */

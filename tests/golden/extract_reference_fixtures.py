@@ -18,12 +18,26 @@ Sources (reference paths relative to its ``test/data``):
   section (``test/solvers_test.cpp:388``, ``solve_hf`` at ``:27-79``): G(beta), the overlap
   S-k, the Madelung constant and V^Q_ij for nk=1, X2C, nao=10, NQ=154, with the expected
   result/Sigma1 (the X2C Hartree-Fock Fock matrix).
+* ``GW/input.h5`` (identical to ``HF/input.h5``) -- the nk=27 (3x3x3) H2 system with space-group
+  symmetry: ns=2, nao=nso=2, NQ=36, ink=4 irreducible k points, 196 of 378 k pairs stored.
+  Written to ``sym27.npz``: params, ``symmetry/k`` (bz2ibz, ibz2bz, k_sym_transform_ao,
+  tr_conj, mesh_scaled), ``symmetry/pairs`` (conj/trans pair lists, kpair_irre_list,
+  kpair_idx), HF/S-k and HF/madelung.
+* ``HF/data.h5`` + ``HF/df_hf_int/VQ_0.h5`` -- the ``HF`` section (``solvers_test.cpp:387``):
+  G_tau (irreducible k), result/Sigma1 -> ``hf_sym27.npz``.
+* ``GW/data.h5`` + ``GW/df_int/VQ_0.h5`` -- the ``GW`` section (``solvers_test.cpp:389``):
+  G_tau, result/Sigma_tau (irreducible k) -> ``gw_sym27.npz``.
+* The integral files themselves, byte for byte, for the HDF5 loader: ``GW/df_int/{meta,VQ_0}.h5``
+  -> ``h5/gw_sym27_df/``; ``Input/df_int*/meta.h5`` (no version attribute, 0.3.0, 1.0.0, 1.1.0;
+  ``solvers_test.cpp:470-495``) -> ``h5/versions/<dir>/meta.h5``; and the ``params``,
+  ``symmetry/k`` and ``symmetry/pairs`` groups of ``GW/input.h5`` -> ``h5/sym27_input.h5``.
 * ``HF_orth_invariance/none/data.h5:/iter1/G_tau/mesh`` -- the 110-node IR tau mesh
   (Lambda = 1e4, beta = 100) that the ``ir/1e4.h5`` grid used by all these fixtures has.
 
 Only data (inputs and expected outputs) is written; nothing from the reference's code.
 """
 import os
+import shutil
 import sys
 
 import h5py
@@ -69,6 +83,40 @@ def main(data_dir):
         madelung = float(f["HF/madelung"][()])
     V = _v_chunk(os.path.join(data_dir, "HF_X2C/df_hf_int/VQ_0.h5"))
     np.savez(os.path.join(OUT, "hf_x2c.npz"), G_beta=G_beta, Sigma1=S1, S_k=S_k, madelung=madelung, V=V, **p)
+    sym_in = os.path.join(data_dir, "GW/input.h5")
+    with h5py.File(sym_in, "r") as f:
+        sym = {k: int(f["params/" + k][()]) for k in ("nao", "nso", "ns", "NQ", "nk")}
+        sym["ink"] = int(f["symmetry/k/ink"][()])
+        for k in ("bz2ibz", "ibz2bz", "k_sym_transform_ao", "tr_conj", "mesh_scaled"):
+            sym["k_" + k] = f["symmetry/k/" + k][()]
+        for k in ("conj_pairs_list", "trans_pairs_list", "kpair_irre_list", "kpair_idx", "num_kpair_stored"):
+            sym["pairs_" + k] = f["symmetry/pairs/" + k][()]
+        sym["S_k"] = np.ascontiguousarray(f["HF/S-k"][()]).view(np.complex128)[..., 0]
+        sym["madelung"] = float(f["HF/madelung"][()])
+    np.savez(os.path.join(OUT, "sym27.npz"), **sym)
+    with h5py.File(os.path.join(data_dir, "HF/data.h5"), "r") as f:
+        hf = dict(G_tau=f["G_tau"][()], Sigma1=f["result/Sigma1"][()])
+    np.savez(os.path.join(OUT, "hf_sym27.npz"), V=_v_chunk(os.path.join(data_dir, "HF/df_hf_int/VQ_0.h5")), **hf)
+    with h5py.File(os.path.join(data_dir, "GW/data.h5"), "r") as f:
+        gw = dict(G_tau=f["G_tau"][()], Sigma_tau=f["result/Sigma_tau"][()])
+    np.savez(os.path.join(OUT, "gw_sym27.npz"), V=_v_chunk(os.path.join(data_dir, "GW/df_int/VQ_0.h5")), tau=mesh,
+             beta=100.0, lam=1e4, **gw)
+
+    # integral files for the HDF5 loader (data files of the reference's tests, copied unchanged)
+    h5dir = os.path.join(OUT, "h5")
+    os.makedirs(os.path.join(h5dir, "gw_sym27_df"), exist_ok=True)
+    for name in ("meta.h5", "VQ_0.h5"):
+        shutil.copyfile(os.path.join(data_dir, "GW/df_int", name), os.path.join(h5dir, "gw_sym27_df", name))
+    for d in ("df_int", "df_int_0.3.0", "df_int_1.0.0", "df_int_1.1.0"):
+        os.makedirs(os.path.join(h5dir, "versions", d), exist_ok=True)
+        shutil.copyfile(os.path.join(data_dir, "Input", d, "meta.h5"), os.path.join(h5dir, "versions", d, "meta.h5"))
+    with h5py.File(sym_in, "r") as src, h5py.File(os.path.join(h5dir, "sym27_input.h5"), "w") as dst:
+        for a, v in src.attrs.items():
+            dst.attrs[a] = v
+        src.copy(src["params"], dst, name="params")
+        dst.create_group("symmetry")
+        for g in ("k", "pairs"):
+            src.copy(src["symmetry/" + g], dst["symmetry"], name=g)
     print("wrote", os.listdir(OUT))
 
 

@@ -15,6 +15,14 @@ def test_library_exports_every_header_symbol():
         assert hasattr(lib, n), n
 
 
+def test_reader_library_exports_every_header_symbol():
+    h5 = gl.load_h5()
+    names = gl.header_functions(gl.H5_HEADER)
+    assert len(names) >= 7
+    for n in names:
+        assert hasattr(h5, n), n
+
+
 def _create(**kw):
     lib = gl.load()
     base = dict(nk=1, mesh=(1, 1, 1), nao=2, nso=2, ns=1, NQ=3, nts=4, nw_b=3, beta=10.0)

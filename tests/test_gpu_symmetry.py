@@ -1,0 +1,127 @@
+"""Irreducible-wedge entry points and the file loader on the device -- needs an MI355X.
+
+All through the C ABI:
+* the reference's nk = 27 GW fixture (test/solvers_test.cpp:389) end to end from its own files:
+  gwhip_h5_load (integrals, k +- q maps, symmetry tables) then gwhip_eval_sigma_ibz with G on the
+  4 irreducible points; against result/Sigma_tau at the reference's 1e-6 and against the oracle's
+  full-mesh Sigma gathered at the irreducible points at 1e-10 relative;
+* the reference's nk = 27 HF fixture (solvers_test.cpp:387) through gwhip_hf_ibz;
+* a synthetic system with random unitary U and time reversal on some points (nk = 8, ns = 2):
+  device unfolding equals the oracle's value_AO, Sigma and F to 1e-10 relative;
+* ink = nk with the identity tables gives the full-mesh result bit for bit.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from gwhip import lib as gl
+from helpers import GOLDEN, rel_err, synthetic, transformer
+from oracle import gw_oracle as go
+from oracle import hf_oracle as ho
+from oracle import integrals as oi
+from oracle import symmetry as osy
+
+pytestmark = pytest.mark.gpu
+
+H5 = os.path.join(GOLDEN, "h5")
+
+
+def _sym():
+    z = np.load(os.path.join(GOLDEN, "sym27.npz"), allow_pickle=False)
+    return {k: z[k] for k in z.files}
+
+
+def test_gw_sym27_from_files():
+    z = _sym()
+    g = np.load(os.path.join(GOLDEN, "gw_sym27.npz"), allow_pickle=False)
+    T = transformer(float(g["lam"]), float(g["beta"]), g["tau"])
+    k = gl.GwHipKernel(27, 2, 2, 2, 36, T.nts, T.nw_b, float(g["beta"]), mesh=(3, 3, 3))
+    try:
+        k.set_grid(T.T_bt, T.T_tb)
+        k.load_files(os.path.join(H5, "gw_sym27_df"), os.path.join(H5, "sym27_input.h5"))
+        S = k.solve_ibz(g["G_tau"])
+    finally:
+        k.close()
+    err_ref = np.abs(S - g["Sigma_tau"]).max()
+    Vp = oi.packed_from_irreducible(g["V"], 27, z["pairs_conj_pairs_list"], z["pairs_trans_pairs_list"],
+                                    z["pairs_kpair_irre_list"])
+    G = osy.unfold(g["G_tau"], z["k_bz2ibz"], z["k_ibz2bz"], z["k_k_sym_transform_ao"], z["k_tr_conj"])
+    S_or = osy.full_to_ibz(go.gw_sigma(G, Vp, T.T_bt, T.T_tb, 27, 2, 2, 2, 36, mesh=(3, 3, 3)), z["k_ibz2bz"])
+    err_or = rel_err(S, S_or)
+    print(f"nk=27 GW fixture: {err_ref:.2e} abs vs reference, {err_or:.2e} rel vs oracle", flush=True)
+    assert err_ref < 1e-6
+    assert err_or < 1e-10
+
+
+def test_hf_sym27_fixture():
+    z = _sym()
+    h = np.load(os.path.join(GOLDEN, "hf_sym27.npz"), allow_pickle=False)
+    Vp = oi.packed_from_irreducible(h["V"], 27, z["pairs_conj_pairs_list"], z["pairs_trans_pairs_list"],
+                                    z["pairs_kpair_irre_list"])
+    S_ibz = osy.full_to_ibz(z["S_k"], z["k_ibz2bz"])
+    dm = ho.dm_from_G(h["G_tau"][-1], 2, False)
+    k = gl.GwHipKernel(27, 2, 2, 2, 36, 4, 3, 100.0, mesh=(3, 3, 3))
+    try:
+        k.upload_V(Vp)
+        k.set_k_symmetry(4, z["k_bz2ibz"], z["k_ibz2bz"], z["k_k_sym_transform_ao"], z["k_tr_conj"])
+        F = k.hf_ibz(dm, S_ibz, float(z["madelung"]))
+    finally:
+        k.close()
+    err = np.abs(F - h["Sigma1"]).max()
+    print(f"nk=27 HF fixture: {err:.2e} abs", flush=True)
+    assert err < 1e-6 and err < 1e-11
+
+
+def _random_symmetry(nk, n, seed):
+    """Representatives 0..ink-1 of a made-up wedge, random unitary U, time reversal on odd k."""
+    rng = np.random.default_rng(seed)
+    ink = 3
+    i2b = np.arange(ink)
+    b2i = np.array([k if k < ink else k % ink for k in range(nk)], np.int64)
+    U = np.empty((nk, n, n), complex)
+    for k in range(nk):
+        q, r = np.linalg.qr(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+        U[k] = np.eye(n) if k < ink else q
+    tr = np.array([1 if (k >= ink and k % 2) else 0 for k in range(nk)], np.int64)
+    return ink, b2i, i2b, U, tr
+
+
+def test_random_wedge_matches_oracle():
+    G, V, T, d = synthetic(8, 5, 12, ns=2, seed=3)
+    ink, b2i, i2b, U, tr = _random_symmetry(8, 5, 4)
+    G_ibz = np.ascontiguousarray(G[:, :, :ink])
+    rng = np.random.default_rng(5)
+    x = rng.standard_normal((2, ink, 5, 5)) + 1j * rng.standard_normal((2, ink, 5, 5))
+    dm_ibz = 0.5 * (x + np.conj(np.swapaxes(x, -1, -2)))
+    S_ibz = np.broadcast_to(np.eye(5), dm_ibz.shape).astype(complex)
+    k = gl.GwHipKernel(8, 5, 5, 2, 12, T.nts, T.nw_b, d["beta"], mesh=d["mesh"])
+    try:
+        k.set_grid(T.T_bt, T.T_tb)
+        k.upload_V(V)
+        k.set_k_symmetry(ink, b2i, i2b, U, tr)
+        S = k.solve_ibz(G_ibz)
+        F = k.hf_ibz(dm_ibz, S_ibz, 0.3)
+    finally:
+        k.close()
+    Gf = osy.unfold(G_ibz, b2i, i2b, U, tr)
+    S_or = osy.full_to_ibz(go.gw_sigma(Gf, V, T.T_bt, T.T_tb, 8, 5, 5, 2, 12, mesh=d["mesh"]), i2b)
+    dmf = osy.unfold(dm_ibz, b2i, i2b, U, tr)
+    Sf = osy.unfold(S_ibz, b2i, i2b, U, tr)
+    F_or = osy.full_to_ibz(ho.hf_scalar(dmf, V, Sf, 0.3, 8, 5, 2, 12), i2b)
+    print(f"random wedge: Sigma {rel_err(S, S_or):.2e}, F {rel_err(F, F_or):.2e}", flush=True)
+    assert rel_err(S, S_or) < 1e-10 and rel_err(F, F_or) < 1e-10
+
+
+def test_identity_wedge_is_the_full_mesh():
+    G, V, T, d = synthetic(8, 4, 10, ns=1, seed=6)
+    k = gl.GwHipKernel(8, 4, 4, 1, 10, T.nts, T.nw_b, d["beta"], mesh=d["mesh"])
+    try:
+        k.set_grid(T.T_bt, T.T_tb)
+        k.upload_V(V)
+        S_full = k.solve(G)
+        k.set_k_symmetry(8, np.arange(8), np.arange(8), np.stack([np.eye(4, dtype=complex)] * 8), np.zeros(8, np.int64))
+        S_ibz = k.solve_ibz(G)
+    finally:
+        k.close()
+    assert np.array_equal(S_full, S_ibz)

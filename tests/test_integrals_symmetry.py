@@ -1,0 +1,169 @@
+"""The on-disk integral reader and the irreducible-wedge maps -- CPU only.
+
+Oracle pins (oracle/integrals.py, oracle/symmetry.py):
+* the reference's nk = 27 HF fixture (test/solvers_test.cpp:387; H2, ns = 2, nao = 2, NQ = 36,
+  4 irreducible k points, 196 of 378 k pairs stored): full-mesh HF from the expanded pairs and the
+  unfolded density matrix, gathered at the irreducible points, against result/Sigma1;
+* the reference's nk = 27 GW fixture (test/solvers_test.cpp:389) the same way through the GW
+  oracle, against result/Sigma_tau at the reference's 1e-6;
+* planted mistakes (in the oracle, on the CPU): conj and trans pair operations swapped, the
+  transform applied as U^H X U -- each rejected by the HF fixture.
+* the time-reversal branch of value_AO (tr_conj = 1) appears in no fixture with a non-identity
+  U; only the closed form (G(-k) = conj G(k) in a real AO basis, U = I) pins it: parity unpinned
+  for tr_conj = 1 with U != I.
+
+Host reader (libgwhip_h5.so, no GPU needed):
+* the version rule against the reference's own pass/fail strings (solvers_test.cpp:497-508) and
+  its four meta.h5 files (:470-495);
+* the packed integrals read from the reference's files equal the oracle's expansion bit for bit;
+  the symmetry tables equal the input file's; k +- q from fractional coordinates equal the
+  Monkhorst-Pack index arithmetic;
+* missing files and wrong shapes give ERR_IO.
+"""
+import os
+
+import numpy as np
+import pytest
+
+from gwhip import lib as gl
+from helpers import GOLDEN, transformer
+from oracle import gw_oracle as go
+from oracle import hf_oracle as ho
+from oracle import integrals as oi
+from oracle import symmetry as osy
+
+H5 = os.path.join(GOLDEN, "h5")
+DF = os.path.join(H5, "gw_sym27_df")
+INPUT = os.path.join(H5, "sym27_input.h5")
+
+
+def _sym():
+    z = np.load(os.path.join(GOLDEN, "sym27.npz"), allow_pickle=False)
+    return {k: z[k] for k in z.files}
+
+
+def _pairs(z):
+    return z["pairs_conj_pairs_list"], z["pairs_trans_pairs_list"], z["pairs_kpair_irre_list"]
+
+
+def _hf_fixture(z, v_irre, transform=None):
+    h = np.load(os.path.join(GOLDEN, "hf_sym27.npz"), allow_pickle=False)
+    nk, nao, ns, NQ = int(z["nk"]), int(z["nao"]), int(z["ns"]), int(z["NQ"])
+    Vp = oi.packed_from_irreducible(h["V"] if v_irre is None else v_irre, nk, *_pairs(z))
+    dm_ibz = ho.dm_from_G(h["G_tau"][-1], ns, False)
+    if transform is None:
+        dm = osy.unfold(dm_ibz, z["k_bz2ibz"], z["k_ibz2bz"], z["k_k_sym_transform_ao"], z["k_tr_conj"])
+    else:
+        dm = np.empty((ns, nk, nao, nao), complex)
+        for s in range(ns):
+            for k in range(nk):
+                ik = osy.reduced_point(z["k_bz2ibz"], z["k_ibz2bz"], k)
+                x = dm_ibz[s, ik]
+                dm[s, k] = x if z["k_ibz2bz"][ik] == k else transform(x, z["k_k_sym_transform_ao"][k])
+    F = ho.hf_scalar(dm, Vp, z["S_k"], float(z["madelung"]), nk, nao, ns, NQ)
+    return osy.full_to_ibz(F, z["k_ibz2bz"]), h["Sigma1"]
+
+
+def test_hf_sym27_fixture():
+    F, ref = _hf_fixture(_sym(), None)
+    err = np.abs(F - ref).max()
+    assert err < 1e-6      # reference tolerance (IsCloseTo default)
+    assert err < 1e-11     # reached: 2.8e-13
+
+
+def test_gw_sym27_fixture():
+    z = _sym()
+    g = np.load(os.path.join(GOLDEN, "gw_sym27.npz"), allow_pickle=False)
+    nk, nao, ns, NQ = int(z["nk"]), int(z["nao"]), int(z["ns"]), int(z["NQ"])
+    Vp = oi.packed_from_irreducible(g["V"], nk, *_pairs(z))
+    G = osy.unfold(g["G_tau"], z["k_bz2ibz"], z["k_ibz2bz"], z["k_k_sym_transform_ao"], z["k_tr_conj"])
+    T = transformer(float(g["lam"]), float(g["beta"]), g["tau"])
+    S = go.gw_sigma(G, Vp, T.T_bt, T.T_tb, nk, nao, nao, ns, NQ, mesh=(3, 3, 3))
+    err = np.abs(osy.full_to_ibz(S, z["k_ibz2bz"]) - g["Sigma_tau"]).max()
+    assert err < 1e-6      # reference tolerance (solvers_test.cpp:134)
+    assert err < 2e-8      # reached: 8.9e-9 (full mesh here, q stars with rotated P0 in the reference)
+
+
+def test_planted_pair_operation_swap_is_rejected(monkeypatch):
+    z = _sym()
+    orig = oi.pair_source
+
+    def swapped(*a):
+        row, sign, op = orig(*a)
+        return row, sign, {"conj": "trans", "trans": "conj"}.get(op, op)
+
+    monkeypatch.setattr(oi, "pair_source", swapped)
+    F, ref = _hf_fixture(z, None)
+    assert np.abs(F - ref).max() > 1e-6
+
+
+def test_planted_transform_order_is_rejected():
+    F, ref = _hf_fixture(_sym(), None, transform=lambda x, u: np.conj(u.T) @ x @ u)
+    assert np.abs(F - ref).max() > 1e-6
+
+
+def test_time_reversal_closed_form():
+    """A time-reversal partner with U = I is the complex conjugate (real AO basis)."""
+    rng = np.random.default_rng(0)
+    x = rng.standard_normal((1, 2, 2)) + 1j * rng.standard_normal((1, 2, 2))
+    y = osy.value_AO(x, 1, np.array([0, 0]), np.array([0]), np.stack([np.eye(2)] * 2), np.array([0, 1]))
+    assert np.array_equal(y, np.conj(x[0]))
+
+
+# ------------------------------------------------------------------ host reader (libgwhip_h5.so)
+
+FAIL = ["0.2.4", "0.2.4b10", "0.3.0", "0.3.0b8", "0.3.1", "0.3.1b10"]
+PASS = ["1.0.0", "1.1.0", "1.1.2b"]
+
+
+def test_version_strings():
+    for v in FAIL:
+        assert gl.h5_check_version(v) == 0 and not oi.check_version(v), v
+    for v in PASS:
+        assert gl.h5_check_version(v) == 1 and oi.check_version(v), v
+    assert gl.h5_check_version("1.0") == -1
+    with pytest.raises(ValueError):
+        oi.check_version("1.0")
+
+
+def test_meta_versions():
+    with pytest.raises(gl.OutdatedInput):
+        gl.h5_read_meta(os.path.join(H5, "versions", "df_int"))
+    with pytest.raises(gl.OutdatedInput):
+        gl.h5_read_meta(os.path.join(H5, "versions", "df_int_0.3.0"))
+    assert gl.h5_read_meta(os.path.join(H5, "versions", "df_int_1.0.0")) == 196
+    assert gl.h5_read_meta(os.path.join(H5, "versions", "df_int_1.1.0")) == 196
+
+
+def test_read_packed_matches_oracle_expansion():
+    z = _sym()
+    g = np.load(os.path.join(GOLDEN, "gw_sym27.npz"), allow_pickle=False)
+    V = gl.h5_read_packed(DF, INPUT, 27, 2, 36)
+    ref = oi.packed_from_irreducible(g["V"], 27, *_pairs(z))
+    assert np.array_equal(V, ref)
+
+
+def test_read_k_symmetry_and_maps():
+    z = _sym()
+    s = gl.h5_read_k_symmetry(INPUT, 27, 2)
+    assert s["ink"] == int(z["ink"]) == 4
+    assert np.array_equal(s["bz2ibz"], z["k_bz2ibz"]) and np.array_equal(s["ibz2bz"], z["k_ibz2bz"])
+    assert np.array_equal(s["U"], z["k_k_sym_transform_ao"]) and np.array_equal(s["tr_conj"], z["k_tr_conj"])
+    kp, km = gl.h5_kq_maps(s["mesh_scaled"])
+    okp, okm = osy.kq_maps(z["k_mesh_scaled"])
+    assert np.array_equal(kp, okp) and np.array_equal(km, okm)
+    for q in range(27):
+        for k in range(27):
+            assert kp[q, k] == go.k_add(k, q, (3, 3, 3), 1) and km[q, k] == go.k_add(k, q, (3, 3, 3), -1)
+
+
+def test_reader_errors():
+    with pytest.raises(gl.GwhipError) as e:
+        gl.h5_read_packed(os.path.join(H5, "no_such_dir"), INPUT, 27, 2, 36)
+    assert e.value.status == gl.ERR_IO
+    with pytest.raises(gl.GwhipError) as e:
+        gl.h5_read_packed(DF, INPUT, 27, 3, 36)       # wrong nao
+    assert e.value.status == gl.ERR_IO
+    with pytest.raises(gl.GwhipError) as e:
+        gl.h5_read_packed(DF, INPUT, 26, 2, 36)       # pair lists of another mesh
+    assert e.value.status == gl.ERR_IO
