@@ -25,6 +25,7 @@
 #include <green/utils/mpi_shared.h>
 
 #include <functional>
+#include <iostream>
 #include <memory>
 #include <string>
 #include <tuple>
@@ -95,7 +96,7 @@ namespace green::mbpt::hip {
   class gw_hip_kernel {
   public:
     gw_hip_kernel(bool X2C, const params::params& p, size_t nao, size_t nso, size_t ns, size_t NQ,
-                  const grids::transformer_t& ft, const bz_utils_t& bz) : _ft(ft), _bz(bz) {
+                  const grids::transformer_t& ft, const bz_utils_t& bz, const ztensor<4>& S_k) : _ft(ft), _bz(bz) {
       detail::require_one_node();
       _ibz  = bz.ink() != bz.nk();
       _nts  = ft.sd().repn_fermi().nts();
@@ -115,6 +116,19 @@ namespace green::mbpt::hip {
         const std::string path = p["dfintegral_file"];
         detail::load_files(h, bz, path, p["input_file"]);
         detail::resident() = {_h, path};
+        // q -> 0 treatment EXTRAPOLATE (kernels.h:48, mbpt_q0_utils_t.h:19-38): AqQ.h5 of the same
+        // directory; S_k on the irreducible points, as the solver passes it
+        const sigma_q0_treatment_e q0 = p["q0_treatment"];
+        if (q0 == extrapolate) {
+          int enabled = 0;
+          const std::string input = p["input_file"];
+          check(gwhip_h5_load_q0(h, path.c_str(), input.c_str(), bz.ink(), reinterpret_cast<const double*>(S_k.data()),
+                                 &enabled),
+                nullptr);
+          if (!enabled)
+            std::cout << "## Warning: " << path << "/AqQ.h5 is not found! Extrapolation treatment for GW self-energy will be disabled."
+                      << std::endl;
+        }
       }
     }
 
@@ -163,8 +177,8 @@ namespace green::mbpt::hip {
   /** GW factory hook (src/green/mbpt/kernel_factory.h:28-31). */
   inline std::tuple<std::shared_ptr<void>, std::function<void(G_type&, G_type&)>> custom_gw_kernel(
       bool X2C, const params::params& p, size_t nao, size_t nso, size_t ns, size_t NQ, const grids::transformer_t& ft,
-      const bz_utils_t& bz_utils, const ztensor<4>& /*S_k*/) {
-    auto kernel = std::make_shared<gw_hip_kernel>(X2C, p, nao, nso, ns, NQ, ft, bz_utils);
+      const bz_utils_t& bz_utils, const ztensor<4>& S_k) {
+    auto kernel = std::make_shared<gw_hip_kernel>(X2C, p, nao, nso, ns, NQ, ft, bz_utils, S_k);
     std::function<void(G_type&, G_type&)> callback = [kernel](G_type& g, G_type& s) { kernel->solve(g, s); };
     return std::tuple{std::static_pointer_cast<void>(kernel), callback};
   }

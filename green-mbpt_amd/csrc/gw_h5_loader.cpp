@@ -371,6 +371,44 @@ int gwhip_h5_load(gwhip_handle* h, const char* df_path, const char* input_h5, in
   });
 }
 
+int gwhip_h5_load_q0(gwhip_handle* h, const char* df_path, const char* input_h5, int64_t nk_S, const double* S,
+                     int* enabled) {
+  if (!h || !df_path || !input_h5 || !S || !enabled) return GWHIP_ERR_ARG;
+  return guarded([&] {
+    *enabled = 0;
+    gwhip_dims d{};
+    int st = gwhip_get_dims(h, &d);
+    if (st != GWHIP_OK) throw h5_error(st, "gwhip_get_dims failed");
+    const std::string aq = std::string(df_path) + "/AqQ.h5";
+    quiet();
+    if (H5Fis_hdf5(aq.c_str()) <= 0) return;  // no extrapolation data: the correction stays off
+    Id f(open_file(aq), H5Fclose);
+    const auto dims = dims_of(f.get(), "AqQ");
+    if (dims.size() != 2 || (int64_t)dims[1] != d.NQ)
+      throw h5_error(GWHIP_ERR_IO, "AqQ is not [inq][NQ] for NQ = " + std::to_string(d.NQ));
+    const int64_t inq = (int64_t)dims[0];
+    std::vector<std::complex<double>> A((size_t)(inq * d.NQ));
+    read_complex(f.get(), "AqQ", A.data(), A.size());
+    std::vector<double> qabs(inq);
+    if (count_of(dims_of(f.get(), "q_abs")) != (size_t)inq) throw h5_error(GWHIP_ERR_IO, "q_abs is not [inq]");
+    read_into(f.get(), "q_abs", H5T_NATIVE_DOUBLE, qabs.data(), qabs.size());
+    double mad = 0.0;
+    read_into(f.get(), "madelung", H5T_NATIVE_DOUBLE, &mad, 1);
+    std::vector<int64_t> qir(inq);
+    Id in(open_file(input_h5), H5Fclose);
+    if (has_path(in.get(), "symmetry/q/ibz2bz")) {
+      qir = read_i64(in.get(), "symmetry/q/ibz2bz");
+      if ((int64_t)qir.size() != inq) throw h5_error(GWHIP_ERR_IO, "symmetry/q/ibz2bz does not have inq entries");
+    } else {
+      for (int64_t i = 0; i < inq; ++i) qir[i] = i;
+    }
+    st = gwhip_set_q0_extrapolation(h, inq, qir.data(), reinterpret_cast<const double*>(A.data()), qabs.data(), mad,
+                                    nk_S, S);
+    if (st != GWHIP_OK) throw h5_error(st, gwhip_last_error(h));
+    *enabled = 1;
+  });
+}
+
 const char* gwhip_h5_last_error(void) { return g_err.c_str(); }
 
 }  // extern "C"

@@ -125,7 +125,8 @@ gw_hip_kernel::~gw_hip_kernel() {
   (void)hipSetDevice(device_);
   if (stream_) (void)hipStreamSynchronize(stream_);
   for (void* p : {(void*)Vq_, (void*)Vt_, (void*)P_, (void*)Tbt_fold_, (void*)Ttb_, (void*)G_dev_, (void*)S_dev_,
-                  (void*)kplus_tab_, (void*)kminus_tab_, (void*)flag_, (void*)hcheck_, (void*)hf_work_, (void*)hf_x_, (void*)ksym_idx_, (void*)ksym_U_, (void*)hf_sym_})
+                  (void*)kplus_tab_, (void*)kminus_tab_, (void*)flag_, (void*)hcheck_, (void*)hf_work_, (void*)hf_x_, (void*)ksym_idx_, (void*)ksym_U_, (void*)hf_sym_, (void*)q0_qiq_, (void*)q0_A_, (void*)q0_eps_dev_,
+                  (void*)q0_S_, (void*)q0_tmp_, (void*)q0_epst_})
     if (p) (void)hipFree(p);
   if (own_ws_)
     for (void* p : {(void*)Pw_, (void*)X_, (void*)Y_, (void*)scratch_})
@@ -193,6 +194,7 @@ void gw_hip_kernel::set_grid(const double2* T_bt, const double2* T_tb) {
   GWHIP_CHECK(hipSetDevice(device_));
   GWHIP_CHECK(hipMemcpy(Tbt_fold_, fold.data(), fold.size() * sizeof(double2), hipMemcpyHostToDevice));
   GWHIP_CHECK(hipMemcpy(Ttb_, T_tb, (size_t)nts * nwb * sizeof(double2), hipMemcpyHostToDevice));
+  Ttb_host_.assign(T_tb, T_tb + (size_t)nts * nwb);
   grid_set_ = true;
   stored_valid_ = false;
 }
@@ -672,6 +674,11 @@ void gw_hip_kernel::stage_dyson() {
       a.alpha = make_double2(1.0, 0.0);
       zgemm_f64(a, true, false, stream_);
     }
+    if (q0_eps_) {  // aux_to_PW_00 for the irreducible q of this chunk
+      hipLaunchKernelGGL(q0_eps_kernel, dim3((unsigned)nwb, (unsigned)qc), dim3(256), 0, stream_, Y_, q0_A_, q0_qiq_,
+                         (int)q0, (int)nwb, (int)NQ, (int)q0_inq_, q0_eps_dev_);
+      GWHIP_CHECK(hipGetLastError());
+    }
     mark(e2);
     {  // P-tilde(i Omega_b) -> P-tilde(tau_f), all nts
       ZGemmArgs a{};
@@ -756,7 +763,7 @@ void gw_hip_kernel::enqueue_eval(const double2* G, double2* Sigma, bool split, i
   }
 }
 
-void gw_hip_kernel::solve_device(const double2* G, double2* Sigma, int precision) {
+void gw_hip_kernel::solve_device(const double2* G, double2* Sigma, int precision, bool q0_apply) {
   check_ready(precision);
   if (!G || !Sigma) throw status_error(GWHIP_ERR_ARG, "null G or Sigma");
   const bool split = precision == GWHIP_PREC_SPLIT;
@@ -782,10 +789,13 @@ void gw_hip_kernel::solve_device(const double2* G, double2* Sigma, int precision
                                  hipMemcpyDeviceToDevice, stream_));
   }
   GWHIP_CHECK(hipEventRecord(t1, stream_));
+  q0_eps_ = q0_on_;
   stage_dyson();
+  q0_eps_ = false;
   GWHIP_CHECK(hipEventRecord(t2, stream_));
   stage_sigma(G, Sigma, split);
   finish_sigma(Sigma);
+  if (q0_on_ && q0_apply) apply_q0(Sigma, G, d_.nk);
   GWHIP_CHECK(hipEventRecord(t3, stream_));
   hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, stream_, flag_, (int64_t)1);
   hipLaunchKernelGGL(nonfinite_kernel, dim3(1024), dim3(256), 0, stream_, Sigma, nsig, flag_);
@@ -908,6 +918,7 @@ void gw_hip_kernel::run_stages_host(const double2* G, int last_stage) {
 #include "gw_vjp_host.hpp"
 #include "gw_hf_host.hpp"
 #include "gw_symmetry.hpp"
+#include "gw_q0.hpp"
 #include "gw_batch.hpp"
 
 namespace gwhip {
@@ -1399,6 +1410,22 @@ int gwhip_hf_ibz_device(gwhip_handle* h, const void* dm_ibz, const void* S_ibz, 
   if (!h) return GWHIP_ERR_ARG;
   return guarded(h,
                  [&] { h->k->hf_ibz_device((const double2*)dm_ibz, (const double2*)S_ibz, madelung, (double2*)F_ibz); });
+}
+
+int gwhip_set_q0_extrapolation(gwhip_handle* h, int64_t inq, const int64_t* q_ir, const double* AqQ,
+                               const double* q_abs, double madelung, int64_t nk_S, const double* S) {
+  if (!h) return GWHIP_ERR_ARG;
+  return guarded(h, [&] {
+    h->k->set_q0_extrapolation(inq, q_ir, (const double2*)AqQ, q_abs, madelung, nk_S, (const double2*)S);
+  });
+}
+
+int gwhip_get_q0_eps(const gwhip_handle* h, double* eps_tau) {
+  if (!h || !eps_tau) return GWHIP_ERR_ARG;
+  const auto& e = h->k->q0_eps_tau();
+  if (e.empty()) return GWHIP_ERR_STATE;
+  std::memcpy(eps_tau, e.data(), e.size() * sizeof(double2));
+  return GWHIP_OK;
 }
 
 int gwhip_get_dims(const gwhip_handle* h, gwhip_dims* out) {

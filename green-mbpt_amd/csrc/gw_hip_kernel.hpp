@@ -89,7 +89,9 @@ public:
   void upload_V(const double2* V_packed);
   void generate_V(uint64_t seed, double sigma);
   // G_dev, Sigma_dev: device pointers [nts][ns][nk][nso][nso]
-  void solve_device(const double2* G_dev, double2* Sigma_dev, int precision);
+  // q0_apply: add the q -> 0 correction here (full-mesh entry); the irreducible entry applies it
+  // after gathering instead
+  void solve_device(const double2* G_dev, double2* Sigma_dev, int precision, bool q0_apply = true);
   void solve_host(const double2* G, double2* Sigma, int precision);
   // test hook: stop after stage 1 (P0) or 2 (P-tilde); stage 3 runs Sigma as usual
   void run_stages_host(const double2* G, int last_stage);
@@ -109,6 +111,13 @@ public:
   void set_k_symmetry(int64_t ink, const int64_t* bz2ibz, const int64_t* ibz2bz, const double2* U,
                       const int64_t* tr_conj);
   int64_t ink() const { return ink_; }
+  // q -> 0 extrapolation (q0_treatment EXTRAPOLATE): inq irreducible q (full-mesh indices q_ir),
+  // AqQ [inq][NQ], q_abs [inq], the Madelung constant of AqQ.h5 and the overlap S [ns][nkS][nso^2]
+  // with nkS = nk (full-mesh entry) or ink (irreducible entry).  inq = 0 switches it off.
+  void set_q0_extrapolation(int64_t inq, const int64_t* q_ir, const double2* AqQ, const double* q_abs,
+                            double madelung, int64_t nkS, const double2* S);
+  bool q0_enabled() const { return q0_on_; }
+  const std::vector<double2>& q0_eps_tau() const { return q0_eps_t_; }
   void solve_ibz_device(const double2* G_ibz, double2* Sigma_ibz, int precision);
   void solve_ibz_host(const double2* G_ibz, double2* Sigma_ibz, int precision);
   void hf_ibz_device(const double2* dm_ibz, const double2* S_ibz, double madelung, double2* F_ibz);
@@ -208,6 +217,18 @@ private:
   // k symmetry: [IBZ ordinal | representative flag | time reversal | ibz2bz] ints, U per k, and
   // the full-mesh dm, S, F and scratch of the irreducible HF entry
   int64_t ink_ = 0;
+  // q -> 0 extrapolation
+  bool q0_on_ = false, q0_eps_ = false;
+  int64_t q0_inq_ = 0, q0_nS_ = 0;
+  double q0_mad_ = 0.0;
+  std::vector<double> q0_qabs_;
+  std::vector<double2> q0_eps_t_, Ttb_host_;
+  int* q0_qiq_ = nullptr;           // [nk] row of AqQ for an irreducible q, else -1
+  double2* q0_A_ = nullptr;         // [inq][NQ]
+  double2* q0_eps_dev_ = nullptr;   // [nw_b][inq]
+  double2* q0_S_ = nullptr;         // [ns][nkS][nso^2]
+  double2* q0_tmp_ = nullptr;       // G S, [nts][ns][nkS][nso^2]
+  double2* q0_epst_ = nullptr;      // [nts]
   int* ksym_idx_ = nullptr;
   double2* ksym_U_ = nullptr;
   double2* hf_sym_ = nullptr;
@@ -229,6 +250,8 @@ private:
   void zero(double2* X, int64_t n);
   void unfold(const double2* X, double2* T, double2* Y, int64_t nb);
   void gather_ibz(const double2* Full, double2* Ibz, int64_t nb);
+  std::vector<int> q0_samples() const;
+  void apply_q0(double2* Sigma, const double2* G, int64_t nkS);
   void p0_prep(int64_t q, const double2* G, int64_t t0, int64_t tc, int qpw);
   void p0_prep_slice(int64_t q, const double2* G, int64_t t0, int64_t tc, int qpw);
   void p0_gemm(int64_t q, int64_t tc, double2* Pt0, int shape);

@@ -286,4 +286,38 @@ __global__ void nonfinite_kernel(const double2* X, int64_t n, int* flag) {
   if (bad) atomicOr(flag, 1);
 }
 
+// q -> 0 extrapolation (aux_to_PW_00, src/mbpt_q0_utils_t.cpp:11-20): eps[n][iq] = conj(A_iq)^T Y(n) A_iq
+// for the q of this Dyson chunk that are irreducible; one block per (n, q in the chunk).
+__global__ __launch_bounds__(256) void q0_eps_kernel(const double2* __restrict__ Y, const double2* __restrict__ A,
+                                                     const int* __restrict__ qiq, int q0, int nwb, int NQ, int inq,
+                                                     double2* __restrict__ eps) {
+  const int n = blockIdx.x, qc = blockIdx.y;
+  const int iq = qiq[q0 + qc];
+  if (iq < 0) return;
+  const int64_t NQ2 = (int64_t)NQ * NQ;
+  const double2* P = Y + ((int64_t)qc * nwb + n) * NQ2;
+  const double2* a = A + (int64_t)iq * NQ;
+  double sr = 0.0, si = 0.0;
+  for (int64_t e = threadIdx.x; e < NQ2; e += blockDim.x) {
+    const int p = (int)(e / NQ), r = (int)(e - (e / NQ) * NQ);
+    const double2 x = P[e], ap = a[p], ar = a[r];
+    // conj(ap) * x * ar
+    const double tr = ap.x * x.x + ap.y * x.y, ti = ap.x * x.y - ap.y * x.x;
+    sr += tr * ar.x - ti * ar.y;
+    si += tr * ar.y + ti * ar.x;
+  }
+  __shared__ double red[2][256];
+  red[0][threadIdx.x] = sr;
+  red[1][threadIdx.x] = si;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o) {
+      red[0][threadIdx.x] += red[0][threadIdx.x + o];
+      red[1][threadIdx.x] += red[1][threadIdx.x + o];
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) eps[(int64_t)n * inq + iq] = make_double2(red[0][0], red[1][0]);
+}
+
 }  // namespace gwhip

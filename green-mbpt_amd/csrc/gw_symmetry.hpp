@@ -147,8 +147,9 @@ void gw_hip_kernel::solve_ibz_device(const double2* G_ibz, double2* Sigma_ibz, i
   if (!G_dev_) G_dev_ = dalloc<double2>(nsig);
   if (!S_dev_) S_dev_ = dalloc<double2>(nsig);
   unfold(G_ibz, S_dev_, G_dev_, d_.nts * d_.ns);  // Sigma's buffer is the scratch until the eval
-  solve_device(G_dev_, S_dev_, precision);
+  solve_device(G_dev_, S_dev_, precision, false);
   gather_ibz(S_dev_, Sigma_ibz, d_.nts * d_.ns);
+  if (q0_on_) apply_q0(Sigma_ibz, G_ibz, ink_);  // on the irreducible points, as the reference does
   GWHIP_CHECK(hipStreamSynchronize(stream_));
 }
 

@@ -127,13 +127,16 @@ def load() -> ctypes.CDLL:
     lib.gwhip_hf_ibz.argtypes = [P, P, P, ctypes.c_double, P]
     lib.gwhip_hf_ibz_device.argtypes = [P, P, P, ctypes.c_double, P]
     lib.gwhip_get_dims.argtypes = [P, ctypes.POINTER(Dims)]
+    lib.gwhip_set_q0_extrapolation.argtypes = [P, ctypes.c_int64, P, P, P, ctypes.c_double, ctypes.c_int64, P]
+    lib.gwhip_get_q0_eps.argtypes = [P, P]
     for f in ("gwhip_debug_zgemm_split", "gwhip_set_split_slices", "gwhip_set_kq_maps", "gwhip_debug_mfma_peak",
               "gwhip_debug_gemm_bench", "gwhip_debug_zgemm", "gwhip_debug_stage", "gwhip_set_grid", "gwhip_upload_V",
               "gwhip_generate_V", "gwhip_eval_sigma", "gwhip_eval_sigma_device", "gwhip_get_P_tilde", "gwhip_timings",
               "gwhip_kernel_stats", "gwhip_set_vjp_store", "gwhip_vjp", "gwhip_vjp_device", "gwhip_autotune",
               "gwhip_get_tuning", "gwhip_set_tuning", "gwhip_batch_create", "gwhip_batch_autotune",
               "gwhip_batch_eval_device", "gwhip_batch_stats", "gwhip_set_k_symmetry", "gwhip_eval_sigma_ibz",
-              "gwhip_eval_sigma_ibz_device", "gwhip_hf_ibz", "gwhip_hf_ibz_device", "gwhip_get_dims"):
+              "gwhip_eval_sigma_ibz_device", "gwhip_hf_ibz", "gwhip_hf_ibz_device", "gwhip_get_dims",
+              "gwhip_set_q0_extrapolation", "gwhip_get_q0_eps"):
         getattr(lib, f).restype = ctypes.c_int
     _lib = lib
     return lib
@@ -171,6 +174,8 @@ def load_h5() -> ctypes.CDLL:
     h5.gwhip_h5_read_k_symmetry.argtypes = [ctypes.c_char_p, ctypes.c_int64, ctypes.c_int64, I64P, P, P, P, P, P]
     h5.gwhip_h5_kq_maps.argtypes = [P, ctypes.c_int64, P, P]
     h5.gwhip_h5_load.argtypes = [P, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
+    h5.gwhip_h5_load_q0.argtypes = [P, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int64, P,
+                                    ctypes.POINTER(ctypes.c_int)]
     h5.gwhip_h5_last_error.restype = ctypes.c_char_p
     _h5 = h5
     return h5
@@ -403,6 +408,29 @@ class GwHipKernel:
         _h5check(st)
         if with_symmetry:
             self.dims["ink"] = h5_read_k_symmetry(input_h5, self.dims["nk"], self.dims["nso"])["ink"]
+
+    def set_q0_extrapolation(self, q_ir, AqQ, q_abs, madelung: float, S):
+        """q -> 0 (EXTRAPOLATE) correction; S [ns, nk or ink, nso, nso]; an empty q_ir switches it off."""
+        qi = np.ascontiguousarray(q_ir, np.int64)
+        A = np.ascontiguousarray(AqQ, np.complex128)
+        qa = np.ascontiguousarray(q_abs, np.float64)
+        Sm = np.ascontiguousarray(S, np.complex128)
+        self._check(self.lib.gwhip_set_q0_extrapolation(self.h, qi.shape[0], _ptr(qi), _ptr(A), _ptr(qa),
+                                                        float(madelung), Sm.shape[1], _ptr(Sm)))
+
+    def load_q0(self, df_path: str, input_h5: str, S) -> bool:
+        """AqQ.h5 of an integral directory; False (correction off) when the file is absent."""
+        Sm = np.ascontiguousarray(S, np.complex128)
+        en = ctypes.c_int()
+        _h5check(load_h5().gwhip_h5_load_q0(self.h, df_path.encode(), input_h5.encode(), Sm.shape[1], _ptr(Sm),
+                                            ctypes.byref(en)))
+        return bool(en.value)
+
+    def q0_eps(self) -> np.ndarray:
+        """eps_q0 on the tau grid of the last corrected evaluation [nts]."""
+        out = np.empty(self.dims["nts"], np.complex128)
+        self._check(self.lib.gwhip_get_q0_eps(self.h, _ptr(out)))
+        return out
 
     def ibz_shape(self, with_tau: bool = True):
         d = self.dims

@@ -220,6 +220,23 @@ int gwhip_hf_ibz(gwhip_handle* h, const double* dm_ibz, const double* S_ibz, dou
 int gwhip_hf_ibz_device(gwhip_handle* h, const void* dm_ibz_dev, const void* S_ibz_dev, double madelung,
                         void* F_ibz_dev);
 
+/* q -> 0 finite-size correction of Sigma (q0_treatment EXTRAPOLATE): mbpt_q0_utils_t
+ * (src/mbpt_q0_utils_t.cpp:11-178) as gw_cpu_kernel::solve applies it (src/gw_cpu_kernel.cpp:59-64,
+ * 266-270).  For each of the inq irreducible q (full-mesh index q_ir[iq]) the Dyson stage records
+ * eps(n, iq) = conj(A_iq)^T P-tilde(i Omega_n, q) A_iq; after Sigma, per frequency, Re eps over the
+ * distinct |q| in (0, 1] is fitted by a degree-2 polynomial (normal equations), the constant term
+ * is carried to tau with the T_tb of gwhip_set_grid, and
+ *   Sigma(t, s, k) -= madelung * eps_q0(t) * S(s, k) G(t, s, k) S(s, k)
+ * (X2C: aa, bb, ab blocks with S_aa, ba = ab^H) on the k points the entry point returns.
+ * AqQ [inq][NQ] complex, q_abs [inq] and madelung are AqQ.h5's; S [ns][nk_S][nso][nso] with
+ * nk_S = nk for gwhip_eval_sigma or ink for gwhip_eval_sigma_ibz.  inq = 0 switches it off.
+ * Needs at least three distinct |q| in (0, 1].  Not applied by the batch entry points. */
+int gwhip_set_q0_extrapolation(gwhip_handle* h, int64_t inq, const int64_t* q_ir, const double* AqQ,
+                               const double* q_abs, double madelung, int64_t nk_S, const double* S);
+
+/* eps_q0 on the fermionic tau grid of the last evaluation with the correction on: [nts] complex. */
+int gwhip_get_q0_eps(const gwhip_handle* h, double* eps_tau);
+
 /* The dimensions the handle was created with. */
 int gwhip_get_dims(const gwhip_handle* h, gwhip_dims* out);
 

@@ -58,6 +58,15 @@ int gwhip_h5_kq_maps(const double* mesh_scaled, int64_t nk, int32_t* kplus, int3
  * (gwhip_set_k_symmetry).  The handle's dims give nk, nao, nso and NQ. */
 int gwhip_h5_load(gwhip_handle* h, const char* df_path, const char* input_h5, int with_symmetry);
 
+/* The q -> 0 extrapolation data of <df_path>/AqQ.h5 (AqQ [inq][NQ] complex, q_abs [inq], madelung;
+ * mbpt_q0_utils_t's constructor, src/green/mbpt/mbpt_q0_utils_t.h:19-38) with the irreducible q of
+ * input_h5 (/symmetry/q/ibz2bz; all q in order without it), passed to gwhip_set_q0_extrapolation
+ * with the overlap S [ns][nk_S][nso][nso].  As in the reference, a missing AqQ.h5 switches the
+ * correction off (*enabled = 0) instead of failing; an aux size other than the handle's NQ is an
+ * error (GWHIP_ERR_IO). */
+int gwhip_h5_load_q0(gwhip_handle* h, const char* df_path, const char* input_h5, int64_t nk_S, const double* S,
+                     int* enabled);
+
 /* Message of the last failed gwhip_h5_* call on this thread. */
 const char* gwhip_h5_last_error(void);
 

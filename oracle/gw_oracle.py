@@ -130,8 +130,9 @@ def symmetrize_P0(P0, nk):
         P0[nts - it - 1] = P0[it]
 
 
-def eval_P_tilde(P0, T_bt, T_tb):
-    """``gw_cpu_kernel.cpp:227-279`` on one q: returns P-tilde(tau) [nts, NQ, NQ]."""
+def eval_P_tilde(P0, T_bt, T_tb, return_w=False):
+    """``gw_cpu_kernel.cpp:227-279`` on one q: returns P-tilde(tau) [nts, NQ, NQ] (and P-tilde on the
+    bosonic frequencies [nw_b, NQ, NQ] when ``return_w``, the array ``aux_to_PW_00`` reads at :267-270)."""
     NQ = P0.shape[-1]
     Pw = np.tensordot(T_bt, P0, axes=(1, 0))                 # tau_f_to_w_b
     I = np.eye(NQ, dtype=np.complex128)
@@ -140,7 +141,8 @@ def eval_P_tilde(P0, T_bt, T_tb):
         temp = np.linalg.solve(temp, I)
         temp = 0.5 * (temp + np.conj(temp.T))
         Pw[n] = temp @ Pw[n]
-    return np.tensordot(T_tb, Pw, axes=(1, 0))               # w_b_to_tau_f
+    Pt = np.tensordot(T_tb, Pw, axes=(1, 0))                 # w_b_to_tau_f
+    return (Pt, Pw) if return_w else Pt
 
 
 # ---------------------------------------------------------------- Sigma
@@ -179,12 +181,13 @@ def selfenergy_contraction(Gk, vm, P, nao, NQ):
 
 # ---------------------------------------------------------------- driver
 def gw_sigma(G, Vp, T_bt, T_tb, nk, nao, nso, ns, NQ, X2C=None, mesh=None,
-             P_sp=False, Sigma_sp=False, return_P=False):
+             P_sp=False, Sigma_sp=False, return_P=False, return_Pw=False):
     """Sigma(tau, s, k) from G(tau, s, k) -- ``gw_cpu_kernel::solve``.
 
     G: [nts, ns, nk, nso, nso] complex128; Vp: [nk(nk+1)/2, NQ, nao, nao] complex128;
     T_bt: [nw_b, nts]; T_tb: [nts, nw_b].  Returns Sigma with G's shape (and the
-    P-tilde(tau, q) stack [nq, nts, NQ, NQ] when ``return_P``).
+    P-tilde(tau, q) stack [nq, nts, NQ, NQ] when ``return_P``, or the frequency-domain stack
+    [nq, nw_b, NQ, NQ] when ``return_Pw``).
     """
     G = np.asarray(G, dtype=np.complex128)
     nts = G.shape[0]
@@ -199,12 +202,14 @@ def gw_sigma(G, Vp, T_bt, T_tb, nk, nao, nso, ns, NQ, X2C=None, mesh=None,
         P0 = np.zeros((nts, NQ, NQ), dtype=np.complex128)
         eval_P0_tilde(P0, G, Vp, q, nk, nao, nso, ns, NQ, X2C, mesh, P_sp)
         symmetrize_P0(P0, nk)
-        Pt = eval_P_tilde(P0, T_bt, T_tb)
+        Pt, Pw = eval_P_tilde(P0, T_bt, T_tb, return_w=True)
         if return_P:
             Ps.append(Pt)
+        elif return_Pw:
+            Ps.append(Pw)
         eval_selfenergy(Sigma, G, Vp, Pt, q, nk, nao, nso, ns, NQ, X2C, mesh, Sigma_sp)
     Sigma /= nk
-    if return_P:
+    if return_P or return_Pw:
         return Sigma, np.stack(Ps)
     return Sigma
 

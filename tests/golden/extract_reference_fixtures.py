@@ -31,6 +31,12 @@ Sources (reference paths relative to its ``test/data``):
   -> ``h5/gw_sym27_df/``; ``Input/df_int*/meta.h5`` (no version attribute, 0.3.0, 1.0.0, 1.1.0;
   ``solvers_test.cpp:470-495``) -> ``h5/versions/<dir>/meta.h5``; and the ``params``,
   ``symmetry/k`` and ``symmetry/pairs`` groups of ``GW/input.h5`` -> ``h5/sym27_input.h5``.
+* ``GW/input_tr_symm.h5`` + ``GW/df_hf_int/{VQ_0,AqQ}.h5`` + ``GW/data_c.h5`` -- the ``GW_C``
+  section (``solvers_test.cpp:390``, q0_treatment EXTRAPOLATE): the same H2 system with 14
+  irreducible k (time reversal on 13 of the 27 points), the q -> 0 data (AqQ [inq][NQ], q_abs,
+  madelung) and G_tau / result/Sigma_tau -> ``gwc_sym27.npz``; the integral directory byte for byte
+  -> ``h5/gw_c_df/``; ``params``, ``symmetry/k``, ``symmetry/pairs`` and the small ``symmetry/q``
+  tables -> ``h5/sym27_tr_input.h5``.
 * ``HF_orth_invariance/none/data.h5:/iter1/G_tau/mesh`` -- the 110-node IR tau mesh
   (Lambda = 1e4, beta = 100) that the ``ir/1e4.h5`` grid used by all these fixtures has.
 
@@ -117,6 +123,40 @@ def main(data_dir):
         dst.create_group("symmetry")
         for g in ("k", "pairs"):
             src.copy(src["symmetry/" + g], dst["symmetry"], name=g)
+    tr_in = os.path.join(data_dir, "GW/input_tr_symm.h5")
+    c = {}
+    with h5py.File(tr_in, "r") as f:
+        c["ink"] = int(f["symmetry/k/ink"][()])
+        c["inq"] = int(f["symmetry/q/inq"][()])
+        for k in ("bz2ibz", "ibz2bz", "k_sym_transform_ao", "tr_conj", "mesh_scaled"):
+            c["k_" + k] = f["symmetry/k/" + k][()]
+        for k in ("bz2ibz", "ibz2bz", "tr_conj"):
+            c["q_" + k] = f["symmetry/q/" + k][()]
+        for k in ("conj_pairs_list", "trans_pairs_list", "kpair_irre_list"):
+            c["pairs_" + k] = f["symmetry/pairs/" + k][()]
+        c["S_k"] = np.ascontiguousarray(f["HF/S-k"][()]).view(np.complex128)[..., 0]
+    with h5py.File(os.path.join(data_dir, "GW/df_hf_int/AqQ.h5"), "r") as f:
+        c["AqQ"] = f["AqQ"][()]
+        c["q_abs"] = f["q_abs"][()]
+        c["q0_madelung"] = float(f["madelung"][()])
+    with h5py.File(os.path.join(data_dir, "GW/data_c.h5"), "r") as f:
+        c["G_tau"] = f["G_tau"][()]
+        c["Sigma_tau"] = f["result/Sigma_tau"][()]
+    np.savez(os.path.join(OUT, "gwc_sym27.npz"), V=_v_chunk(os.path.join(data_dir, "GW/df_hf_int/VQ_0.h5")), tau=mesh,
+             beta=100.0, lam=1e4, **c)
+    os.makedirs(os.path.join(h5dir, "gw_c_df"), exist_ok=True)
+    for name in ("meta.h5", "VQ_0.h5", "AqQ.h5"):
+        shutil.copyfile(os.path.join(data_dir, "GW/df_hf_int", name), os.path.join(h5dir, "gw_c_df", name))
+    with h5py.File(tr_in, "r") as src, h5py.File(os.path.join(h5dir, "sym27_tr_input.h5"), "w") as dst:
+        for a, v in src.attrs.items():
+            dst.attrs[a] = v
+        src.copy(src["params"], dst, name="params")
+        dst.create_group("symmetry")
+        for g in ("k", "pairs"):
+            src.copy(src["symmetry/" + g], dst["symmetry"], name=g)
+        q = dst["symmetry"].create_group("q")
+        for k in ("bz2ibz", "ibz2bz", "inq", "tr_conj", "mesh", "mesh_scaled", "nq", "weight_ibz"):
+            q[k] = src["symmetry/q/" + k][()]
     print("wrote", os.listdir(OUT))
 
 

@@ -125,3 +125,73 @@ def test_identity_wedge_is_the_full_mesh():
     finally:
         k.close()
     assert np.array_equal(S_full, S_ibz)
+
+
+def test_gw_c_extrapolate_from_files():
+    """GW_C (solvers_test.cpp:390): time-reversal wedge + q0_treatment EXTRAPOLATE, from the
+    reference's files (integrals, symmetry tables, AqQ.h5)."""
+    from oracle import q0
+    c = np.load(os.path.join(GOLDEN, "gwc_sym27.npz"), allow_pickle=False)
+    T = transformer(float(c["lam"]), float(c["beta"]), c["tau"])
+    S_ibz = osy.full_to_ibz(c["S_k"], c["k_ibz2bz"])
+    k = gl.GwHipKernel(27, 2, 2, 2, 36, T.nts, T.nw_b, float(c["beta"]), mesh=(3, 3, 3))
+    try:
+        k.set_grid(T.T_bt, T.T_tb)
+        df, inp = os.path.join(H5, "gw_c_df"), os.path.join(H5, "sym27_tr_input.h5")
+        k.load_files(df, inp)
+        assert k.load_q0(df, inp, S_ibz)
+        S = k.solve_ibz(c["G_tau"])
+        eps_t = k.q0_eps()
+    finally:
+        k.close()
+    err_ref = np.abs(S - c["Sigma_tau"]).max()
+    Vp = oi.packed_from_irreducible(c["V"], 27, c["pairs_conj_pairs_list"], c["pairs_trans_pairs_list"],
+                                    c["pairs_kpair_irre_list"])
+    G = osy.unfold(c["G_tau"], c["k_bz2ibz"], c["k_ibz2bz"], c["k_k_sym_transform_ao"], c["k_tr_conj"])
+    S_full, Pw = go.gw_sigma(G, Vp, T.T_bt, T.T_tb, 27, 2, 2, 2, 36, mesh=(3, 3, 3), return_Pw=True)
+    et = q0.eps_q0_tau(q0.eps_inv_wq(Pw, c["AqQ"], c["q_ibz2bz"]), c["q_abs"], T.T_tb)
+    S_or = q0.apply_q0_correction(osy.full_to_ibz(S_full, c["k_ibz2bz"]), c["G_tau"], S_ibz, et,
+                                  float(c["q0_madelung"]))
+    print(f"GW_C: {err_ref:.2e} abs vs reference, {rel_err(S, S_or):.2e} rel vs oracle, "
+          f"eps {rel_err(eps_t, et):.2e}", flush=True)
+    assert err_ref < 1e-6 and err_ref < 1e-9
+    assert rel_err(S, S_or) < 1e-10 and rel_err(eps_t, et) < 1e-10
+
+
+def _q0_synthetic(nso_factor, seed):
+    """nk = 8 (2x2x2) system with made-up q -> 0 data: |q| from a cubic cell, random AqQ and S."""
+    from oracle import q0
+    nao, NQ = 4, 10
+    nso = nso_factor * nao
+    G, V, T, d = synthetic(8, nao, NQ, ns=1, seed=seed, nso=nso)
+    rng = np.random.default_rng(seed)
+    frac = np.array(np.unravel_index(np.arange(8), (2, 2, 2))).T / 2.0
+    q_abs = np.linalg.norm(frac, axis=1) * 1.1      # three distinct |q| in (0, 1]
+    A = (rng.standard_normal((8, NQ)) + 1j * rng.standard_normal((8, NQ))) / np.sqrt(NQ)
+    x = rng.standard_normal((1, 8, nso, nso)) + 1j * rng.standard_normal((1, 8, nso, nso))
+    S = np.eye(nso) + 0.05 * (x + np.conj(np.swapaxes(x, -1, -2)))
+    mad = 0.21
+    k = gl.GwHipKernel(8, nao, nso, 1, NQ, T.nts, T.nw_b, d["beta"], mesh=d["mesh"])
+    try:
+        k.set_grid(T.T_bt, T.T_tb)
+        k.upload_V(V)
+        k.set_q0_extrapolation(np.arange(8), A, q_abs, mad, S)
+        Sg = k.solve(G)
+        k.set_q0_extrapolation(np.zeros(0, np.int64), A[:0], q_abs[:0], mad, S)   # off again
+        Sg_off = k.solve(G)
+    finally:
+        k.close()
+    S_or, Pw = go.gw_sigma(G, V, T.T_bt, T.T_tb, 8, nao, nso, 1, NQ, mesh=d["mesh"], return_Pw=True)
+    et = q0.eps_q0_tau(q0.eps_inv_wq(Pw, A, np.arange(8)), q_abs, T.T_tb)
+    S_cor = q0.apply_q0_correction(S_or.copy(), G, S, et, mad, X2C=nso != nao)
+    return Sg, Sg_off, S_cor, S_or
+
+
+@pytest.mark.parametrize("nso_factor", [1, 2])
+def test_q0_full_mesh_matches_oracle(nso_factor):
+    Sg, Sg_off, S_cor, S_or = _q0_synthetic(nso_factor, 20 + nso_factor)
+    print(f"q0 full mesh (nso = {nso_factor} nao): {rel_err(Sg, S_cor):.2e}, off: {rel_err(Sg_off, S_or):.2e}",
+          flush=True)
+    assert rel_err(Sg, S_cor) < 1e-10
+    assert rel_err(Sg_off, S_or) < 1e-10
+    assert np.abs(S_cor - S_or).max() > 1e-6     # the correction is not negligible here

@@ -8,9 +8,12 @@ Oracle pins (oracle/integrals.py, oracle/symmetry.py):
   oracle, against result/Sigma_tau at the reference's 1e-6;
 * planted mistakes (in the oracle, on the CPU): conj and trans pair operations swapped, the
   transform applied as U^H X U -- each rejected by the HF fixture.
-* the time-reversal branch of value_AO (tr_conj = 1) appears in no fixture with a non-identity
-  U; only the closed form (G(-k) = conj G(k) in a real AO basis, U = I) pins it: parity unpinned
-  for tr_conj = 1 with U != I.
+* the reference's GW_C fixture (solvers_test.cpp:390): a time-reversal wedge (14 of 27 k points,
+  tr_conj = 1 on 13) with q0_treatment EXTRAPOLATE (oracle/q0.py) against result/Sigma_tau;
+  planted mistakes (correction dropped, real and imaginary parts of eps swapped) rejected; the
+  q -> 0 fit against an exact quadratic.
+* the time-reversal branch of value_AO is pinned by GW_C only where U = I (conjugation alone);
+  with tr_conj = 1 and U != I the order of U and conjugation is parity unpinned.
 
 Host reader (libgwhip_h5.so, no GPU needed):
 * the version rule against the reference's own pass/fail strings (solvers_test.cpp:497-508) and
@@ -167,3 +170,50 @@ def test_reader_errors():
     with pytest.raises(gl.GwhipError) as e:
         gl.h5_read_packed(DF, INPUT, 26, 2, 36)       # pair lists of another mesh
     assert e.value.status == gl.ERR_IO
+
+
+# ------------------------------------------------------------------ q -> 0 extrapolation (oracle/q0.py)
+
+def _gwc_oracle(plant=None):
+    from oracle import q0
+    c = np.load(os.path.join(GOLDEN, "gwc_sym27.npz"), allow_pickle=False)
+    Vp = oi.packed_from_irreducible(c["V"], 27, c["pairs_conj_pairs_list"], c["pairs_trans_pairs_list"],
+                                    c["pairs_kpair_irre_list"])
+    G = osy.unfold(c["G_tau"], c["k_bz2ibz"], c["k_ibz2bz"], c["k_k_sym_transform_ao"], c["k_tr_conj"])
+    T = transformer(float(c["lam"]), float(c["beta"]), c["tau"])
+    S, Pw = go.gw_sigma(G, Vp, T.T_bt, T.T_tb, 27, 2, 2, 2, 36, mesh=(3, 3, 3), return_Pw=True)
+    Si = osy.full_to_ibz(S, c["k_ibz2bz"])
+    eps = q0.eps_inv_wq(Pw, c["AqQ"], c["q_ibz2bz"])
+    if plant == "imag_part":
+        eps = eps.imag + 1j * eps.real
+    et = q0.eps_q0_tau(eps, c["q_abs"], T.T_tb)
+    if plant == "no_correction":
+        return Si, c["Sigma_tau"]
+    S_ibz = osy.full_to_ibz(c["S_k"], c["k_ibz2bz"])
+    return q0.apply_q0_correction(Si, c["G_tau"], S_ibz, et, float(c["q0_madelung"])), c["Sigma_tau"]
+
+
+def test_gw_c_fixture_with_extrapolation():
+    """GW_C section (solvers_test.cpp:390): time-reversal wedge (14 of 27 k) and EXTRAPOLATE."""
+    S, ref = _gwc_oracle()
+    err = np.abs(S - ref).max()
+    assert err < 1e-6      # reference tolerance
+    assert err < 1e-10     # reached: 2.8e-11
+
+
+@pytest.mark.parametrize("plant", ["no_correction", "imag_part"])
+def test_gw_c_planted_mistakes(plant):
+    S, ref = _gwc_oracle(plant)
+    assert np.abs(S - ref).max() > 1e-6
+
+
+def test_q0_fit_closed_form():
+    """The extrapolation of an exact quadratic in |q| returns its constant term; duplicates and
+    |q| = 0 or > 1 are ignored."""
+    from oracle import q0
+    qa = np.array([0.0, 0.3, 0.3, 0.5, 0.7, 1.2])
+    y = 0.9 - 0.4 * qa + 0.25 * qa ** 2
+    y[2] = 99.0          # a repeated |q| value: only its first occurrence is used
+    y[5] = -50.0         # beyond q_max
+    assert q0.filter_q_abs(qa) == [1, 3, 4]
+    assert abs(q0.extrapolate_q0(y.astype(complex), qa) - 0.9) < 1e-12
