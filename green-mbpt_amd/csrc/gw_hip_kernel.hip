@@ -919,6 +919,7 @@ void gw_hip_kernel::run_stages_host(const double2* G, int last_stage) {
 #include "gw_hf_host.hpp"
 #include "gw_symmetry.hpp"
 #include "gw_q0.hpp"
+#include "gw_dyson.hpp"
 #include "gw_batch.hpp"
 
 namespace gwhip {
@@ -1174,6 +1175,11 @@ struct gwhip_handle {
   std::string err;
 };
 
+struct gwhip_dyson {
+  std::unique_ptr<gwhip::gw_dyson> d;
+  std::string err;
+};
+
 struct gwhip_batch {
   std::unique_ptr<gwhip::gw_hip_batch> b;
   std::vector<std::unique_ptr<gwhip_handle>> systems;
@@ -1207,6 +1213,14 @@ int guarded(gwhip_handle* h, F&& f) {
     if (h) h->err = e.what();
     return GWHIP_ERR_DEVICE;
   }
+}
+
+template <typename F>
+int guarded_d(gwhip_dyson* d, F&& f) {
+  gwhip_handle tmp;
+  const int st = guarded(&tmp, f);
+  if (st != GWHIP_OK && d) d->err = tmp.err;
+  return st;
 }
 
 template <typename F>
@@ -1426,6 +1440,74 @@ int gwhip_get_q0_eps(const gwhip_handle* h, double* eps_tau) {
   if (e.empty()) return GWHIP_ERR_STATE;
   std::memcpy(eps_tau, e.data(), e.size() * sizeof(double2));
   return GWHIP_OK;
+}
+
+int gwhip_dyson_create(const gwhip_dyson_dims* dims, int device, gwhip_dyson** out) {
+  if (!dims || !out) {
+    g_create_error = "null argument";
+    return GWHIP_ERR_ARG;
+  }
+  *out = nullptr;
+  auto d = std::make_unique<gwhip_dyson>();
+  gwhip::DysonDims dd;
+  dd.nts = dims->nts; dd.nw = dims->nw; dd.ns = dims->ns; dd.nk = dims->nk; dd.ink = dims->ink;
+  dd.nao = dims->nao; dd.nso = dims->nso;
+  const int st = guarded_d(d.get(), [&] { d->d = std::make_unique<gwhip::gw_dyson>(dd, device); });
+  if (st != GWHIP_OK) {
+    g_create_error = d->err;
+    return st;
+  }
+  *out = d.release();
+  return GWHIP_OK;
+}
+
+void gwhip_dyson_destroy(gwhip_dyson* d) { delete d; }
+
+const char* gwhip_dyson_last_error(const gwhip_dyson* d) { return d ? d->err.c_str() : g_create_error.c_str(); }
+
+int gwhip_dyson_set_grid(gwhip_dyson* d, const double* T_ft, const double* T_tf, const double* wn) {
+  if (!d) return GWHIP_ERR_ARG;
+  return guarded_d(d, [&] { d->d->set_grid((const double2*)T_ft, (const double2*)T_tf, wn); });
+}
+
+int gwhip_dyson_set_system(gwhip_dyson* d, const double* H, const double* S, const double* weights) {
+  if (!d) return GWHIP_ERR_ARG;
+  return guarded_d(d, [&] { d->d->set_system((const double2*)H, (const double2*)S, weights); });
+}
+
+int gwhip_dyson_compute_G(gwhip_dyson* d, double* Sigma1, double* Sigma_tau, double mu, double* G_tau) {
+  if (!d || !Sigma1 || !Sigma_tau || !G_tau) return GWHIP_ERR_ARG;
+  return guarded_d(d, [&] { d->d->compute_G_host((double2*)Sigma1, (double2*)Sigma_tau, mu, (double2*)G_tau); });
+}
+
+int gwhip_dyson_compute_G_device(gwhip_dyson* d, void* Sigma1, void* Sigma_tau, double mu, void* G_tau) {
+  if (!d) return GWHIP_ERR_ARG;
+  return guarded_d(d, [&] { d->d->compute_G_device((double2*)Sigma1, (double2*)Sigma_tau, mu, (double2*)G_tau); });
+}
+
+int gwhip_dyson_find_mu(gwhip_dyson* d, double* Sigma1, double* Sigma_tau, double mu0, double nel, double tol,
+                        double* mu, double* nel_found) {
+  if (!d || !mu || !nel_found) return GWHIP_ERR_ARG;
+  return guarded_d(d, [&] {
+    const auto r = d->d->find_mu_host((double2*)Sigma1, (double2*)Sigma_tau, mu0, nel, tol);
+    *nel_found = r.first;
+    *mu = r.second;
+  });
+}
+
+int gwhip_dyson_find_mu_device(gwhip_dyson* d, void* Sigma1, void* Sigma_tau, double mu0, double nel, double tol,
+                               double* mu, double* nel_found) {
+  if (!d || !mu || !nel_found) return GWHIP_ERR_ARG;
+  return guarded_d(d, [&] {
+    const auto r = d->d->find_mu_device((double2*)Sigma1, (double2*)Sigma_tau, mu0, nel, tol);
+    *nel_found = r.first;
+    *mu = r.second;
+  });
+}
+
+int gwhip_dyson_number_of_electrons(gwhip_dyson* d, double* Sigma1, double* Sigma_tau, double mu, double* nel) {
+  if (!d || !nel) return GWHIP_ERR_ARG;
+  return guarded_d(d, [&] { *nel = d->d->number_of_electrons_host((double2*)Sigma1, (double2*)Sigma_tau, mu); });
 }
 
 int gwhip_get_dims(const gwhip_handle* h, gwhip_dims* out) {

@@ -21,7 +21,7 @@ H5_HEADER = os.path.join(os.path.dirname(HEADER), "gwhip_h5.h")
 HDF5_ROOT = os.environ.get("HDF5_ROOT", "/opt/conda")
 
 (OK, ERR_ARG, ERR_WRONG_GRID, ERR_STATE, ERR_OOM, ERR_DEVICE, ERR_NONFINITE, ERR_UNSUPPORTED, ERR_OUTDATED_INPUT,
- ERR_IO) = range(10)
+ ERR_IO, ERR_MU_SEARCH) = range(11)
 PREC_FP64, PREC_SPLIT = 0, 1
 VJP_RECOMPUTE, VJP_STORED = 0, 1
 TUNING_KEYS = ("p0_tile", "sigma_tile", "aux_per_wave", "p0_group", "sigma_group", "p0_split_tile", "sigma_split_tile",
@@ -44,11 +44,17 @@ class OutdatedInput(GwhipError):
     """Mirror of ``mbpt_outdated_input`` (integral files older than format 1.0.0)."""
 
 
+class ChemicalPotentialSearchFailure(GwhipError):
+    """Mirror of ``mbpt_chemical_potential_search_failure``."""
+
+
 def _raise(st: int, msg: str):
     if st == ERR_WRONG_GRID:
         raise WrongGrid(st, msg)
     if st == ERR_OUTDATED_INPUT:
         raise OutdatedInput(st, msg)
+    if st == ERR_MU_SEARCH:
+        raise ChemicalPotentialSearchFailure(st, msg)
     raise GwhipError(st, msg)
 
 
@@ -56,6 +62,11 @@ class Dims(ctypes.Structure):
     _fields_ = [("nk", ctypes.c_int64), ("mesh", ctypes.c_int64 * 3), ("nao", ctypes.c_int64),
                 ("nso", ctypes.c_int64), ("ns", ctypes.c_int64), ("NQ", ctypes.c_int64),
                 ("nts", ctypes.c_int64), ("nw_b", ctypes.c_int64), ("beta", ctypes.c_double)]
+
+
+class DysonDims(ctypes.Structure):
+    _fields_ = [("nts", ctypes.c_int64), ("nw", ctypes.c_int64), ("ns", ctypes.c_int64), ("nk", ctypes.c_int64),
+                ("ink", ctypes.c_int64), ("nao", ctypes.c_int64), ("nso", ctypes.c_int64)]
 
 
 _lib = None
@@ -129,6 +140,19 @@ def load() -> ctypes.CDLL:
     lib.gwhip_get_dims.argtypes = [P, ctypes.POINTER(Dims)]
     lib.gwhip_set_q0_extrapolation.argtypes = [P, ctypes.c_int64, P, P, P, ctypes.c_double, ctypes.c_int64, P]
     lib.gwhip_get_q0_eps.argtypes = [P, P]
+    lib.gwhip_dyson_create.argtypes = [ctypes.POINTER(DysonDims), ctypes.c_int, ctypes.POINTER(P)]
+    lib.gwhip_dyson_destroy.argtypes = [P]
+    lib.gwhip_dyson_destroy.restype = None
+    lib.gwhip_dyson_last_error.argtypes = [P]
+    lib.gwhip_dyson_last_error.restype = ctypes.c_char_p
+    lib.gwhip_dyson_set_grid.argtypes = [P, P, P, P]
+    lib.gwhip_dyson_set_system.argtypes = [P, P, P, P]
+    lib.gwhip_dyson_compute_G.argtypes = [P, P, P, ctypes.c_double, P]
+    lib.gwhip_dyson_compute_G_device.argtypes = [P, P, P, ctypes.c_double, P]
+    DP = ctypes.POINTER(ctypes.c_double)
+    lib.gwhip_dyson_find_mu.argtypes = [P, P, P, ctypes.c_double, ctypes.c_double, ctypes.c_double, DP, DP]
+    lib.gwhip_dyson_find_mu_device.argtypes = [P, P, P, ctypes.c_double, ctypes.c_double, ctypes.c_double, DP, DP]
+    lib.gwhip_dyson_number_of_electrons.argtypes = [P, P, P, ctypes.c_double, DP]
     for f in ("gwhip_debug_zgemm_split", "gwhip_set_split_slices", "gwhip_set_kq_maps", "gwhip_debug_mfma_peak",
               "gwhip_debug_gemm_bench", "gwhip_debug_zgemm", "gwhip_debug_stage", "gwhip_set_grid", "gwhip_upload_V",
               "gwhip_generate_V", "gwhip_eval_sigma", "gwhip_eval_sigma_device", "gwhip_get_P_tilde", "gwhip_timings",
@@ -136,7 +160,9 @@ def load() -> ctypes.CDLL:
               "gwhip_get_tuning", "gwhip_set_tuning", "gwhip_batch_create", "gwhip_batch_autotune",
               "gwhip_batch_eval_device", "gwhip_batch_stats", "gwhip_set_k_symmetry", "gwhip_eval_sigma_ibz",
               "gwhip_eval_sigma_ibz_device", "gwhip_hf_ibz", "gwhip_hf_ibz_device", "gwhip_get_dims",
-              "gwhip_set_q0_extrapolation", "gwhip_get_q0_eps"):
+              "gwhip_set_q0_extrapolation", "gwhip_get_q0_eps", "gwhip_dyson_create", "gwhip_dyson_set_grid",
+              "gwhip_dyson_set_system", "gwhip_dyson_compute_G", "gwhip_dyson_compute_G_device", "gwhip_dyson_find_mu",
+              "gwhip_dyson_find_mu_device", "gwhip_dyson_number_of_electrons"):
         getattr(lib, f).restype = ctypes.c_int
     _lib = lib
     return lib
@@ -569,3 +595,77 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+class Dyson:
+    """Device Dyson step and chemical-potential search (``dyson<G, S1, St>``, ``src/dyson.cpp:19-236``).
+
+    Arrays on the irreducible k points: H, S, Sigma1 [ns, ink, nso, nso]; Sigma(tau), G(tau)
+    [nts, ns, ink, nso, nso].  As in the reference, Sigma1 and Sigma(tau) are made Hermitian in
+    place (the host arrays passed in are overwritten with their Hermitian parts).
+    """
+
+    def __init__(self, nts, nw, ns, nk, ink, nao, nso, device=0):
+        self.lib = load()
+        self.dims = dict(nts=nts, nw=nw, ns=ns, nk=nk, ink=ink, nao=nao, nso=nso)
+        d = DysonDims(nts, nw, ns, nk, ink, nao, nso)
+        h = ctypes.c_void_p()
+        st = self.lib.gwhip_dyson_create(ctypes.byref(d), device, ctypes.byref(h))
+        if st != OK:
+            _raise(st, self.lib.gwhip_dyson_last_error(None).decode())
+        self.h = h
+
+    def _check(self, st):
+        if st != OK:
+            _raise(st, self.lib.gwhip_dyson_last_error(self.h).decode())
+
+    def close(self):
+        if getattr(self, "h", None):
+            self.lib.gwhip_dyson_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_grid(self, T_ft, T_tf, wn):
+        self._T = (np.ascontiguousarray(T_ft, np.complex128), np.ascontiguousarray(T_tf, np.complex128),
+                   np.ascontiguousarray(wn, np.float64))
+        self._check(self.lib.gwhip_dyson_set_grid(self.h, *[_ptr(a) for a in self._T]))
+
+    def set_system(self, H, S, weights=None):
+        h = np.ascontiguousarray(H, np.complex128)
+        s = np.ascontiguousarray(S, np.complex128)
+        w = None if weights is None else np.ascontiguousarray(weights, np.float64)
+        self._check(self.lib.gwhip_dyson_set_system(self.h, _ptr(h), _ptr(s), None if w is None else _ptr(w)))
+
+    def _sig(self, Sigma1, Sigma_tau):
+        s1 = np.ascontiguousarray(Sigma1, np.complex128).copy()
+        st = np.ascontiguousarray(Sigma_tau, np.complex128).copy()
+        return s1, st
+
+    def compute_G(self, Sigma1, Sigma_tau, mu: float) -> np.ndarray:
+        s1, st = self._sig(Sigma1, Sigma_tau)
+        G = np.empty_like(st)
+        self._check(self.lib.gwhip_dyson_compute_G(self.h, _ptr(s1), _ptr(st), float(mu), _ptr(G)))
+        return G
+
+    def compute_G_device(self, s1_ptr: int, st_ptr: int, mu: float, G_ptr: int):
+        self._check(self.lib.gwhip_dyson_compute_G_device(self.h, ctypes.c_void_p(s1_ptr), ctypes.c_void_p(st_ptr),
+                                                          float(mu), ctypes.c_void_p(G_ptr)))
+
+    def find_mu(self, Sigma1, Sigma_tau, mu0: float, nel: float, tol: float):
+        """(electron count found, mu)."""
+        s1, st = self._sig(Sigma1, Sigma_tau)
+        mu, nf = ctypes.c_double(), ctypes.c_double()
+        self._check(self.lib.gwhip_dyson_find_mu(self.h, _ptr(s1), _ptr(st), float(mu0), float(nel), float(tol),
+                                                 ctypes.byref(mu), ctypes.byref(nf)))
+        return nf.value, mu.value
+
+    def number_of_electrons(self, Sigma1, Sigma_tau, mu: float) -> float:
+        s1, st = self._sig(Sigma1, Sigma_tau)
+        n = ctypes.c_double()
+        self._check(self.lib.gwhip_dyson_number_of_electrons(self.h, _ptr(s1), _ptr(st), float(mu), ctypes.byref(n)))
+        return n.value

@@ -38,7 +38,9 @@ enum gwhip_status {
   GWHIP_ERR_UNSUPPORTED = 7, /* request outside what this build implements               */
   GWHIP_ERR_OUTDATED_INPUT = 8, /* integral files older than format 1.0.0: mbpt_outdated_input
                                    (src/green/mbpt/df_integral_t.h:42-51, except.h)          */
-  GWHIP_ERR_IO = 9           /* a file or dataset is missing or has the wrong shape        */
+  GWHIP_ERR_IO = 9,          /* a file or dataset is missing or has the wrong shape        */
+  GWHIP_ERR_MU_SEARCH = 10   /* chemical-potential bisection stalled:
+                                mbpt_chemical_potential_search_failure (src/dyson.cpp:148-150) */
 };
 
 /* Arithmetic of the P0 and Sigma contractions (the Dyson step, transforms and
@@ -236,6 +238,51 @@ int gwhip_set_q0_extrapolation(gwhip_handle* h, int64_t inq, const int64_t* q_ir
 
 /* eps_q0 on the fermionic tau grid of the last evaluation with the correction on: [nts] complex. */
 int gwhip_get_q0_eps(const gwhip_handle* h, double* eps_tau);
+
+/* ---- Dyson step and chemical-potential search (dyson<G, S1, St>, src/dyson.cpp:19-236) ----
+ * A separate handle: it needs the fermionic tau <-> i w_n transforms, not the bosonic ones.
+ * Arrays on the irreducible k points as the reference keeps them: H, S, Sigma1 [ns][ink][nso][nso],
+ * Sigma(tau), G(tau) [nts][ns][ink][nso][nso], complex. */
+typedef struct gwhip_dyson gwhip_dyson;
+
+typedef struct gwhip_dyson_dims {
+  int64_t nts;   /* fermionic tau nodes                   */
+  int64_t nw;    /* fermionic Matsubara frequencies       */
+  int64_t ns;    /* spins (1 for X2C)                     */
+  int64_t nk;    /* k points of the full mesh             */
+  int64_t ink;   /* irreducible k points (nk without symmetry) */
+  int64_t nao;   /* atomic orbitals                       */
+  int64_t nso;   /* nao, or 2 nao for X2C (at most 96)    */
+} gwhip_dyson_dims;
+
+/* Replaces: dyson<G, S1, St>::dyson (src/dyson.cpp:19-46) minus the file reading. */
+int gwhip_dyson_create(const gwhip_dyson_dims* dims, int device, gwhip_dyson** out);
+void gwhip_dyson_destroy(gwhip_dyson* d);
+const char* gwhip_dyson_last_error(const gwhip_dyson* d);
+
+/* green-grids' fermionic tau_to_omega / omega_to_tau as matrices: T_ft [nw][nts], T_tf [nts][nw]
+ * complex, and the Matsubara frequencies w_n [nw] (real). */
+int gwhip_dyson_set_grid(gwhip_dyson* d, const double* T_ft, const double* T_tf, const double* wn);
+
+/* H and S (made Hermitian on the device, src/dyson.cpp:42-43) and the number of full-mesh points
+ * each irreducible point stands for ([ink]; null: 1 each), used by the electron count. */
+int gwhip_dyson_set_system(gwhip_dyson* d, const double* H, const double* S, const double* weights);
+
+/* compute_G (src/dyson.cpp:174-236): G(tau) at mu.  Sigma1 and Sigma(tau) are made Hermitian in
+ * place, as the reference does.  Host buffers; the _device variant takes device pointers. */
+int gwhip_dyson_compute_G(gwhip_dyson* d, double* Sigma1, double* Sigma_tau, double mu, double* G_tau);
+int gwhip_dyson_compute_G_device(gwhip_dyson* d, void* Sigma1_dev, void* Sigma_tau_dev, double mu, void* G_tau_dev);
+
+/* find_mu (src/dyson.cpp:100-172) from mu0 to electron count nel within relative tolerance tol:
+ * writes the chemical potential and the count found.  GWHIP_ERR_MU_SEARCH when the bisection stalls. */
+int gwhip_dyson_find_mu(gwhip_dyson* d, double* Sigma1, double* Sigma_tau, double mu0, double nel, double tol,
+                        double* mu, double* nel_found);
+int gwhip_dyson_find_mu_device(gwhip_dyson* d, void* Sigma1_dev, void* Sigma_tau_dev, double mu0, double nel,
+                               double tol, double* mu, double* nel_found);
+
+/* compute_number_of_electrons (src/dyson.cpp:70-98) at mu, evaluated as
+ * -pref Re sum_n T_tf[nts-1][n] sum_{s,k} (w_k / nk) tr(G(i w_n; mu) S). */
+int gwhip_dyson_number_of_electrons(gwhip_dyson* d, double* Sigma1, double* Sigma_tau, double mu, double* nel);
 
 /* The dimensions the handle was created with. */
 int gwhip_get_dims(const gwhip_handle* h, gwhip_dims* out);

@@ -37,6 +37,9 @@ Sources (reference paths relative to its ``test/data``):
   madelung) and G_tau / result/Sigma_tau -> ``gwc_sym27.npz``; the integral directory byte for byte
   -> ``h5/gw_c_df/``; ``params``, ``symmetry/k``, ``symmetry/pairs`` and the small ``symmetry/q``
   tables -> ``h5/sym27_tr_input.h5``.
+* ``HF_orth_invariance/{none,lowdin,mo,natural}/{input,data}.h5`` (``solvers_test.cpp:296-452``): the
+  stored first iteration of an SC run (nk=1, ns=2, nao=10, nel=2): HF/H-k, HF/S-k, iter1/Sigma1,
+  iter1/Selfenergy, iter1/mu and the Dyson output iter1/G_tau -> ``dyson_orth.npz``.
 * ``HF_orth_invariance/none/data.h5:/iter1/G_tau/mesh`` -- the 110-node IR tau mesh
   (Lambda = 1e4, beta = 100) that the ``ir/1e4.h5`` grid used by all these fixtures has.
 
@@ -157,6 +160,19 @@ def main(data_dir):
         q = dst["symmetry"].create_group("q")
         for k in ("bz2ibz", "ibz2bz", "inq", "tr_conj", "mesh", "mesh_scaled", "nq", "weight_ibz"):
             q[k] = src["symmetry/q/" + k][()]
+    orth = {}
+    for mode in ("none", "lowdin", "mo", "natural"):
+        base = os.path.join(data_dir, "HF_orth_invariance", mode)
+        with h5py.File(os.path.join(base, "input.h5"), "r") as f:
+            orth[mode + "_H"] = np.ascontiguousarray(f["HF/H-k"][()]).view(np.complex128)[..., 0]
+            orth[mode + "_S"] = np.ascontiguousarray(f["HF/S-k"][()]).view(np.complex128)[..., 0]
+            orth[mode + "_nel"] = int(f["params/nel_cell"][()])
+        with h5py.File(os.path.join(base, "data.h5"), "r") as f:
+            orth[mode + "_Sigma1"] = f["iter1/Sigma1"][()]
+            orth[mode + "_Sigma_tau"] = f["iter1/Selfenergy/data"][()]
+            orth[mode + "_G_tau"] = f["iter1/G_tau/data"][()]
+            orth[mode + "_mu"] = float(f["iter1/mu"][()])
+    np.savez_compressed(os.path.join(OUT, "dyson_orth.npz"), tau=mesh, beta=100.0, lam=1e4, **orth)
     print("wrote", os.listdir(OUT))
 
 
