@@ -99,6 +99,8 @@ def main():
                     help="time the Sigma[G] vector-Jacobian product on the nk=27 nao=26 naux=250 system instead")
     ap.add_argument("--batch", action="store_true",
                     help="time the 8 uneven ns=2 systems evaluated as one captured graph instead")
+    ap.add_argument("--dyson", action="store_true",
+                    help="time the Dyson G step and the chemical-potential search at nk=64 nso=26 instead")
     ap.add_argument("--hf", action="store_true",
                     help="time the Hartree-Fock Fock build on the same nk=64 nao=26 naux=300 integrals instead")
     ap.add_argument("--no-autotune", action="store_true",
@@ -119,6 +121,8 @@ def main():
         return bench_batch(args)
     if args.hf:
         return bench_hf(args)
+    if args.dyson:
+        return bench_dyson(args)
 
     import numpy as np
     import torch
@@ -379,6 +383,71 @@ def bench_hf(args):
     if not args.no_cpu:
         out["cpu_baseline"] = hf_cpu_baseline(w)
     print(json.dumps(out), flush=True)
+
+
+def bench_dyson(args):
+    """Dyson G step (Sigma(i w_n), batched inverse, G(tau)) and the chemical-potential search."""
+    import numpy as np
+    import torch
+
+    from gwhip import irgrid
+    from gwhip import lib as gl
+
+    w = WORKLOAD
+    nk, n, ns, beta = w["nk"], w["nao"], w["ns"], w["beta"]
+    T = irgrid.Transformer.from_grid(irgrid.load_grid(w["lam"]), beta)
+    wn = (2 * T.n_fermi + 1) * np.pi / beta
+    nts, nw = T.nts, len(wn)
+    rng = np.random.default_rng(3)
+    c = lambda *sh: rng.standard_normal(sh) + 1j * rng.standard_normal(sh)
+    herm = lambda x: 0.5 * (x + np.conj(np.swapaxes(x, -1, -2)))
+    U = np.linalg.qr(c(ns, nk, n, n))[0]
+    eps = rng.uniform(-1.0, 1.0, (ns, nk, n))
+    H = np.einsum("skij,skj,sklj->skil", U, eps, np.conj(U))
+    S = np.eye(n) + 0.05 * herm(c(ns, nk, n, n)) / np.sqrt(n)
+    S1 = 0.05 * herm(c(ns, nk, n, n))
+    St = 0.002 * herm(c(nts, ns, nk, n, n))
+    nel = float(2 * np.sum(eps < 0.0) / nk)      # filled to mu ~ 0
+    d = gl.Dyson(nts, nw, ns, nk, nk, n, n)
+    d.set_grid(T.T_ft, T.T_tf, wn)
+    d.set_system(H, S)
+    dev = torch.device("cuda:0")
+    s1_d = torch.from_numpy(np.ascontiguousarray(S1).view(np.float64)).to(dev)
+    st_d = torch.from_numpy(np.ascontiguousarray(St).view(np.float64)).to(dev)
+    g_d = torch.empty_like(st_d)
+    torch.cuda.synchronize()
+    for _ in range(args.warmup):
+        d.compute_G_device(s1_d.data_ptr(), st_d.data_ptr(), 0.0, g_d.data_ptr())
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        d.compute_G_device(s1_d.data_ptr(), st_d.data_ptr(), 0.0, g_d.data_ptr())
+        print(f"[bench] dyson step {i} done", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    ms_g = (time.perf_counter() - t0) / args.steps * 1e3
+    t0 = time.perf_counter()
+    nel_found, mu = d.find_mu(S1, St, 0.0, nel, 1e-9)
+    ms_mu = (time.perf_counter() - t0) * 1e3
+    d.close()
+    # CPU: the oracle's batched inverse (numpy / LAPACK) on the same shapes
+    from oracle import dyson as od
+    t0 = time.perf_counter()
+    od.compute_G(H, S, S1, St, 0.0, T.T_ft, T.T_tf, wn)
+    ms_cpu = (time.perf_counter() - t0) * 1e3
+    nmat = nw * ns * nk
+    flops = 8.0 * n ** 3 * nmat                     # Gauss-Jordan inverse: n^3 complex multiply-adds
+    print(json.dumps({
+        "metric": "Dyson G steps/s @nk64 nso26 (FP64)", "value": 1e3 / ms_g, "unit": "Dyson steps/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms_g, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (random Hermitian H with eigenvalues in [-1, 1], S near I, small Hermitian Sigma)",
+        "config": {"workload": "Dyson G(tau) at nk 64, nso 26, ns 1, beta 100, IR 1e4", "nts": nts, "nw": nw,
+                   "parallelism": "single GPU"},
+        "inverse_flops_per_step": flops, "inverse_matrices": nmat,
+        "find_mu": {"ms": ms_mu, "mu": mu, "nel": nel_found, "target": nel},
+        "cpu_baseline": {"value": 1e3 / ms_cpu, "unit": "Dyson steps/s", "cores": os.cpu_count() or 1, "kind": "port",
+                         "sample": "the whole step with the oracle (numpy batched inverse, LAPACK)"},
+    }), flush=True)
 
 
 def copy_calibration():
