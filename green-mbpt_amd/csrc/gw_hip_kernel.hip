@@ -869,14 +869,15 @@ void gw_hip_kernel::autotune(const double2* G, int precision) {
       });
     }
     // candidates (block tile, row-tile group) encoded as 100 * tile + group
-    const int cands[14] = {1, 2, 4, 8, 16, 32, 64, 101, 102, 104, 108, 116, 132, 164};
+    const int cands[21] = {1, 2, 4, 8, 16, 32, 64, 101, 102, 104, 108, 116, 132, 164,
+                           201, 202, 204, 208, 216, 232, 264};
     p0_prep(0, G, 0, tcp, tune_.qpw);
     p0_slice(tcp);
-    const int bp = best_of(14, cands, [&](int c) { p0_gemm_split(0, tcp, P_, c % 100, c / 100); });
+    const int bp = best_of(21, cands, [&](int c) { p0_gemm_split(0, tcp, P_, c % 100, c / 100); });
     tune_.p0_oz_tile = bp / 100;
     tune_.p0_group = bp % 100;
     sigma_slice(0, 0, tcs);
-    const int bs = best_of(14, cands, [&](int c) { sigma_gemm_split(0, tcs, c % 100, c / 100); });
+    const int bs = best_of(21, cands, [&](int c) { sigma_gemm_split(0, tcs, c % 100, c / 100); });
     tune_.s_oz_tile = bs / 100;
     tune_.s_group = bs % 100;
   }

@@ -40,8 +40,9 @@ struct KernelStat {
 // orbital kernel writes the int8 slices itself.
 struct Tuning {
   int p0_shape = 0, s_shape = 0, qpw = 4;
-  int p0_group = 1, s_group = 8;  // split precision: row tiles per L2 group of the int8 GEMMs
-  int p0_oz_tile = 0, s_oz_tile = 0;  // split precision: int8 GEMM block tile (0: 32x64, 1: 64x64)
+  int p0_group = 64, s_group = 8;  // split precision: row tiles per L2 group of the int8 GEMMs
+  int p0_oz_tile = 2, s_oz_tile = 2;  // split precision: int8 GEMM block tile (0: 32x64, 1: 64x64,
+                                      // 2: 32x64 software-pipelined)
   int p0_fused = 1;  // split precision, one spin block: P0 orbital kernel writes int8 slices directly
   static constexpr int count = 8;
   void to_array(int* v) const {
@@ -50,7 +51,7 @@ struct Tuning {
   }
   bool from_array(const int* v) {
     if (v[0] < 0 || v[0] > 2 || v[1] < 0 || v[1] > 2 || (v[2] != 1 && v[2] != 2 && v[2] != 4 && v[2] != 8) ||
-        v[3] < 1 || v[4] < 1 || v[5] < 0 || v[5] > 1 || v[6] < 0 || v[6] > 1 || v[7] < 0 || v[7] > 1)
+        v[3] < 1 || v[4] < 1 || v[5] < 0 || v[5] > 2 || v[6] < 0 || v[6] > 2 || v[7] < 0 || v[7] > 1)
       return false;
     p0_shape = v[0]; s_shape = v[1]; qpw = v[2]; p0_group = v[3]; s_group = v[4]; p0_oz_tile = v[5]; s_oz_tile = v[6];
     p0_fused = v[7];

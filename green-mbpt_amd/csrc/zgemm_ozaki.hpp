@@ -146,7 +146,8 @@ struct OzGemmArgs {
   int tiles_m, tiles_n;
   int64_t nblocks, per_xcd;
   int group_m;  // row tiles (over batch x M) swept together across all column tiles; 0 = heuristic
-  int tile;     // 0: 32 x 64 block tile, two blocks per CU; 1: 64 x 64 with register prefetch, one per CU
+  int tile;     // 0: 32 x 64 block tile, two blocks per CU; 1: 64 x 64 with register prefetch, one per CU;
+                // 2: 32 x 64, software-pipelined (next K step in flight, double-buffered A)
   int upper;    // 1: skip block tiles that lie wholly below the diagonal (Hermitian products)
 };
 
@@ -296,6 +297,185 @@ __global__ __launch_bounds__(256, 2) void zgemm_ozaki_kernel(OzGemmArgs p) {
         cr[x][r] += sr;
         ci[x][r] += si;
       }
+  }
+
+  const int n = n0 + wn + lrow;
+  if (n >= p.N) return;
+  const int no = n / p.nin;
+  const int64_t coff = (int64_t)no * p.sCno + (int64_t)(n - no * p.nin) * p.sCni;
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int m = m0 + 16 * x + OZ_ROW(lg, r);
+      if (m >= p.M) continue;
+      const double vr = cr[x][r], vi = ci[x][r];
+      double2 v = make_double2(p.alpha.x * vr - p.alpha.y * vi, p.alpha.x * vi + p.alpha.y * vr);
+      double2* dst = p.C + b * p.sCb + (int64_t)m * p.sCm + coff;
+      if (p.accumulate) {
+        const double2 o = *dst;
+        v.x += o.x;
+        v.y += o.y;
+      }
+      *dst = v;
+    }
+}
+
+// The 32 x 64 tile with a software pipeline: while the MFMAs of K step g run, the A pieces and B
+// fragments of step g + 1 are already in flight into registers; A goes to the other half of a
+// double-buffered LDS panel after the multiplies, so one barrier per K step suffices and no
+// wave waits on global memory between steps.  The chunk exponents are read straight from
+// global memory (L2) in the chunk epilogue.  Same products, same order: results are bit
+// identical to zgemm_ozaki_kernel.
+template <int NS>
+__global__ __launch_bounds__(256, 2) void zgemm_ozaki_pipe_kernel(OzGemmArgs p) {
+  constexpr int PA = 2, NPC = PA * NS / 2;  // plane-slices staged per thread and K step
+  constexpr int BUF = PA * NS * OZ_BM * 64;
+  extern __shared__ __attribute__((aligned(16))) int8_t oz_lds[];
+
+  const int64_t hw = blockIdx.x;
+  const int64_t logical = (hw % 8) * p.per_xcd + hw / 8;
+  if (logical >= p.nblocks) return;
+  const int64_t rows = (int64_t)p.batch * p.tiles_m;
+  const int64_t in_group = (int64_t)p.group_m * p.tiles_n;
+  const int64_t grp = logical / in_group;
+  const int64_t first = grp * p.group_m;
+  const int64_t gsize = rows - first < p.group_m ? rows - first : p.group_m;
+  const int64_t idx = logical - grp * in_group;
+  const int64_t rt = first + idx % gsize;
+  const int tn = (int)(idx / gsize);
+  const int64_t b = rt / p.tiles_m;
+  const int tm = (int)(rt - b * p.tiles_m);
+  const int m0 = tm * OZ_BM, n0 = tn * OZ_BN;
+  if (p.upper && m0 >= n0 + OZ_BN) return;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lrow = lane & 15, lg = lane >> 4;
+  const int wn = wave * 16;
+
+  const int a_row = tid >> 3, a_piece = tid & 3, a_par = (tid >> 2) & 1;
+  const int8_t* a_ptr = nullptr;
+  if (m0 + a_row < p.M) a_ptr = p.A + b * p.sAb + (int64_t)(m0 + a_row) * p.sAm + 16 * a_piece;
+  const int a_lds = oz_lds_off(0, OZ_BM, a_row, a_piece);
+  const int bcol = n0 + wn + lrow;
+  const int8_t* b_ptr = nullptr;
+  if (bcol < p.N) {
+    const int no = bcol / p.nin;
+    b_ptr = p.B + b * p.sBb + (int64_t)no * p.sBno + (int64_t)(bcol - no * p.nin) * p.sBni + 16 * lg;
+  }
+  const int nks = p.kc / OZ_KS, nsteps = p.nchunk * nks;
+  auto b_at = [&](int g) -> const int8_t* {  // this lane's B bytes of K step g
+    const int c = g / nks, ks = (g - c * nks) * OZ_KS;
+    return b_ptr + (int64_t)c * p.sBc + ks;
+  };
+
+  // A pieces of K step g into registers, later stored to one half of the LDS panel
+  v4i a_nx[NPC];
+  auto load_a = [&](int g) {
+    const int c = g / nks, ks = (g - c * nks) * OZ_KS;
+#pragma unroll
+    for (int j = 0; j < NPC; ++j) {
+      const int ps = a_par + 2 * j;
+      a_nx[j] = a_ptr ? *reinterpret_cast<const v4i*>(a_ptr + (int64_t)c * p.sAc + (int64_t)ps * p.kc + ks)
+                      : v4i{0, 0, 0, 0};
+    }
+  };
+  auto store_a = [&](int buf) {
+#pragma unroll
+    for (int j = 0; j < NPC; ++j)
+      *reinterpret_cast<v4i*>(oz_lds + buf * BUF + a_lds + (a_par + 2 * j) * OZ_BM * 64) = a_nx[j];
+  };
+
+  double cr[2][4], ci[2][4];
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) { cr[x][r] = 0.0; ci[x][r] = 0.0; }
+  v4i ar[NS][2], ai[NS][2];
+#pragma unroll
+  for (int d = 0; d < NS; ++d)
+#pragma unroll
+    for (int x = 0; x < 2; ++x) { ar[d][x] = v4i{0, 0, 0, 0}; ai[d][x] = v4i{0, 0, 0, 0}; }
+
+  // B fragments of the current step; fb[t] is last used by slice s = NS - 1 - t, after which its
+  // registers take the next step's fragment, so B for step g + 1 is in flight during step g
+  // without a second register set
+  v4i fb_r[NS], fb_i[NS];
+  load_a(0);
+  store_a(0);
+  {
+    const int8_t* bc = b_ptr ? b_at(0) : nullptr;
+#pragma unroll
+    for (int t = 0; t < NS; ++t) {
+      fb_r[t] = bc ? *reinterpret_cast<const v4i*>(bc + (int64_t)(0 * NS + t) * p.kc) : v4i{0, 0, 0, 0};
+      fb_i[t] = bc ? *reinterpret_cast<const v4i*>(bc + (int64_t)(1 * NS + t) * p.kc) : v4i{0, 0, 0, 0};
+    }
+  }
+  __syncthreads();
+
+  for (int g = 0; g < nsteps; ++g) {
+    const bool more = g + 1 < nsteps;
+    const int c = g / nks;
+    const bool chunk_end = g - c * nks == nks - 1;
+    if (more) load_a(g + 1);
+    const int8_t* bn = (more && b_ptr) ? b_at(g + 1) : nullptr;
+    const int8_t* As = oz_lds + (g & 1) * BUF;
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      v4i fa_r[2], fa_i[2], fa_n[2];
+#pragma unroll
+      for (int x = 0; x < 2; ++x) {
+        const int row = 16 * x + lrow;
+        fa_r[x] = *reinterpret_cast<const v4i*>(As + oz_lds_off(0 * NS + s, OZ_BM, row, lg));
+        fa_i[x] = *reinterpret_cast<const v4i*>(As + oz_lds_off(1 * NS + s, OZ_BM, row, lg));
+        fa_n[x] = neg_i8x16(fa_i[x]);
+      }
+#pragma unroll
+      for (int t = 0; t + s < NS; ++t) {
+#pragma unroll
+        for (int x = 0; x < 2; ++x) {
+          ar[s + t][x] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa_r[x], fb_r[t], ar[s + t][x], 0, 0, 0);
+          ai[s + t][x] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa_r[x], fb_i[t], ai[s + t][x], 0, 0, 0);
+          ar[s + t][x] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa_n[x], fb_i[t], ar[s + t][x], 0, 0, 0);
+          ai[s + t][x] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa_i[x], fb_r[t], ai[s + t][x], 0, 0, 0);
+        }
+      }
+      if (more) {
+        const int t = NS - 1 - s;
+        fb_r[t] = bn ? *reinterpret_cast<const v4i*>(bn + (int64_t)(0 * NS + t) * p.kc) : v4i{0, 0, 0, 0};
+        fb_i[t] = bn ? *reinterpret_cast<const v4i*>(bn + (int64_t)(1 * NS + t) * p.kc) : v4i{0, 0, 0, 0};
+      }
+    }
+    if (chunk_end) {  // int32 -> FP64 with the (row, column, chunk) scale
+      int eb = 0;
+      const int n = n0 + wn + lrow;
+      if (n < p.N) {
+        const int no = n / p.nin;
+        eb = p.eB[b * p.seBb + (int64_t)no * p.seBno + (int64_t)(n - no * p.nin) * p.seBni + c];
+      }
+#pragma unroll
+      for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int m = m0 + 16 * x + OZ_ROW(lg, r);
+          const int ea = m < p.M ? p.eA[b * p.seAb + (int64_t)m * p.seAm + c] : 0;
+          double sc = ldexp(1.0, ea + eb - 14);
+          double sr = 0.0, si = 0.0;
+#pragma unroll
+          for (int d = 0; d < NS; ++d) {
+            sr = fma((double)ar[d][x][r], sc, sr);
+            si = fma((double)ai[d][x][r], sc, si);
+            sc *= 0.0078125;  // 2^-7
+          }
+          cr[x][r] += sr;
+          ci[x][r] += si;
+        }
+#pragma unroll
+      for (int d = 0; d < NS; ++d)
+#pragma unroll
+        for (int x = 0; x < 2; ++x) { ar[d][x] = v4i{0, 0, 0, 0}; ai[d][x] = v4i{0, 0, 0, 0}; }
+    }
+    if (more) store_a((g + 1) & 1);
+    __syncthreads();
   }
 
   const int n = n0 + wn + lrow;
@@ -513,6 +693,8 @@ inline void oz_allow_lds(K kernel, size_t bytes) {
   if (bytes > 65536) GWHIP_CHECK(hipFuncSetAttribute((const void*)kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)bytes));
 }
 
+inline size_t ozaki_pipe_lds_bytes(int ns) { return (size_t)2 * 2 * OZ_BM * ns * 64; }
+
 inline size_t ozaki64_lds_bytes(int ns) {
   return (size_t)(2 * OZ4_BM + 2 * OZ4_BN) * ns * 64 + (OZ4_BM + OZ4_BN) * sizeof(int);
 }
@@ -527,6 +709,10 @@ inline void ozaki_launch(const OzGemmArgs& p, int64_t grid, hipStream_t stream) 
     const size_t lds = ozaki64_lds_bytes(NS);
     oz_allow_lds(zgemm_ozaki64_kernel<NS, false>, lds);
     hipLaunchKernelGGL((zgemm_ozaki64_kernel<NS, false>), dim3((unsigned)grid), dim3(256), lds, stream, p);
+  } else if (p.tile == 2) {
+    const size_t lds = ozaki_pipe_lds_bytes(NS);
+    oz_allow_lds(zgemm_ozaki_pipe_kernel<NS>, lds);
+    hipLaunchKernelGGL(zgemm_ozaki_pipe_kernel<NS>, dim3((unsigned)grid), dim3(256), lds, stream, p);
   } else {
     const size_t lds = ozaki_lds_bytes(NS);
     oz_allow_lds(zgemm_ozaki_kernel<NS>, lds);
@@ -537,7 +723,7 @@ inline void ozaki_launch(const OzGemmArgs& p, int64_t grid, hipStream_t stream) 
 inline void zgemm_ozaki(OzGemmArgs p, int ns, hipStream_t stream) {
   if (p.M <= 0 || p.N <= 0 || p.batch <= 0) return;
   if (p.kc % OZ_KS != 0 || p.nchunk <= 0 || p.nin <= 0) throw std::runtime_error("zgemm_ozaki: bad chunking");
-  if (p.tile != 0 && p.tile != 1) throw std::runtime_error("zgemm_ozaki: tile must be 0 or 1");
+  if (p.tile < 0 || p.tile > 2) throw std::runtime_error("zgemm_ozaki: tile must be 0, 1 or 2");
   // exact int32 accumulation: ns pairs at most share one accumulator, 4 products each ... bounded
   if ((double)p.kc * 2.0 * 127.0 * 127.0 * ns > 2147483647.0) throw std::runtime_error("zgemm_ozaki: chunk too long");
   const int BM = p.tile == 1 ? OZ4_BM : OZ_BM, BN = p.tile == 1 ? OZ4_BN : OZ_BN;

@@ -134,7 +134,8 @@ int gwhip_eval_sigma_device(gwhip_handle* h, const void* G_dev, void* Sigma_dev,
 int gwhip_autotune(gwhip_handle* h, const void* G_dev, int precision);
 /* The tuning as an array of GWHIP_TUNING_COUNT ints: [0] FP64 P0 GEMM tile, [1] FP64 Sigma GEMM
  * tile, [2] aux indices per wave, [3] int8 P0 GEMM row-tile group, [4] int8 Sigma GEMM row-tile
- * group, [5] int8 P0 GEMM block tile (0: 32x64, 1: 64x64), [6] int8 Sigma GEMM block tile,
+ * group, [5] int8 P0 GEMM block tile (0: 32x64, 1: 64x64, 2: 32x64 software-pipelined), [6] int8
+ * Sigma GEMM block tile,
  * [7] split P0 orbital kernel writes the int8 slices itself (1) or via a separate pass (0). */
 #define GWHIP_TUNING_COUNT 8
 int gwhip_get_tuning(const gwhip_handle* h, int* values, int n);

@@ -108,7 +108,7 @@ def test_split_tiles_and_orders_give_identical_sigma(p0_group, s_group):
         k.upload_V(V)
         ref = k.solve(G, gl.PREC_SPLIT)
         out = []
-        for tile in (0, 1):
+        for tile in (0, 1, 2):
             for fused in (0, 1):
                 k.set_tuning(p0_split_tile=tile, sigma_split_tile=tile, p0_group=p0_group, sigma_group=s_group,
                              p0_fused_slices=fused)
