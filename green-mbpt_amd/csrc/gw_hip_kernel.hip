@@ -423,7 +423,9 @@ template <int NB, typename T>
 void launch_z_prep_t(dim3 grid, size_t lds, hipStream_t st, const DevDims& dd, const double2* G, const double2* Vt,
                      double2* Z, const int* kplus, int t0, int QPW) {
   allow_lds(z_prep_mfma_kernel<NB, T>, lds);
-  hipLaunchKernelGGL((z_prep_mfma_kernel<NB, T>), grid, dim3(256), lds, st, dd, G, Vt, Z, kplus, t0, QPW);
+  const XcdGrid xg = make_xcd_grid(grid);
+  hipLaunchKernelGGL((z_prep_mfma_kernel<NB, T>), xcd_launch_grid(xg), dim3(256), lds, st, xg, dd, G, Vt, Z, kplus, t0,
+                     QPW);
   GWHIP_CHECK(hipGetLastError());
 }
 // The orbital-size products stay FP64 in both precision modes: FP32 matrix cores here put the
@@ -437,7 +439,9 @@ template <int NB, int NS>
 void launch_z_prep_slice_t(dim3 grid, size_t lds, hipStream_t st, const DevDims& dd, const double2* G, const double2* Vt,
                            int8_t* dst, int* exps, int kc, const int* kplus, int t0, int QPW) {
   allow_lds(z_prep_slice_kernel<NB, NS>, lds);
-  hipLaunchKernelGGL((z_prep_slice_kernel<NB, NS>), grid, dim3(256), lds, st, dd, G, Vt, dst, exps, kc, kplus, t0, QPW);
+  const XcdGrid xg = make_xcd_grid(grid);
+  hipLaunchKernelGGL((z_prep_slice_kernel<NB, NS>), xcd_launch_grid(xg), dim3(256), lds, st, xg, dd, G, Vt, dst, exps,
+                     kc, kplus, t0, QPW);
   GWHIP_CHECK(hipGetLastError());
 }
 template <int NB>
@@ -451,8 +455,9 @@ template <int NB, typename T>
 void launch_sigma_epi_t(dim3 grid, size_t lds, hipStream_t st, const DevDims& dd, const double2* G, const double2* Vt,
                         const double2* R, double2* S, const int* kminus, int t0, double scale, int upper) {
   allow_lds(sigma_epi_mfma_kernel<NB, T>, lds);
-  hipLaunchKernelGGL((sigma_epi_mfma_kernel<NB, T>), grid, dim3(256), lds, st, dd, G, Vt, R, S, kminus, t0, scale,
-                     upper);
+  const XcdGrid xg = make_xcd_grid(grid);
+  hipLaunchKernelGGL((sigma_epi_mfma_kernel<NB, T>), xcd_launch_grid(xg), dim3(256), lds, st, xg, dd, G, Vt, R, S,
+                     kminus, t0, scale, upper);
   GWHIP_CHECK(hipGetLastError());
 }
 template <int NB>
@@ -723,7 +728,7 @@ void gw_hip_kernel::stage_sigma(const double2* G, double2* Sigma, bool split) {
         timed(K_SIGMA_GEMM, [&] { sigma_gemm(q, t0, tc, tune_.s_shape); });
       }
       timed(K_SIGMA_EPI, [&] {
-        const dim3 grid((unsigned)nk, (unsigned)tc);
+        const dim3 grid((unsigned)nk, 1u, (unsigned)tc);
         const int* km = kminus_tab_ + q * nk;
         const int up = sigma_upper() ? 1 : 0;
         if (NB == 1) launch_sigma_epi<1>(grid, lds, stream_, dd, G, Vneg, scratch_, Sigma, km, (int)t0, scale, up);

@@ -17,6 +17,33 @@
 
 namespace gwhip {
 
+// Block order of the orbital kernels.  Consecutive hardware workgroup ids go to different XCDs,
+// so each XCD is given a contiguous range of logical blocks, and the logical index runs over tau
+// fastest: the blocks that read the same V(k, .) panel for different tau then run on one XCD
+// and find it in that XCD's L2.  Logical (x, y, z) over a gx x gy x gz grid: z fastest, then x.
+struct XcdGrid {
+  int gx, gy, gz;
+  int64_t per_xcd;  // ceil(gx gy gz / 8); the launch has 8 per_xcd workgroups
+};
+
+__device__ __forceinline__ bool xcd_block(const XcdGrid& g, int& x, int& y, int& z) {
+  const int64_t hw = blockIdx.x;
+  const int64_t l = (hw % 8) * g.per_xcd + hw / 8;
+  if (l >= (int64_t)g.gx * g.gy * g.gz) return false;
+  z = (int)(l % g.gz);
+  const int64_t r = l / g.gz;
+  x = (int)(r % g.gx);
+  y = (int)(r / g.gx);
+  return true;
+}
+
+inline XcdGrid make_xcd_grid(dim3 g) {
+  XcdGrid x{(int)g.x, (int)g.y, (int)g.z, 0};
+  x.per_xcd = ((int64_t)g.x * g.y * g.z + 7) / 8;
+  return x;
+}
+inline dim3 xcd_launch_grid(const XcdGrid& x) { return dim3((unsigned)(x.per_xcd * 8)); }
+
 typedef float v4f __attribute__((ext_vector_type(4)));
 
 // Matrix-core flavour of the orbital contractions: FP64 (v_mfma_f64_16x16x4f64) for the
@@ -62,7 +89,7 @@ __device__ __forceinline__ v4d v4zero() { return v4d{0.0, 0.0, 0.0, 0.0}; }
 // T = double: FP64 matrix cores; T = float: FP32 matrix cores (split-precision path), K <= 16NB
 // per MFMA chain, result written in FP64.
 template <int NB, typename T>
-__global__ __launch_bounds__(256) void z_prep_mfma_kernel(DevDims d, const double2* __restrict__ G,
+__global__ __launch_bounds__(256) void z_prep_mfma_kernel(XcdGrid xg, DevDims d, const double2* __restrict__ G,
                                                           const double2* __restrict__ Vt,
                                                           double2* __restrict__ Z, const int* __restrict__ kplus,
                                                           int t0, int QPW) {
@@ -75,8 +102,10 @@ __global__ __launch_bounds__(256) void z_prep_mfma_kernel(DevDims d, const doubl
   T* Gqr = Gbi + PL;
   T* Gqi = Gqr + PL;
   const int nao = d.nao, nao2 = nao * nao;
-  const int tz = blockIdx.z, tau = t0 + tz, tt = d.nts - 1 - tau;
-  const int k = blockIdx.y, kq = kplus[k];
+  int bx, by, bz;
+  if (!xcd_block(xg, bx, by, bz)) return;
+  const int tz = bz, tau = t0 + tz, tt = d.nts - 1 - tau;
+  const int k = by, kq = kplus[k];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lrow = lane & 15, lk = lane >> 4;
   const int nslab = (nao + 3) / 4;
@@ -100,7 +129,7 @@ __global__ __launch_bounds__(256) void z_prep_mfma_kernel(DevDims d, const doubl
     }
     __syncthreads();
     for (int qi = 0; qi < QPW; ++qi) {
-      const int Q = (blockIdx.x * 4 + wave) * QPW + qi;
+      const int Q = (bx * 4 + wave) * QPW + qi;
       if (Q >= d.NQ) break;
       const double2* vt = Vt + ((int64_t)k * d.NQ + Q) * nao2;  // vt[m * nao + p] = V[p][m]
       double2* zq = Z + (((int64_t)tz * d.nk + k) * d.NQ + Q) * nao2;
@@ -168,7 +197,7 @@ __global__ __launch_bounds__(256) void z_prep_mfma_kernel(DevDims d, const doubl
 // the slices are bit-identical to slicing the FP64 Z, without Z ever reaching HBM.
 // Slice layout: dst[((tz*NQ + Q)*nk + k)][plane][slice][kc], exps[(tz*NQ + Q)*nk + k].
 template <int NB, int NS>
-__global__ __launch_bounds__(256) void z_prep_slice_kernel(DevDims d, const double2* __restrict__ G,
+__global__ __launch_bounds__(256) void z_prep_slice_kernel(XcdGrid xg, DevDims d, const double2* __restrict__ G,
                                                            const double2* __restrict__ Vt, int8_t* __restrict__ dst,
                                                            int* __restrict__ exps, int kc,
                                                            const int* __restrict__ kplus, int t0, int QPW) {
@@ -179,8 +208,10 @@ __global__ __launch_bounds__(256) void z_prep_slice_kernel(DevDims d, const doub
   double* Gqr = Gbi + PL;
   double* Gqi = Gqr + PL;
   const int nao = d.nao, nao2 = nao * nao;
-  const int tz = blockIdx.z, tau = t0 + tz, tt = d.nts - 1 - tau;
-  const int k = blockIdx.y, kq = kplus[k];
+  int bx, by, bz;
+  if (!xcd_block(xg, bx, by, bz)) return;
+  const int tz = bz, tau = t0 + tz, tt = d.nts - 1 - tau;
+  const int k = by, kq = kplus[k];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lrow = lane & 15, lk = lane >> 4;
   const int nslab = (nao + 3) / 4;
@@ -202,7 +233,7 @@ __global__ __launch_bounds__(256) void z_prep_slice_kernel(DevDims d, const doub
   }
   __syncthreads();
   for (int qi = 0; qi < QPW; ++qi) {
-    const int Q = (blockIdx.x * 4 + wave) * QPW + qi;
+    const int Q = (bx * 4 + wave) * QPW + qi;
     if (Q >= d.NQ) break;
     const double2* vt = Vt + ((int64_t)k * d.NQ + Q) * nao2;  // vt[m * nao + p] = V[p][m]
     v4d zr[NB][NB], zi[NB][NB];                                 // [bn][bt]
@@ -292,10 +323,11 @@ __global__ __launch_bounds__(256) void z_prep_slice_kernel(DevDims d, const doub
 // with V = V(k, k - q), G_s = G(tau, k - q) block (a, b)  (src/gw_cpu_kernel.cpp:359-373,
 // Y2 = Y1^T P done by the big GEMM as R = P-tilde conj(V)).  The first MFMA produces
 // T^T = G^T V^T so that its result registers feed the second MFMA's A operand.
-// grid: (nk, tcount); wave w takes Q = w, w + 4, ...; the four partial sums meet in LDS.
+// grid: (nk, 1, tcount) in the XcdGrid order; wave w takes Q = w, w + 4, ...; the four partial
+// sums meet in LDS.
 // With T = float each Q's product (K <= 16 NB) is added to FP64 accumulators.
 template <int NB, typename T>
-__global__ __launch_bounds__(256) void sigma_epi_mfma_kernel(DevDims d, const double2* __restrict__ G,
+__global__ __launch_bounds__(256) void sigma_epi_mfma_kernel(XcdGrid xg, DevDims d, const double2* __restrict__ G,
                                                              const double2* __restrict__ Vt,
                                                              const double2* __restrict__ R,
                                                              double2* __restrict__ Sigma,
@@ -310,7 +342,9 @@ __global__ __launch_bounds__(256) void sigma_epi_mfma_kernel(DevDims d, const do
   T* Gi = Gr + PL;
   double* red = lds_d + PL;  // [NB*NB*8][64], after the T planes (2 PL T <= PL doubles)
   const int nao = d.nao, nao2 = nao * nao;
-  const int k = blockIdx.x, tz = blockIdx.y, tau = t0 + tz;
+  int bx, by, bz;
+  if (!xcd_block(xg, bx, by, bz)) return;
+  const int k = bx, tz = bz, tau = t0 + tz;
   const int kmq = kminus[k];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lrow = lane & 15, lk = lane >> 4;
