@@ -237,21 +237,26 @@ __global__ __launch_bounds__(256) void z_prep_slice_kernel(XcdGrid xg, DevDims d
     if (Q >= d.NQ) break;
     const double2* vt = Vt + ((int64_t)k * d.NQ + Q) * nao2;  // vt[m * nao + p] = V[p][m]
     v4d zr[NB][NB], zi[NB][NB];                                 // [bn][bt]
+    double2 avs[4 * NB][NB];                                    // V_Q, loaded once per Q
+#pragma unroll
+    for (int sl = 0; sl < 4 * NB; ++sl)
+#pragma unroll
+      for (int bp = 0; bp < NB; ++bp) {
+        const int m = 4 * sl + lk, p = 16 * bp + lrow;
+        avs[sl][bp] = (p < nao && m < nao) ? vt[m * nao + p] : make_double2(0.0, 0.0);
+      }
 #pragma unroll
     for (int bn = 0; bn < NB; ++bn) {
       v4d wr[NB], wi[NB];
 #pragma unroll
       for (int x = 0; x < NB; ++x) { wr[x] = v4zero(); wi[x] = v4zero(); }
-      for (int sl = 0; sl < nslab; ++sl) {
+#pragma unroll
+      for (int sl = 0; sl < 4 * NB; ++sl) {
+        if (sl >= nslab) break;
         const int m = 4 * sl + lk;
         const double br = Gqr[m * LD + 16 * bn + lrow], bi = Gqi[m * LD + 16 * bn + lrow];
 #pragma unroll
-        for (int bp = 0; bp < NB; ++bp) {
-          const int p = 16 * bp + lrow;
-          double2 av = make_double2(0.0, 0.0);
-          if (p < nao && m < nao) av = vt[m * nao + p];
-          cmfma(wr[bp], wi[bp], av.x, av.y, br, bi);
-        }
+        for (int bp = 0; bp < NB; ++bp) cmfma(wr[bp], wi[bp], avs[sl][bp].x, avs[sl][bp].y, br, bi);
       }
 #pragma unroll
       for (int x = 0; x < NB; ++x) { zr[bn][x] = v4zero(); zi[bn][x] = v4zero(); }
@@ -384,14 +389,20 @@ __global__ __launch_bounds__(256) void sigma_epi_mfma_kernel(XcdGrid xg, DevDims
 #pragma unroll
         for (int x = 0; x < NB; ++x) { tr[x] = M::zero(); ti[x] = M::zero(); }
         const int i = 16 * bi + lrow;
-        for (int sl = 0; sl < nslab; ++sl) {
+        double2 bvs[4 * NB];  // this lane's V_Q elements of row block bi, all slabs in flight together
+#pragma unroll
+        for (int sl = 0; sl < 4 * NB; ++sl) {
           const int m = 4 * sl + lk;
-          double2 bv = make_double2(0.0, 0.0);
-          if (m < nao && i < nao) bv = vt[m * nao + i];
+          bvs[sl] = (m < nao && i < nao) ? vt[m * nao + i] : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int sl = 0; sl < 4 * NB; ++sl) {
+          if (sl >= nslab) break;
+          const int m = 4 * sl + lk;
 #pragma unroll
           for (int bn = 0; bn < NB; ++bn) {
             const int n = 16 * bn + lrow;
-            cmfma<T>(tr[bn], ti[bn], Gr[m * LD + n], Gi[m * LD + n], (T)bv.x, (T)bv.y);
+            cmfma<T>(tr[bn], ti[bn], Gr[m * LD + n], Gi[m * LD + n], (T)bvs[sl].x, (T)bvs[sl].y);
           }
         }
         acc_t pr[NB], pi[NB];
