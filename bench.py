@@ -31,8 +31,10 @@ WORKLOAD = dict(nk=64, mesh=(4, 4, 4), nao=26, NQ=300, ns=1, beta=100.0, lam=1e4
 # scaled by what the counter reads for a plain 2 GiB device copy (bench.py --copy-calibration):
 # FETCH_SIZE reads 0.5x the bytes the copy reads, WRITE_SIZE 1.0x the bytes it writes.
 # Source: profiles/gw_sigma_nk64_nao26_naux300_hbm_traffic.md
-PMC_PER_LAUNCH_KIB = {"split": (8.6671e6, 2.0407e7), "fp64": (5.1556e7, 2.0376e7)}
+PMC_PER_LAUNCH_KIB = {"split": (8.5755e6, 2.0403e7), "fp64": (5.1575e7, 2.0371e7)}
 PMC_CALIBRATION = (0.5, 1.0)
+# int8 GEMM kernel per block-tile setting (tuning key sigma_split_tile / p0_split_tile)
+OZAKI_KERNELS = {0: "zgemm_ozaki_kernel", 1: "zgemm_ozaki64_kernel", 2: "zgemm_ozaki_pipe_kernel"}
 
 
 def algorithmic_flops(nk, nao, NQ, nts, nw_b, pseudo_ns=1):
@@ -183,7 +185,7 @@ def main():
         # int8 MFMA operations actually issued: 4 real products per complex product, ns(ns+1)/2 slice pairs
         ops = 2.0 * 4 * (ns * (ns + 1) // 2) * w["NQ"] * kpad * w["nk"] * w["nao"] ** 2 * T.nts
         roof = {"bound": "mfma", "achieved": ops / avg_s / 1e12, "peak": PEAK_INT8_TOPS, "unit": "TOPS",
-                "kernel": f"zgemm_ozaki_kernel<{ns}> (Sigma contraction, int8 slices)",
+                "kernel": f"{OZAKI_KERNELS[k.tuning()['sigma_split_tile']]}<{ns}> (Sigma contraction, int8 slices)",
                 "algorithmic_fp64_tflops": per_launch / avg_s / 1e12}
     roof["frac"] = roof["achieved"] / roof["peak"]
     fetch_kib, write_kib = PMC_PER_LAUNCH_KIB[args.precision]
