@@ -174,14 +174,14 @@ private:
     size_t sc_full = 0, sc_min = 0, dy_full = 0, dy_min = 0, sl_full = 0, sl_min = 0, ex_full = 0, ex_min = 0;
     for (auto& s : systems_) {
       const Dims& d = s->dims();
-      sc_full = std::max(sc_full, s->scratch_per_t() * d.nts);
+      sc_full = std::max(sc_full, s->scratch_per_t() * (d.nts / 2));  // both stages use half tau grids
       sc_min = std::max(sc_min, s->scratch_per_t());
       dy_full = std::max(dy_full, s->dyson_per_q() * d.nk);
       dy_min = std::max(dy_min, s->dyson_per_q());
       if (split) {
-        sl_full = std::max(sl_full, s->slice_bytes_per_t() * d.nts);
+        sl_full = std::max(sl_full, s->slice_bytes_per_t() * (d.nts / 2));
         sl_min = std::max(sl_min, s->slice_bytes_per_t());
-        ex_full = std::max(ex_full, s->exps_per_t() * d.nts);
+        ex_full = std::max(ex_full, s->exps_per_t() * (d.nts / 2));
         ex_min = std::max(ex_min, s->exps_per_t());
       }
     }

@@ -113,7 +113,7 @@ gw_hip_kernel::gw_hip_kernel(const Dims& d, int device, hipStream_t shared_strea
   qchunk_ = std::max<int64_t>(1, std::min<int64_t>(nk, (int64_t)(dyson_share / per_q_dyson)));
   const double scr_budget = budget - qchunk_ * per_q_dyson;
   int64_t tmax = std::max<int64_t>(1, (int64_t)(scr_budget / per_t_scr));
-  tchunk_sigma_ = std::min<int64_t>(d_.nts, tmax);
+  tchunk_sigma_ = std::min<int64_t>(d_.nts / 2, tmax);  // R is formed on the half tau grid
   tchunk_p0_ = std::min<int64_t>(d_.nts / 2, tmax);
   Pw_ = dalloc<double2>((size_t)qchunk_ * d_.nw_b * NQ2);
   X_ = dalloc<double2>((size_t)qchunk_ * d_.nw_b * NQ2);
@@ -271,7 +271,7 @@ void gw_hip_kernel::bind_workspace(const Workspace& ws) {
     tmax = std::min<int64_t>(tmax, (int64_t)(ws.exp_elems / exps_per_t()));
   }
   if (tmax < 1 || qmax < 1) throw status_error(GWHIP_ERR_OOM, "shared workspace too small for this system");
-  tchunk_sigma_ = std::min<int64_t>(d_.nts, tmax);
+  tchunk_sigma_ = std::min<int64_t>(d_.nts / 2, tmax);  // R is formed on the half tau grid
   tchunk_p0_ = std::min<int64_t>(d_.nts / 2, tmax);
   qchunk_ = std::min<int64_t>(d_.nk, qmax);
   const size_t nmat = (size_t)qchunk_ * d_.nw_b * d_.NQ * d_.NQ;
@@ -453,18 +453,19 @@ void launch_z_prep_slice(int ns, dim3 grid, size_t lds, hipStream_t st, const De
 }
 template <int NB, typename T>
 void launch_sigma_epi_t(dim3 grid, size_t lds, hipStream_t st, const DevDims& dd, const double2* G, const double2* Vt,
-                        const double2* R, double2* S, const int* kminus, int t0, double scale, int upper) {
+                        const double2* R, double2* S, const int* kminus, int t0, double scale, int upper,
+                        int mirror) {
   allow_lds(sigma_epi_mfma_kernel<NB, T>, lds);
   const XcdGrid xg = make_xcd_grid(grid);
   hipLaunchKernelGGL((sigma_epi_mfma_kernel<NB, T>), xcd_launch_grid(xg), dim3(256), lds, st, xg, dd, G, Vt, R, S,
-                     kminus, t0, scale, upper);
+                     kminus, t0, scale, upper, mirror);
   GWHIP_CHECK(hipGetLastError());
 }
 template <int NB>
 void launch_sigma_epi(dim3 grid, size_t lds, hipStream_t st, const DevDims& dd, const double2* G,
                       const double2* Vt, const double2* R, double2* S, const int* kminus, int t0, double scale,
-                      int upper) {
-  launch_sigma_epi_t<NB, double>(grid, lds, st, dd, G, Vt, R, S, kminus, t0, scale, upper);
+                      int upper, int mirror) {
+  launch_sigma_epi_t<NB, double>(grid, lds, st, dd, G, Vt, R, S, kminus, t0, scale, upper, mirror);
 }
 }  // namespace
 
@@ -717,10 +718,14 @@ void gw_hip_kernel::stage_sigma(const double2* G, double2* Sigma, bool split) {
   const size_t lds = ((size_t)2 * (16 * NB) * (16 * NB + 1) + (size_t)NB * NB * 8 * 64) * sizeof(double);
   const double scale = -1.0 / (double)nk;
 
+  // P0(beta - tau) = P0(tau) makes P-tilde(beta - tau) = P-tilde(tau) (to the rounding of the
+  // tau <-> Omega transforms, ~1e-12 relative on the IR grids), so R = P-tilde conj(V) is formed on
+  // the half grid only and serves both tau and beta - tau in the epilogue.
+  const int64_t half = nts / 2;
   for (int64_t q = 0; q < nk; ++q) {
     const double2* Vneg = Vt_ + (int64_t)neg_q_[q] * nk * NQ * nao2;  // V(k, k - q), per-Q transposed
-    for (int64_t t0 = 0; t0 < nts; t0 += tchunk_sigma_) {
-      const int64_t tc = std::min(tchunk_sigma_, nts - t0);
+    for (int64_t t0 = 0; t0 < half; t0 += tchunk_sigma_) {
+      const int64_t tc = std::min(tchunk_sigma_, half - t0);
       if (split) {
         sigma_slice(q, t0, tc);
         timed(K_SIGMA_GEMM, [&] { sigma_gemm_split(q, tc, tune_.s_group, tune_.s_oz_tile); });
@@ -728,12 +733,12 @@ void gw_hip_kernel::stage_sigma(const double2* G, double2* Sigma, bool split) {
         timed(K_SIGMA_GEMM, [&] { sigma_gemm(q, t0, tc, tune_.s_shape); });
       }
       timed(K_SIGMA_EPI, [&] {
-        const dim3 grid((unsigned)nk, 1u, (unsigned)tc);
+        const dim3 grid((unsigned)nk, 1u, (unsigned)(2 * tc));
         const int* km = kminus_tab_ + q * nk;
         const int up = sigma_upper() ? 1 : 0;
-        if (NB == 1) launch_sigma_epi<1>(grid, lds, stream_, dd, G, Vneg, scratch_, Sigma, km, (int)t0, scale, up);
-        else if (NB == 2) launch_sigma_epi<2>(grid, lds, stream_, dd, G, Vneg, scratch_, Sigma, km, (int)t0, scale, up);
-        else launch_sigma_epi<3>(grid, lds, stream_, dd, G, Vneg, scratch_, Sigma, km, (int)t0, scale, up);
+        if (NB == 1) launch_sigma_epi<1>(grid, lds, stream_, dd, G, Vneg, scratch_, Sigma, km, (int)t0, scale, up, 1);
+        else if (NB == 2) launch_sigma_epi<2>(grid, lds, stream_, dd, G, Vneg, scratch_, Sigma, km, (int)t0, scale, up, 1);
+        else launch_sigma_epi<3>(grid, lds, stream_, dd, G, Vneg, scratch_, Sigma, km, (int)t0, scale, up, 1);
       });
     }
   }
@@ -827,7 +832,7 @@ void gw_hip_kernel::autotune(const double2* G, int precision) {
   const bool split = precision == GWHIP_PREC_SPLIT;
   if (split) ensure_slices();
   detect_hermitian_G(G);
-  const int64_t half = d_.nts / 2, tcp = std::min(tchunk_p0_, half), tcs = std::min(tchunk_sigma_, d_.nts);
+  const int64_t half = d_.nts / 2, tcp = std::min(tchunk_p0_, half), tcs = std::min(tchunk_sigma_, half);
   hipEvent_t a, b;
   GWHIP_CHECK(hipEventCreate(&a));
   GWHIP_CHECK(hipEventCreate(&b));

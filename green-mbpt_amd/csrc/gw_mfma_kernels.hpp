@@ -328,8 +328,8 @@ __global__ __launch_bounds__(256) void z_prep_slice_kernel(XcdGrid xg, DevDims d
 // with V = V(k, k - q), G_s = G(tau, k - q) block (a, b)  (src/gw_cpu_kernel.cpp:359-373,
 // Y2 = Y1^T P done by the big GEMM as R = P-tilde conj(V)).  The first MFMA produces
 // T^T = G^T V^T so that its result registers feed the second MFMA's A operand.
-// grid: (nk, 1, tcount) in the XcdGrid order; wave w takes Q = w, w + 4, ...; the four partial
-// sums meet in LDS.
+// grid: (nk, 1, tcount) in the XcdGrid order (tcount = 2 x the R rows when mirror is set); wave w
+// takes Q = w, w + 4, ...; the four partial sums meet in LDS.
 // With T = float each Q's product (K <= 16 NB) is added to FP64 accumulators.
 template <int NB, typename T>
 __global__ __launch_bounds__(256) void sigma_epi_mfma_kernel(XcdGrid xg, DevDims d, const double2* __restrict__ G,
@@ -337,7 +337,7 @@ __global__ __launch_bounds__(256) void sigma_epi_mfma_kernel(XcdGrid xg, DevDims
                                                              const double2* __restrict__ R,
                                                              double2* __restrict__ Sigma,
                                                              const int* __restrict__ kminus, int t0, double scale,
-                                                             int upper) {
+                                                             int upper, int mirror) {
   typedef Mfma<T> M;
   typedef typename M::acc_t acc_t;
   constexpr bool F64 = sizeof(T) == 8;
@@ -349,7 +349,10 @@ __global__ __launch_bounds__(256) void sigma_epi_mfma_kernel(XcdGrid xg, DevDims
   const int nao = d.nao, nao2 = nao * nao;
   int bx, by, bz;
   if (!xcd_block(xg, bx, by, bz)) return;
-  const int k = bx, tz = bz, tau = t0 + tz;
+  // mirror != 0: R holds tau = t0 + bz/2 of the half grid and serves tau and beta - tau
+  // (P-tilde(beta - tau) = P-tilde(tau)); the pair runs on neighbouring blocks
+  const int tz = mirror ? (bz >> 1) : bz;
+  const int k = bx, tau = mirror && (bz & 1) ? d.nts - 1 - (t0 + tz) : t0 + tz;
   const int kmq = kminus[k];
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lrow = lane & 15, lk = lane >> 4;
