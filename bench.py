@@ -26,12 +26,18 @@ PEAK_INT8_TOPS = 5000.0
 
 WORKLOAD = dict(nk=64, mesh=(4, 4, 4), nao=26, NQ=300, ns=1, beta=100.0, lam=1e4, seed=0, sigma=0.1)
 
-# HBM bytes per launch of the Sigma-contraction GEMM on this workload, from two rocprofv3 --pmc
-# runs (FETCH_SIZE, WRITE_SIZE in KiB per dispatch) of this command with the same build; each
-# scaled by what the counter reads for a plain 2 GiB device copy (bench.py --copy-calibration):
-# FETCH_SIZE reads 0.5x the bytes the copy reads, WRITE_SIZE 1.0x the bytes it writes.
+# HBM bytes per launch of the four main kernels on this workload, from two rocprofv3 --pmc runs
+# (FETCH_SIZE, WRITE_SIZE in KiB per dispatch) of `bench.py --steps 1 --warmup 0 --no-cpu
+# [--no-check | --precision fp64] --no-autotune` with the same build; each scaled by what the
+# counter reads for a plain 2 GiB device copy (bench.py --copy-calibration): FETCH_SIZE reads 0.5x
+# the bytes the copy reads, WRITE_SIZE 1.0x the bytes it writes.
 # Source: profiles/gw_sigma_nk64_nao26_naux300_hbm_traffic.md
-PMC_PER_LAUNCH_KIB = {"split": (8.5755e6, 2.0403e7), "fp64": (5.1575e7, 2.0371e7)}
+PMC_PER_LAUNCH_KIB = {
+    "split": {"Sigma GEMM": (4.3304e6, 1.0202e7), "Sigma epilogue": (5.7919e6, 6.7600e4),
+              "P0 GEMM": (3.4184e7, 4.2315e4), "P0 prep": (7.7595e5, 6.7702e6)},
+    "fp64": {"Sigma GEMM": (2.5800e7, 1.0178e7), "Sigma epilogue": (5.8315e6, 6.7600e4),
+             "P0 GEMM": (1.1350e7, 4.2515e4), "P0 prep": (7.4522e5, 1.1822e7)},
+}
 PMC_CALIBRATION = (0.5, 1.0)
 # int8 GEMM kernel per block-tile setting (tuning key sigma_split_tile / p0_split_tile)
 OZAKI_KERNELS = {0: "zgemm_ozaki_kernel", 1: "zgemm_ozaki64_kernel", 2: "zgemm_ozaki_pipe_kernel"}
@@ -172,26 +178,54 @@ def main():
     S_host = S_d.cpu().numpy().view(np.complex128).copy()
     herm = float(np.abs(S_host - np.conj(np.swapaxes(S_host, -1, -2))).max() / np.abs(S_host).max())
 
-    # dominant kernel: the Sigma contraction GEMM (R = P~ conj(V)), one launch per q (all 100 tau)
-    n_l, ms_l = stats["Sigma GEMM"]
-    avg_s = ms_l / max(n_l, 1) * 1e-3
-    per_launch = 8.0 * w["NQ"] * w["NQ"] * w["nk"] * w["nao"] ** 2 * T.nts   # complex MACs x 8, all k and tau of one q
+    # roofline of the kernel with the largest share of the step.  Work per launch (one q, all k):
+    # the Sigma GEMM and the P0 GEMM run over the half tau grid (R is formed for tau < nts/2 and
+    # mirrored; P0(beta - tau) = P0(tau)), the Sigma epilogue over all nts, the P0 prep over nts/2.
+    nk_, nao_, NQ_, half = w["nk"], w["nao"], w["NQ"], T.nts // 2
+    ns = args.slices
+    kpad = -(-NQ_ // 64) * 64
+    kp0 = -(-nao_ * nao_ // 64) * 64
+    f64_gemm = 8.0 * NQ_ * NQ_ * nk_ * nao_ ** 2 * half                 # complex MACs x 8
+    orb = 16.0 * NQ_ * nao_ ** 3 * nk_                                   # two nao^3 products per (k, tau, Q)
+    kname = OZAKI_KERNELS[k.tuning()["sigma_split_tile"]]
+    pname = OZAKI_KERNELS[k.tuning()["p0_split_tile"]]
     if prec == gl.PREC_FP64:
-        roof = {"bound": "mfma", "achieved": per_launch / avg_s / 1e12, "peak": PEAK_FP64_TFLOPS, "unit": "TFLOP/s",
-                "kernel": "zgemm_f64_kernel<64,64,true,false> (Sigma contraction R = P~ conj(V), FP64 MFMA)"}
+        table = {
+            "Sigma GEMM": (f64_gemm, PEAK_FP64_TFLOPS, "TFLOP/s",
+                           "zgemm_f64_kernel<64,64,true,false> (Sigma contraction R = P~ conj(V), FP64 MFMA)"),
+            "P0 GEMM": (f64_gemm, PEAK_FP64_TFLOPS, "TFLOP/s", "zgemm_f64_kernel<64,64,true,true> (P0 contraction, FP64 MFMA)"),
+        }
     else:
-        ns = args.slices
-        kpad = -(-w["NQ"] // 64) * 64
-        # int8 MFMA operations actually issued: 4 real products per complex product, ns(ns+1)/2 slice pairs
-        ops = 2.0 * 4 * (ns * (ns + 1) // 2) * w["NQ"] * kpad * w["nk"] * w["nao"] ** 2 * T.nts
-        roof = {"bound": "mfma", "achieved": ops / avg_s / 1e12, "peak": PEAK_INT8_TOPS, "unit": "TOPS",
-                "kernel": f"{OZAKI_KERNELS[k.tuning()['sigma_split_tile']]}<{ns}> (Sigma contraction, int8 slices)",
-                "algorithmic_fp64_tflops": per_launch / avg_s / 1e12}
+        pairs = ns * (ns + 1) // 2                                       # slice pairs; 4 real products each
+        table = {
+            "Sigma GEMM": (2.0 * 4 * pairs * NQ_ * kpad * nk_ * nao_ ** 2 * half, PEAK_INT8_TOPS, "TOPS",
+                           f"{kname}<{ns}> (Sigma contraction, int8 slices; int8 operations issued)"),
+            "P0 GEMM": (2.0 * 4 * pairs * NQ_ * NQ_ * nk_ * kp0 * half, PEAK_INT8_TOPS, "TOPS",
+                        f"{pname}<{ns}> (P0 contraction, int8 slices; operations of the full product)"),
+        }
+    table["Sigma epilogue"] = (orb * T.nts, PEAK_FP64_TFLOPS, "TFLOP/s",
+                               "sigma_epi_mfma_kernel<2,double> (Sigma += (V G) R, FP64 MFMA, algorithmic flops)")
+    table["P0 prep"] = (orb * half, PEAK_FP64_TFLOPS, "TFLOP/s",
+                        ("z_prep_slice_kernel<2,5>" if prec == gl.PREC_SPLIT else "z_prep_mfma_kernel<2,double>")
+                        + " (Z = Gb V Gq, FP64 MFMA, algorithmic flops)")
+    dom = max(table, key=lambda name: stats[name][1])
+    n_l, ms_l = stats[dom]
+    avg_s = ms_l / max(n_l, 1) * 1e-3
+    work, peak, unit, kdesc = table[dom]
+    roof = {"bound": "mfma", "achieved": work / avg_s / 1e12, "peak": peak, "unit": unit, "kernel": kdesc,
+            "stage": dom}
     roof["frac"] = roof["achieved"] / roof["peak"]
-    fetch_kib, write_kib = PMC_PER_LAUNCH_KIB[args.precision]
+    fetch_kib, write_kib = PMC_PER_LAUNCH_KIB[args.precision][dom]
     traffic = 1024.0 * (fetch_kib / PMC_CALIBRATION[0] + write_kib / PMC_CALIBRATION[1])
     roof.update({"traffic": traffic, "traffic_unit": "bytes/launch", "traffic_GBps": traffic / avg_s / 1e9,
                  "launches": n_l, "avg_launch_ms": avg_s * 1e3})
+    others = {}
+    for name, (wk, pk, un, _) in table.items():
+        n2, ms2 = stats[name]
+        if n2:
+            t2 = ms2 / n2 * 1e-3
+            others[name] = {"avg_launch_ms": t2 * 1e3, "achieved": wk / t2 / 1e12, "unit": un, "frac": wk / t2 / 1e12 / pk}
+    roof["all_stages"] = others
 
     check = {}
     if prec == gl.PREC_SPLIT and not args.no_check:
