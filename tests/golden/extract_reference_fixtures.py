@@ -21,7 +21,8 @@ Sources (reference paths relative to its ``test/data``):
 * ``GW/input.h5`` (identical to ``HF/input.h5``) -- the nk=27 (3x3x3) H2 system with space-group
   symmetry: ns=2, nao=nso=2, NQ=36, ink=4 irreducible k points, 196 of 378 k pairs stored.
   Written to ``sym27.npz``: params, ``symmetry/k`` (bz2ibz, ibz2bz, k_sym_transform_ao,
-  tr_conj, mesh_scaled), ``symmetry/pairs`` (conj/trans pair lists, kpair_irre_list,
+  tr_conj, mesh_scaled), ``symmetry/q`` (inq, bz2ibz, ibz2bz, tr_conj and the auxiliary-basis
+  matrices k_sym_transform_p0), ``symmetry/pairs`` (conj/trans pair lists, kpair_irre_list,
   kpair_idx), HF/S-k and HF/madelung.
 * ``HF/data.h5`` + ``HF/df_hf_int/VQ_0.h5`` -- the ``HF`` section (``solvers_test.cpp:387``):
   G_tau (irreducible k), result/Sigma1 -> ``hf_sym27.npz``.
@@ -30,7 +31,8 @@ Sources (reference paths relative to its ``test/data``):
 * The integral files themselves, byte for byte, for the HDF5 loader: ``GW/df_int/{meta,VQ_0}.h5``
   -> ``h5/gw_sym27_df/``; ``Input/df_int*/meta.h5`` (no version attribute, 0.3.0, 1.0.0, 1.1.0;
   ``solvers_test.cpp:470-495``) -> ``h5/versions/<dir>/meta.h5``; and the ``params``,
-  ``symmetry/k`` and ``symmetry/pairs`` groups of ``GW/input.h5`` -> ``h5/sym27_input.h5``.
+  ``symmetry/k`` and ``symmetry/pairs`` groups and the ``symmetry/q`` tables of ``GW/input.h5``
+  -> ``h5/sym27_input.h5``.
 * ``GW/input_tr_symm.h5`` + ``GW/df_hf_int/{VQ_0,AqQ}.h5`` + ``GW/data_c.h5`` -- the ``GW_C``
   section (``solvers_test.cpp:390``, q0_treatment EXTRAPOLATE): the same H2 system with 14
   irreducible k (time reversal on 13 of the 27 points), the q -> 0 data (AqQ [inq][NQ], q_abs,
@@ -100,6 +102,9 @@ def main(data_dir):
             sym["k_" + k] = f["symmetry/k/" + k][()]
         for k in ("conj_pairs_list", "trans_pairs_list", "kpair_irre_list", "kpair_idx", "num_kpair_stored"):
             sym["pairs_" + k] = f["symmetry/pairs/" + k][()]
+        sym["inq"] = int(f["symmetry/q/inq"][()])
+        for k in ("bz2ibz", "ibz2bz", "tr_conj", "k_sym_transform_p0"):
+            sym["q_" + k] = f["symmetry/q/" + k][()]
         sym["S_k"] = np.ascontiguousarray(f["HF/S-k"][()]).view(np.complex128)[..., 0]
         sym["madelung"] = float(f["HF/madelung"][()])
     np.savez(os.path.join(OUT, "sym27.npz"), **sym)
@@ -126,6 +131,9 @@ def main(data_dir):
         dst.create_group("symmetry")
         for g in ("k", "pairs"):
             src.copy(src["symmetry/" + g], dst["symmetry"], name=g)
+        q = dst["symmetry"].create_group("q")
+        for k in ("bz2ibz", "ibz2bz", "inq", "tr_conj", "k_sym_transform_p0", "mesh_scaled", "nq"):
+            q[k] = src["symmetry/q/" + k][()]
     tr_in = os.path.join(data_dir, "GW/input_tr_symm.h5")
     c = {}
     with h5py.File(tr_in, "r") as f:
@@ -133,7 +141,7 @@ def main(data_dir):
         c["inq"] = int(f["symmetry/q/inq"][()])
         for k in ("bz2ibz", "ibz2bz", "k_sym_transform_ao", "tr_conj", "mesh_scaled"):
             c["k_" + k] = f["symmetry/k/" + k][()]
-        for k in ("bz2ibz", "ibz2bz", "tr_conj"):
+        for k in ("bz2ibz", "ibz2bz", "tr_conj", "k_sym_transform_p0"):
             c["q_" + k] = f["symmetry/q/" + k][()]
         for k in ("conj_pairs_list", "trans_pairs_list", "kpair_irre_list"):
             c["pairs_" + k] = f["symmetry/pairs/" + k][()]
@@ -158,7 +166,8 @@ def main(data_dir):
         for g in ("k", "pairs"):
             src.copy(src["symmetry/" + g], dst["symmetry"], name=g)
         q = dst["symmetry"].create_group("q")
-        for k in ("bz2ibz", "ibz2bz", "inq", "tr_conj", "mesh", "mesh_scaled", "nq", "weight_ibz"):
+        for k in ("bz2ibz", "ibz2bz", "inq", "tr_conj", "k_sym_transform_p0", "mesh", "mesh_scaled", "nq",
+                  "weight_ibz"):
             q[k] = src["symmetry/q/" + k][()]
     orth = {}
     for mode in ("none", "lowdin", "mo", "natural"):

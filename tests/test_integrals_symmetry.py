@@ -14,6 +14,12 @@ Oracle pins (oracle/integrals.py, oracle/symmetry.py):
   q -> 0 fit against an exact quadratic.
 * the time-reversal branch of value_AO is pinned by GW_C only where U = I (conjugation alone);
   with tr_conj = 1 and U != I the order of U and conjugation is parity unpinned.
+* the reference's own star-of-q evaluation (oracle/symmetry.py gw_sigma_star: P-tilde of the 4
+  irreducible q, Sigma of the 4 irreducible k over every q of each star, P-tilde rotated by the
+  auxiliary-basis matrices k_sym_transform_p0) against the GW fixture at 2.6e-11 and the GW_C
+  fixture at 2.8e-11; the same evaluation without the rotation is rejected (2e-2).  The order
+  "rotate, then conjugate" of eval_p0_bz_from_ibz is read from gw_cpu_kernel.cpp:281-293; no
+  fixture has tr_conj = 1 with U_q != I, so that combination is parity unpinned.
 
 Host reader (libgwhip_h5.so, no GPU needed):
 * the version rule against the reference's own pass/fail strings (solvers_test.cpp:497-508) and
@@ -85,6 +91,49 @@ def test_gw_sym27_fixture():
     err = np.abs(osy.full_to_ibz(S, z["k_ibz2bz"]) - g["Sigma_tau"]).max()
     assert err < 1e-6      # reference tolerance (solvers_test.cpp:134)
     assert err < 2e-8      # reached: 8.9e-9 (full mesh here, q stars with rotated P0 in the reference)
+
+
+def _star(sym, g, rotate=True):
+    Vp = oi.packed_from_irreducible(g["V"], 27, *_pairs(sym))
+    T = transformer(float(g["lam"]), float(g["beta"]), g["tau"])
+    if not rotate:
+        sym = dict(sym)
+        sym["q_k_sym_transform_p0"] = np.stack([np.eye(36)] * 27)
+    return osy.gw_sigma_star(g["G_tau"], Vp, T.T_bt, T.T_tb, sym, 27, 2, 2, 2, 36, (3, 3, 3), return_Pw=True)
+
+
+def test_gw_sym27_fixture_star_of_q():
+    """The reference's own algorithm on the reduced mesh: 4 irreducible q, 4 irreducible k."""
+    g = np.load(os.path.join(GOLDEN, "gw_sym27.npz"), allow_pickle=False)
+    S, _ = _star(_sym(), g)
+    err = np.abs(S - g["Sigma_tau"]).max()
+    assert err < 1e-10     # reached: 2.6e-11 (the full-mesh evaluation: 8.9e-9)
+
+
+def test_gw_sym27_star_without_rotation_is_rejected():
+    g = np.load(os.path.join(GOLDEN, "gw_sym27.npz"), allow_pickle=False)
+    S, _ = _star(_sym(), g, rotate=False)
+    assert np.abs(S - g["Sigma_tau"]).max() > 1e-3
+
+
+def test_q_stars_partition_the_mesh():
+    z = _sym()
+    stars = [osy.q_star(z["q_bz2ibz"], z["q_ibz2bz"], iq) for iq in range(int(z["inq"]))]
+    assert sorted(q for st in stars for q in st) == list(range(27))
+    assert [len(st) for st in stars] == [1, 6, 12, 8]      # input.h5:/symmetry/q/stars
+    assert stars[1] == [4, 8, 10, 12, 20, 24]
+    for iq, st in enumerate(stars):
+        assert int(z["q_ibz2bz"][iq]) in st
+
+
+def test_p_bz_from_ibz_closed_forms():
+    rng = np.random.default_rng(1)
+    x = rng.standard_normal((5, 5)) + 1j * rng.standard_normal((5, 5))
+    x = x + np.conj(x.T)
+    assert np.array_equal(osy.p_bz_from_ibz(x, np.eye(5), 1), np.conj(x))
+    u, _ = np.linalg.qr(rng.standard_normal((5, 5)) + 1j * rng.standard_normal((5, 5)))
+    y = osy.p_bz_from_ibz(x, u, 0)
+    assert np.allclose(np.linalg.eigvalsh(y), np.linalg.eigvalsh(x), atol=1e-12)
 
 
 def test_planted_pair_operation_swap_is_rejected(monkeypatch):
@@ -199,6 +248,19 @@ def test_gw_c_fixture_with_extrapolation():
     err = np.abs(S - ref).max()
     assert err < 1e-6      # reference tolerance
     assert err < 1e-10     # reached: 2.8e-11
+
+
+def test_gw_c_fixture_star_of_q_with_extrapolation():
+    """GW_C through the star-of-q evaluation: 14 irreducible q, P-tilde conjugated on the
+    time-reversal partners; eps from the irreducible q's P-tilde(i Omega) directly."""
+    from oracle import q0
+    c = np.load(os.path.join(GOLDEN, "gwc_sym27.npz"), allow_pickle=False)
+    sym = {k: c[k] for k in c.files}
+    S, Pw = _star(sym, c)
+    eps = q0.eps_inv_wq(Pw, c["AqQ"], np.arange(len(c["q_ibz2bz"])))
+    et = q0.eps_q0_tau(eps, c["q_abs"], transformer(float(c["lam"]), float(c["beta"]), c["tau"]).T_tb)
+    S = q0.apply_q0_correction(S, c["G_tau"], osy.full_to_ibz(c["S_k"], c["k_ibz2bz"]), et, float(c["q0_madelung"]))
+    assert np.abs(S - c["Sigma_tau"]).max() < 1e-10     # reached: 2.8e-11
 
 
 @pytest.mark.parametrize("plant", ["no_correction", "imag_part"])
