@@ -83,8 +83,9 @@ namespace green::mbpt::hip {
     }
 
     // Integrals (every stored chunk, expanded to all k pairs, df_integral_t.h:40-187) and, on a
-    // symmetry-reduced mesh, the irreducible-wedge tables of the input file; then the reference's
-    // own k +- q maps.
+    // symmetry-reduced mesh, the k and q symmetry tables of the input file (so that
+    // gwhip_eval_sigma_ibz runs the reference's star-of-q evaluation); then the reference's own
+    // k +- q maps.
     inline void load_files(gwhip_handle* h, const bz_utils_t& bz, const std::string& df_path,
                            const std::string& input_file) {
       check(gwhip_h5_load(h, df_path.c_str(), input_file.c_str(), bz.ink() != bz.nk() ? 1 : 0), nullptr);

@@ -333,6 +333,26 @@ int gwhip_h5_read_k_symmetry(const char* input_h5, int64_t nk, int64_t nso, int6
   });
 }
 
+int gwhip_h5_read_q_symmetry(const char* input_h5, int64_t nq, int64_t NQ, int64_t* inq, int64_t* bz2ibz,
+                             int64_t* ibz2bz, double* U, int64_t* tr_conj) {
+  if (!input_h5 || !inq || !bz2ibz || !ibz2bz || !U || !tr_conj) return GWHIP_ERR_ARG;
+  return guarded([&] {
+    Id f(open_file(input_h5), H5Fclose);
+    const std::string q = "symmetry/q/";
+    *inq = read_scalar_i64(f.get(), q + "inq");
+    if (*inq < 1 || *inq > nq) throw h5_error(GWHIP_ERR_IO, "inq out of range");
+    const auto b = read_i64(f.get(), q + "bz2ibz");
+    const auto i = read_i64(f.get(), q + "ibz2bz");
+    const auto t = read_i64(f.get(), q + "tr_conj");
+    if ((int64_t)b.size() != nq || (int64_t)t.size() != nq || (int64_t)i.size() != *inq)
+      throw h5_error(GWHIP_ERR_IO, "symmetry/q tables do not match nq = " + std::to_string(nq));
+    std::copy(b.begin(), b.end(), bz2ibz);
+    std::copy(i.begin(), i.end(), ibz2bz);
+    std::copy(t.begin(), t.end(), tr_conj);
+    read_complex(f.get(), q + "k_sym_transform_p0", reinterpret_cast<std::complex<double>*>(U), (size_t)(nq * NQ * NQ));
+  });
+}
+
 int gwhip_h5_kq_maps(const double* mesh_scaled, int64_t nk, int32_t* kplus, int32_t* kminus) {
   if (!mesh_scaled || !kplus || !kminus || nk <= 0) return GWHIP_ERR_ARG;
   return guarded([&] { kq_maps(mesh_scaled, nk, kplus, kminus); });
@@ -366,8 +386,20 @@ int gwhip_h5_load(gwhip_handle* h, const char* df_path, const char* input_h5, in
     std::vector<std::complex<double>> V((size_t)(d.nk * (d.nk + 1) / 2 * d.NQ * d.nao * d.nao));
     read_packed(df_path, input_h5, d.nk, d.nao, d.NQ, V.data());
     call(gwhip_upload_V(h, reinterpret_cast<const double*>(V.data())));
-    if (with_symmetry)
+    if (with_symmetry) {
       call(gwhip_set_k_symmetry(h, ink, b.data(), i.data(), reinterpret_cast<const double*>(U.data()), t.data()));
+      if (has_path(f.get(), "symmetry/q/k_sym_transform_p0")) {  // the star-of-q evaluation
+        int64_t inq = 0;
+        std::vector<int64_t> qb(d.nk), qi(d.nk), qt(d.nk);
+        std::vector<std::complex<double>> Uq((size_t)(d.nk * d.NQ * d.NQ));
+        st = gwhip_h5_read_q_symmetry(input_h5, d.nk, d.NQ, &inq, qb.data(), qi.data(),
+                                      reinterpret_cast<double*>(Uq.data()), qt.data());
+        if (st != GWHIP_OK) throw h5_error(st, g_err);
+        call(gwhip_set_q_symmetry(h, inq, qb.data(), qi.data(), reinterpret_cast<const double*>(Uq.data()), qt.data()));
+      } else {
+        call(gwhip_set_q_symmetry(h, 0, nullptr, nullptr, nullptr, nullptr));
+      }
+    }
   });
 }
 

@@ -126,7 +126,8 @@ gw_hip_kernel::~gw_hip_kernel() {
   if (stream_) (void)hipStreamSynchronize(stream_);
   for (void* p : {(void*)Vq_, (void*)Vt_, (void*)P_, (void*)Tbt_fold_, (void*)Ttb_, (void*)G_dev_, (void*)S_dev_,
                   (void*)kplus_tab_, (void*)kminus_tab_, (void*)flag_, (void*)hcheck_, (void*)hf_work_, (void*)hf_x_, (void*)ksym_idx_, (void*)ksym_U_, (void*)hf_sym_, (void*)q0_qiq_, (void*)q0_A_, (void*)q0_eps_dev_,
-                  (void*)q0_S_, (void*)q0_tmp_, (void*)q0_epst_})
+                  (void*)q0_S_, (void*)q0_tmp_, (void*)q0_epst_, (void*)qsym_U_, (void*)star_P_, (void*)star_V_,
+                  (void*)star_Vs_, (void*)star_eVs_, (void*)star_km_, (void*)star_q0row_})
     if (p) (void)hipFree(p);
   if (own_ws_)
     for (void* p : {(void*)Pw_, (void*)X_, (void*)Y_, (void*)scratch_})
@@ -170,6 +171,8 @@ void gw_hip_kernel::upload_maps(const std::vector<int>& kpq, const std::vector<i
     for (int k = 0; k < nk; ++k) kplus_host_[(size_t)k * nk + q] = kpq[(size_t)q * nk + k];
   GWHIP_CHECK(hipMemcpy(kplus_tab_, kpq.data(), kpq.size() * sizeof(int), hipMemcpyHostToDevice));
   GWHIP_CHECK(hipMemcpy(kminus_tab_, kmq.data(), kmq.size() * sizeof(int), hipMemcpyHostToDevice));
+  kminus_host_ = kmq;
+  star_dirty_ = true;
 }
 
 void gw_hip_kernel::set_kq_maps(const int32_t* kplus, const int32_t* kminus) {
@@ -453,19 +456,19 @@ void launch_z_prep_slice(int ns, dim3 grid, size_t lds, hipStream_t st, const De
 }
 template <int NB, typename T>
 void launch_sigma_epi_t(dim3 grid, size_t lds, hipStream_t st, const DevDims& dd, const double2* G, const double2* Vt,
-                        const double2* R, double2* S, const int* kminus, int t0, double scale, int upper,
+                        const double2* R, double2* S, const int* kminus, int nks, int t0, double scale, int upper,
                         int mirror) {
   allow_lds(sigma_epi_mfma_kernel<NB, T>, lds);
   const XcdGrid xg = make_xcd_grid(grid);
   hipLaunchKernelGGL((sigma_epi_mfma_kernel<NB, T>), xcd_launch_grid(xg), dim3(256), lds, st, xg, dd, G, Vt, R, S,
-                     kminus, t0, scale, upper, mirror);
+                     kminus, nks, t0, scale, upper, mirror);
   GWHIP_CHECK(hipGetLastError());
 }
 template <int NB>
 void launch_sigma_epi(dim3 grid, size_t lds, hipStream_t st, const DevDims& dd, const double2* G,
-                      const double2* Vt, const double2* R, double2* S, const int* kminus, int t0, double scale,
-                      int upper, int mirror) {
-  launch_sigma_epi_t<NB, double>(grid, lds, st, dd, G, Vt, R, S, kminus, t0, scale, upper, mirror);
+                      const double2* Vt, const double2* R, double2* S, const int* kminus, int nks, int t0,
+                      double scale, int upper, int mirror) {
+  launch_sigma_epi_t<NB, double>(grid, lds, st, dd, G, Vt, R, S, kminus, nks, t0, scale, upper, mirror);
 }
 }  // namespace
 
@@ -519,14 +522,15 @@ void gw_hip_kernel::p0_gemm(int64_t q, int64_t tc, double2* Pt0, int shape) {
   zgemm_f64(a, true, true, stream_, shape);
 }
 
-// R = P-tilde(t0 .. t0+tc) conj(V(k, k - q)) into scratch_, FP64 (tile `shape`).
-void gw_hip_kernel::sigma_gemm(int64_t q, int64_t t0, int64_t tc, int shape) {
-  const int64_t nk = d_.nk, NQ = d_.NQ, nao2 = d_.nao * d_.nao, NQ2 = NQ * NQ, nts = d_.nts;
+// R = P-tilde(t0 .. t0+tc) conj(V) into scratch_, FP64 (tile `shape`): Pt0 = P-tilde at t0
+// [tc][NQ^2], Vb = the per-Q transposed V(k, k - q) of nkb k points [nkb][NQ][nao^2].
+void gw_hip_kernel::sigma_gemm(const double2* Pt0, const double2* Vb, int64_t nkb, int64_t tc, int shape) {
+  const int64_t NQ = d_.NQ, nao2 = d_.nao * d_.nao, NQ2 = NQ * NQ;
   ZGemmArgs a{};
-  a.M = (int)NQ; a.N = (int)(nk * nao2); a.K = (int)NQ; a.batch = (int)tc; a.kin = (int)NQ; a.nin = (int)nao2;
-  a.A = P_ + (q * nts + t0) * NQ2; a.sAb = NQ2; a.sAm = NQ; a.sAko = 0; a.sAki = 1;
-  a.B = Vt_ + (int64_t)neg_q_[q] * nk * NQ * nao2; a.sBb = 0; a.sBko = 0; a.sBki = nao2; a.sBno = NQ * nao2; a.sBni = 1;
-  a.C = scratch_; a.sCb = nk * NQ * nao2; a.sCm = nao2; a.sCno = NQ * nao2; a.sCni = 1;
+  a.M = (int)NQ; a.N = (int)(nkb * nao2); a.K = (int)NQ; a.batch = (int)tc; a.kin = (int)NQ; a.nin = (int)nao2;
+  a.A = Pt0; a.sAb = NQ2; a.sAm = NQ; a.sAko = 0; a.sAki = 1;
+  a.B = Vb; a.sBb = 0; a.sBko = 0; a.sBki = nao2; a.sBno = NQ * nao2; a.sBni = 1;
+  a.C = scratch_; a.sCb = nkb * NQ * nao2; a.sCm = nao2; a.sCno = NQ * nao2; a.sCni = 1;
   a.alpha = make_double2(1.0, 0.0);
   a.accumulate = 0; a.conjA = 0; a.conjB = 1;
   zgemm_f64(a, true, false, stream_, shape);
@@ -562,29 +566,30 @@ void gw_hip_kernel::p0_gemm_split(int64_t q, int64_t tc, double2* Pt0, int group
   zgemm_ozaki(o, ns, stream_);
 }
 
-// Split precision, Sigma: int8 slices of P-tilde(q, t0 .. t0+tc) into Ps_.
-void gw_hip_kernel::sigma_slice(int64_t q, int64_t t0, int64_t tc) {
-  const int64_t NQ = d_.NQ, NQ2 = NQ * NQ, nts = d_.nts;
+// Split precision, Sigma: int8 slices of P-tilde(t0 .. t0+tc) (Pt0: [tc][NQ^2]) into Ps_.
+void gw_hip_kernel::sigma_slice(const double2* Pt0, int64_t tc) {
+  const int64_t NQ = d_.NQ;
   SliceArgs sa{};
-  sa.src = P_ + (q * nts + t0) * NQ2; sa.nrows = tc * NQ; sa.row_div = tc * NQ; sa.s_rhi = 0; sa.s_rlo = NQ;
+  sa.src = Pt0; sa.nrows = tc * NQ; sa.row_div = tc * NQ; sa.s_rhi = 0; sa.s_rlo = NQ;
   sa.nchunk = 1; sa.s_c = 0; sa.klen = (int)NQ; sa.s_k = 1; sa.kc = (int)kc_s_;
   sa.conj = 0; sa.nplanes = 2; sa.dst = Ps_; sa.exps = ePs_;
   slice_rows(sa, split_ns_, stream_);
 }
 
-// Split precision, R = P-tilde conj(V(k, k - q)) into scratch_ from the slices.
-void gw_hip_kernel::sigma_gemm_split(int64_t q, int64_t tc, int group, int tile) {
-  const int64_t nk = d_.nk, NQ = d_.NQ, nao2 = d_.nao * d_.nao;
+// Split precision, R = P-tilde conj(V) into scratch_ from the slices: Vsb / eVsb = the slices and
+// exponents of V(k, k - q) for nkb k points ([nkb][nao^2][2][ns][kc_s], [nkb][nao^2]).
+void gw_hip_kernel::sigma_gemm_split(const int8_t* Vsb, const int* eVsb, int64_t nkb, int64_t tc, int group,
+                                     int tile) {
+  const int64_t NQ = d_.NQ, nao2 = d_.nao * d_.nao;
   const int ns = split_ns_;
-  const int qn = neg_q_[q];
   OzGemmArgs o{};
-  o.M = (int)NQ; o.N = (int)(nk * nao2); o.batch = (int)tc; o.kc = (int)kc_s_; o.nchunk = 1; o.nin = (int)nao2;
+  o.M = (int)NQ; o.N = (int)(nkb * nao2); o.batch = (int)tc; o.kc = (int)kc_s_; o.nchunk = 1; o.nin = (int)nao2;
   o.A = Ps_; o.sAm = 2 * ns * kc_s_; o.sAb = NQ * o.sAm; o.sAc = 0;
   o.eA = ePs_; o.seAb = NQ; o.seAm = 1;
-  o.B = Vs_s_ + (int64_t)qn * nk * nao2 * 2 * ns * kc_s_; o.sBb = 0; o.sBno = nao2 * 2 * ns * kc_s_;
+  o.B = Vsb; o.sBb = 0; o.sBno = nao2 * 2 * ns * kc_s_;
   o.sBni = 2 * ns * kc_s_; o.sBc = 0;
-  o.eB = eVs_s_ + (int64_t)qn * nk * nao2; o.seBb = 0; o.seBno = nao2; o.seBni = 1;
-  o.C = scratch_; o.sCb = nk * NQ * nao2; o.sCm = nao2; o.sCno = NQ * nao2; o.sCni = 1;
+  o.eB = eVsb; o.seBb = 0; o.seBno = nao2; o.seBni = 1;
+  o.C = scratch_; o.sCb = nkb * NQ * nao2; o.sCm = nao2; o.sCno = NQ * nao2; o.sCni = 1;
   o.alpha = make_double2(1.0, 0.0);
   o.accumulate = 0;
   o.group_m = group;
@@ -592,8 +597,10 @@ void gw_hip_kernel::sigma_gemm_split(int64_t q, int64_t tc, int group, int tile)
   zgemm_ozaki(o, ns, stream_);
 }
 
-void gw_hip_kernel::stage_p0(const double2* G, bool split) {
-  for (int64_t q = 0; q < d_.nk; ++q) p0_one_q(q, G, split, P_ + q * d_.nts * d_.NQ * d_.NQ);
+// P0 of every q in `qs` (all q of the mesh when null): the i-th into P_ row i.
+void gw_hip_kernel::stage_p0(const double2* G, bool split, const std::vector<int>* qs) {
+  const int64_t n = qs ? (int64_t)qs->size() : d_.nk;
+  for (int64_t i = 0; i < n; ++i) p0_one_q(qs ? (*qs)[i] : i, G, split, P_ + i * d_.nts * d_.NQ * d_.NQ);
 }
 
 // P0(tau, q) for the half tau grid into Pq[t][Q][Q'] (t < nts/2), Hermitised.
@@ -640,8 +647,10 @@ void gw_hip_kernel::gj_inverse(double2* X, int64_t nmat) {
   GWHIP_CHECK(hipGetLastError());
 }
 
-void gw_hip_kernel::stage_dyson() {
-  const int64_t nk = d_.nk, NQ = d_.NQ, NQ2 = NQ * NQ, half = d_.nts / 2, nwb = d_.nw_b, nts = d_.nts;
+// P-tilde of the first nq rows of P_; q0_row[i] is the AqQ row of row i (-1: none), read when the
+// q -> 0 data are being collected.
+void gw_hip_kernel::stage_dyson(int64_t nq, const int* q0_row) {
+  const int64_t NQ = d_.NQ, NQ2 = NQ * NQ, half = d_.nts / 2, nwb = d_.nw_b, nts = d_.nts;
 
   hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
   if (timing_) {
@@ -650,8 +659,8 @@ void gw_hip_kernel::stage_dyson() {
   }
   auto mark = [&](hipEvent_t e) { if (timing_) GWHIP_CHECK(hipEventRecord(e, stream_)); };
   float tw = 0.f, bse = 0.f, wt = 0.f;
-  for (int64_t q0 = 0; q0 < nk; q0 += qchunk_) {
-    const int64_t qc = std::min(qchunk_, nk - q0);
+  for (int64_t q0 = 0; q0 < nq; q0 += qchunk_) {
+    const int64_t qc = std::min(qchunk_, nq - q0);
     mark(e0);
     {  // P0(tau_f) -> P0(i Omega_b) with the folded matrix
       ZGemmArgs a{};
@@ -681,7 +690,7 @@ void gw_hip_kernel::stage_dyson() {
       zgemm_f64(a, true, false, stream_);
     }
     if (q0_eps_) {  // aux_to_PW_00 for the irreducible q of this chunk
-      hipLaunchKernelGGL(q0_eps_kernel, dim3((unsigned)nwb, (unsigned)qc), dim3(256), 0, stream_, Y_, q0_A_, q0_qiq_,
+      hipLaunchKernelGGL(q0_eps_kernel, dim3((unsigned)nwb, (unsigned)qc), dim3(256), 0, stream_, Y_, q0_A_, q0_row,
                          (int)q0, (int)nwb, (int)NQ, (int)q0_inq_, q0_eps_dev_);
       GWHIP_CHECK(hipGetLastError());
     }
@@ -712,35 +721,55 @@ void gw_hip_kernel::stage_dyson() {
 }
 
 void gw_hip_kernel::stage_sigma(const double2* G, double2* Sigma, bool split) {
-  const int64_t nk = d_.nk, NQ = d_.NQ, nao2 = d_.nao * d_.nao, NQ2 = NQ * NQ, nts = d_.nts;
+  for (int64_t q = 0; q < d_.nk; ++q)
+    sigma_q(G, Sigma, split, P_ + q * d_.nts * d_.NQ * d_.NQ, Vt_neg(q), Vs_neg(q), eVs_neg(q), kminus_tab_ + q * d_.nk,
+            d_.nk);
+}
+
+// V(k, k - q) lives in the k + q' store of q' = neg_q_[q]: per-Q transposed, and its slices.
+const double2* gw_hip_kernel::Vt_neg(int64_t q) const {
+  return Vt_ + (int64_t)neg_q_[q] * d_.nk * d_.NQ * d_.nao * d_.nao;
+}
+const int8_t* gw_hip_kernel::Vs_neg(int64_t q) const {
+  return Vs_s_ ? Vs_s_ + (int64_t)neg_q_[q] * d_.nk * d_.nao * d_.nao * 2 * split_ns_ * kc_s_ : nullptr;
+}
+const int* gw_hip_kernel::eVs_neg(int64_t q) const {
+  return eVs_s_ ? eVs_s_ + (int64_t)neg_q_[q] * d_.nk * d_.nao * d_.nao : nullptr;
+}
+
+// Sigma += -1/nk sum over the nkb k points of one q: Pq = P-tilde of this q ([nts][NQ^2], the half
+// grid read), Vb / Vsb / eVsb = V(k, k - q) of the nkb points (per-Q transposed; slices and
+// exponents for split precision), km = the full-mesh index of k - q for each, Sigma
+// [nts][ns][nkb][nso^2].
+void gw_hip_kernel::sigma_q(const double2* G, double2* Sigma, bool split, const double2* Pq, const double2* Vb,
+                            const int8_t* Vsb, const int* eVsb, const int* km, int64_t nkb) {
+  const int64_t nk = d_.nk, NQ = d_.NQ, NQ2 = NQ * NQ, nts = d_.nts;
   DevDims dd{(int)nts, (int)d_.ns, (int)nk, (int)d_.nso, (int)d_.nao, (int)NQ, (int)d_.pns, (int)d_.pseudo_ns};
   const int NB = (int)ceil_div(d_.nao, 16);
   const size_t lds = ((size_t)2 * (16 * NB) * (16 * NB + 1) + (size_t)NB * NB * 8 * 64) * sizeof(double);
   const double scale = -1.0 / (double)nk;
+  if (nkb < 1 || nkb > nk) throw status_error(GWHIP_ERR_ARG, "Sigma stage k count out of range");
 
   // P0(beta - tau) = P0(tau) makes P-tilde(beta - tau) = P-tilde(tau) (to the rounding of the
   // tau <-> Omega transforms, ~1e-12 relative on the IR grids), so R = P-tilde conj(V) is formed on
   // the half grid only and serves both tau and beta - tau in the epilogue.
   const int64_t half = nts / 2;
-  for (int64_t q = 0; q < nk; ++q) {
-    const double2* Vneg = Vt_ + (int64_t)neg_q_[q] * nk * NQ * nao2;  // V(k, k - q), per-Q transposed
-    for (int64_t t0 = 0; t0 < half; t0 += tchunk_sigma_) {
-      const int64_t tc = std::min(tchunk_sigma_, half - t0);
-      if (split) {
-        sigma_slice(q, t0, tc);
-        timed(K_SIGMA_GEMM, [&] { sigma_gemm_split(q, tc, tune_.s_group, tune_.s_oz_tile); });
-      } else {
-        timed(K_SIGMA_GEMM, [&] { sigma_gemm(q, t0, tc, tune_.s_shape); });
-      }
-      timed(K_SIGMA_EPI, [&] {
-        const dim3 grid((unsigned)nk, 1u, (unsigned)(2 * tc));
-        const int* km = kminus_tab_ + q * nk;
-        const int up = sigma_upper() ? 1 : 0;
-        if (NB == 1) launch_sigma_epi<1>(grid, lds, stream_, dd, G, Vneg, scratch_, Sigma, km, (int)t0, scale, up, 1);
-        else if (NB == 2) launch_sigma_epi<2>(grid, lds, stream_, dd, G, Vneg, scratch_, Sigma, km, (int)t0, scale, up, 1);
-        else launch_sigma_epi<3>(grid, lds, stream_, dd, G, Vneg, scratch_, Sigma, km, (int)t0, scale, up, 1);
-      });
+  for (int64_t t0 = 0; t0 < half; t0 += tchunk_sigma_) {
+    const int64_t tc = std::min(tchunk_sigma_, half - t0);
+    if (split) {
+      sigma_slice(Pq + t0 * NQ2, tc);
+      timed(K_SIGMA_GEMM, [&] { sigma_gemm_split(Vsb, eVsb, nkb, tc, tune_.s_group, tune_.s_oz_tile); });
+    } else {
+      timed(K_SIGMA_GEMM, [&] { sigma_gemm(Pq + t0 * NQ2, Vb, nkb, tc, tune_.s_shape); });
     }
+    timed(K_SIGMA_EPI, [&] {
+      const dim3 grid((unsigned)nkb, 1u, (unsigned)(2 * tc));
+      const int up = sigma_upper() ? 1 : 0;
+      const int n = (int)nkb, t = (int)t0;
+      if (NB == 1) launch_sigma_epi<1>(grid, lds, stream_, dd, G, Vb, scratch_, Sigma, km, n, t, scale, up, 1);
+      else if (NB == 2) launch_sigma_epi<2>(grid, lds, stream_, dd, G, Vb, scratch_, Sigma, km, n, t, scale, up, 1);
+      else launch_sigma_epi<3>(grid, lds, stream_, dd, G, Vb, scratch_, Sigma, km, n, t, scale, up, 1);
+    });
   }
 }
 
@@ -748,9 +777,9 @@ void gw_hip_kernel::stage_sigma(const double2* G, double2* Sigma, bool split) {
 // blocks were accumulated, so fill the rest from them.
 bool gw_hip_kernel::sigma_upper() const { return g_herm_ && d_.pns == 1; }
 
-void gw_hip_kernel::finish_sigma(double2* Sigma) {
+void gw_hip_kernel::finish_sigma(double2* Sigma, int64_t nkS) {
   if (!sigma_upper()) return;
-  const int64_t n = d_.nso, nmat = d_.nts * d_.ns * d_.nk;
+  const int64_t n = d_.nso, nmat = d_.nts * d_.ns * nkS;
   for (int64_t m0 = 0; m0 < nmat; m0 += 65535) {  // grid.y limit
     const int64_t mc = std::min<int64_t>(65535, nmat - m0);
     hipLaunchKernelGGL(mirror_upper_kernel, dim3((unsigned)ceil_div(n * n, 256), (unsigned)mc), dim3(256), 0, stream_,
@@ -764,9 +793,9 @@ void gw_hip_kernel::enqueue_eval(const double2* G, double2* Sigma, bool split, i
   const int64_t nsig = d_.nts * d_.ns * d_.nk * d_.nso * d_.nso;
   zero(Sigma, nsig);
   stage_p0(G, split);
-  stage_dyson();
+  stage_dyson(d_.nk, q0_qiq_);
   stage_sigma(G, Sigma, split);
-  finish_sigma(Sigma);
+  finish_sigma(Sigma, d_.nk);
   if (nonfinite_flag) {
     hipLaunchKernelGGL(nonfinite_kernel, dim3(1024), dim3(256), 0, stream_, Sigma, nsig, nonfinite_flag);
     GWHIP_CHECK(hipGetLastError());
@@ -800,11 +829,11 @@ void gw_hip_kernel::solve_device(const double2* G, double2* Sigma, int precision
   }
   GWHIP_CHECK(hipEventRecord(t1, stream_));
   q0_eps_ = q0_on_;
-  stage_dyson();
+  stage_dyson(d_.nk, q0_qiq_);
   q0_eps_ = false;
   GWHIP_CHECK(hipEventRecord(t2, stream_));
   stage_sigma(G, Sigma, split);
-  finish_sigma(Sigma);
+  finish_sigma(Sigma, d_.nk);
   if (q0_on_ && q0_apply) apply_q0(Sigma, G, d_.nk);
   GWHIP_CHECK(hipEventRecord(t3, stream_));
   hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, stream_, flag_, (int64_t)1);
@@ -865,7 +894,7 @@ void gw_hip_kernel::autotune(const double2* G, int precision) {
     const int shapes[3] = {0, 1, 2};
     p0_prep(0, G, 0, tcp, tune_.qpw);
     tune_.p0_shape = best_of(3, shapes, [&](int c) { p0_gemm(0, tcp, P_, c); });
-    tune_.s_shape = best_of(3, shapes, [&](int c) { sigma_gemm(0, 0, tcs, c); });
+    tune_.s_shape = best_of(3, shapes, [&](int c) { sigma_gemm(P_, Vt_neg(0), d_.nk, tcs, c); });
   } else {
     if (d_.pseudo_ns == 1) {  // fused prep + slicing vs the two kernels
       const int fz[2] = {0, 1};
@@ -886,8 +915,9 @@ void gw_hip_kernel::autotune(const double2* G, int precision) {
     const int bp = best_of(21, cands, [&](int c) { p0_gemm_split(0, tcp, P_, c % 100, c / 100); });
     tune_.p0_oz_tile = bp / 100;
     tune_.p0_group = bp % 100;
-    sigma_slice(0, 0, tcs);
-    const int bs = best_of(21, cands, [&](int c) { sigma_gemm_split(0, tcs, c % 100, c / 100); });
+    sigma_slice(P_, tcs);
+    const int bs =
+        best_of(21, cands, [&](int c) { sigma_gemm_split(Vs_neg(0), eVs_neg(0), d_.nk, tcs, c % 100, c / 100); });
     tune_.s_oz_tile = bs / 100;
     tune_.s_group = bs % 100;
   }
@@ -919,7 +949,7 @@ void gw_hip_kernel::run_stages_host(const double2* G, int last_stage) {
   detect_hermitian_G(G_dev_);
   GWHIP_CHECK(hipMemsetAsync(P_, 0, (size_t)d_.nk * d_.nts * d_.NQ * d_.NQ * sizeof(double2), stream_));
   stage_p0(G_dev_, false);
-  if (last_stage >= 2) stage_dyson();
+  if (last_stage >= 2) stage_dyson(d_.nk, q0_qiq_);
   GWHIP_CHECK(hipStreamSynchronize(stream_));
   resolve_timings();
 }
@@ -1414,6 +1444,12 @@ int gwhip_set_k_symmetry(gwhip_handle* h, int64_t ink, const int64_t* bz2ibz, co
                          const int64_t* tr_conj) {
   if (!h) return GWHIP_ERR_ARG;
   return guarded(h, [&] { h->k->set_k_symmetry(ink, bz2ibz, ibz2bz, (const double2*)U, tr_conj); });
+}
+
+int gwhip_set_q_symmetry(gwhip_handle* h, int64_t inq, const int64_t* bz2ibz, const int64_t* ibz2bz, const double* U,
+                         const int64_t* tr_conj) {
+  if (!h) return GWHIP_ERR_ARG;
+  return guarded(h, [&] { h->k->set_q_symmetry(inq, bz2ibz, ibz2bz, (const double2*)U, tr_conj); });
 }
 
 int gwhip_eval_sigma_ibz(gwhip_handle* h, const double* G_ibz, double* Sigma_ibz, int precision) {

@@ -112,6 +112,13 @@ public:
   void set_k_symmetry(int64_t ink, const int64_t* bz2ibz, const int64_t* ibz2bz, const double2* U,
                       const int64_t* tr_conj);
   int64_t ink() const { return ink_; }
+  // q symmetry (green-symmetry's q_symmetry(), input.h5:/symmetry/q): inq irreducible q, bz2ibz
+  // [nq], ibz2bz [inq], the auxiliary-basis matrices U [nq][NQ][NQ] (k_sym_transform_p0) and
+  // tr_conj [nq].  With it (and the k symmetry) the irreducible entry runs the reference's
+  // star-of-q evaluation.  inq = 0 switches it off.
+  void set_q_symmetry(int64_t inq, const int64_t* bz2ibz, const int64_t* ibz2bz, const double2* U,
+                      const int64_t* tr_conj);
+  int64_t inq() const { return inq_; }
   // q -> 0 extrapolation (q0_treatment EXTRAPOLATE): inq irreducible q (full-mesh indices q_ir),
   // AqQ [inq][NQ], q_abs [inq], the Madelung constant of AqQ.h5 and the overlap S [ns][nkS][nso^2]
   // with nkS = nk (full-mesh entry) or ink (irreducible entry).  inq = 0 switches it off.
@@ -232,9 +239,25 @@ private:
   double2* q0_epst_ = nullptr;      // [nts]
   int* ksym_idx_ = nullptr;
   double2* ksym_U_ = nullptr;
+  std::vector<int> ksym_i2b_;       // host copy of ibz2bz
+  // q symmetry and the star-of-q evaluation: irreducible q, stars, whether U_q is exactly the
+  // identity, U_q on the device, and per-evaluation buffers (rotated P-tilde, the gathered
+  // V(k, k - q) of the irreducible k and its slices, k - q of the irreducible k per q, AqQ rows)
+  int64_t inq_ = 0;
+  std::vector<int> qsym_i2b_, qsym_tr_, qsym_ident_, qs_q0row_host_;
+  std::vector<std::vector<int>> qstar_;
+  double2* qsym_U_ = nullptr;       // [nq][NQ^2]
+  double2* star_P_ = nullptr;       // [2][nts/2][NQ^2]: rotated P-tilde and U P-tilde
+  double2* star_V_ = nullptr;       // [ink][NQ][nao^2]
+  int8_t* star_Vs_ = nullptr;       // [ink][nao^2][2][ns][kc_s]
+  int* star_eVs_ = nullptr;         // [ink][nao^2]
+  int* star_km_ = nullptr;          // [nq][ink]
+  int* star_q0row_ = nullptr;       // [inq]
+  int64_t star_ink_ = 0, star_vs_bytes_ = 0;
+  bool star_dirty_ = true;
   double2* hf_sym_ = nullptr;
   int64_t hf_exchange_launches_ = 0;
-  std::vector<int> kplus_host_, neg_q_;
+  std::vector<int> kplus_host_, kminus_host_, neg_q_, q0_qiq_host_;
 
   double region_ms_[R_COUNT] = {0};
   KernelStat kstat_[K_COUNT];
@@ -256,18 +279,26 @@ private:
   void p0_prep(int64_t q, const double2* G, int64_t t0, int64_t tc, int qpw);
   void p0_prep_slice(int64_t q, const double2* G, int64_t t0, int64_t tc, int qpw);
   void p0_gemm(int64_t q, int64_t tc, double2* Pt0, int shape);
-  void sigma_gemm(int64_t q, int64_t t0, int64_t tc, int shape);
+  void sigma_gemm(const double2* Pt0, const double2* Vb, int64_t nkb, int64_t tc, int shape);
   void p0_slice(int64_t tc);
   void p0_gemm_split(int64_t q, int64_t tc, double2* Pt0, int group, int tile);
-  void sigma_slice(int64_t q, int64_t t0, int64_t tc);
-  void sigma_gemm_split(int64_t q, int64_t tc, int group, int tile);
-  void stage_p0(const double2* G, bool split);
+  void sigma_slice(const double2* Pt0, int64_t tc);
+  void sigma_gemm_split(const int8_t* Vsb, const int* eVsb, int64_t nkb, int64_t tc, int group, int tile);
+  void stage_p0(const double2* G, bool split, const std::vector<int>* qs = nullptr);
   void p0_one_q(int64_t q, const double2* G, bool split, double2* Pq);
   void gj_inverse(double2* X, int64_t nmat);
-  void stage_dyson();
+  void stage_dyson(int64_t nq, const int* q0_row);
   void stage_sigma(const double2* G, double2* Sigma, bool split);
+  void sigma_q(const double2* G, double2* Sigma, bool split, const double2* Pq, const double2* Vb, const int8_t* Vsb,
+               const int* eVsb, const int* km, int64_t nkb);
+  const double2* Vt_neg(int64_t q) const;
+  const int8_t* Vs_neg(int64_t q) const;
+  const int* eVs_neg(int64_t q) const;
   bool sigma_upper() const;
-  void finish_sigma(double2* Sigma);
+  void finish_sigma(double2* Sigma, int64_t nkS);
+  void star_tables();
+  const double2* star_rotate(int64_t iq, int64_t q);
+  void solve_star_device(const double2* G_ibz, double2* Sigma_ibz, int precision);
   template <typename F>
   void timed(int kstat, F&& launch);
   void resolve_timings();

@@ -328,16 +328,19 @@ __global__ __launch_bounds__(256) void z_prep_slice_kernel(XcdGrid xg, DevDims d
 // with V = V(k, k - q), G_s = G(tau, k - q) block (a, b)  (src/gw_cpu_kernel.cpp:359-373,
 // Y2 = Y1^T P done by the big GEMM as R = P-tilde conj(V)).  The first MFMA produces
 // T^T = G^T V^T so that its result registers feed the second MFMA's A operand.
-// grid: (nk, 1, tcount) in the XcdGrid order (tcount = 2 x the R rows when mirror is set); wave w
-// takes Q = w, w + 4, ...; the four partial sums meet in LDS.
+// grid: (nks, 1, tcount) in the XcdGrid order (tcount = 2 x the R rows when mirror is set); wave w
+// takes Q = w, w + 4, ...; the four partial sums meet in LDS.  Block x = k is the position in this
+// launch's k list: V (Vt[k]), R ([t][nks][Q][nao^2]) and Sigma ([nts][ns][nks][nso^2]) use it
+// directly, G (full mesh) is read at kminus[k].  The full mesh has nks = nk; the star-of-q
+// evaluation passes the irreducible k points only.
 // With T = float each Q's product (K <= 16 NB) is added to FP64 accumulators.
 template <int NB, typename T>
 __global__ __launch_bounds__(256) void sigma_epi_mfma_kernel(XcdGrid xg, DevDims d, const double2* __restrict__ G,
                                                              const double2* __restrict__ Vt,
                                                              const double2* __restrict__ R,
                                                              double2* __restrict__ Sigma,
-                                                             const int* __restrict__ kminus, int t0, double scale,
-                                                             int upper, int mirror) {
+                                                             const int* __restrict__ kminus, int nks, int t0,
+                                                             double scale, int upper, int mirror) {
   typedef Mfma<T> M;
   typedef typename M::acc_t acc_t;
   constexpr bool F64 = sizeof(T) == 8;
@@ -385,7 +388,7 @@ __global__ __launch_bounds__(256) void sigma_epi_mfma_kernel(XcdGrid xg, DevDims
 
     for (int Q = wave; Q < d.NQ; Q += 4) {
       const double2* vt = Vt + ((int64_t)k * d.NQ + Q) * nao2;                   // vt[m * nao + i] = V[i][m]
-      const double2* rq = R + (((int64_t)tz * d.nk + k) * d.NQ + Q) * nao2;     // rq[n * nao + j]
+      const double2* rq = R + (((int64_t)tz * nks + k) * d.NQ + Q) * nao2;      // rq[n * nao + j]
 #pragma unroll
       for (int bi = 0; bi < NB; ++bi) {
         acc_t tr[NB], ti[NB];
@@ -484,7 +487,7 @@ __global__ __launch_bounds__(256) void sigma_epi_mfma_kernel(XcdGrid xg, DevDims
       }
     }
     if (wave == 0) {
-      double2* S = Sigma + g_index(d, tau, is, k);
+      double2* S = Sigma + (((int64_t)tau * d.ns + is) * nks + k) * d.nso * d.nso;
 #pragma unroll
       for (int x = 0; x < NB; ++x)
 #pragma unroll

@@ -93,6 +93,7 @@ void gw_hip_kernel::set_q0_extrapolation(int64_t inq, const int64_t* q_ir, const
                                          double madelung, int64_t nkS, const double2* S) {
   GWHIP_CHECK(hipSetDevice(device_));
   GWHIP_CHECK(hipStreamSynchronize(stream_));
+  star_dirty_ = true;
   if (inq == 0) {  // disable
     q0_on_ = false;
     return;
@@ -126,6 +127,7 @@ void gw_hip_kernel::set_q0_extrapolation(int64_t inq, const int64_t* q_ir, const
   q0_tmp_ = dalloc<double2>((size_t)d_.nts * d_.ns * nkS * nso * nso);
   if (!q0_epst_) q0_epst_ = dalloc<double2>((size_t)d_.nts);
   GWHIP_CHECK(hipMemcpy(q0_qiq_, qiq.data(), (size_t)nk * sizeof(int), hipMemcpyHostToDevice));
+  q0_qiq_host_ = qiq;
   GWHIP_CHECK(hipMemcpy(q0_A_, AqQ, (size_t)inq * NQ * sizeof(double2), hipMemcpyHostToDevice));
   GWHIP_CHECK(hipMemcpy(q0_S_, S, (size_t)d_.ns * nkS * nso * nso * sizeof(double2), hipMemcpyHostToDevice));
   q0_on_ = true;

@@ -133,6 +133,7 @@ def load() -> ctypes.CDLL:
     lib.gwhip_hf_ms.restype = ctypes.c_double
     lib.gwhip_hf_exchange_stats.argtypes = [P, ctypes.POINTER(ctypes.c_double), ctypes.POINTER(ctypes.c_int64)]
     lib.gwhip_set_k_symmetry.argtypes = [P, ctypes.c_int64, P, P, P, P]
+    lib.gwhip_set_q_symmetry.argtypes = [P, ctypes.c_int64, P, P, P, P]
     lib.gwhip_eval_sigma_ibz.argtypes = [P, P, P, ctypes.c_int]
     lib.gwhip_eval_sigma_ibz_device.argtypes = [P, P, P, ctypes.c_int]
     lib.gwhip_hf_ibz.argtypes = [P, P, P, ctypes.c_double, P]
@@ -158,7 +159,8 @@ def load() -> ctypes.CDLL:
               "gwhip_generate_V", "gwhip_eval_sigma", "gwhip_eval_sigma_device", "gwhip_get_P_tilde", "gwhip_timings",
               "gwhip_kernel_stats", "gwhip_set_vjp_store", "gwhip_vjp", "gwhip_vjp_device", "gwhip_autotune",
               "gwhip_get_tuning", "gwhip_set_tuning", "gwhip_batch_create", "gwhip_batch_autotune",
-              "gwhip_batch_eval_device", "gwhip_batch_stats", "gwhip_set_k_symmetry", "gwhip_eval_sigma_ibz",
+              "gwhip_batch_eval_device", "gwhip_batch_stats", "gwhip_set_k_symmetry", "gwhip_set_q_symmetry",
+              "gwhip_eval_sigma_ibz",
               "gwhip_eval_sigma_ibz_device", "gwhip_hf_ibz", "gwhip_hf_ibz_device", "gwhip_get_dims",
               "gwhip_set_q0_extrapolation", "gwhip_get_q0_eps", "gwhip_dyson_create", "gwhip_dyson_set_grid",
               "gwhip_dyson_set_system", "gwhip_dyson_compute_G", "gwhip_dyson_compute_G_device", "gwhip_dyson_find_mu",
@@ -198,6 +200,7 @@ def load_h5() -> ctypes.CDLL:
     h5.gwhip_h5_read_packed.argtypes = [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int64, ctypes.c_int64,
                                         ctypes.c_int64, P]
     h5.gwhip_h5_read_k_symmetry.argtypes = [ctypes.c_char_p, ctypes.c_int64, ctypes.c_int64, I64P, P, P, P, P, P]
+    h5.gwhip_h5_read_q_symmetry.argtypes = [ctypes.c_char_p, ctypes.c_int64, ctypes.c_int64, I64P, P, P, P, P]
     h5.gwhip_h5_kq_maps.argtypes = [P, ctypes.c_int64, P, P]
     h5.gwhip_h5_load.argtypes = [P, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int]
     h5.gwhip_h5_load_q0.argtypes = [P, ctypes.c_char_p, ctypes.c_char_p, ctypes.c_int64, P,
@@ -242,6 +245,18 @@ def h5_read_k_symmetry(input_h5: str, nk: int, nso: int) -> dict:
     _h5check(load_h5().gwhip_h5_read_k_symmetry(input_h5.encode(), nk, nso, ctypes.byref(ink), _ptr(b), _ptr(i),
                                                 _ptr(U), _ptr(t), _ptr(m)))
     return dict(ink=ink.value, bz2ibz=b, ibz2bz=i[:ink.value].copy(), U=U, tr_conj=t, mesh_scaled=m)
+
+
+def h5_read_q_symmetry(input_h5: str, nq: int, NQ: int) -> dict:
+    """/symmetry/q tables: inq, bz2ibz, ibz2bz, U (k_sym_transform_p0), tr_conj."""
+    inq = ctypes.c_int64()
+    b = np.empty(nq, np.int64)
+    i = np.empty(nq, np.int64)
+    t = np.empty(nq, np.int64)
+    U = np.empty((nq, NQ, NQ), np.complex128)
+    _h5check(load_h5().gwhip_h5_read_q_symmetry(input_h5.encode(), nq, NQ, ctypes.byref(inq), _ptr(b), _ptr(i),
+                                                _ptr(U), _ptr(t)))
+    return dict(inq=inq.value, bz2ibz=b, ibz2bz=i[:inq.value].copy(), U=U, tr_conj=t)
 
 
 def h5_kq_maps(mesh_scaled: np.ndarray):
@@ -428,8 +443,23 @@ class GwHipKernel:
         self._check(self.lib.gwhip_set_k_symmetry(self.h, ink, _ptr(b), _ptr(i), _ptr(u), _ptr(t)))
         self.dims["ink"] = ink
 
+    def set_q_symmetry(self, inq: int, bz2ibz=None, ibz2bz=None, U=None, tr_conj=None):
+        """q-symmetry tables (input.h5:/symmetry/q); U: [nq, NQ, NQ] complex.  With the k symmetry
+        set, solve_ibz then runs the star-of-q evaluation; inq = 0 switches it off."""
+        d = self.dims
+        if inq == 0:
+            self._check(self.lib.gwhip_set_q_symmetry(self.h, 0, None, None, None, None))
+            return
+        b = np.ascontiguousarray(bz2ibz, np.int64)
+        i = np.ascontiguousarray(ibz2bz, np.int64)
+        u = np.ascontiguousarray(U, np.complex128)
+        t = np.ascontiguousarray(tr_conj, np.int64)
+        assert b.shape == (d["nk"],) and i.shape == (inq,) and t.shape == (d["nk"],)
+        assert u.shape == (d["nk"], d["NQ"], d["NQ"])
+        self._check(self.lib.gwhip_set_q_symmetry(self.h, inq, _ptr(b), _ptr(i), _ptr(u), _ptr(t)))
+
     def load_files(self, df_path: str, input_h5: str, with_symmetry: bool = True):
-        """k +- q maps, integrals and (optionally) k symmetry from green-mbpt's files."""
+        """k +- q maps, integrals and (optionally) the k and q symmetry from green-mbpt's files."""
         st = load_h5().gwhip_h5_load(self.h, df_path.encode(), input_h5.encode(), 1 if with_symmetry else 0)
         _h5check(st)
         if with_symmetry:

@@ -210,9 +210,22 @@ double gwhip_hf_ms(const gwhip_handle* h);
 int gwhip_set_k_symmetry(gwhip_handle* h, int64_t ink, const int64_t* bz2ibz, const int64_t* ibz2bz, const double* U,
                          const int64_t* tr_conj);
 
+/* The q symmetry of green-symmetry's brillouin_zone_utils (input.h5:/symmetry/q; the q mesh is
+ * the k mesh): bz2ibz [nq], ibz2bz [inq] (the irreducible q), U [nq][NQ][NQ] complex
+ * k_sym_transform_p0 (auxiliary-basis matrices) and tr_conj [nq].  A q of the star of an
+ * irreducible q_ir is every q with bz2ibz[q] = q_ir, in ascending order.  P-tilde is carried from
+ * q_ir to q as U_q P-tilde U_q^H, conjugated when tr_conj[q] = 1 (eval_p0_bz_from_ibz,
+ * src/gw_cpu_kernel.cpp:281-293).  With both symmetries set, gwhip_eval_sigma_ibz runs the
+ * reference's star-of-q evaluation.  inq = 0 switches it off. */
+int gwhip_set_q_symmetry(gwhip_handle* h, int64_t inq, const int64_t* bz2ibz, const int64_t* ibz2bz, const double* U,
+                         const int64_t* tr_conj);
+
 /* Sigma on the irreducible k points from G on the irreducible k points: gw_cpu_kernel::solve with
- * symmetry (src/gw_cpu_kernel.cpp:31-134, the q-star loop at :113-130).  G is unfolded on the
- * device, the full-mesh kernel runs, and the representatives' Sigma is returned.  Host buffers. */
+ * symmetry (src/gw_cpu_kernel.cpp:31-134).  With the q symmetry set: the reference's scheme, P0
+ * and P-tilde for the irreducible q over all k (G carried to the full mesh on the device), Sigma
+ * for the irreducible k summed over the star of each irreducible q (:113-130).  Without it: the
+ * full-mesh kernel runs on the unfolded G and the representatives' Sigma is returned (equal up to
+ * how exactly the input obeys its symmetry).  Host buffers. */
 int gwhip_eval_sigma_ibz(gwhip_handle* h, const double* G_ibz, double* Sigma_ibz, int precision);
 int gwhip_eval_sigma_ibz_device(gwhip_handle* h, const void* G_ibz_dev, void* Sigma_ibz_dev, int precision);
 

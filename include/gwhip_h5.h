@@ -13,7 +13,8 @@
  *                            c = first stored row of the chunk
  *   <input_h5>               /symmetry/pairs/{conj_pairs_list, trans_pairs_list, kpair_irre_list}
  *                            (all pairs stored in order when the group is absent),
- *                            /symmetry/k/{ink, bz2ibz, ibz2bz, k_sym_transform_ao, tr_conj, mesh_scaled}
+ *                            /symmetry/k/{ink, bz2ibz, ibz2bz, k_sym_transform_ao, tr_conj, mesh_scaled},
+ *                            /symmetry/q/{inq, bz2ibz, ibz2bz, k_sym_transform_p0, tr_conj}
  *
  * Status codes are gwhip.h's.  Not thread-safe (HDF5 is used without its thread-safe build).
  */
@@ -49,13 +50,21 @@ int gwhip_h5_read_packed(const char* df_path, const char* input_h5, int64_t nk, 
 int gwhip_h5_read_k_symmetry(const char* input_h5, int64_t nk, int64_t nso, int64_t* ink, int64_t* bz2ibz,
                              int64_t* ibz2bz, double* U, int64_t* tr_conj, double* mesh_scaled);
 
+/* The q-symmetry tables of input_h5 (/symmetry/q) for gwhip_set_q_symmetry: *inq, bz2ibz [nq],
+ * ibz2bz [nq] (first *inq entries used), U [nq][NQ][NQ] complex (k_sym_transform_p0) and
+ * tr_conj [nq]. */
+int gwhip_h5_read_q_symmetry(const char* input_h5, int64_t nq, int64_t NQ, int64_t* inq, int64_t* bz2ibz,
+                             int64_t* ibz2bz, double* U, int64_t* tr_conj);
+
 /* k + q and k - q ([q][k] tables, as gwhip_set_kq_maps takes them) from the fractional k
  * coordinates: the mesh point at (k +- q) mod 1, matched to 1e-6. */
 int gwhip_h5_kq_maps(const double* mesh_scaled, int64_t nk, int32_t* kplus, int32_t* kminus);
 
 /* Everything a handle needs from the files: k +- q maps from mesh_scaled, the integrals
  * (gwhip_upload_V), and, when `with_symmetry` is non-zero, the k-symmetry tables
- * (gwhip_set_k_symmetry).  The handle's dims give nk, nao, nso and NQ. */
+ * (gwhip_set_k_symmetry) and, where the file has /symmetry/q/k_sym_transform_p0, the q-symmetry
+ * tables (gwhip_set_q_symmetry; switched off otherwise).  The handle's dims give nk, nao, nso
+ * and NQ. */
 int gwhip_h5_load(gwhip_handle* h, const char* df_path, const char* input_h5, int with_symmetry);
 
 /* The q -> 0 extrapolation data of <df_path>/AqQ.h5 (AqQ [inq][NQ] complex, q_abs [inq], madelung;

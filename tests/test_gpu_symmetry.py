@@ -2,9 +2,15 @@
 
 All through the C ABI:
 * the reference's nk = 27 GW fixture (test/solvers_test.cpp:389) end to end from its own files:
-  gwhip_h5_load (integrals, k +- q maps, symmetry tables) then gwhip_eval_sigma_ibz with G on the
-  4 irreducible points; against result/Sigma_tau at the reference's 1e-6 and against the oracle's
-  full-mesh Sigma gathered at the irreducible points at 1e-10 relative;
+  gwhip_h5_load (integrals, k +- q maps, k and q symmetry tables) then gwhip_eval_sigma_ibz with G
+  on the 4 irreducible points, i.e. the star-of-q evaluation (4 irreducible q); against
+  result/Sigma_tau at 1e-9 (the reference's own bar is 1e-6) and against the oracle's star-of-q
+  Sigma at 1e-10 relative, in FP64; split precision against the FP64 result at 1e-8; with the q
+  symmetry switched off, the full-mesh evaluation against the oracle's full-mesh Sigma gathered at
+  the irreducible points at 1e-10;
+* a synthetic nk = 8 system with a made-up q wedge (random unitary U_q, time reversal with and
+  without U_q = I): the star-of-q evaluation against the oracle's at 1e-10 (FP64) and 1e-8 (split);
+* identity k and q wedges (ink = nk, inq = nq, U = I) give the full-mesh result bit for bit;
 * the reference's nk = 27 HF fixture (solvers_test.cpp:387) through gwhip_hf_ibz;
 * a synthetic system with random unitary U and time reversal on some points (nk = 8, ns = 2):
   device unfolding equals the oracle's value_AO, Sigma and F to 1e-10 relative;
@@ -32,6 +38,15 @@ def _sym():
     return {k: z[k] for k in z.files}
 
 
+def _sym27_oracles(z, g, T):
+    Vp = oi.packed_from_irreducible(g["V"], 27, z["pairs_conj_pairs_list"], z["pairs_trans_pairs_list"],
+                                    z["pairs_kpair_irre_list"])
+    S_star = osy.gw_sigma_star(g["G_tau"], Vp, T.T_bt, T.T_tb, z, 27, 2, 2, 2, 36, (3, 3, 3))
+    G = osy.unfold(g["G_tau"], z["k_bz2ibz"], z["k_ibz2bz"], z["k_k_sym_transform_ao"], z["k_tr_conj"])
+    S_full = osy.full_to_ibz(go.gw_sigma(G, Vp, T.T_bt, T.T_tb, 27, 2, 2, 2, 36, mesh=(3, 3, 3)), z["k_ibz2bz"])
+    return S_star, S_full
+
+
 def test_gw_sym27_from_files():
     z = _sym()
     g = np.load(os.path.join(GOLDEN, "gw_sym27.npz"), allow_pickle=False)
@@ -41,17 +56,71 @@ def test_gw_sym27_from_files():
         k.set_grid(T.T_bt, T.T_tb)
         k.load_files(os.path.join(H5, "gw_sym27_df"), os.path.join(H5, "sym27_input.h5"))
         S = k.solve_ibz(g["G_tau"])
+        S_split = k.solve_ibz(g["G_tau"], gl.PREC_SPLIT)
+        k.set_q_symmetry(0)
+        S_fm = k.solve_ibz(g["G_tau"])
     finally:
         k.close()
+    S_star, S_full = _sym27_oracles(z, g, T)
     err_ref = np.abs(S - g["Sigma_tau"]).max()
-    Vp = oi.packed_from_irreducible(g["V"], 27, z["pairs_conj_pairs_list"], z["pairs_trans_pairs_list"],
-                                    z["pairs_kpair_irre_list"])
-    G = osy.unfold(g["G_tau"], z["k_bz2ibz"], z["k_ibz2bz"], z["k_k_sym_transform_ao"], z["k_tr_conj"])
-    S_or = osy.full_to_ibz(go.gw_sigma(G, Vp, T.T_bt, T.T_tb, 27, 2, 2, 2, 36, mesh=(3, 3, 3)), z["k_ibz2bz"])
-    err_or = rel_err(S, S_or)
-    print(f"nk=27 GW fixture: {err_ref:.2e} abs vs reference, {err_or:.2e} rel vs oracle", flush=True)
-    assert err_ref < 1e-6
-    assert err_or < 1e-10
+    err_or, err_split, err_fm = rel_err(S, S_star), rel_err(S_split, S), rel_err(S_fm, S_full)
+    print(f"nk=27 GW fixture, star of q: {err_ref:.2e} abs vs reference, {err_or:.2e} rel vs oracle, "
+          f"split {err_split:.2e}; full mesh: {err_fm:.2e} rel vs oracle", flush=True)
+    assert err_ref < 1e-9      # the reference's bar is 1e-6; the oracle reaches 2.6e-11
+    assert err_or < 1e-10 and err_fm < 1e-10
+    assert err_split < 1e-8
+
+
+def _random_q_symmetry(nq, NQ, seed):
+    """Irreducible q 0, 1, 2 (U = I); the others: random unitary U_q on odd q, time reversal on
+    q = 4 (with U = I) and q = 5 (with U != I)."""
+    rng = np.random.default_rng(seed)
+    i2b = np.arange(3)
+    b2i = np.array([q if q < 3 else q % 3 for q in range(nq)], np.int64)
+    U = np.empty((nq, NQ, NQ), complex)
+    for q in range(nq):
+        u, _ = np.linalg.qr(rng.standard_normal((NQ, NQ)) + 1j * rng.standard_normal((NQ, NQ)))
+        U[q] = u if (q >= 3 and q % 2) else np.eye(NQ)
+    tr = np.array([1 if q in (4, 5) else 0 for q in range(nq)], np.int64)
+    return 3, b2i, i2b, U, tr
+
+
+def test_random_q_wedge_star_matches_oracle():
+    G, V, T, d = synthetic(8, 5, 12, ns=2, seed=7)
+    ink, kb2i, ki2b, Uk, ktr = _random_symmetry(8, 5, 8)
+    inq, qb2i, qi2b, Uq, qtr = _random_q_symmetry(8, 12, 9)
+    G_ibz = np.ascontiguousarray(G[:, :, :ink])
+    k = gl.GwHipKernel(8, 5, 5, 2, 12, T.nts, T.nw_b, d["beta"], mesh=d["mesh"])
+    try:
+        k.set_grid(T.T_bt, T.T_tb)
+        k.upload_V(V)
+        k.set_k_symmetry(ink, kb2i, ki2b, Uk, ktr)
+        k.set_q_symmetry(inq, qb2i, qi2b, Uq, qtr)
+        S = k.solve_ibz(G_ibz)
+        S_split = k.solve_ibz(G_ibz, gl.PREC_SPLIT)
+    finally:
+        k.close()
+    sym = dict(k_bz2ibz=kb2i, k_ibz2bz=ki2b, k_k_sym_transform_ao=Uk, k_tr_conj=ktr, q_bz2ibz=qb2i, q_ibz2bz=qi2b,
+               q_k_sym_transform_p0=Uq, q_tr_conj=qtr)
+    S_or = osy.gw_sigma_star(G_ibz, V, T.T_bt, T.T_tb, sym, 8, 5, 5, 2, 12, d["mesh"])
+    print(f"random q wedge, star of q: FP64 {rel_err(S, S_or):.2e}, split {rel_err(S_split, S_or):.2e}", flush=True)
+    assert rel_err(S, S_or) < 1e-10
+    assert rel_err(S_split, S_or) < 1e-8
+
+
+def test_identity_q_wedge_is_the_full_mesh():
+    G, V, T, d = synthetic(8, 4, 10, ns=1, seed=11)
+    k = gl.GwHipKernel(8, 4, 4, 1, 10, T.nts, T.nw_b, d["beta"], mesh=d["mesh"])
+    try:
+        k.set_grid(T.T_bt, T.T_tb)
+        k.upload_V(V)
+        S_full = k.solve(G)
+        k.set_k_symmetry(8, np.arange(8), np.arange(8), np.stack([np.eye(4, dtype=complex)] * 8), np.zeros(8, np.int64))
+        k.set_q_symmetry(8, np.arange(8), np.arange(8), np.stack([np.eye(10, dtype=complex)] * 8), np.zeros(8, np.int64))
+        S_star = k.solve_ibz(G)
+    finally:
+        k.close()
+    assert np.array_equal(S_full, S_star)
 
 
 def test_hf_sym27_fixture():
@@ -129,7 +198,8 @@ def test_identity_wedge_is_the_full_mesh():
 
 def test_gw_c_extrapolate_from_files():
     """GW_C (solvers_test.cpp:390): time-reversal wedge + q0_treatment EXTRAPOLATE, from the
-    reference's files (integrals, symmetry tables, AqQ.h5)."""
+    reference's files (integrals, k and q symmetry tables, AqQ.h5): the star-of-q evaluation over
+    the 14 irreducible q, P-tilde conjugated on the time-reversal partners."""
     from oracle import q0
     c = np.load(os.path.join(GOLDEN, "gwc_sym27.npz"), allow_pickle=False)
     T = transformer(float(c["lam"]), float(c["beta"]), c["tau"])
@@ -147,11 +217,10 @@ def test_gw_c_extrapolate_from_files():
     err_ref = np.abs(S - c["Sigma_tau"]).max()
     Vp = oi.packed_from_irreducible(c["V"], 27, c["pairs_conj_pairs_list"], c["pairs_trans_pairs_list"],
                                     c["pairs_kpair_irre_list"])
-    G = osy.unfold(c["G_tau"], c["k_bz2ibz"], c["k_ibz2bz"], c["k_k_sym_transform_ao"], c["k_tr_conj"])
-    S_full, Pw = go.gw_sigma(G, Vp, T.T_bt, T.T_tb, 27, 2, 2, 2, 36, mesh=(3, 3, 3), return_Pw=True)
-    et = q0.eps_q0_tau(q0.eps_inv_wq(Pw, c["AqQ"], c["q_ibz2bz"]), c["q_abs"], T.T_tb)
-    S_or = q0.apply_q0_correction(osy.full_to_ibz(S_full, c["k_ibz2bz"]), c["G_tau"], S_ibz, et,
-                                  float(c["q0_madelung"]))
+    S_star, Pw = osy.gw_sigma_star(c["G_tau"], Vp, T.T_bt, T.T_tb, {n: c[n] for n in c.files}, 27, 2, 2, 2, 36,
+                                   (3, 3, 3), return_Pw=True)
+    et = q0.eps_q0_tau(q0.eps_inv_wq(Pw, c["AqQ"], np.arange(len(c["q_ibz2bz"]))), c["q_abs"], T.T_tb)
+    S_or = q0.apply_q0_correction(S_star, c["G_tau"], S_ibz, et, float(c["q0_madelung"]))
     print(f"GW_C: {err_ref:.2e} abs vs reference, {rel_err(S, S_or):.2e} rel vs oracle, "
           f"eps {rel_err(eps_t, et):.2e}", flush=True)
     assert err_ref < 1e-6 and err_ref < 1e-9
