@@ -109,6 +109,8 @@ def main():
                     help="time the 8 uneven ns=2 systems evaluated as one captured graph instead")
     ap.add_argument("--dyson", action="store_true",
                     help="time the Dyson G step and the chemical-potential search at nk=64 nso=26 instead")
+    ap.add_argument("--star", action="store_true",
+                    help="the star-of-q evaluation on a made-up 8-point irreducible wedge of the same workload")
     ap.add_argument("--hf", action="store_true",
                     help="time the Hartree-Fock Fock build on the same nk=64 nao=26 naux=300 integrals instead")
     ap.add_argument("--no-autotune", action="store_true",
@@ -129,6 +131,8 @@ def main():
         return bench_batch(args)
     if args.hf:
         return bench_hf(args)
+    if args.star:
+        return bench_star(args)
     if args.dyson:
         return bench_dyson(args)
 
@@ -418,6 +422,75 @@ def bench_hf(args):
     }
     if not args.no_cpu:
         out["cpu_baseline"] = hf_cpu_baseline(w)
+    print(json.dumps(out), flush=True)
+
+
+def bench_star(args):
+    """Sigma on a symmetry-reduced mesh by the star-of-q scheme: the nk = 64 workload with a made-up
+    wedge of 8 irreducible k and q points (k -> k mod 8; random unitary U_k and U_q elsewhere).  The
+    tables are not a real space group, so only the time is meaningful: P0 and P-tilde for 8 q, Sigma
+    for 8 k over all 64 q, against 64 and 64 on the full mesh."""
+    import numpy as np
+    import torch
+
+    from gwhip import inputs, irgrid
+    from gwhip import lib as gl
+
+    w = WORKLOAD
+    nk, nao, NQ, ns, nir = w["nk"], w["nao"], w["NQ"], w["ns"], 8
+    T = irgrid.Transformer.from_grid(irgrid.load_grid(w["lam"]), w["beta"])
+    G = inputs.generate_G(w["seed"], nk, nao, ns, T.tau, w["beta"])
+    G_ibz = np.ascontiguousarray(G[:, :, :nir])
+    rng = np.random.default_rng(3)
+
+    def unitaries(n):
+        U = np.empty((nk, n, n), np.complex128)
+        for i in range(nk):
+            q, _ = np.linalg.qr(rng.standard_normal((n, n)) + 1j * rng.standard_normal((n, n)))
+            U[i] = np.eye(n) if i < nir else q
+        return U
+
+    b2i = np.arange(nk, dtype=np.int64) % nir
+    i2b = np.arange(nir, dtype=np.int64)
+    zeros = np.zeros(nk, np.int64)
+    k = gl.GwHipKernel(nk, nao, nao, ns, NQ, T.nts, T.nw_b, w["beta"], mesh=w["mesh"])
+    k.set_grid(T.T_bt, T.T_tb)
+    k.generate_V(w["seed"], w["sigma"])
+    k.set_k_symmetry(nir, b2i, i2b, unitaries(nao), zeros)
+    k.set_q_symmetry(nir, b2i, i2b, unitaries(NQ), zeros)
+    prec = gl.PREC_FP64 if args.precision == "fp64" else gl.PREC_SPLIT
+    if prec == gl.PREC_SPLIT:
+        k.set_split_slices(args.slices)
+    dev = torch.device("cuda:0")
+    G_d = torch.from_numpy(G_ibz.view(np.float64)).to(dev)
+    S_d = torch.empty_like(G_d)
+    torch.cuda.synchronize()
+    for i in range(args.warmup):
+        k.solve_ibz_device(G_d.data_ptr(), S_d.data_ptr(), prec)
+        print(f"[bench] star warmup {i} done", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    stats = {name: [0, 0.0] for name in gl.KSTATS}
+    t0 = time.perf_counter()
+    for i in range(args.steps):
+        k.solve_ibz_device(G_d.data_ptr(), S_d.data_ptr(), prec)
+        for name, (n, ms) in k.kernel_stats().items():
+            stats[name][0] += n
+            stats[name][1] += ms
+        print(f"[bench] star step {i} done", file=sys.stderr, flush=True)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / args.steps * 1e3
+    k.close()
+    out = {
+        "metric": "GW Sigma-evals/s @nk64 nao26 naux300, star of q over 8 irreducible k and q", "value": 1e3 / ms,
+        "unit": "Sigma-evals/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup, "ms_per_step": ms,
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "f64" if prec == gl.PREC_FP64 else "int8-slices/i32/f64",
+        "data": "synthetic (seed 0 G and V as the main workload; made-up wedge k -> k mod 8, random unitary "
+                "U_k, U_q; timing only)",
+        "config": {"workload": "GW Sigma(tau, k_ir), 4x4x4 k mesh, 8 irreducible k and q, nao 26, naux 300, ns 1, "
+                               "beta 100, IR 1e4", "precision": args.precision, "parallelism": "single GPU"},
+        "kernel_ms": {name: v[1] / args.steps for name, v in stats.items()},
+    }
     print(json.dumps(out), flush=True)
 
 
