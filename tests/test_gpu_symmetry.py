@@ -85,12 +85,15 @@ def _random_q_symmetry(nq, NQ, seed):
     return 3, b2i, i2b, U, tr
 
 
-def test_random_q_wedge_star_matches_oracle():
-    G, V, T, d = synthetic(8, 5, 12, ns=2, seed=7)
-    ink, kb2i, ki2b, Uk, ktr = _random_symmetry(8, 5, 8)
+@pytest.mark.parametrize("ns,nso_factor", [(2, 1), (1, 2)])
+def test_random_q_wedge_star_matches_oracle(ns, nso_factor):
+    """Unrestricted, and X2C spin blocks (nso = 2 nao)."""
+    nso = 5 * nso_factor
+    G, V, T, d = synthetic(8, 5, 12, ns=ns, seed=7, nso=nso)
+    ink, kb2i, ki2b, Uk, ktr = _random_symmetry(8, nso, 8)
     inq, qb2i, qi2b, Uq, qtr = _random_q_symmetry(8, 12, 9)
     G_ibz = np.ascontiguousarray(G[:, :, :ink])
-    k = gl.GwHipKernel(8, 5, 5, 2, 12, T.nts, T.nw_b, d["beta"], mesh=d["mesh"])
+    k = gl.GwHipKernel(8, 5, nso, ns, 12, T.nts, T.nw_b, d["beta"], mesh=d["mesh"])
     try:
         k.set_grid(T.T_bt, T.T_tb)
         k.upload_V(V)
@@ -102,8 +105,9 @@ def test_random_q_wedge_star_matches_oracle():
         k.close()
     sym = dict(k_bz2ibz=kb2i, k_ibz2bz=ki2b, k_k_sym_transform_ao=Uk, k_tr_conj=ktr, q_bz2ibz=qb2i, q_ibz2bz=qi2b,
                q_k_sym_transform_p0=Uq, q_tr_conj=qtr)
-    S_or = osy.gw_sigma_star(G_ibz, V, T.T_bt, T.T_tb, sym, 8, 5, 5, 2, 12, d["mesh"])
-    print(f"random q wedge, star of q: FP64 {rel_err(S, S_or):.2e}, split {rel_err(S_split, S_or):.2e}", flush=True)
+    S_or = osy.gw_sigma_star(G_ibz, V, T.T_bt, T.T_tb, sym, 8, 5, nso, ns, 12, d["mesh"])
+    print(f"random q wedge (ns = {ns}, nso = {nso}), star of q: FP64 {rel_err(S, S_or):.2e}, "
+          f"split {rel_err(S_split, S_or):.2e}", flush=True)
     assert rel_err(S, S_or) < 1e-10
     assert rel_err(S_split, S_or) < 1e-8
 
