@@ -48,6 +48,7 @@ public:
       if (!systems_[i]->has_inputs() || !G[i])
         throw status_error(GWHIP_ERR_STATE, "system " + std::to_string(i) + ": grid, integrals or G missing");
     prepare(precision == GWHIP_PREC_SPLIT);
+    for (auto& s : systems_) s->ensure_orb_buffers();
     for (int i = 0; i < size(); ++i) systems_[i]->autotune(G[i], precision);
     drop_graphs();
   }
@@ -62,7 +63,10 @@ public:
       if (!systems_[i]->has_inputs())
         throw status_error(GWHIP_ERR_STATE, "system " + std::to_string(i) + ": grid or integrals not set");
     prepare(precision == GWHIP_PREC_SPLIT);
-    for (auto& s : systems_) s->check_ready(precision);
+    for (auto& s : systems_) {
+      s->check_ready(precision);
+      s->ensure_orb_buffers();  // before any capture: the GEMM form of the orbital products allocates
+    }
     std::vector<const void*> key;
     for (int i = 0; i < size(); ++i) {
       key.push_back(G[i]);

@@ -127,6 +127,8 @@ void block_gemm(hipStream_t st, int n, int64_t nb, const double2* A, const doubl
 void gw_hip_kernel::hf_device(const double2* dm, const double2* S, double madelung, double2* F) {
   if (!v_set_) throw status_error(GWHIP_ERR_STATE, "integrals not set (gwhip_upload_V / gwhip_generate_V)");
   if (!dm || !S || !F) throw status_error(GWHIP_ERR_ARG, "null dm, S or F");
+  if (d_.nao > kOrbMaxNao)
+    throw status_error(GWHIP_ERR_UNSUPPORTED, "the Hartree-Fock kernels handle nao <= " + std::to_string(kOrbMaxNao));
   GWHIP_CHECK(hipSetDevice(device_));
   const int64_t nk = d_.nk, NQ = d_.NQ, nao = d_.nao, nso = d_.nso, nao2 = nao * nao, nso2 = nso * nso;
   const bool x2c = d_.X2C;

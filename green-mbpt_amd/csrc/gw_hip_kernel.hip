@@ -73,7 +73,6 @@ gw_hip_kernel::gw_hip_kernel(const Dims& d, int device, hipStream_t shared_strea
   d_.pns = d_.nso / d_.nao;
   d_.pseudo_ns = d_.X2C ? 4 : d_.ns;
   d_.prefactor = (d_.ns == 2 || d_.X2C) ? 1.0 : 2.0;
-  if (d_.nao > 48) throw status_error(GWHIP_ERR_UNSUPPORTED, "nao > 48 not supported by the orbital contractions");
 
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= device || device < 0) {
@@ -127,7 +126,8 @@ gw_hip_kernel::~gw_hip_kernel() {
   for (void* p : {(void*)Vq_, (void*)Vt_, (void*)P_, (void*)Tbt_fold_, (void*)Ttb_, (void*)G_dev_, (void*)S_dev_,
                   (void*)kplus_tab_, (void*)kminus_tab_, (void*)flag_, (void*)hcheck_, (void*)hf_work_, (void*)hf_x_, (void*)ksym_idx_, (void*)ksym_U_, (void*)hf_sym_, (void*)q0_qiq_, (void*)q0_A_, (void*)q0_eps_dev_,
                   (void*)q0_S_, (void*)q0_tmp_, (void*)q0_epst_, (void*)qsym_U_, (void*)star_P_, (void*)star_V_,
-                  (void*)star_Vs_, (void*)star_eVs_, (void*)star_km_, (void*)star_q0row_})
+                  (void*)star_Vs_, (void*)star_eVs_, (void*)star_km_, (void*)star_q0row_, (void*)orb_W_,
+                  (void*)orb_G_})
     if (p) (void)hipFree(p);
   if (own_ws_)
     for (void* p : {(void*)Pw_, (void*)X_, (void*)Y_, (void*)scratch_})
@@ -480,6 +480,10 @@ void gw_hip_kernel::zero(double2* X, int64_t n) {
 
 // Z_Q = sum_s Gb_s V_Q Gq_s for q and tau in [t0, t0 + tc) into scratch_.
 void gw_hip_kernel::p0_prep(int64_t q, const double2* G, int64_t t0, int64_t tc, int qpw) {
+  if (orb_gemm()) {
+    p0_prep_gemm(q, G, t0, tc);
+    return;
+  }
   const int64_t nk = d_.nk, NQ = d_.NQ, nao2 = d_.nao * d_.nao;
   DevDims dd{(int)d_.nts, (int)d_.ns, (int)nk, (int)d_.nso, (int)d_.nao, (int)NQ, (int)d_.pns, (int)d_.pseudo_ns};
   const int NB = (int)ceil_div(d_.nao, 16);
@@ -608,7 +612,7 @@ void gw_hip_kernel::p0_one_q(int64_t q, const double2* G, bool split, double2* P
   const int64_t nk = d_.nk, NQ = d_.NQ, nao2 = d_.nao * d_.nao, NQ2 = NQ * NQ, half = d_.nts / 2;
   for (int64_t t0 = 0; t0 < half; t0 += tchunk_p0_) {
     const int64_t tc = std::min(tchunk_p0_, half - t0);
-    if (split && d_.pseudo_ns == 1 && tune_.p0_fused) {
+    if (split && d_.pseudo_ns == 1 && tune_.p0_fused && !orb_gemm()) {
       timed(K_P0_PREP, [&] { p0_prep_slice(q, G, t0, tc, tune_.qpw); });
     } else {
       timed(K_P0_PREP, [&] { p0_prep(q, G, t0, tc, tune_.qpw); });
@@ -763,6 +767,10 @@ void gw_hip_kernel::sigma_q(const double2* G, double2* Sigma, bool split, const 
       timed(K_SIGMA_GEMM, [&] { sigma_gemm(Pq + t0 * NQ2, Vb, nkb, tc, tune_.s_shape); });
     }
     timed(K_SIGMA_EPI, [&] {
+      if (orb_gemm()) {
+        sigma_epi_gemm(G, Sigma, Vb, km, nkb, t0, tc, scale);
+        return;
+      }
       const dim3 grid((unsigned)nkb, 1u, (unsigned)(2 * tc));
       const int up = sigma_upper() ? 1 : 0;
       const int n = (int)nkb, t = (int)t0;
@@ -775,7 +783,92 @@ void gw_hip_kernel::sigma_q(const double2* G, double2* Sigma, bool split, const 
 
 // Sigma of Hermitian G is Hermitian; with one spin block per (tau, s, k) only its upper 16 x 16
 // blocks were accumulated, so fill the rest from them.
-bool gw_hip_kernel::sigma_upper() const { return g_herm_ && d_.pns == 1; }
+bool gw_hip_kernel::sigma_upper() const { return g_herm_ && d_.pns == 1 && !orb_gemm(); }
+
+void gw_hip_kernel::ensure_orb_buffers() {
+  if (!orb_gemm()) return;
+  if (!orb_W_) orb_W_ = dalloc<double2>((size_t)d_.nk * d_.NQ * d_.nao * d_.nao);
+  if (!orb_G_) orb_G_ = dalloc<double2>((size_t)d_.nk * d_.nso * d_.nso);
+}
+
+// orb_G_[k] = G(tau, is, map[k]) for k < n.
+void gw_hip_kernel::gather_g(const double2* G, int64_t tau, int64_t is, const int* map, int64_t n) {
+  const int64_t nso2 = d_.nso * d_.nso;
+  if (n < 1 || n > 65535) throw status_error(GWHIP_ERR_UNSUPPORTED, "gather_g: k count out of range");
+  hipLaunchKernelGGL(gather_g_kernel, dim3((unsigned)ceil_div(nso2, 256), (unsigned)n), dim3(256), 0, stream_,
+                     G + (tau * d_.ns + is) * d_.nk * nso2, orb_G_, map, nso2);
+  GWHIP_CHECK(hipGetLastError());
+}
+
+// The P0 orbital product as two batched FP64 GEMMs over k per (tau, spin block), for any nao:
+//   W[k][Q][p][n] = sum_m V(k, k+q)_Q[p][m] Gq(k)[m][n],     Gq = G(tau, is, k + q) block (a, b)
+//   Z[k][Q][t][n] (+)= sum_p Gb(k)[t][p] W[k][Q][p][n],      Gb = G(beta - tau, is, k) block (b, a)
+// (the same Z_Q = Gb V_Q Gq as z_prep_mfma_kernel, src/gw_cpu_kernel.cpp:200-215).
+void gw_hip_kernel::p0_prep_gemm(int64_t q, const double2* G, int64_t t0, int64_t tc) {
+  const int64_t nk = d_.nk, NQ = d_.NQ, nao = d_.nao, nao2 = nao * nao, nso = d_.nso, nso2 = nso * nso;
+  const int64_t vq = NQ * nao2;
+  if (!orb_W_ || !orb_G_) throw status_error(GWHIP_ERR_STATE, "orbital GEMM buffers not allocated");
+  for (int64_t t = 0; t < tc; ++t) {
+    const int64_t tau = t0 + t, tt = d_.nts - 1 - tau;
+    for (int64_t s = 0; s < d_.pseudo_ns; ++s) {
+      const int64_t is = s / (d_.pns * d_.pns), a = (s / d_.pns) % d_.pns, b = s % d_.pns;
+      gather_g(G, tau, is, kplus_tab_ + q * nk, nk);
+      ZGemmArgs w{};
+      w.M = (int)(NQ * nao); w.N = (int)nao; w.K = (int)nao; w.batch = (int)nk; w.kin = (int)nao; w.nin = (int)nao;
+      w.A = Vq_ + q * nk * vq; w.sAb = vq; w.sAm = nao; w.sAko = 0; w.sAki = 1;
+      w.B = orb_G_ + a * nao * nso + b * nao; w.sBb = nso2; w.sBko = 0; w.sBki = nso; w.sBno = 0; w.sBni = 1;
+      w.C = orb_W_; w.sCb = vq; w.sCm = nao; w.sCno = 0; w.sCni = 1;
+      w.alpha = make_double2(1.0, 0.0);
+      zgemm_f64(w, true, false, stream_);
+      ZGemmArgs z{};
+      z.M = (int)nao; z.N = (int)(NQ * nao); z.K = (int)nao; z.batch = (int)nk; z.kin = (int)nao; z.nin = (int)nao;
+      z.A = G + (tt * d_.ns + is) * nk * nso2 + b * nao * nso + a * nao; z.sAb = nso2; z.sAm = nso; z.sAko = 0;
+      z.sAki = 1;
+      z.B = orb_W_; z.sBb = vq; z.sBko = 0; z.sBki = nao; z.sBno = nao2; z.sBni = 1;
+      z.C = scratch_ + t * nk * vq; z.sCb = vq; z.sCm = nao; z.sCno = nao2; z.sCni = 1;
+      z.alpha = make_double2(1.0, 0.0);
+      z.accumulate = s > 0 ? 1 : 0;
+      zgemm_f64(z, true, false, stream_);
+    }
+  }
+}
+
+// The Sigma epilogue as two batched FP64 GEMMs over the nkb k points per (tau, spin block), for
+// any nao; R(tz) in scratch_ [tz][nkb][Q][n][j] serves tau = t0 + tz and beta - tau:
+//   XT[k][Q][n][i] = sum_m G(k)[m][n] V_Q[i][m],   G = G(tau, is, km[k]) block (a, b), i.e. X = V_Q G
+//   Sigma(tau, is, k)[a,b](i, j) += scale sum_{Q, n} X[i][Q, n] R[k][Q][n][j]
+// (sigma_epi_mfma_kernel's products, src/gw_cpu_kernel.cpp:359-373).
+void gw_hip_kernel::sigma_epi_gemm(const double2* G, double2* Sigma, const double2* Vb, const int* km, int64_t nkb,
+                                   int64_t t0, int64_t tc, double scale) {
+  const int64_t NQ = d_.NQ, nao = d_.nao, nao2 = nao * nao, nso = d_.nso, nso2 = nso * nso, vq = NQ * nao2;
+  if (!orb_W_ || !orb_G_) throw status_error(GWHIP_ERR_STATE, "orbital GEMM buffers not allocated");
+  for (int64_t tz = 0; tz < tc; ++tz)
+    for (int side = 0; side < 2; ++side) {
+      const int64_t tau = side ? d_.nts - 1 - (t0 + tz) : t0 + tz;
+      for (int64_t s = 0; s < d_.pseudo_ns; ++s) {
+        const int64_t is = s / (d_.pns * d_.pns), a = (s / d_.pns) % d_.pns, b = s % d_.pns;
+        gather_g(G, tau, is, km, nkb);
+        ZGemmArgs x{};
+        x.M = (int)nao; x.N = (int)(NQ * nao); x.K = (int)nao; x.batch = (int)nkb; x.kin = (int)nao;
+        x.nin = (int)nao;
+        x.A = orb_G_ + a * nao * nso + b * nao; x.sAb = nso2; x.sAm = 1; x.sAko = 0; x.sAki = nso;
+        x.B = Vb; x.sBb = vq; x.sBko = 0; x.sBki = nao; x.sBno = nao2; x.sBni = 1;
+        x.C = orb_W_; x.sCb = vq; x.sCm = nao; x.sCno = nao2; x.sCni = 1;
+        x.alpha = make_double2(1.0, 0.0);
+        zgemm_f64(x, false, false, stream_);
+        ZGemmArgs y{};
+        y.M = (int)nao; y.N = (int)nao; y.K = (int)(NQ * nao); y.batch = (int)nkb; y.kin = (int)nao;
+        y.nin = (int)nao;
+        y.A = orb_W_; y.sAb = vq; y.sAm = 1; y.sAko = nao2; y.sAki = nao;
+        y.B = scratch_ + tz * nkb * vq; y.sBb = vq; y.sBko = nao2; y.sBki = nao; y.sBno = 0; y.sBni = 1;
+        y.C = Sigma + (tau * d_.ns + is) * nkb * nso2 + a * nao * nso + b * nao; y.sCb = nso2; y.sCm = nso;
+        y.sCno = 0; y.sCni = 1;
+        y.alpha = make_double2(scale, 0.0);
+        y.accumulate = 1;
+        zgemm_f64(y, false, false, stream_);
+      }
+    }
+}
 
 void gw_hip_kernel::finish_sigma(double2* Sigma, int64_t nkS) {
   if (!sigma_upper()) return;
@@ -814,6 +907,7 @@ void gw_hip_kernel::solve_device(const double2* G, double2* Sigma, int precision
   for (double& r : region_ms_) r = 0.0;
   for (auto& k : kstat_) k = KernelStat{};
   const int64_t nsig = d_.nts * d_.ns * d_.nk * d_.nso * d_.nso;
+  ensure_orb_buffers();
 
   hipEvent_t t0, t1, t2, t3;
   GWHIP_CHECK(hipEventCreate(&t0)); GWHIP_CHECK(hipEventCreate(&t1));
@@ -860,6 +954,7 @@ void gw_hip_kernel::autotune(const double2* G, int precision) {
   GWHIP_CHECK(hipSetDevice(device_));
   const bool split = precision == GWHIP_PREC_SPLIT;
   if (split) ensure_slices();
+  ensure_orb_buffers();
   detect_hermitian_G(G);
   const int64_t half = d_.nts / 2, tcp = std::min(tchunk_p0_, half), tcs = std::min(tchunk_sigma_, half);
   hipEvent_t a, b;
@@ -896,7 +991,7 @@ void gw_hip_kernel::autotune(const double2* G, int precision) {
     tune_.p0_shape = best_of(3, shapes, [&](int c) { p0_gemm(0, tcp, P_, c); });
     tune_.s_shape = best_of(3, shapes, [&](int c) { sigma_gemm(P_, Vt_neg(0), d_.nk, tcs, c); });
   } else {
-    if (d_.pseudo_ns == 1) {  // fused prep + slicing vs the two kernels
+    if (d_.pseudo_ns == 1 && !orb_gemm()) {  // fused prep + slicing vs the two kernels
       const int fz[2] = {0, 1};
       tune_.p0_fused = best_of(2, fz, [&](int c) {
         if (c) {
@@ -946,6 +1041,7 @@ void gw_hip_kernel::run_stages_host(const double2* G, int last_stage) {
   if (!G_dev_) G_dev_ = dalloc<double2>(nsig);
   GWHIP_CHECK(hipMemcpy(G_dev_, G, nsig * sizeof(double2), hipMemcpyHostToDevice));
   stored_valid_ = false;
+  ensure_orb_buffers();
   detect_hermitian_G(G_dev_);
   GWHIP_CHECK(hipMemsetAsync(P_, 0, (size_t)d_.nk * d_.nts * d_.NQ * d_.NQ * sizeof(double2), stream_));
   stage_p0(G_dev_, false);

@@ -44,17 +44,19 @@ struct Tuning {
   int p0_oz_tile = 2, s_oz_tile = 2;  // split precision: int8 GEMM block tile (0: 32x64, 1: 64x64,
                                       // 2: 32x64 software-pipelined)
   int p0_fused = 1;  // split precision, one spin block: P0 orbital kernel writes int8 slices directly
-  static constexpr int count = 8;
+  int orb_gemm = 0;  // orbital contractions as batched FP64 GEMMs (always so for nao > 48)
+  static constexpr int count = 9;
   void to_array(int* v) const {
-    const int a[count] = {p0_shape, s_shape, qpw, p0_group, s_group, p0_oz_tile, s_oz_tile, p0_fused};
+    const int a[count] = {p0_shape, s_shape, qpw, p0_group, s_group, p0_oz_tile, s_oz_tile, p0_fused, orb_gemm};
     for (int i = 0; i < count; ++i) v[i] = a[i];
   }
   bool from_array(const int* v) {
     if (v[0] < 0 || v[0] > 2 || v[1] < 0 || v[1] > 2 || (v[2] != 1 && v[2] != 2 && v[2] != 4 && v[2] != 8) ||
-        v[3] < 1 || v[4] < 1 || v[5] < 0 || v[5] > 2 || v[6] < 0 || v[6] > 2 || v[7] < 0 || v[7] > 1)
+        v[3] < 1 || v[4] < 1 || v[5] < 0 || v[5] > 2 || v[6] < 0 || v[6] > 2 || v[7] < 0 || v[7] > 1 || v[8] < 0 ||
+        v[8] > 1)
       return false;
     p0_shape = v[0]; s_shape = v[1]; qpw = v[2]; p0_group = v[3]; s_group = v[4]; p0_oz_tile = v[5]; s_oz_tile = v[6];
-    p0_fused = v[7];
+    p0_fused = v[7]; orb_gemm = v[8];
     return true;
   }
 };
@@ -155,6 +157,10 @@ public:
     ++generation_;
   }
   void set_timing(bool on) { timing_ = on; }
+  // buffers of the GEMM form of the orbital contractions; allocates, so call before a capture
+  void ensure_orb_buffers();
+  // the register-chained orbital kernels handle nao up to 3 x 16
+  static constexpr int64_t kOrbMaxNao = 48;
   bool has_inputs() const { return grid_set_ && v_set_; }
   uint64_t generation() const { return generation_; }
   hipStream_t stream() const { return stream_; }
@@ -237,6 +243,8 @@ private:
   double2* q0_S_ = nullptr;         // [ns][nkS][nso^2]
   double2* q0_tmp_ = nullptr;       // G S, [nts][ns][nkS][nso^2]
   double2* q0_epst_ = nullptr;      // [nts]
+  double2* orb_W_ = nullptr;  // GEMM form of the orbital contractions: [nk][NQ][nao^2] intermediate
+  double2* orb_G_ = nullptr;  // and one (tau, spin) slice of G at mapped k points, [nk][nso^2]
   int* ksym_idx_ = nullptr;
   double2* ksym_U_ = nullptr;
   std::vector<int> ksym_i2b_;       // host copy of ibz2bz
@@ -253,7 +261,7 @@ private:
   int* star_eVs_ = nullptr;         // [ink][nao^2]
   int* star_km_ = nullptr;          // [nq][ink]
   int* star_q0row_ = nullptr;       // [inq]
-  int64_t star_ink_ = 0, star_vs_bytes_ = 0;
+  int64_t star_ink_ = 0, star_vs_bytes_ = 0, star_evs_elems_ = 0;
   bool star_dirty_ = true;
   double2* hf_sym_ = nullptr;
   int64_t hf_exchange_launches_ = 0;
@@ -277,6 +285,12 @@ private:
   std::vector<int> q0_samples() const;
   void apply_q0(double2* Sigma, const double2* G, int64_t nkS);
   void p0_prep(int64_t q, const double2* G, int64_t t0, int64_t tc, int qpw);
+  // orbital contractions as batched GEMMs (nao > 48, or tuning orb_gemm)
+  bool orb_gemm() const { return d_.nao > kOrbMaxNao || tune_.orb_gemm != 0; }
+  void p0_prep_gemm(int64_t q, const double2* G, int64_t t0, int64_t tc);
+  void sigma_epi_gemm(const double2* G, double2* Sigma, const double2* Vb, const int* km, int64_t nkb, int64_t t0,
+                      int64_t tc, double scale);
+  void gather_g(const double2* G, int64_t tau, int64_t is, const int* map, int64_t n);
   void p0_prep_slice(int64_t q, const double2* G, int64_t t0, int64_t tc, int qpw);
   void p0_gemm(int64_t q, int64_t tc, double2* Pt0, int shape);
   void sigma_gemm(const double2* Pt0, const double2* Vb, int64_t nkb, int64_t tc, int shape);

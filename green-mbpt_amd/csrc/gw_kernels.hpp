@@ -270,6 +270,16 @@ __global__ void zero_kernel(double2* X, int64_t n) {
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) X[i] = make_double2(0.0, 0.0);
 }
 
+// Out[k] = G[base + map[k] nso^2 ...] for k < grid.y: one (tau, spin) slice of G at mapped k
+// points, whole nso x nso matrices.
+__global__ void gather_g_kernel(const double2* __restrict__ G, double2* __restrict__ out,
+                                const int* __restrict__ map, int64_t nso2) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nso2) return;
+  const int64_t k = blockIdx.y;
+  out[k * nso2 + e] = G[(int64_t)map[k] * nso2 + e];
+}
+
 __global__ void zero_int_kernel(int* X, int64_t n) {
   const int64_t step = (int64_t)gridDim.x * blockDim.x;
   for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += step) X[i] = 0;

@@ -279,15 +279,17 @@ void gw_hip_kernel::solve_star_device(const double2* G_ibz, double2* Sigma_ibz, 
   const int64_t nk = d_.nk, ink = ink_, NQ = d_.NQ, nao2 = d_.nao * d_.nao;
   const bool split = precision == GWHIP_PREC_SPLIT;
   if (split) ensure_slices();
+  ensure_orb_buffers();
   star_tables();
   if (split) {
     const int64_t vs = ink * nao2 * 2 * split_ns_ * kc_s_;
     if (vs != star_vs_bytes_) {
       if (star_Vs_) (void)hipFree(star_Vs_), bytes_ -= (size_t)star_vs_bytes_, star_Vs_ = nullptr;
-      if (star_eVs_) (void)hipFree(star_eVs_), bytes_ -= (size_t)star_ink_ * nao2 * sizeof(int), star_eVs_ = nullptr;
+      if (star_eVs_) (void)hipFree(star_eVs_), bytes_ -= (size_t)star_evs_elems_ * sizeof(int), star_eVs_ = nullptr;
       star_Vs_ = dalloc<int8_t>((size_t)vs);
       star_eVs_ = dalloc<int>((size_t)ink * nao2);
       star_vs_bytes_ = vs;
+      star_evs_elems_ = ink * nao2;
     }
   }
   const size_t nfull = (size_t)d_.nts * d_.ns * nk * d_.nso * d_.nso;

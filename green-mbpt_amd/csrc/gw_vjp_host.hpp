@@ -144,11 +144,14 @@ void gw_hip_kernel::vjp_device(const double2* G, const double2* Sbar, double2* G
   if (!v_set_) throw status_error(GWHIP_ERR_STATE, "integrals not set (gwhip_upload_V / gwhip_generate_V)");
   if (!G || !Sbar || !Gbar) throw status_error(GWHIP_ERR_ARG, "null G, Sigma_bar or G_bar");
   if (mode != GWHIP_VJP_RECOMPUTE && mode != GWHIP_VJP_STORED) throw status_error(GWHIP_ERR_ARG, "unknown VJP mode");
+  if (d_.nao > kOrbMaxNao)
+    throw status_error(GWHIP_ERR_UNSUPPORTED, "the VJP's orbital kernels handle nao <= " + std::to_string(kOrbMaxNao));
   const bool stored = mode == GWHIP_VJP_STORED;
   if (stored && !stored_valid_)
     throw status_error(GWHIP_ERR_STATE, "no stored forward pass (gwhip_set_vjp_store(h, 1), then gwhip_eval_sigma)");
   GWHIP_CHECK(hipSetDevice(device_));
   ensure_vjp_buffers();
+  ensure_orb_buffers();
   detect_hermitian_G(G);
 
   const int64_t nk = d_.nk, NQ = d_.NQ, nao = d_.nao, nso = d_.nso, nao2 = nao * nao, NQ2 = NQ * NQ;

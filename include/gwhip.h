@@ -136,8 +136,10 @@ int gwhip_autotune(gwhip_handle* h, const void* G_dev, int precision);
  * tile, [2] aux indices per wave, [3] int8 P0 GEMM row-tile group, [4] int8 Sigma GEMM row-tile
  * group, [5] int8 P0 GEMM block tile (0: 32x64, 1: 64x64, 2: 32x64 software-pipelined), [6] int8
  * Sigma GEMM block tile,
- * [7] split P0 orbital kernel writes the int8 slices itself (1) or via a separate pass (0). */
-#define GWHIP_TUNING_COUNT 8
+ * [7] split P0 orbital kernel writes the int8 slices itself (1) or via a separate pass (0),
+ * [8] orbital-size products (Z = Gb V Gq of P0, V G R of Sigma) as batched FP64 GEMMs (1) or the
+ * register-chained MFMA kernels (0); always 1 in effect for nao > 48. */
+#define GWHIP_TUNING_COUNT 9
 int gwhip_get_tuning(const gwhip_handle* h, int* values, int n);
 int gwhip_set_tuning(gwhip_handle* h, const int* values, int n);
 
@@ -173,7 +175,8 @@ int gwhip_set_vjp_store(gwhip_handle* h, int keep);
  * src/gw_cpu_kernel.cpp:31-373) at G for the cotangent Sigma_bar: writes G_bar with
  * Re<Sigma_bar, dSigma> = Re<G_bar, dG> for every dG (<a,b> = sum conj(a) b; the convention of
  * torch.autograd for complex tensors).  Host buffers, all [nts][ns][nk][nso][nso] complex.
- * FP64 throughout.  `mode` is a gwhip_vjp_mode.  No reference counterpart. */
+ * FP64 throughout.  `mode` is a gwhip_vjp_mode.  nao > 48: GWHIP_ERR_UNSUPPORTED.  No reference
+ * counterpart. */
 int gwhip_vjp(gwhip_handle* h, const double* G, const double* Sigma_bar, double* G_bar, int mode);
 
 /* As gwhip_vjp with device buffers.  Synchronous with respect to the host. */
@@ -190,7 +193,8 @@ double gwhip_vjp_ms(const gwhip_handle* h);
  * pref = 1/2 for ns = 1 and 1 for ns = 2.  X2C (nso = 2 nao, ns = 1): spin blocks aa, bb, ab with
  * pref 1, the direct term from dm_aa + dm_bb into aa and bb, the Madelung term with S_aa on both
  * sides, F_ba = F_ab^H.  dm is the density matrix the reference forms (spin prefactor times
- * G(beta), src/hf_solver.cpp:9-16).  Host buffers dm, S, F: [ns][nk][nso][nso] complex. FP64. */
+ * G(beta), src/hf_solver.cpp:9-16).  Host buffers dm, S, F: [ns][nk][nso][nso] complex. FP64.
+ * nao > 48: GWHIP_ERR_UNSUPPORTED. */
 int gwhip_hf(gwhip_handle* h, const double* dm, const double* S, double madelung, double* F);
 
 /* As gwhip_hf with device buffers.  Synchronous with respect to the host. */

@@ -1,7 +1,10 @@
 """HIP path vs the oracle, through the C ABI (needs an MI355X).
 
 Bar: FP64 Sigma within 1e-10 relative (max-abs difference over max |Sigma|) of the oracle
-on the same seeded inputs; the reference's own fixtures within its 1e-6 abs tolerance.
+on the same seeded inputs; the reference's own fixtures within its 1e-6 abs tolerance.  The
+orbital-size products have two forms -- register-chained MFMA kernels (nao <= 48) and batched
+FP64 GEMMs (any nao; tuning key orbital_gemm) -- and both are checked against the oracle,
+including nao = 50 (> 48, GEMM form only), X2C, unrestricted and split precision.
 """
 import numpy as np
 import pytest
@@ -146,3 +149,36 @@ def test_nonfinite_input_is_reported():
         assert e.value.status == gl.ERR_NONFINITE
     finally:
         k.close()
+
+
+@pytest.mark.parametrize("case", [
+    dict(nk=4, mesh=(2, 2, 1), nao=26, NQ=30, ns=1),              # the benchmark's nao, GEMM form forced
+    dict(nk=3, mesh=(3, 1, 1), nao=7, NQ=9, ns=2),                # unrestricted, GEMM form forced
+    dict(nk=2, mesh=(2, 1, 1), nao=50, NQ=20, ns=1),              # nao > 48: GEMM form only
+    dict(nk=2, mesh=(2, 1, 1), nao=50, NQ=8, ns=1, nso=100),      # X2C spin blocks at nao > 48
+])
+def test_orbital_gemm_form(case):
+    nso = case.get("nso", case["nao"])
+    G, V, T, d = synthetic(case["nk"], case["nao"], case["NQ"], ns=case["ns"], beta=20.0, lam=1e3,
+                           mesh=case["mesh"], seed=13, nso=nso, sigma=0.3)
+    k = _kernel(d, T)
+    try:
+        k.set_grid(T.T_bt, T.T_tb)
+        k.upload_V(V)
+        if case["nao"] <= 48:
+            S_mfma = k.solve(G)
+            k.set_tuning(orbital_gemm=1)
+        S = k.solve(G)
+        P = k.P_tilde()
+        S_split = k.solve(G, gl.PREC_SPLIT)
+    finally:
+        k.close()
+    S_ref, P_ref = _oracle(G, V, T, d)
+    msg = f"GEMM form {case}: P {rel_err(P, P_ref):.2e}, Sigma {rel_err(S, S_ref):.2e}, split {rel_err(S_split, S_ref):.2e}"
+    if case["nao"] <= 48:
+        msg += f", vs MFMA kernels {rel_err(S, S_mfma):.2e}"
+        assert rel_err(S, S_mfma) < 1e-12
+    print(msg, flush=True)
+    assert rel_err(P, P_ref) < TOL_FP64
+    assert rel_err(S, S_ref) < TOL_FP64
+    assert rel_err(S_split, S_ref) < 1e-8

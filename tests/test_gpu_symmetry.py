@@ -85,9 +85,9 @@ def _random_q_symmetry(nq, NQ, seed):
     return 3, b2i, i2b, U, tr
 
 
-@pytest.mark.parametrize("ns,nso_factor", [(2, 1), (1, 2)])
-def test_random_q_wedge_star_matches_oracle(ns, nso_factor):
-    """Unrestricted, and X2C spin blocks (nso = 2 nao)."""
+@pytest.mark.parametrize("ns,nso_factor,orb_gemm", [(2, 1, 0), (1, 2, 0), (2, 1, 1)])
+def test_random_q_wedge_star_matches_oracle(ns, nso_factor, orb_gemm):
+    """Unrestricted, X2C spin blocks (nso = 2 nao), and the GEMM form of the orbital products."""
     nso = 5 * nso_factor
     G, V, T, d = synthetic(8, 5, 12, ns=ns, seed=7, nso=nso)
     ink, kb2i, ki2b, Uk, ktr = _random_symmetry(8, nso, 8)
@@ -99,6 +99,7 @@ def test_random_q_wedge_star_matches_oracle(ns, nso_factor):
         k.upload_V(V)
         k.set_k_symmetry(ink, kb2i, ki2b, Uk, ktr)
         k.set_q_symmetry(inq, qb2i, qi2b, Uq, qtr)
+        k.set_tuning(orbital_gemm=orb_gemm)
         S = k.solve_ibz(G_ibz)
         S_split = k.solve_ibz(G_ibz, gl.PREC_SPLIT)
     finally:
@@ -106,7 +107,7 @@ def test_random_q_wedge_star_matches_oracle(ns, nso_factor):
     sym = dict(k_bz2ibz=kb2i, k_ibz2bz=ki2b, k_k_sym_transform_ao=Uk, k_tr_conj=ktr, q_bz2ibz=qb2i, q_ibz2bz=qi2b,
                q_k_sym_transform_p0=Uq, q_tr_conj=qtr)
     S_or = osy.gw_sigma_star(G_ibz, V, T.T_bt, T.T_tb, sym, 8, 5, nso, ns, 12, d["mesh"])
-    print(f"random q wedge (ns = {ns}, nso = {nso}), star of q: FP64 {rel_err(S, S_or):.2e}, "
+    print(f"random q wedge (ns = {ns}, nso = {nso}, GEMM form {orb_gemm}), star of q: FP64 {rel_err(S, S_or):.2e}, "
           f"split {rel_err(S_split, S_or):.2e}", flush=True)
     assert rel_err(S, S_or) < 1e-10
     assert rel_err(S_split, S_or) < 1e-8
