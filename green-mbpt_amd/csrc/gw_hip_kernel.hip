@@ -127,7 +127,7 @@ gw_hip_kernel::~gw_hip_kernel() {
                   (void*)kplus_tab_, (void*)kminus_tab_, (void*)flag_, (void*)hcheck_, (void*)hf_work_, (void*)hf_x_, (void*)ksym_idx_, (void*)ksym_U_, (void*)hf_sym_, (void*)q0_qiq_, (void*)q0_A_, (void*)q0_eps_dev_,
                   (void*)q0_S_, (void*)q0_tmp_, (void*)q0_epst_, (void*)qsym_U_, (void*)star_P_, (void*)star_V_,
                   (void*)star_Vs_, (void*)star_eVs_, (void*)star_km_, (void*)star_q0row_, (void*)orb_W_,
-                  (void*)orb_G_})
+                  (void*)orb_G_, (void*)orb_part_})
     if (p) (void)hipFree(p);
   if (own_ws_)
     for (void* p : {(void*)Pw_, (void*)X_, (void*)Y_, (void*)scratch_})
@@ -789,6 +789,7 @@ void gw_hip_kernel::ensure_orb_buffers() {
   if (!orb_gemm()) return;
   if (!orb_W_) orb_W_ = dalloc<double2>((size_t)d_.nk * d_.NQ * d_.nao * d_.nao);
   if (!orb_G_) orb_G_ = dalloc<double2>((size_t)d_.nk * d_.nso * d_.nso);
+  if (!orb_part_) orb_part_ = dalloc<double2>((size_t)kOrbKsplit * d_.nk * d_.nao * d_.nao);
 }
 
 // orb_G_[k] = G(tau, is, map[k]) for k < n.
@@ -865,7 +866,10 @@ void gw_hip_kernel::sigma_epi_gemm(const double2* G, double2* Sigma, const doubl
         y.sCno = 0; y.sCni = 1;
         y.alpha = make_double2(scale, 0.0);
         y.accumulate = 1;
-        zgemm_f64(y, false, false, stream_);
+        // only nkb x (nao / 64)^2 output tiles, each over K = NQ nao: cut K so that ~1024 blocks run
+        const int64_t tiles = nkb * ceil_div(nao, 64) * ceil_div(nao, 64);
+        const int ksplit = (int)std::min<int64_t>(kOrbKsplit, std::max<int64_t>(1, ceil_div(1024, tiles)));
+        zgemm_f64_splitk(y, ksplit, orb_part_, false, false, stream_);
       }
     }
 }

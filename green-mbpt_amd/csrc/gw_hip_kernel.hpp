@@ -245,6 +245,8 @@ private:
   double2* q0_epst_ = nullptr;      // [nts]
   double2* orb_W_ = nullptr;  // GEMM form of the orbital contractions: [nk][NQ][nao^2] intermediate
   double2* orb_G_ = nullptr;  // and one (tau, spin) slice of G at mapped k points, [nk][nso^2]
+  double2* orb_part_ = nullptr;  // split-K partial Sigma blocks, [kOrbKsplit][nk][nao^2]
+  static constexpr int64_t kOrbKsplit = 64;
   int* ksym_idx_ = nullptr;
   double2* ksym_U_ = nullptr;
   std::vector<int> ksym_i2b_;       // host copy of ibz2bz

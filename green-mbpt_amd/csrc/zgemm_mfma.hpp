@@ -37,8 +37,11 @@ struct ZGemmArgs {
   int conjA, conjB;
   int upper;       // 1: skip block tiles that lie wholly below the diagonal (Hermitian products)
   int tiles_m, tiles_n;
-  int64_t nblocks;   // batch * tiles_m * tiles_n
+  int64_t nblocks;   // batch * tiles_m * tiles_n * ksplit
   int64_t per_xcd;   // ceil(nblocks / 8)
+  int ksplit;        // K cut into ksplit slices of kslice (a multiple of the K step), one block each;
+  int kslice;        // slice s writes its partial product at C + s * sCk (0 or 1: no split)
+  int64_t sCk;
 };
 
 constexpr int ZG_BK = 16, ZG_PAD = 2;
@@ -74,8 +77,17 @@ __global__ __launch_bounds__(256) void zgemm_f64_kernel(ZGemmArgs p) {
   // XCD-aware tile order: consecutive hardware workgroup ids go to different XCDs, so give
   // each XCD a contiguous range of logical tiles (tiles of one batch entry share operands).
   const int64_t hw = blockIdx.x;
-  const int64_t logical = (hw % 8) * p.per_xcd + hw / 8;
+  int64_t logical = (hw % 8) * p.per_xcd + hw / 8;
   if (logical >= p.nblocks) return;
+  int kbeg = 0, kend = p.K;
+  int64_t cks = 0;
+  if (p.ksplit > 1) {  // K slice of this block
+    const int64_t ks = logical % p.ksplit;
+    logical /= p.ksplit;
+    kbeg = (int)ks * p.kslice;
+    kend = kbeg + p.kslice < p.K ? kbeg + p.kslice : p.K;
+    cks = ks * p.sCk;
+  }
   const int tiles = p.tiles_m * p.tiles_n;
   const int64_t b = logical / tiles;
   const int rem = (int)(logical - b * tiles);
@@ -85,7 +97,7 @@ __global__ __launch_bounds__(256) void zgemm_f64_kernel(ZGemmArgs p) {
 
   const double2* __restrict__ A = p.A + b * p.sAb;
   const double2* __restrict__ B = p.B + b * p.sBb;
-  double2* __restrict__ C = p.C + b * p.sCb;
+  double2* __restrict__ C = p.C + b * p.sCb + cks;
 
   const int tid = threadIdx.x;
   const int lane = tid & 63, wave = tid >> 6;
@@ -126,7 +138,7 @@ __global__ __launch_bounds__(256) void zgemm_f64_kernel(ZGemmArgs p) {
     const int m = m0 + a_m[j];
     a_ok[j] = m < p.M;
     a_moff[j] = (int64_t)(a_ok[j] ? m : 0) * p.sAm;
-    ka[j].init(a_k[j], p.kin, p.sAko, p.sAki);
+    ka[j].init(kbeg + a_k[j], p.kin, p.sAko, p.sAki);
   }
 #pragma unroll
   for (int j = 0; j < EB; ++j) {
@@ -135,21 +147,21 @@ __global__ __launch_bounds__(256) void zgemm_f64_kernel(ZGemmArgs p) {
     const int nn = b_ok[j] ? n : 0;
     const int no = nn / p.nin;
     b_noff[j] = (int64_t)no * p.sBno + (int64_t)(nn - no * p.nin) * p.sBni;
-    kb[j].init(b_k[j], p.kin, p.sBko, p.sBki);
+    kb[j].init(kbeg + b_k[j], p.kin, p.sBko, p.sBki);
   }
 
   double2 ra[EA], rb[EB];
   auto load_tiles = [&](int k0) {
-    const bool full = k0 + BK <= p.K;
+    const bool full = k0 + BK <= kend;
 #pragma unroll
     for (int j = 0; j < EA; ++j) {
-      const bool ok = a_ok[j] && (full || k0 + a_k[j] < p.K);
+      const bool ok = a_ok[j] && (full || k0 + a_k[j] < kend);
       ra[j] = ok ? A[a_moff[j] + ka[j].off] : make_double2(0.0, 0.0);
       ka[j].advance(p.kin, p.sAko, p.sAki);
     }
 #pragma unroll
     for (int j = 0; j < EB; ++j) {
-      const bool ok = b_ok[j] && (full || k0 + b_k[j] < p.K);
+      const bool ok = b_ok[j] && (full || k0 + b_k[j] < kend);
       rb[j] = ok ? B[kb[j].off + b_noff[j]] : make_double2(0.0, 0.0);
       kb[j].advance(p.kin, p.sBko, p.sBki);
     }
@@ -178,12 +190,12 @@ __global__ __launch_bounds__(256) void zgemm_f64_kernel(ZGemmArgs p) {
   const int wm = (wave >> 1) * WM, wn = (wave & 1) * WN;
   const int lrow = lane & 15, lk = lane >> 4;
 
-  load_tiles(0);
-  for (int k0 = 0; k0 < p.K; k0 += BK) {
+  load_tiles(kbeg);
+  for (int k0 = kbeg; k0 < kend; k0 += BK) {
     __syncthreads();
     store_tiles();
     __syncthreads();
-    if (k0 + BK < p.K) load_tiles(k0 + BK);
+    if (k0 + BK < kend) load_tiles(k0 + BK);
 #pragma unroll
     for (int ks = 0; ks < BK; ks += 4) {
       const int kr = ks + lk;
@@ -263,10 +275,12 @@ inline void zgemm_launch_shape(const ZGemmArgs& p, bool a_kc, bool b_kc, dim3 g,
 inline void zgemm_f64(ZGemmArgs p, bool a_kc, bool b_kc, hipStream_t stream, int shape = 0) {
   if (p.M <= 0 || p.N <= 0 || p.batch <= 0) return;
   if (p.kin <= 0 || p.nin <= 0) throw std::runtime_error("zgemm_f64: kin/nin must be positive");
+  if (p.ksplit > 1 && (p.kslice <= 0 || p.kslice % ZG_BK != 0 || (int64_t)p.ksplit * p.kslice < p.K))
+    throw std::runtime_error("zgemm_f64: bad K split");
   const int BM = shape == 1 ? 128 : 64, BN = shape == 2 ? 128 : 64;
   p.tiles_m = (int)ceil_div(p.M, BM);
   p.tiles_n = (int)ceil_div(p.N, BN);
-  p.nblocks = (int64_t)p.batch * p.tiles_m * p.tiles_n;
+  p.nblocks = (int64_t)p.batch * p.tiles_m * p.tiles_n * (p.ksplit > 1 ? p.ksplit : 1);
   p.per_xcd = ceil_div(p.nblocks, 8);
   const int64_t grid = p.per_xcd * 8;
   if (grid > 0x7fffffffLL) throw std::runtime_error("zgemm_f64: grid too large");
@@ -277,6 +291,61 @@ inline void zgemm_f64(ZGemmArgs p, bool a_kc, bool b_kc, hipStream_t stream, int
     zgemm_launch_shape<64, 128>(p, a_kc, b_kc, g, stream);
   else
     zgemm_launch_shape<64, 64>(p, a_kc, b_kc, g, stream);
+  GWHIP_CHECK(hipGetLastError());
+}
+
+// Sum of the ksplit partial products Part[s][b][m][n] (n fastest) into
+// C(b, m, n) = alpha * sum + (accumulate ? C : 0), with C addressed as in ZGemmArgs.
+__global__ void zgemm_splitk_reduce_kernel(ZGemmArgs p, const double2* __restrict__ part) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const int64_t mn = (int64_t)p.M * p.N;
+  if (e >= mn * p.batch) return;
+  const int64_t b = e / mn;
+  const int m = (int)((e - b * mn) / p.N), n = (int)(e - b * mn - (int64_t)m * p.N);
+  double sr = 0.0, si = 0.0;
+  for (int s = 0; s < p.ksplit; ++s) {
+    const double2 v = part[s * p.sCk + e];
+    sr += v.x;
+    si += v.y;
+  }
+  const int no = n / p.nin;
+  double2* dst = p.C + b * p.sCb + (int64_t)m * p.sCm + (int64_t)no * p.sCno + (int64_t)(n - no * p.nin) * p.sCni;
+  double2 v = make_double2(p.alpha.x * sr - p.alpha.y * si, p.alpha.x * si + p.alpha.y * sr);
+  if (p.accumulate) {
+    const double2 o = *dst;
+    v.x += o.x;
+    v.y += o.y;
+  }
+  *dst = v;
+}
+
+// C (+)= alpha A B with K cut into `ksplit` slices computed by separate blocks (for few, long-K
+// products); part holds ksplit * batch * M * N elements.  The slices are summed in slice order,
+// so the result does not depend on the block schedule.
+inline void zgemm_f64_splitk(ZGemmArgs p, int ksplit, double2* part, bool a_kc, bool b_kc, hipStream_t stream) {
+  if (p.M <= 0 || p.N <= 0 || p.batch <= 0) return;
+  if (ksplit < 2) {
+    zgemm_f64(p, a_kc, b_kc, stream);
+    return;
+  }
+  ZGemmArgs q = p;
+  q.kslice = (int)(ceil_div(ceil_div(p.K, ksplit), ZG_BK) * ZG_BK);
+  q.ksplit = (int)ceil_div(p.K, q.kslice);
+  q.sCk = (int64_t)p.batch * p.M * p.N;
+  q.C = part;
+  q.sCb = (int64_t)p.M * p.N;
+  q.sCm = p.N;
+  q.sCno = p.nin;
+  q.sCni = 1;
+  q.alpha = make_double2(1.0, 0.0);
+  q.accumulate = 0;
+  q.upper = 0;
+  zgemm_f64(q, a_kc, b_kc, stream);
+  ZGemmArgs r = p;
+  r.ksplit = q.ksplit;
+  r.sCk = q.sCk;
+  const int64_t n = (int64_t)p.batch * p.M * p.N;
+  hipLaunchKernelGGL(zgemm_splitk_reduce_kernel, dim3((unsigned)ceil_div(n, 256)), dim3(256), 0, stream, r, part);
   GWHIP_CHECK(hipGetLastError());
 }
 
