@@ -127,9 +127,8 @@ void block_gemm(hipStream_t st, int n, int64_t nb, const double2* A, const doubl
 void gw_hip_kernel::hf_device(const double2* dm, const double2* S, double madelung, double2* F) {
   if (!v_set_) throw status_error(GWHIP_ERR_STATE, "integrals not set (gwhip_upload_V / gwhip_generate_V)");
   if (!dm || !S || !F) throw status_error(GWHIP_ERR_ARG, "null dm, S or F");
-  if (d_.nao > kOrbMaxNao)
-    throw status_error(GWHIP_ERR_UNSUPPORTED, "the Hartree-Fock kernels handle nao <= " + std::to_string(kOrbMaxNao));
   GWHIP_CHECK(hipSetDevice(device_));
+  ensure_orb_buffers();
   const int64_t nk = d_.nk, NQ = d_.NQ, nao = d_.nao, nso = d_.nso, nao2 = nao * nao, nso2 = nso * nso;
   const bool x2c = d_.X2C;
   const int64_t nspin = d_.ns;  // 1 for X2C
@@ -171,7 +170,16 @@ void gw_hip_kernel::hf_device(const double2* dm, const double2* S, double madelu
   oa.kmap_st = nk;
   for (int64_t s = 0; s < nspin; ++s) {
     zero(hf_x_, nblk * nk * nk * nao2);
-    for (int bl = 0; bl < nblk; ++bl) {
+    if (orb_gemm()) {
+      GWHIP_CHECK(hipEventRecord(ex0, stream_));
+      hf_exchange_gemm(dm + s * nk * nso2, nblk);
+      GWHIP_CHECK(hipEventRecord(ex1, stream_));
+      GWHIP_CHECK(hipEventSynchronize(ex1));
+      float m = 0.f;
+      GWHIP_CHECK(hipEventElapsedTime(&m, ex0, ex1));
+      ex_ms += m;
+    }
+    for (int bl = 0; bl < nblk && !orb_gemm(); ++bl) {
       const int64_t ro = (bl == 1) ? nao : 0, co = (bl == 0) ? 0 : nao;  // aa, bb, ab
       const OrbOp M{dm + s * nk * nso2 + ro * nso + co, 0, nso2, 0, nso, 1, 1, 0};
       const OrbOut O{hf_x_ + bl * nk * nk * nao2, nk * nao2, nao2, 0, nao, 1, 0};
@@ -212,6 +220,37 @@ void gw_hip_kernel::hf_device(const double2* dm, const double2* S, double madelu
   (void)hipEventDestroy(e1);
   (void)hipEventDestroy(ex0);
   (void)hipEventDestroy(ex1);
+}
+
+// The exchange products X[blk][q][k] = sum_Q V(k, k+q)_Q dm(k+q)_blk V(k, k+q)_Q^H as batched FP64
+// GEMMs over k per q (any nao): Y = V_Q dm with the rows (Q, i) stacked, then X = sum_{Q, b} Y V^H
+// with K = (Q, b) cut into slices (zgemm_f64_splitk).  dm_s: one spin's [nk][nso^2].
+void gw_hip_kernel::hf_exchange_gemm(const double2* dm_s, int nblk) {
+  const int64_t nk = d_.nk, NQ = d_.NQ, nao = d_.nao, nao2 = nao * nao, nso = d_.nso, nso2 = nso * nso;
+  const int64_t vq = NQ * nao2;
+  const int64_t tiles = nk * ceil_div(nao, 64) * ceil_div(nao, 64);
+  const int ksplit = (int)std::min<int64_t>(kOrbKsplit, std::max<int64_t>(1, ceil_div(1024, tiles)));
+  for (int64_t q = 0; q < nk; ++q) {
+    gather_mats(dm_s, kplus_tab_ + q * nk, nk);  // dm(k + q)
+    for (int bl = 0; bl < nblk; ++bl) {
+      const int64_t ro = (bl == 1) ? nao : 0, co = (bl == 0) ? 0 : nao;  // aa, bb, ab
+      ZGemmArgs y{};
+      y.M = (int)(NQ * nao); y.N = (int)nao; y.K = (int)nao; y.batch = (int)nk; y.kin = (int)nao; y.nin = (int)nao;
+      y.A = Vq_ + q * nk * vq; y.sAb = vq; y.sAm = nao; y.sAko = 0; y.sAki = 1;
+      y.B = orb_G_ + ro * nso + co; y.sBb = nso2; y.sBko = 0; y.sBki = nso; y.sBno = 0; y.sBni = 1;
+      y.C = orb_W_; y.sCb = vq; y.sCm = nao; y.sCno = 0; y.sCni = 1;
+      y.alpha = make_double2(1.0, 0.0);
+      zgemm_f64(y, true, false, stream_);
+      ZGemmArgs x{};
+      x.M = (int)nao; x.N = (int)nao; x.K = (int)(NQ * nao); x.batch = (int)nk; x.kin = (int)nao; x.nin = (int)nao;
+      x.A = orb_W_; x.sAb = vq; x.sAm = nao; x.sAko = nao2; x.sAki = 1;
+      x.B = Vq_ + q * nk * vq; x.sBb = vq; x.sBko = nao2; x.sBki = 1; x.sBno = 0; x.sBni = nao;
+      x.conjB = 1;
+      x.C = hf_x_ + (bl * nk + q) * nk * nao2; x.sCb = nao2; x.sCm = nao; x.sCno = 0; x.sCni = 1;
+      x.alpha = make_double2(1.0, 0.0);
+      zgemm_f64_splitk(x, ksplit, orb_part_, true, true, stream_);
+    }
+  }
 }
 
 void gw_hip_kernel::hf_host(const double2* dm, const double2* S, double madelung, double2* F) {

@@ -794,10 +794,15 @@ void gw_hip_kernel::ensure_orb_buffers() {
 
 // orb_G_[k] = G(tau, is, map[k]) for k < n.
 void gw_hip_kernel::gather_g(const double2* G, int64_t tau, int64_t is, const int* map, int64_t n) {
+  gather_mats(G + (tau * d_.ns + is) * d_.nk * d_.nso * d_.nso, map, n);
+}
+
+// orb_G_[k] = base[map[k]] (nso x nso matrices) for k < n.
+void gw_hip_kernel::gather_mats(const double2* base, const int* map, int64_t n) {
   const int64_t nso2 = d_.nso * d_.nso;
-  if (n < 1 || n > 65535) throw status_error(GWHIP_ERR_UNSUPPORTED, "gather_g: k count out of range");
-  hipLaunchKernelGGL(gather_g_kernel, dim3((unsigned)ceil_div(nso2, 256), (unsigned)n), dim3(256), 0, stream_,
-                     G + (tau * d_.ns + is) * d_.nk * nso2, orb_G_, map, nso2);
+  if (n < 1 || n > 65535) throw status_error(GWHIP_ERR_UNSUPPORTED, "gather: k count out of range");
+  hipLaunchKernelGGL(gather_g_kernel, dim3((unsigned)ceil_div(nso2, 256), (unsigned)n), dim3(256), 0, stream_, base,
+                     orb_G_, map, nso2);
   GWHIP_CHECK(hipGetLastError());
 }
 

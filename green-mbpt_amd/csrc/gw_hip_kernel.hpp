@@ -293,6 +293,8 @@ private:
   void sigma_epi_gemm(const double2* G, double2* Sigma, const double2* Vb, const int* km, int64_t nkb, int64_t t0,
                       int64_t tc, double scale);
   void gather_g(const double2* G, int64_t tau, int64_t is, const int* map, int64_t n);
+  void gather_mats(const double2* base, const int* map, int64_t n);
+  void hf_exchange_gemm(const double2* dm_s, int nblk);
   void p0_prep_slice(int64_t q, const double2* G, int64_t t0, int64_t tc, int qpw);
   void p0_gemm(int64_t q, int64_t tc, double2* Pt0, int shape);
   void sigma_gemm(const double2* Pt0, const double2* Vb, int64_t nkb, int64_t tc, int shape);

@@ -193,8 +193,7 @@ double gwhip_vjp_ms(const gwhip_handle* h);
  * pref = 1/2 for ns = 1 and 1 for ns = 2.  X2C (nso = 2 nao, ns = 1): spin blocks aa, bb, ab with
  * pref 1, the direct term from dm_aa + dm_bb into aa and bb, the Madelung term with S_aa on both
  * sides, F_ba = F_ab^H.  dm is the density matrix the reference forms (spin prefactor times
- * G(beta), src/hf_solver.cpp:9-16).  Host buffers dm, S, F: [ns][nk][nso][nso] complex. FP64.
- * nao > 48: GWHIP_ERR_UNSUPPORTED. */
+ * G(beta), src/hf_solver.cpp:9-16).  Host buffers dm, S, F: [ns][nk][nso][nso] complex. FP64. */
 int gwhip_hf(gwhip_handle* h, const double* dm, const double* S, double madelung, double* F);
 
 /* As gwhip_hf with device buffers.  Synchronous with respect to the host. */

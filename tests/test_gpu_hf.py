@@ -7,7 +7,9 @@ HF_X2C fixture and a brute-force einsum in test_hf_oracle.py):
 * random ns = 1, ns = 2 and X2C systems with a non-zero Madelung constant, up to nk = 27
   (3x3x3), nao = 26, naux = 250: 1e-10 relative (FP64 throughout);
 * the device-pointer entry point gives the same F as the host one, and a second call on the
-  same handle (scratch reused) gives it again.
+  same handle (scratch reused) gives it again;
+* the exchange as batched FP64 GEMMs (tuning key orbital_gemm; always so for nao > 48): against
+  the register-chained kernels at small nao, and against the oracle at nao = 50 (ns = 1, X2C).
 """
 import os
 
@@ -91,3 +93,27 @@ def test_hf_device_pointers_and_reuse():
         k.close()
     for F in out:
         assert np.array_equal(F, F_host)
+
+
+@pytest.mark.parametrize("c", [
+    dict(nk=4, mesh=(2, 2, 1), nao=9, nso=9, ns=2, NQ=21, gemm_only=False),
+    dict(nk=2, mesh=(2, 1, 1), nao=7, nso=14, ns=1, NQ=12, gemm_only=False),
+    dict(nk=2, mesh=(2, 1, 1), nao=50, nso=50, ns=1, NQ=16, gemm_only=True),
+    dict(nk=2, mesh=(2, 1, 1), nao=50, nso=100, ns=1, NQ=6, gemm_only=True),
+])
+def test_hf_gemm_form(c):
+    dm, Vp, S, mad = random_system(c["nk"], c["nao"], c["nso"], c["ns"], c["NQ"], 300 + c["nao"])
+    F_ref = ho.hf_fock(dm, Vp, S, mad, c["nk"], c["nao"], c["nso"], c["ns"], c["NQ"])
+    k = _kernel(c["nk"], c["nao"], c["nso"], c["ns"], c["NQ"], mesh=c["mesh"])
+    try:
+        k.upload_V(Vp)
+        if not c["gemm_only"]:
+            F_kern = k.hf(dm, S, mad)
+            k.set_tuning(orbital_gemm=1)
+        F = k.hf(dm, S, mad)
+    finally:
+        k.close()
+    print(f"HF GEMM form {c}: rel err {rel_err(F, F_ref):.2e}", flush=True)
+    assert rel_err(F, F_ref) < 1e-10
+    if not c["gemm_only"]:
+        assert rel_err(F, F_kern) < 1e-13
