@@ -19,20 +19,26 @@ struct DysonDims {
   int64_t nts = 0, nw = 0, ns = 0, nk = 0, ink = 0, nao = 0, nso = 0;
 };
 
+// Largest nso whose matrix the inverse keeps in LDS; larger ones are eliminated in global memory.
 constexpr int DYSON_MAX_N = 96;
 
 // M = z S - HS1 - Sw with z = i w_n + mu; TRACE = false: write M^-1; TRACE = true: write
-// wk * tr(M^-1 S).  grid (nsk, nw); dynamic LDS (n^2 + 2n) double2.
-template <bool TRACE>
+// wk * tr(M^-1 S).  grid (nsk, nw).  GLOB = false: the matrix sits in LDS (dynamic LDS n^2 + 2n
+// double2 + n int); GLOB = true (any n): it is eliminated in place in global memory, in `out`
+// itself (TRACE = false) or in `work` (TRACE = true), both [nw][nsk][n^2] (dynamic LDS 2n double2
+// + n int).  Same arithmetic in both.
+template <bool TRACE, bool GLOB>
 __global__ __launch_bounds__(256) void dyson_invert_kernel(const double2* __restrict__ HS1, const double2* __restrict__ S,
                                                            const double2* __restrict__ Sw, const double* __restrict__ wn,
                                                            double mu, int n, int nsk, int ink,
-                                                           const double* __restrict__ wk, double2* __restrict__ out) {
+                                                           const double* __restrict__ wk, double2* __restrict__ out,
+                                                           double2* __restrict__ work) {
   extern __shared__ double2 dl[];
-  double2* A = dl;
-  double2* colp = dl + n * n;
+  const int64_t mat = ((int64_t)blockIdx.y * nsk + blockIdx.x) * n * n;
+  double2* A = GLOB ? (TRACE ? work : out) + mat : dl;
+  double2* colp = GLOB ? dl : dl + n * n;
   double2* rowp = colp + n;
-  __shared__ int perm[DYSON_MAX_N];
+  int* perm = reinterpret_cast<int*>(rowp + n);
   __shared__ int piv;
   __shared__ double red[2][256];
   const int sk = blockIdx.x, w = blockIdx.y, tid = threadIdx.x;
@@ -103,8 +109,10 @@ __global__ __launch_bounds__(256) void dyson_invert_kernel(const double2* __rest
     __syncthreads();
   }
   if (!TRACE) {
-    double2* o = out + ((int64_t)w * nsk + sk) * n2;
-    for (int e = tid; e < n2; e += blockDim.x) o[e] = A[e];
+    if (!GLOB) {
+      double2* o = out + ((int64_t)w * nsk + sk) * n2;
+      for (int e = tid; e < n2; e += blockDim.x) o[e] = A[e];
+    }
     return;
   }
   double tr = 0.0, ti = 0.0;  // tr(A S) = sum_ij A_ij S_ji
@@ -143,7 +151,6 @@ public:
         (d.nso != d.nao && d.nso != 2 * d.nao))
       throw status_error(GWHIP_ERR_ARG, "bad Dyson dimensions");
     if (d.nso == 2 * d.nao && d.ns != 1) throw status_error(GWHIP_ERR_ARG, "X2C requires ns == 1");
-    if (d.nso > DYSON_MAX_N) throw status_error(GWHIP_ERR_UNSUPPORTED, "nso > 96 not supported by the Dyson inverse");
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || device < 0 || device >= count) {
       (void)hipGetLastError();
@@ -347,17 +354,23 @@ private:
     stack_transform(stream_, Tft_, (int)d_.nw, (int)d_.nts, false, Sigma_tau, N, Sw_);
   }
 
-  template <bool TRACE>
-  void launch_invert(double mu, double2* out) {
+  template <bool TRACE, bool GLOB>
+  void launch_invert_t(double mu, double2* out) {
     const int n = (int)d_.nso;
-    const size_t lds = (size_t)(n * n + 2 * n) * sizeof(double2);
-    if (lds > 65536) GWHIP_CHECK(hipFuncSetAttribute((const void*)dyson_invert_kernel<TRACE>,
+    const size_t lds = (size_t)((GLOB ? 0 : n * n) + 2 * n) * sizeof(double2) + (size_t)n * sizeof(int);
+    if (lds > 65536) GWHIP_CHECK(hipFuncSetAttribute((const void*)dyson_invert_kernel<TRACE, GLOB>,
                                                      hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds));
     const int nsk = (int)(d_.ns * d_.ink);
-    if (d_.nw > 65535) throw status_error(GWHIP_ERR_UNSUPPORTED, "more than 65535 Matsubara frequencies");
-    hipLaunchKernelGGL((dyson_invert_kernel<TRACE>), dim3((unsigned)nsk, (unsigned)d_.nw), dim3(256), lds, stream_, HS1_,
-                       S_, Sw_, wn_, mu, n, nsk, (int)d_.ink, wk_, out);
+    if (d_.nw > 65535 || nsk > 0x7fffffff) throw status_error(GWHIP_ERR_UNSUPPORTED, "too many Dyson matrices");
+    // the global-memory form works in Gw_ ([nw][nsk][n^2]), which TRACE mode does not otherwise use
+    hipLaunchKernelGGL((dyson_invert_kernel<TRACE, GLOB>), dim3((unsigned)nsk, (unsigned)d_.nw), dim3(256), lds, stream_,
+                       HS1_, S_, Sw_, wn_, mu, n, nsk, (int)d_.ink, wk_, out, Gw_);
     GWHIP_CHECK(hipGetLastError());
+  }
+  template <bool TRACE>
+  void launch_invert(double mu, double2* out) {
+    if (d_.nso <= DYSON_MAX_N) launch_invert_t<TRACE, false>(mu, out);
+    else launch_invert_t<TRACE, true>(mu, out);
   }
 
   void invert(double mu, bool trace) {

@@ -269,7 +269,7 @@ typedef struct gwhip_dyson_dims {
   int64_t nk;    /* k points of the full mesh             */
   int64_t ink;   /* irreducible k points (nk without symmetry) */
   int64_t nao;   /* atomic orbitals                       */
-  int64_t nso;   /* nao, or 2 nao for X2C (at most 96)    */
+  int64_t nso;   /* nao, or 2 nao for X2C                 */
 } gwhip_dyson_dims;
 
 /* Replaces: dyson<G, S1, St>::dyson (src/dyson.cpp:19-46) minus the file reading. */

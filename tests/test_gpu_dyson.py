@@ -5,7 +5,8 @@ fixtures in test_dyson_oracle.py):
 * the four HF_orth_invariance fixtures: G(tau) against the stored G (reference tolerance 1e-6) and
   against the oracle at 1e-10 relative; find_mu from 0 with tolerance 1e-9 returns the stored -0.25;
 * random systems with a non-zero Sigma(tau): ns = 2 with three weighted irreducible points, X2C,
-  and nso = 96 (the largest the LDS inverse takes): G and N(mu) against the oracle.
+  nso = 96 (the largest the LDS inverse takes) and nso = 100, 130 (eliminated in global memory):
+  G and N(mu) against the oracle.
 """
 import os
 
@@ -68,7 +69,9 @@ def _random(ns, ink, nao, nso, seed, nts, S_scale=0.1):
 
 @pytest.mark.parametrize("case", [dict(ns=2, nk=8, ink=3, nao=6, nso=6, w=[1.0, 3.0, 4.0]),
                                   dict(ns=1, nk=4, ink=4, nao=5, nso=10, w=None),
-                                  dict(ns=1, nk=1, ink=1, nao=48, nso=96, w=None)])
+                                  dict(ns=1, nk=1, ink=1, nao=48, nso=96, w=None),
+                                  dict(ns=1, nk=2, ink=1, nao=50, nso=100, w=[2.0]),
+                                  dict(ns=2, nk=1, ink=1, nao=130, nso=130, w=None)])
 def test_random_systems(case):
     z, T, wn = _fixture()
     H, S, S1, St = _random(case["ns"], case["ink"], case["nao"], case["nso"], 7, T.nts)
