@@ -127,7 +127,7 @@ gw_hip_kernel::~gw_hip_kernel() {
                   (void*)kplus_tab_, (void*)kminus_tab_, (void*)flag_, (void*)hcheck_, (void*)hf_work_, (void*)hf_x_, (void*)ksym_idx_, (void*)ksym_U_, (void*)hf_sym_, (void*)q0_qiq_, (void*)q0_A_, (void*)q0_eps_dev_,
                   (void*)q0_S_, (void*)q0_tmp_, (void*)q0_epst_, (void*)qsym_U_, (void*)star_P_, (void*)star_V_,
                   (void*)star_Vs_, (void*)star_eVs_, (void*)star_km_, (void*)star_q0row_, (void*)orb_W_,
-                  (void*)orb_G_, (void*)orb_part_})
+                  (void*)orb_G_, (void*)orb_part_, (void*)orb_O_})
     if (p) (void)hipFree(p);
   if (own_ws_)
     for (void* p : {(void*)Pw_, (void*)X_, (void*)Y_, (void*)scratch_})
@@ -790,6 +790,7 @@ void gw_hip_kernel::ensure_orb_buffers() {
   if (!orb_W_) orb_W_ = dalloc<double2>((size_t)d_.nk * d_.NQ * d_.nao * d_.nao);
   if (!orb_G_) orb_G_ = dalloc<double2>((size_t)d_.nk * d_.nso * d_.nso);
   if (!orb_part_) orb_part_ = dalloc<double2>((size_t)kOrbKsplit * d_.nk * d_.nao * d_.nao);
+  if (!orb_O_) orb_O_ = dalloc<double2>((size_t)d_.nk * d_.nao * d_.nao);
 }
 
 // orb_G_[k] = G(tau, is, map[k]) for k < n.

@@ -74,6 +74,10 @@ struct Workspace {
   size_t exp_elems = 0;
 };
 
+struct OrbOp;
+struct OrbOut;
+struct OrbArgs;
+
 class gw_hip_kernel {
 public:
   enum Region { R_P0 = 0, R_TW = 1, R_BSE = 2, R_WT = 3, R_SIGMA = 4, R_TOTAL = 5, R_COUNT = 6 };
@@ -246,6 +250,7 @@ private:
   double2* orb_W_ = nullptr;  // GEMM form of the orbital contractions: [nk][NQ][nao^2] intermediate
   double2* orb_G_ = nullptr;  // and one (tau, spin) slice of G at mapped k points, [nk][nso^2]
   double2* orb_part_ = nullptr;  // split-K partial Sigma blocks, [kOrbKsplit][nk][nao^2]
+  double2* orb_O_ = nullptr;     // one reduction result per k before it is added to its output, [nk][nao^2]
   static constexpr int64_t kOrbKsplit = 64;
   int* ksym_idx_ = nullptr;
   double2* ksym_U_ = nullptr;
@@ -295,6 +300,12 @@ private:
   void gather_g(const double2* G, int64_t tau, int64_t is, const int* map, int64_t n);
   void gather_mats(const double2* base, const int* map, int64_t n);
   void hf_exchange_gemm(const double2* dm_s, int nblk);
+  // the VJP's orbital products: register-chained kernels, or batched GEMMs (orb_gemm())
+  void vjp_sandwich(bool has_l, bool has_r, int NB, dim3 grid, const OrbArgs& a, const OrbOp& L, const OrbOp& V,
+                    const OrbOp& R, const OrbOut& X);
+  void vjp_reduce(bool has_m, int NB, dim3 grid, const OrbArgs& a, const OrbOp& A, const OrbOp& M, const OrbOp& C,
+                  const OrbOut& O);
+  void orb_gather_view(const OrbArgs& a, const OrbOp& op, int tz, int64_t n);
   void p0_prep_slice(int64_t q, const double2* G, int64_t t0, int64_t tc, int qpw);
   void p0_gemm(int64_t q, int64_t tc, double2* Pt0, int shape);
   void sigma_gemm(const double2* Pt0, const double2* Vb, int64_t nkb, int64_t tc, int shape);

@@ -89,6 +89,99 @@ void stack_transform(hipStream_t st, const double2* T, int rows, int cols, bool 
 
 }  // namespace
 
+void gw_hip_kernel::orb_gather_view(const OrbArgs& a, const OrbOp& op, int tz, int64_t n) {
+  const int64_t nao2 = d_.nao * d_.nao;
+  if (n < 1 || n > 65535) throw status_error(GWHIP_ERR_UNSUPPORTED, "gather: k count out of range");
+  hipLaunchKernelGGL(orb_gather_view_kernel, dim3((unsigned)ceil_div(nao2, 256), (unsigned)n), dim3(256), 0, stream_, a,
+                     op, tz, orb_G_);
+  GWHIP_CHECK(hipGetLastError());
+}
+
+// X_Q = L V_Q R for the grid's (k, tz): the kernel, or (orb_gemm()) one batched GEMM over k per
+// tz with the other factor gathered per k.  The GEMM form takes the shapes the VJP uses: exactly
+// one of L, R; V and X without a k map; rows (Q, r) of V and X equally spaced when R is present.
+void gw_hip_kernel::vjp_sandwich(bool has_l, bool has_r, int NB, dim3 grid, const OrbArgs& a, const OrbOp& L,
+                                 const OrbOp& V, const OrbOp& R, const OrbOut& X) {
+  if (!orb_gemm()) {
+    if (has_l && !has_r) launch_sandwich<true, false>(NB, grid, stream_, a, L, V, R, X);
+    else if (!has_l && has_r) launch_sandwich<false, true>(NB, grid, stream_, a, L, V, R, X);
+    else launch_sandwich<true, true>(NB, grid, stream_, a, L, V, R, X);
+    return;
+  }
+  const int64_t nk = a.nk, nao = a.nao, NQ = a.NQ, nao2 = nao * nao;
+  if (has_l == has_r || V.kmap != 0 || X.kmap != 0)
+    throw status_error(GWHIP_ERR_UNSUPPORTED, "GEMM form of this orbital product not implemented");
+  if (has_r && (V.s_q != nao * V.rs || X.s_q != nao * X.rs))
+    throw status_error(GWHIP_ERR_UNSUPPORTED, "GEMM form needs equally spaced (Q, row) rows");
+  for (int tz = 0; tz < (int)grid.z; ++tz) {
+    orb_gather_view(a, has_l ? L : R, tz, nk);
+    ZGemmArgs g{};
+    g.batch = (int)nk; g.kin = (int)nao; g.nin = (int)nao; g.K = (int)nao;
+    g.alpha = make_double2(1.0, 0.0);
+    if (has_l) {  // X[k](r, (Q, c)) = sum_r' L[k](r, r') V_Q(r', c)
+      g.M = (int)nao; g.N = (int)(NQ * nao);
+      g.A = orb_G_; g.sAb = nao2; g.sAm = nao; g.sAko = 0; g.sAki = 1;
+      g.B = V.p + tz * V.s_t; g.sBb = V.s_k; g.sBko = 0; g.sBki = V.rs; g.sBno = V.s_q; g.sBni = V.cs;
+      g.conjB = V.conj;
+      g.C = X.p + tz * X.s_t; g.sCb = X.s_k; g.sCm = X.rs; g.sCno = X.s_q; g.sCni = X.cs;
+      zgemm_f64(g, true, false, stream_);
+    } else {      // X[k]((Q, r), c) = sum_c' V_Q(r, c') R[k](c', c)
+      g.M = (int)(NQ * nao); g.N = (int)nao;
+      g.A = V.p + tz * V.s_t; g.sAb = V.s_k; g.sAm = V.rs; g.sAko = 0; g.sAki = V.cs;
+      g.conjA = V.conj;
+      g.B = orb_G_; g.sBb = nao2; g.sBko = 0; g.sBki = nao; g.sBno = 0; g.sBni = 1;
+      g.C = X.p + tz * X.s_t; g.sCb = X.s_k; g.sCm = X.rs; g.sCno = 0; g.sCni = X.cs;
+      zgemm_f64(g, V.cs == 1, false, stream_);
+    }
+  }
+}
+
+// O(tz, kidx(k)) += scale sum_Q A_Q M C_Q: the kernel, or (orb_gemm()) per tz Z'_Q = M C_Q as one
+// batched GEMM (M gathered per k), then sum_{Q, m} A_Q(i, m) Z'_Q(m, j) as a split-K GEMM per k,
+// then added to O through the k map.  The GEMM form needs A and C without a k map.
+void gw_hip_kernel::vjp_reduce(bool has_m, int NB, dim3 grid, const OrbArgs& a, const OrbOp& A, const OrbOp& M,
+                               const OrbOp& C, const OrbOut& O) {
+  if (!orb_gemm()) {
+    if (has_m) launch_reduce<true>(NB, grid, stream_, a, A, M, C, O);
+    else launch_reduce<false>(NB, grid, stream_, a, A, M, C, O);
+    return;
+  }
+  const int64_t nk = a.nk, nao = a.nao, NQ = a.NQ, nao2 = nao * nao, vq = NQ * nao2;
+  if (A.kmap != 0 || C.kmap != 0)
+    throw status_error(GWHIP_ERR_UNSUPPORTED, "GEMM form of this orbital reduction not implemented");
+  const int64_t tiles = nk * ceil_div(nao, 64) * ceil_div(nao, 64);
+  const int ksplit = (int)std::min<int64_t>(kOrbKsplit, std::max<int64_t>(1, ceil_div(1024, tiles)));
+  for (int tz = 0; tz < (int)grid.y; ++tz) {
+    const double2* cp = C.p + tz * C.s_t;
+    int64_t c_sk = C.s_k, c_sq = C.s_q, c_rs = C.rs, c_cs = C.cs;
+    int c_conj = C.conj;
+    if (has_m) {  // Z'[k](i, (Q, j)) = sum_m M[k](i, m) C_Q(m, j), canonical [k][Q][nao^2]
+      orb_gather_view(a, M, tz, nk);
+      ZGemmArgs z{};
+      z.M = (int)nao; z.N = (int)(NQ * nao); z.K = (int)nao; z.batch = (int)nk; z.kin = (int)nao; z.nin = (int)nao;
+      z.A = orb_G_; z.sAb = nao2; z.sAm = nao; z.sAko = 0; z.sAki = 1;
+      z.B = cp; z.sBb = C.s_k; z.sBko = 0; z.sBki = C.rs; z.sBno = C.s_q; z.sBni = C.cs;
+      z.conjB = C.conj;
+      z.C = orb_W_; z.sCb = vq; z.sCm = nao; z.sCno = nao2; z.sCni = 1;
+      z.alpha = make_double2(1.0, 0.0);
+      zgemm_f64(z, true, false, stream_);
+      cp = orb_W_; c_sk = vq; c_sq = nao2; c_rs = nao; c_cs = 1; c_conj = 0;
+    }
+    ZGemmArgs y{};  // orb_O_[k](i, j) = sum_{Q, m} A_Q(i, m) Z'_Q(m, j)
+    y.M = (int)nao; y.N = (int)nao; y.K = (int)(NQ * nao); y.batch = (int)nk; y.kin = (int)nao; y.nin = (int)nao;
+    y.A = A.p + tz * A.s_t; y.sAb = A.s_k; y.sAm = A.rs; y.sAko = A.s_q; y.sAki = A.cs;
+    y.conjA = A.conj;
+    y.B = cp; y.sBb = c_sk; y.sBko = c_sq; y.sBki = c_rs; y.sBno = 0; y.sBni = c_cs;
+    y.conjB = c_conj;
+    y.C = orb_O_; y.sCb = nao2; y.sCm = nao; y.sCno = 0; y.sCni = 1;
+    y.alpha = make_double2(1.0, 0.0);
+    zgemm_f64_splitk(y, ksplit, orb_part_, A.cs == 1, c_rs == 1, stream_);
+    hipLaunchKernelGGL(orb_scatter_add_kernel, dim3((unsigned)ceil_div(nao2, 256), (unsigned)nk), dim3(256), 0, stream_,
+                       a, O, tz, orb_O_);
+    GWHIP_CHECK(hipGetLastError());
+  }
+}
+
 void gw_hip_kernel::set_vjp_store(bool keep) {
   GWHIP_CHECK(hipSetDevice(device_));
   keep_p0_ = keep;
@@ -144,8 +237,6 @@ void gw_hip_kernel::vjp_device(const double2* G, const double2* Sbar, double2* G
   if (!v_set_) throw status_error(GWHIP_ERR_STATE, "integrals not set (gwhip_upload_V / gwhip_generate_V)");
   if (!G || !Sbar || !Gbar) throw status_error(GWHIP_ERR_ARG, "null G, Sigma_bar or G_bar");
   if (mode != GWHIP_VJP_RECOMPUTE && mode != GWHIP_VJP_STORED) throw status_error(GWHIP_ERR_ARG, "unknown VJP mode");
-  if (d_.nao > kOrbMaxNao)
-    throw status_error(GWHIP_ERR_UNSUPPORTED, "the VJP's orbital kernels handle nao <= " + std::to_string(kOrbMaxNao));
   const bool stored = mode == GWHIP_VJP_STORED;
   if (stored && !stored_valid_)
     throw status_error(GWHIP_ERR_STATE, "no stored forward pass (gwhip_set_vjp_store(h, 1), then gwhip_eval_sigma)");
@@ -211,7 +302,7 @@ void gw_hip_kernel::vjp_device(const double2* G, const double2* Sbar, double2* G
         // W_Q' = Sbar_s V_Q'
         const OrbOp Ls{Sbar + goff(t0, is) + blk, Gt, nso2, 0, nso, 1, 0, 0};
         const OrbOut W{vW_, per_t, NQ * nao2, nao2, nao, 1, 0};
-        launch_sandwich<true, false>(NB, dim3(gx, (unsigned)nk, (unsigned)tc), stream_, oa, Ls, Vrow, none, W);
+        vjp_sandwich(true, false, NB, dim3(gx, (unsigned)nk, (unsigned)tc), oa, Ls, Vrow, none, W);
         {  // Y = conj(P-tilde) W
           ZGemmArgs g{};
           g.M = (int)NQ; g.N = (int)(nk * nao2); g.K = (int)NQ; g.batch = (int)tc; g.kin = (int)NQ; g.nin = (int)nao2;
@@ -227,12 +318,12 @@ void gw_hip_kernel::vjp_device(const double2* G, const double2* Sbar, double2* G
           oc.scale = c;
           const OrbOp Y{scratch_, per_t, NQ * nao2, nao2, nao, 1, 0, 0};
           const OrbOut O{Gbar + goff(t0, is) + blk, Gt, nso2, 0, nso, 1, 2};
-          launch_reduce<false>(NB, dim3((unsigned)nk, (unsigned)tc), stream_, oc, VmH, none, Y, O);
+          vjp_reduce(false, NB, dim3((unsigned)nk, (unsigned)tc), oc, VmH, none, Y, O);
         }
         {  // T_Q = V_Q G(tau, k - q)_s
           const OrbOp Gs{G + goff(t0, is) + blk, Gt, nso2, 0, nso, 1, 2, 0};
           const OrbOut T{scratch_, per_t, NQ * nao2, nao2, nao, 1, 0};
-          launch_sandwich<false, true>(NB, dim3(gx, (unsigned)nk, (unsigned)tc), stream_, oa, none, Vrow, Gs, T);
+          vjp_sandwich(false, true, NB, dim3(gx, (unsigned)nk, (unsigned)tc), oa, none, Vrow, Gs, T);
         }
         {  // Ptbar += c conj(T) W^T
           ZGemmArgs g{};
@@ -283,12 +374,12 @@ void gw_hip_kernel::vjp_device(const double2* G, const double2* Sbar, double2* G
         {  // Gbar(beta - tau, k)_(b,a) += sum_Q Zbar_Q Gq^H V_Q^H,  Gq = G(tau, k + q)_(a,b)
           const OrbOp GqH{G + goff(t0, is) + blk_ab, Gt, nso2, 0, 1, nso, 1, 1};
           const OrbOut O{Gbar + goff(tt0, is) + blk_ba, -Gt, nso2, 0, nso, 1, 0};
-          launch_reduce<true>(NB, dim3((unsigned)nk, (unsigned)tc), stream_, oa, Zb, GqH, VpH, O);
+          vjp_reduce(true, NB, dim3((unsigned)nk, (unsigned)tc), oa, Zb, GqH, VpH, O);
         }
         {  // Gbar(tau, k + q)_(a,b) += sum_Q V_Q^H Gb^H Zbar_Q,  Gb = G(beta - tau, k)_(b,a)
           const OrbOp GbH{G + goff(tt0, is) + blk_ba, -Gt, nso2, 0, 1, nso, 0, 1};
           const OrbOut O{Gbar + goff(t0, is) + blk_ab, Gt, nso2, 0, nso, 1, 1};
-          launch_reduce<true>(NB, dim3((unsigned)nk, (unsigned)tc), stream_, oa, VpH, GbH, Zb, O);
+          vjp_reduce(true, NB, dim3((unsigned)nk, (unsigned)tc), oa, VpH, GbH, Zb, O);
         }
       }
     }

@@ -54,6 +54,30 @@ __device__ __forceinline__ double2 orb_load(const double2* base, const OrbOp& op
   return v;
 }
 
+// GEMM form of the orbital products (any nao): one view at tz, for k = blockIdx.y, copied to a
+// canonical row-major nao x nao matrix out[k] (conjugated when the view is).
+__global__ void orb_gather_view_kernel(OrbArgs a, OrbOp op, int tz, double2* __restrict__ out) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= a.nao * a.nao) return;
+  const int k = blockIdx.y, r = e / a.nao, c = e - (e / a.nao) * a.nao;
+  const double2* base = op.p + (int64_t)tz * op.s_t + (int64_t)orb_kidx(a, op.kmap, k, tz) * op.s_k;
+  out[(int64_t)k * a.nao * a.nao + e] = orb_load(base, op, r, c);
+}
+
+// O(tz, kidx(k))(r, c) += scale * T[k][r][c]; k = blockIdx.y (a permutation of the outputs).
+__global__ void orb_scatter_add_kernel(OrbArgs a, OrbOut O, int tz, const double2* __restrict__ T) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= a.nao * a.nao) return;
+  const int k = blockIdx.y, r = e / a.nao, c = e - (e / a.nao) * a.nao;
+  double2* dst = O.p + (int64_t)tz * O.s_t + (int64_t)orb_kidx(a, O.kmap, k, tz) * O.s_k + (int64_t)r * O.rs +
+                 (int64_t)c * O.cs;
+  const double2 t = T[(int64_t)k * a.nao * a.nao + e];
+  double2 v = *dst;
+  v.x += a.scale * t.x;
+  v.y += a.scale * t.y;
+  *dst = v;
+}
+
 // Stage a shared nao x nao operand into LDS planes [16NB][16NB+1], zero outside nao.
 template <int NB>
 __device__ __forceinline__ void orb_stage(const OrbArgs& a, const OrbOp& op, int tz, int k, double* Xr, double* Xi) {

@@ -175,8 +175,7 @@ int gwhip_set_vjp_store(gwhip_handle* h, int keep);
  * src/gw_cpu_kernel.cpp:31-373) at G for the cotangent Sigma_bar: writes G_bar with
  * Re<Sigma_bar, dSigma> = Re<G_bar, dG> for every dG (<a,b> = sum conj(a) b; the convention of
  * torch.autograd for complex tensors).  Host buffers, all [nts][ns][nk][nso][nso] complex.
- * FP64 throughout.  `mode` is a gwhip_vjp_mode.  nao > 48: GWHIP_ERR_UNSUPPORTED.  No reference
- * counterpart. */
+ * FP64 throughout.  `mode` is a gwhip_vjp_mode.  No reference counterpart. */
 int gwhip_vjp(gwhip_handle* h, const double* G, const double* Sigma_bar, double* G_bar, int mode);
 
 /* As gwhip_vjp with device buffers.  Synchronous with respect to the host. */

@@ -4,6 +4,8 @@ Checks, all through the C ABI (and once through the torch.autograd harness):
 * against torch autograd of the dense FP64 einsum forward (oracle/gw_vjp.py) on small systems
   (ns = 1, ns = 2, X2C) and on the nk = 8, nao = 8, naux = 40 system: 1e-10 relative;
 * stored (P0/P-tilde kept by the forward) and recomputed modes agree;
+* the GEMM form of the orbital products (tuning key orbital_gemm; always so for nao > 48) on the
+  small systems and at nao = 50: the same 1e-10 against autograd;
 * at nk = 27 (3x3x3), nao = 26, naux = 250, beta = 100, IR 1e4 with a Hermitian-per-tau N(0,1)
   cotangent: central finite differences (step 1e-6) of the device FP64 forward along random
   directions agree with Re<G_bar, dG>, and both modes are timed.
@@ -40,13 +42,15 @@ def _kernel(d, T):
     return k
 
 
-def _compare(G, V, T, d, seed=11):
+def _compare(G, V, T, d, seed=11, gemm=False):
     Sbar = _herm_noise(G.shape, seed)
     args = (V, T.T_bt, T.T_tb, d["nk"], d["nao"], d["nso"], d["ns"], d["NQ"], d["mesh"])
     _, g_ref = gv.vjp_autograd(G, Sbar, *args)
     k = _kernel(d, T)
     try:
         k.upload_V(V)
+        if gemm:
+            k.set_tuning(orbital_gemm=1)
         g_rec = k.vjp(G, Sbar, gl.VJP_RECOMPUTE)
         k.set_vjp_store(True)
         k.solve(G)
@@ -64,6 +68,17 @@ def test_vjp_small_cases(ci):
                            nso=nso, sigma=0.3)
     g_ref, g_rec, g_sto = _compare(G, V, T, d)
     print(f"case {c}: recompute {rel_err(g_rec, g_ref):.2e} stored {rel_err(g_sto, g_ref):.2e}")
+    assert rel_err(g_rec, g_ref) < 1e-10
+    assert rel_err(g_sto, g_ref) < 1e-10
+
+
+@pytest.mark.parametrize("c", SMALL[:3] + [dict(nk=2, mesh=(2, 1, 1), nao=50, NQ=6, ns=1)])
+def test_vjp_gemm_form(c):
+    nso = c.get("nso", c["nao"])
+    G, V, T, d = synthetic(c["nk"], c["nao"], c["NQ"], ns=c["ns"], beta=20.0, lam=1e3, mesh=c["mesh"], seed=5,
+                           nso=nso, sigma=0.3)
+    g_ref, g_rec, g_sto = _compare(G, V, T, d, gemm=True)
+    print(f"GEMM form {c}: recompute {rel_err(g_rec, g_ref):.2e} stored {rel_err(g_sto, g_ref):.2e}", flush=True)
     assert rel_err(g_rec, g_ref) < 1e-10
     assert rel_err(g_sto, g_ref) < 1e-10
 
