@@ -638,10 +638,11 @@ void gw_hip_kernel::p0_one_q(int64_t q, const double2* G, bool split, double2* P
 // In-place inverse of nmat Hermitian positive definite NQ x NQ matrices (I - P0(i Omega)).
 void gw_hip_kernel::gj_inverse(double2* X, int64_t nmat) {
   const int64_t NQ = d_.NQ, NQ2 = NQ * NQ;
+  // the column panel [NQ][nb] sits in LDS: narrower panels for larger NQ (nb = 1 up to NQ ~ 10^4)
   int nb = GJ_NB;
   size_t gj_lds = (size_t)(GJ_NB * GJ_NB + NQ * nb) * sizeof(double2);
-  if ((int64_t)gj_lds > max_lds_) {
-    nb = 8;
+  while ((int64_t)gj_lds > max_lds_ && nb > 1) {
+    nb /= 2;
     gj_lds = (size_t)(GJ_NB * GJ_NB + NQ * nb) * sizeof(double2);
   }
   if ((int64_t)gj_lds > max_lds_) throw status_error(GWHIP_ERR_UNSUPPORTED, "NQ too large for the Dyson kernel");

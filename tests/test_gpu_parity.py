@@ -63,6 +63,8 @@ def test_reference_fixtures(name):
     dict(nk=2, mesh=(2, 1, 1), nao=3, NQ=7, ns=1, nso=6),      # X2C spin blocks, k > 1
     dict(nk=1, mesh=(1, 1, 1), nao=1, NQ=1, ns=1),             # scalar
     dict(nk=4, mesh=(2, 2, 1), nao=5, NQ=70, ns=1),            # NQ spans two GEMM tiles
+    dict(nk=2, mesh=(2, 1, 1), nao=2, NQ=700, ns=1),           # Dyson panel narrowed to 8 columns
+    dict(nk=1, mesh=(1, 1, 1), nao=2, NQ=1300, ns=1),          # ... and to 4
 ])
 def test_small_cases(case):
     nso = case.get("nso", case["nao"])
