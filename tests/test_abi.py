@@ -23,6 +23,28 @@ def test_reader_library_exports_every_header_symbol():
         assert hasattr(h5, n), n
 
 
+def test_headers_are_plain_c():
+    """The boundary headers compile as C99 and as C++17 (a C caller and the C++ adapter both
+    include them), with warnings as errors."""
+    import os
+    import shutil
+    import subprocess
+    import tempfile
+
+    inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    src = '#include "gwhip.h"\n#include "gwhip_h5.h"\nint main(void) { return GWHIP_OK; }\n'
+    with tempfile.TemporaryDirectory() as d:
+        for compiler, ext, std in (("gcc", "c", "-std=c99"), ("g++", "cpp", "-std=c++17")):
+            if not shutil.which(compiler):
+                pytest.skip(f"{compiler} not found")
+            path = os.path.join(d, "probe." + ext)
+            with open(path, "w") as f:
+                f.write(src)
+            r = subprocess.run([compiler, std, "-Wall", "-Wextra", "-pedantic", "-Werror", "-fsyntax-only", "-I", inc,
+                                path], capture_output=True, text=True)
+            assert r.returncode == 0, r.stderr
+
+
 def _create(**kw):
     lib = gl.load()
     base = dict(nk=1, mesh=(1, 1, 1), nao=2, nso=2, ns=1, NQ=3, nts=4, nw_b=3, beta=10.0)
