@@ -551,23 +551,32 @@ void gw_hip_kernel::p0_slice(int64_t tc) {
 }
 
 // Split precision, P0 = -(pref/nk) Z conj(V)^T from the slices (row-tile group `group`).
-void gw_hip_kernel::p0_gemm_split(int64_t q, int64_t tc, double2* Pt0, int group, int tile) {
+void gw_hip_kernel::p0_gemm_split(int64_t q, int64_t tc, double2* Pt0, int group, int tile, int kchunks) {
   const int64_t nk = d_.nk, NQ = d_.NQ, NQ2 = NQ * NQ;
   const int ns = split_ns_;
-  OzGemmArgs o{};
-  o.M = (int)NQ; o.N = (int)NQ; o.batch = (int)tc; o.kc = (int)kc_p0_; o.nchunk = (int)nk; o.nin = (int)NQ;
-  o.A = Zs_; o.sAm = nk * 2 * ns * kc_p0_; o.sAb = NQ * o.sAm; o.sAc = 2 * ns * kc_p0_;
-  o.eA = eZs_; o.seAb = NQ * nk; o.seAm = nk;
-  o.B = Vs_p0_ + q * NQ * nk * 2 * ns * kc_p0_; o.sBb = 0; o.sBno = 0; o.sBni = nk * 2 * ns * kc_p0_;
-  o.sBc = 2 * ns * kc_p0_;
-  o.eB = eVs_p0_ + q * NQ * nk; o.seBb = 0; o.seBno = 0; o.seBni = nk;
-  o.C = Pt0; o.sCb = NQ2; o.sCm = NQ; o.sCno = 0; o.sCni = 1;
-  o.alpha = make_double2(-d_.prefactor / (double)nk, 0.0);
-  o.accumulate = 0;
-  o.group_m = group;
-  o.tile = tile;
-  o.upper = g_herm_ ? 1 : 0;
-  zgemm_ozaki(o, ns, stream_);
+  const int64_t sc = 2 * ns * kc_p0_;  // bytes of one k chunk of a row
+  // K = nk chunks of kc: in launches over `kchunks` consecutive chunks each, the first writing C
+  // and the others adding to it, so that one launch's A and B chunks are reused from L2 by all
+  // its tiles instead of streaming the whole K past every tile (partial sums in FP64, as between
+  // chunks inside a launch).  At nk = 64, naux = 300: 770 ms per evaluation in one launch per
+  // (q, tau chunk), 604 ms with 4 chunks per launch.
+  const int64_t nsl = kchunks <= 0 ? 1 : ceil_div(nk, std::min<int64_t>(kchunks, nk));
+  for (int64_t sl = 0; sl < nsl; ++sl) {
+    const int64_t c0 = nk * sl / nsl, c1 = nk * (sl + 1) / nsl;
+    OzGemmArgs o{};
+    o.M = (int)NQ; o.N = (int)NQ; o.batch = (int)tc; o.kc = (int)kc_p0_; o.nchunk = (int)(c1 - c0); o.nin = (int)NQ;
+    o.A = Zs_ + c0 * sc; o.sAm = nk * sc; o.sAb = NQ * o.sAm; o.sAc = sc;
+    o.eA = eZs_ + c0; o.seAb = NQ * nk; o.seAm = nk;
+    o.B = Vs_p0_ + q * NQ * nk * sc + c0 * sc; o.sBb = 0; o.sBno = 0; o.sBni = nk * sc; o.sBc = sc;
+    o.eB = eVs_p0_ + q * NQ * nk + c0; o.seBb = 0; o.seBno = 0; o.seBni = nk;
+    o.C = Pt0; o.sCb = NQ2; o.sCm = NQ; o.sCno = 0; o.sCni = 1;
+    o.alpha = make_double2(-d_.prefactor / (double)nk, 0.0);
+    o.accumulate = sl > 0 ? 1 : 0;
+    o.group_m = group;
+    o.tile = tile;
+    o.upper = g_herm_ ? 1 : 0;
+    zgemm_ozaki(o, ns, stream_);
+  }
 }
 
 // Split precision, Sigma: int8 slices of P-tilde(t0 .. t0+tc) (Pt0: [tc][NQ^2]) into Ps_.
@@ -619,7 +628,9 @@ void gw_hip_kernel::p0_one_q(int64_t q, const double2* G, bool split, double2* P
       if (split) p0_slice(tc);
     }
     if (split) {
-      timed(K_P0_GEMM, [&] { p0_gemm_split(q, tc, Pq + t0 * NQ2, tune_.p0_group, tune_.p0_oz_tile); });
+      timed(K_P0_GEMM, [&] {
+        p0_gemm_split(q, tc, Pq + t0 * NQ2, tune_.p0_group, tune_.p0_oz_tile, tune_.p0_kchunks);
+      });
     } else {
       timed(K_P0_GEMM, [&] { p0_gemm(q, tc, Pq + t0 * NQ2, tune_.p0_shape); });
     }
@@ -1018,9 +1029,11 @@ void gw_hip_kernel::autotune(const double2* G, int precision) {
                            201, 202, 204, 208, 216, 232, 264};
     p0_prep(0, G, 0, tcp, tune_.qpw);
     p0_slice(tcp);
-    const int bp = best_of(21, cands, [&](int c) { p0_gemm_split(0, tcp, P_, c % 100, c / 100); });
+    const int bp = best_of(21, cands, [&](int c) { p0_gemm_split(0, tcp, P_, c % 100, c / 100, tune_.p0_kchunks); });
     tune_.p0_oz_tile = bp / 100;
     tune_.p0_group = bp % 100;
+    const int kch[5] = {0, 16, 8, 4, 2};
+    tune_.p0_kchunks = best_of(5, kch, [&](int c) { p0_gemm_split(0, tcp, P_, tune_.p0_group, tune_.p0_oz_tile, c); });
     sigma_slice(P_, tcs);
     const int bs =
         best_of(21, cands, [&](int c) { sigma_gemm_split(Vs_neg(0), eVs_neg(0), d_.nk, tcs, c % 100, c / 100); });

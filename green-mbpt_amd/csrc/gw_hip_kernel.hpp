@@ -40,23 +40,26 @@ struct KernelStat {
 // orbital kernel writes the int8 slices itself.
 struct Tuning {
   int p0_shape = 0, s_shape = 0, qpw = 4;
-  int p0_group = 64, s_group = 8;  // split precision: row tiles per L2 group of the int8 GEMMs
+  int p0_group = 8, s_group = 8;  // split precision: row tiles per L2 group of the int8 GEMMs
   int p0_oz_tile = 2, s_oz_tile = 2;  // split precision: int8 GEMM block tile (0: 32x64, 1: 64x64,
                                       // 2: 32x64 software-pipelined)
   int p0_fused = 1;  // split precision, one spin block: P0 orbital kernel writes int8 slices directly
   int orb_gemm = 0;  // orbital contractions as batched FP64 GEMMs (always so for nao > 48)
-  static constexpr int count = 9;
+  int p0_kchunks = 4;  // split precision: k chunks per launch of the int8 P0 GEMM (0: all in one), so
+                       // that each launch's operand window stays in L2
+  static constexpr int count = 10;
   void to_array(int* v) const {
-    const int a[count] = {p0_shape, s_shape, qpw, p0_group, s_group, p0_oz_tile, s_oz_tile, p0_fused, orb_gemm};
+    const int a[count] = {p0_shape, s_shape, qpw, p0_group, s_group, p0_oz_tile, s_oz_tile, p0_fused, orb_gemm,
+                          p0_kchunks};
     for (int i = 0; i < count; ++i) v[i] = a[i];
   }
   bool from_array(const int* v) {
     if (v[0] < 0 || v[0] > 2 || v[1] < 0 || v[1] > 2 || (v[2] != 1 && v[2] != 2 && v[2] != 4 && v[2] != 8) ||
         v[3] < 1 || v[4] < 1 || v[5] < 0 || v[5] > 2 || v[6] < 0 || v[6] > 2 || v[7] < 0 || v[7] > 1 || v[8] < 0 ||
-        v[8] > 1)
+        v[8] > 1 || v[9] < 0)
       return false;
     p0_shape = v[0]; s_shape = v[1]; qpw = v[2]; p0_group = v[3]; s_group = v[4]; p0_oz_tile = v[5]; s_oz_tile = v[6];
-    p0_fused = v[7]; orb_gemm = v[8];
+    p0_fused = v[7]; orb_gemm = v[8]; p0_kchunks = v[9];
     return true;
   }
 };
@@ -310,7 +313,7 @@ private:
   void p0_gemm(int64_t q, int64_t tc, double2* Pt0, int shape);
   void sigma_gemm(const double2* Pt0, const double2* Vb, int64_t nkb, int64_t tc, int shape);
   void p0_slice(int64_t tc);
-  void p0_gemm_split(int64_t q, int64_t tc, double2* Pt0, int group, int tile);
+  void p0_gemm_split(int64_t q, int64_t tc, double2* Pt0, int group, int tile, int kchunks);
   void sigma_slice(const double2* Pt0, int64_t tc);
   void sigma_gemm_split(const int8_t* Vsb, const int* eVsb, int64_t nkb, int64_t tc, int group, int tile);
   void stage_p0(const double2* G, bool split, const std::vector<int>* qs = nullptr);

@@ -138,8 +138,9 @@ int gwhip_autotune(gwhip_handle* h, const void* G_dev, int precision);
  * Sigma GEMM block tile,
  * [7] split P0 orbital kernel writes the int8 slices itself (1) or via a separate pass (0),
  * [8] orbital-size products (Z = Gb V Gq of P0, V G R of Sigma) as batched FP64 GEMMs (1) or the
- * register-chained MFMA kernels (0); always 1 in effect for nao > 48. */
-#define GWHIP_TUNING_COUNT 9
+ * register-chained MFMA kernels (0); always 1 in effect for nao > 48, [9] int8 P0 GEMM: k chunks
+ * per launch (0: all in one launch), so that each launch's operands stay in L2. */
+#define GWHIP_TUNING_COUNT 10
 int gwhip_get_tuning(const gwhip_handle* h, int* values, int n);
 int gwhip_set_tuning(gwhip_handle* h, const int* values, int n);
 

@@ -119,6 +119,29 @@ def test_split_tiles_and_orders_give_identical_sigma(p0_group, s_group):
         assert np.array_equal(o, ref)
 
 
+def test_p0_k_chunks_per_launch():
+    """The int8 P0 GEMM over k chunks in one launch or in launches of 1, 3 or 8 chunks: the same
+    exact int32 slice products, FP64 partial sums added in another order (1e-13)."""
+    c = CONFIG0
+    G, V, T, d = synthetic(c["nk"], c["nao"], c["NQ"], beta=c["beta"], lam=c["lam"], mesh=c["mesh"])
+    k = _kernel(d, T)
+    try:
+        k.set_grid(T.T_bt, T.T_tb)
+        k.upload_V(V)
+        out = {}
+        for kch in (0, 1, 3, 8):
+            k.set_tuning(p0_kchunks=kch)
+            out[kch] = k.solve(G, gl.PREC_SPLIT)
+    finally:
+        k.close()
+    S_ref = go.gw_sigma(G, V, T.T_bt, T.T_tb, d["nk"], d["nao"], d["nso"], d["ns"], d["NQ"], mesh=d["mesh"])
+    for kch, S in out.items():
+        print(f"P0 k chunks per launch {kch}: vs one launch {rel_err(S, out[0]):.1e}, vs oracle {rel_err(S, S_ref):.1e}",
+              flush=True)
+        assert rel_err(S, out[0]) < 1e-13
+        assert rel_err(S, S_ref) < TOL_SPLIT
+
+
 def test_split_reference_fixture():
     G, V, T, d, S_fix = load_fixture("gw_x2c")
     S_split, _, S_ref = _split_vs_oracle(G, V, T, d)
