@@ -33,8 +33,8 @@ WORKLOAD = dict(nk=64, mesh=(4, 4, 4), nao=26, NQ=300, ns=1, beta=100.0, lam=1e4
 # the bytes the copy reads, WRITE_SIZE 1.0x the bytes it writes.
 # Source: profiles/gw_sigma_nk64_nao26_naux300_hbm_traffic.md
 PMC_PER_LAUNCH_KIB = {
-    "split": {"Sigma GEMM": (4.3304e6, 1.0202e7), "Sigma epilogue": (5.7919e6, 6.7600e4),
-              "P0 GEMM": (3.4184e7, 4.2315e4), "P0 prep": (7.7595e5, 6.7702e6)},
+    "split": {"Sigma GEMM": (4.3293e6, 1.0200e7), "Sigma epilogue": (5.7963e6, 6.7600e4),
+              "P0 GEMM": (1.1368e7, 6.7984e5), "P0 prep": (7.6667e5, 6.7426e6)},  # P0: 16 launches per q
     "fp64": {"Sigma GEMM": (2.5800e7, 1.0178e7), "Sigma epilogue": (5.8315e6, 6.7600e4),
              "P0 GEMM": (1.1350e7, 4.2515e4), "P0 prep": (7.4522e5, 1.1822e7)},
 }
