@@ -1160,6 +1160,54 @@ __global__ __launch_bounds__(256) void valu_peak_kernel(double* out, int iters) 
   out[(int64_t)blockIdx.x * blockDim.x + threadIdx.x] = s;
 }
 
+// FP64 vector-FMA complex outer products with operands from LDS (probe): each lane accumulates a
+// 4 x 4 complex tile over K = 32, A and B as real/imaginary planes [k][32] read 4 at a time.
+__global__ __launch_bounds__(256) void valu_cgemm_probe_kernel(double* out, int iters) {
+  __shared__ double Ar[32][32], Ai[32][32], Br[32][32], Bi[32][32];
+  for (int e = threadIdx.x; e < 32 * 32; e += blockDim.x) {
+    const int r = e / 32, c = e % 32;
+    Ar[r][c] = 1e-3 * (r + c);
+    Ai[r][c] = 1e-3 * (r - c);
+    Br[r][c] = 1e-3 * (2 * r + c);
+    Bi[r][c] = 1e-3 * (r - 2 * c);
+  }
+  __syncthreads();
+  const int lane = threadIdx.x & 63, ti = (lane & 7) * 4, tj = (lane >> 3) * 4;
+  double cr[4][4], ci[4][4];
+#pragma unroll
+  for (int x = 0; x < 4; ++x)
+#pragma unroll
+    for (int y = 0; y < 4; ++y) { cr[x][y] = 0.0; ci[x][y] = 0.0; }
+  for (int it = 0; it < iters; ++it) {
+#pragma unroll 4
+    for (int k = 0; k < 32; ++k) {
+      double ar[4], ai[4], br[4], bi[4];
+#pragma unroll
+      for (int x = 0; x < 4; ++x) {
+        ar[x] = Ar[k][ti + x];
+        ai[x] = Ai[k][ti + x];
+        br[x] = Br[k][tj + x];
+        bi[x] = Bi[k][tj + x];
+      }
+#pragma unroll
+      for (int x = 0; x < 4; ++x)
+#pragma unroll
+        for (int y = 0; y < 4; ++y) {
+          cr[x][y] = fma(ar[x], br[y], cr[x][y]);
+          cr[x][y] = fma(-ai[x], bi[y], cr[x][y]);
+          ci[x][y] = fma(ar[x], bi[y], ci[x][y]);
+          ci[x][y] = fma(ai[x], br[y], ci[x][y]);
+        }
+    }
+  }
+  double s = 0.0;
+#pragma unroll
+  for (int x = 0; x < 4; ++x)
+#pragma unroll
+    for (int y = 0; y < 4; ++y) s += cr[x][y] + ci[x][y];
+  out[(int64_t)blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
 typedef int v4i_t __attribute__((ext_vector_type(4)));
 typedef float v4f_t __attribute__((ext_vector_type(4)));
 typedef __bf16 v8bf_t __attribute__((ext_vector_type(8)));
@@ -1190,7 +1238,9 @@ __global__ __launch_bounds__(256) void lowp_peak_kernel(double* out, int iters) 
 }
 
 double debug_mfma_peak(int chains, int wps, int iters) {
-  if (chains < 1 || (chains > 8 && chains != 10 && chains != 11) || chains == 7 || wps < 1 || wps > 8 || iters < 1) throw status_error(GWHIP_ERR_ARG, "bad mfma probe args");
+  if (chains < 1 || (chains > 8 && chains != 10 && chains != 11 && chains != 12) || chains == 7 || wps < 1 || wps > 8 ||
+      iters < 1)
+    throw status_error(GWHIP_ERR_ARG, "bad mfma probe args");
   GWHIP_CHECK(hipSetDevice(0));
   hipDeviceProp_t prop;
   GWHIP_CHECK(hipGetDeviceProperties(&prop, 0));
@@ -1201,6 +1251,7 @@ double debug_mfma_peak(int chains, int wps, int iters) {
     switch (chains) {
       case 10: hipLaunchKernelGGL(lowp_peak_kernel<0>, dim3(nblocks), dim3(256), 0, nullptr, out, iters); break;
       case 11: hipLaunchKernelGGL(lowp_peak_kernel<1>, dim3(nblocks), dim3(256), 0, nullptr, out, iters); break;
+      case 12: hipLaunchKernelGGL(valu_cgemm_probe_kernel, dim3(nblocks), dim3(256), 0, nullptr, out, iters); break;
       case 5: hipLaunchKernelGGL(valu_peak_kernel<1>, dim3(nblocks), dim3(256), 0, nullptr, out, iters); break;
       case 6: hipLaunchKernelGGL(valu_peak_kernel<2>, dim3(nblocks), dim3(256), 0, nullptr, out, iters); break;
       case 1: hipLaunchKernelGGL(mfma_peak_kernel<1>, dim3(nblocks), dim3(256), 0, nullptr, out, iters); break;
@@ -1226,6 +1277,7 @@ double debug_mfma_peak(int chains, int wps, int iters) {
   double per_wave_iter;
   if (chains == 10) per_wave_iter = 8 * 16.0 * 16 * 64 * 2;
   else if (chains == 11) per_wave_iter = 8 * 16.0 * 16 * 32 * 2;
+  else if (chains == 12) per_wave_iter = 32 * 64 * 2.0 * 64;  // 32 k x 64 FMA per lane
   else if (chains == 5) per_wave_iter = 32.0 * 64 * 2;                   // 32 FMA per lane
   else if (chains == 6) per_wave_iter = 32.0 * 64 * 2 + 4 * 2048.0;  // + 4 MFMA
   else per_wave_iter = (chains == 3 ? 4 : (chains > 4 ? 8 : chains)) * 2048.0;
