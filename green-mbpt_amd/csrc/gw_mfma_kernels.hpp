@@ -81,20 +81,32 @@ __device__ __forceinline__ void cmfma(v4d& cr, v4d& ci, double ar, double ai, do
 
 __device__ __forceinline__ v4d v4zero() { return v4d{0.0, 0.0, 0.0, 0.0}; }
 
+// Complex accumulator of a 16x16 MFMA fragment chain: four real MFMAs per complex product
+// (Re += ar br - ai bi, Im += ar bi + ai br).  Gauss's three-product form (p1 = ar br, p2 = ai bi,
+// p3 = (ar + ai)(br + bi)) was measured slower in these kernels: Sigma epilogue 1522 ms against
+// 1100 ms per evaluation at nk = 64, nao = 26, naux = 300, P0 prep 643 against 571 ms.
+template <typename T> struct CAcc {
+  typename Mfma<T>::acc_t cr, ci;
+  __device__ __forceinline__ void zero() { cr = ci = Mfma<T>::zero(); }
+  __device__ __forceinline__ void mac(T ar, T ai, T br, T bi) { cmfma<T>(cr, ci, ar, ai, br, bi); }
+  __device__ __forceinline__ T re(int r) const { return cr[r]; }
+  __device__ __forceinline__ T im(int r) const { return ci[r]; }
+};
+
 // ---------------------------------------------------------------------------------------
 // P0 left operand  Z_Q = sum_s Gb_s V_Q Gq_s  (Gb_s = G(beta - tau, k) block (b, a),
 // Gq_s = G(tau, k + q) block (a, b)); src/gw_cpu_kernel.cpp:170-213 regrouped.
 // grid: (ceil(NQ / (4 QPW)), nk, tcount); 4 waves, each handles QPW consecutive Q.
 // LDS per spin block: Gb and Gq as zero-padded real/imag planes [16NB][16NB+1] of T.
 // T = double: FP64 matrix cores; T = float: FP32 matrix cores (split-precision path), K <= 16NB
-// per MFMA chain, result written in FP64.
+// per MFMA chain, result written in FP64.  Registers: at most half the file for NB <= 2 (two or
+// more waves per SIMD), all of it for NB = 3.
 template <int NB, typename T>
-__global__ __launch_bounds__(256) void z_prep_mfma_kernel(XcdGrid xg, DevDims d, const double2* __restrict__ G,
+__global__ __launch_bounds__(256, NB < 3 ? 2 : 1) void z_prep_mfma_kernel(XcdGrid xg, DevDims d, const double2* __restrict__ G,
                                                           const double2* __restrict__ Vt,
                                                           double2* __restrict__ Z, const int* __restrict__ kplus,
                                                           int t0, int QPW) {
   typedef Mfma<T> M;
-  typedef typename M::acc_t acc_t;
   constexpr int N16 = 16 * NB, LD = N16 + 1, PL = N16 * LD;
   extern __shared__ double lds_d[];
   T* Gbr = reinterpret_cast<T*>(lds_d);
@@ -135,9 +147,9 @@ __global__ __launch_bounds__(256) void z_prep_mfma_kernel(XcdGrid xg, DevDims d,
       double2* zq = Z + (((int64_t)tz * d.nk + k) * d.NQ + Q) * nao2;
       for (int bn = 0; bn < NB; ++bn) {
         // W[:, bn] = V Gq[:, bn]
-        acc_t wr[NB], wi[NB];
+        CAcc<T> w[NB];
 #pragma unroll
-        for (int x = 0; x < NB; ++x) { wr[x] = M::zero(); wi[x] = M::zero(); }
+        for (int x = 0; x < NB; ++x) w[x].zero();
         for (int sl = 0; sl < nslab; ++sl) {
           const int m = 4 * sl + lk;
           const T br = Gqr[m * LD + 16 * bn + lrow], bi = Gqi[m * LD + 16 * bn + lrow];
@@ -146,24 +158,24 @@ __global__ __launch_bounds__(256) void z_prep_mfma_kernel(XcdGrid xg, DevDims d,
             const int p = 16 * bp + lrow;
             double2 av = make_double2(0.0, 0.0);
             if (p < nao && m < nao) av = vt[m * nao + p];
-            cmfma<T>(wr[bp], wi[bp], (T)av.x, (T)av.y, br, bi);
+            w[bp].mac((T)av.x, (T)av.y, br, bi);
           }
         }
         // Z[:, bn] (+)= Gb W[:, bn]; register r of W holds row p = 16 bp + row_of(lk, r)
-        acc_t zr[NB], zi[NB];
+        CAcc<T> z[NB];
 #pragma unroll
-        for (int x = 0; x < NB; ++x) { zr[x] = M::zero(); zi[x] = M::zero(); }
+        for (int x = 0; x < NB; ++x) z[x].zero();
 #pragma unroll
         for (int bp = 0; bp < NB; ++bp)
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
             if (sizeof(T) == 8 && 16 * bp + 4 * r >= nao) continue;  // K slab wholly in the padding
             const int p = 16 * bp + M::row_of(lk, r);
-            const T bwr = wr[bp][r], bwi = wi[bp][r];
+            const T bwr = w[bp].re(r), bwi = w[bp].im(r);
 #pragma unroll
             for (int bt = 0; bt < NB; ++bt) {
               const int t = 16 * bt + lrow;
-              cmfma<T>(zr[bt], zi[bt], Gbr[t * LD + p], Gbi[t * LD + p], bwr, bwi);
+              z[bt].mac(Gbr[t * LD + p], Gbi[t * LD + p], bwr, bwi);
             }
           }
         const int n = 16 * bn + lrow;
@@ -174,7 +186,7 @@ __global__ __launch_bounds__(256) void z_prep_mfma_kernel(XcdGrid xg, DevDims d,
             for (int r = 0; r < 4; ++r) {
               const int t = 16 * bt + M::row_of(lk, r);
               if (t < nao) {
-                double2 v = make_double2((double)zr[bt][r], (double)zi[bt][r]);
+                double2 v = make_double2((double)z[bt].re(r), (double)z[bt].im(r));
                 if (s > 0) {
                   const double2 o = zq[t * nao + n];
                   v.x += o.x;
@@ -196,8 +208,9 @@ __global__ __launch_bounds__(256) void z_prep_mfma_kernel(XcdGrid xg, DevDims d,
 // and writes the NS slices of Re and Im itself -- the same arithmetic as slice_rows_kernel, so
 // the slices are bit-identical to slicing the FP64 Z, without Z ever reaching HBM.
 // Slice layout: dst[((tz*NQ + Q)*nk + k)][plane][slice][kc], exps[(tz*NQ + Q)*nk + k].
+// Two waves per SIMD for NB <= 2 (three spill and run 15 % slower at nao = 26).
 template <int NB, int NS>
-__global__ __launch_bounds__(256) void z_prep_slice_kernel(XcdGrid xg, DevDims d, const double2* __restrict__ G,
+__global__ __launch_bounds__(256, NB < 3 ? 2 : 1) void z_prep_slice_kernel(XcdGrid xg, DevDims d, const double2* __restrict__ G,
                                                            const double2* __restrict__ Vt, int8_t* __restrict__ dst,
                                                            int* __restrict__ exps, int kc,
                                                            const int* __restrict__ kplus, int t0, int QPW) {
@@ -236,7 +249,7 @@ __global__ __launch_bounds__(256) void z_prep_slice_kernel(XcdGrid xg, DevDims d
     const int Q = (bx * 4 + wave) * QPW + qi;
     if (Q >= d.NQ) break;
     const double2* vt = Vt + ((int64_t)k * d.NQ + Q) * nao2;  // vt[m * nao + p] = V[p][m]
-    v4d zr[NB][NB], zi[NB][NB];                                 // [bn][bt]
+    CAcc<double> z[NB][NB];                                 // [bn][bt]
     double2 avs[4 * NB][NB];                                    // V_Q, loaded once per Q
 #pragma unroll
     for (int sl = 0; sl < 4 * NB; ++sl)
@@ -247,32 +260,44 @@ __global__ __launch_bounds__(256) void z_prep_slice_kernel(XcdGrid xg, DevDims d
       }
 #pragma unroll
     for (int bn = 0; bn < NB; ++bn) {
-      v4d wr[NB], wi[NB];
+      CAcc<double> w[NB];
 #pragma unroll
-      for (int x = 0; x < NB; ++x) { wr[x] = v4zero(); wi[x] = v4zero(); }
+      for (int x = 0; x < NB; ++x) w[x].zero();
 #pragma unroll
       for (int sl = 0; sl < 4 * NB; ++sl) {
         if (sl >= nslab) break;
         const int m = 4 * sl + lk;
         const double br = Gqr[m * LD + 16 * bn + lrow], bi = Gqi[m * LD + 16 * bn + lrow];
 #pragma unroll
-        for (int bp = 0; bp < NB; ++bp) cmfma(wr[bp], wi[bp], avs[sl][bp].x, avs[sl][bp].y, br, bi);
+        for (int bp = 0; bp < NB; ++bp) w[bp].mac(avs[sl][bp].x, avs[sl][bp].y, br, bi);
       }
 #pragma unroll
-      for (int x = 0; x < NB; ++x) { zr[bn][x] = v4zero(); zi[bn][x] = v4zero(); }
+      for (int x = 0; x < NB; ++x) z[bn][x].zero();
 #pragma unroll
       for (int bp = 0; bp < NB; ++bp)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
           if (16 * bp + 4 * r >= nao) continue;  // K slab wholly in the padding
           const int p = 16 * bp + lk + 4 * r;
+          const double bwr = w[bp].re(r), bwi = w[bp].im(r);
 #pragma unroll
           for (int bt = 0; bt < NB; ++bt) {
             const int t = 16 * bt + lrow;
-            cmfma(zr[bn][bt], zi[bn][bt], Gbr[t * LD + p], Gbi[t * LD + p], wr[bp][r], wi[bp][r]);
+            z[bn][bt].mac(Gbr[t * LD + p], Gbi[t * LD + p], bwr, bwi);
           }
         }
     }
+    // Re and Im of the finished products
+    v4d zr[NB][NB], zi[NB][NB];
+#pragma unroll
+    for (int bn = 0; bn < NB; ++bn)
+#pragma unroll
+      for (int bt = 0; bt < NB; ++bt)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          zr[bn][bt][r] = z[bn][bt].re(r);
+          zi[bn][bt][r] = z[bn][bt].im(r);
+        }
     // chunk exponent: max |Re|, |Im| over the nao x nao tile (padding entries are exact zeros)
     double mx = 0.0;
 #pragma unroll
@@ -334,15 +359,17 @@ __global__ __launch_bounds__(256) void z_prep_slice_kernel(XcdGrid xg, DevDims d
 // directly, G (full mesh) is read at kminus[k].  The full mesh has nks = nk; the star-of-q
 // evaluation passes the irreducible k points only.
 // With T = float each Q's product (K <= 16 NB) is added to FP64 accumulators.
+// Occupancy: for NB <= 2 the register budget is set for three waves per SIMD (a 128-byte spill per
+// lane; 1058 ms per evaluation at nk = 64, nao = 26, naux = 300 against 1095 ms at two waves);
+// NB = 3 keeps the whole register file of one wave.
 template <int NB, typename T>
-__global__ __launch_bounds__(256) void sigma_epi_mfma_kernel(XcdGrid xg, DevDims d, const double2* __restrict__ G,
+__global__ __launch_bounds__(256, NB < 3 ? 3 : 1) void sigma_epi_mfma_kernel(XcdGrid xg, DevDims d, const double2* __restrict__ G,
                                                              const double2* __restrict__ Vt,
                                                              const double2* __restrict__ R,
                                                              double2* __restrict__ Sigma,
                                                              const int* __restrict__ kminus, int nks, int t0,
                                                              double scale, int upper, int mirror) {
   typedef Mfma<T> M;
-  typedef typename M::acc_t acc_t;
   constexpr bool F64 = sizeof(T) == 8;
   constexpr int N16 = 16 * NB, LD = N16 + 1, PL = N16 * LD;
   extern __shared__ double lds_d[];
@@ -380,20 +407,20 @@ __global__ __launch_bounds__(256) void sigma_epi_mfma_kernel(XcdGrid xg, DevDims
       for (int y = 0; y < NB; ++y)
 #pragma unroll
         for (int r = 0; r < 4; ++r) { sr[x][y][r] = 0.0; si[x][y][r] = 0.0; }
-    acc_t ar64[NB][NB], ai64[NB][NB];  // FP64 flavour accumulates straight in the MFMA registers
+    CAcc<T> acc64[NB][NB];  // FP64 flavour accumulates straight in the MFMA registers
 #pragma unroll
     for (int x = 0; x < NB; ++x)
 #pragma unroll
-      for (int y = 0; y < NB; ++y) { ar64[x][y] = M::zero(); ai64[x][y] = M::zero(); }
+      for (int y = 0; y < NB; ++y) acc64[x][y].zero();
 
     for (int Q = wave; Q < d.NQ; Q += 4) {
       const double2* vt = Vt + ((int64_t)k * d.NQ + Q) * nao2;                   // vt[m * nao + i] = V[i][m]
       const double2* rq = R + (((int64_t)tz * nks + k) * d.NQ + Q) * nao2;      // rq[n * nao + j]
 #pragma unroll
       for (int bi = 0; bi < NB; ++bi) {
-        acc_t tr[NB], ti[NB];
+        CAcc<T> tq[NB];
 #pragma unroll
-        for (int x = 0; x < NB; ++x) { tr[x] = M::zero(); ti[x] = M::zero(); }
+        for (int x = 0; x < NB; ++x) tq[x].zero();
         const int i = 16 * bi + lrow;
         double2 bvs[4 * NB];  // this lane's V_Q elements of row block bi, all slabs in flight together
 #pragma unroll
@@ -408,14 +435,14 @@ __global__ __launch_bounds__(256) void sigma_epi_mfma_kernel(XcdGrid xg, DevDims
 #pragma unroll
           for (int bn = 0; bn < NB; ++bn) {
             const int n = 16 * bn + lrow;
-            cmfma<T>(tr[bn], ti[bn], Gr[m * LD + n], Gi[m * LD + n], (T)bvs[sl].x, (T)bvs[sl].y);
+            tq[bn].mac(Gr[m * LD + n], Gi[m * LD + n], (T)bvs[sl].x, (T)bvs[sl].y);
           }
         }
-        acc_t pr[NB], pi[NB];
+        CAcc<T> pq[NB];
 #pragma unroll
         for (int y = 0; y < NB; ++y) {
-          pr[y] = F64 ? ar64[bi][y] : M::zero();
-          pi[y] = F64 ? ai64[bi][y] : M::zero();
+          if (F64) pq[y] = acc64[bi][y];
+          else pq[y].zero();
         }
 #pragma unroll
         for (int bn = 0; bn < NB; ++bn)
@@ -423,26 +450,25 @@ __global__ __launch_bounds__(256) void sigma_epi_mfma_kernel(XcdGrid xg, DevDims
           for (int r = 0; r < 4; ++r) {
             if (sizeof(T) == 8 && 16 * bn + 4 * r >= nao) continue;  // K slab wholly in the padding
             const int n = 16 * bn + M::row_of(lk, r);
-            const T xr = tr[bn][r], xi = ti[bn][r];
+            const T xr = tq[bn].re(r), xi = tq[bn].im(r);
 #pragma unroll
             for (int bj = 0; bj < NB; ++bj) {
               if (upper && bj < bi) continue;  // block below the diagonal: mirrored afterwards
               const int j = 16 * bj + lrow;
               double2 rv = make_double2(0.0, 0.0);
               if (n < nao && j < nao) rv = rq[n * nao + j];
-              cmfma<T>(pr[bj], pi[bj], xr, xi, (T)rv.x, (T)rv.y);
+              pq[bj].mac(xr, xi, (T)rv.x, (T)rv.y);
             }
           }
 #pragma unroll
         for (int y = 0; y < NB; ++y) {
           if (F64) {
-            ar64[bi][y] = pr[y];
-            ai64[bi][y] = pi[y];
+            acc64[bi][y] = pq[y];
           } else {
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-              sr[bi][y][r] += (double)pr[y][r];
-              si[bi][y][r] += (double)pi[y][r];
+              sr[bi][y][r] += (double)pq[y].re(r);
+              si[bi][y][r] += (double)pq[y].im(r);
             }
           }
         }
@@ -455,8 +481,8 @@ __global__ __launch_bounds__(256) void sigma_epi_mfma_kernel(XcdGrid xg, DevDims
         for (int y = 0; y < NB; ++y)
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            sr[x][y][r] = (double)ar64[x][y][r];
-            si[x][y][r] = (double)ai64[x][y][r];
+            sr[x][y][r] = (double)acc64[x][y].re(r);
+            si[x][y][r] = (double)acc64[x][y].im(r);
           }
     }
     // reduce the four waves' partial sums into wave 0
