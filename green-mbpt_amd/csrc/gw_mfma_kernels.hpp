@@ -359,9 +359,10 @@ __global__ __launch_bounds__(256, NB < 3 ? 2 : 1) void z_prep_slice_kernel(XcdGr
 // directly, G (full mesh) is read at kminus[k].  The full mesh has nks = nk; the star-of-q
 // evaluation passes the irreducible k points only.
 // With T = float each Q's product (K <= 16 NB) is added to FP64 accumulators.
-// Occupancy: for NB <= 2 the register budget is set for three waves per SIMD (a 128-byte spill per
-// lane; 1058 ms per evaluation at nk = 64, nao = 26, naux = 300 against 1095 ms at two waves);
-// NB = 3 keeps the whole register file of one wave.
+// Occupancy: for NB <= 2 the register budget is set for three waves per SIMD (161 VGPRs, no spill;
+// 1036 ms per evaluation at nk = 64, nao = 26, naux = 300 against 1095 ms at two waves).  V_Q is
+// loaded slab by slab inside the first product, so the scheduler, not an explicit prefetch
+// array, decides how many loads are in flight (the array cost 64 VGPRs and a spill at three).
 template <int NB, typename T>
 __global__ __launch_bounds__(256, NB < 3 ? 3 : 1) void sigma_epi_mfma_kernel(XcdGrid xg, DevDims d, const double2* __restrict__ G,
                                                              const double2* __restrict__ Vt,
@@ -422,20 +423,15 @@ __global__ __launch_bounds__(256, NB < 3 ? 3 : 1) void sigma_epi_mfma_kernel(Xcd
 #pragma unroll
         for (int x = 0; x < NB; ++x) tq[x].zero();
         const int i = 16 * bi + lrow;
-        double2 bvs[4 * NB];  // this lane's V_Q elements of row block bi, all slabs in flight together
-#pragma unroll
-        for (int sl = 0; sl < 4 * NB; ++sl) {
-          const int m = 4 * sl + lk;
-          bvs[sl] = (m < nao && i < nao) ? vt[m * nao + i] : make_double2(0.0, 0.0);
-        }
 #pragma unroll
         for (int sl = 0; sl < 4 * NB; ++sl) {
           if (sl >= nslab) break;
           const int m = 4 * sl + lk;
+          const double2 bv = (m < nao && i < nao) ? vt[m * nao + i] : make_double2(0.0, 0.0);
 #pragma unroll
           for (int bn = 0; bn < NB; ++bn) {
             const int n = 16 * bn + lrow;
-            tq[bn].mac(Gr[m * LD + n], Gi[m * LD + n], (T)bvs[sl].x, (T)bvs[sl].y);
+            tq[bn].mac(Gr[m * LD + n], Gi[m * LD + n], (T)bv.x, (T)bv.y);
           }
         }
         CAcc<T> pq[NB];
