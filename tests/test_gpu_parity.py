@@ -4,7 +4,8 @@ Bar: FP64 Sigma within 1e-10 relative (max-abs difference over max |Sigma|) of t
 on the same seeded inputs; the reference's own fixtures within its 1e-6 abs tolerance.  The
 orbital-size products have two forms -- register-chained MFMA kernels (nao <= 48) and batched
 FP64 GEMMs (any nao; tuning key orbital_gemm) -- and both are checked against the oracle,
-including nao = 50 (> 48, GEMM form only), X2C, unrestricted and split precision.
+including nao = 50 (> 48, GEMM form only), X2C, unrestricted and split precision; the register
+form at one, two and three 16-row blocks of nao (16, 17, 33, 40, 48).
 """
 import numpy as np
 import pytest
@@ -181,6 +182,38 @@ def test_orbital_gemm_form(case):
         msg += f", vs MFMA kernels {rel_err(S, S_mfma):.2e}"
         assert rel_err(S, S_mfma) < 1e-12
     print(msg, flush=True)
+    assert rel_err(P, P_ref) < TOL_FP64
+    assert rel_err(S, S_ref) < TOL_FP64
+    assert rel_err(S_split, S_ref) < 1e-8
+
+
+@pytest.mark.parametrize("case", [
+    dict(nk=2, mesh=(2, 1, 1), nao=16, NQ=12, ns=1),             # one 16-block, no padding
+    dict(nk=2, mesh=(2, 1, 1), nao=17, NQ=12, ns=1),             # two blocks, one row in the second
+    dict(nk=2, mesh=(2, 1, 1), nao=33, NQ=10, ns=1),             # three blocks, ragged
+    dict(nk=2, mesh=(2, 1, 1), nao=48, NQ=10, ns=1),             # three full blocks (largest register form)
+    dict(nk=2, mesh=(2, 1, 1), nao=40, NQ=8, ns=2),              # three blocks, unrestricted
+    dict(nk=1, mesh=(1, 1, 1), nao=40, NQ=6, ns=1, nso=80),      # three blocks, X2C spin blocks
+])
+def test_register_kernels_every_block_count(case):
+    """The register-chained orbital kernels are instantiated for one, two and three 16-row blocks
+    of nao; each instantiation (Hermitian-G upper-block epilogue, fused P0 slicing for one spin
+    block, the separate slicing pass otherwise) against the oracle in both precisions."""
+    nso = case.get("nso", case["nao"])
+    G, V, T, d = synthetic(case["nk"], case["nao"], case["NQ"], ns=case["ns"], beta=20.0, lam=1e3,
+                           mesh=case["mesh"], seed=17, nso=nso, sigma=0.3)
+    k = _kernel(d, T)
+    try:
+        k.set_grid(T.T_bt, T.T_tb)
+        k.upload_V(V)
+        S = k.solve(G)
+        P = k.P_tilde()
+        S_split = k.solve(G, gl.PREC_SPLIT)
+    finally:
+        k.close()
+    S_ref, P_ref = _oracle(G, V, T, d)
+    print(f"register kernels {case}: P {rel_err(P, P_ref):.2e}, Sigma {rel_err(S, S_ref):.2e}, "
+          f"split {rel_err(S_split, S_ref):.2e}", flush=True)
     assert rel_err(P, P_ref) < TOL_FP64
     assert rel_err(S, S_ref) < TOL_FP64
     assert rel_err(S_split, S_ref) < 1e-8
