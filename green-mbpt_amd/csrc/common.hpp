@@ -39,4 +39,19 @@ __device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// A HIP event owned by a scope: destroyed when the scope ends, also when an error unwinds it.
+// Event(false) holds no event (for timing that is switched off).
+struct Event {
+  hipEvent_t e = nullptr;
+  explicit Event(bool create = true) {
+    if (create) GWHIP_CHECK(hipEventCreate(&e));
+  }
+  ~Event() {
+    if (e) (void)hipEventDestroy(e);
+  }
+  Event(const Event&) = delete;
+  Event& operator=(const Event&) = delete;
+  operator hipEvent_t() const { return e; }
+};
+
 }  // namespace gwhip

@@ -89,17 +89,13 @@ public:
       graphs_.back().gen = gen;
       e = &graphs_.back();
     }
-    hipEvent_t a, b;
-    GWHIP_CHECK(hipEventCreate(&a));
-    GWHIP_CHECK(hipEventCreate(&b));
+    Event a, b;
     GWHIP_CHECK(hipEventRecord(a, stream_));
     GWHIP_CHECK(hipGraphLaunch(e->exec, stream_));
     GWHIP_CHECK(hipEventRecord(b, stream_));
     GWHIP_CHECK(hipEventSynchronize(b));
     float ms = 0.f;
     GWHIP_CHECK(hipEventElapsedTime(&ms, a, b));
-    (void)hipEventDestroy(a);
-    (void)hipEventDestroy(b);
     last_ms_ = ms;
     last_nodes_ = e->nodes;
     int flag = 0;

@@ -144,9 +144,7 @@ void gw_hip_kernel::hf_device(const double2* dm, const double2* S, double madelu
   if (q0 < 0) throw status_error(GWHIP_ERR_ARG, "k + q map has no identity row (q = 0)");
   const double2* Vd = Vq_ + q0 * nk * NQ * nao2;
   if (!hf_work_) hf_work_ = dalloc<double2>((size_t)std::max<int64_t>(NQ, nmat * nso2));
-  hipEvent_t e0, e1;
-  GWHIP_CHECK(hipEventCreate(&e0));
-  GWHIP_CHECK(hipEventCreate(&e1));
+  Event e0, e1;
   GWHIP_CHECK(hipEventRecord(e0, stream_));
   zero(F, nmat * nso2);
   hipLaunchKernelGGL(hf_direct_upper_kernel, dim3((unsigned)NQ), dim3(256), 0, stream_, Vd, dm, (int)nk, (int)NQ,
@@ -160,9 +158,7 @@ void gw_hip_kernel::hf_device(const double2* dm, const double2* S, double madelu
   const int NB = (int)ceil_div(nao, 16);
   const int nblk = x2c ? 3 : 1;
   if (!hf_x_) hf_x_ = dalloc<double2>((size_t)nblk * nk * nk * nao2);
-  hipEvent_t ex0, ex1;
-  GWHIP_CHECK(hipEventCreate(&ex0));
-  GWHIP_CHECK(hipEventCreate(&ex1));
+  Event ex0, ex1;
   float ex_ms = 0.f;
   const OrbOp A{Vq_, nk * NQ * nao2, NQ * nao2, nao2, nao, 1, 0, 0};  // V(k, k+q)_Q (i, a)
   const OrbOp C{Vq_, nk * NQ * nao2, NQ * nao2, nao2, 1, nao, 0, 1};  // V(k, k+q)_Q^H (b, j)
@@ -216,10 +212,6 @@ void gw_hip_kernel::hf_device(const double2* dm, const double2* S, double madelu
   hf_ms_ = ms;
   hf_exchange_ms_ = ex_ms;
   hf_exchange_launches_ = nspin * nblk;
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
-  (void)hipEventDestroy(ex0);
-  (void)hipEventDestroy(ex1);
 }
 
 // The exchange products X[blk][q][k] = sum_Q V(k, k+q)_Q dm(k+q)_blk V(k, k+q)_Q^H as batched FP64

@@ -668,11 +668,7 @@ void gw_hip_kernel::gj_inverse(double2* X, int64_t nmat) {
 void gw_hip_kernel::stage_dyson(int64_t nq, const int* q0_row) {
   const int64_t NQ = d_.NQ, NQ2 = NQ * NQ, half = d_.nts / 2, nwb = d_.nw_b, nts = d_.nts;
 
-  hipEvent_t e0 = nullptr, e1 = nullptr, e2 = nullptr, e3 = nullptr;
-  if (timing_) {
-    GWHIP_CHECK(hipEventCreate(&e0)); GWHIP_CHECK(hipEventCreate(&e1));
-    GWHIP_CHECK(hipEventCreate(&e2)); GWHIP_CHECK(hipEventCreate(&e3));
-  }
+  Event e0(timing_), e1(timing_), e2(timing_), e3(timing_);
   auto mark = [&](hipEvent_t e) { if (timing_) GWHIP_CHECK(hipEventRecord(e, stream_)); };
   float tw = 0.f, bse = 0.f, wt = 0.f;
   for (int64_t q0 = 0; q0 < nq; q0 += qchunk_) {
@@ -733,7 +729,6 @@ void gw_hip_kernel::stage_dyson(int64_t nq, const int* q0_row) {
   region_ms_[R_TW] += tw;
   region_ms_[R_BSE] += bse;
   region_ms_[R_WT] += wt;
-  (void)hipEventDestroy(e0); (void)hipEventDestroy(e1); (void)hipEventDestroy(e2); (void)hipEventDestroy(e3);
 }
 
 void gw_hip_kernel::stage_sigma(const double2* G, double2* Sigma, bool split) {
@@ -931,9 +926,7 @@ void gw_hip_kernel::solve_device(const double2* G, double2* Sigma, int precision
   const int64_t nsig = d_.nts * d_.ns * d_.nk * d_.nso * d_.nso;
   ensure_orb_buffers();
 
-  hipEvent_t t0, t1, t2, t3;
-  GWHIP_CHECK(hipEventCreate(&t0)); GWHIP_CHECK(hipEventCreate(&t1));
-  GWHIP_CHECK(hipEventCreate(&t2)); GWHIP_CHECK(hipEventCreate(&t3));
+  Event t0, t1, t2, t3;
   GWHIP_CHECK(hipEventRecord(t0, stream_));
   zero(Sigma, nsig);
   stored_valid_ = false;
@@ -961,7 +954,6 @@ void gw_hip_kernel::solve_device(const double2* G, double2* Sigma, int precision
   GWHIP_CHECK(hipEventElapsedTime(&x, t0, t1)); region_ms_[R_P0] = x;
   GWHIP_CHECK(hipEventElapsedTime(&x, t2, t3)); region_ms_[R_SIGMA] = x;
   GWHIP_CHECK(hipEventElapsedTime(&x, t0, t3)); region_ms_[R_TOTAL] = x;
-  (void)hipEventDestroy(t0); (void)hipEventDestroy(t1); (void)hipEventDestroy(t2); (void)hipEventDestroy(t3);
   int flag = 0;
   GWHIP_CHECK(hipMemcpy(&flag, flag_, sizeof(int), hipMemcpyDeviceToHost));
   if (flag) throw status_error(GWHIP_ERR_NONFINITE, "Sigma contains non-finite values");
@@ -979,9 +971,7 @@ void gw_hip_kernel::autotune(const double2* G, int precision) {
   ensure_orb_buffers();
   detect_hermitian_G(G);
   const int64_t half = d_.nts / 2, tcp = std::min(tchunk_p0_, half), tcs = std::min(tchunk_sigma_, half);
-  hipEvent_t a, b;
-  GWHIP_CHECK(hipEventCreate(&a));
-  GWHIP_CHECK(hipEventCreate(&b));
+  Event a, b;
   auto best_of = [&](int ncand, const int* cand, auto&& run) {
     run(cand[0]);  // warm-up (module load, LDS attribute)
     GWHIP_CHECK(hipStreamSynchronize(stream_));
@@ -1040,8 +1030,6 @@ void gw_hip_kernel::autotune(const double2* G, int precision) {
     tune_.s_oz_tile = bs / 100;
     tune_.s_group = bs % 100;
   }
-  (void)hipEventDestroy(a);
-  (void)hipEventDestroy(b);
   stored_valid_ = false;
   ++generation_;
 }
@@ -1262,17 +1250,13 @@ double debug_mfma_peak(int chains, int wps, int iters) {
   };
   launch();
   GWHIP_CHECK(hipDeviceSynchronize());
-  hipEvent_t e0, e1;
-  GWHIP_CHECK(hipEventCreate(&e0));
-  GWHIP_CHECK(hipEventCreate(&e1));
+  Event e0, e1;
   GWHIP_CHECK(hipEventRecord(e0, nullptr));
   launch();
   GWHIP_CHECK(hipEventRecord(e1, nullptr));
   GWHIP_CHECK(hipEventSynchronize(e1));
   float ms = 0.f;
   GWHIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   (void)hipFree(out);
   double per_wave_iter;
   if (chains == 10) per_wave_iter = 8 * 16.0 * 16 * 64 * 2;
@@ -1319,17 +1303,13 @@ double debug_gemm_bench(int which, int NQ, int nk, int nao, int batch, int shape
   a.conjB = 1;
   zgemm_f64(a, akc, bkc, nullptr, shape);
   GWHIP_CHECK(hipDeviceSynchronize());
-  hipEvent_t e0, e1;
-  GWHIP_CHECK(hipEventCreate(&e0));
-  GWHIP_CHECK(hipEventCreate(&e1));
+  Event e0, e1;
   GWHIP_CHECK(hipEventRecord(e0, nullptr));
   for (int r = 0; r < reps; ++r) zgemm_f64(a, akc, bkc, nullptr, shape);
   GWHIP_CHECK(hipEventRecord(e1, nullptr));
   GWHIP_CHECK(hipEventSynchronize(e1));
   float ms = 0.f;
   GWHIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   (void)hipFree(dA);
   (void)hipFree(dB);
   (void)hipFree(dC);

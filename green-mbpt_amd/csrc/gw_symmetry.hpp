@@ -303,9 +303,7 @@ void gw_hip_kernel::solve_star_device(const double2* G_ibz, double2* Sigma_ibz, 
   for (double& r : region_ms_) r = 0.0;
   for (auto& k : kstat_) k = KernelStat{};
 
-  hipEvent_t t0, t1, t2, t3;
-  GWHIP_CHECK(hipEventCreate(&t0)); GWHIP_CHECK(hipEventCreate(&t1));
-  GWHIP_CHECK(hipEventCreate(&t2)); GWHIP_CHECK(hipEventCreate(&t3));
+  Event t0, t1, t2, t3;
   GWHIP_CHECK(hipEventRecord(t0, stream_));
   zero(Sigma_ibz, nsig);
   stored_valid_ = false;
@@ -340,7 +338,6 @@ void gw_hip_kernel::solve_star_device(const double2* G_ibz, double2* Sigma_ibz, 
   GWHIP_CHECK(hipEventElapsedTime(&x, t0, t1)); region_ms_[R_P0] = x;
   GWHIP_CHECK(hipEventElapsedTime(&x, t2, t3)); region_ms_[R_SIGMA] = x;
   GWHIP_CHECK(hipEventElapsedTime(&x, t0, t3)); region_ms_[R_TOTAL] = x;
-  (void)hipEventDestroy(t0); (void)hipEventDestroy(t1); (void)hipEventDestroy(t2); (void)hipEventDestroy(t3);
   int flag = 0;
   GWHIP_CHECK(hipMemcpy(&flag, flag_, sizeof(int), hipMemcpyDeviceToHost));
   if (flag) throw status_error(GWHIP_ERR_NONFINITE, "Sigma contains non-finite values");

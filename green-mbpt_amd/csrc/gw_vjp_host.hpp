@@ -257,9 +257,7 @@ void gw_hip_kernel::vjp_device(const double2* G, const double2* Sbar, double2* G
   auto goff = [&](int64_t t, int64_t is) { return (t * d_.ns + is) * nk * nso2; };
   const size_t nsig = (size_t)nts * d_.ns * nk * nso2;
 
-  hipEvent_t e0, e1;
-  GWHIP_CHECK(hipEventCreate(&e0));
-  GWHIP_CHECK(hipEventCreate(&e1));
+  Event e0, e1;
   GWHIP_CHECK(hipEventRecord(e0, stream_));
   GWHIP_CHECK(hipMemsetAsync(Gbar, 0, nsig * sizeof(double2), stream_));
 
@@ -393,8 +391,6 @@ void gw_hip_kernel::vjp_device(const double2* G, const double2* Sbar, double2* G
   float ms = 0.f;
   GWHIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
   vjp_ms_ = ms;
-  (void)hipEventDestroy(e0);
-  (void)hipEventDestroy(e1);
   int flag = 0;
   GWHIP_CHECK(hipMemcpy(&flag, flag_, sizeof(int), hipMemcpyDeviceToHost));
   if (flag) throw status_error(GWHIP_ERR_NONFINITE, "G_bar contains non-finite values");

@@ -94,7 +94,7 @@ __device__ __forceinline__ void orb_stage(const OrbArgs& a, const OrbOp& op, int
 
 // X_Q = L V_Q R.  grid (ceil(NQ / (4 QPW)), nk, tcount); LDS 4 planes of [16NB][16NB+1] doubles.
 template <int NB, bool HAS_L, bool HAS_R>
-__global__ __launch_bounds__(256) void orb_sandwich_kernel(OrbArgs a, OrbOp L, OrbOp V, OrbOp R, OrbOut X) {
+__global__ __launch_bounds__(256, NB < 3 ? 3 : 1) void orb_sandwich_kernel(OrbArgs a, OrbOp L, OrbOp V, OrbOp R, OrbOut X) {
   static_assert(HAS_L || HAS_R, "one of L, R must be present");
   constexpr int N16 = 16 * NB, LD = N16 + 1, PL = N16 * LD;
   extern __shared__ double lds_d[];
@@ -189,7 +189,7 @@ __global__ __launch_bounds__(256) void orb_sandwich_kernel(OrbArgs a, OrbOp L, O
 // LDS: M planes (2 x [16NB][16NB+1] doubles) then the cross-wave reduction buffer.
 // Every (k, tz) block owns its output matrix: the k map of O must be one-to-one.
 template <int NB, bool HAS_M>
-__global__ __launch_bounds__(256) void orb_reduce_kernel(OrbArgs a, OrbOp A, OrbOp M, OrbOp C, OrbOut O) {
+__global__ __launch_bounds__(256, NB < 3 ? 3 : 1) void orb_reduce_kernel(OrbArgs a, OrbOp A, OrbOp M, OrbOp C, OrbOut O) {
   constexpr int N16 = 16 * NB, LD = N16 + 1, PL = N16 * LD;
   extern __shared__ double lds_d[];
   double* Mr = lds_d;
