@@ -160,6 +160,10 @@ struct OzGemmArgs {
 // plane-slice row) at a time, its four 16-byte pieces XOR-swizzled by (r >> 2) & 3 so that
 // the 16 rows of one fragment read fall on distinct banks.  Each wave reads its own 16
 // B columns straight from global memory into MFMA fragments (no other wave uses them).
+// SQ_LDS_BANK_CONFLICT per LDS instruction: 4.8 cycles when each LDS store of A covered two
+// plane-slices 2 KB apart, 3.2 with the store of 16 whole rows of one plane-slice (a 1 % faster
+// GEMM); a second swizzle, distinct banks also for every 8 rows of equal parity, left the count
+// unchanged, so what remains is in the fragment reads under a bank mapping not modelled here.
 constexpr int OZ_BM = 32, OZ_BN = 64, OZ_KS = 64;
 
 __device__ __forceinline__ int oz_lds_off(int ps, int rows, int r, int piece) {
@@ -197,7 +201,9 @@ __global__ __launch_bounds__(256, 2) void zgemm_ozaki_kernel(OzGemmArgs p) {
   const int wn = wave * 16;
 
   // A staging slots: 32 rows x 4 pieces, two threads per slot splitting the plane-slices
-  const int a_row = tid >> 3, a_piece = tid & 3, a_par = (tid >> 2) & 1;
+  // A staging: thread -> (plane-slice parity, row, 16-byte piece) with the piece fastest, so the
+  // 64 lanes of one LDS store cover 16 whole rows of one plane-slice (1 KB, no bank conflicts)
+  const int a_row = (tid >> 2) & 31, a_piece = tid & 3, a_par = tid >> 7;
   const int8_t* a_ptr = nullptr;
   if (m0 + a_row < p.M) a_ptr = p.A + b * p.sAb + (int64_t)(m0 + a_row) * p.sAm + 16 * a_piece;
   const int a_lds = oz_lds_off(0, OZ_BM, a_row, a_piece);
@@ -352,7 +358,9 @@ __global__ __launch_bounds__(256, 2) void zgemm_ozaki_pipe_kernel(OzGemmArgs p) 
   const int lrow = lane & 15, lg = lane >> 4;
   const int wn = wave * 16;
 
-  const int a_row = tid >> 3, a_piece = tid & 3, a_par = (tid >> 2) & 1;
+  // A staging: thread -> (plane-slice parity, row, 16-byte piece) with the piece fastest, so the
+  // 64 lanes of one LDS store cover 16 whole rows of one plane-slice (1 KB, no bank conflicts)
+  const int a_row = (tid >> 2) & 31, a_piece = tid & 3, a_par = tid >> 7;
   const int8_t* a_ptr = nullptr;
   if (m0 + a_row < p.M) a_ptr = p.A + b * p.sAb + (int64_t)(m0 + a_row) * p.sAm + 16 * a_piece;
   const int a_lds = oz_lds_off(0, OZ_BM, a_row, a_piece);
