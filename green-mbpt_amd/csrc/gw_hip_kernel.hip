@@ -127,7 +127,7 @@ gw_hip_kernel::~gw_hip_kernel() {
                   (void*)kplus_tab_, (void*)kminus_tab_, (void*)flag_, (void*)hcheck_, (void*)hf_work_, (void*)hf_x_, (void*)ksym_idx_, (void*)ksym_U_, (void*)hf_sym_, (void*)q0_qiq_, (void*)q0_A_, (void*)q0_eps_dev_,
                   (void*)q0_S_, (void*)q0_tmp_, (void*)q0_epst_, (void*)qsym_U_, (void*)star_P_, (void*)star_V_,
                   (void*)star_Vs_, (void*)star_eVs_, (void*)star_km_, (void*)star_q0row_, (void*)orb_W_,
-                  (void*)orb_G_, (void*)orb_part_, (void*)orb_O_})
+                  (void*)orb_G_, (void*)orb_part_, (void*)orb_O_, (void*)epi_part_})
     if (p) (void)hipFree(p);
   if (own_ws_)
     for (void* p : {(void*)Pw_, (void*)X_, (void*)Y_, (void*)scratch_})
@@ -457,18 +457,18 @@ void launch_z_prep_slice(int ns, dim3 grid, size_t lds, hipStream_t st, const De
 template <int NB, typename T>
 void launch_sigma_epi_t(dim3 grid, size_t lds, hipStream_t st, const DevDims& dd, const double2* G, const double2* Vt,
                         const double2* R, double2* S, const int* kminus, int nks, int t0, double scale, int upper,
-                        int mirror) {
+                        int mirror, double2* part) {
   allow_lds(sigma_epi_mfma_kernel<NB, T>, lds);
   const XcdGrid xg = make_xcd_grid(grid);
   hipLaunchKernelGGL((sigma_epi_mfma_kernel<NB, T>), xcd_launch_grid(xg), dim3(256), lds, st, xg, dd, G, Vt, R, S,
-                     kminus, nks, t0, scale, upper, mirror);
+                     kminus, nks, t0, scale, upper, mirror, (int)grid.y, part);
   GWHIP_CHECK(hipGetLastError());
 }
 template <int NB>
 void launch_sigma_epi(dim3 grid, size_t lds, hipStream_t st, const DevDims& dd, const double2* G,
                       const double2* Vt, const double2* R, double2* S, const int* kminus, int nks, int t0,
-                      double scale, int upper, int mirror) {
-  launch_sigma_epi_t<NB, double>(grid, lds, st, dd, G, Vt, R, S, kminus, nks, t0, scale, upper, mirror);
+                      double scale, int upper, int mirror, double2* part) {
+  launch_sigma_epi_t<NB, double>(grid, lds, st, dd, G, Vt, R, S, kminus, nks, t0, scale, upper, mirror, part);
 }
 }  // namespace
 
@@ -778,12 +778,26 @@ void gw_hip_kernel::sigma_q(const double2* G, double2* Sigma, bool split, const 
         sigma_epi_gemm(G, Sigma, Vb, km, nkb, t0, tc, scale);
         return;
       }
-      const dim3 grid((unsigned)nkb, 1u, (unsigned)(2 * tc));
+      const int qs = epi_qsplit(nkb);
+      double2* part = nullptr;
+      if (qs > 1) {
+        ensure_epi_part(nkb);
+        part = epi_part_;
+        zero(part, (int64_t)qs * nts * d_.ns * nkb * d_.nso * d_.nso);
+      }
+      const dim3 grid((unsigned)nkb, (unsigned)qs, (unsigned)(2 * tc));
       const int up = sigma_upper() ? 1 : 0;
       const int n = (int)nkb, t = (int)t0;
-      if (NB == 1) launch_sigma_epi<1>(grid, lds, stream_, dd, G, Vb, scratch_, Sigma, km, n, t, scale, up, 1);
-      else if (NB == 2) launch_sigma_epi<2>(grid, lds, stream_, dd, G, Vb, scratch_, Sigma, km, n, t, scale, up, 1);
-      else launch_sigma_epi<3>(grid, lds, stream_, dd, G, Vb, scratch_, Sigma, km, n, t, scale, up, 1);
+      if (NB == 1) launch_sigma_epi<1>(grid, lds, stream_, dd, G, Vb, scratch_, Sigma, km, n, t, scale, up, 1, part);
+      else if (NB == 2) launch_sigma_epi<2>(grid, lds, stream_, dd, G, Vb, scratch_, Sigma, km, n, t, scale, up, 1, part);
+      else launch_sigma_epi<3>(grid, lds, stream_, dd, G, Vb, scratch_, Sigma, km, n, t, scale, up, 1, part);
+      if (qs > 1) {
+        const int64_t per_tau = d_.ns * nkb * d_.nso * d_.nso;
+        const int64_t nel = 2 * tc * per_tau;
+        hipLaunchKernelGGL(epi_part_reduce_kernel, dim3((unsigned)std::min<int64_t>(4096, ceil_div(nel, 256))), dim3(256),
+                           0, stream_, part, Sigma, qs, nts * per_tau, per_tau, t, (int)tc, (int)nts);
+        GWHIP_CHECK(hipGetLastError());
+      }
     });
   }
 }
@@ -792,7 +806,26 @@ void gw_hip_kernel::sigma_q(const double2* G, double2* Sigma, bool split, const 
 // blocks were accumulated, so fill the rest from them.
 bool gw_hip_kernel::sigma_upper() const { return g_herm_ && d_.pns == 1 && !orb_gemm(); }
 
+int gw_hip_kernel::epi_qsplit(int64_t nks) const {
+  // the full-mesh launch has nks x nts work-groups; under ~2048 (eight per CU) the Q loop is split
+  const int64_t blocks = nks * d_.nts;
+  if (orb_gemm() || blocks >= 2048 || d_.NQ < 32) return 1;
+  return (int)std::min<int64_t>(4, std::min<int64_t>(ceil_div(2048, blocks), d_.NQ / 16));
+}
+
+void gw_hip_kernel::ensure_epi_part(int64_t nks) {
+  const int qs = epi_qsplit(nks);
+  if (qs <= 1) return;
+  const int64_t need = (int64_t)qs * d_.nts * d_.ns * nks * d_.nso * d_.nso;
+  if (need <= epi_part_elems_) return;
+  if (epi_part_) (void)hipFree(epi_part_), bytes_ -= (size_t)epi_part_elems_ * sizeof(double2), epi_part_ = nullptr;
+  epi_part_elems_ = 0;
+  epi_part_ = dalloc<double2>((size_t)need);
+  epi_part_elems_ = need;
+}
+
 void gw_hip_kernel::ensure_orb_buffers() {
+  ensure_epi_part(d_.nk);
   if (!orb_gemm()) return;
   if (!orb_W_) orb_W_ = dalloc<double2>((size_t)d_.nk * d_.NQ * d_.nao * d_.nao);
   if (!orb_G_) orb_G_ = dalloc<double2>((size_t)d_.nk * d_.nso * d_.nso);

@@ -164,7 +164,8 @@ public:
     ++generation_;
   }
   void set_timing(bool on) { timing_ = on; }
-  // buffers of the GEMM form of the orbital contractions; allocates, so call before a capture
+  // buffers of the GEMM form of the orbital contractions and the Sigma epilogue's partial sums
+  // over Q ranges (full mesh); allocates, so call before a capture
   void ensure_orb_buffers();
   // the register-chained orbital kernels handle nao up to 3 x 16
   static constexpr int64_t kOrbMaxNao = 48;
@@ -254,6 +255,8 @@ private:
   double2* orb_G_ = nullptr;  // and one (tau, spin) slice of G at mapped k points, [nk][nso^2]
   double2* orb_part_ = nullptr;  // split-K partial Sigma blocks, [kOrbKsplit][nk][nao^2]
   double2* orb_O_ = nullptr;     // one reduction result per k before it is added to its output, [nk][nao^2]
+  double2* epi_part_ = nullptr;  // Sigma epilogue per Q range, [qsplit][nts][ns][nks][nso^2]
+  int64_t epi_part_elems_ = 0;
   static constexpr int64_t kOrbKsplit = 64;
   int* ksym_idx_ = nullptr;
   double2* ksym_U_ = nullptr;
@@ -327,6 +330,10 @@ private:
   const int8_t* Vs_neg(int64_t q) const;
   const int* eVs_neg(int64_t q) const;
   bool sigma_upper() const;
+  // Q ranges of the register-chained Sigma epilogue for nks k points: 1 when nks x nts work-groups
+  // fill the chip, up to 4 otherwise; ensure_epi_part sizes epi_part_ for it
+  int epi_qsplit(int64_t nks) const;
+  void ensure_epi_part(int64_t nks);
   void finish_sigma(double2* Sigma, int64_t nkS);
   void star_tables();
   const double2* star_rotate(int64_t iq, int64_t q);

@@ -353,8 +353,11 @@ __global__ __launch_bounds__(256, NB < 3 ? 2 : 1) void z_prep_slice_kernel(XcdGr
 // with V = V(k, k - q), G_s = G(tau, k - q) block (a, b)  (src/gw_cpu_kernel.cpp:359-373,
 // Y2 = Y1^T P done by the big GEMM as R = P-tilde conj(V)).  The first MFMA produces
 // T^T = G^T V^T so that its result registers feed the second MFMA's A operand.
-// grid: (nks, 1, tcount) in the XcdGrid order (tcount = 2 x the R rows when mirror is set); wave w
-// takes Q = w, w + 4, ...; the four partial sums meet in LDS.  Block x = k is the position in this
+// grid: (nks, qsplit, tcount) in the XcdGrid order (tcount = 2 x the R rows when mirror is set);
+// block y takes the y-th of qsplit equal Q ranges, wave w its Q = w, w + 4, ...; the four partial
+// sums meet in LDS.  qsplit = 1 (part = nullptr) adds to Sigma; qsplit > 1 stores each range's sum
+// to part[y] ([qsplit][nts][ns][nks][nso^2]), which epi_part_reduce_kernel then adds to Sigma in
+// range order (more work-groups when nks x tcount alone would not fill the chip).  Block x = k is the position in this
 // launch's k list: V (Vt[k]), R ([t][nks][Q][nao^2]) and Sigma ([nts][ns][nks][nso^2]) use it
 // directly, G (full mesh) is read at kminus[k].  The full mesh has nks = nk; the star-of-q
 // evaluation passes the irreducible k points only.
@@ -369,7 +372,8 @@ __global__ __launch_bounds__(256, NB < 3 ? 3 : 1) void sigma_epi_mfma_kernel(Xcd
                                                              const double2* __restrict__ R,
                                                              double2* __restrict__ Sigma,
                                                              const int* __restrict__ kminus, int nks, int t0,
-                                                             double scale, int upper, int mirror) {
+                                                             double scale, int upper, int mirror, int qsplit,
+                                                             double2* __restrict__ part) {
   typedef Mfma<T> M;
   constexpr bool F64 = sizeof(T) == 8;
   constexpr int N16 = 16 * NB, LD = N16 + 1, PL = N16 * LD;
@@ -388,6 +392,7 @@ __global__ __launch_bounds__(256, NB < 3 ? 3 : 1) void sigma_epi_mfma_kernel(Xcd
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lrow = lane & 15, lk = lane >> 4;
   const int nslab = (nao + 3) / 4;
+  const int qspan = (d.NQ + qsplit - 1) / qsplit, qlo = by * qspan, qhi = min(d.NQ, qlo + qspan);
 
   for (int s = 0; s < d.pseudo_ns; ++s) {
     const int is = s / (d.pns * d.pns), a = (s / d.pns) % d.pns, b = s % d.pns;
@@ -414,7 +419,7 @@ __global__ __launch_bounds__(256, NB < 3 ? 3 : 1) void sigma_epi_mfma_kernel(Xcd
 #pragma unroll
       for (int y = 0; y < NB; ++y) acc64[x][y].zero();
 
-    for (int Q = wave; Q < d.NQ; Q += 4) {
+    for (int Q = qlo + wave; Q < qhi; Q += 4) {
       const double2* vt = Vt + ((int64_t)k * d.NQ + Q) * nao2;                   // vt[m * nao + i] = V[i][m]
       const double2* rq = R + (((int64_t)tz * nks + k) * d.NQ + Q) * nao2;      // rq[n * nao + j]
 #pragma unroll
@@ -509,7 +514,9 @@ __global__ __launch_bounds__(256, NB < 3 ? 3 : 1) void sigma_epi_mfma_kernel(Xcd
       }
     }
     if (wave == 0) {
-      double2* S = Sigma + (((int64_t)tau * d.ns + is) * nks + k) * d.nso * d.nso;
+      // whole Q range: added to Sigma; one of qsplit ranges: stored to its own partial slab
+      const int64_t off = (((int64_t)tau * d.ns + is) * nks + k) * d.nso * d.nso;
+      double2* S = part ? part + (int64_t)by * d.nts * d.ns * nks * d.nso * d.nso + off : Sigma + off;
 #pragma unroll
       for (int x = 0; x < NB; ++x)
 #pragma unroll
@@ -521,13 +528,37 @@ __global__ __launch_bounds__(256, NB < 3 ? 3 : 1) void sigma_epi_mfma_kernel(Xcd
             const int i = 16 * x + M::row_of(lk, r);
             if (i >= nao) continue;
             double2* dst = S + (int64_t)(a * nao + i) * d.nso + b * nao + j;
-            double2 v = *dst;
+            double2 v = part ? make_double2(0.0, 0.0) : *dst;
             v.x += scale * sr[x][y][r];
             v.y += scale * si[x][y][r];
             *dst = v;
           }
         }
     }
+  }
+}
+
+// Sigma(tau) += sum over y < qsplit of part[y](tau), in order of y, for the 2 tc tau of one
+// mirrored epilogue launch (tau = t0 + z/2, or nts - 1 - (t0 + z/2) for odd z); per_tau =
+// ns * nks * nso^2 elements, slab = nts * per_tau.
+__global__ void epi_part_reduce_kernel(const double2* __restrict__ part, double2* __restrict__ Sigma, int qsplit,
+                                       int64_t slab, int64_t per_tau, int t0, int tc, int nts) {
+  const int64_t n = (int64_t)2 * tc * per_tau;
+  for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+    const int z = (int)(e / per_tau);
+    const int64_t rem = e - (int64_t)z * per_tau;
+    const int tz = z >> 1, tau = (z & 1) ? nts - 1 - (t0 + tz) : t0 + tz;
+    const int64_t idx = (int64_t)tau * per_tau + rem;
+    double2 acc = part[idx];
+    for (int y = 1; y < qsplit; ++y) {
+      const double2 p = part[(int64_t)y * slab + idx];
+      acc.x += p.x;
+      acc.y += p.y;
+    }
+    double2 v = Sigma[idx];
+    v.x += acc.x;
+    v.y += acc.y;
+    Sigma[idx] = v;
   }
 }
 
