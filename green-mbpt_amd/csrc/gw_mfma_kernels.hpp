@@ -12,6 +12,7 @@
 //   R [tz][k][Q][n][j]                       P-tilde conj(V), per-Q transposed
 #pragma once
 
+#include "../../include/gwhip.h"
 #include "common.hpp"
 #include "gw_kernels.hpp"
 
@@ -37,9 +38,12 @@ __device__ __forceinline__ bool xcd_block(const XcdGrid& g, int& x, int& y, int&
   return true;
 }
 
+// These kernels run 256 work-items per work-group, and a launch holds under 2^32 work-items.
 inline XcdGrid make_xcd_grid(dim3 g) {
   XcdGrid x{(int)g.x, (int)g.y, (int)g.z, 0};
   x.per_xcd = ((int64_t)g.x * g.y * g.z + 7) / 8;
+  if (x.per_xcd * 8 * 256 > (int64_t)0xffffffffLL)
+    throw status_error(GWHIP_ERR_UNSUPPORTED, "orbital kernel launch exceeds 2^32 work-items");
   return x;
 }
 inline dim3 xcd_launch_grid(const XcdGrid& x) { return dim3((unsigned)(x.per_xcd * 8)); }
