@@ -33,10 +33,10 @@ WORKLOAD = dict(nk=64, mesh=(4, 4, 4), nao=26, NQ=300, ns=1, beta=100.0, lam=1e4
 # the bytes the copy reads, WRITE_SIZE 1.0x the bytes it writes.
 # Source: profiles/gw_sigma_nk64_nao26_naux300_hbm_traffic.md
 PMC_PER_LAUNCH_KIB = {
-    "split": {"Sigma GEMM": (4.3294e6, 1.0201e7), "Sigma epilogue": (5.6303e6, 6.7610e4),
-              "P0 GEMM": (1.1327e7, 6.7979e5), "P0 prep": (7.7705e5, 6.7739e6)},  # P0: 16 launches per q
-    "fp64": {"Sigma GEMM": (2.5789e7, 1.0178e7), "Sigma epilogue": (5.6397e6, 6.7608e4),
-             "P0 GEMM": (1.0893e7, 4.2502e4), "P0 prep": (7.4422e5, 1.1678e7)},
+    "split": {"Sigma GEMM": (4.3317e6, 1.0202e7), "Sigma epilogue": (7.0708e6, 2.4060e5),
+              "P0 GEMM": (1.1487e7, 6.7978e5), "P0 prep": (7.7640e5, 6.7749e6)},  # P0: 16 launches per q
+    "fp64": {"Sigma GEMM": (2.6011e7, 1.0218e7), "Sigma epilogue": (7.0731e6, 2.4061e5),
+             "P0 GEMM": (2.2101e7, 4.2422e4), "P0 prep": (7.4386e5, 1.1726e7)},
 }
 PMC_CALIBRATION = (0.5, 1.0)
 # int8 GEMM kernel per block-tile setting (tuning key sigma_split_tile / p0_split_tile)

@@ -366,10 +366,9 @@ __global__ __launch_bounds__(256, NB < 3 ? 2 : 1) void z_prep_slice_kernel(XcdGr
 // directly, G (full mesh) is read at kminus[k].  The full mesh has nks = nk; the star-of-q
 // evaluation passes the irreducible k points only.
 // With T = float each Q's product (K <= 16 NB) is added to FP64 accumulators.
-// Occupancy: for NB <= 2 the register budget is set for three waves per SIMD (161 VGPRs, no spill;
-// 1036 ms per evaluation at nk = 64, nao = 26, naux = 300 against 1095 ms at two waves).  V_Q is
-// loaded slab by slab inside the first product, so the scheduler, not an explicit prefetch
-// array, decides how many loads are in flight (the array cost 64 VGPRs and a spill at three).
+// Occupancy: for NB <= 2 the register budget is set for three waves per SIMD (168 VGPRs and a
+// 112-byte spill per lane; 825 ms per evaluation at nk = 64, nao = 26, naux = 300 against 958 ms at
+// two waves without a spill).  V_Q is loaded slab by slab inside the first product.
 template <int NB, typename T>
 __global__ __launch_bounds__(256, NB < 3 ? 3 : 1) void sigma_epi_mfma_kernel(XcdGrid xg, DevDims d, const double2* __restrict__ G,
                                                              const double2* __restrict__ Vt,
@@ -426,6 +425,58 @@ __global__ __launch_bounds__(256, NB < 3 ? 3 : 1) void sigma_epi_mfma_kernel(Xcd
     for (int Q = qlo + wave; Q < qhi; Q += 4) {
       const double2* vt = Vt + ((int64_t)k * d.NQ + Q) * nao2;                   // vt[m * nao + i] = V[i][m]
       const double2* rq = R + (((int64_t)tz * nks + k) * d.NQ + Q) * nao2;      // rq[n * nao + j]
+      if (F64) {
+        // all row blocks at once: each G element read from LDS and each R element loaded once per Q
+        // serves every row block, and the second product feeds up to NB (NB + 1) / 2 independent
+        // accumulator pairs (one at a time, row block bi = 1 of NB = 2 had a single pair, a
+        // dependent MFMA every other issue); the same sums in the same order as block by block
+        CAcc<T> tq[NB][NB];  // [bi][bn]: T^T block (bn, bi)
+#pragma unroll
+        for (int x = 0; x < NB; ++x)
+#pragma unroll
+          for (int y = 0; y < NB; ++y) tq[x][y].zero();
+#pragma unroll
+        for (int sl = 0; sl < 4 * NB; ++sl) {
+          if (sl >= nslab) break;
+          const int m = 4 * sl + lk;
+          double2 bv[NB];
+#pragma unroll
+          for (int bi = 0; bi < NB; ++bi) {
+            const int i = 16 * bi + lrow;
+            bv[bi] = (m < nao && i < nao) ? vt[m * nao + i] : make_double2(0.0, 0.0);
+          }
+#pragma unroll
+          for (int bn = 0; bn < NB; ++bn) {
+            const int n = 16 * bn + lrow;
+            const T gr = Gr[m * LD + n], gi = Gi[m * LD + n];
+#pragma unroll
+            for (int bi = 0; bi < NB; ++bi) tq[bi][bn].mac(gr, gi, (T)bv[bi].x, (T)bv[bi].y);
+          }
+        }
+#pragma unroll
+        for (int bn = 0; bn < NB; ++bn)
+#pragma unroll
+          for (int r = 0; r < 4; ++r) {
+            if (16 * bn + 4 * r >= nao) continue;  // K slab wholly in the padding
+            const int n = 16 * bn + M::row_of(lk, r);
+            double2 rv[NB];
+#pragma unroll
+            for (int bj = 0; bj < NB; ++bj) {
+              const int j = 16 * bj + lrow;
+              rv[bj] = (n < nao && j < nao) ? rq[n * nao + j] : make_double2(0.0, 0.0);
+            }
+#pragma unroll
+            for (int bi = 0; bi < NB; ++bi) {
+              const T xr = tq[bi][bn].re(r), xi = tq[bi][bn].im(r);
+#pragma unroll
+              for (int bj = 0; bj < NB; ++bj) {
+                if (upper && bj < bi) continue;  // block below the diagonal: mirrored afterwards
+                acc64[bi][bj].mac(xr, xi, (T)rv[bj].x, (T)rv[bj].y);
+              }
+            }
+          }
+        continue;
+      }
 #pragma unroll
       for (int bi = 0; bi < NB; ++bi) {
         CAcc<T> tq[NB];

@@ -66,7 +66,7 @@ struct KWalk {
 };
 
 template <int BM, int BN, bool A_KC, bool B_KC>
-__global__ __launch_bounds__(256) void zgemm_f64_kernel(ZGemmArgs p) {
+__global__ __launch_bounds__(256, (BM * BN > 4096) ? 1 : 2) void zgemm_f64_kernel(ZGemmArgs p) {
   constexpr int BK = ZG_BK;
   constexpr int WM = BM / 2, WN = BN / 2;        // per-wave tile
   constexpr int MB = WM / 16, NB = WN / 16;      // 16x16 blocks per wave
