@@ -216,6 +216,51 @@ __global__ __launch_bounds__(256, NB < 3 ? 3 : 1) void orb_reduce_kernel(OrbArgs
   for (int Q = wave; Q < a.NQ; Q += 4) {
     const double2* ab = ab0 + (int64_t)Q * A.s_q;
     const double2* cb = cb0 + (int64_t)Q * C.s_q;
+    if (HAS_M) {
+      // all row blocks at once (as the Sigma epilogue): each M element from LDS and each C element
+      // loaded once per Q serves every row block, NB^2 independent accumulator pairs in the second
+      // product; the same sums in the same order as row block by row block
+      v4d tr[NB][NB], ti[NB][NB];  // [bi][bn]
+#pragma unroll
+      for (int x = 0; x < NB; ++x)
+#pragma unroll
+        for (int y = 0; y < NB; ++y) { tr[x][y] = v4zero(); ti[x][y] = v4zero(); }
+      for (int sl = 0; sl < nslab; ++sl) {
+        const int m = 4 * sl + lk;
+        double2 av[NB];
+#pragma unroll
+        for (int bi = 0; bi < NB; ++bi) {
+          const int i = 16 * bi + lrow;
+          av[bi] = (m < nao && i < nao) ? ab[(int64_t)i * A.rs + (int64_t)m * A.cs] : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int bn = 0; bn < NB; ++bn) {
+          const int n = 16 * bn + lrow;
+          const double mr = Mr[m * LD + n], mi = Mi[m * LD + n];
+#pragma unroll
+          for (int bi = 0; bi < NB; ++bi) cmfma(tr[bi][bn], ti[bi][bn], mr, mi, av[bi].x, sa * av[bi].y);
+        }
+      }
+#pragma unroll
+      for (int bn = 0; bn < NB; ++bn)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          if (16 * bn + 4 * r >= nao) continue;  // K slab wholly in the padding
+          const int n = 16 * bn + lk + 4 * r;
+          double2 cv[NB];
+#pragma unroll
+          for (int bj = 0; bj < NB; ++bj) {
+            const int j = 16 * bj + lrow;
+            cv[bj] = (n < nao && j < nao) ? cb[(int64_t)n * C.rs + (int64_t)j * C.cs] : make_double2(0.0, 0.0);
+          }
+#pragma unroll
+          for (int bi = 0; bi < NB; ++bi)
+#pragma unroll
+            for (int bj = 0; bj < NB; ++bj)
+              cmfma(acr[bi][bj], aci[bi][bj], tr[bi][bn][r], ti[bi][bn][r], cv[bj].x, sc * cv[bj].y);
+        }
+      continue;
+    }
 #pragma unroll
     for (int bi = 0; bi < NB; ++bi) {
       const int i = 16 * bi + lrow;
