@@ -262,35 +262,41 @@ __global__ __launch_bounds__(256, NB < 3 ? 2 : 1) void z_prep_slice_kernel(XcdGr
         const int m = 4 * sl + lk, p = 16 * bp + lrow;
         avs[sl][bp] = (p < nao && m < nao) ? vt[m * nao + p] : make_double2(0.0, 0.0);
       }
+    // W[:, bn] = V_Q Gq[:, bn] for every column block, then Z[:, bn] = Gb W[:, bn] for all of them
+    // at once: each Gb element read from LDS serves every column block and the second product has
+    // NB^2 independent accumulator pairs; the same sums in the same order as block by block
+    CAcc<double> w[NB][NB];  // [bn][bp]
 #pragma unroll
     for (int bn = 0; bn < NB; ++bn) {
-      CAcc<double> w[NB];
 #pragma unroll
-      for (int x = 0; x < NB; ++x) w[x].zero();
+      for (int x = 0; x < NB; ++x) w[bn][x].zero();
 #pragma unroll
       for (int sl = 0; sl < 4 * NB; ++sl) {
         if (sl >= nslab) break;
         const int m = 4 * sl + lk;
         const double br = Gqr[m * LD + 16 * bn + lrow], bi = Gqi[m * LD + 16 * bn + lrow];
 #pragma unroll
-        for (int bp = 0; bp < NB; ++bp) w[bp].mac(avs[sl][bp].x, avs[sl][bp].y, br, bi);
+        for (int bp = 0; bp < NB; ++bp) w[bn][bp].mac(avs[sl][bp].x, avs[sl][bp].y, br, bi);
       }
+    }
+#pragma unroll
+    for (int bn = 0; bn < NB; ++bn)
 #pragma unroll
       for (int x = 0; x < NB; ++x) z[bn][x].zero();
 #pragma unroll
-      for (int bp = 0; bp < NB; ++bp)
+    for (int bp = 0; bp < NB; ++bp)
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          if (16 * bp + 4 * r >= nao) continue;  // K slab wholly in the padding
-          const int p = 16 * bp + lk + 4 * r;
-          const double bwr = w[bp].re(r), bwi = w[bp].im(r);
+      for (int r = 0; r < 4; ++r) {
+        if (16 * bp + 4 * r >= nao) continue;  // K slab wholly in the padding
+        const int p = 16 * bp + lk + 4 * r;
 #pragma unroll
-          for (int bt = 0; bt < NB; ++bt) {
-            const int t = 16 * bt + lrow;
-            z[bn][bt].mac(Gbr[t * LD + p], Gbi[t * LD + p], bwr, bwi);
-          }
+        for (int bt = 0; bt < NB; ++bt) {
+          const int t = 16 * bt + lrow;
+          const double gr = Gbr[t * LD + p], gi = Gbi[t * LD + p];
+#pragma unroll
+          for (int bn = 0; bn < NB; ++bn) z[bn][bt].mac(gr, gi, w[bn][bp].re(r), w[bn][bp].im(r));
         }
-    }
+      }
     // Re and Im of the finished products
     v4d zr[NB][NB], zi[NB][NB];
 #pragma unroll
