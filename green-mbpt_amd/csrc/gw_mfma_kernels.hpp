@@ -149,57 +149,68 @@ __global__ __launch_bounds__(256, NB < 3 ? 2 : 1) void z_prep_mfma_kernel(XcdGri
       if (Q >= d.NQ) break;
       const double2* vt = Vt + ((int64_t)k * d.NQ + Q) * nao2;  // vt[m * nao + p] = V[p][m]
       double2* zq = Z + (((int64_t)tz * d.nk + k) * d.NQ + Q) * nao2;
-      for (int bn = 0; bn < NB; ++bn) {
-        // W[:, bn] = V Gq[:, bn]
-        CAcc<T> w[NB];
+      // W[:, bn] = V Gq[:, bn] for every column block, then Z[:, bn] (+)= Gb W[:, bn] for all of them
+      // at once (each Gb element read from LDS serves every column block; NB^2 independent
+      // accumulator pairs); the same sums in the same order as column block by column block
+      CAcc<T> w[NB][NB];  // [bn][bp]
 #pragma unroll
-        for (int x = 0; x < NB; ++x) w[x].zero();
-        for (int sl = 0; sl < nslab; ++sl) {
-          const int m = 4 * sl + lk;
+      for (int bn = 0; bn < NB; ++bn)
+#pragma unroll
+        for (int x = 0; x < NB; ++x) w[bn][x].zero();
+      for (int sl = 0; sl < nslab; ++sl) {
+        const int m = 4 * sl + lk;
+        double2 av[NB];
+#pragma unroll
+        for (int bp = 0; bp < NB; ++bp) {
+          const int p = 16 * bp + lrow;
+          av[bp] = (p < nao && m < nao) ? vt[m * nao + p] : make_double2(0.0, 0.0);
+        }
+#pragma unroll
+        for (int bn = 0; bn < NB; ++bn) {
           const T br = Gqr[m * LD + 16 * bn + lrow], bi = Gqi[m * LD + 16 * bn + lrow];
 #pragma unroll
-          for (int bp = 0; bp < NB; ++bp) {
-            const int p = 16 * bp + lrow;
-            double2 av = make_double2(0.0, 0.0);
-            if (p < nao && m < nao) av = vt[m * nao + p];
-            w[bp].mac((T)av.x, (T)av.y, br, bi);
+          for (int bp = 0; bp < NB; ++bp) w[bn][bp].mac((T)av[bp].x, (T)av[bp].y, br, bi);
+        }
+      }
+      // register r of W holds row p = 16 bp + row_of(lk, r)
+      CAcc<T> z[NB][NB];  // [bn][bt]
+#pragma unroll
+      for (int bn = 0; bn < NB; ++bn)
+#pragma unroll
+        for (int x = 0; x < NB; ++x) z[bn][x].zero();
+#pragma unroll
+      for (int bp = 0; bp < NB; ++bp)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          if (sizeof(T) == 8 && 16 * bp + 4 * r >= nao) continue;  // K slab wholly in the padding
+          const int p = 16 * bp + M::row_of(lk, r);
+#pragma unroll
+          for (int bt = 0; bt < NB; ++bt) {
+            const int t = 16 * bt + lrow;
+            const T gr = Gbr[t * LD + p], gi = Gbi[t * LD + p];
+#pragma unroll
+            for (int bn = 0; bn < NB; ++bn) z[bn][bt].mac(gr, gi, w[bn][bp].re(r), w[bn][bp].im(r));
           }
         }
-        // Z[:, bn] (+)= Gb W[:, bn]; register r of W holds row p = 16 bp + row_of(lk, r)
-        CAcc<T> z[NB];
 #pragma unroll
-        for (int x = 0; x < NB; ++x) z[x].zero();
+      for (int bn = 0; bn < NB; ++bn) {
+        const int n = 16 * bn + lrow;
+        if (n >= nao) continue;
 #pragma unroll
-        for (int bp = 0; bp < NB; ++bp)
+        for (int bt = 0; bt < NB; ++bt)
 #pragma unroll
           for (int r = 0; r < 4; ++r) {
-            if (sizeof(T) == 8 && 16 * bp + 4 * r >= nao) continue;  // K slab wholly in the padding
-            const int p = 16 * bp + M::row_of(lk, r);
-            const T bwr = w[bp].re(r), bwi = w[bp].im(r);
-#pragma unroll
-            for (int bt = 0; bt < NB; ++bt) {
-              const int t = 16 * bt + lrow;
-              z[bt].mac(Gbr[t * LD + p], Gbi[t * LD + p], bwr, bwi);
+            const int t = 16 * bt + M::row_of(lk, r);
+            if (t < nao) {
+              double2 v = make_double2((double)z[bn][bt].re(r), (double)z[bn][bt].im(r));
+              if (s > 0) {
+                const double2 o = zq[t * nao + n];
+                v.x += o.x;
+                v.y += o.y;
+              }
+              zq[t * nao + n] = v;
             }
           }
-        const int n = 16 * bn + lrow;
-        if (n < nao) {
-#pragma unroll
-          for (int bt = 0; bt < NB; ++bt)
-#pragma unroll
-            for (int r = 0; r < 4; ++r) {
-              const int t = 16 * bt + M::row_of(lk, r);
-              if (t < nao) {
-                double2 v = make_double2((double)z[bt].re(r), (double)z[bt].im(r));
-                if (s > 0) {
-                  const double2 o = zq[t * nao + n];
-                  v.x += o.x;
-                  v.y += o.y;
-                }
-                zq[t * nao + n] = v;
-              }
-            }
-        }
       }
     }
   }
