@@ -26,7 +26,7 @@ PEAK_INT8_TOPS = 5000.0
 
 WORKLOAD = dict(nk=64, mesh=(4, 4, 4), nao=26, NQ=300, ns=1, beta=100.0, lam=1e4, seed=0, sigma=0.1)
 
-# HBM bytes per launch of the four main kernels on this workload, from two rocprofv3 --pmc runs
+# HBM bytes per launch of the main kernels on this workload, from two rocprofv3 --pmc runs
 # (FETCH_SIZE, WRITE_SIZE in KiB per dispatch) of `bench.py --steps 1 --warmup 0 --no-cpu
 # [--no-check | --precision fp64] --no-autotune` with the same build; each scaled by what the
 # counter reads for a plain 2 GiB device copy (bench.py --copy-calibration): FETCH_SIZE reads 0.5x
@@ -39,6 +39,7 @@ PMC_PER_LAUNCH_KIB = {
              "P0 GEMM": (2.2101e7, 4.2422e4), "P0 prep": (7.4386e5, 1.1726e7)},
 }
 PMC_CALIBRATION = (0.5, 1.0)
+PMC_SOURCE = "profiles/gw_sigma_nk64_nao26_naux300_hbm_traffic.md"
 # int8 GEMM kernel per block-tile setting (tuning key sigma_split_tile / p0_split_tile)
 OZAKI_KERNELS = {0: "zgemm_ozaki_kernel", 1: "zgemm_ozaki64_kernel", 2: "zgemm_ozaki_pipe_kernel"}
 
@@ -51,6 +52,108 @@ def algorithmic_flops(nk, nao, NQ, nts, nw_b, pseudo_ns=1):
     f_dys = 8 * nk * nw_b * (7.0 / 3.0) * NQ ** 3
     f_ir = 8 * nk * 2 * nts * nw_b * NQ ** 2
     return dict(P0=f_p0, Sigma=f_s, Dyson=f_dys, IR=f_ir, total=f_p0 + f_s + f_dys + f_ir)
+
+
+def stage_work(w, T, slices):
+    """Work of each kernel family over one Sigma evaluation of the full mesh (nq = nk):
+    algorithmic flops (the reference's GEMM count, 8 flops per complex MAC; for the two big GEMMs on
+    the half tau grid, which is all the reference computes for P0 and all that R = P-tilde conj(V)
+    needs), int8 operations issued by the split-precision GEMMs, and algorithmic bytes (each operand
+    read once and each result written once, in the representation the kernel reads and writes)."""
+    nk, nao, NQ, nts, nwb = w["nk"], w["nao"], w["NQ"], T.nts, T.nw_b
+    half, nao2, B = nts // 2, nao * nao, 16.0
+    pairs = slices * (slices + 1) // 2
+    kc_s = -(-NQ // 64) * 64
+    kc_p = -(-nao2 // 64) * 64
+    sl = 2 * slices                                   # int8 bytes per element and K step: planes x slices
+    V = nk * NQ * nao2 * B                            # V(k, k +- q) of one q
+    out = {
+        "Sigma GEMM": dict(flops=8.0 * NQ * NQ * nk * nao2 * half * nk,
+                           issued=2.0 * 4 * pairs * NQ * kc_s * nk * nao2 * half * nk,
+                           bytes_fp64=nk * (half * NQ * NQ * B + V + half * nk * NQ * nao2 * B),
+                           bytes_split=nk * (half * NQ * sl * kc_s + nk * nao2 * sl * kc_s + half * nk * NQ * nao2 * B)),
+        "P0 GEMM": dict(flops=8.0 * NQ * NQ * nk * nao2 * half * nk,
+                        issued=2.0 * 4 * pairs * NQ * NQ * nk * kc_p * half * nk,
+                        bytes_fp64=nk * (half * nk * NQ * nao2 * B + V + half * NQ * NQ * B),
+                        bytes_split=nk * (half * NQ * nk * sl * kc_p + NQ * nk * sl * kc_p + half * NQ * NQ * B)),
+        "Sigma epilogue": dict(flops=16.0 * NQ * nao ** 3 * nk * nts * nk, issued=None,
+                               bytes_fp64=nk * (half * nk * NQ * nao2 * B + V + 3 * nts * nk * nao2 * B)),
+        "P0 prep": dict(flops=16.0 * NQ * nao ** 3 * nk * half * nk, issued=None,
+                        bytes_fp64=nk * (V + 2 * half * nk * nao2 * B + half * nk * NQ * nao2 * B),
+                        bytes_split=nk * (V + 2 * half * nk * nao2 * B + half * NQ * nk * sl * kc_p)),
+        "Dyson inverse": dict(flops=8.0 * (4.0 / 3.0) * NQ ** 3 * nk * nwb, issued=None,
+                              bytes_fp64=2.0 * nk * nwb * NQ * NQ * B),
+    }
+    return out
+
+
+def roofline(stats, precision, w, T, slices, tuning):
+    """Roofline of the kernel family with the largest share of the step, from the per-launch times
+    measured with HIP events on the library's stream: achieved = algorithmic flops per launch / mean
+    launch time, frac = achieved / the published peak of the arithmetic the kernel uses (int8 5.0 POP/s
+    for the split GEMMs, FP64 78.6 TFLOP/s otherwise); issued_frac = int8 operations issued / time /
+    peak; traffic = the rocprofv3 counter bytes per launch of this build, traffic_ratio = traffic /
+    algorithmic bytes.  Every family's figures are listed under all_stages."""
+    work = stage_work(w, T, slices)
+    split = precision == "split"
+    int8 = {"Sigma GEMM", "P0 GEMM"} if split else set()
+    names = {"Sigma GEMM": (f"{OZAKI_KERNELS[tuning['sigma_split_tile']]}<{slices}> (R = P-tilde conj(V), int8 slices)"
+                            if split else "zgemm_f64_kernel<64,64,true,false> (R = P-tilde conj(V), FP64 MFMA)"),
+             "P0 GEMM": (f"{OZAKI_KERNELS[tuning['p0_split_tile']]}<{slices}> (P0 = Z conj(V)^T, int8 slices)"
+                         if split else "zgemm_f64_kernel<64,64,true,true> (P0 = Z conj(V)^T, FP64 MFMA)"),
+             "Sigma epilogue": "sigma_epi_mfma_kernel<2,double> (Sigma += (V G) R, FP64 MFMA)",
+             "P0 prep": ("z_prep_slice_kernel<2,5>" if split else "z_prep_mfma_kernel<2,double>") +
+                        " (Z = Gb V Gq, FP64 MFMA)",
+             "Dyson inverse": "gj_inverse_kernel ((I - P0)^-1 per (q, Omega), FP64)"}
+    rows = {}
+    for name, (n, ms) in stats.items():
+        if not n or name not in work:
+            continue
+        wk = work[name]
+        t = ms / n * 1e-3
+        peak = PEAK_INT8_TOPS if name in int8 else PEAK_FP64_TFLOPS
+        flops = wk["flops"] / n
+        alg_bytes = (wk.get("bytes_split") if split and "bytes_split" in wk else wk["bytes_fp64"]) / n
+        r = {"kernel": names[name], "launches": n, "avg_launch_ms": t * 1e3,
+             "achieved": flops / t / 1e12, "peak": peak, "unit": "TFLOP/s",
+             "peak_unit": "TOPS (int8)" if name in int8 else "TFLOP/s (FP64)",
+             "frac": flops / t / 1e12 / peak, "algorithmic_flops_per_launch": flops,
+             "algorithmic_bytes": alg_bytes}
+        if name in int8 and wk["issued"]:
+            r["issued_tops"] = wk["issued"] / n / t / 1e12
+            r["issued_frac"] = r["issued_tops"] / peak
+        pmc = PMC_PER_LAUNCH_KIB.get(precision, {}).get(name)
+        if pmc is not None:
+            tr = 1024.0 * (pmc[0] / PMC_CALIBRATION[0] + pmc[1] / PMC_CALIBRATION[1])
+            r.update({"traffic": tr, "traffic_ratio": tr / alg_bytes, "traffic_source": PMC_SOURCE})
+        else:
+            r["traffic"] = None
+        rows[name] = r
+    dom = max(rows, key=lambda nm: rows[nm]["avg_launch_ms"] * rows[nm]["launches"])
+    roof = {"bound": "mfma", "stage": dom}
+    roof.update({k_: v_ for k_, v_ in rows[dom].items()})
+    roof["all_stages"] = rows
+    return roof
+
+
+def device_gm_energy(G_d, S_d, T, nk, nao, ns):
+    """Galitskii-Migdal correlation energy of device G and Sigma (full mesh) by the library's
+    compute_energy (gwhip_dyson_energy_device)."""
+    import numpy as np
+    import torch
+
+    from gwhip import lib as gl
+
+    wn = (2 * T.n_fermi + 1) * np.pi / T.beta
+    d = gl.Dyson(T.nts, len(wn), ns, nk, nk, nao, nao)
+    try:
+        d.set_grid(T.T_ft, T.T_tf, wn)
+        z = np.zeros((ns, nk, nao, nao), complex)
+        d.set_system(z, z + np.eye(nao))
+        s1 = torch.zeros(2 * z.size, dtype=torch.float64, device=G_d.device)
+        return d.energy_device(G_d.data_ptr(), s1.data_ptr(), S_d.data_ptr())[2]
+    finally:
+        d.close()
 
 
 def cpu_baseline(w, T, budget_s=20.0):
@@ -180,71 +283,32 @@ def main():
     regions = k.timings()
     fl = algorithmic_flops(w["nk"], w["nao"], w["NQ"], T.nts, T.nw_b)
     S_host = S_d.cpu().numpy().view(np.complex128).copy()
-    herm = float(np.abs(S_host - np.conj(np.swapaxes(S_host, -1, -2))).max() / np.abs(S_host).max())
-
-    # roofline of the kernel with the largest share of the step.  Work per launch (one q, all k):
-    # the Sigma GEMM and the P0 GEMM run over the half tau grid (R is formed for tau < nts/2 and
-    # mirrored; P0(beta - tau) = P0(tau)), the Sigma epilogue over all nts, the P0 prep over nts/2.
-    nk_, nao_, NQ_, half = w["nk"], w["nao"], w["NQ"], T.nts // 2
-    ns = args.slices
-    kpad = -(-NQ_ // 64) * 64
-    kp0 = -(-nao_ * nao_ // 64) * 64
-    f64_gemm = 8.0 * NQ_ * NQ_ * nk_ * nao_ ** 2 * half                 # complex MACs x 8
-    orb = 16.0 * NQ_ * nao_ ** 3 * nk_                                   # two nao^3 products per (k, tau, Q)
-    kname = OZAKI_KERNELS[k.tuning()["sigma_split_tile"]]
-    pname = OZAKI_KERNELS[k.tuning()["p0_split_tile"]]
-    if prec == gl.PREC_FP64:
-        table = {
-            "Sigma GEMM": (f64_gemm, PEAK_FP64_TFLOPS, "TFLOP/s",
-                           "zgemm_f64_kernel<64,64,true,false> (Sigma contraction R = P~ conj(V), FP64 MFMA)"),
-            "P0 GEMM": (f64_gemm, PEAK_FP64_TFLOPS, "TFLOP/s", "zgemm_f64_kernel<64,64,true,true> (P0 contraction, FP64 MFMA)"),
-        }
-    else:
-        pairs = ns * (ns + 1) // 2                                       # slice pairs; 4 real products each
-        table = {
-            "Sigma GEMM": (2.0 * 4 * pairs * NQ_ * kpad * nk_ * nao_ ** 2 * half, PEAK_INT8_TOPS, "TOPS",
-                           f"{kname}<{ns}> (Sigma contraction, int8 slices; int8 operations issued)"),
-            "P0 GEMM": (2.0 * 4 * pairs * NQ_ * NQ_ * nk_ * kp0 * half, PEAK_INT8_TOPS, "TOPS",
-                        f"{pname}<{ns}> (P0 contraction, int8 slices; operations of the full product)"),
-        }
-    table["Sigma epilogue"] = (orb * T.nts, PEAK_FP64_TFLOPS, "TFLOP/s",
-                               "sigma_epi_mfma_kernel<2,double> (Sigma += (V G) R, FP64 MFMA, algorithmic flops)")
-    table["P0 prep"] = (orb * half, PEAK_FP64_TFLOPS, "TFLOP/s",
-                        ("z_prep_slice_kernel<2,5>" if prec == gl.PREC_SPLIT else "z_prep_mfma_kernel<2,double>")
-                        + " (Z = Gb V Gq, FP64 MFMA, algorithmic flops)")
-    dom = max(table, key=lambda name: stats[name][1])
-    n_l, ms_l = stats[dom]
-    avg_s = ms_l / max(n_l, 1) * 1e-3
-    work, peak, unit, kdesc = table[dom]
-    roof = {"bound": "mfma", "achieved": work / avg_s / 1e12, "peak": peak, "unit": unit, "kernel": kdesc,
-            "stage": dom}
-    roof["frac"] = roof["achieved"] / roof["peak"]
-    fetch_kib, write_kib = PMC_PER_LAUNCH_KIB[args.precision][dom]
-    traffic = 1024.0 * (fetch_kib / PMC_CALIBRATION[0] + write_kib / PMC_CALIBRATION[1])
-    roof.update({"traffic": traffic, "traffic_unit": "bytes/launch", "traffic_GBps": traffic / avg_s / 1e9,
-                 "launches": n_l, "avg_launch_ms": avg_s * 1e3})
-    others = {}
-    for name, (wk, pk, un, _) in table.items():
-        n2, ms2 = stats[name]
-        if n2:
-            t2 = ms2 / n2 * 1e-3
-            others[name] = {"avg_launch_ms": t2 * 1e3, "achieved": wk / t2 / 1e12, "unit": un, "frac": wk / t2 / 1e12 / pk}
-    roof["all_stages"] = others
+    roof = roofline(stats, args.precision, w, T, args.slices, k.tuning())
 
     check = {}
     if prec == gl.PREC_SPLIT and not args.no_check:
-        from oracle import gw_oracle as go
-        t1 = time.perf_counter()
-        k.solve_device(G_d.data_ptr(), S_d.data_ptr(), gl.PREC_FP64)
+        # the plain FP64 kernel on the same inputs, tuned for FP64, one untimed evaluation, then timed
+        S64_d = torch.empty_like(S_d)
+        if not args.no_autotune:
+            k.autotune(G_d.data_ptr(), gl.PREC_FP64)
+        k.solve_device(G_d.data_ptr(), S64_d.data_ptr(), gl.PREC_FP64)
         torch.cuda.synchronize()
-        fp64_ms = (time.perf_counter() - t1) * 1e3
-        S_ref = S_d.cpu().numpy().view(np.complex128)
+        n64 = max(1, min(args.steps, 3))
+        t1 = time.perf_counter()
+        for i in range(n64):
+            k.solve_device(G_d.data_ptr(), S64_d.data_ptr(), gl.PREC_FP64)
+            print(f"[bench] fp64 step {i} done", file=sys.stderr, flush=True)
+        torch.cuda.synchronize()
+        fp64_ms = (time.perf_counter() - t1) / n64 * 1e3
+        S_ref = S64_d.cpu().numpy().view(np.complex128)
         err = float(np.abs(S_host - S_ref).max() / np.abs(S_ref).max())
-        E_ref = go.gm_energy(G, S_ref, T.T_ft, T.T_tf, w["nk"], w["ns"])
-        E_split = go.gm_energy(G, S_host, T.T_ft, T.T_tf, w["nk"], w["ns"])
+        E_ref = device_gm_energy(G_d, S64_d, T, w["nk"], w["nao"], w["ns"])
+        E_split = device_gm_energy(G_d, S_d, T, w["nk"], w["nao"], w["ns"])
         check = {"fp64_ms_per_step": fp64_ms, "speedup_vs_fp64": fp64_ms / ms_per_step,
                  "max_rel_sigma_err_vs_fp64": err, "gm_energy_fp64": E_ref, "gm_energy_split": E_split,
-                 "gm_energy_rel_err": abs(E_split - E_ref) / abs(E_ref), "slices": args.slices}
+                 "gm_energy_rel_err": abs(E_split - E_ref) / abs(E_ref), "slices": args.slices,
+                 "fp64_steps": n64, "fp64_kernel_ms": {k_: v_[1] for k_, v_ in k.kernel_stats().items()},
+                 "fp64_tuning": k.tuning()}
 
     out = {
         "metric": METRIC, "value": 1e3 / ms_per_step, "unit": "Σ-evals/s", "n_gpus": 1,
@@ -260,7 +324,6 @@ def main():
         "whole_step_tflops": fl["total"] / (ms_per_step * 1e-3) / 1e12,
         "regions_ms": regions,
         "kernel_ms": {k_: v_[1] / args.steps for k_, v_ in stats.items()},
-        "sigma_hermiticity": herm,
         "tuning": k.tuning(), "autotune_s": t_tune,
     }
     if not args.no_cpu:
@@ -587,7 +650,6 @@ def bench_batch(args):
 
     from gwhip import inputs, irgrid
     from gwhip import lib as gl
-    from oracle import gw_oracle as go
 
     beta, lam = 100.0, 1e4
     T = irgrid.Transformer.from_grid(irgrid.load_grid(lam), beta)
@@ -606,7 +668,7 @@ def bench_batch(args):
     gp, sp = [g.data_ptr() for g in G_d], [x.data_ptr() for x in S_d]
     torch.cuda.synchronize()
     flops = sum(algorithmic_flops(nk, nao, NQ, T.nts, T.nw_b, pseudo_ns=2)["total"] for nk, nao, NQ in BATCH_SYSTEMS)
-    res, host = {}, {}
+    res, host, energies = {}, {}, {}
     for prec, name in ((gl.PREC_SPLIT, "split"), (gl.PREC_FP64, "fp64")):
         t0 = time.perf_counter()
         b.autotune(gp, prec)
@@ -626,11 +688,12 @@ def bench_batch(args):
                      "workspace_gb": st["workspace_bytes"] / 1e9, "autotune_s": tune_s, "first_call_s": first_s,
                      "tuning": [s.tuning() for s in b.systems], "algorithmic_tflops": flops / (ms * 1e-3) / 1e12}
         host[name] = [x.cpu().numpy().view(np.complex128).copy() for x in S_d]
+        energies[name] = [device_gm_energy(G_d[i], S_d[i], T, nk, nao, 2)
+                          for i, (nk, nao, NQ) in enumerate(BATCH_SYSTEMS)]
     per_sys = []
     for i, (nk, nao, NQ) in enumerate(BATCH_SYSTEMS):
         Sf, Ss = host["fp64"][i], host["split"][i]
-        Ef = go.gm_energy(Gs[i], Sf, T.T_ft, T.T_tf, nk, 2)
-        Es = go.gm_energy(Gs[i], Ss, T.T_ft, T.T_tf, nk, 2)
+        Ef, Es = energies["fp64"][i], energies["split"][i]
         per_sys.append({"nk": nk, "nao": nao, "naux": NQ,
                         "max_rel_sigma_err_split_vs_fp64": float(np.abs(Ss - Sf).max() / np.abs(Sf).max()),
                         "gm_energy_fp64": Ef, "gm_energy_rel_err": abs(Es - Ef) / abs(Ef)})

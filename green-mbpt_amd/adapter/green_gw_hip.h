@@ -9,15 +9,21 @@
  * three functions below (src/green/mbpt/kernel_factory.h:28-36, common_defs.h:169-180), and
  * `mbpt.exe --kernel HIP` selects it.
  *
+ * Precision: the reference's own flags select the arithmetic of the two contractions --
+ * --P_sp (polarisation P0) and --Sigma_sp (self-energy), src/green/mbpt/common_defs.h:162-163 --
+ * each mapped onto the kernel's split-precision products for that contraction (GWHIP_PREC_SPLIT,
+ * _SPLIT_P0, _SPLIT_SIGMA); the Dyson step, the transforms and all accumulation stay FP64.
+ *
  * This header needs green-mbpt's own headers (params, grids, symmetry, utils, ndarray) and is
- * compiled only inside a green-mbpt build.  Everything device-side sits behind include/gwhip.h;
- * the integral files and symmetry tables are read by libgwhip_h5 (include/gwhip_h5.h), so link
- * both gwhip and gwhip_h5.
+ * compiled inside a green-mbpt build (a CPU test compiles it against declaration stubs of those
+ * types).  Everything device-side sits behind include/gwhip.h; the integral files and symmetry
+ * tables are read by libgwhip_h5 (include/gwhip_h5.h), so link both gwhip and gwhip_h5.
  */
 #ifndef GREEN_MBPT_GW_HIP_H
 #define GREEN_MBPT_GW_HIP_H
 
 #include <green/grids/transformer_t.h>
+#include <green/mbpt/common_utils.h>
 #include <green/mbpt/except.h>
 #include <green/mbpt/kernels.h>
 #include <green/params/params.h>
@@ -39,10 +45,19 @@ namespace green::mbpt::hip {
   using G_type     = utils::shared_object<ztensor<5>>;
   using bz_utils_t = symmetry::brillouin_zone_utils;
 
-  /** Flags of this kernel (hook at src/green/mbpt/common_defs.h:169-180). */
+  /** Flags of this kernel (hook at src/green/mbpt/common_defs.h:169-180).  The precision is taken
+   *  from the reference's --P_sp / --Sigma_sp, so the kernel defines no switch of its own for it. */
   inline void custom_kernel_parameters(params::params& p) {
     p.define<int>("hip_device", "HIP device used by the GW kernel", 0);
-    p.define<bool>("hip_split_precision", "Split-precision P0/Sigma contractions in the HIP GW kernel", false);
+  }
+
+  /** The kernel's precision mode for the reference's P_sp / Sigma_sp flags (common_defs.h:162-163:
+   *  single-precision polarisation / self-energy contractions, gw_cpu_kernel.cpp:96-100,123-127). */
+  inline int precision_mode(bool P_sp, bool Sigma_sp) {
+    if (P_sp && Sigma_sp) return GWHIP_PREC_SPLIT;
+    if (P_sp) return GWHIP_PREC_SPLIT_P0;
+    if (Sigma_sp) return GWHIP_PREC_SPLIT_SIGMA;
+    return GWHIP_PREC_FP64;
   }
 
   namespace detail {
@@ -99,6 +114,8 @@ namespace green::mbpt::hip {
     gw_hip_kernel(bool X2C, const params::params& p, size_t nao, size_t nso, size_t ns, size_t NQ,
                   const grids::transformer_t& ft, const bz_utils_t& bz, const ztensor<4>& S_k) : _ft(ft), _bz(bz) {
       detail::require_one_node();
+      // gw_solver sets X2C = (nao != nso) (src/green/mbpt/gw_solver.h:70); the kernel derives it the same way
+      if (X2C != (nso != nao)) throw mbpt_kernel_error("HIP GW kernel: X2C flag disagrees with nso != nao");
       _ibz  = bz.ink() != bz.nk();
       _nts  = ft.sd().repn_fermi().nts();
       _nw_b = ft.sd().repn_bose().nw();
@@ -108,12 +125,15 @@ namespace green::mbpt::hip {
       d.nao  = nao; d.nso = nso; d.ns = ns; d.NQ = NQ;
       d.nts  = _nts; d.nw_b = _nw_b;
       d.beta = p["BETA"];
-      _split = p["hip_split_precision"];
+      _NQ   = NQ;
+      _prec = precision_mode(p["P_sp"], p["Sigma_sp"]);
       if (utils::context().node_rank == 0) {
         gwhip_handle* h = nullptr;
         check(gwhip_create(&d, p["hip_device"], &h), nullptr);
         _h.reset(h, gwhip_destroy);
         set_transforms();
+        // P0 and P-tilde of the first q are kept for the reference's leakage print
+        check(gwhip_set_leakage_capture(h, utils::context().global_rank == 0 ? 1 : 0), h);
         const std::string path = p["dfintegral_file"];
         detail::load_files(h, bz, path, p["input_file"]);
         detail::resident() = {_h, path};
@@ -140,8 +160,8 @@ namespace green::mbpt::hip {
       if (utils::context().node_rank == 0) {
         const auto* G = reinterpret_cast<const double*>(g.object().data());
         auto*       S = reinterpret_cast<double*>(sigma_tau.object().data());
-        const int prec = _split ? GWHIP_PREC_SPLIT : GWHIP_PREC_FP64;
-        check(_ibz ? gwhip_eval_sigma_ibz(_h.get(), G, S, prec) : gwhip_eval_sigma(_h.get(), G, S, prec), _h.get());
+        check(_ibz ? gwhip_eval_sigma_ibz(_h.get(), G, S, _prec) : gwhip_eval_sigma(_h.get(), G, S, _prec), _h.get());
+        if (utils::context().global_rank == 0) print_leakages();
       }
       sigma_tau.fence();
     }
@@ -150,10 +170,21 @@ namespace green::mbpt::hip {
     const grids::transformer_t& _ft;
     const bz_utils_t&           _bz;
     std::shared_ptr<gwhip_handle> _h;
-    size_t _nts = 0, _nw_b = 0;
-    bool   _split = false, _ibz = false;
+    size_t _nts = 0, _nw_b = 0, _NQ = 0;
+    int    _prec = GWHIP_PREC_FP64;
+    bool   _ibz = false;
 
     static void check(int st, const gwhip_handle* h) { detail::check(st, h); }
+
+    // The reference's leakage print for q_ir == 0 (gw_cpu_kernel.cpp:237-239,276-278): green-grids'
+    // check_chebyshev on P0(tau) and on P-tilde(tau) of that q, through mbpt's print_leakage.
+    void print_leakages() {
+      ztensor<4> P0(_nts, 1, _NQ, _NQ), Pt(_nts, 1, _NQ, _NQ);
+      check(gwhip_get_leakage_inputs(_h.get(), reinterpret_cast<double*>(P0.data()), reinterpret_cast<double*>(Pt.data())),
+            _h.get());
+      print_leakage(_ft.check_chebyshev(P0), "P0");
+      print_leakage(_ft.check_chebyshev(Pt), "P");
+    }
 
     // The transforms are linear maps per aux element: probe them with unit vectors so the
     // device applies exactly green-grids' tau_f_to_w_b / w_b_to_tau_f (gw_cpu_kernel.cpp:234,264).

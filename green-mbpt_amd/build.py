@@ -1,4 +1,5 @@
-"""Build libgwhip.so (HIP, gfx950) and libgwhip_h5.so (host, HDF5 integral reader) in-tree.
+"""Build libgwhip.so (HIP, gfx950), libgwhip_debug.so (throughput probes, HIP) and libgwhip_h5.so
+(host, HDF5 integral reader) in-tree.
 
     python green-mbpt_amd/build.py            # compile
     python green-mbpt_amd/build.py --print    # show the commands only
@@ -18,17 +19,22 @@ ROOT = os.path.dirname(HERE)
 CSRC = os.path.join(HERE, "csrc")
 OUT = os.path.join(HERE, "gwhip", "libgwhip.so")
 OUT_H5 = os.path.join(HERE, "gwhip", "libgwhip_h5.so")
+OUT_DEBUG = os.path.join(HERE, "gwhip", "libgwhip_debug.so")
 HDF5_ROOT = os.environ.get("HDF5_ROOT", "/opt/conda")
 SOURCES = ["gw_hip_kernel.hip"]
 ARCH = os.environ.get("GWHIP_ARCH", "gfx950")
 
 
-def command(out: str = OUT) -> list[str]:
+def command(out: str = OUT, sources=None) -> list[str]:
     hipcc = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "bin", "hipcc")
     return [hipcc, f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
             "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
             "-I", os.path.join(ROOT, "include"), "-I", CSRC,
-            "-o", out] + [os.path.join(CSRC, s) for s in SOURCES]
+            "-o", out] + [os.path.join(CSRC, s) for s in (sources or SOURCES)]
+
+
+def command_debug(out: str = OUT_DEBUG) -> list[str]:
+    return command(out, ["gwhip_debug.hip"])
 
 
 # libhdf5 is reached through a private directory holding only a link to it, so that the run path
@@ -67,6 +73,11 @@ def build(verbose: bool = True) -> str:
         print(" ".join(cmd), flush=True)
     subprocess.run(cmd, check=True)
     os.replace(OUT + ".tmp", OUT)
+    cmd = command_debug(OUT_DEBUG + ".tmp")
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    subprocess.run(cmd, check=True)
+    os.replace(OUT_DEBUG + ".tmp", OUT_DEBUG)
     if os.path.exists(os.path.join(HDF5_ROOT, "include", "hdf5.h")):
         _link_hdf5()
         cmd = command_h5(OUT_H5 + ".tmp")
@@ -82,6 +93,7 @@ def build(verbose: bool = True) -> str:
 if __name__ == "__main__":
     if "--print" in sys.argv:
         print(" ".join(command()))
+        print(" ".join(command_debug()))
         print(" ".join(command_h5()))
     else:
         build()

@@ -39,6 +39,12 @@ __device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -
 
 inline int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
 
+// Precision modes of include/gwhip.h (gwhip_precision): which contractions use split products.
+inline bool precision_valid(int p) { return p >= 0 && p <= 3; }
+inline bool p0_split(int p) { return p == 1 || p == 2; }
+inline bool sigma_split(int p) { return p == 1 || p == 3; }
+inline bool any_split(int p) { return p >= 1 && p <= 3; }
+
 // A HIP event owned by a scope: destroyed when the scope ends, also when an error unwinds it.
 // Event(false) holds no event (for timing that is switched off).
 struct Event {

@@ -47,7 +47,8 @@ public:
     for (int i = 0; i < size(); ++i)
       if (!systems_[i]->has_inputs() || !G[i])
         throw status_error(GWHIP_ERR_STATE, "system " + std::to_string(i) + ": grid, integrals or G missing");
-    prepare(precision == GWHIP_PREC_SPLIT);
+    if (!precision_valid(precision)) throw status_error(GWHIP_ERR_ARG, "unknown precision");
+    prepare(any_split(precision));
     for (auto& s : systems_) s->ensure_orb_buffers();
     for (int i = 0; i < size(); ++i) systems_[i]->autotune(G[i], precision);
     drop_graphs();
@@ -57,12 +58,12 @@ public:
     if (!G || !S) throw status_error(GWHIP_ERR_ARG, "null G or Sigma list");
     for (int i = 0; i < size(); ++i)
       if (!G[i] || !S[i]) throw status_error(GWHIP_ERR_ARG, "null G or Sigma of system " + std::to_string(i));
-    if (precision != GWHIP_PREC_FP64 && precision != GWHIP_PREC_SPLIT) throw status_error(GWHIP_ERR_ARG, "unknown precision");
+    if (!precision_valid(precision)) throw status_error(GWHIP_ERR_ARG, "unknown precision");
     GWHIP_CHECK(hipSetDevice(device_));
     for (int i = 0; i < size(); ++i)
       if (!systems_[i]->has_inputs())
         throw status_error(GWHIP_ERR_STATE, "system " + std::to_string(i) + ": grid or integrals not set");
-    prepare(precision == GWHIP_PREC_SPLIT);
+    prepare(any_split(precision));
     for (auto& s : systems_) {
       s->check_ready(precision);
       s->ensure_orb_buffers();  // before any capture: the GEMM form of the orbital products allocates
@@ -98,6 +99,10 @@ public:
     GWHIP_CHECK(hipEventElapsedTime(&ms, a, b));
     last_ms_ = ms;
     last_nodes_ = e->nodes;
+    for (int i = 0; i < size(); ++i)
+      if (systems_[i]->dyson_failed())
+        throw status_error(GWHIP_ERR_NOT_POSITIVE_DEFINITE,
+                           "system " + std::to_string(i) + ": I - P0(i Omega) is not positive definite for some (q, Omega)");
     int flag = 0;
     GWHIP_CHECK(hipMemcpy(&flag, flag_, sizeof(int), hipMemcpyDeviceToHost));
     if (flag) throw status_error(GWHIP_ERR_NONFINITE, "Sigma of some system contains non-finite values");
@@ -208,14 +213,13 @@ private:
   }
 
   GraphEntry capture(const double2* const* G, double2* const* S, int precision) {
-    const bool split = precision == GWHIP_PREC_SPLIT;
     for (auto& s : systems_) s->set_timing(false);
     hipGraph_t graph = nullptr;
     GWHIP_CHECK(hipStreamBeginCapture(stream_, hipStreamCaptureModeRelaxed));
     try {
       hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, stream_, flag_, (int64_t)1);
       GWHIP_CHECK(hipGetLastError());
-      for (int i = 0; i < size(); ++i) systems_[i]->enqueue_eval(G[i], S[i], split, flag_);
+      for (int i = 0; i < size(); ++i) systems_[i]->enqueue_eval(G[i], S[i], precision, flag_);
     } catch (...) {
       (void)hipStreamEndCapture(stream_, &graph);
       if (graph) (void)hipGraphDestroy(graph);

@@ -138,6 +138,39 @@ __global__ __launch_bounds__(256) void dyson_invert_kernel(const double2* __rest
   }
 }
 
+// out[m] = w[m % ink] * tr(A_m B_m) = w sum_ij A_m[i][j] B_m[j][i] for nmat n x n matrices
+// (A_m at A + m*n^2, B_m at B + (m % nb)*n^2); one work-group per matrix, fixed-order reduction.
+__global__ __launch_bounds__(256) void trace_prod_kernel(const double2* __restrict__ A, const double2* __restrict__ B,
+                                                         int n, int nb, int ink, const double* __restrict__ w,
+                                                         double2* __restrict__ out) {
+  __shared__ double red[2][256];
+  const int64_t m = blockIdx.x;
+  const int n2 = n * n, tid = threadIdx.x;
+  const double2* a = A + m * n2;
+  const double2* b = B + (m % nb) * n2;
+  double tr = 0.0, ti = 0.0;
+  for (int e = tid; e < n2; e += blockDim.x) {
+    const int i = e / n, j = e - (e / n) * n;
+    const double2 x = a[e], y = b[j * n + i];
+    tr += x.x * y.x - x.y * y.y;
+    ti += x.x * y.y + x.y * y.x;
+  }
+  red[0][tid] = tr;
+  red[1][tid] = ti;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if (tid < o) {
+      red[0][tid] += red[0][tid + o];
+      red[1][tid] += red[1][tid + o];
+    }
+    __syncthreads();
+  }
+  if (tid == 0) {
+    const double f = w[(m % nb) % ink];
+    out[m] = make_double2(f * red[0][0], f * red[1][0]);
+  }
+}
+
 // C = A + B elementwise (n elements).
 __global__ void add2_kernel(const double2* __restrict__ A, const double2* __restrict__ B, double2* __restrict__ C, int64_t n) {
   for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
@@ -192,9 +225,9 @@ public:
     (void)hipSetDevice(device_);
     if (stream_) (void)hipStreamSynchronize(stream_);
     for (void* p : {(void*)Tft_, (void*)Ttf_, (void*)wn_, (void*)wk_, (void*)H_, (void*)S_, (void*)HS1_, (void*)Sw_,
-                    (void*)Gw_, (void*)tr_, (void*)in_})
+                    (void*)Gw_, (void*)tr_, (void*)in_, (void*)etr_, (void*)eG_})
       if (p) (void)hipFree(p);
-    Tft_ = Ttf_ = H_ = S_ = HS1_ = Sw_ = Gw_ = tr_ = in_ = nullptr;
+    Tft_ = Ttf_ = H_ = S_ = HS1_ = Sw_ = Gw_ = tr_ = in_ = etr_ = eG_ = nullptr;
     wn_ = wk_ = nullptr;
     if (stream_) (void)hipStreamDestroy(stream_);
     stream_ = nullptr;
@@ -287,6 +320,65 @@ public:
     return {nel, mu};
   }
 
+  // compute_energy (src/common_utils.cpp:5-55) of G(tau), Sigma1, Sigma(tau) (device pointers, read
+  // only) with the handle's H and k weights: {E_1b, E_HF, E_corr}.  G and Sigma go to the Matsubara
+  // axis by one FP64 GEMM each; per (w, s, k) the trace of G(i w) Sigma(i w) and per (s, k) those of
+  // dm H and dm Sigma1 (dm = -(2 or 1) G(beta)) are reduced on the device, the sums over (s, k) and
+  // frequencies on the host in a fixed order.
+  std::array<double, 3> energy_device(const double2* G, const double2* Sigma1, const double2* Sigma_tau) {
+    if (!grid_set_ || !sys_set_) throw status_error(GWHIP_ERR_STATE, "grid or H/S not set");
+    if (!G || !Sigma1 || !Sigma_tau) throw status_error(GWHIP_ERR_ARG, "null G, Sigma1 or Sigma");
+    GWHIP_CHECK(hipSetDevice(device_));
+    const int64_t nsk = d_.ns * d_.ink, N = nsk_n2(), n = d_.nso;
+    stack_transform(stream_, Tft_, (int)d_.nw, (int)d_.nts, false, G, N, Gw_);
+    stack_transform(stream_, Tft_, (int)d_.nw, (int)d_.nts, false, Sigma_tau, N, Sw_);
+    if (d_.nw * nsk > 0x7fffffff) throw status_error(GWHIP_ERR_UNSUPPORTED, "too many (w, s, k) matrices");
+    hipLaunchKernelGGL(trace_prod_kernel, dim3((unsigned)(d_.nw * nsk)), dim3(256), 0, stream_, Gw_, Sw_, (int)n,
+                       (int)(d_.nw * nsk), (int)d_.ink, wk_, tr_);
+    GWHIP_CHECK(hipGetLastError());
+    // tr(G(beta) H) and tr(G(beta) Sigma1) per (s, k), after the nw * nsk traces of tr_
+    const double2* Gb = G + (d_.nts - 1) * N;
+    if (!etr_) etr_ = alloc<double2>((size_t)2 * nsk);
+    hipLaunchKernelGGL(trace_prod_kernel, dim3((unsigned)nsk), dim3(256), 0, stream_, Gb, H_, (int)n, (int)nsk,
+                       (int)d_.ink, wk_, etr_);
+    hipLaunchKernelGGL(trace_prod_kernel, dim3((unsigned)nsk), dim3(256), 0, stream_, Gb, Sigma1, (int)n, (int)nsk,
+                       (int)d_.ink, wk_, etr_ + nsk);
+    GWHIP_CHECK(hipGetLastError());
+    std::vector<double2> tr((size_t)d_.nw * nsk), et((size_t)2 * nsk);
+    GWHIP_CHECK(hipMemcpyAsync(tr.data(), tr_, tr.size() * sizeof(double2), hipMemcpyDeviceToHost, stream_));
+    GWHIP_CHECK(hipMemcpyAsync(et.data(), etr_, et.size() * sizeof(double2), hipMemcpyDeviceToHost, stream_));
+    GWHIP_CHECK(hipStreamSynchronize(stream_));
+    const bool x2c = d_.nso == 2 * d_.nao;
+    const double cdm = (d_.ns == 2 || x2c) ? -1.0 : -2.0;     // dm = cdm G(beta)
+    const double pref = (d_.ns == 1 && !x2c) ? 1.0 : 0.5;     // energy_prefactor
+    double gs_r = 0.0;  // Re sum_w T_tf[beta][w] sum_sk w_k tr(G(i w) Sigma(i w))
+    for (int64_t w = 0; w < d_.nw; ++w) {
+      double sr = 0.0, si = 0.0;
+      for (int64_t sk = 0; sk < nsk; ++sk) {
+        sr += tr[(size_t)w * nsk + sk].x;
+        si += tr[(size_t)w * nsk + sk].y;
+      }
+      gs_r += TtB_[w].x * sr - TtB_[w].y * si;
+    }
+    double e1 = 0.0, ehf = 0.0;
+    for (int64_t sk = 0; sk < nsk; ++sk) {
+      const double tH = cdm * et[sk].x, tS1 = cdm * et[nsk + sk].x;
+      e1 += tH;
+      ehf += 0.5 * (tS1 + 2.0 * tH);
+    }
+    return {e1, ehf, -gs_r * pref};
+  }
+
+  std::array<double, 3> energy_host(const double2* G, const double2* Sigma1, const double2* Sigma_tau) {
+    upload(Sigma1, Sigma_tau);
+    const int64_t N = nsk_n2();
+    double2* g = nullptr;
+    if (!eG_) eG_ = alloc<double2>((size_t)d_.nts * N);
+    g = eG_;
+    GWHIP_CHECK(hipMemcpy(g, G, (size_t)d_.nts * N * sizeof(double2), hipMemcpyHostToDevice));
+    return energy_device(g, in_ + d_.nts * N, in_);
+  }
+
   // Host-buffer variants: Sigma1 [ns][ink][nso^2], Sigma(tau), G [nts][ns][ink][nso^2]; the
   // Hermitised Sigma1 and Sigma(tau) are written back to the caller's arrays.
   void compute_G_host(double2* Sigma1, double2* Sigma_tau, double mu, double2* G) {
@@ -313,7 +405,7 @@ private:
   hipStream_t stream_ = nullptr;
   bool grid_set_ = false, sys_set_ = false;
   double2 *Tft_ = nullptr, *Ttf_ = nullptr, *H_ = nullptr, *S_ = nullptr, *HS1_ = nullptr, *Sw_ = nullptr,
-          *Gw_ = nullptr, *tr_ = nullptr, *in_ = nullptr;
+          *Gw_ = nullptr, *tr_ = nullptr, *in_ = nullptr, *etr_ = nullptr, *eG_ = nullptr;
   double *wn_ = nullptr, *wk_ = nullptr;
   std::vector<double2> TtB_;  // row nts-1 of T_tf (tau = beta)
 

@@ -98,6 +98,8 @@ gw_hip_kernel::gw_hip_kernel(const Dims& d, int device, hipStream_t shared_strea
   kplus_tab_ = dalloc<int>((size_t)nk * nk);
   kminus_tab_ = dalloc<int>((size_t)nk * nk);
   flag_ = dalloc<int>(1);
+  dyson_flag_ = dalloc<int>(1);
+  GWHIP_CHECK(hipMemset(dyson_flag_, 0, sizeof(int)));
   hcheck_ = dalloc<unsigned long long>(2);
   build_maps();
   if (!own_ws_) return;  // a batch binds a shared workspace later
@@ -124,7 +126,7 @@ gw_hip_kernel::~gw_hip_kernel() {
   (void)hipSetDevice(device_);
   if (stream_) (void)hipStreamSynchronize(stream_);
   for (void* p : {(void*)Vq_, (void*)Vt_, (void*)P_, (void*)Tbt_fold_, (void*)Ttb_, (void*)G_dev_, (void*)S_dev_,
-                  (void*)kplus_tab_, (void*)kminus_tab_, (void*)flag_, (void*)hcheck_, (void*)hf_work_, (void*)hf_x_, (void*)ksym_idx_, (void*)ksym_U_, (void*)hf_sym_, (void*)q0_qiq_, (void*)q0_A_, (void*)q0_eps_dev_,
+                  (void*)kplus_tab_, (void*)kminus_tab_, (void*)flag_, (void*)dyson_flag_, (void*)leak_P0_, (void*)hcheck_, (void*)hf_work_, (void*)hf_x_, (void*)ksym_idx_, (void*)ksym_U_, (void*)hf_sym_, (void*)q0_qiq_, (void*)q0_A_, (void*)q0_eps_dev_,
                   (void*)q0_S_, (void*)q0_tmp_, (void*)q0_epst_, (void*)qsym_U_, (void*)star_P_, (void*)star_V_,
                   (void*)star_Vs_, (void*)star_eVs_, (void*)star_km_, (void*)star_q0row_, (void*)orb_W_,
                   (void*)orb_G_, (void*)orb_part_, (void*)orb_O_, (void*)epi_part_})
@@ -306,9 +308,9 @@ bool gw_hip_kernel::detect_hermitian_G(const double2* G) {
 void gw_hip_kernel::check_ready(int precision) const {
   if (!grid_set_) throw status_error(GWHIP_ERR_STATE, "transform grid not set (gwhip_set_grid)");
   if (!v_set_) throw status_error(GWHIP_ERR_STATE, "integrals not set (gwhip_upload_V / gwhip_generate_V)");
-  if (precision != GWHIP_PREC_FP64 && precision != GWHIP_PREC_SPLIT) throw status_error(GWHIP_ERR_ARG, "unknown precision");
+  if (!precision_valid(precision)) throw status_error(GWHIP_ERR_ARG, "unknown precision");
   if (!scratch_) throw status_error(GWHIP_ERR_STATE, "no workspace bound");
-  if (precision == GWHIP_PREC_SPLIT && !own_ws_ && (!split_ready() || !Zs_))
+  if (any_split(precision) && !own_ws_ && (!split_ready() || !Zs_))
     throw status_error(GWHIP_ERR_STATE, "split-precision stores or workspace not prepared");
 }
 
@@ -325,6 +327,13 @@ void gw_hip_kernel::ensure_slices() {
   const int64_t nk = d_.nk, NQ = d_.NQ, nao2 = d_.nao * d_.nao;
   kc_p0_ = ceil_div(nao2, 64) * 64;
   kc_s_ = ceil_div(NQ, 64) * 64;
+  // int32 exactness inside a K chunk: one accumulator sums the products of up to `ns` slice pairs
+  // and two planes, each |x y| <= 127^2 < 2^14, over kc elements
+  const int64_t kc_max = split_kc_limit(ns);
+  if (kc_p0_ > kc_max || kc_s_ > kc_max)
+    throw status_error(GWHIP_ERR_UNSUPPORTED,
+                       "split precision: K chunk of " + std::to_string(std::max(kc_p0_, kc_s_)) +
+                           " exceeds the int32-exact limit " + std::to_string(kc_max) + " (nao^2 and naux must stay below it)");
   Vs_p0_ = dalloc<int8_t>((size_t)nk * NQ * nk * 2 * ns * kc_p0_);
   eVs_p0_ = dalloc<int>((size_t)nk * NQ * nk);
   Vs_s_ = dalloc<int8_t>((size_t)nk * nk * nao2 * 2 * ns * kc_s_);
@@ -659,8 +668,22 @@ void gw_hip_kernel::gj_inverse(double2* X, int64_t nmat) {
   if ((int64_t)gj_lds > max_lds_) throw status_error(GWHIP_ERR_UNSUPPORTED, "NQ too large for the Dyson kernel");
   allow_lds(gj_inverse_kernel, gj_lds);
   const int gj_threads = (int)std::min<int64_t>(1024, std::max<int64_t>(256, ceil_div(NQ, 64) * 64));
-  hipLaunchKernelGGL(gj_inverse_kernel, dim3((unsigned)nmat), dim3(gj_threads), gj_lds, stream_, X, (int)NQ, NQ2, nb);
+  hipLaunchKernelGGL(gj_inverse_kernel, dim3((unsigned)nmat), dim3(gj_threads), gj_lds, stream_, X, (int)NQ, NQ2, nb,
+                     dyson_flag_);
   GWHIP_CHECK(hipGetLastError());
+}
+
+bool gw_hip_kernel::dyson_failed() const {
+  int f = 0;
+  GWHIP_CHECK(hipMemcpy(&f, dyson_flag_, sizeof(int), hipMemcpyDeviceToHost));
+  return f != 0;
+}
+
+void gw_hip_kernel::check_dyson_flag() {
+  if (dyson_failed())
+    throw status_error(GWHIP_ERR_NOT_POSITIVE_DEFINITE,
+                       "I - P0(i Omega) is not positive definite for some (q, Omega): a pivot of its factorisation "
+                       "is not positive (P0 is not the polarisation of a physical G)");
 }
 
 // P-tilde of the first nq rows of P_; q0_row[i] is the AqQ row of row i (-1: none), read when the
@@ -670,6 +693,7 @@ void gw_hip_kernel::stage_dyson(int64_t nq, const int* q0_row) {
 
   Event e0(timing_), e1(timing_), e2(timing_), e3(timing_);
   auto mark = [&](hipEvent_t e) { if (timing_) GWHIP_CHECK(hipEventRecord(e, stream_)); };
+  hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, stream_, dyson_flag_, (int64_t)1);
   float tw = 0.f, bse = 0.f, wt = 0.f;
   for (int64_t q0 = 0; q0 < nq; q0 += qchunk_) {
     const int64_t qc = std::min(qchunk_, nq - q0);
@@ -932,12 +956,12 @@ void gw_hip_kernel::finish_sigma(double2* Sigma, int64_t nkS) {
 }
 
 // One evaluation on the stream, kernel nodes only (Sigma zeroed by a fill kernel).
-void gw_hip_kernel::enqueue_eval(const double2* G, double2* Sigma, bool split, int* nonfinite_flag) {
+void gw_hip_kernel::enqueue_eval(const double2* G, double2* Sigma, int precision, int* nonfinite_flag) {
   const int64_t nsig = d_.nts * d_.ns * d_.nk * d_.nso * d_.nso;
   zero(Sigma, nsig);
-  stage_p0(G, split);
+  stage_p0(G, p0_split(precision));
   stage_dyson(d_.nk, q0_qiq_);
-  stage_sigma(G, Sigma, split);
+  stage_sigma(G, Sigma, sigma_split(precision));
   finish_sigma(Sigma, d_.nk);
   if (nonfinite_flag) {
     hipLaunchKernelGGL(nonfinite_kernel, dim3(1024), dim3(256), 0, stream_, Sigma, nsig, nonfinite_flag);
@@ -948,9 +972,8 @@ void gw_hip_kernel::enqueue_eval(const double2* G, double2* Sigma, bool split, i
 void gw_hip_kernel::solve_device(const double2* G, double2* Sigma, int precision, bool q0_apply) {
   check_ready(precision);
   if (!G || !Sigma) throw status_error(GWHIP_ERR_ARG, "null G or Sigma");
-  const bool split = precision == GWHIP_PREC_SPLIT;
   GWHIP_CHECK(hipSetDevice(device_));
-  if (split) ensure_slices();
+  if (any_split(precision)) ensure_slices();
   timing_ = true;
   resolve_timings();
   detect_hermitian_G(G);
@@ -963,7 +986,9 @@ void gw_hip_kernel::solve_device(const double2* G, double2* Sigma, int precision
   GWHIP_CHECK(hipEventRecord(t0, stream_));
   zero(Sigma, nsig);
   stored_valid_ = false;
-  stage_p0(G, split);
+  leak_valid_ = false;
+  stage_p0(G, p0_split(precision));
+  capture_leakage_p0();
   if (keep_p0_) {  // half-grid P0 of every q, for the stored-mode VJP
     const size_t row = (size_t)(d_.nts / 2) * d_.NQ * d_.NQ * sizeof(double2);
     GWHIP_CHECK(hipMemcpy2DAsync(P0keep_, row, P_, (size_t)d_.nts * d_.NQ * d_.NQ * sizeof(double2), row, (size_t)d_.nk,
@@ -974,7 +999,7 @@ void gw_hip_kernel::solve_device(const double2* G, double2* Sigma, int precision
   stage_dyson(d_.nk, q0_qiq_);
   q0_eps_ = false;
   GWHIP_CHECK(hipEventRecord(t2, stream_));
-  stage_sigma(G, Sigma, split);
+  stage_sigma(G, Sigma, sigma_split(precision));
   finish_sigma(Sigma, d_.nk);
   if (q0_on_ && q0_apply) apply_q0(Sigma, G, d_.nk);
   GWHIP_CHECK(hipEventRecord(t3, stream_));
@@ -987,10 +1012,12 @@ void gw_hip_kernel::solve_device(const double2* G, double2* Sigma, int precision
   GWHIP_CHECK(hipEventElapsedTime(&x, t0, t1)); region_ms_[R_P0] = x;
   GWHIP_CHECK(hipEventElapsedTime(&x, t2, t3)); region_ms_[R_SIGMA] = x;
   GWHIP_CHECK(hipEventElapsedTime(&x, t0, t3)); region_ms_[R_TOTAL] = x;
+  check_dyson_flag();
   int flag = 0;
   GWHIP_CHECK(hipMemcpy(&flag, flag_, sizeof(int), hipMemcpyDeviceToHost));
   if (flag) throw status_error(GWHIP_ERR_NONFINITE, "Sigma contains non-finite values");
   stored_valid_ = keep_p0_;
+  leak_valid_ = leak_on_;
 }
 
 // Time the launch-parameter candidates on this system's own operands (q = 0, first tau chunk)
@@ -999,7 +1026,7 @@ void gw_hip_kernel::autotune(const double2* G, int precision) {
   check_ready(precision);
   if (!G) throw status_error(GWHIP_ERR_ARG, "null G");
   GWHIP_CHECK(hipSetDevice(device_));
-  const bool split = precision == GWHIP_PREC_SPLIT;
+  const bool split = any_split(precision);
   if (split) ensure_slices();
   ensure_orb_buffers();
   detect_hermitian_G(G);
@@ -1078,6 +1105,79 @@ void gw_hip_kernel::solve_host(const double2* G, double2* Sigma, int precision) 
   GWHIP_CHECK(hipMemcpy(Sigma, S_dev_, nsig * sizeof(double2), hipMemcpyDeviceToHost));
 }
 
+void gw_hip_kernel::set_leakage_capture(bool on) {
+  GWHIP_CHECK(hipSetDevice(device_));
+  if (on && !leak_P0_) leak_P0_ = dalloc<double2>((size_t)(d_.nts / 2) * d_.NQ * d_.NQ);
+  leak_on_ = on;
+  leak_valid_ = false;
+}
+
+// Half-grid P0 of P_ row 0 (the first q evaluated) into leak_P0_, before the Dyson stage overwrites it.
+void gw_hip_kernel::capture_leakage_p0() {
+  if (!leak_on_) return;
+  const size_t n = (size_t)(d_.nts / 2) * d_.NQ * d_.NQ;
+  GWHIP_CHECK(hipMemcpyAsync(leak_P0_, P_, n * sizeof(double2), hipMemcpyDeviceToDevice, stream_));
+}
+
+// P0 on the full tau grid from its half grid: P0(beta - tau) = P0(tau) (src/gw_cpu_kernel.cpp:217-225).
+static void mirror_half_grid(std::vector<double2>& half, double2* full, int64_t nts, int64_t n2) {
+  for (int64_t t = 0; t < nts / 2; ++t) {
+    std::memcpy(full + t * n2, half.data() + t * n2, n2 * sizeof(double2));
+    std::memcpy(full + (nts - 1 - t) * n2, half.data() + t * n2, n2 * sizeof(double2));
+  }
+}
+
+void gw_hip_kernel::copy_leakage_inputs(double2* P0, double2* Pt) {
+  if (!leak_on_ || !leak_valid_)
+    throw status_error(GWHIP_ERR_STATE, "no leakage inputs: enable the capture, then evaluate");
+  GWHIP_CHECK(hipSetDevice(device_));
+  const int64_t n2 = d_.NQ * d_.NQ, half = d_.nts / 2;
+  if (P0) {
+    std::vector<double2> h((size_t)half * n2);
+    GWHIP_CHECK(hipMemcpy(h.data(), leak_P0_, h.size() * sizeof(double2), hipMemcpyDeviceToHost));
+    mirror_half_grid(h, P0, d_.nts, n2);
+  }
+  if (Pt) GWHIP_CHECK(hipMemcpy(Pt, P_, (size_t)d_.nts * n2 * sizeof(double2), hipMemcpyDeviceToHost));
+}
+
+void gw_hip_kernel::eval_one_q_host(const double2* G, int64_t q, int precision, double2* P0, double2* Pt,
+                                    double2* Sq) {
+  check_ready(precision);
+  if (!G) throw status_error(GWHIP_ERR_ARG, "null G");
+  if (q < 0 || q >= d_.nk) throw status_error(GWHIP_ERR_ARG, "q out of range");
+  GWHIP_CHECK(hipSetDevice(device_));
+  const size_t nsig = (size_t)d_.nts * d_.ns * d_.nk * d_.nso * d_.nso;
+  const int64_t n2 = d_.NQ * d_.NQ, half = d_.nts / 2;
+  if (!G_dev_) G_dev_ = dalloc<double2>(nsig);
+  if (!S_dev_) S_dev_ = dalloc<double2>(nsig);
+  GWHIP_CHECK(hipMemcpy(G_dev_, G, nsig * sizeof(double2), hipMemcpyHostToDevice));
+  if (any_split(precision)) ensure_slices();
+  ensure_orb_buffers();
+  stored_valid_ = false;
+  leak_valid_ = false;
+  detect_hermitian_G(G_dev_);
+  p0_one_q(q, G_dev_, p0_split(precision), P_);
+  if (P0) {
+    std::vector<double2> h((size_t)half * n2);
+    GWHIP_CHECK(hipStreamSynchronize(stream_));
+    GWHIP_CHECK(hipMemcpy(h.data(), P_, h.size() * sizeof(double2), hipMemcpyDeviceToHost));
+    mirror_half_grid(h, P0, d_.nts, n2);
+  }
+  stage_dyson(1, nullptr);
+  GWHIP_CHECK(hipStreamSynchronize(stream_));
+  check_dyson_flag();
+  if (Pt) GWHIP_CHECK(hipMemcpy(Pt, P_, (size_t)d_.nts * n2 * sizeof(double2), hipMemcpyDeviceToHost));
+  if (Sq) {
+    zero(S_dev_, (int64_t)nsig);
+    sigma_q(G_dev_, S_dev_, sigma_split(precision), P_, Vt_neg(q), Vs_neg(q), eVs_neg(q), kminus_tab_ + q * d_.nk,
+            d_.nk);
+    finish_sigma(S_dev_, d_.nk);
+    GWHIP_CHECK(hipStreamSynchronize(stream_));
+    GWHIP_CHECK(hipMemcpy(Sq, S_dev_, nsig * sizeof(double2), hipMemcpyDeviceToHost));
+  }
+  resolve_timings();
+}
+
 void gw_hip_kernel::run_stages_host(const double2* G, int last_stage) {
   if (!grid_set_ || !v_set_) throw status_error(GWHIP_ERR_STATE, "grid or integrals not set");
   if (!G) throw status_error(GWHIP_ERR_ARG, "null G");
@@ -1131,228 +1231,12 @@ void debug_zgemm(int M, int N, int K, const double2* A, const double2* B, double
   (void)hipFree(dC);
 }
 
-__global__ void fill_hash_kernel(double2* X, int64_t n, uint64_t salt) {
-  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (gid >= n) return;
-  X[gid] = make_double2(u01(salt, 2 * gid) - 0.5, u01(salt, 2 * gid + 1) - 0.5);
-}
-
-template <int CH>
-__global__ __launch_bounds__(256) void mfma_peak_kernel(double* out, int iters) {
-  v4d acc[CH];
-#pragma unroll
-  for (int j = 0; j < CH; ++j) acc[j] = v4d{0.0, 0.0, 0.0, 0.0};
-  const double a = 1e-3 * threadIdx.x, b = 1.0 + 1e-6 * blockIdx.x;
-  for (int i = 0; i < iters; ++i) {
-#pragma unroll
-    for (int j = 0; j < CH; ++j) acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[j], 0, 0, 0);
-  }
-  double s = 0.0;
-#pragma unroll
-  for (int j = 0; j < CH; ++j) s += acc[j][0] + acc[j][1] + acc[j][2] + acc[j][3];
-  out[(int64_t)blockIdx.x * blockDim.x + threadIdx.x] = s;
-}
-
-// FP64 vector FMA probe (mode 1) and MFMA + vector FMA in the same wave (mode 2).
-template <int MODE>
-__global__ __launch_bounds__(256) void valu_peak_kernel(double* out, int iters) {
-  double x[8];
-#pragma unroll
-  for (int j = 0; j < 8; ++j) x[j] = 1e-3 * (threadIdx.x + j);
-  v4d acc[4];
-#pragma unroll
-  for (int j = 0; j < 4; ++j) acc[j] = v4d{0.0, 0.0, 0.0, 0.0};
-  const double a = 1.0 - 1e-9 * blockIdx.x, b = 1e-7;
-  for (int i = 0; i < iters; ++i) {
-    if (MODE == 2) {
-#pragma unroll
-      for (int j = 0; j < 4; ++j) acc[j] = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc[j], 0, 0, 0);
-    }
-#pragma unroll
-    for (int r = 0; r < 4; ++r)
-#pragma unroll
-      for (int j = 0; j < 8; ++j) x[j] = fma(x[j], a, b);
-  }
-  double s = 0.0;
-#pragma unroll
-  for (int j = 0; j < 8; ++j) s += x[j];
-#pragma unroll
-  for (int j = 0; j < 4; ++j) s += acc[j][0];
-  out[(int64_t)blockIdx.x * blockDim.x + threadIdx.x] = s;
-}
-
-// FP64 vector-FMA complex outer products with operands from LDS (probe): each lane accumulates a
-// 4 x 4 complex tile over K = 32, A and B as real/imaginary planes [k][32] read 4 at a time.
-__global__ __launch_bounds__(256) void valu_cgemm_probe_kernel(double* out, int iters) {
-  __shared__ double Ar[32][32], Ai[32][32], Br[32][32], Bi[32][32];
-  for (int e = threadIdx.x; e < 32 * 32; e += blockDim.x) {
-    const int r = e / 32, c = e % 32;
-    Ar[r][c] = 1e-3 * (r + c);
-    Ai[r][c] = 1e-3 * (r - c);
-    Br[r][c] = 1e-3 * (2 * r + c);
-    Bi[r][c] = 1e-3 * (r - 2 * c);
-  }
-  __syncthreads();
-  const int lane = threadIdx.x & 63, ti = (lane & 7) * 4, tj = (lane >> 3) * 4;
-  double cr[4][4], ci[4][4];
-#pragma unroll
-  for (int x = 0; x < 4; ++x)
-#pragma unroll
-    for (int y = 0; y < 4; ++y) { cr[x][y] = 0.0; ci[x][y] = 0.0; }
-  for (int it = 0; it < iters; ++it) {
-#pragma unroll 4
-    for (int k = 0; k < 32; ++k) {
-      double ar[4], ai[4], br[4], bi[4];
-#pragma unroll
-      for (int x = 0; x < 4; ++x) {
-        ar[x] = Ar[k][ti + x];
-        ai[x] = Ai[k][ti + x];
-        br[x] = Br[k][tj + x];
-        bi[x] = Bi[k][tj + x];
-      }
-#pragma unroll
-      for (int x = 0; x < 4; ++x)
-#pragma unroll
-        for (int y = 0; y < 4; ++y) {
-          cr[x][y] = fma(ar[x], br[y], cr[x][y]);
-          cr[x][y] = fma(-ai[x], bi[y], cr[x][y]);
-          ci[x][y] = fma(ar[x], bi[y], ci[x][y]);
-          ci[x][y] = fma(ai[x], br[y], ci[x][y]);
-        }
-    }
-  }
-  double s = 0.0;
-#pragma unroll
-  for (int x = 0; x < 4; ++x)
-#pragma unroll
-    for (int y = 0; y < 4; ++y) s += cr[x][y] + ci[x][y];
-  out[(int64_t)blockIdx.x * blockDim.x + threadIdx.x] = s;
-}
-
-typedef int v4i_t __attribute__((ext_vector_type(4)));
-typedef float v4f_t __attribute__((ext_vector_type(4)));
-typedef __bf16 v8bf_t __attribute__((ext_vector_type(8)));
-
-// Low-precision MFMA probes: MODE 0 = v_mfma_i32_16x16x64_i8, MODE 1 = v_mfma_f32_16x16x32_bf16;
-// eight independent accumulator chains per wave.
-template <int MODE>
-__global__ __launch_bounds__(256) void lowp_peak_kernel(double* out, int iters) {
-  v4i_t ai = {(int)threadIdx.x, 3, 5, 7}, bi = {11, (int)blockIdx.x, 13, 17};
-  v8bf_t ab, bb;
-#pragma unroll
-  for (int j = 0; j < 8; ++j) { ab[j] = (__bf16)(0.001f * (threadIdx.x + j)); bb[j] = (__bf16)(0.5f); }
-  v4i_t ci[8];
-  v4f_t cf[8];
-#pragma unroll
-  for (int j = 0; j < 8; ++j) { ci[j] = v4i_t{0, 0, 0, 0}; cf[j] = v4f_t{0.f, 0.f, 0.f, 0.f}; }
-  for (int i = 0; i < iters; ++i) {
-#pragma unroll
-    for (int j = 0; j < 8; ++j) {
-      if (MODE == 0) ci[j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ai, bi, ci[j], 0, 0, 0);
-      else cf[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(ab, bb, cf[j], 0, 0, 0);
-    }
-  }
-  double s = 0.0;
-#pragma unroll
-  for (int j = 0; j < 8; ++j) s += (double)ci[j][0] + (double)cf[j][0];
-  out[(int64_t)blockIdx.x * blockDim.x + threadIdx.x] = s;
-}
-
-double debug_mfma_peak(int chains, int wps, int iters) {
-  if (chains < 1 || (chains > 8 && chains != 10 && chains != 11 && chains != 12) || chains == 7 || wps < 1 || wps > 8 ||
-      iters < 1)
-    throw status_error(GWHIP_ERR_ARG, "bad mfma probe args");
-  GWHIP_CHECK(hipSetDevice(0));
-  hipDeviceProp_t prop;
-  GWHIP_CHECK(hipGetDeviceProperties(&prop, 0));
-  const int nblocks = prop.multiProcessorCount * wps;  // 4 waves per block -> one per SIMD
-  double* out = nullptr;
-  GWHIP_CHECK(hipMalloc(&out, (size_t)nblocks * 256 * sizeof(double)));
-  auto launch = [&] {
-    switch (chains) {
-      case 10: hipLaunchKernelGGL(lowp_peak_kernel<0>, dim3(nblocks), dim3(256), 0, nullptr, out, iters); break;
-      case 11: hipLaunchKernelGGL(lowp_peak_kernel<1>, dim3(nblocks), dim3(256), 0, nullptr, out, iters); break;
-      case 12: hipLaunchKernelGGL(valu_cgemm_probe_kernel, dim3(nblocks), dim3(256), 0, nullptr, out, iters); break;
-      case 5: hipLaunchKernelGGL(valu_peak_kernel<1>, dim3(nblocks), dim3(256), 0, nullptr, out, iters); break;
-      case 6: hipLaunchKernelGGL(valu_peak_kernel<2>, dim3(nblocks), dim3(256), 0, nullptr, out, iters); break;
-      case 1: hipLaunchKernelGGL(mfma_peak_kernel<1>, dim3(nblocks), dim3(256), 0, nullptr, out, iters); break;
-      case 2: hipLaunchKernelGGL(mfma_peak_kernel<2>, dim3(nblocks), dim3(256), 0, nullptr, out, iters); break;
-      case 4: hipLaunchKernelGGL(mfma_peak_kernel<4>, dim3(nblocks), dim3(256), 0, nullptr, out, iters); break;
-      default: hipLaunchKernelGGL(mfma_peak_kernel<8>, dim3(nblocks), dim3(256), 0, nullptr, out, iters); break;
-    }
-  };
-  launch();
-  GWHIP_CHECK(hipDeviceSynchronize());
-  Event e0, e1;
-  GWHIP_CHECK(hipEventRecord(e0, nullptr));
-  launch();
-  GWHIP_CHECK(hipEventRecord(e1, nullptr));
-  GWHIP_CHECK(hipEventSynchronize(e1));
-  float ms = 0.f;
-  GWHIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-  (void)hipFree(out);
-  double per_wave_iter;
-  if (chains == 10) per_wave_iter = 8 * 16.0 * 16 * 64 * 2;
-  else if (chains == 11) per_wave_iter = 8 * 16.0 * 16 * 32 * 2;
-  else if (chains == 12) per_wave_iter = 32 * 64 * 2.0 * 64;  // 32 k x 64 FMA per lane
-  else if (chains == 5) per_wave_iter = 32.0 * 64 * 2;                   // 32 FMA per lane
-  else if (chains == 6) per_wave_iter = 32.0 * 64 * 2 + 4 * 2048.0;  // + 4 MFMA
-  else per_wave_iter = (chains == 3 ? 4 : (chains > 4 ? 8 : chains)) * 2048.0;
-  const double flops = (double)nblocks * 4 * iters * per_wave_iter;
-  return flops / (ms * 1e-3) / 1e12;
-}
-
-double debug_gemm_bench(int which, int NQ, int nk, int nao, int batch, int shape, int reps) {
-  if (NQ <= 0 || nk <= 0 || nao <= 0 || batch <= 0 || reps <= 0 || shape < 0 || shape > 2)
-    throw status_error(GWHIP_ERR_ARG, "bad debug_gemm_bench arguments");
-  GWHIP_CHECK(hipSetDevice(0));
-  const int64_t nao2 = (int64_t)nao * nao, NQ2 = (int64_t)NQ * NQ, vsz = (int64_t)nk * NQ * nao2;
-  ZGemmArgs a{};
-  int64_t na, nb, nc;
-  bool akc = true, bkc = false;
-  if (which == 0) {  // Sigma: R[tau] = P[tau] conj(V)
-    na = batch * NQ2; nb = vsz; nc = batch * vsz;
-    a.M = NQ; a.N = (int)(nk * nao2); a.K = NQ; a.batch = batch; a.kin = NQ; a.nin = (int)nao2;
-    a.sAb = NQ2; a.sAm = NQ; a.sAko = 0; a.sAki = 1;
-    a.sBb = 0; a.sBko = 0; a.sBki = nao2; a.sBno = NQ * nao2; a.sBni = 1;
-    a.sCb = vsz; a.sCm = nao2; a.sCno = NQ * nao2; a.sCni = 1;
-  } else {           // P0: P[tau] = Z[tau] conj(V)^T
-    na = batch * vsz; nb = vsz; nc = batch * NQ2;
-    a.M = NQ; a.N = NQ; a.K = (int)(nk * nao2); a.batch = batch; a.kin = (int)nao2; a.nin = NQ;
-    a.sAb = vsz; a.sAm = nao2; a.sAko = NQ * nao2; a.sAki = 1;
-    a.sBb = 0; a.sBko = NQ * nao2; a.sBki = 1; a.sBno = 0; a.sBni = nao2;
-    a.sCb = NQ2; a.sCm = NQ; a.sCno = 0; a.sCni = 1;
-    bkc = true;
-  }
-  double2 *dA = nullptr, *dB = nullptr, *dC = nullptr;
-  GWHIP_CHECK(hipMalloc(&dA, na * sizeof(double2)));
-  GWHIP_CHECK(hipMalloc(&dB, nb * sizeof(double2)));
-  GWHIP_CHECK(hipMalloc(&dC, nc * sizeof(double2)));
-  hipLaunchKernelGGL(fill_hash_kernel, dim3((unsigned)ceil_div(na, 256)), dim3(256), 0, nullptr, dA, na, 11ull);
-  hipLaunchKernelGGL(fill_hash_kernel, dim3((unsigned)ceil_div(nb, 256)), dim3(256), 0, nullptr, dB, nb, 12ull);
-  GWHIP_CHECK(hipGetLastError());
-  a.A = dA; a.B = dB; a.C = dC;
-  a.alpha = make_double2(1.0, 0.0);
-  a.conjB = 1;
-  zgemm_f64(a, akc, bkc, nullptr, shape);
-  GWHIP_CHECK(hipDeviceSynchronize());
-  Event e0, e1;
-  GWHIP_CHECK(hipEventRecord(e0, nullptr));
-  for (int r = 0; r < reps; ++r) zgemm_f64(a, akc, bkc, nullptr, shape);
-  GWHIP_CHECK(hipEventRecord(e1, nullptr));
-  GWHIP_CHECK(hipEventSynchronize(e1));
-  float ms = 0.f;
-  GWHIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
-  (void)hipFree(dA);
-  (void)hipFree(dB);
-  (void)hipFree(dC);
-  return ms / reps;
-}
-
 void debug_zgemm_split(int M, int N, int K, const double2* A, const double2* B, double2* C, int ns) {
   if (M <= 0 || N <= 0 || K <= 0 || !A || !B || !C) throw status_error(GWHIP_ERR_ARG, "bad debug_zgemm_split arguments");
   GWHIP_CHECK(hipSetDevice(0));
   const int kc = (int)ceil_div(K, 64) * 64;
+  if (ns < 3 || ns > 5) throw status_error(GWHIP_ERR_ARG, "split slices must be 3, 4 or 5");
+  if (kc > split_kc_limit(ns)) throw status_error(GWHIP_ERR_UNSUPPORTED, "K exceeds the int32-exact chunk limit");
   double2 *dA = nullptr, *dB = nullptr, *dC = nullptr;
   int8_t *sA = nullptr, *sB = nullptr;
   int *eA = nullptr, *eB = nullptr;
@@ -1742,6 +1626,24 @@ int gwhip_dyson_number_of_electrons(gwhip_dyson* d, double* Sigma1, double* Sigm
   return guarded_d(d, [&] { *nel = d->d->number_of_electrons_host((double2*)Sigma1, (double2*)Sigma_tau, mu); });
 }
 
+int gwhip_dyson_energy(gwhip_dyson* d, const double* G_tau, const double* Sigma1, const double* Sigma_tau,
+                       double* energies) {
+  if (!d || !energies) return GWHIP_ERR_ARG;
+  return guarded_d(d, [&] {
+    const auto e = d->d->energy_host((const double2*)G_tau, (const double2*)Sigma1, (const double2*)Sigma_tau);
+    for (int i = 0; i < 3; ++i) energies[i] = e[i];
+  });
+}
+
+int gwhip_dyson_energy_device(gwhip_dyson* d, const void* G_tau, const void* Sigma1, const void* Sigma_tau,
+                              double* energies) {
+  if (!d || !energies) return GWHIP_ERR_ARG;
+  return guarded_d(d, [&] {
+    const auto e = d->d->energy_device((const double2*)G_tau, (const double2*)Sigma1, (const double2*)Sigma_tau);
+    for (int i = 0; i < 3; ++i) energies[i] = e[i];
+  });
+}
+
 int gwhip_get_dims(const gwhip_handle* h, gwhip_dims* out) {
   if (!h || !out) return GWHIP_ERR_ARG;
   const gwhip::Dims& d = h->k->dims();
@@ -1788,23 +1690,29 @@ int gwhip_debug_zgemm(int M, int N, int K, const double* A, const double* B, dou
   });
 }
 
-int gwhip_debug_gemm_bench(int which, int NQ, int nk, int nao, int batch, int shape, int reps, double* ms) {
-  gwhip_handle tmp;
-  if (!ms) return GWHIP_ERR_ARG;
-  return guarded(&tmp, [&] { *ms = gwhip::debug_gemm_bench(which, NQ, nk, nao, batch, shape, reps); });
-}
-
-int gwhip_debug_mfma_peak(int chains, int waves_per_simd, int iters, double* tflops) {
-  gwhip_handle tmp;
-  if (!tflops) return GWHIP_ERR_ARG;
-  return guarded(&tmp, [&] { *tflops = gwhip::debug_mfma_peak(chains, waves_per_simd, iters); });
-}
-
 int gwhip_debug_zgemm_split(int M, int N, int K, const double* A, const double* B, double* C, int slices) {
   gwhip_handle tmp;
   return guarded(&tmp, [&] {
     gwhip::debug_zgemm_split(M, N, K, (const double2*)A, (const double2*)B, (double2*)C, slices);
   });
+}
+
+int gwhip_debug_eval_q(gwhip_handle* h, const double* G, int64_t q, int precision, double* P0, double* P_tilde,
+                       double* Sigma_q) {
+  if (!h) return GWHIP_ERR_ARG;
+  return guarded(h, [&] {
+    h->k->eval_one_q_host((const double2*)G, q, precision, (double2*)P0, (double2*)P_tilde, (double2*)Sigma_q);
+  });
+}
+
+int gwhip_set_leakage_capture(gwhip_handle* h, int on) {
+  if (!h) return GWHIP_ERR_ARG;
+  return guarded(h, [&] { h->k->set_leakage_capture(on != 0); });
+}
+
+int gwhip_get_leakage_inputs(gwhip_handle* h, double* P0, double* P_tilde) {
+  if (!h) return GWHIP_ERR_ARG;
+  return guarded(h, [&] { h->k->copy_leakage_inputs((double2*)P0, (double2*)P_tilde); });
 }
 
 int gwhip_debug_stage(gwhip_handle* h, const double* G, int last_stage) {

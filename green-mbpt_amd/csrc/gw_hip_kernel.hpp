@@ -106,6 +106,14 @@ public:
   // test hook: stop after stage 1 (P0) or 2 (P-tilde); stage 3 runs Sigma as usual
   void run_stages_host(const double2* G, int last_stage);
   void copy_P_tilde(double2* host) const;
+  // test hook: the stages of one q alone -- P0(tau, q) (full tau grid, after Hermitisation and the
+  // tau mirror), P-tilde(tau, q) and q's contribution to Sigma (with the 1/nk of the full sum);
+  // host buffers, any may be null
+  void eval_one_q_host(const double2* G, int64_t q, int precision, double2* P0, double2* Pt, double2* Sq);
+  // keep P0(tau) of the first q of each evaluation (the q_ir == 0 of the reference's leakage
+  // print, src/gw_cpu_kernel.cpp:237-239,276-278); copy it and that q's P-tilde out (full tau grid)
+  void set_leakage_capture(bool on);
+  void copy_leakage_inputs(double2* P0, double2* Pt);
   // vector-Jacobian product of Sigma[G]: G_bar for the cotangent Sigma_bar (device pointers)
   void set_vjp_store(bool keep);
   void vjp_device(const double2* G_dev, const double2* Sbar_dev, double2* Gbar_dev, int mode);
@@ -152,7 +160,7 @@ public:
   bool split_ready() const { return slices_ns_ == split_ns_ && Vs_p0_ != nullptr; }
   // Enqueue one evaluation on the stream: no host synchronisation, no memset nodes, no
   // allocation (safe under stream capture once prepare_split / bind_workspace are done).
-  void enqueue_eval(const double2* G_dev, double2* Sigma_dev, bool split, int* nonfinite_flag);
+  void enqueue_eval(const double2* G_dev, double2* Sigma_dev, int precision, int* nonfinite_flag);
   void check_ready(int precision) const;
   // Whether G is Hermitian per (tau, spin, k) to 1e-12 relative; when it is, only the upper block
   // tiles of P0 are computed and mirrored.  Runs on the stream and synchronises.
@@ -170,6 +178,9 @@ public:
   // the register-chained orbital kernels handle nao up to 3 x 16
   static constexpr int64_t kOrbMaxNao = 48;
   bool has_inputs() const { return grid_set_ && v_set_; }
+  // after the stream has drained: whether the last evaluation's Dyson step met a matrix I - P0
+  // that is not positive definite
+  bool dyson_failed() const;
   uint64_t generation() const { return generation_; }
   hipStream_t stream() const { return stream_; }
   int device() const { return device_; }
@@ -211,6 +222,7 @@ private:
   int* kplus_tab_ = nullptr;   // [nk][nq] : k + q
   int* kminus_tab_ = nullptr;  // [nq][nk] : k - q  (row q)
   int* flag_ = nullptr;
+  int* dyson_flag_ = nullptr;  // set by the Dyson inverse when a pivot is not positive
   // split precision: int8 slice stores of conj(V) for the P0 and Sigma contractions, and the
   // per-evaluation slices of Z and P-tilde (zgemm_ozaki.hpp)
   int split_ns_ = 5, slices_ns_ = 0;
@@ -230,6 +242,9 @@ private:
   double2 *vM1_ = nullptr, *vM2_ = nullptr, *vM3_ = nullptr, *vPt_ = nullptr, *vPtbar_ = nullptr, *vPbar_ = nullptr;
   double2* vW_ = nullptr;      // [tc][nk][NQ][nao^2]
   double2* P0keep_ = nullptr;  // [nq][nts/2][NQ^2]
+  double2* leak_P0_ = nullptr; // [nts/2][NQ^2]: half-grid P0 of the first q of the last evaluation
+  bool leak_on_ = false, leak_valid_ = false;
+  void capture_leakage_p0();
   int64_t tchunk_vjp_ = 1;
   bool keep_p0_ = false, stored_valid_ = false;
   double vjp_ms_ = 0.0;
@@ -322,6 +337,7 @@ private:
   void stage_p0(const double2* G, bool split, const std::vector<int>* qs = nullptr);
   void p0_one_q(int64_t q, const double2* G, bool split, double2* Pq);
   void gj_inverse(double2* X, int64_t nmat);
+  void check_dyson_flag();  // reads dyson_flag_ (synchronous) and throws when it is set
   void stage_dyson(int64_t nq, const int* q0_row);
   void stage_sigma(const double2* G, double2* Sigma, bool split);
   void sigma_q(const double2* G, double2* Sigma, bool split, const double2* Pq, const double2* Vb, const int8_t* Vsb,

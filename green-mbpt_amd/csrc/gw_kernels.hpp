@@ -94,14 +94,18 @@ __global__ void identity_minus_kernel(double2* X, const double2* Pw, int N, int6
 // ---------------------------------------------------------------------------------------
 // Batched in-place inverse of Hermitian positive-definite N x N matrices by blocked
 // Gauss-Jordan sweeps without pivoting (the reference uses Eigen's LDLT on I - P0,
-// src/gw_cpu_kernel.cpp:243-252; I - P0(i Omega) has eigenvalues >= 1 since P0 is negative
-// semidefinite, so every pivot is positive).  One workgroup per matrix; per block column J:
+// src/gw_cpu_kernel.cpp:243-252; for the polarisation of a physical G, I - P0(i Omega) is
+// positive definite, so every pivot is positive).  One workgroup per matrix; per block column J:
 //   Dinv = A_JJ^-1 (small sweep in LDS);  R = Dinv A_JI;  C = A_IJ (old);
 //   A_II -= C R;  A_IJ = -C Dinv;  A_JI = R;  A_JJ = Dinv.
 // Threads own columns; the column panel C lives in LDS ([N][nb]) and the row panel in registers.
+// Without pivoting the k-th pivot is the k-th leading Schur complement's corner, real and positive
+// exactly when the matrix is Hermitian positive definite; a pivot whose real part is not positive
+// (or not finite) sets *status to 1, so that the caller reports the matrix instead of returning a
+// silently degraded inverse.
 constexpr int GJ_NB = 16;
 
-__global__ void gj_inverse_kernel(double2* A, int N, int64_t stride, int nb) {
+__global__ void gj_inverse_kernel(double2* A, int N, int64_t stride, int nb, int* status) {
   extern __shared__ double2 lds[];
   double2* D = lds;               // [GJ_NB][GJ_NB]
   double2* Cp = D + GJ_NB * GJ_NB;  // [N][nb]
@@ -130,6 +134,7 @@ __global__ void gj_inverse_kernel(double2* A, int N, int64_t stride, int nb) {
         r = tid / jb;
         c = tid - r * jb;
         const double2 pv = D[piv * GJ_NB + piv];
+        if (tid == 0 && !(pv.x > 0.0 && fabs(pv.y) <= pv.x)) atomicOr(status, 1);
         const double den = pv.x * pv.x + pv.y * pv.y;
         const double2 pinv = make_double2(pv.x / den, -pv.y / den);
         if (r == piv && c == piv) {

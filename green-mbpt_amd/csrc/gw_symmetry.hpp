@@ -277,8 +277,8 @@ const double2* gw_hip_kernel::star_rotate(int64_t iq, int64_t q) {
 // [nts][ns][ink][nso^2] on the device.
 void gw_hip_kernel::solve_star_device(const double2* G_ibz, double2* Sigma_ibz, int precision) {
   const int64_t nk = d_.nk, ink = ink_, NQ = d_.NQ, nao2 = d_.nao * d_.nao;
-  const bool split = precision == GWHIP_PREC_SPLIT;
-  if (split) ensure_slices();
+  const bool split = sigma_split(precision);
+  if (any_split(precision)) ensure_slices();
   ensure_orb_buffers();
   star_tables();
   if (split) {
@@ -307,7 +307,9 @@ void gw_hip_kernel::solve_star_device(const double2* G_ibz, double2* Sigma_ibz, 
   GWHIP_CHECK(hipEventRecord(t0, stream_));
   zero(Sigma_ibz, nsig);
   stored_valid_ = false;
-  stage_p0(G_dev_, split, &qsym_i2b_);
+  leak_valid_ = false;
+  stage_p0(G_dev_, p0_split(precision), &qsym_i2b_);
+  capture_leakage_p0();
   GWHIP_CHECK(hipEventRecord(t1, stream_));
   q0_eps_ = q0_on_;
   stage_dyson(inq_, star_q0row_);
@@ -338,9 +340,11 @@ void gw_hip_kernel::solve_star_device(const double2* G_ibz, double2* Sigma_ibz, 
   GWHIP_CHECK(hipEventElapsedTime(&x, t0, t1)); region_ms_[R_P0] = x;
   GWHIP_CHECK(hipEventElapsedTime(&x, t2, t3)); region_ms_[R_SIGMA] = x;
   GWHIP_CHECK(hipEventElapsedTime(&x, t0, t3)); region_ms_[R_TOTAL] = x;
+  check_dyson_flag();
   int flag = 0;
   GWHIP_CHECK(hipMemcpy(&flag, flag_, sizeof(int), hipMemcpyDeviceToHost));
   if (flag) throw status_error(GWHIP_ERR_NONFINITE, "Sigma contains non-finite values");
+  leak_valid_ = leak_on_;
 }
 
 // X [nb][ink][n][n] -> Y [nb][nk][n][n] through the scratch T (same size as Y).

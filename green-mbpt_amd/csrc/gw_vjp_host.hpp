@@ -260,6 +260,7 @@ void gw_hip_kernel::vjp_device(const double2* G, const double2* Sbar, double2* G
   Event e0, e1;
   GWHIP_CHECK(hipEventRecord(e0, stream_));
   GWHIP_CHECK(hipMemsetAsync(Gbar, 0, nsig * sizeof(double2), stream_));
+  GWHIP_CHECK(hipMemsetAsync(dyson_flag_, 0, sizeof(int), stream_));
 
   for (int64_t q = 0; q < nk; ++q) {
     OrbArgs oa{(int)nao, (int)NQ, (int)nk, QPW, kplus_tab_ + q * nk, kminus_tab_ + q * nk, 1.0};
@@ -391,6 +392,7 @@ void gw_hip_kernel::vjp_device(const double2* G, const double2* Sbar, double2* G
   float ms = 0.f;
   GWHIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
   vjp_ms_ = ms;
+  check_dyson_flag();
   int flag = 0;
   GWHIP_CHECK(hipMemcpy(&flag, flag_, sizeof(int), hipMemcpyDeviceToHost));
   if (flag) throw status_error(GWHIP_ERR_NONFINITE, "G_bar contains non-finite values");

@@ -124,6 +124,11 @@ __device__ __forceinline__ v4i neg_i8x16(v4i w) {
   return r;
 }
 
+// Largest K chunk (bytes per plane-slice, a multiple of 64) whose int32 accumulation stays exact
+// with ns slices: an accumulator of slice level d sums up to ns slice pairs of two planes, each
+// product below 2^14 in magnitude, so 2 * ns * 2^14 * kc < 2^31.
+inline int64_t split_kc_limit(int ns) { return ((((int64_t)1 << 31) - 1) / ((int64_t)2 * ns << 14)) / 64 * 64; }
+
 struct OzGemmArgs {
   int M, N, batch;
   int kc, nchunk;        // K = nchunk * kc (int8 elements per plane-slice)

@@ -15,6 +15,8 @@ copy of V equal to this one up to the last bits of log/cos.
 """
 from __future__ import annotations
 
+import math
+
 import numpy as np
 
 _M64 = np.uint64(0xFFFFFFFFFFFFFFFF)
@@ -67,6 +69,38 @@ def generate_V(seed: int, nk: int, NQ: int, nao: int, sigma: float = 0.1) -> np.
         p = k * (k + 1) // 2 + k
         V[p] = 0.5 * (V[p] + np.conj(np.transpose(V[p], (0, 2, 1))))
     return V
+
+
+def generate_V_pair(seed: int, pair: int, NQ: int, nao: int, sigma: float = 0.1) -> np.ndarray:
+    """One block [NQ, nao, nao] of the packed V of ``generate_V`` (same counters), without the rest."""
+    n = NQ * nao * nao
+    idx = np.uint64(pair) * np.uint64(n) + np.arange(n, dtype=np.uint64)
+    re, im = normal_pairs(seed, STREAM_V, idx)
+    v = (sigma * (re + 1j * im)).reshape(NQ, nao, nao)
+    k = (math.isqrt(8 * int(pair) + 1) - 1) // 2       # pair = k1 (k1 + 1)/2 + k2 with k2 <= k1
+    if pair - k * (k + 1) // 2 == k:                   # k1 == k2: Hermitised per Q
+        v = 0.5 * (v + np.conj(np.transpose(v, (0, 2, 1))))
+    return v
+
+
+class LazyPackedV:
+    """The packed V of ``generate_V`` indexed by pair, each block generated when first read (for
+    meshes whose whole packed store would not fit host memory); keeps the blocks it has made."""
+
+    def __init__(self, seed: int, nk: int, NQ: int, nao: int, sigma: float = 0.1):
+        self.seed, self.nk, self.NQ, self.nao, self.sigma = seed, nk, NQ, nao, sigma
+        self._cache = {}
+
+    def __len__(self):
+        return pair_count(self.nk)
+
+    def __getitem__(self, pair: int) -> np.ndarray:
+        pair = int(pair)
+        if not 0 <= pair < len(self):
+            raise IndexError(pair)
+        if pair not in self._cache:
+            self._cache[pair] = generate_V_pair(self.seed, pair, self.NQ, self.nao, self.sigma)
+        return self._cache[pair]
 
 
 def fock_eigensystem(seed: int, nk: int, nso: int):

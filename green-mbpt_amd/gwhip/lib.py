@@ -18,11 +18,13 @@ H5_LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libgwhip
 HEADER = os.path.join(os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__)))),
                       "include", "gwhip.h")
 H5_HEADER = os.path.join(os.path.dirname(HEADER), "gwhip_h5.h")
+DEBUG_LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libgwhip_debug.so")
+DEBUG_HEADER = os.path.join(os.path.dirname(HEADER), "gwhip_debug.h")
 HDF5_ROOT = os.environ.get("HDF5_ROOT", "/opt/conda")
 
 (OK, ERR_ARG, ERR_WRONG_GRID, ERR_STATE, ERR_OOM, ERR_DEVICE, ERR_NONFINITE, ERR_UNSUPPORTED, ERR_OUTDATED_INPUT,
- ERR_IO, ERR_MU_SEARCH) = range(11)
-PREC_FP64, PREC_SPLIT = 0, 1
+ ERR_IO, ERR_MU_SEARCH, ERR_NOT_POSITIVE_DEFINITE) = range(12)
+PREC_FP64, PREC_SPLIT, PREC_SPLIT_P0, PREC_SPLIT_SIGMA = 0, 1, 2, 3
 VJP_RECOMPUTE, VJP_STORED = 0, 1
 TUNING_KEYS = ("p0_tile", "sigma_tile", "aux_per_wave", "p0_group", "sigma_group", "p0_split_tile", "sigma_split_tile",
                "p0_fused_slices", "orbital_gemm", "p0_kchunks")
@@ -104,9 +106,10 @@ def load() -> ctypes.CDLL:
     lib.gwhip_debug_zgemm.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, P, ctypes.c_int, ctypes.c_int,
                                       ctypes.c_int]
     lib.gwhip_debug_stage.argtypes = [P, P, ctypes.c_int]
+    lib.gwhip_debug_eval_q.argtypes = [P, P, ctypes.c_int64, ctypes.c_int, P, P, P]
+    lib.gwhip_set_leakage_capture.argtypes = [P, ctypes.c_int]
+    lib.gwhip_get_leakage_inputs.argtypes = [P, P, P]
     lib.gwhip_debug_zgemm_split.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, P, ctypes.c_int]
-    lib.gwhip_debug_gemm_bench.argtypes = [ctypes.c_int] * 7 + [ctypes.POINTER(ctypes.c_double)]
-    lib.gwhip_debug_mfma_peak.argtypes = [ctypes.c_int] * 3 + [ctypes.POINTER(ctypes.c_double)]
     lib.gwhip_autotune.argtypes = [P, P, ctypes.c_int]
     IP = ctypes.POINTER(ctypes.c_int)
     lib.gwhip_get_tuning.argtypes = [P, IP, ctypes.c_int]
@@ -154,8 +157,10 @@ def load() -> ctypes.CDLL:
     lib.gwhip_dyson_find_mu.argtypes = [P, P, P, ctypes.c_double, ctypes.c_double, ctypes.c_double, DP, DP]
     lib.gwhip_dyson_find_mu_device.argtypes = [P, P, P, ctypes.c_double, ctypes.c_double, ctypes.c_double, DP, DP]
     lib.gwhip_dyson_number_of_electrons.argtypes = [P, P, P, ctypes.c_double, DP]
-    for f in ("gwhip_debug_zgemm_split", "gwhip_set_split_slices", "gwhip_set_kq_maps", "gwhip_debug_mfma_peak",
-              "gwhip_debug_gemm_bench", "gwhip_debug_zgemm", "gwhip_debug_stage", "gwhip_set_grid", "gwhip_upload_V",
+    lib.gwhip_dyson_energy.argtypes = [P, P, P, P, DP]
+    lib.gwhip_dyson_energy_device.argtypes = [P, P, P, P, DP]
+    for f in ("gwhip_debug_zgemm_split", "gwhip_set_split_slices", "gwhip_set_kq_maps", "gwhip_debug_zgemm", "gwhip_debug_eval_q",
+              "gwhip_set_leakage_capture", "gwhip_get_leakage_inputs", "gwhip_debug_stage", "gwhip_set_grid", "gwhip_upload_V",
               "gwhip_generate_V", "gwhip_eval_sigma", "gwhip_eval_sigma_device", "gwhip_get_P_tilde", "gwhip_timings",
               "gwhip_kernel_stats", "gwhip_set_vjp_store", "gwhip_vjp", "gwhip_vjp_device", "gwhip_autotune",
               "gwhip_get_tuning", "gwhip_set_tuning", "gwhip_batch_create", "gwhip_batch_autotune",
@@ -164,7 +169,8 @@ def load() -> ctypes.CDLL:
               "gwhip_eval_sigma_ibz_device", "gwhip_hf_ibz", "gwhip_hf_ibz_device", "gwhip_get_dims",
               "gwhip_set_q0_extrapolation", "gwhip_get_q0_eps", "gwhip_dyson_create", "gwhip_dyson_set_grid",
               "gwhip_dyson_set_system", "gwhip_dyson_compute_G", "gwhip_dyson_compute_G_device", "gwhip_dyson_find_mu",
-              "gwhip_dyson_find_mu_device", "gwhip_dyson_number_of_electrons"):
+              "gwhip_dyson_find_mu_device", "gwhip_dyson_number_of_electrons", "gwhip_dyson_energy",
+              "gwhip_dyson_energy_device"):
         getattr(lib, f).restype = ctypes.c_int
     _lib = lib
     return lib
@@ -298,13 +304,41 @@ def debug_zgemm_split(A: np.ndarray, B: np.ndarray, slices: int = 4) -> np.ndarr
     return c
 
 
+_dbg = None
+
+
+def load_debug() -> ctypes.CDLL:
+    """Load libgwhip_debug.so, the throughput probes (include/gwhip_debug.h)."""
+    global _dbg
+    if _dbg is not None:
+        return _dbg
+    if not os.path.exists(DEBUG_LIB_PATH):
+        raise RuntimeError(f"{DEBUG_LIB_PATH} is missing: build it with `python green-mbpt_amd/build.py`")
+    d = ctypes.CDLL(DEBUG_LIB_PATH)
+    d.gwhip_debug_gemm_bench.argtypes = [ctypes.c_int] * 7 + [ctypes.POINTER(ctypes.c_double)]
+    d.gwhip_debug_mfma_peak.argtypes = [ctypes.c_int] * 3 + [ctypes.POINTER(ctypes.c_double)]
+    d.gwhip_debug_gemm_bench.restype = ctypes.c_int
+    d.gwhip_debug_mfma_peak.restype = ctypes.c_int
+    _dbg = d
+    return d
+
+
 def debug_gemm_bench(which: int, NQ: int, nk: int, nao: int, batch: int, shape: int = 0, reps: int = 3) -> float:
-    """Mean ms per launch of the FP64 GEMM in the Sigma (0) or P0 (1) layout (test hook)."""
+    """Mean ms per launch of the FP64 GEMM in the Sigma (0) or P0 (1) layout (probe library)."""
     ms = ctypes.c_double()
-    st = load().gwhip_debug_gemm_bench(which, NQ, nk, nao, batch, shape, reps, ctypes.byref(ms))
+    st = load_debug().gwhip_debug_gemm_bench(which, NQ, nk, nao, batch, shape, reps, ctypes.byref(ms))
     if st != OK:
         raise GwhipError(st, "debug_gemm_bench failed")
     return ms.value
+
+
+def debug_mfma_peak(chains: int, waves_per_simd: int = 2, iters: int = 4096) -> float:
+    """Register-only throughput probe (probe library): TFLOP/s or TOPS."""
+    v = ctypes.c_double()
+    st = load_debug().gwhip_debug_mfma_peak(chains, waves_per_simd, iters, ctypes.byref(v))
+    if st != OK:
+        raise GwhipError(st, "debug_mfma_peak failed")
+    return v.value
 
 
 def _ptr(a: np.ndarray) -> ctypes.c_void_p:
@@ -551,6 +585,30 @@ class GwHipKernel:
         self._check(self.lib.gwhip_get_P_tilde(self.h, _ptr(P)))
         return P
 
+    def eval_q(self, G: np.ndarray, q: int, precision: int = PREC_FP64, want=("P0", "P_tilde", "Sigma_q")) -> dict:
+        """P0(tau, q), P-tilde(tau, q) and q's Sigma contribution (1/nk included) from one q alone."""
+        d = self.dims
+        g = np.ascontiguousarray(G, dtype=np.complex128)
+        pq = (d["nts"], d["NQ"], d["NQ"])
+        out = {"P0": np.empty(pq, complex) if "P0" in want else None,
+               "P_tilde": np.empty(pq, complex) if "P_tilde" in want else None,
+               "Sigma_q": np.empty(self.sigma_shape(), complex) if "Sigma_q" in want else None}
+        ptr = {n: (_ptr(a) if a is not None else None) for n, a in out.items()}
+        self._check(self.lib.gwhip_debug_eval_q(self.h, _ptr(g), int(q), int(precision), ptr["P0"], ptr["P_tilde"],
+                                                ptr["Sigma_q"]))
+        return {n: a for n, a in out.items() if a is not None}
+
+    def set_leakage_capture(self, on: bool):
+        self._check(self.lib.gwhip_set_leakage_capture(self.h, 1 if on else 0))
+
+    def leakage_inputs(self):
+        """(P0, P-tilde) of the first q of the last evaluation, [nts, NQ, NQ] each."""
+        d = self.dims
+        P0 = np.empty((d["nts"], d["NQ"], d["NQ"]), complex)
+        Pt = np.empty_like(P0)
+        self._check(self.lib.gwhip_get_leakage_inputs(self.h, _ptr(P0), _ptr(Pt)))
+        return P0, Pt
+
     def timings(self) -> dict:
         buf = (ctypes.c_double * len(REGIONS))()
         n = self.lib.gwhip_timings(self.h, buf, len(REGIONS))
@@ -699,3 +757,18 @@ class Dyson:
         n = ctypes.c_double()
         self._check(self.lib.gwhip_dyson_number_of_electrons(self.h, _ptr(s1), _ptr(st), float(mu), ctypes.byref(n)))
         return n.value
+
+    def energy(self, G_tau, Sigma1, Sigma_tau):
+        """compute_energy on the device: (E_1b, E_HF, E_corr) (``src/common_utils.cpp:5-55``)."""
+        g = np.ascontiguousarray(G_tau, dtype=np.complex128)
+        s1 = np.ascontiguousarray(Sigma1, dtype=np.complex128)
+        st = np.ascontiguousarray(Sigma_tau, dtype=np.complex128)
+        e = (ctypes.c_double * 3)()
+        self._check(self.lib.gwhip_dyson_energy(self.h, _ptr(g), _ptr(s1), _ptr(st), e))
+        return float(e[0]), float(e[1]), float(e[2])
+
+    def energy_device(self, G_ptr: int, s1_ptr: int, st_ptr: int):
+        e = (ctypes.c_double * 3)()
+        self._check(self.lib.gwhip_dyson_energy_device(self.h, ctypes.c_void_p(G_ptr), ctypes.c_void_p(s1_ptr),
+                                                       ctypes.c_void_p(st_ptr), e))
+        return float(e[0]), float(e[1]), float(e[2])

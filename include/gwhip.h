@@ -39,15 +39,26 @@ enum gwhip_status {
   GWHIP_ERR_OUTDATED_INPUT = 8, /* integral files older than format 1.0.0: mbpt_outdated_input
                                    (src/green/mbpt/df_integral_t.h:42-51, except.h)          */
   GWHIP_ERR_IO = 9,          /* a file or dataset is missing or has the wrong shape        */
-  GWHIP_ERR_MU_SEARCH = 10   /* chemical-potential bisection stalled:
+  GWHIP_ERR_MU_SEARCH = 10,  /* chemical-potential bisection stalled:
                                 mbpt_chemical_potential_search_failure (src/dyson.cpp:148-150) */
+  GWHIP_ERR_NOT_POSITIVE_DEFINITE = 11 /* some I - P0(i Omega_n, q) is not Hermitian positive definite
+                                (a pivot of its factorisation is not positive): P0 is not the
+                                negative semidefinite polarisation of a physical G.  The reference
+                                factorises it with a pivoted LDLT (src/gw_cpu_kernel.cpp:245-250)
+                                and carries on; this kernel reports it instead of returning a
+                                degraded Sigma                                                  */
 };
 
 /* Arithmetic of the P0 and Sigma contractions (the Dyson step, transforms and
  * accumulation across k/q are FP64 in every mode). */
 enum gwhip_precision {
-  GWHIP_PREC_FP64 = 0,       /* complex128 throughout (reference default, P_sp = Sigma_sp = false) */
-  GWHIP_PREC_SPLIT = 1       /* split-precision contractions, FP64 accumulation             */
+  GWHIP_PREC_FP64 = 0,        /* complex128 throughout (reference default, P_sp = Sigma_sp = false) */
+  GWHIP_PREC_SPLIT = 1,       /* split-precision P0 and Sigma contractions, FP64 accumulation
+                                 (P_sp = Sigma_sp = true, src/green/mbpt/common_defs.h:162-163)   */
+  GWHIP_PREC_SPLIT_P0 = 2,    /* split P0 contraction only (P_sp = true, Sigma_sp = false;
+                                 the reference's flag at src/gw_cpu_kernel.cpp:96-100)            */
+  GWHIP_PREC_SPLIT_SIGMA = 3  /* split Sigma contraction only (P_sp = false, Sigma_sp = true;
+                                 src/gw_cpu_kernel.cpp:123-127)                                   */
 };
 
 /* Where the vector-Jacobian product takes P-tilde(tau, q) and P0(tau, q) from. */
@@ -301,6 +312,19 @@ int gwhip_dyson_find_mu_device(gwhip_dyson* d, void* Sigma1_dev, void* Sigma_tau
  * -pref Re sum_n T_tf[nts-1][n] sum_{s,k} (w_k / nk) tr(G(i w_n; mu) S). */
 int gwhip_dyson_number_of_electrons(gwhip_dyson* d, double* Sigma1, double* Sigma_tau, double mu, double* nel);
 
+/* compute_energy (src/common_utils.cpp:5-55, called by dyson::diff, src/dyson.cpp:290-309) on the
+ * device: energies[0] = E_1b = nkpw sum_{s,k} w_k Re tr(dm H), energies[1] = E_HF = nkpw sum_{s,k}
+ * w_k Re tr(dm (Sigma1 + 2 H)) / 2, energies[2] = E_corr (Galitskii-Migdal) = -pref nkpw Re
+ * sum_n T_tf[nts-1][n] sum_{s,k} w_k tr(G(i w_n) Sigma(i w_n)), with dm = -(2 or 1) G(beta),
+ * pref = 1 (ns = 1 without X2C) or 1/2, H and the weights of gwhip_dyson_set_system and the
+ * fermionic transforms of gwhip_dyson_set_grid.  G, Sigma(tau) [nts][ns][ink][nso][nso],
+ * Sigma1 [ns][ink][nso][nso]; read only (the reference does not Hermitise them here).  The
+ * reference's Energy_HF record adds the nuclear repulsion to E_HF (src/dyson.cpp:326). */
+int gwhip_dyson_energy(gwhip_dyson* d, const double* G_tau, const double* Sigma1, const double* Sigma_tau,
+                       double* energies);
+int gwhip_dyson_energy_device(gwhip_dyson* d, const void* G_tau, const void* Sigma1, const void* Sigma_tau,
+                              double* energies);
+
 /* The dimensions the handle was created with. */
 int gwhip_get_dims(const gwhip_handle* h, gwhip_dims* out);
 
@@ -312,6 +336,18 @@ int gwhip_hf_exchange_stats(const gwhip_handle* h, double* ms, int64_t* launches
  * Exposes the intermediate the reference keeps in P0_tilde_s after eval_P_tilde
  * (src/gw_cpu_kernel.cpp:227-279); used by tests. */
 int gwhip_get_P_tilde(gwhip_handle* h, double* P);
+
+/* Keep (on = 1) P0(tau) of the first q of every evaluation, so that gwhip_get_leakage_inputs can
+ * hand it out with that q's P-tilde: the two objects whose IR leakage the reference prints for
+ * q_ir == 0 (ft.check_chebyshev + print_leakage, src/gw_cpu_kernel.cpp:237-239,276-278).
+ * Costs nts/2 * NQ^2 complex of device memory while on. */
+int gwhip_set_leakage_capture(gwhip_handle* h, int on);
+
+/* P0(tau, q0) and P-tilde(tau, q0) of the last evaluation, q0 the first q evaluated (q = 0 on the
+ * full mesh, the first irreducible q on a reduced one): host buffers [nts][NQ][NQ] complex, full
+ * tau grid (P0 after Hermitisation, P0(beta - tau) = P0(tau)).  Either pointer may be null.
+ * GWHIP_ERR_STATE unless the capture was on during that evaluation. */
+int gwhip_get_leakage_inputs(gwhip_handle* h, double* P0, double* P_tilde);
 
 /* Timings of the last eval, in milliseconds, for the region names of the reference's
  * statistics object (src/gw_cpu_kernel.cpp:43-68,205,233-274,363):
@@ -340,24 +376,18 @@ int gwhip_debug_zgemm(int M, int N, int K, const double* A, const double* B, dou
  * one K chunk; row-major host arrays as in gwhip_debug_zgemm. */
 int gwhip_debug_zgemm_split(int M, int N, int K, const double* A, const double* B, double* C, int slices);
 
+/* The stages of one q alone, for checking intermediates at full size: from G (host,
+ * [nts][ns][nk][nso][nso]) compute P0(tau, q) (full tau grid after Hermitisation and the tau
+ * mirror, [nts][NQ][NQ]), P-tilde(tau, q) ([nts][NQ][NQ]) and q's contribution to Sigma, including
+ * the 1/nk of the full sum ([nts][ns][nk][nso][nso]) -- the intermediates of
+ * selfenergy_innerloop(q) (src/gw_cpu_kernel.cpp:74-134).  Any output may be null. */
+int gwhip_debug_eval_q(gwhip_handle* h, const double* G, int64_t q, int precision, double* P0, double* P_tilde,
+                       double* Sigma_q);
+
 /* Run an evaluation only up to stage `last_stage` (1: P0 on the half tau grid after
  * Hermitisation, 2: P-tilde on all tau) and leave the result where gwhip_get_P_tilde
  * reads it.  Sigma is not produced. */
 int gwhip_debug_stage(gwhip_handle* h, const double* G, int last_stage);
-
-/* Time the FP64 GEMM on device 0 with device-resident operands laid out as one q of the
- * Sigma contraction (which = 0: M = NQ, N = nk*nao^2, K = NQ, `batch` tau) or of the P0
- * contraction (which = 1: M = N = NQ, K = nk*nao^2, `batch` tau); `shape` selects the tile
- * (0: 64x64, 1: 128x64, 2: 64x128).  Writes the mean milliseconds per launch over `reps`. */
-int gwhip_debug_gemm_bench(int which, int NQ, int nk, int nao, int batch, int shape, int reps, double* ms);
-
-/* Register-only FP64 MFMA throughput probe on device 0: `chains` independent accumulators
- * per wave (1, 2, 4 or 8; 5 = FP64 vector FMA only, 6 = four MFMA chains plus vector FMAs in
- * the same wave, 10 = int8 16x16x64, 11 = bf16 16x16x32, 12 = FP64 vector-FMA complex 4 x 4
- * outer products with operands from LDS), `waves_per_simd` waves on every SIMD,
- * `iters` rounds.  Writes the achieved
- * TFLOP/s (2048 flop per v_mfma_f64_16x16x4f64, 2 per FMA lane). */
-int gwhip_debug_mfma_peak(int chains, int waves_per_simd, int iters, double* tflops);
 
 #ifdef __cplusplus
 }

@@ -25,7 +25,8 @@ zgemm through numpy's BLAS), keeping the reference's loop nest and GEMM order:
                              the pair index of ``read_integrals`` (``:64-84``) for a mesh
                              without symmetry (no conj/trans redirection).
 * ``gm_energy``           -- ``compute_energy`` (``src/common_utils.cpp:5-55``),
-                             correlation part (Galitskii-Migdal).
+                             correlation part (Galitskii-Migdal); ``compute_energy`` all three
+                             energies (one-body, Hartree-Fock, correlation).
 
 k/q maps follow the variable names at ``gw_cpu_kernel.cpp:90,117,311``: k1_from_k2q(k, q)
 = k + q and k2_from_k1q(k, q) = k - q on a Monkhorst-Pack mesh, componentwise modulo the
@@ -212,6 +213,33 @@ def gw_sigma(G, Vp, T_bt, T_tb, nk, nao, nso, ns, NQ, X2C=None, mesh=None,
     if return_P or return_Pw:
         return Sigma, np.stack(Ps)
     return Sigma
+
+
+def compute_energy(G, Sigma1, Sigma, H, T_ft, T_tf, nk, ns, X2C=False, weights=None):
+    """``compute_energy`` (``src/common_utils.cpp:5-55``): (E_1b, E_HF, E_corr).
+
+    G, Sigma [nts, ns, ink, nso, nso]; Sigma1, H [ns, ink, nso, nso]; weights [ink] (mesh points per
+    irreducible point; None: one each); nkpw = 1/nk.  dm = -(2 or 1) G(beta) (``:16-17``);
+    E_1b = nkpw sum_{s,k} w_k Re tr(dm H) (``:50``), E_HF = nkpw sum 1/2 w_k Re tr(dm (Sigma1 + 2 H))
+    (``:47-48``), E_corr = -pref Re[T_tf[beta] . sum_{s,k} w_k tr(G(i w) Sigma(i w))] nkpw (``:28-41``).
+    """
+    G = np.asarray(G)
+    ink = G.shape[2]
+    w = np.ones(ink) if weights is None else np.asarray(weights, dtype=float)
+    c = 1.0 if (ns == 2 or X2C) else 2.0
+    dm = -c * G[-1]
+    e1 = 0.0
+    ehf = 0.0
+    for s in range(ns):
+        for k in range(ink):
+            e1 += w[k] * np.trace(dm[s, k] @ H[s, k]).real
+            ehf += 0.5 * w[k] * np.trace(dm[s, k] @ (Sigma1[s, k] + 2.0 * H[s, k])).real
+    Gw = np.tensordot(T_ft, G, axes=(1, 0))
+    Sw = np.tensordot(T_ft, Sigma, axes=(1, 0))
+    GS_w = np.einsum("k,nskij,nskji->n", w, Gw, Sw)
+    pref = 1.0 if (ns == 1 and not X2C) else 0.5
+    ecorr = -(T_tf[-1] @ GS_w).real * pref
+    return e1 / nk, ehf / nk, float(ecorr / nk)
 
 
 def gm_energy(G, Sigma, T_ft, T_tf, nk, ns, X2C=False):

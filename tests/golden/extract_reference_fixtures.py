@@ -182,7 +182,24 @@ def main(data_dir):
             orth[mode + "_G_tau"] = f["iter1/G_tau/data"][()]
             orth[mode + "_mu"] = float(f["iter1/mu"][()])
     np.savez_compressed(os.path.join(OUT, "dyson_orth.npz"), tau=mesh, beta=100.0, lam=1e4, **orth)
+    extract_orth_energies(data_dir)
     print("wrote", os.listdir(OUT))
+
+
+def extract_orth_energies(data_dir: str):
+    """The energies the reference stored for the first iteration of each HF_orth_invariance run
+    (iter1/Energy_1b, iter1/Energy_HF = E_HF + E_nuc, iter1/Energy_2b; HF/Energy_nuc of input.h5):
+    compute_energy's outputs (src/common_utils.cpp:5-55, written by dyson.cpp:319-339) for the
+    G_tau, Sigma1 and Sigma_tau of ``dyson_orth.npz`` -> ``orth_energies.npz``."""
+    out = {}
+    for mode in ("none", "lowdin", "mo", "natural"):
+        base = os.path.join(data_dir, "HF_orth_invariance", mode)
+        with h5py.File(os.path.join(base, "input.h5"), "r") as f:
+            out[mode + "_E_nuc"] = float(f["HF/Energy_nuc"][()])
+        with h5py.File(os.path.join(base, "data.h5"), "r") as f:
+            for key in ("Energy_1b", "Energy_HF", "Energy_2b"):
+                out[mode + "_" + key] = float(f["iter1/" + key][()])
+    np.savez(os.path.join(OUT, "orth_energies.npz"), **out)
 
 
 if __name__ == "__main__":

@@ -23,6 +23,17 @@ def test_reader_library_exports_every_header_symbol():
         assert hasattr(h5, n), n
 
 
+def test_debug_library_exports_every_header_symbol():
+    """The probes live in their own library; libgwhip.so exports none of them."""
+    dbg = gl.load_debug()
+    names = gl.header_functions(gl.DEBUG_HEADER)
+    assert names == ["gwhip_debug_gemm_bench", "gwhip_debug_mfma_peak"]
+    lib = gl.load()
+    for n in names:
+        assert hasattr(dbg, n), n
+        assert not hasattr(lib, n), n
+
+
 def test_headers_are_plain_c():
     """The boundary headers compile as C99 and as C++17 (a C caller and the C++ adapter both
     include them), with warnings as errors."""
@@ -32,7 +43,7 @@ def test_headers_are_plain_c():
     import tempfile
 
     inc = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
-    src = '#include "gwhip.h"\n#include "gwhip_h5.h"\nint main(void) { return GWHIP_OK; }\n'
+    src = '#include "gwhip.h"\n#include "gwhip_h5.h"\n#include "gwhip_debug.h"\nint main(void) { return GWHIP_OK; }\n'
     with tempfile.TemporaryDirectory() as d:
         for compiler, ext, std in (("gcc", "c", "-std=c99"), ("g++", "cpp", "-std=c++17")):
             if not shutil.which(compiler):

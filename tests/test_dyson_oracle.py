@@ -105,3 +105,49 @@ def test_search_failure_is_raised():
     """A flat N(mu) that never reaches the target within the bracket raises (find_mu :148-150)."""
     with pytest.raises(od.ChemicalPotentialSearchFailure):
         od.find_mu(0.0, 2.0, 1e-9, lambda mu: 1.0 if mu < 0.3 else 3.0)
+
+
+def _energies():
+    return np.load(os.path.join(GOLDEN, "orth_energies.npz"), allow_pickle=False)
+
+
+@pytest.mark.parametrize("mode", MODES)
+def test_compute_energy_matches_stored_energies(mode):
+    """oracle.gw_oracle.compute_energy on the stored first iteration reproduces the energies the
+    reference wrote for it (iter1/Energy_1b; iter1/Energy_HF minus HF/Energy_nuc; Energy_2b = 0 with
+    Sigma(tau) = 0).  All four bases give the same energies (the reference's own check, 1e-10)."""
+    from oracle import gw_oracle as go
+
+    z, T, _ = _fixture()
+    e = _energies()
+    G, S1, St, H = (z[mode + k] for k in ("_G_tau", "_Sigma1", "_Sigma_tau", "_H"))
+    e1, ehf, ec = go.compute_energy(G, S1, St, H, T.T_ft, T.T_tf, nk=1, ns=2)
+    assert abs(e1 - float(e[mode + "_Energy_1b"])) < 1e-12
+    assert abs(ehf + float(e[mode + "_E_nuc"]) - float(e[mode + "_Energy_HF"])) < 1e-12
+    assert abs(ec - float(e[mode + "_Energy_2b"])) < 1e-14
+
+
+def test_compute_energy_planted_mistakes():
+    """The density matrix with the wrong sign (dm = +G(beta)) or half the one-body H in E_HF miss the
+    stored values."""
+    from oracle import gw_oracle as go
+
+    z, T, _ = _fixture()
+    e = _energies()
+    G, S1, St, H = (z["none" + k] for k in ("_G_tau", "_Sigma1", "_Sigma_tau", "_H"))
+    e1, ehf, _ = go.compute_energy(-G, S1, St, H, T.T_ft, T.T_tf, nk=1, ns=2)
+    assert abs(e1 - float(e["none_Energy_1b"])) > 1.0
+    e1, ehf, _ = go.compute_energy(G, S1, St, 0.5 * H, T.T_ft, T.T_tf, nk=1, ns=2)
+    assert abs(ehf + float(e["none_E_nuc"]) - float(e["none_Energy_HF"])) > 0.1
+
+
+def test_compute_energy_corr_equals_gm_energy():
+    """The correlation part of compute_energy is gm_energy (closed form pinned in test_oracle.py)."""
+    from oracle import gw_oracle as go
+
+    rng = np.random.default_rng(3)
+    z, T, _ = _fixture()
+    G = z["none_G_tau"]
+    St = 0.01 * (rng.standard_normal(G.shape) + 1j * rng.standard_normal(G.shape))
+    _, _, ec = go.compute_energy(G, z["none_Sigma1"], St, z["none_H"], T.T_ft, T.T_tf, nk=1, ns=2)
+    assert abs(ec - go.gm_energy(G, St, T.T_ft, T.T_tf, 1, 2)) < 1e-15
