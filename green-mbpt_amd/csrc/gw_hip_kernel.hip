@@ -91,7 +91,6 @@ gw_hip_kernel::gw_hip_kernel(const Dims& d, int device, hipStream_t shared_strea
 
   const int64_t nk = d_.nk, NQ = d_.NQ, nao2 = d_.nao * d_.nao, NQ2 = NQ * NQ;
   Vq_ = dalloc<double2>((size_t)nk * nk * NQ * nao2);
-  Vt_ = dalloc<double2>((size_t)nk * nk * NQ * nao2);
   P_ = dalloc<double2>((size_t)nk * d_.nts * NQ2);
   Tbt_fold_ = dalloc<double2>((size_t)d_.nw_b * (d_.nts / 2));
   Ttb_ = dalloc<double2>((size_t)d_.nts * d_.nw_b);
@@ -125,7 +124,7 @@ gw_hip_kernel::gw_hip_kernel(const Dims& d, int device, hipStream_t shared_strea
 gw_hip_kernel::~gw_hip_kernel() {
   (void)hipSetDevice(device_);
   if (stream_) (void)hipStreamSynchronize(stream_);
-  for (void* p : {(void*)Vq_, (void*)Vt_, (void*)P_, (void*)Tbt_fold_, (void*)Ttb_, (void*)G_dev_, (void*)S_dev_,
+  for (void* p : {(void*)Vq_, (void*)P_, (void*)Tbt_fold_, (void*)Ttb_, (void*)G_dev_, (void*)S_dev_,
                   (void*)kplus_tab_, (void*)kminus_tab_, (void*)flag_, (void*)dyson_flag_, (void*)leak_P0_, (void*)hcheck_, (void*)hf_work_, (void*)hf_x_, (void*)ksym_idx_, (void*)ksym_U_, (void*)hf_sym_, (void*)q0_qiq_, (void*)q0_A_, (void*)q0_eps_dev_,
                   (void*)q0_S_, (void*)q0_tmp_, (void*)q0_epst_, (void*)qsym_U_, (void*)star_P_, (void*)star_V_,
                   (void*)star_Vs_, (void*)star_eVs_, (void*)star_km_, (void*)star_q0row_, (void*)orb_W_,
@@ -228,7 +227,7 @@ void gw_hip_kernel::upload_V(const double2* V_packed) {
   (void)hipFree(tmp);
   if (tab) (void)hipFree(tab);
   GWHIP_CHECK(e);
-  make_transposed_V();
+  v_changed();
   v_set_ = true;
   stored_valid_ = false;
 }
@@ -350,9 +349,10 @@ void gw_hip_kernel::ensure_slices() {
     a.conj = 1; a.nplanes = 2; a.dst = Vs_p0_; a.exps = eVs_p0_;
     slice_rows(a, ns, stream_);
   }
-  {  // Sigma operand: rows (q, k, (n, j)) of the transposed store, elements Q'
+  {  // Sigma operand: rows (q, k, (n, j)) = V^Q'_jn(k, k+q) (per-Q transposed in place), elements Q'
     SliceArgs a{};
-    a.src = Vt_; a.nrows = nk * nk * nao2; a.row_div = nao2; a.s_rhi = NQ * nao2; a.s_rlo = 1;
+    a.src = Vq_; a.nrows = nk * nk * nao2; a.row_div = nao2; a.s_rhi = NQ * nao2;
+    a.row_div2 = d_.nao; a.s_rlo = 1; a.s_rlo2 = d_.nao;
     a.nchunk = 1; a.s_c = 0; a.klen = (int)NQ; a.s_k = nao2; a.kc = (int)kc_s_;
     a.conj = 1; a.nplanes = 2; a.dst = Vs_s_; a.exps = eVs_s_;
     slice_rows(a, ns, stream_);
@@ -362,15 +362,7 @@ void gw_hip_kernel::ensure_slices() {
   ++generation_;
 }
 
-void gw_hip_kernel::make_transposed_V() {
-  free_slices();
-  const int64_t nmat = d_.nk * d_.nk * d_.NQ;
-  const int64_t total = nmat * d_.nao * d_.nao;
-  hipLaunchKernelGGL(transpose_v_kernel, dim3((unsigned)ceil_div(total, 256)), dim3(256), 0, stream_, Vq_, Vt_,
-                     (int)d_.nao, nmat);
-  GWHIP_CHECK(hipGetLastError());
-  GWHIP_CHECK(hipStreamSynchronize(stream_));
-}
+void gw_hip_kernel::v_changed() { free_slices(); }
 
 void gw_hip_kernel::generate_V(uint64_t seed, double sigma) {
   GWHIP_CHECK(hipSetDevice(device_));
@@ -395,7 +387,7 @@ void gw_hip_kernel::generate_V(uint64_t seed, double sigma) {
   }
   (void)hipFree(tab);
   GWHIP_CHECK(e);
-  make_transposed_V();
+  v_changed();
   v_set_ = true;
   stored_valid_ = false;
 }
@@ -497,7 +489,7 @@ void gw_hip_kernel::p0_prep(int64_t q, const double2* G, int64_t t0, int64_t tc,
   DevDims dd{(int)d_.nts, (int)d_.ns, (int)nk, (int)d_.nso, (int)d_.nao, (int)NQ, (int)d_.pns, (int)d_.pseudo_ns};
   const int NB = (int)ceil_div(d_.nao, 16);
   const size_t lds = (size_t)4 * (16 * NB) * (16 * NB + 1) * sizeof(double);
-  const double2* Vtq = Vt_ + q * nk * NQ * nao2;
+  const double2* Vtq = Vq_ + q * nk * NQ * nao2;
   const dim3 grid((unsigned)ceil_div(NQ, 4 * qpw), (unsigned)nk, (unsigned)tc);
   const int* kp = kplus_tab_ + q * nk;
   if (NB == 1) launch_z_prep<1>(grid, lds, stream_, dd, G, Vtq, scratch_, kp, (int)t0, qpw);
@@ -511,7 +503,7 @@ void gw_hip_kernel::p0_prep_slice(int64_t q, const double2* G, int64_t t0, int64
   DevDims dd{(int)d_.nts, (int)d_.ns, (int)nk, (int)d_.nso, (int)d_.nao, (int)NQ, (int)d_.pns, (int)d_.pseudo_ns};
   const int NB = (int)ceil_div(d_.nao, 16);
   const size_t lds = (size_t)4 * (16 * NB) * (16 * NB + 1) * sizeof(double);
-  const double2* Vtq = Vt_ + q * nk * NQ * nao2;
+  const double2* Vtq = Vq_ + q * nk * NQ * nao2;
   const dim3 grid((unsigned)ceil_div(NQ, 4 * qpw), (unsigned)nk, (unsigned)tc);
   const int* kp = kplus_tab_ + q * nk;
   const int ns = split_ns_;
@@ -536,13 +528,15 @@ void gw_hip_kernel::p0_gemm(int64_t q, int64_t tc, double2* Pt0, int shape) {
 }
 
 // R = P-tilde(t0 .. t0+tc) conj(V) into scratch_, FP64 (tile `shape`): Pt0 = P-tilde at t0
-// [tc][NQ^2], Vb = the per-Q transposed V(k, k - q) of nkb k points [nkb][NQ][nao^2].
+// [tc][NQ^2], Vb = V(k, k - q) of nkb k points [nkb][NQ][nao][nao], read per-Q transposed
+// (column (k, n, j) of the product is V^Q'_jn).
 void gw_hip_kernel::sigma_gemm(const double2* Pt0, const double2* Vb, int64_t nkb, int64_t tc, int shape) {
   const int64_t NQ = d_.NQ, nao2 = d_.nao * d_.nao, NQ2 = NQ * NQ;
   ZGemmArgs a{};
   a.M = (int)NQ; a.N = (int)(nkb * nao2); a.K = (int)NQ; a.batch = (int)tc; a.kin = (int)NQ; a.nin = (int)nao2;
   a.A = Pt0; a.sAb = NQ2; a.sAm = NQ; a.sAko = 0; a.sAki = 1;
   a.B = Vb; a.sBb = 0; a.sBko = 0; a.sBki = nao2; a.sBno = NQ * nao2; a.sBni = 1;
+  a.nin2 = (int)d_.nao; a.sBni2 = d_.nao;
   a.C = scratch_; a.sCb = nkb * NQ * nao2; a.sCm = nao2; a.sCno = NQ * nao2; a.sCni = 1;
   a.alpha = make_double2(1.0, 0.0);
   a.accumulate = 0; a.conjA = 0; a.conjB = 1;
@@ -757,13 +751,13 @@ void gw_hip_kernel::stage_dyson(int64_t nq, const int* q0_row) {
 
 void gw_hip_kernel::stage_sigma(const double2* G, double2* Sigma, bool split) {
   for (int64_t q = 0; q < d_.nk; ++q)
-    sigma_q(G, Sigma, split, P_ + q * d_.nts * d_.NQ * d_.NQ, Vt_neg(q), Vs_neg(q), eVs_neg(q), kminus_tab_ + q * d_.nk,
+    sigma_q(G, Sigma, split, P_ + q * d_.nts * d_.NQ * d_.NQ, Vq_neg(q), Vs_neg(q), eVs_neg(q), kminus_tab_ + q * d_.nk,
             d_.nk);
 }
 
-// V(k, k - q) lives in the k + q' store of q' = neg_q_[q]: per-Q transposed, and its slices.
-const double2* gw_hip_kernel::Vt_neg(int64_t q) const {
-  return Vt_ + (int64_t)neg_q_[q] * d_.nk * d_.NQ * d_.nao * d_.nao;
+// V(k, k - q) lives in the k + q' store of q' = neg_q_[q], and so do its slices.
+const double2* gw_hip_kernel::Vq_neg(int64_t q) const {
+  return Vq_ + (int64_t)neg_q_[q] * d_.nk * d_.NQ * d_.nao * d_.nao;
 }
 const int8_t* gw_hip_kernel::Vs_neg(int64_t q) const {
   return Vs_s_ ? Vs_s_ + (int64_t)neg_q_[q] * d_.nk * d_.nao * d_.nao * 2 * split_ns_ * kc_s_ : nullptr;
@@ -773,7 +767,7 @@ const int* gw_hip_kernel::eVs_neg(int64_t q) const {
 }
 
 // Sigma += -1/nk sum over the nkb k points of one q: Pq = P-tilde of this q ([nts][NQ^2], the half
-// grid read), Vb / Vsb / eVsb = V(k, k - q) of the nkb points (per-Q transposed; slices and
+// grid read), Vb / Vsb / eVsb = V(k, k - q) of the nkb points ([nkb][NQ][nao][nao]; slices and
 // exponents for split precision), km = the full-mesh index of k - q for each, Sigma
 // [nts][ns][nkb][nso^2].
 void gw_hip_kernel::sigma_q(const double2* G, double2* Sigma, bool split, const double2* Pq, const double2* Vb,
@@ -923,7 +917,7 @@ void gw_hip_kernel::sigma_epi_gemm(const double2* G, double2* Sigma, const doubl
         x.M = (int)nao; x.N = (int)(NQ * nao); x.K = (int)nao; x.batch = (int)nkb; x.kin = (int)nao;
         x.nin = (int)nao;
         x.A = orb_G_ + a * nao * nso + b * nao; x.sAb = nso2; x.sAm = 1; x.sAko = 0; x.sAki = nso;
-        x.B = Vb; x.sBb = vq; x.sBko = 0; x.sBki = nao; x.sBno = nao2; x.sBni = 1;
+        x.B = Vb; x.sBb = vq; x.sBko = 0; x.sBki = 1; x.sBno = nao2; x.sBni = nao;
         x.C = orb_W_; x.sCb = vq; x.sCm = nao; x.sCno = nao2; x.sCni = 1;
         x.alpha = make_double2(1.0, 0.0);
         zgemm_f64(x, false, false, stream_);
@@ -1061,7 +1055,7 @@ void gw_hip_kernel::autotune(const double2* G, int precision) {
     const int shapes[3] = {0, 1, 2};
     p0_prep(0, G, 0, tcp, tune_.qpw);
     tune_.p0_shape = best_of(3, shapes, [&](int c) { p0_gemm(0, tcp, P_, c); });
-    tune_.s_shape = best_of(3, shapes, [&](int c) { sigma_gemm(P_, Vt_neg(0), d_.nk, tcs, c); });
+    tune_.s_shape = best_of(3, shapes, [&](int c) { sigma_gemm(P_, Vq_neg(0), d_.nk, tcs, c); });
   } else {
     if (d_.pseudo_ns == 1 && !orb_gemm()) {  // fused prep + slicing vs the two kernels
       const int fz[2] = {0, 1};
@@ -1169,7 +1163,7 @@ void gw_hip_kernel::eval_one_q_host(const double2* G, int64_t q, int precision, 
   if (Pt) GWHIP_CHECK(hipMemcpy(Pt, P_, (size_t)d_.nts * n2 * sizeof(double2), hipMemcpyDeviceToHost));
   if (Sq) {
     zero(S_dev_, (int64_t)nsig);
-    sigma_q(G_dev_, S_dev_, sigma_split(precision), P_, Vt_neg(q), Vs_neg(q), eVs_neg(q), kminus_tab_ + q * d_.nk,
+    sigma_q(G_dev_, S_dev_, sigma_split(precision), P_, Vq_neg(q), Vs_neg(q), eVs_neg(q), kminus_tab_ + q * d_.nk,
             d_.nk);
     finish_sigma(S_dev_, d_.nk);
     GWHIP_CHECK(hipStreamSynchronize(stream_));

@@ -6,7 +6,8 @@
 //   1. P0(tau, q) = -(pref/nk) sum_k Z_k(tau) V_k^H   for the half tau grid  (MFMA GEMM)
 //      with Z_k = sum_s Gbar_s V_k G_s                                      (prep kernel)
 //   2. Hermitise; P0(i Omega) = T_bt' P0(tau);  P-tilde = (I - P0)^-1 P0;  P-tilde(tau) = T_tb P-tilde(i Omega)
-//   3. Sigma(tau, k) += -(1/nk) sum_{Q,n} (V G_s)[Q,i,n] R_k[Q,j,n],  R = P-tilde(tau, q) conj(V)
+//   3. Sigma(tau, k) += -(1/nk) sum_{Q,n} (V G_s)[Q,i,n] R_k[Q,n,j],  R = P-tilde(tau, q) conj(V)
+// V is held once, as V^Q_ab(k, k + q) for every (q, k); V(k, k - q) is the k + q' row of q' = -q.
 // where T_bt' folds P0(beta - tau) = P0(tau) into the tau -> Omega matrix.
 #pragma once
 
@@ -208,8 +209,7 @@ private:
   bool grid_set_ = false, v_set_ = false;
 
   // device buffers
-  double2* Vq_ = nullptr;      // [nq][nk][NQ][nao^2]   V^Q_ab(k, k+q)
-  double2* Vt_ = nullptr;      // [nq][nk][NQ][nao^2]   V^Q_ba(k, k+q)
+  double2* Vq_ = nullptr;      // [nq][nk][NQ][nao^2]   V^Q_ab(k, k+q): the one resident copy of V
   double2* P_ = nullptr;       // [nq][nts][NQ^2]
   double2* Pw_ = nullptr;      // [qc][nw_b][NQ^2]
   double2* X_ = nullptr;       // [qc][nw_b][NQ^2]
@@ -302,7 +302,7 @@ private:
   T* dalloc(size_t count);
   void build_maps();
   void upload_maps(const std::vector<int>& kpq, const std::vector<int>& kmq);
-  void make_transposed_V();
+  void v_changed();  // V was (re)written: drop derived stores
   void ensure_slices();
   void free_slices();
   void ensure_vjp_buffers();
@@ -342,7 +342,7 @@ private:
   void stage_sigma(const double2* G, double2* Sigma, bool split);
   void sigma_q(const double2* G, double2* Sigma, bool split, const double2* Pq, const double2* Vb, const int8_t* Vsb,
                const int* eVsb, const int* km, int64_t nkb);
-  const double2* Vt_neg(int64_t q) const;
+  const double2* Vq_neg(int64_t q) const;  // V(k, k - q) for every k: the k + q' rows of q' = -q
   const int8_t* Vs_neg(int64_t q) const;
   const int* eVs_neg(int64_t q) const;
   bool sigma_upper() const;

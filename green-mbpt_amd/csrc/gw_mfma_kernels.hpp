@@ -7,9 +7,11 @@
 // NB blocks of 16 (nao <= 16 NB); complex products are four real MFMAs.
 //
 // Layouts (complex = double2):
-//   Vt[k][Q][m][p]  = V^Q_pm(k, k + q)      per-Q transpose of the resident integrals
+//   V [k][Q][p][m]  = V^Q_pm(k, k +- q)     the resident integrals of one q (the only copy)
 //   Z [tz][k][Q][t][n]                       P0 left operand (row-major per Q)
 //   R [tz][k][Q][n][j]                       P-tilde conj(V), per-Q transposed
+// The orbital products take V_Q as the operand whose K index runs over its second orbital index,
+// so a lane group reads four consecutive elements of a row (64 bytes) per slab.
 #pragma once
 
 #include "../../include/gwhip.h"
@@ -107,7 +109,7 @@ template <typename T> struct CAcc {
 // more waves per SIMD), all of it for NB = 3.
 template <int NB, typename T>
 __global__ __launch_bounds__(256, NB < 3 ? 2 : 1) void z_prep_mfma_kernel(XcdGrid xg, DevDims d, const double2* __restrict__ G,
-                                                          const double2* __restrict__ Vt,
+                                                          const double2* __restrict__ V,
                                                           double2* __restrict__ Z, const int* __restrict__ kplus,
                                                           int t0, int QPW) {
   typedef Mfma<T> M;
@@ -147,7 +149,7 @@ __global__ __launch_bounds__(256, NB < 3 ? 2 : 1) void z_prep_mfma_kernel(XcdGri
     for (int qi = 0; qi < QPW; ++qi) {
       const int Q = (bx * 4 + wave) * QPW + qi;
       if (Q >= d.NQ) break;
-      const double2* vt = Vt + ((int64_t)k * d.NQ + Q) * nao2;  // vt[m * nao + p] = V[p][m]
+      const double2* vq = V + ((int64_t)k * d.NQ + Q) * nao2;  // vq[p * nao + m] = V[p][m]
       double2* zq = Z + (((int64_t)tz * d.nk + k) * d.NQ + Q) * nao2;
       // W[:, bn] = V Gq[:, bn] for every column block, then Z[:, bn] (+)= Gb W[:, bn] for all of them
       // at once (each Gb element read from LDS serves every column block; NB^2 independent
@@ -163,7 +165,7 @@ __global__ __launch_bounds__(256, NB < 3 ? 2 : 1) void z_prep_mfma_kernel(XcdGri
 #pragma unroll
         for (int bp = 0; bp < NB; ++bp) {
           const int p = 16 * bp + lrow;
-          av[bp] = (p < nao && m < nao) ? vt[m * nao + p] : make_double2(0.0, 0.0);
+          av[bp] = (p < nao && m < nao) ? vq[p * nao + m] : make_double2(0.0, 0.0);
         }
 #pragma unroll
         for (int bn = 0; bn < NB; ++bn) {
@@ -226,7 +228,7 @@ __global__ __launch_bounds__(256, NB < 3 ? 2 : 1) void z_prep_mfma_kernel(XcdGri
 // Two waves per SIMD for NB <= 2 (three spill and run 15 % slower at nao = 26).
 template <int NB, int NS>
 __global__ __launch_bounds__(256, NB < 3 ? 2 : 1) void z_prep_slice_kernel(XcdGrid xg, DevDims d, const double2* __restrict__ G,
-                                                           const double2* __restrict__ Vt, int8_t* __restrict__ dst,
+                                                           const double2* __restrict__ V, int8_t* __restrict__ dst,
                                                            int* __restrict__ exps, int kc,
                                                            const int* __restrict__ kplus, int t0, int QPW) {
   constexpr int N16 = 16 * NB, LD = N16 + 1, PL = N16 * LD;
@@ -263,7 +265,7 @@ __global__ __launch_bounds__(256, NB < 3 ? 2 : 1) void z_prep_slice_kernel(XcdGr
   for (int qi = 0; qi < QPW; ++qi) {
     const int Q = (bx * 4 + wave) * QPW + qi;
     if (Q >= d.NQ) break;
-    const double2* vt = Vt + ((int64_t)k * d.NQ + Q) * nao2;  // vt[m * nao + p] = V[p][m]
+    const double2* vq = V + ((int64_t)k * d.NQ + Q) * nao2;  // vq[p * nao + m] = V[p][m]
     CAcc<double> z[NB][NB];                                 // [bn][bt]
     double2 avs[4 * NB][NB];                                    // V_Q, loaded once per Q
 #pragma unroll
@@ -271,7 +273,7 @@ __global__ __launch_bounds__(256, NB < 3 ? 2 : 1) void z_prep_slice_kernel(XcdGr
 #pragma unroll
       for (int bp = 0; bp < NB; ++bp) {
         const int m = 4 * sl + lk, p = 16 * bp + lrow;
-        avs[sl][bp] = (p < nao && m < nao) ? vt[m * nao + p] : make_double2(0.0, 0.0);
+        avs[sl][bp] = (p < nao && m < nao) ? vq[p * nao + m] : make_double2(0.0, 0.0);
       }
     // W[:, bn] = V_Q Gq[:, bn] for every column block, then Z[:, bn] = Gb W[:, bn] for all of them
     // at once: each Gb element read from LDS serves every column block and the second product has
@@ -379,7 +381,7 @@ __global__ __launch_bounds__(256, NB < 3 ? 2 : 1) void z_prep_slice_kernel(XcdGr
 // sums meet in LDS.  qsplit = 1 (part = nullptr) adds to Sigma; qsplit > 1 stores each range's sum
 // to part[y] ([qsplit][nts][ns][nks][nso^2]), which epi_part_reduce_kernel then adds to Sigma in
 // range order (more work-groups when nks x tcount alone would not fill the chip).  Block x = k is the position in this
-// launch's k list: V (Vt[k]), R ([t][nks][Q][nao^2]) and Sigma ([nts][ns][nks][nso^2]) use it
+// launch's k list: V (V[k] = V(k, k - q)), R ([t][nks][Q][nao^2]) and Sigma ([nts][ns][nks][nso^2]) use it
 // directly, G (full mesh) is read at kminus[k].  The full mesh has nks = nk; the star-of-q
 // evaluation passes the irreducible k points only.
 // With T = float each Q's product (K <= 16 NB) is added to FP64 accumulators.
@@ -388,7 +390,7 @@ __global__ __launch_bounds__(256, NB < 3 ? 2 : 1) void z_prep_slice_kernel(XcdGr
 // two waves without a spill).  V_Q is loaded slab by slab inside the first product.
 template <int NB, typename T>
 __global__ __launch_bounds__(256, NB < 3 ? 3 : 1) void sigma_epi_mfma_kernel(XcdGrid xg, DevDims d, const double2* __restrict__ G,
-                                                             const double2* __restrict__ Vt,
+                                                             const double2* __restrict__ V,
                                                              const double2* __restrict__ R,
                                                              double2* __restrict__ Sigma,
                                                              const int* __restrict__ kminus, int nks, int t0,
@@ -440,7 +442,7 @@ __global__ __launch_bounds__(256, NB < 3 ? 3 : 1) void sigma_epi_mfma_kernel(Xcd
       for (int y = 0; y < NB; ++y) acc64[x][y].zero();
 
     for (int Q = qlo + wave; Q < qhi; Q += 4) {
-      const double2* vt = Vt + ((int64_t)k * d.NQ + Q) * nao2;                   // vt[m * nao + i] = V[i][m]
+      const double2* vq = V + ((int64_t)k * d.NQ + Q) * nao2;                    // vq[i * nao + m] = V[i][m]
       const double2* rq = R + (((int64_t)tz * nks + k) * d.NQ + Q) * nao2;      // rq[n * nao + j]
       if (F64) {
         // all row blocks at once: each G element read from LDS and each R element loaded once per Q
@@ -460,7 +462,7 @@ __global__ __launch_bounds__(256, NB < 3 ? 3 : 1) void sigma_epi_mfma_kernel(Xcd
 #pragma unroll
           for (int bi = 0; bi < NB; ++bi) {
             const int i = 16 * bi + lrow;
-            bv[bi] = (m < nao && i < nao) ? vt[m * nao + i] : make_double2(0.0, 0.0);
+            bv[bi] = (m < nao && i < nao) ? vq[i * nao + m] : make_double2(0.0, 0.0);
           }
 #pragma unroll
           for (int bn = 0; bn < NB; ++bn) {
@@ -504,7 +506,7 @@ __global__ __launch_bounds__(256, NB < 3 ? 3 : 1) void sigma_epi_mfma_kernel(Xcd
         for (int sl = 0; sl < 4 * NB; ++sl) {
           if (sl >= nslab) break;
           const int m = 4 * sl + lk;
-          const double2 bv = (m < nao && i < nao) ? vt[m * nao + i] : make_double2(0.0, 0.0);
+          const double2 bv = (m < nao && i < nao) ? vq[i * nao + m] : make_double2(0.0, 0.0);
 #pragma unroll
           for (int bn = 0; bn < NB; ++bn) {
             const int n = 16 * bn + lrow;
@@ -632,17 +634,6 @@ __global__ void epi_part_reduce_kernel(const double2* __restrict__ part, double2
     v.y += acc.y;
     Sigma[idx] = v;
   }
-}
-
-// Per-Q transpose of the resident integrals: Vt[..][Q][b][a] = V[..][Q][a][b].
-__global__ void transpose_v_kernel(const double2* __restrict__ V, double2* __restrict__ Vt, int nao, int64_t nmat) {
-  const int64_t gid = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  const int64_t nao2 = (int64_t)nao * nao;
-  if (gid >= nmat * nao2) return;
-  const int64_t mat = gid / nao2;
-  const int e = (int)(gid - mat * nao2);
-  const int a = e / nao, b = e - (e / nao) * nao;
-  Vt[mat * nao2 + (int64_t)b * nao + a] = V[gid];
 }
 
 }  // namespace gwhip

@@ -319,7 +319,7 @@ void gw_hip_kernel::solve_star_device(const double2* G_ibz, double2* Sigma_ibz, 
   for (int64_t iq = 0; iq < inq_; ++iq)
     for (int q : qstar_[iq]) {
       const double2* Pq = star_rotate(iq, q);
-      launch_gather_rows(Vt_neg(q), star_V_, rows, NQ * nao2, ink, stream_);
+      launch_gather_rows(Vq_neg(q), star_V_, rows, NQ * nao2, ink, stream_);
       if (split) {
         const int64_t len = nao2 * 2 * split_ns_ * kc_s_;  // a multiple of 128 bytes (kc_s is of 64)
         launch_gather_rows(reinterpret_cast<const uint4*>(Vs_neg(q)), reinterpret_cast<uint4*>(star_Vs_), rows,

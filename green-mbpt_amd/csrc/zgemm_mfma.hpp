@@ -7,6 +7,8 @@
 // so every operand element is located by
 //   A(m, k) = A + b*sAb + m*sAm + (k / kin)*sAko + (k % kin)*sAki
 //   B(k, n) = B + b*sBb + (k / kin)*sBko + (k % kin)*sBki + (n / nin)*sBno + (n % nin)*sBni
+// (with nin2 > 0 the inner part splits once more: ni = n % nin -> (ni / nin2)*sBni + (ni % nin2)*sBni2,
+// which reads a per-block transposed operand in place)
 //   C(m, n) = C + b*sCb + m*sCm + (n / nin)*sCno + (n % nin)*sCni
 // which lets one kernel run the P0 contraction, the Sigma contraction, the IR transforms
 // and the Dyson products straight out of their resident layouts.
@@ -30,6 +32,8 @@ struct ZGemmArgs {
   int64_t sAb, sAm, sAko, sAki;
   const double2* B;
   int64_t sBb, sBko, sBki, sBno, sBni;
+  int nin2;          // 0: B's inner N index is one level; > 0: split as above
+  int64_t sBni2;
   double2* C;
   int64_t sCb, sCm, sCno, sCni;
   double2 alpha;
@@ -145,8 +149,13 @@ __global__ __launch_bounds__(256, (BM * BN > 4096) ? 1 : 2) void zgemm_f64_kerne
     const int n = n0 + b_n[j];
     b_ok[j] = n < p.N;
     const int nn = b_ok[j] ? n : 0;
-    const int no = nn / p.nin;
-    b_noff[j] = (int64_t)no * p.sBno + (int64_t)(nn - no * p.nin) * p.sBni;
+    const int no = nn / p.nin, ni = nn - no * p.nin;
+    if (p.nin2 > 0) {
+      const int nh = ni / p.nin2;
+      b_noff[j] = (int64_t)no * p.sBno + (int64_t)nh * p.sBni + (int64_t)(ni - nh * p.nin2) * p.sBni2;
+    } else {
+      b_noff[j] = (int64_t)no * p.sBno + (int64_t)ni * p.sBni;
+    }
     kb[j].init(kbeg + b_k[j], p.kin, p.sBko, p.sBki);
   }
 

@@ -27,11 +27,14 @@ typedef int v4i __attribute__((ext_vector_type(4)));
 
 // ------------------------------------------------------------------------------ slicing
 // Row r = r_hi * row_div + r_lo of a strided FP64 complex source; element kk of chunk c at
-//   src + r_hi*s_rhi + r_lo*s_rlo + c*s_c + kk*s_k,   kk < klen (zero up to kc).
+//   src + r_hi*s_rhi + r_lo*s_rlo + c*s_c + kk*s_k,   kk < klen (zero up to kc);
+// with row_div2 > 0 the low part splits once more, r_lo = a * row_div2 + b at a*s_rlo + b*s_rlo2
+// (rows that run over a per-block transposed source).
 struct SliceArgs {
   const double2* src;
   int64_t nrows;
   int64_t row_div, s_rhi, s_rlo;
+  int64_t row_div2, s_rlo2;
   int nchunk;
   int64_t s_c;
   int klen;
@@ -58,7 +61,12 @@ __global__ __launch_bounds__(256) void slice_rows_kernel(SliceArgs a) {
   const int64_t row = task / a.nchunk;
   const int c = (int)(task - row * a.nchunk);
   const int64_t rhi = row / a.row_div, rlo = row - rhi * a.row_div;
-  const double2* s = a.src + rhi * a.s_rhi + rlo * a.s_rlo + (int64_t)c * a.s_c;
+  int64_t roff = rlo * a.s_rlo;
+  if (a.row_div2 > 0) {
+    const int64_t rh = rlo / a.row_div2;
+    roff = rh * a.s_rlo + (rlo - rh * a.row_div2) * a.s_rlo2;
+  }
+  const double2* s = a.src + rhi * a.s_rhi + roff + (int64_t)c * a.s_c;
   double m = 0.0;
   for (int kk = lane; kk < a.klen; kk += 64) {
     const double2 v = s[(int64_t)kk * a.s_k];
