@@ -40,8 +40,8 @@ PMC_PER_LAUNCH_KIB = {
 }
 PMC_CALIBRATION = (0.5, 1.0)
 PMC_SOURCE = "profiles/gw_sigma_nk64_nao26_naux300_hbm_traffic.md"
-# int8 GEMM kernel per block-tile setting (tuning key sigma_split_tile / p0_split_tile)
-OZAKI_KERNELS = {0: "zgemm_ozaki_kernel", 1: "zgemm_ozaki64_kernel", 2: "zgemm_ozaki_pipe_kernel"}
+# the int8 GEMM kernel (one block tile: 32 x 64, software-pipelined)
+OZAKI_KERNEL = "zgemm_ozaki_pipe_kernel"
 
 
 def algorithmic_flops(nk, nao, NQ, nts, nw_b, pseudo_ns=1):
@@ -97,9 +97,9 @@ def roofline(stats, precision, w, T, slices, tuning):
     work = stage_work(w, T, slices)
     split = precision == "split"
     int8 = {"Sigma GEMM", "P0 GEMM"} if split else set()
-    names = {"Sigma GEMM": (f"{OZAKI_KERNELS[tuning['sigma_split_tile']]}<{slices}> (R = P-tilde conj(V), int8 slices)"
+    names = {"Sigma GEMM": (f"{OZAKI_KERNEL}<{slices}> (R = P-tilde conj(V), int8 slices)"
                             if split else "zgemm_f64_kernel<64,64,true,false> (R = P-tilde conj(V), FP64 MFMA)"),
-             "P0 GEMM": (f"{OZAKI_KERNELS[tuning['p0_split_tile']]}<{slices}> (P0 = Z conj(V)^T, int8 slices)"
+             "P0 GEMM": (f"{OZAKI_KERNEL}<{slices}> (P0 = Z conj(V)^T, int8 slices)"
                          if split else "zgemm_f64_kernel<64,64,true,true> (P0 = Z conj(V)^T, FP64 MFMA)"),
              "Sigma epilogue": "sigma_epi_mfma_kernel<2,double> (Sigma += (V G) R, FP64 MFMA)",
              "P0 prep": ("z_prep_slice_kernel<2,5>" if split else "z_prep_mfma_kernel<2,double>") +

@@ -262,7 +262,7 @@ size_t gw_hip_kernel::scratch_per_t() const { return (size_t)d_.nk * d_.NQ * d_.
 size_t gw_hip_kernel::dyson_per_q() const { return (size_t)3 * d_.nw_b * d_.NQ * d_.NQ; }
 size_t gw_hip_kernel::slice_bytes_per_t() const {
   const size_t kcp = (size_t)ceil_div(d_.nao * d_.nao, 64) * 64, kcs = (size_t)ceil_div(d_.NQ, 64) * 64;
-  return std::max((size_t)d_.NQ * d_.nk * 2 * split_ns_ * kcp, (size_t)d_.NQ * 2 * split_ns_ * kcs);
+  return std::max((size_t)d_.NQ * d_.nk * OZ_PLANES * split_ns_ * kcp, (size_t)d_.NQ * OZ_PLANES * split_ns_ * kcs);
 }
 size_t gw_hip_kernel::exps_per_t() const { return (size_t)d_.NQ * d_.nk; }
 
@@ -333,20 +333,20 @@ void gw_hip_kernel::ensure_slices() {
     throw status_error(GWHIP_ERR_UNSUPPORTED,
                        "split precision: K chunk of " + std::to_string(std::max(kc_p0_, kc_s_)) +
                            " exceeds the int32-exact limit " + std::to_string(kc_max) + " (nao^2 and naux must stay below it)");
-  Vs_p0_ = dalloc<int8_t>((size_t)nk * NQ * nk * 2 * ns * kc_p0_);
+  Vs_p0_ = dalloc<int8_t>((size_t)nk * NQ * nk * OZ_PLANES * ns * kc_p0_);
   eVs_p0_ = dalloc<int>((size_t)nk * NQ * nk);
-  Vs_s_ = dalloc<int8_t>((size_t)nk * nk * nao2 * 2 * ns * kc_s_);
+  Vs_s_ = dalloc<int8_t>((size_t)nk * nk * nao2 * OZ_PLANES * ns * kc_s_);
   eVs_s_ = dalloc<int>((size_t)nk * nk * nao2);
   if (own_ws_) {  // Z slices (P0 stage) and P-tilde slices (Sigma stage) share one buffer
-    Zs_ = Ps_ = dalloc<int8_t>(std::max((size_t)tchunk_p0_ * NQ * nk * 2 * ns * kc_p0_,
-                                        (size_t)tchunk_sigma_ * NQ * 2 * ns * kc_s_));
+    Zs_ = Ps_ = dalloc<int8_t>(std::max((size_t)tchunk_p0_ * NQ * nk * OZ_PLANES * ns * kc_p0_,
+                                        (size_t)tchunk_sigma_ * NQ * OZ_PLANES * ns * kc_s_));
     eZs_ = ePs_ = dalloc<int>(std::max((size_t)tchunk_p0_ * NQ * nk, (size_t)tchunk_sigma_ * NQ));
   }
   {  // P0 operand: rows (q, Q'), chunks k, elements (t, n) of V(k, k+q)
     SliceArgs a{};
     a.src = Vq_; a.nrows = nk * NQ; a.row_div = NQ; a.s_rhi = nk * NQ * nao2; a.s_rlo = nao2;
     a.nchunk = (int)nk; a.s_c = NQ * nao2; a.klen = (int)nao2; a.s_k = 1; a.kc = (int)kc_p0_;
-    a.conj = 1; a.nplanes = 2; a.dst = Vs_p0_; a.exps = eVs_p0_;
+    a.conj = 1; a.dst = Vs_p0_; a.exps = eVs_p0_;
     slice_rows(a, ns, stream_);
   }
   {  // Sigma operand: rows (q, k, (n, j)) = V^Q'_jn(k, k+q) (per-Q transposed in place), elements Q'
@@ -354,7 +354,7 @@ void gw_hip_kernel::ensure_slices() {
     a.src = Vq_; a.nrows = nk * nk * nao2; a.row_div = nao2; a.s_rhi = NQ * nao2;
     a.row_div2 = d_.nao; a.s_rlo = 1; a.s_rlo2 = d_.nao;
     a.nchunk = 1; a.s_c = 0; a.klen = (int)NQ; a.s_k = nao2; a.kc = (int)kc_s_;
-    a.conj = 1; a.nplanes = 2; a.dst = Vs_s_; a.exps = eVs_s_;
+    a.conj = 1; a.dst = Vs_s_; a.exps = eVs_s_;
     slice_rows(a, ns, stream_);
   }
   GWHIP_CHECK(hipStreamSynchronize(stream_));
@@ -549,15 +549,15 @@ void gw_hip_kernel::p0_slice(int64_t tc) {
   SliceArgs sa{};
   sa.src = scratch_; sa.nrows = tc * NQ; sa.row_div = NQ; sa.s_rhi = nk * NQ * nao2; sa.s_rlo = nao2;
   sa.nchunk = (int)nk; sa.s_c = NQ * nao2; sa.klen = (int)nao2; sa.s_k = 1; sa.kc = (int)kc_p0_;
-  sa.conj = 0; sa.nplanes = 2; sa.dst = Zs_; sa.exps = eZs_;
+  sa.conj = 0; sa.dst = Zs_; sa.exps = eZs_;
   slice_rows(sa, split_ns_, stream_);
 }
 
 // Split precision, P0 = -(pref/nk) Z conj(V)^T from the slices (row-tile group `group`).
-void gw_hip_kernel::p0_gemm_split(int64_t q, int64_t tc, double2* Pt0, int group, int tile, int kchunks) {
+void gw_hip_kernel::p0_gemm_split(int64_t q, int64_t tc, double2* Pt0, int group, int kchunks) {
   const int64_t nk = d_.nk, NQ = d_.NQ, NQ2 = NQ * NQ;
   const int ns = split_ns_;
-  const int64_t sc = 2 * ns * kc_p0_;  // bytes of one k chunk of a row
+  const int64_t sc = OZ_PLANES * ns * kc_p0_;  // bytes of one k chunk of a row
   // K = nk chunks of kc: in launches over `kchunks` consecutive chunks each, the first writing C
   // and the others adding to it, so that one launch's A and B chunks are reused from L2 by all
   // its tiles instead of streaming the whole K past every tile (partial sums in FP64, as between
@@ -576,7 +576,6 @@ void gw_hip_kernel::p0_gemm_split(int64_t q, int64_t tc, double2* Pt0, int group
     o.alpha = make_double2(-d_.prefactor / (double)nk, 0.0);
     o.accumulate = sl > 0 ? 1 : 0;
     o.group_m = group;
-    o.tile = tile;
     o.upper = g_herm_ ? 1 : 0;
     zgemm_ozaki(o, ns, stream_);
   }
@@ -588,28 +587,26 @@ void gw_hip_kernel::sigma_slice(const double2* Pt0, int64_t tc) {
   SliceArgs sa{};
   sa.src = Pt0; sa.nrows = tc * NQ; sa.row_div = tc * NQ; sa.s_rhi = 0; sa.s_rlo = NQ;
   sa.nchunk = 1; sa.s_c = 0; sa.klen = (int)NQ; sa.s_k = 1; sa.kc = (int)kc_s_;
-  sa.conj = 0; sa.nplanes = 2; sa.dst = Ps_; sa.exps = ePs_;
+  sa.conj = 0; sa.dst = Ps_; sa.exps = ePs_;
   slice_rows(sa, split_ns_, stream_);
 }
 
 // Split precision, R = P-tilde conj(V) into scratch_ from the slices: Vsb / eVsb = the slices and
 // exponents of V(k, k - q) for nkb k points ([nkb][nao^2][2][ns][kc_s], [nkb][nao^2]).
-void gw_hip_kernel::sigma_gemm_split(const int8_t* Vsb, const int* eVsb, int64_t nkb, int64_t tc, int group,
-                                     int tile) {
+void gw_hip_kernel::sigma_gemm_split(const int8_t* Vsb, const int* eVsb, int64_t nkb, int64_t tc, int group) {
   const int64_t NQ = d_.NQ, nao2 = d_.nao * d_.nao;
   const int ns = split_ns_;
   OzGemmArgs o{};
   o.M = (int)NQ; o.N = (int)(nkb * nao2); o.batch = (int)tc; o.kc = (int)kc_s_; o.nchunk = 1; o.nin = (int)nao2;
-  o.A = Ps_; o.sAm = 2 * ns * kc_s_; o.sAb = NQ * o.sAm; o.sAc = 0;
+  o.A = Ps_; o.sAm = OZ_PLANES * ns * kc_s_; o.sAb = NQ * o.sAm; o.sAc = 0;
   o.eA = ePs_; o.seAb = NQ; o.seAm = 1;
-  o.B = Vsb; o.sBb = 0; o.sBno = nao2 * 2 * ns * kc_s_;
-  o.sBni = 2 * ns * kc_s_; o.sBc = 0;
+  o.B = Vsb; o.sBb = 0; o.sBno = nao2 * OZ_PLANES * ns * kc_s_;
+  o.sBni = OZ_PLANES * ns * kc_s_; o.sBc = 0;
   o.eB = eVsb; o.seBb = 0; o.seBno = nao2; o.seBni = 1;
   o.C = scratch_; o.sCb = nkb * NQ * nao2; o.sCm = nao2; o.sCno = NQ * nao2; o.sCni = 1;
   o.alpha = make_double2(1.0, 0.0);
   o.accumulate = 0;
   o.group_m = group;
-  o.tile = tile;
   zgemm_ozaki(o, ns, stream_);
 }
 
@@ -632,7 +629,7 @@ void gw_hip_kernel::p0_one_q(int64_t q, const double2* G, bool split, double2* P
     }
     if (split) {
       timed(K_P0_GEMM, [&] {
-        p0_gemm_split(q, tc, Pq + t0 * NQ2, tune_.p0_group, tune_.p0_oz_tile, tune_.p0_kchunks);
+        p0_gemm_split(q, tc, Pq + t0 * NQ2, tune_.p0_group, tune_.p0_kchunks);
       });
     } else {
       timed(K_P0_GEMM, [&] { p0_gemm(q, tc, Pq + t0 * NQ2, tune_.p0_shape); });
@@ -760,7 +757,7 @@ const double2* gw_hip_kernel::Vq_neg(int64_t q) const {
   return Vq_ + (int64_t)neg_q_[q] * d_.nk * d_.NQ * d_.nao * d_.nao;
 }
 const int8_t* gw_hip_kernel::Vs_neg(int64_t q) const {
-  return Vs_s_ ? Vs_s_ + (int64_t)neg_q_[q] * d_.nk * d_.nao * d_.nao * 2 * split_ns_ * kc_s_ : nullptr;
+  return Vs_s_ ? Vs_s_ + (int64_t)neg_q_[q] * d_.nk * d_.nao * d_.nao * OZ_PLANES * split_ns_ * kc_s_ : nullptr;
 }
 const int* gw_hip_kernel::eVs_neg(int64_t q) const {
   return eVs_s_ ? eVs_s_ + (int64_t)neg_q_[q] * d_.nk * d_.nao * d_.nao : nullptr;
@@ -787,7 +784,7 @@ void gw_hip_kernel::sigma_q(const double2* G, double2* Sigma, bool split, const 
     const int64_t tc = std::min(tchunk_sigma_, half - t0);
     if (split) {
       sigma_slice(Pq + t0 * NQ2, tc);
-      timed(K_SIGMA_GEMM, [&] { sigma_gemm_split(Vsb, eVsb, nkb, tc, tune_.s_group, tune_.s_oz_tile); });
+      timed(K_SIGMA_GEMM, [&] { sigma_gemm_split(Vsb, eVsb, nkb, tc, tune_.s_group); });
     } else {
       timed(K_SIGMA_GEMM, [&] { sigma_gemm(Pq + t0 * NQ2, Vb, nkb, tc, tune_.s_shape); });
     }
@@ -1068,21 +1065,15 @@ void gw_hip_kernel::autotune(const double2* G, int precision) {
         }
       });
     }
-    // candidates (block tile, row-tile group) encoded as 100 * tile + group
-    const int cands[21] = {1, 2, 4, 8, 16, 32, 64, 101, 102, 104, 108, 116, 132, 164,
-                           201, 202, 204, 208, 216, 232, 264};
+    // row-tile group of the L2-aware tile order
+    const int groups[7] = {1, 2, 4, 8, 16, 32, 64};
     p0_prep(0, G, 0, tcp, tune_.qpw);
     p0_slice(tcp);
-    const int bp = best_of(21, cands, [&](int c) { p0_gemm_split(0, tcp, P_, c % 100, c / 100, tune_.p0_kchunks); });
-    tune_.p0_oz_tile = bp / 100;
-    tune_.p0_group = bp % 100;
+    tune_.p0_group = best_of(7, groups, [&](int c) { p0_gemm_split(0, tcp, P_, c, tune_.p0_kchunks); });
     const int kch[5] = {0, 16, 8, 4, 2};
-    tune_.p0_kchunks = best_of(5, kch, [&](int c) { p0_gemm_split(0, tcp, P_, tune_.p0_group, tune_.p0_oz_tile, c); });
+    tune_.p0_kchunks = best_of(5, kch, [&](int c) { p0_gemm_split(0, tcp, P_, tune_.p0_group, c); });
     sigma_slice(P_, tcs);
-    const int bs =
-        best_of(21, cands, [&](int c) { sigma_gemm_split(Vs_neg(0), eVs_neg(0), d_.nk, tcs, c % 100, c / 100); });
-    tune_.s_oz_tile = bs / 100;
-    tune_.s_group = bs % 100;
+    tune_.s_group = best_of(7, groups, [&](int c) { sigma_gemm_split(Vs_neg(0), eVs_neg(0), d_.nk, tcs, c); });
   }
   stored_valid_ = false;
   ++generation_;
@@ -1237,24 +1228,24 @@ void debug_zgemm_split(int M, int N, int K, const double2* A, const double2* B, 
   GWHIP_CHECK(hipMalloc(&dA, (size_t)M * K * sizeof(double2)));
   GWHIP_CHECK(hipMalloc(&dB, (size_t)K * N * sizeof(double2)));
   GWHIP_CHECK(hipMalloc(&dC, (size_t)M * N * sizeof(double2)));
-  GWHIP_CHECK(hipMalloc(&sA, (size_t)M * 2 * ns * kc));
-  GWHIP_CHECK(hipMalloc(&sB, (size_t)N * 2 * ns * kc));
+  GWHIP_CHECK(hipMalloc(&sA, (size_t)M * OZ_PLANES * ns * kc));
+  GWHIP_CHECK(hipMalloc(&sB, (size_t)N * OZ_PLANES * ns * kc));
   GWHIP_CHECK(hipMalloc(&eA, (size_t)M * sizeof(int)));
   GWHIP_CHECK(hipMalloc(&eB, (size_t)N * sizeof(int)));
   GWHIP_CHECK(hipMemcpy(dA, A, (size_t)M * K * sizeof(double2), hipMemcpyHostToDevice));
   GWHIP_CHECK(hipMemcpy(dB, B, (size_t)K * N * sizeof(double2), hipMemcpyHostToDevice));
   SliceArgs a{};
   a.src = dA; a.nrows = M; a.row_div = M; a.s_rhi = 0; a.s_rlo = K; a.nchunk = 1; a.s_c = 0; a.klen = K; a.s_k = 1;
-  a.kc = kc; a.conj = 0; a.nplanes = 2; a.dst = sA; a.exps = eA;
+  a.kc = kc; a.conj = 0; a.dst = sA; a.exps = eA;
   slice_rows(a, ns, nullptr);
   SliceArgs bsl{};
   bsl.src = dB; bsl.nrows = N; bsl.row_div = N; bsl.s_rhi = 0; bsl.s_rlo = 1; bsl.nchunk = 1; bsl.s_c = 0; bsl.klen = K;
-  bsl.s_k = N; bsl.kc = kc; bsl.conj = 0; bsl.nplanes = 2; bsl.dst = sB; bsl.exps = eB;
+  bsl.s_k = N; bsl.kc = kc; bsl.conj = 0; bsl.dst = sB; bsl.exps = eB;
   slice_rows(bsl, ns, nullptr);
   OzGemmArgs o{};
   o.M = M; o.N = N; o.batch = 1; o.kc = kc; o.nchunk = 1; o.nin = N;
-  o.A = sA; o.sAm = 2 * ns * kc; o.eA = eA; o.seAm = 1;
-  o.B = sB; o.sBni = 2 * ns * kc; o.eB = eB; o.seBni = 1;
+  o.A = sA; o.sAm = OZ_PLANES * ns * kc; o.eA = eA; o.seAm = 1;
+  o.B = sB; o.sBni = OZ_PLANES * ns * kc; o.eB = eB; o.seBni = 1;
   o.C = dC; o.sCm = N; o.sCni = 1;
   o.alpha = make_double2(1.0, 0.0);
   zgemm_ozaki(o, ns, nullptr);

@@ -37,13 +37,13 @@ struct KernelStat {
 // Launch parameters picked by the autotuner (defaults = the best measured at nk = 64, nao = 26,
 // naux = 300): FP64 GEMM tile of the P0 and Sigma contractions (0: 64x64, 1: 128x64,
 // 2: 64x128), the number of aux indices per wave of the P0 prep kernel, and for split precision
-// the int8 GEMM block tile, the row-tile group of its L2-aware tile order and whether the P0
-// orbital kernel writes the int8 slices itself.
+// the row-tile group of the int8 GEMM's L2-aware tile order and whether the P0 orbital kernel
+// writes the int8 slices itself.
 struct Tuning {
   int p0_shape = 0, s_shape = 0, qpw = 4;
   int p0_group = 8, s_group = 8;  // split precision: row tiles per L2 group of the int8 GEMMs
-  int p0_oz_tile = 2, s_oz_tile = 2;  // split precision: int8 GEMM block tile (0: 32x64, 1: 64x64,
-                                      // 2: 32x64 software-pipelined)
+  int p0_oz_tile = 2, s_oz_tile = 2;  // int8 GEMM block tile: 2 (32x64 software-pipelined, the only
+                                      // one since the three-plane slices); kept in the tuning array
   int p0_fused = 1;  // split precision, one spin block: P0 orbital kernel writes int8 slices directly
   int orb_gemm = 0;  // orbital contractions as batched FP64 GEMMs (always so for nao > 48)
   int p0_kchunks = 4;  // split precision: k chunks per launch of the int8 P0 GEMM (0: all in one), so
@@ -56,7 +56,7 @@ struct Tuning {
   }
   bool from_array(const int* v) {
     if (v[0] < 0 || v[0] > 2 || v[1] < 0 || v[1] > 2 || (v[2] != 1 && v[2] != 2 && v[2] != 4 && v[2] != 8) ||
-        v[3] < 1 || v[4] < 1 || v[5] < 0 || v[5] > 2 || v[6] < 0 || v[6] > 2 || v[7] < 0 || v[7] > 1 || v[8] < 0 ||
+        v[3] < 1 || v[4] < 1 || v[5] != 2 || v[6] != 2 || v[7] < 0 || v[7] > 1 || v[8] < 0 ||
         v[8] > 1 || v[9] < 0)
       return false;
     p0_shape = v[0]; s_shape = v[1]; qpw = v[2]; p0_group = v[3]; s_group = v[4]; p0_oz_tile = v[5]; s_oz_tile = v[6];
@@ -331,9 +331,9 @@ private:
   void p0_gemm(int64_t q, int64_t tc, double2* Pt0, int shape);
   void sigma_gemm(const double2* Pt0, const double2* Vb, int64_t nkb, int64_t tc, int shape);
   void p0_slice(int64_t tc);
-  void p0_gemm_split(int64_t q, int64_t tc, double2* Pt0, int group, int tile, int kchunks);
+  void p0_gemm_split(int64_t q, int64_t tc, double2* Pt0, int group, int kchunks);
   void sigma_slice(const double2* Pt0, int64_t tc);
-  void sigma_gemm_split(const int8_t* Vsb, const int* eVsb, int64_t nkb, int64_t tc, int group, int tile);
+  void sigma_gemm_split(const int8_t* Vsb, const int* eVsb, int64_t nkb, int64_t tc, int group);
   void stage_p0(const double2* G, bool split, const std::vector<int>* qs = nullptr);
   void p0_one_q(int64_t q, const double2* G, bool split, double2* Pq);
   void gj_inverse(double2* X, int64_t nmat);

@@ -17,6 +17,7 @@
 #include "../../include/gwhip.h"
 #include "common.hpp"
 #include "gw_kernels.hpp"
+#include "zgemm_ozaki.hpp"
 
 namespace gwhip {
 
@@ -222,9 +223,9 @@ __global__ __launch_bounds__(256, NB < 3 ? 2 : 1) void z_prep_mfma_kernel(XcdGri
 // Split-precision P0 operand straight into int8 slices (one spin block, pseudo_ns == 1): the
 // wave that forms Z_Q(tau, k) = Gb V_Q Gq in registers also owns the whole slice row-chunk
 // (row (tau, Q), chunk k, nao^2 elements), so it finds the chunk exponent with a wave reduction
-// and writes the NS slices of Re and Im itself -- the same arithmetic as slice_rows_kernel, so
-// the slices are bit-identical to slicing the FP64 Z, without Z ever reaching HBM.
-// Slice layout: dst[((tz*NQ + Q)*nk + k)][plane][slice][kc], exps[(tz*NQ + Q)*nk + k].
+// and writes the NS slices of both planes (Re, Im) itself -- the same arithmetic as
+// slice_rows_kernel, so the slices are bit-identical to slicing the FP64 Z, without Z ever reaching
+// HBM.  Slice layout: dst[((tz*NQ + Q)*nk + k)][plane][slice][kc], exps[(tz*NQ + Q)*nk + k].
 // Two waves per SIMD for NB <= 2 (three spill and run 15 % slower at nao = 26).
 template <int NB, int NS>
 __global__ __launch_bounds__(256, NB < 3 ? 2 : 1) void z_prep_slice_kernel(XcdGrid xg, DevDims d, const double2* __restrict__ G,
@@ -335,8 +336,8 @@ __global__ __launch_bounds__(256, NB < 3 ? 2 : 1) void z_prep_slice_kernel(XcdGr
     if (mx > 0.0) (void)frexp(mx, &e);
     const int64_t task = ((int64_t)tz * d.NQ + Q) * d.nk + k;
     if (lane == 0) exps[task] = e;
-    const double sc = ldexp(1.0, -e);
-    int8_t* out = dst + task * (int64_t)2 * NS * kc;
+    const double sc = ldexp(OZ_FIRST, -e);
+    int8_t* out = dst + task * (int64_t)OZ_PLANES * NS * kc;
 #pragma unroll
     for (int bn = 0; bn < NB; ++bn) {
       const int n = 16 * bn + lrow;
@@ -347,22 +348,18 @@ __global__ __launch_bounds__(256, NB < 3 ? 2 : 1) void z_prep_slice_kernel(XcdGr
           const int t = 16 * bt + lk + 4 * r;
           if (n >= nao || t >= nao) continue;
           const int kk = t * nao + n;
-          double vr = zr[bn][bt][r] * sc, vi = zi[bn][bt][r] * sc;
+          int dr[NS], di[NS];
+          oz_element_digits<NS>(zr[bn][bt][r], zi[bn][bt][r], sc, dr, di);
 #pragma unroll
           for (int sl = 0; sl < NS; ++sl) {
-            vr *= 128.0;
-            vi *= 128.0;
-            const double qr = trunc(vr), qi2 = trunc(vi);
-            vr -= qr;
-            vi -= qi2;
-            out[(int64_t)(0 * NS + sl) * kc + kk] = (int8_t)(int)qr;
-            out[(int64_t)(1 * NS + sl) * kc + kk] = (int8_t)(int)qi2;
+            out[(int64_t)(0 * NS + sl) * kc + kk] = (int8_t)dr[sl];
+            out[(int64_t)(1 * NS + sl) * kc + kk] = (int8_t)di[sl];
           }
         }
     }
     // zero the chunk padding [nao^2, kc) of every plane-slice
     const int padn = kc - nao2;
-    for (int i = lane; i < 2 * NS * padn; i += 64) {
+    for (int i = lane; i < OZ_PLANES * NS * padn; i += 64) {
       const int ps = i / padn, kk = nao2 + (i - ps * padn);
       out[(int64_t)ps * kc + kk] = 0;
     }

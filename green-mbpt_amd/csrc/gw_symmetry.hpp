@@ -282,7 +282,7 @@ void gw_hip_kernel::solve_star_device(const double2* G_ibz, double2* Sigma_ibz, 
   ensure_orb_buffers();
   star_tables();
   if (split) {
-    const int64_t vs = ink * nao2 * 2 * split_ns_ * kc_s_;
+    const int64_t vs = ink * nao2 * OZ_PLANES * split_ns_ * kc_s_;
     if (vs != star_vs_bytes_) {
       if (star_Vs_) (void)hipFree(star_Vs_), bytes_ -= (size_t)star_vs_bytes_, star_Vs_ = nullptr;
       if (star_eVs_) (void)hipFree(star_eVs_), bytes_ -= (size_t)star_evs_elems_ * sizeof(int), star_eVs_ = nullptr;
@@ -321,7 +321,7 @@ void gw_hip_kernel::solve_star_device(const double2* G_ibz, double2* Sigma_ibz, 
       const double2* Pq = star_rotate(iq, q);
       launch_gather_rows(Vq_neg(q), star_V_, rows, NQ * nao2, ink, stream_);
       if (split) {
-        const int64_t len = nao2 * 2 * split_ns_ * kc_s_;  // a multiple of 128 bytes (kc_s is of 64)
+        const int64_t len = nao2 * OZ_PLANES * split_ns_ * kc_s_;  // a multiple of 64 bytes (kc_s is of 64)
         launch_gather_rows(reinterpret_cast<const uint4*>(Vs_neg(q)), reinterpret_cast<uint4*>(star_Vs_), rows,
                            len / 16, ink, stream_);
         launch_gather_rows(eVs_neg(q), star_eVs_, rows, nao2, ink, stream_);

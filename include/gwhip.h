@@ -119,9 +119,10 @@ int gwhip_upload_V(gwhip_handle* h, const double* V_packed);
  * No reference counterpart: synthetic-input path for benchmarks. */
 int gwhip_generate_V(gwhip_handle* h, uint64_t seed, double sigma);
 
-/* Number of int8 slices per operand in GWHIP_PREC_SPLIT (3, 4 or 5; default 5): the P0 and
- * Sigma contractions keep 7*slices bits of each operand relative to its row/column maximum
- * per K chunk and accumulate slice products exactly (int32), then in FP64.
+/* Number of int8 slices per operand in the split-precision modes (3, 4 or 5; default 5): the P0
+ * and Sigma contractions keep about 8 bits per slice of each operand relative to its row/column
+ * maximum per K chunk (balanced radix-254 digits) and accumulate slice products exactly (int32),
+ * then in FP64.
  * No reference counterpart (the reference's P_sp / Sigma_sp use complex<float>). */
 int gwhip_set_split_slices(gwhip_handle* h, int slices);
 
@@ -139,14 +140,15 @@ int gwhip_eval_sigma_device(gwhip_handle* h, const void* G_dev, void* Sigma_dev,
  * Time the candidate launch parameters of this system on its own operands (G in device memory)
  * and keep the fastest: the FP64 GEMM tiles of the P0 and Sigma contractions (0: 64x64,
  * 1: 128x64, 2: 64x128), the aux indices per wave of the P0 orbital kernel (1, 2, 4, 8) and, for
- * GWHIP_PREC_SPLIT, the block tile of the int8 GEMMs and the number of row tiles that sweep the
- * column tiles together in their L2-aware tile order (1 .. 64).  Results do not change, only speed.
+ * the split-precision modes, the number of row tiles that sweep the column tiles together in the
+ * int8 GEMMs' L2-aware tile order (1 .. 64) and the k chunks per launch of the int8 P0 GEMM.
+ * Results do not change, only speed.
  * No reference counterpart. */
 int gwhip_autotune(gwhip_handle* h, const void* G_dev, int precision);
 /* The tuning as an array of GWHIP_TUNING_COUNT ints: [0] FP64 P0 GEMM tile, [1] FP64 Sigma GEMM
  * tile, [2] aux indices per wave, [3] int8 P0 GEMM row-tile group, [4] int8 Sigma GEMM row-tile
- * group, [5] int8 P0 GEMM block tile (0: 32x64, 1: 64x64, 2: 32x64 software-pipelined), [6] int8
- * Sigma GEMM block tile,
+ * group, [5] int8 P0 GEMM block tile and [6] int8 Sigma GEMM block tile (both 2: the 32x64
+ * software-pipelined tile, the only one),
  * [7] split P0 orbital kernel writes the int8 slices itself (1) or via a separate pass (0),
  * [8] orbital-size products (Z = Gb V Gq of P0, V G R of Sigma) as batched FP64 GEMMs (1) or the
  * register-chained MFMA kernels (0); always 1 in effect for nao > 48, [9] int8 P0 GEMM: k chunks

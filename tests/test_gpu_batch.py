@@ -116,7 +116,7 @@ def test_recapture_rules_and_autotune():
             t = s.tuning()
             assert t["p0_group"] in (1, 2, 4, 8, 16, 32, 64) and t["sigma_group"] in (1, 2, 4, 8, 16, 32, 64)
         b.systems[1].set_tuning(p0_tile=1, sigma_tile=2, aux_per_wave=1, p0_group=4, sigma_group=8,
-                                p0_split_tile=1, sigma_split_tile=1)
+                                p0_split_tile=2, sigma_split_tile=2)
         b.eval_device(gp, [s.data_ptr() for s in S1])
         assert b.stats()["captures"] == 3
         for a, s in zip(first, S1):

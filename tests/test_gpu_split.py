@@ -1,9 +1,9 @@
-"""Split-precision path (int8 slices, exact int32 accumulation, FP64 accumulation across
-chunks) -- needs an MI355X.
+"""Split-precision path (int8 balanced-digit slices, exact int32 accumulation, FP64 accumulation
+across chunks) -- needs an MI355X.
 
-Bars: the slice GEMM reproduces exactly representable products bit for bit and random
-products within the slicing bound; the split Sigma is within 1e-8 relative (max-abs
-difference over max |Sigma|) of the FP64 oracle.
+Bars: the slice GEMM puts every product in its place (identity probe) and stays within the slicing
+bound on random products; the split Sigma is within 1e-8 relative (max-abs difference over max
+|Sigma|) of the FP64 oracle.
 """
 import numpy as np
 import pytest
@@ -18,13 +18,15 @@ TOL_SPLIT = 1e-8
 
 
 def test_split_gemm_identity_probe():
-    """A = I and small integers in B are exact in every slice count: checks the int8 MFMA layout."""
+    """A = I times distinct entries of B returns B to the slicing accuracy (radix-254 digits do not
+    represent every dyadic number exactly): checks the int8 MFMA layout, where a misplaced element
+    would be off by O(1)."""
     n = 16
     A = np.eye(n, dtype=complex)
     B = (np.arange(n)[:, None] * 100 + np.arange(n)[None, :]).astype(complex) * (1 - 2j)
     C = gl.debug_zgemm_split(A, B, 4)
-    if not np.array_equal(C, B):
-        bad = np.argwhere(C != B)
+    bad = np.argwhere(np.abs(C - B) > 1e-6 * np.abs(B).max())
+    if len(bad):
         raise AssertionError(f"layout mismatch at {len(bad)} entries; first: C{tuple(bad[0])}="
                              f"{C[tuple(bad[0])]} expected {B[tuple(bad[0])]}")
 
@@ -38,8 +40,9 @@ def test_split_gemm_random(shape, ns):
     B = rng.standard_normal((K, N)) + 1j * rng.standard_normal((K, N))
     C = gl.debug_zgemm_split(A, B, ns)
     ref = A @ B
-    # per-element bound: each operand keeps 7*ns bits relative to its row / column maximum
-    bound = 8.0 * K * 2.0 ** (-7 * ns) * np.abs(A).max() * np.abs(B).max()
+    # per-element bound: each operand keeps ns balanced radix-254 digits relative to 2^e >= its
+    # row / column maximum (truncation and the dropped pairs s + t >= ns, two real products per part)
+    bound = 8.0 * (ns + 1) * K * 254.0 ** (-ns) * np.abs(A).max() * np.abs(B).max()
     assert np.abs(C - ref).max() < bound
 
 
@@ -88,18 +91,22 @@ def test_split_small_cases(case):
     assert rel_err(S_split, S_ref) < TOL_SPLIT
 
 
-def test_four_slices_are_looser_but_bounded():
-    """4 slices keep 28 bits per operand: accurate to ~1e-7 here, so 5 is the default."""
+@pytest.mark.parametrize("ns", [3, 4])
+def test_fewer_slices_are_looser_but_bounded(ns):
+    """Three and four balanced slices keep about 24 and 32 bits per operand: looser than five but
+    bounded; the measured error is printed."""
     c = CONFIG0
     G, V, T, d = synthetic(c["nk"], c["nao"], c["NQ"], beta=c["beta"], lam=c["lam"], mesh=c["mesh"])
-    S_split, _, S_ref = _split_vs_oracle(G, V, T, d, ns=4)
-    assert 1e-10 < rel_err(S_split, S_ref) < 1e-6
+    S_split, _, S_ref = _split_vs_oracle(G, V, T, d, ns=ns)
+    err = rel_err(S_split, S_ref)
+    print(f"split ns={ns}: rel err vs oracle {err:.3e}", flush=True)
+    assert 1e-13 < err < (1e-7 if ns == 4 else 1e-4)
 
 
 @pytest.mark.parametrize("p0_group,s_group", [(1, 16), (12, 1), (64, 64)])
 def test_split_tiles_and_orders_give_identical_sigma(p0_group, s_group):
-    """Both int8 block tiles, any L2 tile order and the fused or separate slicing of Z produce
-    bit-identical Sigma (exact int32 sums, identical slices)."""
+    """Any L2 tile order and the fused or separate slicing of Z produce bit-identical Sigma (exact int32
+    sums, identical slices)."""
     c = CONFIG0
     G, V, T, d = synthetic(c["nk"], c["nao"], c["NQ"], beta=c["beta"], lam=c["lam"], mesh=c["mesh"])
     k = _kernel(d, T)
@@ -108,11 +115,9 @@ def test_split_tiles_and_orders_give_identical_sigma(p0_group, s_group):
         k.upload_V(V)
         ref = k.solve(G, gl.PREC_SPLIT)
         out = []
-        for tile in (0, 1, 2):
-            for fused in (0, 1):
-                k.set_tuning(p0_split_tile=tile, sigma_split_tile=tile, p0_group=p0_group, sigma_group=s_group,
-                             p0_fused_slices=fused)
-                out.append(k.solve(G, gl.PREC_SPLIT))
+        for fused in (0, 1):
+            k.set_tuning(p0_group=p0_group, sigma_group=s_group, p0_fused_slices=fused)
+            out.append(k.solve(G, gl.PREC_SPLIT))
     finally:
         k.close()
     for o in out:
