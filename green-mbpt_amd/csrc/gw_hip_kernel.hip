@@ -107,7 +107,7 @@ gw_hip_kernel::gw_hip_kernel(const Dims& d, int device, hipStream_t shared_strea
   size_t free_b = 0, total_b = 0;
   GWHIP_CHECK(hipMemGetInfo(&free_b, &total_b));
   const double budget = 0.7 * (double)free_b;
-  const double per_q_dyson = 3.0 * d_.nw_b * NQ2 * 16.0;
+  const double per_q_dyson = 1.0 * d_.nw_b * NQ2 * 16.0;
   const double per_t_scr = (double)nk * NQ * nao2 * 16.0;
   const double dyson_share = std::min(0.3 * budget, per_q_dyson * nk);
   qchunk_ = std::max<int64_t>(1, std::min<int64_t>(nk, (int64_t)(dyson_share / per_q_dyson)));
@@ -116,9 +116,8 @@ gw_hip_kernel::gw_hip_kernel(const Dims& d, int device, hipStream_t shared_strea
   tchunk_sigma_ = std::min<int64_t>(d_.nts / 2, tmax);  // R is formed on the half tau grid
   tchunk_p0_ = std::min<int64_t>(d_.nts / 2, tmax);
   Pw_ = dalloc<double2>((size_t)qchunk_ * d_.nw_b * NQ2);
-  X_ = dalloc<double2>((size_t)qchunk_ * d_.nw_b * NQ2);
-  Y_ = dalloc<double2>((size_t)qchunk_ * d_.nw_b * NQ2);
-  scratch_ = dalloc<double2>((size_t)std::max(tchunk_sigma_, tchunk_p0_) * nk * NQ * nao2);
+  scratch_elems_ = std::max(tchunk_sigma_, tchunk_p0_) * nk * NQ * nao2;
+  scratch_ = dalloc<double2>((size_t)scratch_elems_);
 }
 
 gw_hip_kernel::~gw_hip_kernel() {
@@ -131,7 +130,7 @@ gw_hip_kernel::~gw_hip_kernel() {
                   (void*)orb_G_, (void*)orb_part_, (void*)orb_O_, (void*)epi_part_})
     if (p) (void)hipFree(p);
   if (own_ws_)
-    for (void* p : {(void*)Pw_, (void*)X_, (void*)Y_, (void*)scratch_})
+    for (void* p : {(void*)Pw_, (void*)scratch_})
       if (p) (void)hipFree(p);
   free_slices();
   free_vjp_buffers();
@@ -259,7 +258,7 @@ void gw_hip_kernel::set_split_slices(int ns) {
 }
 
 size_t gw_hip_kernel::scratch_per_t() const { return (size_t)d_.nk * d_.NQ * d_.nao * d_.nao; }
-size_t gw_hip_kernel::dyson_per_q() const { return (size_t)3 * d_.nw_b * d_.NQ * d_.NQ; }
+size_t gw_hip_kernel::dyson_per_q() const { return (size_t)d_.nw_b * d_.NQ * d_.NQ; }
 size_t gw_hip_kernel::slice_bytes_per_t() const {
   const size_t kcp = (size_t)ceil_div(d_.nao * d_.nao, 64) * 64, kcs = (size_t)ceil_div(d_.NQ, 64) * 64;
   return std::max((size_t)d_.NQ * d_.nk * OZ_PLANES * split_ns_ * kcp, (size_t)d_.NQ * OZ_PLANES * split_ns_ * kcs);
@@ -278,11 +277,9 @@ void gw_hip_kernel::bind_workspace(const Workspace& ws) {
   tchunk_sigma_ = std::min<int64_t>(d_.nts / 2, tmax);  // R is formed on the half tau grid
   tchunk_p0_ = std::min<int64_t>(d_.nts / 2, tmax);
   qchunk_ = std::min<int64_t>(d_.nk, qmax);
-  const size_t nmat = (size_t)qchunk_ * d_.nw_b * d_.NQ * d_.NQ;
   scratch_ = ws.scratch;
+  scratch_elems_ = (int64_t)ws.scratch_elems;
   Pw_ = ws.dyson;
-  X_ = Pw_ + nmat;
-  Y_ = X_ + nmat;
   Zs_ = Ps_ = ws.slices;
   eZs_ = ePs_ = ws.exps;
   ++generation_;
@@ -677,8 +674,73 @@ void gw_hip_kernel::check_dyson_flag() {
                        "is not positive (P0 is not the polarisation of a physical G)");
 }
 
+// Block Gauss-Jordan inverse: per block column J (width nb <= 64) the column panel C and the diagonal
+// block are copied out, the block is inverted in place (gj_inverse_kernel, the pivot guard included),
+// then three batched GEMMs do the update -- R = D A_J*, A -= C R, A_*J = -C D -- and the pivot rows
+// are set to R and D.  Each step streams the matrix through the MFMA GEMM once: NQ / nb passes
+// instead of the NQ / 16 of gj_inverse_kernel's register-panel sweeps.
+void gw_hip_kernel::block_inverse(double2* X, int64_t nmat) {
+  const int64_t n = d_.NQ, n2 = n * n;
+  const int nb = (int)std::min<int64_t>(64, n);
+  const int64_t per = 2 * n * nb + (int64_t)nb * nb;
+  const int64_t mb_max = std::min<int64_t>(65535, scratch_elems_ / per);
+  if (mb_max < 1) throw status_error(GWHIP_ERR_UNSUPPORTED, "scratch too small for the Dyson inverse panels");
+  const size_t gj_lds = (size_t)(GJ_NB * GJ_NB + nb * GJ_NB) * sizeof(double2);
+  for (int64_t m0 = 0; m0 < nmat; m0 += mb_max) {
+    const int64_t mb = std::min(mb_max, nmat - m0);
+    double2* A = X + m0 * n2;
+    double2* C = scratch_;                 // [mb][n][jb]     (allocation stride n nb)
+    double2* R = C + mb * n * nb;          // [mb][jb][n]     (stride nb n)
+    double2* D = R + mb * (int64_t)nb * n; // [mb][jb][jb]    (stride nb^2)
+    const int64_t sC = n * nb, sR = (int64_t)nb * n, sD = (int64_t)nb * nb;
+    for (int J0 = 0; J0 < n; J0 += nb) {
+      const int jb = (int)std::min<int64_t>(nb, n - J0);
+      hipLaunchKernelGGL(bgj_panel_kernel, dim3((unsigned)ceil_div(n * jb, 256), (unsigned)mb), dim3(256), 0, stream_,
+                         A, (int)n, n2, J0, jb, C, sC, D, sD);
+      GWHIP_CHECK(hipGetLastError());
+      hipLaunchKernelGGL(gj_inverse_kernel, dim3((unsigned)mb), dim3(256), gj_lds, stream_, D, jb, sD, GJ_NB,
+                         dyson_flag_);
+      GWHIP_CHECK(hipGetLastError());
+      {  // R = D A[J0 .. J0+jb)[:]
+        ZGemmArgs a{};
+        a.M = jb; a.N = (int)n; a.K = jb; a.batch = (int)mb; a.kin = jb; a.nin = (int)n;
+        a.A = D; a.sAb = sD; a.sAm = jb; a.sAko = 0; a.sAki = 1;
+        a.B = A + (int64_t)J0 * n; a.sBb = n2; a.sBko = 0; a.sBki = n; a.sBno = 0; a.sBni = 1;
+        a.C = R; a.sCb = sR; a.sCm = n; a.sCno = 0; a.sCni = 1;
+        a.alpha = make_double2(1.0, 0.0);
+        zgemm_f64(a, true, false, stream_);
+      }
+      {  // A -= C R
+        ZGemmArgs a{};
+        a.M = (int)n; a.N = (int)n; a.K = jb; a.batch = (int)mb; a.kin = jb; a.nin = (int)n;
+        a.A = C; a.sAb = sC; a.sAm = jb; a.sAko = 0; a.sAki = 1;
+        a.B = R; a.sBb = sR; a.sBko = 0; a.sBki = n; a.sBno = 0; a.sBni = 1;
+        a.C = A; a.sCb = n2; a.sCm = n; a.sCno = 0; a.sCni = 1;
+        a.alpha = make_double2(-1.0, 0.0);
+        a.accumulate = 1;
+        zgemm_f64(a, true, false, stream_);
+      }
+      {  // A[:, J0 .. J0+jb) = -C D
+        ZGemmArgs a{};
+        a.M = (int)n; a.N = jb; a.K = jb; a.batch = (int)mb; a.kin = jb; a.nin = jb;
+        a.A = C; a.sAb = sC; a.sAm = jb; a.sAko = 0; a.sAki = 1;
+        a.B = D; a.sBb = sD; a.sBko = 0; a.sBki = jb; a.sBno = 0; a.sBni = 1;
+        a.C = A + J0; a.sCb = n2; a.sCm = n; a.sCno = 0; a.sCni = 1;
+        a.alpha = make_double2(-1.0, 0.0);
+        zgemm_f64(a, true, false, stream_);
+      }
+      hipLaunchKernelGGL(bgj_rows_kernel, dim3((unsigned)ceil_div((int64_t)jb * n, 256), (unsigned)mb), dim3(256), 0,
+                         stream_, A, (int)n, n2, J0, jb, R, sR, D, sD);
+      GWHIP_CHECK(hipGetLastError());
+    }
+  }
+}
+
 // P-tilde of the first nq rows of P_; q0_row[i] is the AqQ row of row i (-1: none), read when the
-// q -> 0 data are being collected.
+// q -> 0 data are being collected.  Per q chunk, in one buffer: P0(i Omega) = T_bt' P0(tau), then
+// A = I - P0(i Omega), its inverse, and P-tilde(i Omega) = (A^-1 + A^-H)/2 - I -- the reference's
+// Hermitised (I - P0)^-1 times P0 (src/gw_cpu_kernel.cpp:241-252), since (I - P0)^-1 P0 =
+// (I - P0)^-1 - I -- then P-tilde(tau) = T_tb P-tilde(i Omega).
 void gw_hip_kernel::stage_dyson(int64_t nq, const int* q0_row) {
   const int64_t NQ = d_.NQ, NQ2 = NQ * NQ, half = d_.nts / 2, nwb = d_.nw_b, nts = d_.nts;
 
@@ -700,24 +762,16 @@ void gw_hip_kernel::stage_dyson(int64_t nq, const int* q0_row) {
     }
     mark(e1);
     const int64_t nmat = qc * nwb;
-    hipLaunchKernelGGL(identity_minus_kernel, dim3((unsigned)ceil_div(nmat * NQ2, 256)), dim3(256), 0, stream_, X_, Pw_,
+    if (nmat > 65535) throw status_error(GWHIP_ERR_UNSUPPORTED, "more than 65535 Dyson matrices in one q chunk");
+    hipLaunchKernelGGL(identity_minus_kernel, dim3((unsigned)ceil_div(nmat * NQ2, 256)), dim3(256), 0, stream_, Pw_, Pw_,
                        (int)NQ, nmat);
     GWHIP_CHECK(hipGetLastError());
-    timed(K_DYSON_INV, [&] { gj_inverse(X_, nmat); });
-    hipLaunchKernelGGL(hermitize_kernel, dim3((unsigned)ceil_div(NQ2, 256), (unsigned)nmat), dim3(256), 0, stream_, X_,
-                       (int)NQ, NQ2);
+    timed(K_DYSON_INV, [&] { block_inverse(Pw_, nmat); });
+    hipLaunchKernelGGL(herm_minus_identity_kernel, dim3((unsigned)ceil_div(NQ2, 256), (unsigned)nmat), dim3(256), 0,
+                       stream_, Pw_, (int)NQ, NQ2);
     GWHIP_CHECK(hipGetLastError());
-    {  // P-tilde(i Omega) = X P0(i Omega)
-      ZGemmArgs a{};
-      a.M = (int)NQ; a.N = (int)NQ; a.K = (int)NQ; a.batch = (int)nmat; a.kin = (int)NQ; a.nin = (int)NQ;
-      a.A = X_; a.sAb = NQ2; a.sAm = NQ; a.sAko = 0; a.sAki = 1;
-      a.B = Pw_; a.sBb = NQ2; a.sBko = 0; a.sBki = NQ; a.sBno = 0; a.sBni = 1;
-      a.C = Y_; a.sCb = NQ2; a.sCm = NQ; a.sCno = 0; a.sCni = 1;
-      a.alpha = make_double2(1.0, 0.0);
-      zgemm_f64(a, true, false, stream_);
-    }
     if (q0_eps_) {  // aux_to_PW_00 for the irreducible q of this chunk
-      hipLaunchKernelGGL(q0_eps_kernel, dim3((unsigned)nwb, (unsigned)qc), dim3(256), 0, stream_, Y_, q0_A_, q0_row,
+      hipLaunchKernelGGL(q0_eps_kernel, dim3((unsigned)nwb, (unsigned)qc), dim3(256), 0, stream_, Pw_, q0_A_, q0_row,
                          (int)q0, (int)nwb, (int)NQ, (int)q0_inq_, q0_eps_dev_);
       GWHIP_CHECK(hipGetLastError());
     }
@@ -726,7 +780,7 @@ void gw_hip_kernel::stage_dyson(int64_t nq, const int* q0_row) {
       ZGemmArgs a{};
       a.M = (int)nts; a.N = (int)NQ2; a.K = (int)nwb; a.batch = (int)qc; a.kin = (int)nwb; a.nin = (int)NQ2;
       a.A = Ttb_; a.sAb = 0; a.sAm = nwb; a.sAko = 0; a.sAki = 1;
-      a.B = Y_; a.sBb = nwb * NQ2; a.sBko = 0; a.sBki = NQ2; a.sBno = 0; a.sBni = 1;
+      a.B = Pw_; a.sBb = nwb * NQ2; a.sBko = 0; a.sBki = NQ2; a.sBno = 0; a.sBni = 1;
       a.C = P_ + q0 * nts * NQ2; a.sCb = nts * NQ2; a.sCm = NQ2; a.sCno = 0; a.sCni = 1;
       a.alpha = make_double2(1.0, 0.0);
       zgemm_f64(a, true, false, stream_);

@@ -70,7 +70,7 @@ struct Tuning {
 struct Workspace {
   double2* scratch = nullptr;  // Z or R, [tc][nk][NQ][nao^2]
   size_t scratch_elems = 0;
-  double2* dyson = nullptr;    // Pw, X, Y, each [qc][nw_b][NQ^2]
+  double2* dyson = nullptr;    // Pw [qc][nw_b][NQ^2]
   size_t dyson_elems = 0;
   int8_t* slices = nullptr;    // int8 slices of Z (P0 stage) or P-tilde (Sigma stage)
   size_t slice_bytes = 0;
@@ -211,10 +211,9 @@ private:
   // device buffers
   double2* Vq_ = nullptr;      // [nq][nk][NQ][nao^2]   V^Q_ab(k, k+q): the one resident copy of V
   double2* P_ = nullptr;       // [nq][nts][NQ^2]
-  double2* Pw_ = nullptr;      // [qc][nw_b][NQ^2]
-  double2* X_ = nullptr;       // [qc][nw_b][NQ^2]
-  double2* Y_ = nullptr;       // [qc][nw_b][NQ^2]
-  double2* scratch_ = nullptr; // Z or R: [tc][nk][NQ][nao^2]
+  double2* Pw_ = nullptr;      // [qc][nw_b][NQ^2]: P0, then I - P0, its inverse and P-tilde on the bosonic grid
+  double2* scratch_ = nullptr; // Z or R: [tc][nk][NQ][nao^2]; the Dyson stage's panels in between
+  int64_t scratch_elems_ = 0;
   double2* Tbt_fold_ = nullptr;  // [nw_b][nts/2]
   double2* Ttb_ = nullptr;       // [nts][nw_b]
   double2* G_dev_ = nullptr;   // host-path staging
@@ -337,6 +336,9 @@ private:
   void stage_p0(const double2* G, bool split, const std::vector<int>* qs = nullptr);
   void p0_one_q(int64_t q, const double2* G, bool split, double2* Pq);
   void gj_inverse(double2* X, int64_t nmat);
+  // in-place inverse of nmat Hermitian positive-definite NQ x NQ matrices by block Gauss-Jordan steps
+  // whose updates are batched FP64 MFMA GEMMs (panels in scratch_)
+  void block_inverse(double2* X, int64_t nmat);
   void check_dyson_flag();  // reads dyson_flag_ (synchronous) and throws when it is set
   void stage_dyson(int64_t nq, const int* q0_row);
   void stage_sigma(const double2* G, double2* Sigma, bool split);

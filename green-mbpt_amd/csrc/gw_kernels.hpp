@@ -91,6 +91,46 @@ __global__ void identity_minus_kernel(double2* X, const double2* Pw, int N, int6
   X[gid] = make_double2((i == j ? 1.0 : 0.0) - p.x, -p.y);
 }
 
+// (X + X^H)/2 - I, in place, for every matrix of the batch: P-tilde(i Omega) from the inverse X of
+// I - P0(i Omega), since (I - P0)^-1 P0 = (I - P0)^-1 - I.
+__global__ void herm_minus_identity_kernel(double2* X, int N, int64_t stride) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)N * N) return;
+  const int i = (int)(e / N), j = (int)(e - (e / N) * N);
+  if (i > j) return;
+  double2* M = X + (int64_t)blockIdx.y * stride;
+  const double2 u = M[(int64_t)i * N + j], l = M[(int64_t)j * N + i];
+  const double2 h = make_double2(0.5 * (u.x + l.x) - (i == j ? 1.0 : 0.0), 0.5 * (u.y - l.y));
+  M[(int64_t)i * N + j] = h;
+  M[(int64_t)j * N + i] = make_double2(h.x, i == j ? 0.0 : -h.y);
+}
+
+// Block Gauss-Jordan step J (columns [J0, J0 + jb)) of the batched inverse, data movement only:
+// the column panel C[m][i][l] = A[m][i][J0 + l] (all N rows) and the diagonal block
+// D[m][l][l'] = A[m][J0 + l][J0 + l'] (row stride jb), both copied out before the GEMMs update A.
+__global__ void bgj_panel_kernel(const double2* __restrict__ A, int N, int64_t sA, int J0, int jb,
+                                 double2* __restrict__ C, int64_t sC, double2* __restrict__ D, int64_t sD) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)N * jb) return;
+  const int i = (int)(e / jb), l = (int)(e - (e / jb) * jb);
+  const int64_t m = blockIdx.y;
+  const double2 v = A[m * sA + (int64_t)i * N + J0 + l];
+  C[m * sC + (int64_t)i * jb + l] = v;
+  if (i >= J0 && i < J0 + jb) D[m * sD + (int64_t)(i - J0) * jb + l] = v;
+}
+
+// The pivot rows of step J after the GEMMs: A[J0 + l][j] = R[l][j] outside the pivot block and
+// A[J0 + l][J0 + l'] = D[l][l'] (the inverted diagonal block) inside it.
+__global__ void bgj_rows_kernel(double2* __restrict__ A, int N, int64_t sA, int J0, int jb,
+                                const double2* __restrict__ R, int64_t sR, const double2* __restrict__ D, int64_t sD) {
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)jb * N) return;
+  const int l = (int)(e / N), j = (int)(e - (e / N) * N);
+  const int64_t m = blockIdx.y;
+  const bool in_block = j >= J0 && j < J0 + jb;
+  A[m * sA + (int64_t)(J0 + l) * N + j] = in_block ? D[m * sD + (int64_t)l * jb + (j - J0)] : R[m * sR + (int64_t)l * N + j];
+}
+
 // ---------------------------------------------------------------------------------------
 // Batched in-place inverse of Hermitian positive-definite N x N matrices by blocked
 // Gauss-Jordan sweeps without pivoting (the reference uses Eigen's LDLT on I - P0,
