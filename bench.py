@@ -156,6 +156,12 @@ def device_gm_energy(G_d, S_d, T, nk, nao, ns):
         d.close()
 
 
+def host_cores():
+    """Threads for the CPU baseline: the host share of one GPU (OMP_NUM_THREADS where the
+    environment sets it), never above 16."""
+    return min(16, int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or os.cpu_count() or 1)
+
+
 def cpu_baseline(w, T, budget_s=20.0):
     """Time the oracle's per-unit contractions on the host and scale to one Sigma evaluation."""
     import numpy as np
@@ -165,7 +171,7 @@ def cpu_baseline(w, T, budget_s=20.0):
 
     nao, NQ, nk, nts, nwb = w["nao"], w["NQ"], w["nk"], T.nts, T.nw_b
     rng = np.random.default_rng(0)
-    cores = os.cpu_count() or 1
+    cores = host_cores()
     with threadpool_limits(limits=cores):
         v = (rng.standard_normal((NQ, nao, nao)) + 1j * rng.standard_normal((NQ, nao, nao))) * 0.1
         Gb = rng.standard_normal((nao, nao)) + 1j * rng.standard_normal((nao, nao))
@@ -197,6 +203,42 @@ def cpu_baseline(w, T, budget_s=20.0):
                         f"Sigma evaluation (nq*nk*nts/2, nq*nk*nts, nq*nw_b); OpenBLAS zgemm, {cores} threads"))
 
 
+def bench_capacity(args):
+    """Capacity check: the split-precision evaluation at nk = 64, nao = 64, naux = 300 (V of 80 GB held
+    once, its two int8 slice stores, P-tilde and the workspace on one MI355X); prints the device
+    memory the handle holds, what is left free, and the time of one evaluation."""
+    import numpy as np
+    import torch
+
+    from gwhip import inputs, irgrid
+    from gwhip import lib as gl
+
+    nk, mesh, nao, NQ, beta = 64, (4, 4, 4), 64, 300, 100.0
+    T = irgrid.Transformer.from_grid(irgrid.load_grid(1e4), beta)
+    G = inputs.generate_G(0, nk, nao, 1, T.tau, beta)
+    k = gl.GwHipKernel(nk, nao, nao, 1, NQ, T.nts, T.nw_b, beta, mesh=mesh)
+    try:
+        k.set_grid(T.T_bt, T.T_tb)
+        k.generate_V(0, 0.1)
+        dev = torch.device("cuda:0")
+        G_d = torch.from_numpy(G.view(np.float64)).to(dev)
+        S_d = torch.empty_like(G_d)
+        torch.cuda.synchronize()
+        print("[bench] capacity: system built", file=sys.stderr, flush=True)
+        t0 = time.perf_counter()
+        k.solve_device(G_d.data_ptr(), S_d.data_ptr(), gl.PREC_SPLIT)
+        torch.cuda.synchronize()
+        ms = (time.perf_counter() - t0) * 1e3
+        free_b, total_b = torch.cuda.mem_get_info()
+        print(json.dumps({"workload": "GW Sigma, 4x4x4 k mesh, nao 64, naux 300, ns 1, split precision",
+                          "ms_first_eval": ms, "handle_device_gb": k.device_bytes() / 1e9,
+                          "free_gb_after": free_b / 1e9, "total_gb": total_b / 1e9,
+                          "kernel_ms": {k_: v_[1] for k_, v_ in k.kernel_stats().items()},
+                          "sigma_finite": bool(torch.isfinite(S_d).all().item())}), flush=True)
+    finally:
+        k.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -218,6 +260,8 @@ def main():
                     help="time the Hartree-Fock Fock build on the same nk=64 nao=26 naux=300 integrals instead")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch parameters (the autotuner's choice for this workload)")
+    ap.add_argument("--capacity", action="store_true",
+                    help="one split-precision evaluation at nk=64, nao=64, naux=300 (device memory held, time)")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a plain 2 GiB device-to-device copy (reference point for HBM byte counters)")
     args = ap.parse_args()
@@ -238,6 +282,8 @@ def main():
         return bench_star(args)
     if args.dyson:
         return bench_dyson(args)
+    if args.capacity:
+        return bench_capacity(args)
 
     import numpy as np
     import torch
@@ -409,7 +455,7 @@ def hf_cpu_baseline(w, budget_s=15.0):
 
     nao, NQ, nk, ns = w["nao"], w["NQ"], w["nk"], w["ns"]
     rng = np.random.default_rng(0)
-    cores = os.cpu_count() or 1
+    cores = host_cores()
     with threadpool_limits(limits=cores):
         v = (rng.standard_normal((NQ, nao, nao)) + 1j * rng.standard_normal((NQ, nao, nao))) * 0.1
         dm = rng.standard_normal((nao, nao)) + 1j * rng.standard_normal((nao, nao))
@@ -602,10 +648,13 @@ def bench_dyson(args):
     ms_mu = (time.perf_counter() - t0) * 1e3
     d.close()
     # CPU: the oracle's batched inverse (numpy / LAPACK) on the same shapes
+    from threadpoolctl import threadpool_limits
+
     from oracle import dyson as od
-    t0 = time.perf_counter()
-    od.compute_G(H, S, S1, St, 0.0, T.T_ft, T.T_tf, wn)
-    ms_cpu = (time.perf_counter() - t0) * 1e3
+    with threadpool_limits(limits=host_cores()):
+        t0 = time.perf_counter()
+        od.compute_G(H, S, S1, St, 0.0, T.T_ft, T.T_tf, wn)
+        ms_cpu = (time.perf_counter() - t0) * 1e3
     nmat = nw * ns * nk
     flops = 8.0 * n ** 3 * nmat                     # Gauss-Jordan inverse: n^3 complex multiply-adds
     print(json.dumps({
@@ -617,7 +666,7 @@ def bench_dyson(args):
                    "parallelism": "single GPU"},
         "inverse_flops_per_step": flops, "inverse_matrices": nmat,
         "find_mu": {"ms": ms_mu, "mu": mu, "nel": nel_found, "target": nel},
-        "cpu_baseline": {"value": 1e3 / ms_cpu, "unit": "Dyson steps/s", "cores": os.cpu_count() or 1, "kind": "port",
+        "cpu_baseline": {"value": 1e3 / ms_cpu, "unit": "Dyson steps/s", "cores": host_cores(), "kind": "port",
                          "sample": "the whole step with the oracle (numpy batched inverse, LAPACK)"},
     }), flush=True)
 

@@ -107,7 +107,7 @@ gw_hip_kernel::gw_hip_kernel(const Dims& d, int device, hipStream_t shared_strea
   size_t free_b = 0, total_b = 0;
   GWHIP_CHECK(hipMemGetInfo(&free_b, &total_b));
   const double budget = 0.7 * (double)free_b;
-  const double per_q_dyson = 1.0 * d_.nw_b * NQ2 * 16.0;
+  const double per_q_dyson = 16.0 * (double)dyson_per_q();
   const double per_t_scr = (double)nk * NQ * nao2 * 16.0;
   const double dyson_share = std::min(0.3 * budget, per_q_dyson * nk);
   qchunk_ = std::max<int64_t>(1, std::min<int64_t>(nk, (int64_t)(dyson_share / per_q_dyson)));
@@ -115,7 +115,7 @@ gw_hip_kernel::gw_hip_kernel(const Dims& d, int device, hipStream_t shared_strea
   int64_t tmax = std::max<int64_t>(1, (int64_t)(scr_budget / per_t_scr));
   tchunk_sigma_ = std::min<int64_t>(d_.nts / 2, tmax);  // R is formed on the half tau grid
   tchunk_p0_ = std::min<int64_t>(d_.nts / 2, tmax);
-  Pw_ = dalloc<double2>((size_t)qchunk_ * d_.nw_b * NQ2);
+  Pw_ = dalloc<double2>((size_t)qchunk_ * dyson_per_q());
   scratch_elems_ = std::max(tchunk_sigma_, tchunk_p0_) * nk * NQ * nao2;
   scratch_ = dalloc<double2>((size_t)scratch_elems_);
 }
@@ -258,7 +258,8 @@ void gw_hip_kernel::set_split_slices(int ns) {
 }
 
 size_t gw_hip_kernel::scratch_per_t() const { return (size_t)d_.nk * d_.NQ * d_.nao * d_.nao; }
-size_t gw_hip_kernel::dyson_per_q() const { return (size_t)d_.nw_b * d_.NQ * d_.NQ; }
+// three regions of nw_b matrices per q: P0(i Omega), (I - P0)^-1, P-tilde(i Omega)
+size_t gw_hip_kernel::dyson_per_q() const { return (size_t)3 * d_.nw_b * d_.NQ * d_.NQ; }
 size_t gw_hip_kernel::slice_bytes_per_t() const {
   const size_t kcp = (size_t)ceil_div(d_.nao * d_.nao, 64) * 64, kcs = (size_t)ceil_div(d_.NQ, 64) * 64;
   return std::max((size_t)d_.NQ * d_.nk * OZ_PLANES * split_ns_ * kcp, (size_t)d_.NQ * OZ_PLANES * split_ns_ * kcs);
@@ -574,6 +575,7 @@ void gw_hip_kernel::p0_gemm_split(int64_t q, int64_t tc, double2* Pt0, int group
     o.accumulate = sl > 0 ? 1 : 0;
     o.group_m = group;
     o.upper = g_herm_ ? 1 : 0;
+    o.tile = tune_.p0_oz_tile;
     zgemm_ozaki(o, ns, stream_);
   }
 }
@@ -604,6 +606,7 @@ void gw_hip_kernel::sigma_gemm_split(const int8_t* Vsb, const int* eVsb, int64_t
   o.alpha = make_double2(1.0, 0.0);
   o.accumulate = 0;
   o.group_m = group;
+  o.tile = tune_.s_oz_tile;
   zgemm_ozaki(o, ns, stream_);
 }
 
@@ -737,12 +740,15 @@ void gw_hip_kernel::block_inverse(double2* X, int64_t nmat) {
 }
 
 // P-tilde of the first nq rows of P_; q0_row[i] is the AqQ row of row i (-1: none), read when the
-// q -> 0 data are being collected.  Per q chunk, in one buffer: P0(i Omega) = T_bt' P0(tau), then
-// A = I - P0(i Omega), its inverse, and P-tilde(i Omega) = (A^-1 + A^-H)/2 - I -- the reference's
-// Hermitised (I - P0)^-1 times P0 (src/gw_cpu_kernel.cpp:241-252), since (I - P0)^-1 P0 =
-// (I - P0)^-1 - I -- then P-tilde(tau) = T_tb P-tilde(i Omega).
+// q -> 0 data are being collected.  Per q chunk the Dyson work buffer holds three regions of
+// qchunk_ x nw_b matrices: W0 = P0(i Omega) = T_bt' P0(tau); W1 = A = I - P0(i Omega), inverted in
+// place and Hermitised; W2 = P-tilde(i Omega) = Herm(A^-1) P0(i Omega), the reference's product
+// (src/gw_cpu_kernel.cpp:241-252; forming Herm(A^-1) - I instead would lose the relative accuracy of
+// P-tilde where |P0| << 1); then P-tilde(tau) = T_tb P-tilde(i Omega).
 void gw_hip_kernel::stage_dyson(int64_t nq, const int* q0_row) {
   const int64_t NQ = d_.NQ, NQ2 = NQ * NQ, half = d_.nts / 2, nwb = d_.nw_b, nts = d_.nts;
+  const int64_t region = qchunk_ * nwb * NQ2;
+  double2 *W0 = Pw_, *W1 = Pw_ + region, *W2 = Pw_ + 2 * region;
 
   Event e0(timing_), e1(timing_), e2(timing_), e3(timing_);
   auto mark = [&](hipEvent_t e) { if (timing_) GWHIP_CHECK(hipEventRecord(e, stream_)); };
@@ -756,22 +762,31 @@ void gw_hip_kernel::stage_dyson(int64_t nq, const int* q0_row) {
       a.M = (int)nwb; a.N = (int)NQ2; a.K = (int)half; a.batch = (int)qc; a.kin = (int)half; a.nin = (int)NQ2;
       a.A = Tbt_fold_; a.sAb = 0; a.sAm = half; a.sAko = 0; a.sAki = 1;
       a.B = P_ + q0 * nts * NQ2; a.sBb = nts * NQ2; a.sBko = 0; a.sBki = NQ2; a.sBno = 0; a.sBni = 1;
-      a.C = Pw_; a.sCb = nwb * NQ2; a.sCm = NQ2; a.sCno = 0; a.sCni = 1;
+      a.C = W0; a.sCb = nwb * NQ2; a.sCm = NQ2; a.sCno = 0; a.sCni = 1;
       a.alpha = make_double2(1.0, 0.0);
       zgemm_f64(a, true, false, stream_);
     }
     mark(e1);
     const int64_t nmat = qc * nwb;
     if (nmat > 65535) throw status_error(GWHIP_ERR_UNSUPPORTED, "more than 65535 Dyson matrices in one q chunk");
-    hipLaunchKernelGGL(identity_minus_kernel, dim3((unsigned)ceil_div(nmat * NQ2, 256)), dim3(256), 0, stream_, Pw_, Pw_,
+    hipLaunchKernelGGL(identity_minus_kernel, dim3((unsigned)ceil_div(nmat * NQ2, 256)), dim3(256), 0, stream_, W1, W0,
                        (int)NQ, nmat);
     GWHIP_CHECK(hipGetLastError());
-    timed(K_DYSON_INV, [&] { block_inverse(Pw_, nmat); });
-    hipLaunchKernelGGL(herm_minus_identity_kernel, dim3((unsigned)ceil_div(NQ2, 256), (unsigned)nmat), dim3(256), 0,
-                       stream_, Pw_, (int)NQ, NQ2);
+    timed(K_DYSON_INV, [&] { block_inverse(W1, nmat); });
+    hipLaunchKernelGGL(hermitize_kernel, dim3((unsigned)ceil_div(NQ2, 256), (unsigned)nmat), dim3(256), 0, stream_, W1,
+                       (int)NQ, NQ2);
     GWHIP_CHECK(hipGetLastError());
+    {  // P-tilde(i Omega) = Herm(A^-1) P0(i Omega), one product per matrix
+      ZGemmArgs a{};
+      a.M = (int)NQ; a.N = (int)NQ; a.K = (int)NQ; a.batch = (int)nmat; a.kin = (int)NQ; a.nin = (int)NQ;
+      a.A = W1; a.sAb = NQ2; a.sAm = NQ; a.sAko = 0; a.sAki = 1;
+      a.B = W0; a.sBb = NQ2; a.sBko = 0; a.sBki = NQ; a.sBno = 0; a.sBni = 1;
+      a.C = W2; a.sCb = NQ2; a.sCm = NQ; a.sCno = 0; a.sCni = 1;
+      a.alpha = make_double2(1.0, 0.0);
+      zgemm_f64(a, true, false, stream_);
+    }
     if (q0_eps_) {  // aux_to_PW_00 for the irreducible q of this chunk
-      hipLaunchKernelGGL(q0_eps_kernel, dim3((unsigned)nwb, (unsigned)qc), dim3(256), 0, stream_, Pw_, q0_A_, q0_row,
+      hipLaunchKernelGGL(q0_eps_kernel, dim3((unsigned)nwb, (unsigned)qc), dim3(256), 0, stream_, W2, q0_A_, q0_row,
                          (int)q0, (int)nwb, (int)NQ, (int)q0_inq_, q0_eps_dev_);
       GWHIP_CHECK(hipGetLastError());
     }
@@ -780,7 +795,7 @@ void gw_hip_kernel::stage_dyson(int64_t nq, const int* q0_row) {
       ZGemmArgs a{};
       a.M = (int)nts; a.N = (int)NQ2; a.K = (int)nwb; a.batch = (int)qc; a.kin = (int)nwb; a.nin = (int)NQ2;
       a.A = Ttb_; a.sAb = 0; a.sAm = nwb; a.sAko = 0; a.sAki = 1;
-      a.B = Pw_; a.sBb = nwb * NQ2; a.sBko = 0; a.sBki = NQ2; a.sBno = 0; a.sBni = 1;
+      a.B = W2; a.sBb = nwb * NQ2; a.sBko = 0; a.sBki = NQ2; a.sBno = 0; a.sBni = 1;
       a.C = P_ + q0 * nts * NQ2; a.sCb = nts * NQ2; a.sCm = NQ2; a.sCno = 0; a.sCni = 1;
       a.alpha = make_double2(1.0, 0.0);
       zgemm_f64(a, true, false, stream_);
@@ -1057,10 +1072,11 @@ void gw_hip_kernel::solve_device(const double2* G, double2* Sigma, int precision
   GWHIP_CHECK(hipEventElapsedTime(&x, t0, t1)); region_ms_[R_P0] = x;
   GWHIP_CHECK(hipEventElapsedTime(&x, t2, t3)); region_ms_[R_SIGMA] = x;
   GWHIP_CHECK(hipEventElapsedTime(&x, t0, t3)); region_ms_[R_TOTAL] = x;
-  check_dyson_flag();
+  // non-finite first: a NaN or Inf in G also fails the Dyson step's pivot test
   int flag = 0;
   GWHIP_CHECK(hipMemcpy(&flag, flag_, sizeof(int), hipMemcpyDeviceToHost));
   if (flag) throw status_error(GWHIP_ERR_NONFINITE, "Sigma contains non-finite values");
+  check_dyson_flag();
   stored_valid_ = keep_p0_;
   leak_valid_ = leak_on_;
 }
@@ -1119,14 +1135,23 @@ void gw_hip_kernel::autotune(const double2* G, int precision) {
         }
       });
     }
-    // row-tile group of the L2-aware tile order
+    // block tile of each int8 GEMM, then the row-tile group of the L2-aware tile order
     const int groups[7] = {1, 2, 4, 8, 16, 32, 64};
+    const int tiles[2] = {2, 3};
     p0_prep(0, G, 0, tcp, tune_.qpw);
     p0_slice(tcp);
+    tune_.p0_oz_tile = best_of(2, tiles, [&](int c) {
+      tune_.p0_oz_tile = c;
+      p0_gemm_split(0, tcp, P_, tune_.p0_group, tune_.p0_kchunks);
+    });
     tune_.p0_group = best_of(7, groups, [&](int c) { p0_gemm_split(0, tcp, P_, c, tune_.p0_kchunks); });
     const int kch[5] = {0, 16, 8, 4, 2};
     tune_.p0_kchunks = best_of(5, kch, [&](int c) { p0_gemm_split(0, tcp, P_, tune_.p0_group, c); });
     sigma_slice(P_, tcs);
+    tune_.s_oz_tile = best_of(2, tiles, [&](int c) {
+      tune_.s_oz_tile = c;
+      sigma_gemm_split(Vs_neg(0), eVs_neg(0), d_.nk, tcs, tune_.s_group);
+    });
     tune_.s_group = best_of(7, groups, [&](int c) { sigma_gemm_split(Vs_neg(0), eVs_neg(0), d_.nk, tcs, c); });
   }
   stored_valid_ = false;

@@ -42,8 +42,9 @@ struct KernelStat {
 struct Tuning {
   int p0_shape = 0, s_shape = 0, qpw = 4;
   int p0_group = 8, s_group = 8;  // split precision: row tiles per L2 group of the int8 GEMMs
-  int p0_oz_tile = 2, s_oz_tile = 2;  // int8 GEMM block tile: 2 (32x64 software-pipelined, the only
-                                      // one since the three-plane slices); kept in the tuning array
+  int p0_oz_tile = 2, s_oz_tile = 2;  // int8 GEMM block tile: 2 (32 x 64, A through a register-staged
+                                      // LDS panel, B fragments from global memory) or 3 (64 x 64, both
+                                      // operands through LDS by direct global -> LDS loads)
   int p0_fused = 1;  // split precision, one spin block: P0 orbital kernel writes int8 slices directly
   int orb_gemm = 0;  // orbital contractions as batched FP64 GEMMs (always so for nao > 48)
   int p0_kchunks = 4;  // split precision: k chunks per launch of the int8 P0 GEMM (0: all in one), so
@@ -56,7 +57,7 @@ struct Tuning {
   }
   bool from_array(const int* v) {
     if (v[0] < 0 || v[0] > 2 || v[1] < 0 || v[1] > 2 || (v[2] != 1 && v[2] != 2 && v[2] != 4 && v[2] != 8) ||
-        v[3] < 1 || v[4] < 1 || v[5] != 2 || v[6] != 2 || v[7] < 0 || v[7] > 1 || v[8] < 0 ||
+        v[3] < 1 || v[4] < 1 || (v[5] != 2 && v[5] != 3) || (v[6] != 2 && v[6] != 3) || v[7] < 0 || v[7] > 1 || v[8] < 0 ||
         v[8] > 1 || v[9] < 0)
       return false;
     p0_shape = v[0]; s_shape = v[1]; qpw = v[2]; p0_group = v[3]; s_group = v[4]; p0_oz_tile = v[5]; s_oz_tile = v[6];

@@ -91,20 +91,6 @@ __global__ void identity_minus_kernel(double2* X, const double2* Pw, int N, int6
   X[gid] = make_double2((i == j ? 1.0 : 0.0) - p.x, -p.y);
 }
 
-// (X + X^H)/2 - I, in place, for every matrix of the batch: P-tilde(i Omega) from the inverse X of
-// I - P0(i Omega), since (I - P0)^-1 P0 = (I - P0)^-1 - I.
-__global__ void herm_minus_identity_kernel(double2* X, int N, int64_t stride) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (int64_t)N * N) return;
-  const int i = (int)(e / N), j = (int)(e - (e / N) * N);
-  if (i > j) return;
-  double2* M = X + (int64_t)blockIdx.y * stride;
-  const double2 u = M[(int64_t)i * N + j], l = M[(int64_t)j * N + i];
-  const double2 h = make_double2(0.5 * (u.x + l.x) - (i == j ? 1.0 : 0.0), 0.5 * (u.y - l.y));
-  M[(int64_t)i * N + j] = h;
-  M[(int64_t)j * N + i] = make_double2(h.x, i == j ? 0.0 : -h.y);
-}
-
 // Block Gauss-Jordan step J (columns [J0, J0 + jb)) of the batched inverse, data movement only:
 // the column panel C[m][i][l] = A[m][i][J0 + l] (all N rows) and the diagonal block
 // D[m][l][l'] = A[m][J0 + l][J0 + l'] (row stride jb), both copied out before the GEMMs update A.

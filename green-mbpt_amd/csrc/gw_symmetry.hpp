@@ -340,10 +340,10 @@ void gw_hip_kernel::solve_star_device(const double2* G_ibz, double2* Sigma_ibz, 
   GWHIP_CHECK(hipEventElapsedTime(&x, t0, t1)); region_ms_[R_P0] = x;
   GWHIP_CHECK(hipEventElapsedTime(&x, t2, t3)); region_ms_[R_SIGMA] = x;
   GWHIP_CHECK(hipEventElapsedTime(&x, t0, t3)); region_ms_[R_TOTAL] = x;
-  check_dyson_flag();
-  int flag = 0;
+  int flag = 0;  // non-finite first: a NaN or Inf in G also fails the Dyson step's pivot test
   GWHIP_CHECK(hipMemcpy(&flag, flag_, sizeof(int), hipMemcpyDeviceToHost));
   if (flag) throw status_error(GWHIP_ERR_NONFINITE, "Sigma contains non-finite values");
+  check_dyson_flag();
   leak_valid_ = leak_on_;
 }
 

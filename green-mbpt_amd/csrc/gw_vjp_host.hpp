@@ -392,10 +392,10 @@ void gw_hip_kernel::vjp_device(const double2* G, const double2* Sbar, double2* G
   float ms = 0.f;
   GWHIP_CHECK(hipEventElapsedTime(&ms, e0, e1));
   vjp_ms_ = ms;
-  check_dyson_flag();
-  int flag = 0;
+  int flag = 0;  // non-finite first: a NaN or Inf in G also fails the Dyson step's pivot test
   GWHIP_CHECK(hipMemcpy(&flag, flag_, sizeof(int), hipMemcpyDeviceToHost));
   if (flag) throw status_error(GWHIP_ERR_NONFINITE, "G_bar contains non-finite values");
+  check_dyson_flag();
 }
 
 void gw_hip_kernel::vjp_host(const double2* G, const double2* Sbar, double2* Gbar, int mode) {

@@ -182,6 +182,7 @@ struct OzGemmArgs {
   int64_t nblocks, per_xcd;
   int group_m;  // row tiles (over batch x M) swept together across all column tiles; 0 = heuristic
   int upper;    // 1: skip block tiles that lie wholly below the diagonal (Hermitian products)
+  int tile;     // block tile: 2 = 32 x 64 (zgemm_ozaki_pipe_kernel), 3 = 64 x 64 (zgemm_ozaki_lds_kernel)
 };
 
 // Output layout of v_mfma_i32_16x16x64_i8: register r of lane l holds row
@@ -206,7 +207,7 @@ __device__ __forceinline__ int oz_lds_off(int ps, int rows, int r, int piece) {
 template <int NS>
 __global__ __launch_bounds__(256, 2) void zgemm_ozaki_pipe_kernel(OzGemmArgs p) {
   constexpr int NPS = OZ_PLANES * NS;          // plane-slices per row
-  constexpr int NPC = (NPS + 1) / 2;           // staged per thread and K step (two parities)
+  constexpr int NPC = NPS / 2;                 // staged per thread and K step (two parities)
   constexpr int BUF = NPS * OZ_BM * 64;
   extern __shared__ __attribute__((aligned(16))) int8_t oz_lds[];
 
@@ -232,14 +233,16 @@ __global__ __launch_bounds__(256, 2) void zgemm_ozaki_pipe_kernel(OzGemmArgs p) 
   const int lrow = lane & 15, lg = lane >> 4;
   const int wn = wave * 16;
 
-  // A staging: thread -> (plane-slice parity, row, 16-byte piece), piece fastest
-  const int a_row = (tid >> 2) & 31, a_piece = tid & 3, a_par = tid >> 7;
-  const int8_t* a_ptr = nullptr;
-  if (m0 + a_row < p.M) a_ptr = p.A + b * p.sAb + (int64_t)(m0 + a_row) * p.sAm + 16 * a_piece;
+  // A staging: thread -> (plane-slice parity, row, 16-byte piece), piece fastest.  Rows past M and
+  // columns past N read the last valid row / column instead: they only feed results that are never
+  // stored, and unconditional loads keep the memory-counter waits of the pipeline exact (a load
+  // under a per-lane branch makes the compiler wait for every load in flight).
+  const int a_row = (tid >> 2) & 31, a_piece = tid & 3, a_par = (tid >> 7) & 1;
+  const int8_t* a_ptr = p.A + b * p.sAb + (int64_t)min(m0 + a_row, p.M - 1) * p.sAm + 16 * a_piece;
   const int a_lds = oz_lds_off(0, OZ_BM, a_row, a_piece);
-  const int bcol = n0 + wn + lrow;
-  const int8_t* b_ptr = nullptr;
-  if (bcol < p.N) {
+  const int bcol = min(n0 + wn + lrow, p.N - 1);
+  const int8_t* b_ptr;
+  {
     const int no = bcol / p.nin;
     b_ptr = p.B + b * p.sBb + (int64_t)no * p.sBno + (int64_t)(bcol - no * p.nin) * p.sBni + 16 * lg;
   }
@@ -249,21 +252,22 @@ __global__ __launch_bounds__(256, 2) void zgemm_ozaki_pipe_kernel(OzGemmArgs p) 
     return b_ptr + (int64_t)c * p.sBc + ks;
   };
 
+  // NPS = 2 NS is even, so the two parities cover every plane-slice: ps = a_par + 2 j < NPS
+  static_assert(NPS % 2 == 0, "plane-slices come in pairs");
   v4i a_nx[NPC];
   auto load_a = [&](int g) {
     const int c = g / nks, ks = (g - c * nks) * OZ_KS;
 #pragma unroll
     for (int j = 0; j < NPC; ++j) {
       const int ps = a_par + 2 * j;
-      a_nx[j] = (a_ptr && ps < NPS) ? *reinterpret_cast<const v4i*>(a_ptr + (int64_t)c * p.sAc + (int64_t)ps * p.kc + ks)
-                                    : v4i{0, 0, 0, 0};
+      a_nx[j] = *reinterpret_cast<const v4i*>(a_ptr + (int64_t)c * p.sAc + (int64_t)ps * p.kc + ks);
     }
   };
   auto store_a = [&](int buf) {
 #pragma unroll
     for (int j = 0; j < NPC; ++j) {
       const int ps = a_par + 2 * j;
-      if (ps < NPS) *reinterpret_cast<v4i*>(oz_lds + buf * BUF + a_lds + ps * OZ_BM * 64) = a_nx[j];
+      *reinterpret_cast<v4i*>(oz_lds + buf * BUF + a_lds + ps * OZ_BM * 64) = a_nx[j];
     }
   };
 
@@ -282,21 +286,22 @@ __global__ __launch_bounds__(256, 2) void zgemm_ozaki_pipe_kernel(OzGemmArgs p) 
   load_a(0);
   store_a(0);
   {
-    const int8_t* bc = b_ptr ? b_at(0) : nullptr;
+    const int8_t* bc = b_at(0);
 #pragma unroll
     for (int pl = 0; pl < OZ_PLANES; ++pl)
 #pragma unroll
-      for (int t = 0; t < NS; ++t)
-        fb[pl][t] = bc ? *reinterpret_cast<const v4i*>(bc + (int64_t)(pl * NS + t) * p.kc) : v4i{0, 0, 0, 0};
+      for (int t = 0; t < NS; ++t) fb[pl][t] = *reinterpret_cast<const v4i*>(bc + (int64_t)(pl * NS + t) * p.kc);
   }
   __syncthreads();
 
   for (int g = 0; g < nsteps; ++g) {
-    const bool more = g + 1 < nsteps;
+    // the loads of the next step are issued unconditionally (the last step reloads its own data,
+    // which nothing reads): a load under a branch makes the compiler wait for every load in flight
+    const int gn = g + 1 < nsteps ? g + 1 : g;
     const int c = g / nks;
     const bool chunk_end = g - c * nks == nks - 1;
-    if (more) load_a(g + 1);
-    const int8_t* bn = (more && b_ptr) ? b_at(g + 1) : nullptr;
+    load_a(gn);
+    const int8_t* bn = b_at(gn);
     const int8_t* As = oz_lds + (g & 1) * BUF;
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
@@ -308,8 +313,10 @@ __global__ __launch_bounds__(256, 2) void zgemm_ozaki_pipe_kernel(OzGemmArgs p) 
         fa_i[x] = *reinterpret_cast<const v4i*>(As + oz_lds_off(1 * NS + s, OZ_BM, row, lg));
         fa_n[x] = neg_i8x16(fa_i[x]);
       }
+      // t from high to low: fb[.][t] was refilled for this step after the previous step's slice
+      // NS - 1 - t, so the highest t arrived first
 #pragma unroll
-      for (int t = 0; t + s < NS; ++t) {
+      for (int t = NS - 1 - s; t >= 0; --t) {
 #pragma unroll
         for (int x = 0; x < 2; ++x) {
           ar[s + t][x] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa_r[x], fb[0][t], ar[s + t][x], 0, 0, 0);
@@ -318,26 +325,22 @@ __global__ __launch_bounds__(256, 2) void zgemm_ozaki_pipe_kernel(OzGemmArgs p) 
           ai[s + t][x] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa_i[x], fb[0][t], ai[s + t][x], 0, 0, 0);
         }
       }
-      if (more) {
+      {
         const int t = NS - 1 - s;
 #pragma unroll
-        for (int pl = 0; pl < OZ_PLANES; ++pl)
-          fb[pl][t] = bn ? *reinterpret_cast<const v4i*>(bn + (int64_t)(pl * NS + t) * p.kc) : v4i{0, 0, 0, 0};
+        for (int pl = 0; pl < OZ_PLANES; ++pl) fb[pl][t] = *reinterpret_cast<const v4i*>(bn + (int64_t)(pl * NS + t) * p.kc);
       }
     }
     if (chunk_end) {  // int32 -> FP64 with the (row, column, chunk) scale 2^(ea + eb) / 127^2
-      int eb = 0;
-      const int n = n0 + wn + lrow;
-      if (n < p.N) {
-        const int no = n / p.nin;
-        eb = p.eB[b * p.seBb + (int64_t)no * p.seBno + (int64_t)(n - no * p.nin) * p.seBni + c];
-      }
+      // rows / columns past the edge take the clamped ones' exponents (their results are not stored)
+      const int no = bcol / p.nin;
+      const int eb = p.eB[b * p.seBb + (int64_t)no * p.seBno + (int64_t)(bcol - no * p.nin) * p.seBni + c];
 #pragma unroll
       for (int x = 0; x < 2; ++x)
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-          const int m = m0 + 16 * x + OZ_ROW(lg, r);
-          const int ea = m < p.M ? p.eA[b * p.seAb + (int64_t)m * p.seAm + c] : 0;
+          const int m = min(m0 + 16 * x + OZ_ROW(lg, r), p.M - 1);
+          const int ea = p.eA[b * p.seAb + (int64_t)m * p.seAm + c];
           double sr = 0.0, si = 0.0;
 #pragma unroll
           for (int d = 0; d < NS; ++d) {  // exact integers times 254^-d
@@ -353,7 +356,7 @@ __global__ __launch_bounds__(256, 2) void zgemm_ozaki_pipe_kernel(OzGemmArgs p) 
 #pragma unroll
         for (int x = 0; x < 2; ++x) ar[d][x] = ai[d][x] = v4i{0, 0, 0, 0};
     }
-    if (more) store_a((g + 1) & 1);
+    store_a((g + 1) & 1);  // on the last step into the buffer no wave reads any more
     __syncthreads();
   }
 
@@ -366,6 +369,183 @@ __global__ __launch_bounds__(256, 2) void zgemm_ozaki_pipe_kernel(OzGemmArgs p) 
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
       const int m = m0 + 16 * x + OZ_ROW(lg, r);
+      if (m >= p.M) continue;
+      const double vr = cr[x][r], vi = ci[x][r];
+      double2 v = make_double2(p.alpha.x * vr - p.alpha.y * vi, p.alpha.x * vi + p.alpha.y * vr);
+      double2* dst = p.C + b * p.sCb + (int64_t)m * p.sCm + coff;
+      if (p.accumulate) {
+        const double2 o = *dst;
+        v.x += o.x;
+        v.y += o.y;
+      }
+      *dst = v;
+    }
+}
+
+// Logical tile of hardware block blockIdx.x in the XCD-aware, L2-grouped order: each XCD takes a
+// contiguous range of logical tiles; within it, groups of group_m row tiles (rows run over
+// batch x M) sweep every column tile, row fastest.  False for the padding blocks of the grid.
+__device__ __forceinline__ bool oz_tile_of(const OzGemmArgs& p, int64_t& b, int& tm, int& tn) {
+  const int64_t hw = blockIdx.x;
+  const int64_t logical = (hw % 8) * p.per_xcd + hw / 8;
+  if (logical >= p.nblocks) return false;
+  const int64_t rows = (int64_t)p.batch * p.tiles_m;
+  const int64_t in_group = (int64_t)p.group_m * p.tiles_n;
+  const int64_t grp = logical / in_group;
+  const int64_t first = grp * p.group_m;
+  const int64_t gsize = rows - first < p.group_m ? rows - first : p.group_m;
+  const int64_t idx = logical - grp * in_group;
+  const int64_t rt = first + idx % gsize;
+  tn = (int)(idx / gsize);
+  b = rt / p.tiles_m;
+  tm = (int)(rt - b * p.tiles_m);
+  return true;
+}
+
+// Block tile 64 x 64 (complex) for eight waves in a 2 x 4 grid of 32 x 16 wave tiles, both operands
+// staged in LDS by direct global -> LDS loads (global_load_lds_dwordx4), double-buffered in two
+// static LDS arrays (distinct objects, so the compiler's counters let a wave read one buffer while
+// the loads into the other are in flight).  The loads of K step g + 1 go out at the start of step
+// g and land while its MFMAs run; the barrier closing the step waits for them.
+//
+// A stage is a list of 1 KB chunks: A's (row block of 16, plane-slice) then B's (column block of
+// 16, plane-slice), each in MFMA fragment order -- lane l's 16 bytes are row (column) l % 16, K
+// bytes 16 (l / 16) .. +16 of the step -- so one wave-wide load fills a chunk and a fragment read is
+// one contiguous 1 KB ds_read_b128 (no bank conflicts).  Waves 0-3 load A's row blocks, waves
+// 4-7 B's column blocks, NPS chunks each.  Rows past M / columns past N load the last valid row /
+// column: they only feed results that are never stored.
+constexpr int OZL_BM = 64, OZL_BN = 64;
+
+template <int NS>
+__global__ __launch_bounds__(512, 1) void zgemm_ozaki_lds_kernel(OzGemmArgs p) {
+  constexpr int NPS = OZ_PLANES * NS;
+  constexpr int STAGE = 8 * NPS * 1024;  // 4 row blocks + 4 column blocks, NPS chunks each
+  __shared__ __attribute__((aligned(1024))) int8_t oz_s0[STAGE];
+  __shared__ __attribute__((aligned(1024))) int8_t oz_s1[STAGE];
+
+  int64_t b;
+  int tm, tn;
+  if (!oz_tile_of(p, b, tm, tn)) return;
+  const int m0 = tm * OZL_BM, n0 = tn * OZL_BN;
+  if (p.upper && m0 >= n0 + OZL_BN) return;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lrow = lane & 15, lg = lane >> 4;
+  const int wm = wave & 1, wn = wave >> 1;  // rows 32 wm .. +32, columns 16 wn .. +16
+
+  // this lane's source row (waves 0-3: A row block `wave`) or column (waves 4-7: B column block)
+  const bool loads_a = wave < 4;
+  const int blk = wave & 3;
+  const int8_t* src;
+  int64_t s_chunk;
+  if (loads_a) {
+    const int m = min(m0 + 16 * blk + lrow, p.M - 1);
+    src = p.A + b * p.sAb + (int64_t)m * p.sAm + 16 * lg;
+    s_chunk = p.sAc;
+  } else {
+    const int n = min(n0 + 16 * blk + lrow, p.N - 1);
+    const int no = n / p.nin;
+    src = p.B + b * p.sBb + (int64_t)no * p.sBno + (int64_t)(n - no * p.nin) * p.sBni + 16 * lg;
+    s_chunk = p.sBc;
+  }
+  const int chunk0 = (loads_a ? 0 : 4 * NPS) + blk * NPS;  // first of this wave's NPS chunks
+  const int nks = p.kc / OZ_KS, nsteps = p.nchunk * nks;
+  auto load_stage = [&](int8_t* st, int g) {
+    const int c = g / nks, ks = (g - c * nks) * OZ_KS;
+    const int8_t* s0 = src + (int64_t)c * s_chunk + ks;
+#pragma unroll
+    for (int ps = 0; ps < NPS; ++ps)
+      __builtin_amdgcn_global_load_lds((const void*)(s0 + (int64_t)ps * p.kc),
+                                       (__attribute__((address_space(3))) void*)(st + (chunk0 + ps) * 1024), 16, 0, 0);
+  };
+
+  double cr[2][4], ci[2][4];
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) { cr[x][r] = 0.0; ci[x][r] = 0.0; }
+  v4i ar[NS][2], ai[NS][2];  // [d][x] int32 accumulators (Re, Im) of slice level d
+#pragma unroll
+  for (int d = 0; d < NS; ++d)
+#pragma unroll
+    for (int x = 0; x < 2; ++x) ar[d][x] = ai[d][x] = v4i{0, 0, 0, 0};
+
+  const int bcol = min(n0 + 16 * wn + lrow, p.N - 1);
+  auto compute = [&](const int8_t* st, int g) {
+    v4i fb[OZ_PLANES][NS];
+#pragma unroll
+    for (int pl = 0; pl < OZ_PLANES; ++pl)
+#pragma unroll
+      for (int t = 0; t < NS; ++t)
+        fb[pl][t] = *reinterpret_cast<const v4i*>(st + (4 * NPS + wn * NPS + pl * NS + t) * 1024 + 16 * lane);
+#pragma unroll
+    for (int s = 0; s < NS; ++s) {
+      v4i fa_r[2], fa_i[2], fa_n[2];
+#pragma unroll
+      for (int x = 0; x < 2; ++x) {
+        const int rb = 2 * wm + x;
+        fa_r[x] = *reinterpret_cast<const v4i*>(st + (rb * NPS + 0 * NS + s) * 1024 + 16 * lane);
+        fa_i[x] = *reinterpret_cast<const v4i*>(st + (rb * NPS + 1 * NS + s) * 1024 + 16 * lane);
+        fa_n[x] = neg_i8x16(fa_i[x]);
+      }
+#pragma unroll
+      for (int t = 0; t + s < NS; ++t) {
+#pragma unroll
+        for (int x = 0; x < 2; ++x) {
+          ar[s + t][x] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa_r[x], fb[0][t], ar[s + t][x], 0, 0, 0);
+          ai[s + t][x] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa_r[x], fb[1][t], ai[s + t][x], 0, 0, 0);
+          ar[s + t][x] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa_n[x], fb[1][t], ar[s + t][x], 0, 0, 0);
+          ai[s + t][x] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa_i[x], fb[0][t], ai[s + t][x], 0, 0, 0);
+        }
+      }
+    }
+    const int c = g / nks;
+    if (g - c * nks == nks - 1) {  // chunk end: int32 -> FP64 with the scale 2^(ea + eb) / 127^2
+      const int no = bcol / p.nin;
+      const int eb = p.eB[b * p.seBb + (int64_t)no * p.seBno + (int64_t)(bcol - no * p.nin) * p.seBni + c];
+#pragma unroll
+      for (int x = 0; x < 2; ++x)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          const int m = min(m0 + 32 * wm + 16 * x + OZ_ROW(lg, r), p.M - 1);
+          const int ea = p.eA[b * p.seAb + (int64_t)m * p.seAm + c];
+          double sr = 0.0, si = 0.0;
+#pragma unroll
+          for (int d = 0; d < NS; ++d) {  // exact integers times 254^-d
+            sr = fma((double)ar[d][x][r], oz_weight(d), sr);
+            si = fma((double)ai[d][x][r], oz_weight(d), si);
+          }
+          const double sc = ldexp(1.0 / (OZ_FIRST * OZ_FIRST), ea + eb);
+          cr[x][r] = fma(sr, sc, cr[x][r]);
+          ci[x][r] = fma(si, sc, ci[x][r]);
+        }
+#pragma unroll
+      for (int d = 0; d < NS; ++d)
+#pragma unroll
+        for (int x = 0; x < 2; ++x) ar[d][x] = ai[d][x] = v4i{0, 0, 0, 0};
+    }
+  };
+
+  load_stage(oz_s0, 0);
+  __syncthreads();
+  for (int g = 0; g < nsteps; g += 2) {
+    if (g + 1 < nsteps) load_stage(oz_s1, g + 1);
+    compute(oz_s0, g);
+    __syncthreads();
+    if (g + 1 >= nsteps) break;
+    if (g + 2 < nsteps) load_stage(oz_s0, g + 2);
+    compute(oz_s1, g + 1);
+    __syncthreads();
+  }
+
+  const int n = n0 + 16 * wn + lrow;
+  if (n >= p.N) return;
+  const int no = n / p.nin;
+  const int64_t coff = (int64_t)no * p.sCno + (int64_t)(n - no * p.nin) * p.sCni;
+#pragma unroll
+  for (int x = 0; x < 2; ++x)
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+      const int m = m0 + 32 * wm + 16 * x + OZ_ROW(lg, r);
       if (m >= p.M) continue;
       const double vr = cr[x][r], vi = ci[x][r];
       double2 v = make_double2(p.alpha.x * vr - p.alpha.y * vi, p.alpha.x * vi + p.alpha.y * vr);
@@ -392,17 +572,25 @@ inline size_t ozaki_pipe_lds_bytes(int ns) { return (size_t)2 * OZ_PLANES * OZ_B
 
 template <int NS>
 inline void ozaki_launch(const OzGemmArgs& p, int64_t grid, hipStream_t stream) {
+  if (p.tile == 3) {
+    hipLaunchKernelGGL(zgemm_ozaki_lds_kernel<NS>, dim3((unsigned)grid), dim3(512), 0, stream, p);
+    return;
+  }
   const size_t lds = ozaki_pipe_lds_bytes(NS);
   oz_allow_lds(zgemm_ozaki_pipe_kernel<NS>, lds);
   hipLaunchKernelGGL(zgemm_ozaki_pipe_kernel<NS>, dim3((unsigned)grid), dim3(256), lds, stream, p);
 }
 
+inline bool ozaki_tile_valid(int tile) { return tile == 2 || tile == 3; }
+
 inline void zgemm_ozaki(OzGemmArgs p, int ns, hipStream_t stream) {
   if (p.M <= 0 || p.N <= 0 || p.batch <= 0) return;
   if (p.kc % OZ_KS != 0 || p.nchunk <= 0 || p.nin <= 0) throw std::runtime_error("zgemm_ozaki: bad chunking");
   if (p.kc > split_kc_limit(ns)) throw std::runtime_error("zgemm_ozaki: chunk too long for exact int32 sums");
-  p.tiles_m = (int)ceil_div(p.M, OZ_BM);
-  p.tiles_n = (int)ceil_div(p.N, OZ_BN);
+  if (p.tile == 0) p.tile = 2;
+  if (!ozaki_tile_valid(p.tile)) throw std::runtime_error("zgemm_ozaki: unknown block tile");
+  p.tiles_m = (int)ceil_div(p.M, p.tile == 3 ? OZL_BM : OZ_BM);
+  p.tiles_n = (int)ceil_div(p.N, p.tile == 3 ? OZL_BN : OZ_BN);
   p.nblocks = (int64_t)p.batch * p.tiles_m * p.tiles_n;
   if (p.group_m <= 0) p.group_m = ozaki_default_group(p.tiles_n);
   p.group_m = (int)std::min<int64_t>(p.group_m, (int64_t)p.batch * p.tiles_m);

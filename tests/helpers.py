@@ -48,3 +48,22 @@ def synthetic(nk, nao, NQ, ns=1, beta=50.0, lam=1e3, mesh=None, seed=0, nso=None
 def rel_err(a, b):
     """max-abs difference over max |b|."""
     return float(np.abs(a - b).max() / np.abs(b).max())
+
+
+def min_eig_I_minus_P0(G, V, T, d):
+    """Smallest eigenvalue of the Hermitian part of I - P0(i Omega_n, q) over every q and bosonic
+    frequency, from the oracle's P0 (the device's Dyson step reports a matrix that is not positive
+    definite)."""
+    from oracle import gw_oracle as go
+
+    nk, NQ = d["nk"], d["NQ"]
+    X2C = d["nso"] != d["nao"]
+    lo = np.inf
+    for q in range(nk):
+        P0 = np.zeros((T.nts, NQ, NQ), complex)
+        go.eval_P0_tilde(P0, G, V, q, nk, d["nao"], d["nso"], d["ns"], NQ, X2C, d["mesh"])
+        go.symmetrize_P0(P0, nk)
+        for p in np.tensordot(T.T_bt, P0, axes=(1, 0)):
+            A = np.eye(NQ) - p
+            lo = min(lo, np.linalg.eigvalsh(0.5 * (A + np.conj(A.T))).min())
+    return float(lo)

@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 from gwhip import lib as gl
-from helpers import CONFIG0, load_fixture, rel_err, synthetic
+from helpers import min_eig_I_minus_P0, CONFIG0, load_fixture, rel_err, synthetic
 from oracle import gw_oracle as go
 
 pytestmark = pytest.mark.gpu
@@ -176,3 +176,34 @@ def test_explicit_kq_maps_match_mesh_maps():
         k1.close()
         k2.close()
     assert np.array_equal(S1, S2)
+
+
+@pytest.mark.parametrize("case", [
+    dict(nk=8, mesh=(2, 2, 2), nao=8, NQ=40, herm=True),     # one 64-row tile, ragged
+    dict(nk=4, mesh=(2, 2, 1), nao=5, NQ=130, herm=True),    # three row tiles, upper-tile skipping
+    dict(nk=4, mesh=(2, 2, 1), nao=5, NQ=130, herm=False),   # every tile (G not Hermitian)
+    dict(nk=2, mesh=(2, 1, 1), nao=3, NQ=70, nso=6, herm=True),  # X2C spin blocks
+])
+def test_int8_block_tiles_give_identical_sigma(case):
+    """The 32 x 64 (register-staged) and 64 x 64 (direct-to-LDS) int8 GEMM tiles compute every element
+    with the same exact int32 sums and the same FP64 chunk order: Sigma is bit-identical."""
+    nso = case.get("nso", case["nao"])
+    G, V, T, d = synthetic(case["nk"], case["nao"], case["NQ"], beta=20.0, lam=1e3, mesh=case["mesh"], seed=4,
+                           nso=nso, sigma=0.1)
+    if not case["herm"]:
+        rng = np.random.default_rng(6)
+        G = G + 1e-3 * (rng.standard_normal(G.shape) + 1j * rng.standard_normal(G.shape))
+    assert min_eig_I_minus_P0(G, V, T, d) > 0.5  # I - P0 positive definite, confirmed on the CPU
+    k = _kernel(d, T)
+    try:
+        k.set_grid(T.T_bt, T.T_tb)
+        k.upload_V(V)
+        out = {}
+        for tile in (2, 3):
+            k.set_tuning(p0_split_tile=tile, sigma_split_tile=tile)
+            out[tile] = k.solve(G, gl.PREC_SPLIT)
+    finally:
+        k.close()
+    S_ref = go.gw_sigma(G, V, T.T_bt, T.T_tb, d["nk"], d["nao"], d["nso"], d["ns"], d["NQ"], mesh=d["mesh"])
+    assert rel_err(out[3], S_ref) < TOL_SPLIT
+    assert np.array_equal(out[2], out[3])
