@@ -21,7 +21,7 @@ OUT = os.path.join(HERE, "gwhip", "libgwhip.so")
 OUT_H5 = os.path.join(HERE, "gwhip", "libgwhip_h5.so")
 OUT_DEBUG = os.path.join(HERE, "gwhip", "libgwhip_debug.so")
 HDF5_ROOT = os.environ.get("HDF5_ROOT", "/opt/conda")
-SOURCES = ["gw_hip_kernel.hip"]
+SOURCES = ["gw_hip_kernel.hip", "gwhip_noalloc.hip"]
 ARCH = os.environ.get("GWHIP_ARCH", "gfx950")
 
 

@@ -1169,6 +1169,115 @@ void gw_hip_kernel::solve_host(const double2* G, double2* Sigma, int precision) 
   GWHIP_CHECK(hipMemcpy(Sigma, S_dev_, nsig * sizeof(double2), hipMemcpyDeviceToHost));
 }
 
+namespace {
+constexpr size_t kWsAlign = 256;
+inline size_t ws_round(size_t b) { return (b + kWsAlign - 1) / kWsAlign * kWsAlign; }
+}  // namespace
+
+// Byte layout of solve_device_in's workspace for tc tau points and qc q values per chunk:
+// [flags 256 B | epilogue partials | Dyson work | orbital scratch | int8 slices | exponents].
+struct gw_hip_kernel::WsPlan {
+  size_t epi = 0, dyson = 0, scratch_elems = 0, scratch = 0, slices = 0, exps = 0, total = 0;
+};
+
+gw_hip_kernel::WsPlan gw_hip_kernel::ws_plan(int64_t tc, int64_t qc, bool split) const {
+  WsPlan p;
+  const int qs = epi_qsplit(d_.nk);
+  if (qs > 1) p.epi = ws_round((size_t)qs * d_.nts * d_.ns * d_.nk * d_.nso * d_.nso * sizeof(double2));
+  p.dyson = ws_round((size_t)qc * dyson_per_q() * sizeof(double2));
+  const size_t nb = (size_t)std::min<int64_t>(64, d_.NQ);  // block_inverse's panels for one matrix
+  p.scratch_elems = std::max((size_t)tc * scratch_per_t(), 2 * (size_t)d_.NQ * nb + nb * nb);
+  p.scratch = ws_round(p.scratch_elems * sizeof(double2));
+  if (split) {
+    p.slices = ws_round((size_t)tc * slice_bytes_per_t());
+    p.exps = ws_round((size_t)tc * exps_per_t() * sizeof(int));
+  }
+  p.total = kWsAlign + p.epi + p.dyson + p.scratch + p.slices + p.exps;
+  return p;
+}
+
+size_t gw_hip_kernel::min_workspace_bytes(int precision) const { return ws_plan(1, 1, any_split(precision)).total; }
+
+void gw_hip_kernel::solve_device_in(const double2* G, double2* Sigma, int precision, void* ws, size_t ws_bytes) {
+  if (!grid_set_) throw status_error(GWHIP_ERR_STATE, "transform grid not set (gwhip_set_grid)");
+  if (!v_set_) throw status_error(GWHIP_ERR_STATE, "integrals not set (gwhip_upload_V / gwhip_generate_V)");
+  if (!precision_valid(precision)) throw status_error(GWHIP_ERR_ARG, "unknown precision");
+  if (!G || !Sigma || !ws) throw status_error(GWHIP_ERR_ARG, "null G, Sigma or workspace");
+  if ((uintptr_t)ws % kWsAlign != 0) throw status_error(GWHIP_ERR_ARG, "workspace not 256-byte aligned");
+  if (orb_gemm())
+    throw status_error(GWHIP_ERR_UNSUPPORTED, "the orbital GEMM form (nao > 48 or tuned on) needs buffers of its own");
+  if (q0_on_) throw status_error(GWHIP_ERR_UNSUPPORTED, "the q -> 0 correction runs a host fit between stages");
+  const bool split = any_split(precision);
+  if (split && !split_ready())
+    throw status_error(GWHIP_ERR_STATE, "int8 slice stores of V not built: evaluate once in a split precision first");
+  const WsPlan p1 = ws_plan(1, 1, split);
+  if (p1.total > ws_bytes)
+    throw status_error(GWHIP_ERR_UNSUPPORTED, "this system needs " + std::to_string(p1.total) +
+                                                  " bytes of workspace at one tau point and one q; " +
+                                                  std::to_string(ws_bytes) + " are available");
+  // as many tau points per chunk as fit (at most the half grid), then as many q
+  const int64_t half = d_.nts / 2;
+  int64_t tc = 1, qc = 1;
+  while (tc < half && ws_plan(tc + 1, 1, split).total <= ws_bytes) ++tc;
+  while (qc < d_.nk && ws_plan(tc, qc + 1, split).total <= ws_bytes) ++qc;
+  const WsPlan p = ws_plan(tc, qc, split);
+  GWHIP_CHECK(hipSetDevice(device_));
+
+  // point the transient buffers into the workspace for this call; the handle's own come back after
+  struct Swap {
+    gw_hip_kernel& k;
+    double2 *scratch, *Pw, *epi;
+    int8_t *Zs, *Ps;
+    int *eZs, *ePs, *flag, *dflag;
+    unsigned long long* hcheck;
+    int64_t scratch_elems, epi_elems, qc, tp0, tsig;
+    bool timing;
+    explicit Swap(gw_hip_kernel& x)
+        : k(x), scratch(x.scratch_), Pw(x.Pw_), epi(x.epi_part_), Zs(x.Zs_), Ps(x.Ps_), eZs(x.eZs_), ePs(x.ePs_),
+          flag(x.flag_), dflag(x.dyson_flag_), hcheck(x.hcheck_), scratch_elems(x.scratch_elems_),
+          epi_elems(x.epi_part_elems_), qc(x.qchunk_), tp0(x.tchunk_p0_), tsig(x.tchunk_sigma_), timing(x.timing_) {}
+    ~Swap() {
+      k.scratch_ = scratch; k.Pw_ = Pw; k.epi_part_ = epi; k.Zs_ = Zs; k.Ps_ = Ps; k.eZs_ = eZs; k.ePs_ = ePs;
+      k.flag_ = flag; k.dyson_flag_ = dflag; k.hcheck_ = hcheck; k.scratch_elems_ = scratch_elems;
+      k.epi_part_elems_ = epi_elems; k.qchunk_ = qc; k.tchunk_p0_ = tp0; k.tchunk_sigma_ = tsig; k.timing_ = timing;
+    }
+  } restore(*this);
+  char* base = static_cast<char*>(ws);
+  flag_ = reinterpret_cast<int*>(base);
+  dyson_flag_ = reinterpret_cast<int*>(base + 4);
+  hcheck_ = reinterpret_cast<unsigned long long*>(base + 16);
+  size_t off = kWsAlign;
+  epi_part_ = p.epi ? reinterpret_cast<double2*>(base + off) : nullptr;
+  epi_part_elems_ = (int64_t)(p.epi / sizeof(double2));
+  off += p.epi;
+  Pw_ = reinterpret_cast<double2*>(base + off);
+  off += p.dyson;
+  scratch_ = reinterpret_cast<double2*>(base + off);
+  scratch_elems_ = (int64_t)p.scratch_elems;
+  off += p.scratch;
+  Zs_ = Ps_ = split ? reinterpret_cast<int8_t*>(base + off) : nullptr;
+  off += p.slices;
+  eZs_ = ePs_ = split ? reinterpret_cast<int*>(base + off) : nullptr;
+  qchunk_ = qc;
+  tchunk_p0_ = tchunk_sigma_ = tc;
+  timing_ = false;
+  stored_valid_ = false;
+  leak_valid_ = false;
+
+  detect_hermitian_G(G);
+  hipLaunchKernelGGL(zero_int_kernel, dim3(1), dim3(64), 0, stream_, flag_, (int64_t)1);
+  GWHIP_CHECK(hipGetLastError());
+  enqueue_eval(G, Sigma, precision, flag_);
+  GWHIP_CHECK(hipStreamSynchronize(stream_));
+  int flags[2] = {0, 0};
+  GWHIP_CHECK(hipMemcpy(flags, base, sizeof(flags), hipMemcpyDeviceToHost));
+  if (flags[0]) throw status_error(GWHIP_ERR_NONFINITE, "Sigma contains non-finite values");
+  if (flags[1])
+    throw status_error(GWHIP_ERR_NOT_POSITIVE_DEFINITE,
+                       "I - P0(i Omega) is not positive definite for some (q, Omega): a pivot of its factorisation "
+                       "is not positive (P0 is not the polarisation of a physical G)");
+}
+
 void gw_hip_kernel::set_leakage_capture(bool on) {
   GWHIP_CHECK(hipSetDevice(device_));
   if (on && !leak_P0_) leak_P0_ = dalloc<double2>((size_t)(d_.nts / 2) * d_.NQ * d_.NQ);
@@ -1785,3 +1894,20 @@ int gwhip_debug_stage(gwhip_handle* h, const double* G, int last_stage) {
 }
 
 }  // extern "C"
+
+namespace gwhip {
+// Used by the no-allocation entry point (gwhip_noalloc.hip), which owns the device workspace.
+int eval_sigma_in_workspace(gwhip_handle* h, const void* G, void* Sigma, int precision, void* ws, size_t ws_bytes) {
+  if (!h) return GWHIP_ERR_ARG;
+  return guarded(h, [&] { h->k->solve_device_in((const double2*)G, (double2*)Sigma, precision, ws, ws_bytes); });
+}
+int workspace_bytes_needed(const gwhip_handle* h, int precision, size_t* bytes) {
+  if (!h || !bytes || !precision_valid(precision)) return GWHIP_ERR_ARG;
+  *bytes = h->k->min_workspace_bytes(precision);
+  return GWHIP_OK;
+}
+void set_handle_error(gwhip_handle* h, const char* msg) {
+  if (h) h->err = msg;
+}
+int handle_device(const gwhip_handle* h) { return h ? h->k->device() : -1; }
+}  // namespace gwhip

@@ -105,6 +105,15 @@ public:
   // after gathering instead
   void solve_device(const double2* G_dev, double2* Sigma_dev, int precision, bool q0_apply = true);
   void solve_host(const double2* G, double2* Sigma, int precision);
+  // As solve_device, with every transient buffer (orbital scratch, Dyson work and panels, int8 slices
+  // of Z and P-tilde, Sigma epilogue partials, status flags) carved from the device memory
+  // [ws, ws + ws_bytes) instead of the handle's own workspace: no allocation, no timing events.
+  // The tau and q chunks shrink to fit; GWHIP_ERR_UNSUPPORTED when one tau point and one q do not.
+  // Needs the persistent stores ready (V; V's int8 slices for the split modes, built by an earlier
+  // evaluation in a split mode); the q -> 0 correction and the orbital GEMM form are not run here.
+  void solve_device_in(const double2* G_dev, double2* Sigma_dev, int precision, void* ws, size_t ws_bytes);
+  // bytes solve_device_in needs at the smallest chunks (one tau point, one q) for `precision`
+  size_t min_workspace_bytes(int precision) const;
   // test hook: stop after stage 1 (P0) or 2 (P-tilde); stage 3 runs Sigma as usual
   void run_stages_host(const double2* G, int last_stage);
   void copy_P_tilde(double2* host) const;
@@ -300,6 +309,8 @@ private:
 
   template <typename T>
   T* dalloc(size_t count);
+  struct WsPlan;
+  WsPlan ws_plan(int64_t tc, int64_t qc, bool split) const;
   void build_maps();
   void upload_maps(const std::vector<int>& kpq, const std::vector<int>& kmq);
   void v_changed();  // V was (re)written: drop derived stores

@@ -97,6 +97,7 @@ def load() -> ctypes.CDLL:
     lib.gwhip_generate_V.argtypes = [P, ctypes.c_uint64, ctypes.c_double]
     lib.gwhip_eval_sigma.argtypes = [P, P, P, ctypes.c_int]
     lib.gwhip_eval_sigma_device.argtypes = [P, P, P, ctypes.c_int]
+    lib.gwhip_eval_sigma_device_noalloc.argtypes = [P, P, P, ctypes.c_int]
     lib.gwhip_get_P_tilde.argtypes = [P, P]
     lib.gwhip_timings.argtypes = [P, ctypes.POINTER(ctypes.c_double), ctypes.c_int]
     lib.gwhip_kernel_stats.argtypes = [P, ctypes.c_int, ctypes.POINTER(ctypes.c_int64),
@@ -427,6 +428,12 @@ class GwHipKernel:
     def solve_device(self, G_ptr: int, S_ptr: int, precision: int = PREC_FP64):
         self._check(self.lib.gwhip_eval_sigma_device(self.h, ctypes.c_void_p(G_ptr), ctypes.c_void_p(S_ptr),
                                                      precision))
+
+    def solve_device_noalloc(self, G_ptr: int, S_ptr: int, precision: int = PREC_FP64):
+        """gwhip_eval_sigma_device_noalloc: the same evaluation inside the library's fixed device
+        workspace (no allocation)."""
+        self._check(self.lib.gwhip_eval_sigma_device_noalloc(self.h, ctypes.c_void_p(G_ptr), ctypes.c_void_p(S_ptr),
+                                                             precision))
 
     def set_vjp_store(self, keep: bool):
         self._check(self.lib.gwhip_set_vjp_store(self.h, 1 if keep else 0))

@@ -136,6 +136,24 @@ int gwhip_eval_sigma(gwhip_handle* h, const double* G, double* Sigma, int precis
  * Synchronous with respect to the host. */
 int gwhip_eval_sigma_device(gwhip_handle* h, const void* G_dev, void* Sigma_dev, int precision);
 
+/* Size of the device workspace of gwhip_eval_sigma_device_noalloc, per GPU: 256 MiB. */
+#define GWHIP_NOALLOC_WORKSPACE_BYTES ((size_t)256 << 20)
+
+/* As gwhip_eval_sigma_device, without allocating: no hipMalloc / hipFree or allocating library call
+ * is made.  Every transient buffer of the evaluation (orbital scratch, Dyson work and panels, int8
+ * slices of Z and P-tilde, Sigma epilogue partials, status flags) lives in one file-scope device
+ * array of GWHIP_NOALLOC_WORKSPACE_BYTES per GPU, held from the first call on that GPU until the
+ * process ends; the tau and q chunks shrink to fit it.  The handle's persistent state must exist:
+ * grid and V (gwhip_set_grid, gwhip_upload_V / gwhip_generate_V) and, for the split modes, V's int8
+ * slice stores, which the first gwhip_eval_sigma_device or gwhip_autotune in a split mode builds.
+ * GWHIP_ERR_UNSUPPORTED when one tau point and one q do not fit the workspace (nk = 64, nao = 26,
+ * naux = 300 needs about 0.64 GB in FP64 and does not), for nao > 48 (the orbital GEMM form) and
+ * with the q -> 0 correction on.  Results are bit-identical to gwhip_eval_sigma_device.  All calls on one
+ * GPU share its array: calls that overlap on one GPU (several host threads, several handles) race
+ * and must be serialised by the caller.
+ * No counterpart in the reference: for callers that cannot allocate device memory. */
+int gwhip_eval_sigma_device_noalloc(gwhip_handle* h, const void* G_dev, void* Sigma_dev, int precision);
+
 /* ---- launch-parameter autotuning ------------------------------------------------------
  * Time the candidate launch parameters of this system on its own operands (G in device memory)
  * and keep the fastest: the FP64 GEMM tiles of the P0 and Sigma contractions (0: 64x64,
