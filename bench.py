@@ -245,7 +245,7 @@ def main():
     ap.add_argument("--steps", type=int, default=2)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--precision", choices=["fp64", "split"], default="split")
-    ap.add_argument("--slices", type=int, default=5, help="int8 slices per operand in split precision")
+    ap.add_argument("--slices", type=int, default=4, help="int8 slices per operand in split precision")
     ap.add_argument("--no-check", action="store_true", help="skip the FP64 reference evaluation")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--vjp", action="store_true",

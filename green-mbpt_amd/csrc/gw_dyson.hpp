@@ -426,8 +426,7 @@ private:
     const int64_t n = d_.nso;
     for (int64_t m0 = 0; m0 < nmat; m0 += 65535) {
       const int64_t mc = std::min<int64_t>(65535, nmat - m0);
-      hipLaunchKernelGGL(hermitize_kernel, dim3((unsigned)ceil_div(n * n, 256), (unsigned)mc), dim3(256), 0, stream_,
-                         X + m0 * n * n, (int)n, n * n);
+      ::gwhip::hermitize(X + m0 * n * n, (int)n, n * n, mc, stream_);
       GWHIP_CHECK(hipGetLastError());
     }
   }

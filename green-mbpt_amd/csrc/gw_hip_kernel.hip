@@ -641,8 +641,7 @@ void gw_hip_kernel::p0_one_q(int64_t q, const double2* G, bool split, double2* P
     hipLaunchKernelGGL(mirror_upper_kernel, dim3((unsigned)ceil_div(NQ2, 256), (unsigned)half), dim3(256), 0, stream_,
                        Pq, (int)NQ, NQ2);
   else
-    hipLaunchKernelGGL(hermitize_kernel, dim3((unsigned)ceil_div(NQ2, 256), (unsigned)half), dim3(256), 0, stream_, Pq,
-                       (int)NQ, NQ2);
+    hermitize(Pq, (int)NQ, NQ2, half, stream_);
   GWHIP_CHECK(hipGetLastError());
 }
 
@@ -773,8 +772,7 @@ void gw_hip_kernel::stage_dyson(int64_t nq, const int* q0_row) {
                        (int)NQ, nmat);
     GWHIP_CHECK(hipGetLastError());
     timed(K_DYSON_INV, [&] { block_inverse(W1, nmat); });
-    hipLaunchKernelGGL(hermitize_kernel, dim3((unsigned)ceil_div(NQ2, 256), (unsigned)nmat), dim3(256), 0, stream_, W1,
-                       (int)NQ, NQ2);
+    hermitize(W1, (int)NQ, NQ2, nmat, stream_);
     GWHIP_CHECK(hipGetLastError());
     {  // P-tilde(i Omega) = Herm(A^-1) P0(i Omega), one product per matrix
       ZGemmArgs a{};
@@ -1137,7 +1135,7 @@ void gw_hip_kernel::autotune(const double2* G, int precision) {
     }
     // block tile of each int8 GEMM, then the row-tile group of the L2-aware tile order
     const int groups[7] = {1, 2, 4, 8, 16, 32, 64};
-    const int tiles[2] = {2, 3};
+    const int tiles[2] = {2, 4};
     p0_prep(0, G, 0, tcp, tune_.qpw);
     p0_slice(tcp);
     tune_.p0_oz_tile = best_of(2, tiles, [&](int c) {
@@ -1404,7 +1402,7 @@ void debug_zgemm(int M, int N, int K, const double2* A, const double2* B, double
   (void)hipFree(dC);
 }
 
-void debug_zgemm_split(int M, int N, int K, const double2* A, const double2* B, double2* C, int ns) {
+void debug_zgemm_split(int M, int N, int K, const double2* A, const double2* B, double2* C, int ns, int tile) {
   if (M <= 0 || N <= 0 || K <= 0 || !A || !B || !C) throw status_error(GWHIP_ERR_ARG, "bad debug_zgemm_split arguments");
   GWHIP_CHECK(hipSetDevice(0));
   const int kc = (int)ceil_div(K, 64) * 64;
@@ -1436,6 +1434,7 @@ void debug_zgemm_split(int M, int N, int K, const double2* A, const double2* B, 
   o.B = sB; o.sBni = OZ_PLANES * ns * kc; o.eB = eB; o.seBni = 1;
   o.C = dC; o.sCm = N; o.sCni = 1;
   o.alpha = make_double2(1.0, 0.0);
+  o.tile = tile;
   zgemm_ozaki(o, ns, nullptr);
   GWHIP_CHECK(hipDeviceSynchronize());
   GWHIP_CHECK(hipMemcpy(C, dC, (size_t)M * N * sizeof(double2), hipMemcpyDeviceToHost));
@@ -1863,10 +1862,11 @@ int gwhip_debug_zgemm(int M, int N, int K, const double* A, const double* B, dou
   });
 }
 
-int gwhip_debug_zgemm_split(int M, int N, int K, const double* A, const double* B, double* C, int slices) {
+int gwhip_debug_zgemm_split(int M, int N, int K, const double* A, const double* B, double* C, int slices, int tile) {
   gwhip_handle tmp;
   return guarded(&tmp, [&] {
-    gwhip::debug_zgemm_split(M, N, K, (const double2*)A, (const double2*)B, (double2*)C, slices);
+    if (tile != 2 && tile != 4) throw gwhip::status_error(GWHIP_ERR_ARG, "int8 GEMM tile must be 2 or 4");
+    gwhip::debug_zgemm_split(M, N, K, (const double2*)A, (const double2*)B, (double2*)C, slices, tile);
   });
 }
 

@@ -41,10 +41,9 @@ struct KernelStat {
 // writes the int8 slices itself.
 struct Tuning {
   int p0_shape = 0, s_shape = 0, qpw = 4;
-  int p0_group = 8, s_group = 8;  // split precision: row tiles per L2 group of the int8 GEMMs
-  int p0_oz_tile = 2, s_oz_tile = 2;  // int8 GEMM block tile: 2 (32 x 64, A through a register-staged
-                                      // LDS panel, B fragments from global memory) or 3 (64 x 64, both
-                                      // operands through LDS by direct global -> LDS loads)
+  int p0_group = 1, s_group = 2;  // split precision: row tiles per L2 group of the int8 GEMMs
+  int p0_oz_tile = 2, s_oz_tile = 2;  // int8 GEMM block tile: 2 (32 x 64, 16 x 16 x 64 MFMAs) or
+                                      // 4 (64 x 64, 32 x 32 x 32 MFMAs, staged by LDS DMA)
   int p0_fused = 1;  // split precision, one spin block: P0 orbital kernel writes int8 slices directly
   int orb_gemm = 0;  // orbital contractions as batched FP64 GEMMs (always so for nao > 48)
   int p0_kchunks = 4;  // split precision: k chunks per launch of the int8 P0 GEMM (0: all in one), so
@@ -57,7 +56,7 @@ struct Tuning {
   }
   bool from_array(const int* v) {
     if (v[0] < 0 || v[0] > 2 || v[1] < 0 || v[1] > 2 || (v[2] != 1 && v[2] != 2 && v[2] != 4 && v[2] != 8) ||
-        v[3] < 1 || v[4] < 1 || (v[5] != 2 && v[5] != 3) || (v[6] != 2 && v[6] != 3) || v[7] < 0 || v[7] > 1 || v[8] < 0 ||
+        v[3] < 1 || v[4] < 1 || (v[5] != 2 && v[5] != 4) || (v[6] != 2 && v[6] != 4) || v[7] < 0 || v[7] > 1 || v[8] < 0 ||
         v[8] > 1 || v[9] < 0)
       return false;
     p0_shape = v[0]; s_shape = v[1]; qpw = v[2]; p0_group = v[3]; s_group = v[4]; p0_oz_tile = v[5]; s_oz_tile = v[6];
@@ -234,7 +233,7 @@ private:
   int* dyson_flag_ = nullptr;  // set by the Dyson inverse when a pivot is not positive
   // split precision: int8 slice stores of conj(V) for the P0 and Sigma contractions, and the
   // per-evaluation slices of Z and P-tilde (zgemm_ozaki.hpp)
-  int split_ns_ = 5, slices_ns_ = 0;
+  int split_ns_ = 4, slices_ns_ = 0;
   int64_t kc_p0_ = 0, kc_s_ = 0;
   int8_t* Vs_p0_ = nullptr;   // [nq][NQ][nk][2][ns][kc_p0]
   int* eVs_p0_ = nullptr;     // [nq][NQ][nk]

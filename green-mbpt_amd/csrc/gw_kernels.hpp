@@ -22,18 +22,52 @@ __device__ __forceinline__ int64_t g_index(const DevDims& d, int t, int is, int 
 }
 
 // ---------------------------------------------------------------------------------------
-// X = (X + X^H)/2 for a batch of N x N matrices (common_defs.h:104-115).  One thread per
-// (i <= j) pair.
-__global__ void hermitize_kernel(double2* X, int N, int64_t stride) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (int64_t)N * N) return;
-  const int i = (int)(e / N), j = (int)(e - (e / N) * N);
-  if (i > j) return;
+// X = (X + X^H)/2 for a batch of N x N matrices (common_defs.h:104-115), with coalesced accesses:
+// block (y = matrix) takes the tile pair (I, J), I <= J,
+// of 32 x 32 tiles (blockIdx.x enumerates the upper tile triangle), stages tile (I, J) and tile (J, I)
+// in LDS, and writes h = (A[a][b] + conj(B[b][a]))/2 to (I, J) and conj(h) to (J, I) (on a diagonal
+// tile the same formula is Hermitian by construction, with a real diagonal).
+constexpr int HERM_TILE = 32;
+__global__ __launch_bounds__(256) void hermitize_tiled_kernel(double2* X, int N, int64_t stride) {
+  __shared__ double2 ta[HERM_TILE][HERM_TILE + 1], tb[HERM_TILE][HERM_TILE + 1];
+  const int T = (N + HERM_TILE - 1) / HERM_TILE;
+  // blockIdx.x -> (I, J) with I <= J, row-major over the upper triangle
+  int I = 0, rem = (int)blockIdx.x;
+  while (rem >= T - I) {
+    rem -= T - I;
+    ++I;
+  }
+  const int J = I + rem;
   double2* M = X + (int64_t)blockIdx.y * stride;
-  const double2 u = M[(int64_t)i * N + j], l = M[(int64_t)j * N + i];
-  const double2 h = make_double2(0.5 * (u.x + l.x), 0.5 * (u.y - l.y));
-  M[(int64_t)i * N + j] = h;
-  M[(int64_t)j * N + i] = make_double2(h.x, -h.y);
+  const int i0 = I * HERM_TILE, j0 = J * HERM_TILE;
+  for (int e = threadIdx.x; e < HERM_TILE * HERM_TILE; e += blockDim.x) {
+    const int a = e / HERM_TILE, b = e % HERM_TILE;
+    if (i0 + a < N && j0 + b < N) ta[a][b] = M[(int64_t)(i0 + a) * N + j0 + b];
+    if (j0 + a < N && i0 + b < N) tb[a][b] = M[(int64_t)(j0 + a) * N + i0 + b];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < HERM_TILE * HERM_TILE; e += blockDim.x) {
+    const int a = e / HERM_TILE, b = e % HERM_TILE;
+    if (i0 + a >= N || j0 + b >= N) continue;
+    const double2 u = ta[a][b], l = tb[b][a];
+    ta[a][b] = make_double2(0.5 * (u.x + l.x), 0.5 * (u.y - l.y));
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < HERM_TILE * HERM_TILE; e += blockDim.x) {
+    const int a = e / HERM_TILE, b = e % HERM_TILE;
+    if (i0 + a < N && j0 + b < N) M[(int64_t)(i0 + a) * N + j0 + b] = ta[a][b];
+    if (I != J && j0 + a < N && i0 + b < N) {
+      const double2 h = ta[b][a];
+      M[(int64_t)(j0 + a) * N + i0 + b] = make_double2(h.x, -h.y);
+    }
+  }
+}
+
+// (X + X^H)/2 for nmat N x N matrices at `stride` apart (nmat <= 65535).
+inline void hermitize(double2* X, int N, int64_t stride, int64_t nmat, hipStream_t st) {
+  const int T = (N + HERM_TILE - 1) / HERM_TILE;
+  hipLaunchKernelGGL(hermitize_tiled_kernel, dim3((unsigned)(T * (T + 1) / 2), (unsigned)nmat), dim3(256), 0, st, X, N,
+                     stride);
 }
 
 // Upper triangle -> Hermitian matrix: X[i][j] = conj(X[j][i]) below the diagonal, real diagonal.

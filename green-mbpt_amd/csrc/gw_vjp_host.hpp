@@ -276,8 +276,7 @@ void gw_hip_kernel::vjp_device(const double2* G, const double2* Sbar, double2* G
     GWHIP_CHECK(hipGetLastError());
     gj_inverse(vBinv_, nwb);
     GWHIP_CHECK(hipMemcpyAsync(vAinv_, vBinv_, nwb * NQ2 * sizeof(double2), hipMemcpyDeviceToDevice, stream_));
-    hipLaunchKernelGGL(hermitize_kernel, dim3((unsigned)ceil_div(NQ2, 256), (unsigned)nwb), dim3(256), 0, stream_,
-                       vAinv_, (int)NQ, NQ2);
+    hermitize(vAinv_, (int)NQ, NQ2, nwb, stream_);
     GWHIP_CHECK(hipGetLastError());
     const double2* Pt = P_ + q * nts * NQ2;
     if (!stored) {
@@ -341,15 +340,13 @@ void gw_hip_kernel::vjp_device(const double2* G, const double2* Sbar, double2* G
     // ---- Dyson adjoint
     stack_transform(stream_, Ttb_, (int)nts, (int)nwb, true, vPtbar_, NQ2, vM1_);              // Pwbar
     mat_gemm(stream_, (int)NQ, nwb, vM1_, false, false, vP0w_, true, true, vM2_, 1.0, false);  // Pwbar P0w^H
-    hipLaunchKernelGGL(hermitize_kernel, dim3((unsigned)ceil_div(NQ2, 256), (unsigned)nwb), dim3(256), 0, stream_, vM2_,
-                       (int)NQ, NQ2);
+    hermitize(vM2_, (int)NQ, NQ2, nwb, stream_);
     GWHIP_CHECK(hipGetLastError());
     mat_gemm(stream_, (int)NQ, nwb, vAinv_, true, true, vM1_, false, false, vM3_, 1.0, false);  // Ainv^H Pwbar
     mat_gemm(stream_, (int)NQ, nwb, vBinv_, true, true, vM2_, false, false, vM1_, 1.0, false);  // Binv^H H(.)
     mat_gemm(stream_, (int)NQ, nwb, vM1_, false, false, vBinv_, true, true, vM3_, 1.0, true);   // (.) Binv^H
     stack_transform(stream_, Tbt_fold_, (int)nwb, (int)half, true, vM3_, NQ2, vPbar_);          // T_fold^H P0wbar
-    hipLaunchKernelGGL(hermitize_kernel, dim3((unsigned)ceil_div(NQ2, 256), (unsigned)half), dim3(256), 0, stream_,
-                       vPbar_, (int)NQ, NQ2);
+    hermitize(vPbar_, (int)NQ, NQ2, half, stream_);
     GWHIP_CHECK(hipGetLastError());
 
     // ---- P0 part: Gbar(beta - tau, k) and Gbar(tau, k + q)

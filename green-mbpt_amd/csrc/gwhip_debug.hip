@@ -137,9 +137,38 @@ __global__ __launch_bounds__(256) void lowp_peak_kernel(double* out, int iters) 
   out[(int64_t)blockIdx.x * blockDim.x + threadIdx.x] = s;
 }
 
+typedef int v16i_t __attribute__((ext_vector_type(16)));
+
+// int8 MFMA shape probe: v_mfma_i32_32x32x32_i8 (65536 ops) with NCH independent 32 x 32 accumulator
+// chains per wave, or (WIDE = 0) v_mfma_i32_16x16x64_i8 (32768 ops) with NCH chains.
+template <int WIDE, int NCH>
+__global__ __launch_bounds__(256) void i8_shape_kernel(double* out, int iters) {
+  v4i_t ai = {(int)threadIdx.x, 3, 5, 7}, bi = {11, (int)blockIdx.x, 13, 17};
+  v16i_t cw[WIDE ? NCH : 1];
+  v4i_t cn[WIDE ? 1 : NCH];
+#pragma unroll
+  for (int j = 0; j < (WIDE ? NCH : 1); ++j)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) cw[j][r] = 0;
+#pragma unroll
+  for (int j = 0; j < (WIDE ? 1 : NCH); ++j) cn[j] = v4i_t{0, 0, 0, 0};
+  for (int i = 0; i < iters; ++i) {
+#pragma unroll
+    for (int j = 0; j < NCH; ++j) {
+      if (WIDE) cw[j] = __builtin_amdgcn_mfma_i32_32x32x32_i8(ai, bi, cw[j], 0, 0, 0);
+      else cn[j] = __builtin_amdgcn_mfma_i32_16x16x64_i8(ai, bi, cn[j], 0, 0, 0);
+    }
+  }
+  double s = 0.0;
+#pragma unroll
+  for (int j = 0; j < (WIDE ? NCH : 1); ++j) s += (double)cw[j][0];
+#pragma unroll
+  for (int j = 0; j < (WIDE ? 1 : NCH); ++j) s += (double)cn[j][0];
+  out[(int64_t)blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
 double debug_mfma_peak(int chains, int wps, int iters) {
-  if (chains < 1 || (chains > 8 && chains != 10 && chains != 11 && chains != 12) || chains == 7 || wps < 1 || wps > 8 ||
-      iters < 1)
+  if (chains < 1 || (chains > 8 && (chains < 10 || chains > 15)) || chains == 7 || wps < 1 || wps > 8 || iters < 1)
     throw status_error(GWHIP_ERR_ARG, "bad mfma probe args");
   GWHIP_CHECK(hipSetDevice(0));
   hipDeviceProp_t prop;
@@ -152,6 +181,9 @@ double debug_mfma_peak(int chains, int wps, int iters) {
       case 10: hipLaunchKernelGGL(lowp_peak_kernel<0>, dim3(nblocks), dim3(256), 0, nullptr, out, iters); break;
       case 11: hipLaunchKernelGGL(lowp_peak_kernel<1>, dim3(nblocks), dim3(256), 0, nullptr, out, iters); break;
       case 12: hipLaunchKernelGGL(valu_cgemm_probe_kernel, dim3(nblocks), dim3(256), 0, nullptr, out, iters); break;
+      case 13: hipLaunchKernelGGL((i8_shape_kernel<1, 4>), dim3(nblocks), dim3(256), 0, nullptr, out, iters); break;
+      case 14: hipLaunchKernelGGL((i8_shape_kernel<1, 8>), dim3(nblocks), dim3(256), 0, nullptr, out, iters); break;
+      case 15: hipLaunchKernelGGL((i8_shape_kernel<0, 16>), dim3(nblocks), dim3(256), 0, nullptr, out, iters); break;
       case 5: hipLaunchKernelGGL(valu_peak_kernel<1>, dim3(nblocks), dim3(256), 0, nullptr, out, iters); break;
       case 6: hipLaunchKernelGGL(valu_peak_kernel<2>, dim3(nblocks), dim3(256), 0, nullptr, out, iters); break;
       case 1: hipLaunchKernelGGL(mfma_peak_kernel<1>, dim3(nblocks), dim3(256), 0, nullptr, out, iters); break;
@@ -174,6 +206,9 @@ double debug_mfma_peak(int chains, int wps, int iters) {
   if (chains == 10) per_wave_iter = 8 * 16.0 * 16 * 64 * 2;
   else if (chains == 11) per_wave_iter = 8 * 16.0 * 16 * 32 * 2;
   else if (chains == 12) per_wave_iter = 32 * 64 * 2.0 * 64;  // 32 k x 64 FMA per lane
+  else if (chains == 13) per_wave_iter = 4 * 32.0 * 32 * 32 * 2;
+  else if (chains == 14) per_wave_iter = 8 * 32.0 * 32 * 32 * 2;
+  else if (chains == 15) per_wave_iter = 16 * 16.0 * 16 * 64 * 2;
   else if (chains == 5) per_wave_iter = 32.0 * 64 * 2;                   // 32 FMA per lane
   else if (chains == 6) per_wave_iter = 32.0 * 64 * 2 + 4 * 2048.0;  // + 4 MFMA
   else per_wave_iter = (chains == 3 ? 4 : (chains > 4 ? 8 : chains)) * 2048.0;

@@ -182,7 +182,8 @@ struct OzGemmArgs {
   int64_t nblocks, per_xcd;
   int group_m;  // row tiles (over batch x M) swept together across all column tiles; 0 = heuristic
   int upper;    // 1: skip block tiles that lie wholly below the diagonal (Hermitian products)
-  int tile;     // block tile: 2 = 32 x 64 (zgemm_ozaki_pipe_kernel), 3 = 64 x 64 (zgemm_ozaki_lds_kernel)
+  int tile;     // block tile: 2 = 32 x 64 of 16 x 16 x 64 MFMAs (zgemm_ozaki_pipe_kernel), 4 = 64 x 64 of
+                // 32 x 32 x 32 MFMAs (zgemm_ozaki32_kernel); 0 = 2
 };
 
 // Output layout of v_mfma_i32_16x16x64_i8: register r of lane l holds row
@@ -402,100 +403,99 @@ __device__ __forceinline__ bool oz_tile_of(const OzGemmArgs& p, int64_t& b, int&
   return true;
 }
 
-// Block tile 64 x 64 (complex) for eight waves in a 2 x 4 grid of 32 x 16 wave tiles, both operands
-// staged in LDS by direct global -> LDS loads (global_load_lds_dwordx4), double-buffered in two
-// static LDS arrays (distinct objects, so the compiler's counters let a wave read one buffer while
-// the loads into the other are in flight).  The loads of K step g + 1 go out at the start of step
-// g and land while its MFMAs run; the barrier closing the step waits for them.
-//
-// A stage is a list of 1 KB chunks: A's (row block of 16, plane-slice) then B's (column block of
-// 16, plane-slice), each in MFMA fragment order -- lane l's 16 bytes are row (column) l % 16, K
-// bytes 16 (l / 16) .. +16 of the step -- so one wave-wide load fills a chunk and a fragment read is
-// one contiguous 1 KB ds_read_b128 (no bank conflicts).  Waves 0-3 load A's row blocks, waves
-// 4-7 B's column blocks, NPS chunks each.  Rows past M / columns past N load the last valid row /
-// column: they only feed results that are never stored.
-constexpr int OZL_BM = 64, OZL_BN = 64;
+// ------------------------------------------------------------------------------ 32 x 32 x 32 tile
+// Output layout of v_mfma_i32_32x32x32_i8: register r of lane l holds row OZ32_ROW(l/32, r),
+// column l%32 (checked by the identity-probe test); each operand lane l holds row (A) / column (B)
+// l%32 and K bytes 16 (l/32) .. +16 of the 32-byte K step.
+#define OZ32_ROW(g, r) (8 * ((r) >> 2) + 4 * (g) + ((r)&3))
+typedef int v16i __attribute__((ext_vector_type(16)));
+
+// Block tile 64 x 64 (complex), four waves in a 2 x 2 grid of 32 x 32 wave tiles on the
+// v_mfma_i32_32x32x32_i8 matrix cores, which keep their pipeline full with a handful of independent
+// accumulators (the 16 x 16 x 64 form needs about sixteen in flight per wave).  Complex products in
+// the four-product form on the two planes of the slice store:
+//   Re C = Re A Re B + (-Im A) Im B,   Im C = Re A Im B + Im A Re B,
+// -Im A negated bytewise in registers; per slice level two int32 accumulators of 16 registers.
+// Operands go to LDS by direct global -> LDS loads (global_load_lds_dwordx4), double-buffered in two
+// static arrays (distinct objects, so a wave reads one while the loads into the other are in
+// flight); the loads of K step g + 1 leave at the start of step g, and the barrier closing step g
+// waits for them.  A stage is a list of 1 KB chunks -- A's (row block of 32, plane-slice), then B's
+// (column block of 32, plane-slice) -- each in MFMA fragment order, so one wave-wide load fills a
+// chunk and a fragment read is one contiguous ds_read_b128 per lane.  Waves 0-1 load A's row blocks,
+// waves 2-3 B's column blocks.  Rows past M / columns past N load the last valid row / column: they
+// only feed results that are never stored.  Two blocks per CU.
+constexpr int OZ32_BM = 64, OZ32_BN = 64, OZ32_KS = 32;
 
 template <int NS>
-__global__ __launch_bounds__(512, 1) void zgemm_ozaki_lds_kernel(OzGemmArgs p) {
-  constexpr int NPS = OZ_PLANES * NS;
-  constexpr int STAGE = 8 * NPS * 1024;  // 4 row blocks + 4 column blocks, NPS chunks each
+__global__ __launch_bounds__(256, 2) void zgemm_ozaki32_kernel(OzGemmArgs p) {
+  constexpr int NPU = 2 * NS;                   // plane-slices read: Re and Im
+  constexpr int STAGE = 4 * NPU * 1024;         // 2 row blocks + 2 column blocks, NPU chunks each
   __shared__ __attribute__((aligned(1024))) int8_t oz_s0[STAGE];
   __shared__ __attribute__((aligned(1024))) int8_t oz_s1[STAGE];
 
   int64_t b;
   int tm, tn;
   if (!oz_tile_of(p, b, tm, tn)) return;
-  const int m0 = tm * OZL_BM, n0 = tn * OZL_BN;
-  if (p.upper && m0 >= n0 + OZL_BN) return;
+  const int m0 = tm * OZ32_BM, n0 = tn * OZ32_BN;
+  if (p.upper && m0 >= n0 + OZ32_BN) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int lrow = lane & 15, lg = lane >> 4;
-  const int wm = wave & 1, wn = wave >> 1;  // rows 32 wm .. +32, columns 16 wn .. +16
+  const int l32 = lane & 31, lg = lane >> 5;
+  const int wm = wave & 1, wn = wave >> 1;  // rows 32 wm .. +32, columns 32 wn .. +32
 
-  // this lane's source row (waves 0-3: A row block `wave`) or column (waves 4-7: B column block)
-  const bool loads_a = wave < 4;
-  const int blk = wave & 3;
+  // this lane's source row (waves 0-1: A row block `wave`) or column (waves 2-3: B column block)
+  const bool loads_a = wave < 2;
+  const int blk = wave & 1;
   const int8_t* src;
   int64_t s_chunk;
   if (loads_a) {
-    const int m = min(m0 + 16 * blk + lrow, p.M - 1);
+    const int m = min(m0 + 32 * blk + l32, p.M - 1);
     src = p.A + b * p.sAb + (int64_t)m * p.sAm + 16 * lg;
     s_chunk = p.sAc;
   } else {
-    const int n = min(n0 + 16 * blk + lrow, p.N - 1);
+    const int n = min(n0 + 32 * blk + l32, p.N - 1);
     const int no = n / p.nin;
     src = p.B + b * p.sBb + (int64_t)no * p.sBno + (int64_t)(n - no * p.nin) * p.sBni + 16 * lg;
     s_chunk = p.sBc;
   }
-  const int chunk0 = (loads_a ? 0 : 4 * NPS) + blk * NPS;  // first of this wave's NPS chunks
-  const int nks = p.kc / OZ_KS, nsteps = p.nchunk * nks;
+  const int chunk0 = (loads_a ? 0 : 2 * NPU) + blk * NPU;  // first of this wave's NPU chunks
+  const int nks = p.kc / OZ32_KS, nsteps = p.nchunk * nks;
   auto load_stage = [&](int8_t* st, int g) {
-    const int c = g / nks, ks = (g - c * nks) * OZ_KS;
+    const int c = g / nks, ks = (g - c * nks) * OZ32_KS;
     const int8_t* s0 = src + (int64_t)c * s_chunk + ks;
 #pragma unroll
-    for (int ps = 0; ps < NPS; ++ps)
+    for (int ps = 0; ps < NPU; ++ps)
       __builtin_amdgcn_global_load_lds((const void*)(s0 + (int64_t)ps * p.kc),
                                        (__attribute__((address_space(3))) void*)(st + (chunk0 + ps) * 1024), 16, 0, 0);
   };
 
-  double cr[2][4], ci[2][4];
+  double cr[16], ci[16];
 #pragma unroll
-  for (int x = 0; x < 2; ++x)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) { cr[x][r] = 0.0; ci[x][r] = 0.0; }
-  v4i ar[NS][2], ai[NS][2];  // [d][x] int32 accumulators (Re, Im) of slice level d
+  for (int r = 0; r < 16; ++r) { cr[r] = 0.0; ci[r] = 0.0; }
+  v16i ar[NS], ai[NS];  // [d] int32 accumulators (Re, Im) of slice level d
 #pragma unroll
   for (int d = 0; d < NS; ++d)
 #pragma unroll
-    for (int x = 0; x < 2; ++x) ar[d][x] = ai[d][x] = v4i{0, 0, 0, 0};
+    for (int r = 0; r < 16; ++r) { ar[d][r] = 0; ai[d][r] = 0; }
 
-  const int bcol = min(n0 + 16 * wn + lrow, p.N - 1);
+  const int bcol = min(n0 + 32 * wn + l32, p.N - 1);
   auto compute = [&](const int8_t* st, int g) {
-    v4i fb[OZ_PLANES][NS];
+    v4i fb[2][NS];
 #pragma unroll
-    for (int pl = 0; pl < OZ_PLANES; ++pl)
+    for (int pl = 0; pl < 2; ++pl)
 #pragma unroll
       for (int t = 0; t < NS; ++t)
-        fb[pl][t] = *reinterpret_cast<const v4i*>(st + (4 * NPS + wn * NPS + pl * NS + t) * 1024 + 16 * lane);
+        fb[pl][t] = *reinterpret_cast<const v4i*>(st + (2 * NPU + wn * NPU + pl * NS + t) * 1024 + 16 * lane);
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
-      v4i fa_r[2], fa_i[2], fa_n[2];
-#pragma unroll
-      for (int x = 0; x < 2; ++x) {
-        const int rb = 2 * wm + x;
-        fa_r[x] = *reinterpret_cast<const v4i*>(st + (rb * NPS + 0 * NS + s) * 1024 + 16 * lane);
-        fa_i[x] = *reinterpret_cast<const v4i*>(st + (rb * NPS + 1 * NS + s) * 1024 + 16 * lane);
-        fa_n[x] = neg_i8x16(fa_i[x]);
-      }
+      const v4i fa_r = *reinterpret_cast<const v4i*>(st + (wm * NPU + 0 * NS + s) * 1024 + 16 * lane);
+      const v4i fa_i = *reinterpret_cast<const v4i*>(st + (wm * NPU + 1 * NS + s) * 1024 + 16 * lane);
+      const v4i fa_n = neg_i8x16(fa_i);
 #pragma unroll
       for (int t = 0; t + s < NS; ++t) {
-#pragma unroll
-        for (int x = 0; x < 2; ++x) {
-          ar[s + t][x] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa_r[x], fb[0][t], ar[s + t][x], 0, 0, 0);
-          ai[s + t][x] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa_r[x], fb[1][t], ai[s + t][x], 0, 0, 0);
-          ar[s + t][x] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa_n[x], fb[1][t], ar[s + t][x], 0, 0, 0);
-          ai[s + t][x] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa_i[x], fb[0][t], ai[s + t][x], 0, 0, 0);
-        }
+        ar[s + t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa_r, fb[0][t], ar[s + t], 0, 0, 0);
+        ai[s + t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa_r, fb[1][t], ai[s + t], 0, 0, 0);
+        ar[s + t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa_n, fb[1][t], ar[s + t], 0, 0, 0);
+        ai[s + t] = __builtin_amdgcn_mfma_i32_32x32x32_i8(fa_i, fb[0][t], ai[s + t], 0, 0, 0);
       }
     }
     const int c = g / nks;
@@ -503,25 +503,23 @@ __global__ __launch_bounds__(512, 1) void zgemm_ozaki_lds_kernel(OzGemmArgs p) {
       const int no = bcol / p.nin;
       const int eb = p.eB[b * p.seBb + (int64_t)no * p.seBno + (int64_t)(bcol - no * p.nin) * p.seBni + c];
 #pragma unroll
-      for (int x = 0; x < 2; ++x)
+      for (int r = 0; r < 16; ++r) {
+        const int m = min(m0 + 32 * wm + OZ32_ROW(lg, r), p.M - 1);
+        const int ea = p.eA[b * p.seAb + (int64_t)m * p.seAm + c];
+        double sr = 0.0, si = 0.0;
 #pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int m = min(m0 + 32 * wm + 16 * x + OZ_ROW(lg, r), p.M - 1);
-          const int ea = p.eA[b * p.seAb + (int64_t)m * p.seAm + c];
-          double sr = 0.0, si = 0.0;
-#pragma unroll
-          for (int d = 0; d < NS; ++d) {  // exact integers times 254^-d
-            sr = fma((double)ar[d][x][r], oz_weight(d), sr);
-            si = fma((double)ai[d][x][r], oz_weight(d), si);
-          }
-          const double sc = ldexp(1.0 / (OZ_FIRST * OZ_FIRST), ea + eb);
-          cr[x][r] = fma(sr, sc, cr[x][r]);
-          ci[x][r] = fma(si, sc, ci[x][r]);
+        for (int d = 0; d < NS; ++d) {  // exact integers times 254^-d
+          sr = fma((double)ar[d][r], oz_weight(d), sr);
+          si = fma((double)ai[d][r], oz_weight(d), si);
         }
+        const double sc = ldexp(1.0 / (OZ_FIRST * OZ_FIRST), ea + eb);
+        cr[r] = fma(sr, sc, cr[r]);
+        ci[r] = fma(si, sc, ci[r]);
+      }
 #pragma unroll
       for (int d = 0; d < NS; ++d)
 #pragma unroll
-        for (int x = 0; x < 2; ++x) ar[d][x] = ai[d][x] = v4i{0, 0, 0, 0};
+        for (int r = 0; r < 16; ++r) { ar[d][r] = 0; ai[d][r] = 0; }
     }
   };
 
@@ -537,26 +535,24 @@ __global__ __launch_bounds__(512, 1) void zgemm_ozaki_lds_kernel(OzGemmArgs p) {
     __syncthreads();
   }
 
-  const int n = n0 + 16 * wn + lrow;
+  const int n = n0 + 32 * wn + l32;
   if (n >= p.N) return;
   const int no = n / p.nin;
   const int64_t coff = (int64_t)no * p.sCno + (int64_t)(n - no * p.nin) * p.sCni;
 #pragma unroll
-  for (int x = 0; x < 2; ++x)
-#pragma unroll
-    for (int r = 0; r < 4; ++r) {
-      const int m = m0 + 32 * wm + 16 * x + OZ_ROW(lg, r);
-      if (m >= p.M) continue;
-      const double vr = cr[x][r], vi = ci[x][r];
-      double2 v = make_double2(p.alpha.x * vr - p.alpha.y * vi, p.alpha.x * vi + p.alpha.y * vr);
-      double2* dst = p.C + b * p.sCb + (int64_t)m * p.sCm + coff;
-      if (p.accumulate) {
-        const double2 o = *dst;
-        v.x += o.x;
-        v.y += o.y;
-      }
-      *dst = v;
+  for (int r = 0; r < 16; ++r) {
+    const int m = m0 + 32 * wm + OZ32_ROW(lg, r);
+    if (m >= p.M) continue;
+    const double vr = cr[r], vi = ci[r];
+    double2 v = make_double2(p.alpha.x * vr - p.alpha.y * vi, p.alpha.x * vi + p.alpha.y * vr);
+    double2* dst = p.C + b * p.sCb + (int64_t)m * p.sCm + coff;
+    if (p.accumulate) {
+      const double2 o = *dst;
+      v.x += o.x;
+      v.y += o.y;
     }
+    *dst = v;
+  }
 }
 
 // About 64 blocks (the ones resident on one XCD) share a group when there are few column
@@ -572,8 +568,8 @@ inline size_t ozaki_pipe_lds_bytes(int ns) { return (size_t)2 * OZ_PLANES * OZ_B
 
 template <int NS>
 inline void ozaki_launch(const OzGemmArgs& p, int64_t grid, hipStream_t stream) {
-  if (p.tile == 3) {
-    hipLaunchKernelGGL(zgemm_ozaki_lds_kernel<NS>, dim3((unsigned)grid), dim3(512), 0, stream, p);
+  if (p.tile == 4) {
+    hipLaunchKernelGGL(zgemm_ozaki32_kernel<NS>, dim3((unsigned)grid), dim3(256), 0, stream, p);
     return;
   }
   const size_t lds = ozaki_pipe_lds_bytes(NS);
@@ -581,7 +577,7 @@ inline void ozaki_launch(const OzGemmArgs& p, int64_t grid, hipStream_t stream) 
   hipLaunchKernelGGL(zgemm_ozaki_pipe_kernel<NS>, dim3((unsigned)grid), dim3(256), lds, stream, p);
 }
 
-inline bool ozaki_tile_valid(int tile) { return tile == 2 || tile == 3; }
+inline bool ozaki_tile_valid(int tile) { return tile == 2 || tile == 4; }
 
 inline void zgemm_ozaki(OzGemmArgs p, int ns, hipStream_t stream) {
   if (p.M <= 0 || p.N <= 0 || p.batch <= 0) return;
@@ -589,8 +585,8 @@ inline void zgemm_ozaki(OzGemmArgs p, int ns, hipStream_t stream) {
   if (p.kc > split_kc_limit(ns)) throw std::runtime_error("zgemm_ozaki: chunk too long for exact int32 sums");
   if (p.tile == 0) p.tile = 2;
   if (!ozaki_tile_valid(p.tile)) throw std::runtime_error("zgemm_ozaki: unknown block tile");
-  p.tiles_m = (int)ceil_div(p.M, p.tile == 3 ? OZL_BM : OZ_BM);
-  p.tiles_n = (int)ceil_div(p.N, p.tile == 3 ? OZL_BN : OZ_BN);
+  p.tiles_m = (int)ceil_div(p.M, p.tile == 4 ? OZ32_BM : OZ_BM);
+  p.tiles_n = (int)ceil_div(p.N, p.tile == 4 ? OZ32_BN : OZ_BN);
   p.nblocks = (int64_t)p.batch * p.tiles_m * p.tiles_n;
   if (p.group_m <= 0) p.group_m = ozaki_default_group(p.tiles_n);
   p.group_m = (int)std::min<int64_t>(p.group_m, (int64_t)p.batch * p.tiles_m);

@@ -110,7 +110,7 @@ def load() -> ctypes.CDLL:
     lib.gwhip_debug_eval_q.argtypes = [P, P, ctypes.c_int64, ctypes.c_int, P, P, P]
     lib.gwhip_set_leakage_capture.argtypes = [P, ctypes.c_int]
     lib.gwhip_get_leakage_inputs.argtypes = [P, P, P]
-    lib.gwhip_debug_zgemm_split.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, P, ctypes.c_int]
+    lib.gwhip_debug_zgemm_split.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int, P, P, P, ctypes.c_int, ctypes.c_int]
     lib.gwhip_autotune.argtypes = [P, P, ctypes.c_int]
     IP = ctypes.POINTER(ctypes.c_int)
     lib.gwhip_get_tuning.argtypes = [P, IP, ctypes.c_int]
@@ -291,15 +291,15 @@ def debug_zgemm(A: np.ndarray, B: np.ndarray, conjA: bool = False, conjB: bool =
     return c
 
 
-def debug_zgemm_split(A: np.ndarray, B: np.ndarray, slices: int = 4) -> np.ndarray:
-    """The split-precision (int8 slice) GEMM on host arrays (test hook)."""
+def debug_zgemm_split(A: np.ndarray, B: np.ndarray, slices: int = 4, tile: int = 2) -> np.ndarray:
+    """The split-precision (int8 slice) GEMM on host arrays with block tile `tile` (2 or 4; test hook)."""
     lib = load()
     a = np.ascontiguousarray(A, dtype=np.complex128)
     b = np.ascontiguousarray(B, dtype=np.complex128)
     M, K = a.shape
     N = b.shape[1]
     c = np.empty((M, N), dtype=np.complex128)
-    st = lib.gwhip_debug_zgemm_split(M, N, K, a.ctypes.data, b.ctypes.data, c.ctypes.data, slices)
+    st = lib.gwhip_debug_zgemm_split(M, N, K, a.ctypes.data, b.ctypes.data, c.ctypes.data, slices, tile)
     if st != OK:
         raise GwhipError(st, "debug_zgemm_split failed")
     return c

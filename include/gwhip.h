@@ -119,7 +119,7 @@ int gwhip_upload_V(gwhip_handle* h, const double* V_packed);
  * No reference counterpart: synthetic-input path for benchmarks. */
 int gwhip_generate_V(gwhip_handle* h, uint64_t seed, double sigma);
 
-/* Number of int8 slices per operand in the split-precision modes (3, 4 or 5; default 5): the P0
+/* Number of int8 slices per operand in the split-precision modes (3, 4 or 5; default 4): the P0
  * and Sigma contractions keep about 8 bits per slice of each operand relative to its row/column
  * maximum per K chunk (balanced radix-254 digits) and accumulate slice products exactly (int32),
  * then in FP64.
@@ -393,8 +393,9 @@ int gwhip_debug_zgemm(int M, int N, int K, const double* A, const double* B, dou
                       int shape);
 
 /* C = A B with the split-precision (int8-slice) GEMM on device 0, `slices` slices per operand,
- * one K chunk; row-major host arrays as in gwhip_debug_zgemm. */
-int gwhip_debug_zgemm_split(int M, int N, int K, const double* A, const double* B, double* C, int slices);
+ * one K chunk, block tile `tile` (2: 32 x 64 of 16x16x64 MFMAs, 4: 64 x 64 of 32x32x32 MFMAs);
+ * row-major host arrays as in gwhip_debug_zgemm. */
+int gwhip_debug_zgemm_split(int M, int N, int K, const double* A, const double* B, double* C, int slices, int tile);
 
 /* The stages of one q alone, for checking intermediates at full size: from G (host,
  * [nts][ns][nk][nso][nso]) compute P0(tau, q) (full tau grid after Hermitisation and the tau

@@ -17,14 +17,14 @@ pytestmark = pytest.mark.gpu
 TOL_SPLIT = 1e-8
 
 
-def test_split_gemm_identity_probe():
+@pytest.mark.parametrize("tile,n", [(2, 16), (4, 16), (4, 64), (4, 70)])
+def test_split_gemm_identity_probe(tile, n):
     """A = I times distinct entries of B returns B to the slicing accuracy (radix-254 digits do not
-    represent every dyadic number exactly): checks the int8 MFMA layout, where a misplaced element
-    would be off by O(1)."""
-    n = 16
+    represent every dyadic number exactly): checks the int8 MFMA layouts of both block tiles, where a
+    misplaced element would be off by O(1)."""
     A = np.eye(n, dtype=complex)
     B = (np.arange(n)[:, None] * 100 + np.arange(n)[None, :]).astype(complex) * (1 - 2j)
-    C = gl.debug_zgemm_split(A, B, 4)
+    C = gl.debug_zgemm_split(A, B, 4, tile)
     bad = np.argwhere(np.abs(C - B) > 1e-6 * np.abs(B).max())
     if len(bad):
         raise AssertionError(f"layout mismatch at {len(bad)} entries; first: C{tuple(bad[0])}="
@@ -33,12 +33,13 @@ def test_split_gemm_identity_probe():
 
 @pytest.mark.parametrize("shape", [(1, 1, 1), (70, 130, 300), (300, 257, 700)])
 @pytest.mark.parametrize("ns", [3, 4, 5])
-def test_split_gemm_random(shape, ns):
+@pytest.mark.parametrize("tile", [2, 4])
+def test_split_gemm_random(shape, ns, tile):
     M, N, K = shape
     rng = np.random.default_rng(M + N + K + ns)
     A = rng.standard_normal((M, K)) + 1j * rng.standard_normal((M, K))
     B = rng.standard_normal((K, N)) + 1j * rng.standard_normal((K, N))
-    C = gl.debug_zgemm_split(A, B, ns)
+    C = gl.debug_zgemm_split(A, B, ns, tile)
     ref = A @ B
     # per-element bound: each operand keeps ns balanced radix-254 digits relative to 2^e >= its
     # row / column maximum (truncation and the dropped pairs s + t >= ns, two real products per part)
@@ -64,7 +65,7 @@ def _split_vs_oracle(G, V, T, d, ns=5):
     return S_split, S_fp64, S_ref
 
 
-@pytest.mark.parametrize("ns", [5])
+@pytest.mark.parametrize("ns", [4, 5])
 def test_split_sigma_config0(ns):
     c = CONFIG0
     G, V, T, d = synthetic(c["nk"], c["nao"], c["NQ"], beta=c["beta"], lam=c["lam"], mesh=c["mesh"])
@@ -179,14 +180,17 @@ def test_explicit_kq_maps_match_mesh_maps():
 
 
 @pytest.mark.parametrize("case", [
-    dict(nk=8, mesh=(2, 2, 2), nao=8, NQ=40, herm=True),     # one 64-row tile, ragged
-    dict(nk=4, mesh=(2, 2, 1), nao=5, NQ=130, herm=True),    # three row tiles, upper-tile skipping
+    dict(nk=8, mesh=(2, 2, 2), nao=8, NQ=40, herm=True),     # one row tile band, ragged
+    dict(nk=4, mesh=(2, 2, 1), nao=5, NQ=130, herm=True),    # several row tiles, upper-tile skipping
     dict(nk=4, mesh=(2, 2, 1), nao=5, NQ=130, herm=False),   # every tile (G not Hermitian)
     dict(nk=2, mesh=(2, 1, 1), nao=3, NQ=70, nso=6, herm=True),  # X2C spin blocks
 ])
-def test_int8_block_tiles_give_identical_sigma(case):
-    """The 32 x 64 (register-staged) and 64 x 64 (direct-to-LDS) int8 GEMM tiles compute every element
-    with the same exact int32 sums and the same FP64 chunk order: Sigma is bit-identical."""
+@pytest.mark.parametrize("ns", [3, 4, 5])
+@pytest.mark.parametrize("tile", [2, 4])
+def test_split_slice_counts_on_ragged_shapes(case, ns, tile):
+    """Both int8 block tiles (32 x 64 of 16 x 16 x 64 MFMAs, 64 x 64 of 32 x 32 x 32 MFMAs) at 3, 4 and
+    5 slices per operand against the oracle: the error falls by about 254 per slice (the measured
+    errors are printed) and 4 slices, the default, stay within the 1e-8 bar."""
     nso = case.get("nso", case["nao"])
     G, V, T, d = synthetic(case["nk"], case["nao"], case["NQ"], beta=20.0, lam=1e3, mesh=case["mesh"], seed=4,
                            nso=nso, sigma=0.1)
@@ -198,12 +202,12 @@ def test_int8_block_tiles_give_identical_sigma(case):
     try:
         k.set_grid(T.T_bt, T.T_tb)
         k.upload_V(V)
-        out = {}
-        for tile in (2, 3):
-            k.set_tuning(p0_split_tile=tile, sigma_split_tile=tile)
-            out[tile] = k.solve(G, gl.PREC_SPLIT)
+        k.set_split_slices(ns)
+        k.set_tuning(p0_split_tile=tile, sigma_split_tile=tile)
+        S = k.solve(G, gl.PREC_SPLIT)
     finally:
         k.close()
     S_ref = go.gw_sigma(G, V, T.T_bt, T.T_tb, d["nk"], d["nao"], d["nso"], d["ns"], d["NQ"], mesh=d["mesh"])
-    assert rel_err(out[3], S_ref) < TOL_SPLIT
-    assert np.array_equal(out[2], out[3])
+    err = rel_err(S, S_ref)
+    print(f"split ns={ns} tile={tile} {case}: rel err vs oracle {err:.3e}", flush=True)
+    assert err < {3: 1e-4, 4: TOL_SPLIT, 5: 1e-10}[ns]
