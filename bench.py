@@ -102,9 +102,10 @@ def roofline(stats, precision, w, T, slices, tuning):
              "P0 GEMM": (f"{OZAKI_KERNEL}<{slices}> (P0 = Z conj(V)^T, int8 slices)"
                          if split else "zgemm_f64_kernel<64,64,true,true> (P0 = Z conj(V)^T, FP64 MFMA)"),
              "Sigma epilogue": "sigma_epi_mfma_kernel<2,double> (Sigma += (V G) R, FP64 MFMA)",
-             "P0 prep": ("z_prep_slice_kernel<2,5>" if split else "z_prep_mfma_kernel<2,double>") +
+             "P0 prep": (f"z_prep_slice_kernel<2,{slices}>" if split else "z_prep_mfma_kernel<2,double>") +
                         " (Z = Gb V Gq, FP64 MFMA)",
-             "Dyson inverse": "gj_inverse_kernel ((I - P0)^-1 per (q, Omega), FP64)"}
+             "Dyson inverse": "block_inverse: Hermitian block Gauss-Jordan, gj_inverse_kernel pivots + "
+                              "zgemm_f64_kernel upper-tile updates ((I - P0)^-1 per (q, Omega), FP64)"}
     rows = {}
     for name, (n, ms) in stats.items():
         if not n or name not in work:
@@ -260,6 +261,8 @@ def main():
                     help="time the Hartree-Fock Fock build on the same nk=64 nao=26 naux=300 integrals instead")
     ap.add_argument("--no-autotune", action="store_true",
                     help="keep the default launch parameters (the autotuner's choice for this workload)")
+    ap.add_argument("--tuning", default="",
+                    help="launch parameters to set after autotuning, e.g. p0_split_tile=3,sigma_split_tile=3")
     ap.add_argument("--capacity", action="store_true",
                     help="one split-precision evaluation at nk=64, nao=64, naux=300 (device memory held, time)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -310,6 +313,8 @@ def main():
     if not args.no_autotune:
         k.autotune(G_d.data_ptr(), prec)
     t_tune = time.perf_counter() - t_tune
+    if args.tuning:
+        k.set_tuning(**{kv.split("=")[0]: int(kv.split("=")[1]) for kv in args.tuning.split(",")})
     print(f"[bench] autotuned in {t_tune:.2f} s: {k.tuning()}", file=sys.stderr, flush=True)
     for i in range(args.warmup):
         k.solve_device(G_d.data_ptr(), S_d.data_ptr(), prec)

@@ -638,8 +638,7 @@ void gw_hip_kernel::p0_one_q(int64_t q, const double2* G, bool split, double2* P
   // Hermitise the half grid of this q (src/gw_cpu_kernel.cpp:217-225); for Hermitian G the
   // product is Hermitian and only its upper tiles were computed
   if (g_herm_)
-    hipLaunchKernelGGL(mirror_upper_kernel, dim3((unsigned)ceil_div(NQ2, 256), (unsigned)half), dim3(256), 0, stream_,
-                       Pq, (int)NQ, NQ2);
+    mirror_upper(Pq, (int)NQ, NQ2, half, stream_);
   else
     hermitize(Pq, (int)NQ, NQ2, half, stream_);
   GWHIP_CHECK(hipGetLastError());
@@ -676,11 +675,14 @@ void gw_hip_kernel::check_dyson_flag() {
                        "is not positive (P0 is not the polarisation of a physical G)");
 }
 
-// Block Gauss-Jordan inverse: per block column J (width nb <= 64) the column panel C and the diagonal
-// block are copied out, the block is inverted in place (gj_inverse_kernel, the pivot guard included),
-// then three batched GEMMs do the update -- R = D A_J*, A -= C R, A_*J = -C D -- and the pivot rows
-// are set to R and D.  Each step streams the matrix through the MFMA GEMM once: NQ / nb passes
-// instead of the NQ / 16 of gj_inverse_kernel's register-panel sweeps.
+// Hermitian block Gauss-Jordan inverse (the sweep operator; hgj_panel_kernel states the invariant):
+// only the upper block triangle of each matrix is kept current, and on return it holds the upper
+// triangle of A^-1 (the diagonal tiles whole; the rest of the lower triangle is stale).  Per block
+// column J (width nb = 64, the FP64 GEMM's tile, so pivot blocks are diagonal tiles):
+//   P = the step's panel (pivot rows, and the swept rows' column J conjugated), D = A_JJ;
+//   D <- D^-1 (gj_inverse_kernel, with the pivot guard);  Q = D P, columns < J0 negated (new pivot rows);
+//   A -= P^H Q on the upper tiles;  A[i < J0][J] = -P^H D for the swept rows;  pivot rows <- Q, D.
+// The update streams only the upper tiles: about 0.6 of the general sweep's GEMM work at NQ = 300.
 void gw_hip_kernel::block_inverse(double2* X, int64_t nmat) {
   const int64_t n = d_.NQ, n2 = n * n;
   const int nb = (int)std::min<int64_t>(64, n);
@@ -691,48 +693,54 @@ void gw_hip_kernel::block_inverse(double2* X, int64_t nmat) {
   for (int64_t m0 = 0; m0 < nmat; m0 += mb_max) {
     const int64_t mb = std::min(mb_max, nmat - m0);
     double2* A = X + m0 * n2;
-    double2* C = scratch_;                 // [mb][n][jb]     (allocation stride n nb)
-    double2* R = C + mb * n * nb;          // [mb][jb][n]     (stride nb n)
-    double2* D = R + mb * (int64_t)nb * n; // [mb][jb][jb]    (stride nb^2)
-    const int64_t sC = n * nb, sR = (int64_t)nb * n, sD = (int64_t)nb * nb;
+    double2* P = scratch_;                 // [mb][jb][n]     (allocation stride nb n)
+    double2* Q = P + mb * (int64_t)nb * n; // [mb][jb][n]
+    double2* D = Q + mb * (int64_t)nb * n; // [mb][jb][jb]    (stride nb^2)
+    const int64_t sP = (int64_t)nb * n, sD = (int64_t)nb * nb;
     for (int J0 = 0; J0 < n; J0 += nb) {
       const int jb = (int)std::min<int64_t>(nb, n - J0);
-      hipLaunchKernelGGL(bgj_panel_kernel, dim3((unsigned)ceil_div(n * jb, 256), (unsigned)mb), dim3(256), 0, stream_,
-                         A, (int)n, n2, J0, jb, C, sC, D, sD);
+      hipLaunchKernelGGL(hgj_panel_kernel, dim3((unsigned)ceil_div(n * jb, 256), (unsigned)mb), dim3(256), 0, stream_,
+                         A, (int)n, n2, J0, jb, P, sP, D, sD);
       GWHIP_CHECK(hipGetLastError());
       hipLaunchKernelGGL(gj_inverse_kernel, dim3((unsigned)mb), dim3(256), gj_lds, stream_, D, jb, sD, GJ_NB,
                          dyson_flag_);
       GWHIP_CHECK(hipGetLastError());
-      {  // R = D A[J0 .. J0+jb)[:]
+      {  // Q = D^-1 P
         ZGemmArgs a{};
         a.M = jb; a.N = (int)n; a.K = jb; a.batch = (int)mb; a.kin = jb; a.nin = (int)n;
         a.A = D; a.sAb = sD; a.sAm = jb; a.sAko = 0; a.sAki = 1;
-        a.B = A + (int64_t)J0 * n; a.sBb = n2; a.sBko = 0; a.sBki = n; a.sBno = 0; a.sBni = 1;
-        a.C = R; a.sCb = sR; a.sCm = n; a.sCno = 0; a.sCni = 1;
+        a.B = P; a.sBb = sP; a.sBko = 0; a.sBki = n; a.sBno = 0; a.sBni = 1;
+        a.C = Q; a.sCb = sP; a.sCm = n; a.sCno = 0; a.sCni = 1;
         a.alpha = make_double2(1.0, 0.0);
         zgemm_f64(a, true, false, stream_);
       }
-      {  // A -= C R
+      if (J0 > 0) {
+        hipLaunchKernelGGL(hgj_negate_kernel, dim3((unsigned)ceil_div((int64_t)jb * J0, 256), (unsigned)mb), dim3(256),
+                           0, stream_, Q, (int)n, sP, J0, jb);
+        GWHIP_CHECK(hipGetLastError());
+      }
+      {  // A -= P^H Q, upper tiles (the old column panel is P^H)
         ZGemmArgs a{};
         a.M = (int)n; a.N = (int)n; a.K = jb; a.batch = (int)mb; a.kin = jb; a.nin = (int)n;
-        a.A = C; a.sAb = sC; a.sAm = jb; a.sAko = 0; a.sAki = 1;
-        a.B = R; a.sBb = sR; a.sBko = 0; a.sBki = n; a.sBno = 0; a.sBni = 1;
+        a.A = P; a.sAb = sP; a.sAm = 1; a.sAko = 0; a.sAki = n; a.conjA = 1;
+        a.B = Q; a.sBb = sP; a.sBko = 0; a.sBki = n; a.sBno = 0; a.sBni = 1;
         a.C = A; a.sCb = n2; a.sCm = n; a.sCno = 0; a.sCni = 1;
         a.alpha = make_double2(-1.0, 0.0);
         a.accumulate = 1;
-        zgemm_f64(a, true, false, stream_);
+        a.upper = 1;
+        zgemm_f64(a, false, false, stream_);
       }
-      {  // A[:, J0 .. J0+jb) = -C D
+      if (J0 > 0) {  // A[i][J] = -P^H D^-1 for the swept rows i < J0 (the rows below are not kept)
         ZGemmArgs a{};
-        a.M = (int)n; a.N = jb; a.K = jb; a.batch = (int)mb; a.kin = jb; a.nin = jb;
-        a.A = C; a.sAb = sC; a.sAm = jb; a.sAko = 0; a.sAki = 1;
+        a.M = J0; a.N = jb; a.K = jb; a.batch = (int)mb; a.kin = jb; a.nin = jb;
+        a.A = P; a.sAb = sP; a.sAm = 1; a.sAko = 0; a.sAki = n; a.conjA = 1;
         a.B = D; a.sBb = sD; a.sBko = 0; a.sBki = jb; a.sBno = 0; a.sBni = 1;
         a.C = A + J0; a.sCb = n2; a.sCm = n; a.sCno = 0; a.sCni = 1;
         a.alpha = make_double2(-1.0, 0.0);
-        zgemm_f64(a, true, false, stream_);
+        zgemm_f64(a, false, false, stream_);
       }
-      hipLaunchKernelGGL(bgj_rows_kernel, dim3((unsigned)ceil_div((int64_t)jb * n, 256), (unsigned)mb), dim3(256), 0,
-                         stream_, A, (int)n, n2, J0, jb, R, sR, D, sD);
+      hipLaunchKernelGGL(hgj_rows_kernel, dim3((unsigned)ceil_div((int64_t)jb * (n - J0), 256), (unsigned)mb), dim3(256),
+                         0, stream_, A, (int)n, n2, J0, jb, Q, sP, D, sD);
       GWHIP_CHECK(hipGetLastError());
     }
   }
@@ -772,17 +780,21 @@ void gw_hip_kernel::stage_dyson(int64_t nq, const int* q0_row) {
                        (int)NQ, nmat);
     GWHIP_CHECK(hipGetLastError());
     timed(K_DYSON_INV, [&] { block_inverse(W1, nmat); });
-    hermitize(W1, (int)NQ, NQ2, nmat, stream_);
+    mirror_upper(W1, (int)NQ, NQ2, nmat, stream_);  // Herm(A^-1) from its upper triangle
     GWHIP_CHECK(hipGetLastError());
-    {  // P-tilde(i Omega) = Herm(A^-1) P0(i Omega), one product per matrix
+    {  // P-tilde(i Omega) = Herm(A^-1) P0(i Omega): A^-1 = (I - P0)^-1 commutes with P0, so the
+       // product is Hermitian and only its upper tiles are formed, then mirrored
       ZGemmArgs a{};
       a.M = (int)NQ; a.N = (int)NQ; a.K = (int)NQ; a.batch = (int)nmat; a.kin = (int)NQ; a.nin = (int)NQ;
       a.A = W1; a.sAb = NQ2; a.sAm = NQ; a.sAko = 0; a.sAki = 1;
       a.B = W0; a.sBb = NQ2; a.sBko = 0; a.sBki = NQ; a.sBno = 0; a.sBni = 1;
       a.C = W2; a.sCb = NQ2; a.sCm = NQ; a.sCno = 0; a.sCni = 1;
       a.alpha = make_double2(1.0, 0.0);
+      a.upper = 1;
       zgemm_f64(a, true, false, stream_);
     }
+    mirror_upper(W2, (int)NQ, NQ2, nmat, stream_);
+    GWHIP_CHECK(hipGetLastError());
     if (q0_eps_) {  // aux_to_PW_00 for the irreducible q of this chunk
       hipLaunchKernelGGL(q0_eps_kernel, dim3((unsigned)nwb, (unsigned)qc), dim3(256), 0, stream_, W2, q0_A_, q0_row,
                          (int)q0, (int)nwb, (int)NQ, (int)q0_inq_, q0_eps_dev_);
@@ -1007,8 +1019,7 @@ void gw_hip_kernel::finish_sigma(double2* Sigma, int64_t nkS) {
   const int64_t n = d_.nso, nmat = d_.nts * d_.ns * nkS;
   for (int64_t m0 = 0; m0 < nmat; m0 += 65535) {  // grid.y limit
     const int64_t mc = std::min<int64_t>(65535, nmat - m0);
-    hipLaunchKernelGGL(mirror_upper_kernel, dim3((unsigned)ceil_div(n * n, 256), (unsigned)mc), dim3(256), 0, stream_,
-                       Sigma + m0 * n * n, (int)n, n * n);
+    mirror_upper(Sigma + m0 * n * n, (int)n, n * n, mc, stream_);
     GWHIP_CHECK(hipGetLastError());
   }
 }

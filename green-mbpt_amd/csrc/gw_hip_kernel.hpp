@@ -41,7 +41,7 @@ struct KernelStat {
 // writes the int8 slices itself.
 struct Tuning {
   int p0_shape = 0, s_shape = 0, qpw = 4;
-  int p0_group = 1, s_group = 2;  // split precision: row tiles per L2 group of the int8 GEMMs
+  int p0_group = 1, s_group = 8;  // split precision: row tiles per L2 group of the int8 GEMMs
   int p0_oz_tile = 2, s_oz_tile = 2;  // int8 GEMM block tile: 2 (32 x 64, 16 x 16 x 64 MFMAs) or
                                       // 4 (64 x 64, 32 x 32 x 32 MFMAs, staged by LDS DMA)
   int p0_fused = 1;  // split precision, one spin block: P0 orbital kernel writes int8 slices directly

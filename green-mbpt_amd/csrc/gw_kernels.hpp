@@ -72,19 +72,39 @@ inline void hermitize(double2* X, int N, int64_t stride, int64_t nmat, hipStream
 
 // Upper triangle -> Hermitian matrix: X[i][j] = conj(X[j][i]) below the diagonal, real diagonal.
 // Used when only the upper tiles of a Hermitian product were computed (the result equals
-// (X + X^H)/2 up to rounding when X is Hermitian, which it is for Hermitian G).
-__global__ void mirror_upper_kernel(double2* X, int N, int64_t stride) {
-  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (int64_t)N * N) return;
-  const int i = (int)(e / N), j = (int)(e - (e / N) * N);
-  if (i < j) return;
-  double2* M = X + (int64_t)blockIdx.y * stride;
-  if (i == j) {
-    M[(int64_t)i * N + i].y = 0.0;
-  } else {
-    const double2 u = M[(int64_t)j * N + i];
-    M[(int64_t)i * N + j] = make_double2(u.x, -u.y);
+// (X + X^H)/2 up to rounding when X is Hermitian).  One work-group per 32 x 32 tile (I, J), I <= J,
+// of the upper triangle, staged in LDS so that both the read of (I, J) and the write of (J, I) are
+// row-contiguous.
+__global__ __launch_bounds__(256) void mirror_upper_tiled_kernel(double2* X, int N, int64_t stride) {
+  __shared__ double2 ta[HERM_TILE][HERM_TILE + 1];
+  const int T = (N + HERM_TILE - 1) / HERM_TILE;
+  int I = 0, rem = (int)blockIdx.x;
+  while (rem >= T - I) {
+    rem -= T - I;
+    ++I;
   }
+  const int J = I + rem;
+  double2* M = X + (int64_t)blockIdx.y * stride;
+  const int i0 = I * HERM_TILE, j0 = J * HERM_TILE;
+  for (int e = threadIdx.x; e < HERM_TILE * HERM_TILE; e += blockDim.x) {
+    const int a = e / HERM_TILE, b = e % HERM_TILE;
+    if (i0 + a < N && j0 + b < N) ta[a][b] = M[(int64_t)(i0 + a) * N + j0 + b];
+  }
+  __syncthreads();
+  for (int e = threadIdx.x; e < HERM_TILE * HERM_TILE; e += blockDim.x) {
+    const int a = e / HERM_TILE, b = e % HERM_TILE;  // element (j0 + a, i0 + b) of the lower tile
+    if (j0 + a >= N || i0 + b >= N) continue;
+    if (I == J && b > a) continue;  // diagonal tile: its upper part stays
+    const double2 u = ta[b][a];
+    M[(int64_t)(j0 + a) * N + i0 + b] = make_double2(u.x, a == b && I == J ? 0.0 : -u.y);
+  }
+}
+
+// Mirror the upper triangles of nmat N x N matrices at `stride` apart (nmat <= 65535).
+inline void mirror_upper(double2* X, int N, int64_t stride, int64_t nmat, hipStream_t st) {
+  const int T = (N + HERM_TILE - 1) / HERM_TILE;
+  hipLaunchKernelGGL(mirror_upper_tiled_kernel, dim3((unsigned)(T * (T + 1) / 2), (unsigned)nmat), dim3(256), 0, st, X,
+                     N, stride);
 }
 
 // max_(i,j) |X[i][j] - conj(X[j][i])| and max |X| over nmat N x N matrices, as the bit patterns
@@ -125,30 +145,55 @@ __global__ void identity_minus_kernel(double2* X, const double2* Pw, int N, int6
   X[gid] = make_double2((i == j ? 1.0 : 0.0) - p.x, -p.y);
 }
 
-// Block Gauss-Jordan step J (columns [J0, J0 + jb)) of the batched inverse, data movement only:
-// the column panel C[m][i][l] = A[m][i][J0 + l] (all N rows) and the diagonal block
-// D[m][l][l'] = A[m][J0 + l][J0 + l'] (row stride jb), both copied out before the GEMMs update A.
-__global__ void bgj_panel_kernel(const double2* __restrict__ A, int N, int64_t sA, int J0, int jb,
-                                 double2* __restrict__ C, int64_t sC, double2* __restrict__ D, int64_t sD) {
+// Hermitian block Gauss-Jordan sweep (the sweep operator on a Hermitian matrix).  After the blocks
+// S = [0, J0) are swept, A holds
+//     [ A_SS^-1            A_SS^-1 A_SU            ]     X = A_SS^-1 A_SU,
+//     [ -(X)^H             A_UU - A_US A_SS^-1 A_SU ]
+// i.e. Hermitian diagonal parts and anti-Hermitian-paired off-diagonal parts, so the upper triangle
+// determines the matrix and the sweep keeps only the upper (block) triangle up to date; after the
+// last block it holds the upper triangle of A^-1.
+// Step J (columns [J0, J0 + jb)), data movement only: the panel
+//   P[m][l][i] = A[J0 + l][i]          for i >= J0  (pivot rows, upper triangle)
+//              = conj(A[i][J0 + l])    for i <  J0  (swept rows' column J, upper triangle)
+// and the pivot block D[m][l][l'] = A[J0 + l][J0 + l'] (a diagonal tile, both triangles current).
+// Then the old column panel is P^H and the old pivot rows are P with the columns i < J0 negated.
+__global__ void hgj_panel_kernel(const double2* __restrict__ A, int N, int64_t sA, int J0, int jb,
+                                 double2* __restrict__ P, int64_t sP, double2* __restrict__ D, int64_t sD) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (e >= (int64_t)N * jb) return;
-  const int i = (int)(e / jb), l = (int)(e - (e / jb) * jb);
+  const int l = (int)(e / N), i = (int)(e - (e / N) * N);
   const int64_t m = blockIdx.y;
-  const double2 v = A[m * sA + (int64_t)i * N + J0 + l];
-  C[m * sC + (int64_t)i * jb + l] = v;
-  if (i >= J0 && i < J0 + jb) D[m * sD + (int64_t)(i - J0) * jb + l] = v;
+  double2 v;
+  if (i >= J0) {
+    v = A[m * sA + (int64_t)(J0 + l) * N + i];
+  } else {
+    const double2 u = A[m * sA + (int64_t)i * N + J0 + l];
+    v = make_double2(u.x, -u.y);
+  }
+  P[m * sP + (int64_t)l * N + i] = v;
+  if (i >= J0 && i < J0 + jb) D[m * sD + (int64_t)l * jb + (i - J0)] = v;
 }
 
-// The pivot rows of step J after the GEMMs: A[J0 + l][j] = R[l][j] outside the pivot block and
-// A[J0 + l][J0 + l'] = D[l][l'] (the inverted diagonal block) inside it.
-__global__ void bgj_rows_kernel(double2* __restrict__ A, int N, int64_t sA, int J0, int jb,
-                                const double2* __restrict__ R, int64_t sR, const double2* __restrict__ D, int64_t sD) {
+// Q[m][l][i] = -Q[m][l][i] for i < J0: D^-1 P becomes the new pivot rows D^-1 A_J* (the old pivot
+// rows at swept columns are -conj of the stored column panel).
+__global__ void hgj_negate_kernel(double2* __restrict__ Q, int N, int64_t sQ, int J0, int jb) {
   const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= (int64_t)jb * N) return;
-  const int l = (int)(e / N), j = (int)(e - (e / N) * N);
+  if (e >= (int64_t)jb * J0) return;
+  const int l = (int)(e / J0), i = (int)(e - (e / J0) * J0);
+  double2* q = Q + blockIdx.y * sQ + (int64_t)l * N + i;
+  *q = make_double2(-q->x, -q->y);
+}
+
+// The pivot rows of step J in the upper triangle: A[J0 + l][j] = Q[l][j] for j >= J0 + jb and
+// A[J0 + l][J0 + l'] = D[l][l'] (the inverted pivot block).
+__global__ void hgj_rows_kernel(double2* __restrict__ A, int N, int64_t sA, int J0, int jb,
+                                const double2* __restrict__ Q, int64_t sQ, const double2* __restrict__ D, int64_t sD) {
+  const int W = N - J0;
+  const int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)jb * W) return;
+  const int l = (int)(e / W), j = J0 + (int)(e - (e / W) * W);
   const int64_t m = blockIdx.y;
-  const bool in_block = j >= J0 && j < J0 + jb;
-  A[m * sA + (int64_t)(J0 + l) * N + j] = in_block ? D[m * sD + (int64_t)l * jb + (j - J0)] : R[m * sR + (int64_t)l * N + j];
+  A[m * sA + (int64_t)(J0 + l) * N + j] = j < J0 + jb ? D[m * sD + (int64_t)l * jb + (j - J0)] : Q[m * sQ + (int64_t)l * N + j];
 }
 
 // ---------------------------------------------------------------------------------------

@@ -158,15 +158,16 @@ int gwhip_eval_sigma_device_noalloc(gwhip_handle* h, const void* G_dev, void* Si
  * Time the candidate launch parameters of this system on its own operands (G in device memory)
  * and keep the fastest: the FP64 GEMM tiles of the P0 and Sigma contractions (0: 64x64,
  * 1: 128x64, 2: 64x128), the aux indices per wave of the P0 orbital kernel (1, 2, 4, 8) and, for
- * the split-precision modes, the number of row tiles that sweep the column tiles together in the
- * int8 GEMMs' L2-aware tile order (1 .. 64) and the k chunks per launch of the int8 P0 GEMM.
+ * the split-precision modes, the block tile of each int8 GEMM, the number of row tiles that sweep
+ * the column tiles together in the int8 GEMMs' L2-aware tile order (1 .. 64) and the k chunks per
+ * launch of the int8 P0 GEMM.
  * Results do not change, only speed.
  * No reference counterpart. */
 int gwhip_autotune(gwhip_handle* h, const void* G_dev, int precision);
 /* The tuning as an array of GWHIP_TUNING_COUNT ints: [0] FP64 P0 GEMM tile, [1] FP64 Sigma GEMM
  * tile, [2] aux indices per wave, [3] int8 P0 GEMM row-tile group, [4] int8 Sigma GEMM row-tile
- * group, [5] int8 P0 GEMM block tile and [6] int8 Sigma GEMM block tile (both 2: the 32x64
- * software-pipelined tile, the only one),
+ * group, [5] int8 P0 GEMM block tile and [6] int8 Sigma GEMM block tile (2: 32x64 of 16x16x64
+ * MFMAs, A staged through registers; 4: 64x64 of 32x32x32 MFMAs; results bit-identical),
  * [7] split P0 orbital kernel writes the int8 slices itself (1) or via a separate pass (0),
  * [8] orbital-size products (Z = Gb V Gq of P0, V G R of Sigma) as batched FP64 GEMMs (1) or the
  * register-chained MFMA kernels (0); always 1 in effect for nao > 48, [9] int8 P0 GEMM: k chunks

@@ -211,3 +211,23 @@ def test_split_slice_counts_on_ragged_shapes(case, ns, tile):
     err = rel_err(S, S_ref)
     print(f"split ns={ns} tile={tile} {case}: rel err vs oracle {err:.3e}", flush=True)
     assert err < {3: 1e-4, 4: TOL_SPLIT, 5: 1e-10}[ns]
+
+
+@pytest.mark.parametrize("ns", [4, 5])
+def test_int8_block_tiles_give_identical_sigma(ns):
+    """Both int8 block tiles sum the same exact int32 slice products per K chunk and convert them to
+    FP64 in the same order, so Sigma is bit-identical across them (ragged NQ = 130, Hermitian G:
+    upper-tile skipping in the P0 GEMM)."""
+    G, V, T, d = synthetic(4, 5, 130, beta=20.0, lam=1e3, mesh=(2, 2, 1), seed=4, sigma=0.1)
+    k = _kernel(d, T)
+    try:
+        k.set_grid(T.T_bt, T.T_tb)
+        k.upload_V(V)
+        k.set_split_slices(ns)
+        out = []
+        for tile in (2, 4):
+            k.set_tuning(p0_split_tile=tile, sigma_split_tile=tile)
+            out.append(k.solve(G, gl.PREC_SPLIT))
+    finally:
+        k.close()
+    assert np.array_equal(out[0], out[1])
