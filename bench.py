@@ -33,15 +33,17 @@ WORKLOAD = dict(nk=64, mesh=(4, 4, 4), nao=26, NQ=300, ns=1, beta=100.0, lam=1e4
 # the bytes the copy reads, WRITE_SIZE 1.0x the bytes it writes.
 # Source: profiles/gw_sigma_nk64_nao26_naux300_hbm_traffic.md
 PMC_PER_LAUNCH_KIB = {
-    "split": {"Sigma GEMM": (4.3317e6, 1.0202e7), "Sigma epilogue": (7.0708e6, 2.4060e5),
-              "P0 GEMM": (1.1487e7, 6.7978e5), "P0 prep": (7.7640e5, 6.7749e6)},  # P0: 16 launches per q
-    "fp64": {"Sigma GEMM": (2.6011e7, 1.0218e7), "Sigma epilogue": (7.0731e6, 2.4061e5),
-             "P0 GEMM": (2.2101e7, 4.2422e4), "P0 prep": (7.4386e5, 1.1726e7)},
+    "split": {"Sigma GEMM": (3.4374e6, 1.0188e7), "Sigma epilogue": (5.5561e6, 1.5085e5),
+              "P0 GEMM": (8.2941e6, 6.8061e5), "P0 prep": (6.9744e5, 5.3279e6),  # P0 GEMM: 16 launches per q
+              "Dyson inverse": (5.7138e7, 6.7430e7)},  # every kernel of the inverse, one evaluation
+    "fp64": {"Sigma GEMM": (2.9810e7, 1.0307e7), "Sigma epilogue": (5.5567e6, 1.5086e5),
+             "P0 GEMM": (2.2280e7, 4.2417e4), "P0 prep": (7.4678e5, 1.0163e7),
+             "Dyson inverse": (5.7137e7, 6.7429e7)},
 }
 PMC_CALIBRATION = (0.5, 1.0)
 PMC_SOURCE = "profiles/gw_sigma_nk64_nao26_naux300_hbm_traffic.md"
-# the int8 GEMM kernel (one block tile: 32 x 64, software-pipelined)
-OZAKI_KERNEL = "zgemm_ozaki_pipe_kernel"
+# the int8 GEMM kernel of each block tile (tuning keys p0_split_tile / sigma_split_tile)
+OZAKI_KERNEL = {2: "zgemm_ozaki_pipe_kernel", 4: "zgemm_ozaki32_kernel"}
 
 
 def algorithmic_flops(nk, nao, NQ, nts, nw_b, pseudo_ns=1):
@@ -87,19 +89,22 @@ def stage_work(w, T, slices):
     return out
 
 
-def roofline(stats, precision, w, T, slices, tuning):
+def roofline(stats, precision, w, T, slices, tuning, steps=1):
     """Roofline of the kernel family with the largest share of the step, from the per-launch times
     measured with HIP events on the library's stream: achieved = algorithmic flops per launch / mean
     launch time, frac = achieved / the published peak of the arithmetic the kernel uses (int8 5.0 POP/s
     for the split GEMMs, FP64 78.6 TFLOP/s otherwise); issued_frac = int8 operations issued / time /
     peak; traffic = the rocprofv3 counter bytes per launch of this build, traffic_ratio = traffic /
-    algorithmic bytes.  Every family's figures are listed under all_stages."""
+    algorithmic bytes.  Every family's figures are listed under all_stages.  stats holds launch counts
+    and times summed over `steps` evaluations; the work figures are per evaluation."""
     work = stage_work(w, T, slices)
     split = precision == "split"
     int8 = {"Sigma GEMM", "P0 GEMM"} if split else set()
-    names = {"Sigma GEMM": (f"{OZAKI_KERNEL}<{slices}> (R = P-tilde conj(V), int8 slices)"
+    s_kernel = OZAKI_KERNEL.get(tuning.get("sigma_split_tile", 2), "?")
+    p_kernel = OZAKI_KERNEL.get(tuning.get("p0_split_tile", 2), "?")
+    names = {"Sigma GEMM": (f"{s_kernel}<{slices}> (R = P-tilde conj(V), int8 slices)"
                             if split else "zgemm_f64_kernel<64,64,true,false> (R = P-tilde conj(V), FP64 MFMA)"),
-             "P0 GEMM": (f"{OZAKI_KERNEL}<{slices}> (P0 = Z conj(V)^T, int8 slices)"
+             "P0 GEMM": (f"{p_kernel}<{slices}> (P0 = Z conj(V)^T, int8 slices)"
                          if split else "zgemm_f64_kernel<64,64,true,true> (P0 = Z conj(V)^T, FP64 MFMA)"),
              "Sigma epilogue": "sigma_epi_mfma_kernel<2,double> (Sigma += (V G) R, FP64 MFMA)",
              "P0 prep": (f"z_prep_slice_kernel<2,{slices}>" if split else "z_prep_mfma_kernel<2,double>") +
@@ -113,15 +118,16 @@ def roofline(stats, precision, w, T, slices, tuning):
         wk = work[name]
         t = ms / n * 1e-3
         peak = PEAK_INT8_TOPS if name in int8 else PEAK_FP64_TFLOPS
-        flops = wk["flops"] / n
-        alg_bytes = (wk.get("bytes_split") if split and "bytes_split" in wk else wk["bytes_fp64"]) / n
-        r = {"kernel": names[name], "launches": n, "avg_launch_ms": t * 1e3,
+        per = steps / n  # evaluations per launch
+        flops = wk["flops"] * per
+        alg_bytes = (wk.get("bytes_split") if split and "bytes_split" in wk else wk["bytes_fp64"]) * per
+        r = {"kernel": names[name], "launches": n / steps, "avg_launch_ms": t * 1e3,
              "achieved": flops / t / 1e12, "peak": peak, "unit": "TFLOP/s",
              "peak_unit": "TOPS (int8)" if name in int8 else "TFLOP/s (FP64)",
              "frac": flops / t / 1e12 / peak, "algorithmic_flops_per_launch": flops,
              "algorithmic_bytes": alg_bytes}
         if name in int8 and wk["issued"]:
-            r["issued_tops"] = wk["issued"] / n / t / 1e12
+            r["issued_tops"] = wk["issued"] * per / t / 1e12
             r["issued_frac"] = r["issued_tops"] / peak
         pmc = PMC_PER_LAUNCH_KIB.get(precision, {}).get(name)
         if pmc is not None:
@@ -334,7 +340,7 @@ def main():
     regions = k.timings()
     fl = algorithmic_flops(w["nk"], w["nao"], w["NQ"], T.nts, T.nw_b)
     S_host = S_d.cpu().numpy().view(np.complex128).copy()
-    roof = roofline(stats, args.precision, w, T, args.slices, k.tuning())
+    roof = roofline(stats, args.precision, w, T, args.slices, k.tuning(), args.steps)
 
     check = {}
     if prec == gl.PREC_SPLIT and not args.no_check:
