@@ -1154,8 +1154,8 @@ void gw_hip_kernel::autotune(const double2* G, int precision) {
       p0_gemm_split(0, tcp, P_, tune_.p0_group, tune_.p0_kchunks);
     });
     tune_.p0_group = best_of(7, groups, [&](int c) { p0_gemm_split(0, tcp, P_, c, tune_.p0_kchunks); });
-    const int kch[5] = {0, 16, 8, 4, 2};
-    tune_.p0_kchunks = best_of(5, kch, [&](int c) { p0_gemm_split(0, tcp, P_, tune_.p0_group, c); });
+    const int kch[6] = {0, 16, 8, 4, 2, 1};
+    tune_.p0_kchunks = best_of(6, kch, [&](int c) { p0_gemm_split(0, tcp, P_, tune_.p0_group, c); });
     sigma_slice(P_, tcs);
     tune_.s_oz_tile = best_of(2, tiles, [&](int c) {
       tune_.s_oz_tile = c;

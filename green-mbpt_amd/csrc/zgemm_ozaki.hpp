@@ -16,8 +16,9 @@
 // Slice store (int8), for each row (A) or column (B):
 //   [chunk][plane][slice][kc]   planes Re, Im
 // (B is stored already conjugated when the contraction needs conj(B)), so
-//   Re C = Re A Re B + (-Im A) Im B,   Im C = Re A Im B + Im A Re B,
-// with -Im A negated in registers (digits lie in [-127, 127], so negation is exact).  Karatsuba's
+//   Re C = Re A Re B + Im A (-Im B),   Im C = Re A Im B + Im A Re B,
+// with -Im B (32 x 64 tile) or -Im A (64 x 64 tile) negated in registers (digits lie in [-127, 127],
+// so negation is exact).  Karatsuba's
 // three-product form needs a third (Re + Im) plane and a third accumulator set: at five slices the
 // 32 x 64 tile then runs out of registers (256 VGPRs and a 117-register spill per lane at two
 // waves per SIMD), so the four-product form is kept.
@@ -191,6 +192,11 @@ struct OzGemmArgs {
 #define OZ_ROW(g, r) (4 * (g) + (r))
 
 // Block tile 32 x 64 (complex); the four waves split N, each owning 32 x 16 = two 16x16 blocks.
+// The products take the four-product form with -Im B: Re C = Re A Re B + Im A (-Im B), negated once
+// per K step for the NS B fragments (negating -Im A instead costs one negation per slice and row
+// block).  ONE (a single K chunk, as the Sigma contraction has): the int32 -> FP64 conversion runs once
+// after the loop, so no FP64 sums are live inside it (156 instead of 220 VGPRs at four slices: three
+// waves per SIMD instead of two).
 // A (32 rows, shared by all waves) is staged through a double-buffered LDS panel one K step (64
 // bytes per plane-slice row) at a time, its four 16-byte pieces XOR-swizzled by (r >> 2) & 3 so
 // that the 16 rows of one fragment read fall on distinct banks; each LDS store covers 16 whole rows
@@ -205,7 +211,7 @@ __device__ __forceinline__ int oz_lds_off(int ps, int rows, int r, int piece) {
   return (ps * rows + r) * 64 + 16 * (piece ^ ((r >> 2) & 3));
 }
 
-template <int NS>
+template <int NS, bool ONE>
 __global__ __launch_bounds__(256, 2) void zgemm_ozaki_pipe_kernel(OzGemmArgs p) {
   constexpr int NPS = OZ_PLANES * NS;          // plane-slices per row
   constexpr int NPC = NPS / 2;                 // staged per thread and K step (two parities)
@@ -283,6 +289,29 @@ __global__ __launch_bounds__(256, 2) void zgemm_ozaki_pipe_kernel(OzGemmArgs p) 
 #pragma unroll
     for (int x = 0; x < 2; ++x) ar[d][x] = ai[d][x] = v4i{0, 0, 0, 0};
 
+  // int32 -> FP64 with the (row, column, chunk) scale 2^(ea + eb) / 127^2; rows / columns past the
+  // edge take the clamped ones' exponents (their results are not stored)
+  auto convert = [&](int c) {
+    const int no = bcol / p.nin;
+    const int eb = p.eB[b * p.seBb + (int64_t)no * p.seBno + (int64_t)(bcol - no * p.nin) * p.seBni + c];
+#pragma unroll
+    for (int x = 0; x < 2; ++x)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int m = min(m0 + 16 * x + OZ_ROW(lg, r), p.M - 1);
+        const int ea = p.eA[b * p.seAb + (int64_t)m * p.seAm + c];
+        double sr = 0.0, si = 0.0;
+#pragma unroll
+        for (int d = 0; d < NS; ++d) {  // exact integers times 254^-d
+          sr = fma((double)ar[d][x][r], oz_weight(d), sr);
+          si = fma((double)ai[d][x][r], oz_weight(d), si);
+        }
+        const double sc = ldexp(1.0 / (OZ_FIRST * OZ_FIRST), ea + eb);
+        cr[x][r] = fma(sr, sc, cr[x][r]);
+        ci[x][r] = fma(si, sc, ci[x][r]);
+      }
+  };
+
   v4i fb[OZ_PLANES][NS];
   load_a(0);
   store_a(0);
@@ -304,25 +333,25 @@ __global__ __launch_bounds__(256, 2) void zgemm_ozaki_pipe_kernel(OzGemmArgs p) 
     load_a(gn);
     const int8_t* bn = b_at(gn);
     const int8_t* As = oz_lds + (g & 1) * BUF;
+    v4i fbn[NS];  // -Im B of this K step
+#pragma unroll
+    for (int t = 0; t < NS; ++t) fbn[t] = neg_i8x16(fb[1][t]);
 #pragma unroll
     for (int s = 0; s < NS; ++s) {
-      v4i fa_r[2], fa_i[2], fa_n[2];
+      v4i fa_r[2], fa_i[2];
 #pragma unroll
       for (int x = 0; x < 2; ++x) {
         const int row = 16 * x + lrow;
         fa_r[x] = *reinterpret_cast<const v4i*>(As + oz_lds_off(0 * NS + s, OZ_BM, row, lg));
         fa_i[x] = *reinterpret_cast<const v4i*>(As + oz_lds_off(1 * NS + s, OZ_BM, row, lg));
-        fa_n[x] = neg_i8x16(fa_i[x]);
       }
-      // t from high to low: fb[.][t] was refilled for this step after the previous step's slice
-      // NS - 1 - t, so the highest t arrived first
 #pragma unroll
       for (int t = NS - 1 - s; t >= 0; --t) {
 #pragma unroll
         for (int x = 0; x < 2; ++x) {
           ar[s + t][x] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa_r[x], fb[0][t], ar[s + t][x], 0, 0, 0);
           ai[s + t][x] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa_r[x], fb[1][t], ai[s + t][x], 0, 0, 0);
-          ar[s + t][x] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa_n[x], fb[1][t], ar[s + t][x], 0, 0, 0);
+          ar[s + t][x] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa_i[x], fbn[t], ar[s + t][x], 0, 0, 0);
           ai[s + t][x] = __builtin_amdgcn_mfma_i32_16x16x64_i8(fa_i[x], fb[0][t], ai[s + t][x], 0, 0, 0);
         }
       }
@@ -332,26 +361,8 @@ __global__ __launch_bounds__(256, 2) void zgemm_ozaki_pipe_kernel(OzGemmArgs p) 
         for (int pl = 0; pl < OZ_PLANES; ++pl) fb[pl][t] = *reinterpret_cast<const v4i*>(bn + (int64_t)(pl * NS + t) * p.kc);
       }
     }
-    if (chunk_end) {  // int32 -> FP64 with the (row, column, chunk) scale 2^(ea + eb) / 127^2
-      // rows / columns past the edge take the clamped ones' exponents (their results are not stored)
-      const int no = bcol / p.nin;
-      const int eb = p.eB[b * p.seBb + (int64_t)no * p.seBno + (int64_t)(bcol - no * p.nin) * p.seBni + c];
-#pragma unroll
-      for (int x = 0; x < 2; ++x)
-#pragma unroll
-        for (int r = 0; r < 4; ++r) {
-          const int m = min(m0 + 16 * x + OZ_ROW(lg, r), p.M - 1);
-          const int ea = p.eA[b * p.seAb + (int64_t)m * p.seAm + c];
-          double sr = 0.0, si = 0.0;
-#pragma unroll
-          for (int d = 0; d < NS; ++d) {  // exact integers times 254^-d
-            sr = fma((double)ar[d][x][r], oz_weight(d), sr);
-            si = fma((double)ai[d][x][r], oz_weight(d), si);
-          }
-          const double sc = ldexp(1.0 / (OZ_FIRST * OZ_FIRST), ea + eb);
-          cr[x][r] = fma(sr, sc, cr[x][r]);
-          ci[x][r] = fma(si, sc, ci[x][r]);
-        }
+    if (!ONE && chunk_end) {
+      convert(c);
 #pragma unroll
       for (int d = 0; d < NS; ++d)
 #pragma unroll
@@ -360,6 +371,7 @@ __global__ __launch_bounds__(256, 2) void zgemm_ozaki_pipe_kernel(OzGemmArgs p) 
     store_a((g + 1) & 1);  // on the last step into the buffer no wave reads any more
     __syncthreads();
   }
+  if (ONE) convert(0);
 
   const int n = n0 + wn + lrow;
   if (n >= p.N) return;
@@ -573,8 +585,13 @@ inline void ozaki_launch(const OzGemmArgs& p, int64_t grid, hipStream_t stream) 
     return;
   }
   const size_t lds = ozaki_pipe_lds_bytes(NS);
-  oz_allow_lds(zgemm_ozaki_pipe_kernel<NS>, lds);
-  hipLaunchKernelGGL(zgemm_ozaki_pipe_kernel<NS>, dim3((unsigned)grid), dim3(256), lds, stream, p);
+  if (p.nchunk == 1) {
+    oz_allow_lds(zgemm_ozaki_pipe_kernel<NS, true>, lds);
+    hipLaunchKernelGGL((zgemm_ozaki_pipe_kernel<NS, true>), dim3((unsigned)grid), dim3(256), lds, stream, p);
+  } else {
+    oz_allow_lds(zgemm_ozaki_pipe_kernel<NS, false>, lds);
+    hipLaunchKernelGGL((zgemm_ozaki_pipe_kernel<NS, false>), dim3((unsigned)grid), dim3(256), lds, stream, p);
+  }
 }
 
 inline bool ozaki_tile_valid(int tile) { return tile == 2 || tile == 4; }
