@@ -882,9 +882,23 @@ void gw_hip_kernel::sigma_q(const double2* G, double2* Sigma, bool split, const 
       const dim3 grid((unsigned)nkb, (unsigned)qs, (unsigned)(2 * tc));
       const int up = sigma_upper() ? 1 : 0;
       const int n = (int)nkb, t = (int)t0;
-      if (NB == 1) launch_sigma_epi<1>(grid, lds, stream_, dd, G, Vb, scratch_, Sigma, km, n, t, scale, up, 1, part);
-      else if (NB == 2) launch_sigma_epi<2>(grid, lds, stream_, dd, G, Vb, scratch_, Sigma, km, n, t, scale, up, 1, part);
-      else launch_sigma_epi<3>(grid, lds, stream_, dd, G, Vb, scratch_, Sigma, km, n, t, scale, up, 1, part);
+      if (d_.nao <= 32) {  // 4 x 4 x 4 MFMAs: orbital dimensions padded to 4 (i to 16)
+        const int nb4 = d_.nao <= 8 ? 2 : d_.nao <= 16 ? 4 : d_.nao <= 28 ? 7 : 8;
+        const size_t l4 = sigma_epi4_lds_bytes(nb4);
+        const XcdGrid xg = make_xcd_grid(grid);
+        auto go = [&](auto kern) {
+          allow_lds(kern, l4);
+          hipLaunchKernelGGL(kern, xcd_launch_grid(xg), dim3(256), l4, stream_, xg, dd, G, Vb, scratch_, Sigma, km, n, t,
+                             scale, up, 1, (int)grid.y, part);
+        };
+        if (nb4 == 2) go(sigma_epi4_kernel<2>);
+        else if (nb4 == 4) go(sigma_epi4_kernel<4>);
+        else if (nb4 == 7) go(sigma_epi4_kernel<7>);
+        else go(sigma_epi4_kernel<8>);
+        GWHIP_CHECK(hipGetLastError());
+      } else {  // 32 < nao <= 48: 16 x 16 x 4 MFMAs
+        launch_sigma_epi<3>(grid, lds, stream_, dd, G, Vb, scratch_, Sigma, km, n, t, scale, up, 1, part);
+      }
       if (qs > 1) {
         const int64_t per_tau = d_.ns * nkb * d_.nso * d_.nso;
         const int64_t nel = 2 * tc * per_tau;

@@ -609,6 +609,162 @@ __global__ __launch_bounds__(256, NB < 3 ? 3 : 1) void sigma_epi_mfma_kernel(Xcd
   }
 }
 
+// ---------------------------------------------------------------------------------------
+// The Sigma epilogue above on v_mfma_f64_4x4x4_4b, for nao <= 4 NB4 <= 32.  One instruction does four
+// independent 4 x 4 x 4 products; its lane layouts (checked by the layout-probe test) are
+//   A_b[r][k] at lane 16 k + 4 b + r,   B_b[k][c] at lane 16 k + 4 b + c,   D_b[r][c] at lane 16 r + 4 b + c,
+// and it issues at the FP64 matrix-core rate with only four independent accumulators per wave.  The four
+// products of an instruction are four consecutive 4-wide blocks of i (a group of 16 i), so orbital
+// dimensions pad to multiples of 4 (n, m, j) and 16 (i) instead of 16 everywhere:
+//   T^T[n][i] = sum_m G[m][n] V[i][m]           A = G^T block (the same for the four), B = V^T
+//   Sigma^T[j][i] += sum_n R[n][j] T^T[n][i]    A = R^T block (the same for the four), B = T^T
+// Lane L holds T^T[n0 + L/16][16 ig + L%16] after the first product of K slab n0, which is exactly
+// the B operand of the second product for that slab, so the intermediate stays in registers.  V_Q
+// (the first product's B operands, all slabs) is loaded once per Q; R_Q one K slab of n ahead.
+// Grid, Q ranges, partial sums and the upper-triangle skipping as in sigma_epi_mfma_kernel: a tile
+// (j block, i group) is skipped when every j in it lies below every i.
+__device__ __forceinline__ void cmfma4(double& cr, double& ci, double ar, double ai, double br, double bi) {
+  cr = __builtin_amdgcn_mfma_f64_4x4x4f64(ar, br, cr, 0, 0, 0);
+  ci = __builtin_amdgcn_mfma_f64_4x4x4f64(ar, bi, ci, 0, 0, 0);
+  cr = __builtin_amdgcn_mfma_f64_4x4x4f64(-ai, bi, cr, 0, 0, 0);
+  ci = __builtin_amdgcn_mfma_f64_4x4x4f64(ai, br, ci, 0, 0, 0);
+}
+
+template <int NB4>
+__global__ __launch_bounds__(256, NB4 <= 4 ? 3 : 2) void sigma_epi4_kernel(XcdGrid xg, DevDims d, const double2* __restrict__ G,
+                                                            const double2* __restrict__ V,
+                                                            const double2* __restrict__ R,
+                                                            double2* __restrict__ Sigma,
+                                                            const int* __restrict__ kminus, int nks, int t0,
+                                                            double scale, int upper, int mirror, int qsplit,
+                                                            double2* __restrict__ part) {
+  constexpr int NG = (NB4 + 3) / 4;          // groups of 16 i
+  constexpr int N4 = 4 * NB4, LD = N4 + 1;   // G planes [N4][LD] in LDS
+  extern __shared__ double lds_d[];
+  double* Gr = lds_d;
+  double* Gi = Gr + N4 * LD;
+  double* red = Gi + N4 * LD;  // [NB4 * NG][2][64]
+  const int nao = d.nao, nao2 = nao * nao;
+  int bx, by, bz;
+  if (!xcd_block(xg, bx, by, bz)) return;
+  const int tz = mirror ? (bz >> 1) : bz;
+  const int k = bx, tau = mirror && (bz & 1) ? d.nts - 1 - (t0 + tz) : t0 + tz;
+  const int kmq = kminus[k];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int lk = lane >> 4, l16 = lane & 15, l4 = lane & 3;
+  const int qspan = (d.NQ + qsplit - 1) / qsplit, qlo = by * qspan, qhi = min(d.NQ, qlo + qspan);
+
+  for (int s = 0; s < d.pseudo_ns; ++s) {
+    const int is = s / (d.pns * d.pns), a = (s / d.pns) % d.pns, b = s % d.pns;
+    const double2* Gk = G + g_index(d, tau, is, kmq);
+    __syncthreads();
+    for (int e = tid; e < N4 * N4; e += blockDim.x) {
+      const int r = e / N4, c = e - (e / N4) * N4;
+      double2 g = make_double2(0.0, 0.0);
+      if (r < nao && c < nao) g = Gk[(int64_t)(a * nao + r) * d.nso + b * nao + c];  // G[m = r][n = c]
+      Gr[r * LD + c] = g.x;
+      Gi[r * LD + c] = g.y;
+    }
+    __syncthreads();
+    double sr[NB4][NG], si[NB4][NG];  // Sigma^T tile (j block, i group), this wave's Q
+#pragma unroll
+    for (int x = 0; x < NB4; ++x)
+#pragma unroll
+      for (int y = 0; y < NG; ++y) { sr[x][y] = 0.0; si[x][y] = 0.0; }
+
+    for (int Q = qlo + wave; Q < qhi; Q += 4) {
+      const double2* vq = V + ((int64_t)k * d.NQ + Q) * nao2;                // vq[i * nao + m] = V[i][m]
+      const double2* rq = R + (((int64_t)tz * nks + k) * d.NQ + Q) * nao2;  // rq[n * nao + j]
+      double vr[NB4][NG], vi[NB4][NG];  // B of the first product: V[16 ig + L%16][4 ms + L/16]
+#pragma unroll
+      for (int ms = 0; ms < NB4; ++ms)
+#pragma unroll
+        for (int ig = 0; ig < NG; ++ig) {
+          const int i = 16 * ig + l16, m = 4 * ms + lk;
+          const double2 v = (i < nao && m < nao) ? vq[i * nao + m] : make_double2(0.0, 0.0);
+          vr[ms][ig] = v.x;
+          vi[ms][ig] = v.y;
+        }
+#pragma unroll
+      for (int nb = 0; nb < NB4; ++nb) {
+        if (4 * nb >= nao) break;  // K slab wholly in the padding
+        const int n = 4 * nb + lk;
+        double rr[NB4], ri[NB4];  // A of the second product: R[4 nb + L/16][4 jb + L%4]
+#pragma unroll
+        for (int jb = 0; jb < NB4; ++jb) {
+          const int j = 4 * jb + l4;
+          const double2 r = (n < nao && j < nao) ? rq[n * nao + j] : make_double2(0.0, 0.0);
+          rr[jb] = r.x;
+          ri[jb] = r.y;
+        }
+        double tr[NG], ti[NG];
+#pragma unroll
+        for (int ig = 0; ig < NG; ++ig) { tr[ig] = 0.0; ti[ig] = 0.0; }
+#pragma unroll
+        for (int ms = 0; ms < NB4; ++ms) {
+          if (4 * ms >= nao) break;
+          const int m = 4 * ms + lk, nn = 4 * nb + l4;  // A = G^T[n][m]: G[4 ms + L/16][4 nb + L%4]
+          const double gr = Gr[m * LD + nn], gi = Gi[m * LD + nn];
+#pragma unroll
+          for (int ig = 0; ig < NG; ++ig) cmfma4(tr[ig], ti[ig], gr, gi, vr[ms][ig], vi[ms][ig]);
+        }
+#pragma unroll
+        for (int jb = 0; jb < NB4; ++jb)
+#pragma unroll
+          for (int ig = 0; ig < NG; ++ig) {
+            if (upper && 4 * jb + 3 < 16 * ig) continue;  // every j below every i: mirrored afterwards
+            cmfma4(sr[jb][ig], si[jb][ig], rr[jb], ri[jb], tr[ig], ti[ig]);
+          }
+      }
+    }
+    // reduce the four waves' partial sums into wave 0
+    for (int w = 1; w < 4; ++w) {
+      __syncthreads();
+      if (wave == w) {
+#pragma unroll
+        for (int x = 0; x < NB4; ++x)
+#pragma unroll
+          for (int y = 0; y < NG; ++y) {
+            red[((x * NG + y) * 2 + 0) * 64 + lane] = sr[x][y];
+            red[((x * NG + y) * 2 + 1) * 64 + lane] = si[x][y];
+          }
+      }
+      __syncthreads();
+      if (wave == 0) {
+#pragma unroll
+        for (int x = 0; x < NB4; ++x)
+#pragma unroll
+          for (int y = 0; y < NG; ++y) {
+            sr[x][y] += red[((x * NG + y) * 2 + 0) * 64 + lane];
+            si[x][y] += red[((x * NG + y) * 2 + 1) * 64 + lane];
+          }
+      }
+    }
+    if (wave == 0) {
+      const int64_t off = (((int64_t)tau * d.ns + is) * nks + k) * d.nso * d.nso;
+      double2* S = part ? part + (int64_t)by * d.nts * d.ns * nks * d.nso * d.nso + off : Sigma + off;
+#pragma unroll
+      for (int x = 0; x < NB4; ++x)
+#pragma unroll
+        for (int y = 0; y < NG; ++y) {
+          const int j = 4 * x + lk, i = 16 * y + l16;  // lane L: Sigma^T[4 x + L/16][16 y + L%16]
+          if (i >= nao || j >= nao) continue;
+          if (upper && 4 * x + 3 < 16 * y) continue;
+          double2* dst = S + (int64_t)(a * nao + i) * d.nso + b * nao + j;
+          double2 v = part ? make_double2(0.0, 0.0) : *dst;
+          v.x += scale * sr[x][y];
+          v.y += scale * si[x][y];
+          *dst = v;
+        }
+    }
+  }
+}
+
+inline size_t sigma_epi4_lds_bytes(int nb4) {
+  const int n4 = 4 * nb4, ng = (nb4 + 3) / 4;
+  return ((size_t)2 * n4 * (n4 + 1) + (size_t)nb4 * ng * 2 * 64) * sizeof(double);
+}
+
 // Sigma(tau) += sum over y < qsplit of part[y](tau), in order of y, for the 2 tc tau of one
 // mirrored epilogue launch (tau = t0 + z/2, or nts - 1 - (t0 + z/2) for odd z); per_tau =
 // ns * nks * nso^2 elements, slab = nts * per_tau.

@@ -16,6 +16,35 @@ __global__ void fill_hash_kernel(double2* X, int64_t n, uint64_t salt) {
   X[gid] = make_double2(u01(salt, 2 * gid) - 0.5, u01(salt, 2 * gid + 1) - 0.5);
 }
 
+// v_mfma_f64_4x4x4_4b: four independent 4x4x4 products per instruction (64 results, one per lane)
+template <int CH>
+__global__ __launch_bounds__(256) void mfma4_peak_kernel(double* out, int iters) {
+  double acc[CH];
+#pragma unroll
+  for (int j = 0; j < CH; ++j) acc[j] = 0.0;
+  const double a = 1e-3 * threadIdx.x, b = 1.0 + 1e-6 * blockIdx.x;
+  for (int i = 0; i < iters; ++i) {
+#pragma unroll
+    for (int j = 0; j < CH; ++j) acc[j] = __builtin_amdgcn_mfma_f64_4x4x4f64(a, b, acc[j], 0, 0, 0);
+  }
+  double s = 0.0;
+#pragma unroll
+  for (int j = 0; j < CH; ++j) s += acc[j];
+  out[(int64_t)blockIdx.x * blockDim.x + threadIdx.x] = s;
+}
+
+// Lane layout of v_mfma_f64_4x4x4_4b, one wave, out[run * 64 + lane] = D of that lane:
+//   run 0: A = 2^(l % 16), B = 2^(16 (l % 16))  -> each D is a sum of 2^(a + 16 b) over its k pairs
+//   run 1: A = l / 16 + 1, B = 1                 -> 4 (block of A feeding lane l) when blocks are l / 16
+//   run 2: A = 1, B = l / 16 + 1
+__global__ __launch_bounds__(64) void mfma4_layout_kernel(double* out) {
+  const int l = threadIdx.x;
+  const double a0 = ldexp(1.0, l % 16), b0 = ldexp(1.0, 16 * (l % 16));
+  out[l] = __builtin_amdgcn_mfma_f64_4x4x4f64(a0, b0, 0.0, 0, 0, 0);
+  out[64 + l] = __builtin_amdgcn_mfma_f64_4x4x4f64((double)(l / 16 + 1), 1.0, 0.0, 0, 0, 0);
+  out[128 + l] = __builtin_amdgcn_mfma_f64_4x4x4f64(1.0, (double)(l / 16 + 1), 0.0, 0, 0, 0);
+}
+
 template <int CH>
 __global__ __launch_bounds__(256) void mfma_peak_kernel(double* out, int iters) {
   v4d acc[CH];
@@ -168,7 +197,7 @@ __global__ __launch_bounds__(256) void i8_shape_kernel(double* out, int iters) {
 }
 
 double debug_mfma_peak(int chains, int wps, int iters) {
-  if (chains < 1 || (chains > 8 && (chains < 10 || chains > 15)) || chains == 7 || wps < 1 || wps > 8 || iters < 1)
+  if (chains < 1 || (chains > 8 && (chains < 10 || chains > 17)) || chains == 7 || wps < 1 || wps > 8 || iters < 1)
     throw status_error(GWHIP_ERR_ARG, "bad mfma probe args");
   GWHIP_CHECK(hipSetDevice(0));
   hipDeviceProp_t prop;
@@ -184,6 +213,8 @@ double debug_mfma_peak(int chains, int wps, int iters) {
       case 13: hipLaunchKernelGGL((i8_shape_kernel<1, 4>), dim3(nblocks), dim3(256), 0, nullptr, out, iters); break;
       case 14: hipLaunchKernelGGL((i8_shape_kernel<1, 8>), dim3(nblocks), dim3(256), 0, nullptr, out, iters); break;
       case 15: hipLaunchKernelGGL((i8_shape_kernel<0, 16>), dim3(nblocks), dim3(256), 0, nullptr, out, iters); break;
+      case 16: hipLaunchKernelGGL(mfma4_peak_kernel<4>, dim3(nblocks), dim3(256), 0, nullptr, out, iters); break;
+      case 17: hipLaunchKernelGGL(mfma4_peak_kernel<8>, dim3(nblocks), dim3(256), 0, nullptr, out, iters); break;
       case 5: hipLaunchKernelGGL(valu_peak_kernel<1>, dim3(nblocks), dim3(256), 0, nullptr, out, iters); break;
       case 6: hipLaunchKernelGGL(valu_peak_kernel<2>, dim3(nblocks), dim3(256), 0, nullptr, out, iters); break;
       case 1: hipLaunchKernelGGL(mfma_peak_kernel<1>, dim3(nblocks), dim3(256), 0, nullptr, out, iters); break;
@@ -209,6 +240,8 @@ double debug_mfma_peak(int chains, int wps, int iters) {
   else if (chains == 13) per_wave_iter = 4 * 32.0 * 32 * 32 * 2;
   else if (chains == 14) per_wave_iter = 8 * 32.0 * 32 * 32 * 2;
   else if (chains == 15) per_wave_iter = 16 * 16.0 * 16 * 64 * 2;
+  else if (chains == 16) per_wave_iter = 4 * 4 * 4.0 * 4 * 4 * 2;
+  else if (chains == 17) per_wave_iter = 8 * 4 * 4.0 * 4 * 4 * 2;
   else if (chains == 5) per_wave_iter = 32.0 * 64 * 2;                   // 32 FMA per lane
   else if (chains == 6) per_wave_iter = 32.0 * 64 * 2 + 4 * 2048.0;  // + 4 MFMA
   else per_wave_iter = (chains == 3 ? 4 : (chains > 4 ? 8 : chains)) * 2048.0;
@@ -291,6 +324,20 @@ int gwhip_debug_gemm_bench(int which, int NQ, int nk, int nao, int batch, int sh
 int gwhip_debug_mfma_peak(int chains, int waves_per_simd, int iters, double* tflops) {
   if (!tflops) return GWHIP_ERR_ARG;
   return guarded([&] { *tflops = gwhip::debug_mfma_peak(chains, waves_per_simd, iters); });
+}
+
+int gwhip_debug_mfma4_layout(double* out) {
+  if (!out) return GWHIP_ERR_ARG;
+  return guarded([&] {
+    GWHIP_CHECK(hipSetDevice(0));
+    double* d = nullptr;
+    GWHIP_CHECK(hipMalloc(&d, 3 * 64 * sizeof(double)));
+    hipLaunchKernelGGL(gwhip::mfma4_layout_kernel, dim3(1), dim3(64), 0, nullptr, d);
+    const hipError_t e = hipGetLastError();
+    if (e == hipSuccess) GWHIP_CHECK(hipMemcpy(out, d, 3 * 64 * sizeof(double), hipMemcpyDeviceToHost));
+    (void)hipFree(d);
+    GWHIP_CHECK(e);
+  });
 }
 
 }  // extern "C"

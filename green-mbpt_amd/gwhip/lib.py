@@ -333,6 +333,18 @@ def debug_gemm_bench(which: int, NQ: int, nk: int, nao: int, batch: int, shape: 
     return ms.value
 
 
+def debug_mfma4_layout() -> np.ndarray:
+    """[3][64] outputs of the v_mfma_f64_4x4x4_4b layout probe (test hook)."""
+    d = load_debug()
+    d.gwhip_debug_mfma4_layout.argtypes = [ctypes.POINTER(ctypes.c_double)]
+    d.gwhip_debug_mfma4_layout.restype = ctypes.c_int
+    out = np.zeros(3 * 64)
+    st = d.gwhip_debug_mfma4_layout(out.ctypes.data_as(ctypes.POINTER(ctypes.c_double)))
+    if st != OK:
+        raise GwhipError(st, "mfma4 layout probe failed")
+    return out.reshape(3, 64)
+
+
 def debug_mfma_peak(chains: int, waves_per_simd: int = 2, iters: int = 4096) -> float:
     """Register-only throughput probe (probe library): TFLOP/s or TOPS."""
     v = ctypes.c_double()

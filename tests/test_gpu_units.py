@@ -22,6 +22,35 @@ def test_zgemm_identity_probe():
                              f"{C[tuple(bad[0])]} expected {B[tuple(bad[0])]}")
 
 
+def _lanes_of(x):
+    """The powers of two in a sum of at most four equal-or-distinct powers of two, as exponents."""
+    out = []
+    for _ in range(4):
+        if x <= 0:
+            break
+        e = int(np.floor(np.log2(x)))
+        out.append(e)
+        x -= 2.0 ** e
+    assert x == 0.0
+    return out
+
+
+def test_mfma_f64_4x4x4_lane_layout():
+    """v_mfma_f64_4x4x4_4b, the instruction of the Sigma epilogue for nao <= 32: D of lane L must be
+    sum_k A[16 k + 4 b + r] B[16 k + 4 b + c] with L = 16 r + 4 b + c (A_b[r][k] at lane 16 k + 4 b + r,
+    B_b[k][c] at lane 16 k + 4 b + c, D_b[r][c] at lane 16 r + 4 b + c).  Probe inputs
+    A = 2^(l % 16), B = 2^(16 (l % 16)) make each D a sum of 2^(a + 16 b) over its four (A lane, B lane)
+    pairs; A = l / 16 + 1 (or B) with the other 1 makes D = 1 + 2 + 3 + 4 when k runs over l / 16."""
+    o = gl.debug_mfma4_layout()
+    assert np.all(o[1] == 10.0) and np.all(o[2] == 10.0)
+    for L in range(64):
+        r, blk, c = L // 16, (L // 4) % 4, L % 4
+        a_lanes = [(16 * k + 4 * blk + r) % 16 for k in range(4)]
+        b_lanes = [(16 * k + 4 * blk + c) % 16 for k in range(4)]
+        expect = sum(2.0 ** (a + 16 * b) for a, b in zip(a_lanes, b_lanes))
+        assert o[0][L] == expect, (L, _lanes_of(o[0][L]))
+
+
 @pytest.mark.parametrize("shape", [(1, 1, 1), (16, 16, 4), (64, 64, 16), (70, 130, 37), (300, 257, 41)])
 @pytest.mark.parametrize("conj", [(False, False), (True, False), (False, True)])
 @pytest.mark.parametrize("tile", [0, 1, 2])
