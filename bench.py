@@ -33,12 +33,12 @@ WORKLOAD = dict(nk=64, mesh=(4, 4, 4), nao=26, NQ=300, ns=1, beta=100.0, lam=1e4
 # the bytes the copy reads, WRITE_SIZE 1.0x the bytes it writes.
 # Source: profiles/gw_sigma_nk64_nao26_naux300_hbm_traffic.md
 PMC_PER_LAUNCH_KIB = {
-    "split": {"Sigma GEMM": (3.4374e6, 1.0188e7), "Sigma epilogue": (5.5561e6, 1.5085e5),
-              "P0 GEMM": (8.2941e6, 6.8061e5), "P0 prep": (6.9744e5, 5.3279e6),  # P0 GEMM: 16 launches per q
-              "Dyson inverse": (5.7138e7, 6.7430e7)},  # every kernel of the inverse, one evaluation
-    "fp64": {"Sigma GEMM": (2.9810e7, 1.0307e7), "Sigma epilogue": (5.5567e6, 1.5086e5),
-             "P0 GEMM": (2.2280e7, 4.2417e4), "P0 prep": (7.4678e5, 1.0163e7),
-             "Dyson inverse": (5.7137e7, 6.7429e7)},
+    "split": {"Sigma GEMM": (3.5381e6, 1.0229e7), "Sigma epilogue": (5.5368e6, 5.1620e4),
+              "P0 GEMM": (7.7133e6, 6.8067e5), "P0 prep": (6.9709e5, 5.3279e6),  # P0 GEMM: 16 launches per q
+              "Dyson inverse": (5.7157e7, 6.7429e7)},  # every kernel of the inverse, one evaluation
+    "fp64": {"Sigma GEMM": (2.9702e7, 1.0307e7), "Sigma epilogue": (5.5368e6, 5.1620e4),
+             "P0 GEMM": (2.2308e7, 4.2419e4), "P0 prep": (7.4680e5, 1.0163e7),
+             "Dyson inverse": (5.7126e7, 6.7429e7)},
 }
 PMC_CALIBRATION = (0.5, 1.0)
 PMC_SOURCE = "profiles/gw_sigma_nk64_nao26_naux300_hbm_traffic.md"
@@ -106,7 +106,7 @@ def roofline(stats, precision, w, T, slices, tuning, steps=1):
                             if split else "zgemm_f64_kernel<64,64,true,false> (R = P-tilde conj(V), FP64 MFMA)"),
              "P0 GEMM": (f"{p_kernel}<{slices}> (P0 = Z conj(V)^T, int8 slices)"
                          if split else "zgemm_f64_kernel<64,64,true,true> (P0 = Z conj(V)^T, FP64 MFMA)"),
-             "Sigma epilogue": "sigma_epi_mfma_kernel<2,double> (Sigma += (V G) R, FP64 MFMA)",
+             "Sigma epilogue": "sigma_epi4_kernel<7> (Sigma += (V G) R, FP64 4x4x4 MFMA)",
              "P0 prep": (f"z_prep_slice_kernel<2,{slices}>" if split else "z_prep_mfma_kernel<2,double>") +
                         " (Z = Gb V Gq, FP64 MFMA)",
              "Dyson inverse": "block_inverse: Hermitian block Gauss-Jordan, gj_inverse_kernel pivots + "
