@@ -5,7 +5,8 @@ on the same seeded inputs; the reference's own fixtures within its 1e-6 abs tole
 orbital-size products have two forms -- register-chained MFMA kernels (nao <= 48) and batched
 FP64 GEMMs (any nao; tuning key orbital_gemm) -- and both are checked against the oracle,
 including nao = 50 (> 48, GEMM form only), X2C, unrestricted and split precision; the register
-form at one, two and three 16-row blocks of nao (16, 17, 33, 40, 48).
+form at one, two and three 16-row blocks of nao (16, 17, 33, 40, 48), and the 4 x 4 x 4 Sigma
+epilogue (nao <= 32) at each of its block counts (6, 16, 17, 30, 32).
 """
 import numpy as np
 import pytest
@@ -191,8 +192,11 @@ def test_orbital_gemm_form(case):
 
 
 @pytest.mark.parametrize("case", [
-    dict(nk=2, mesh=(2, 1, 1), nao=16, NQ=12, ns=1),             # one 16-block, no padding
-    dict(nk=2, mesh=(2, 1, 1), nao=17, NQ=12, ns=1),             # two blocks, one row in the second
+    dict(nk=2, mesh=(2, 1, 1), nao=6, NQ=12, ns=2),              # 4x4x4 epilogue, two 4-blocks, unrestricted
+    dict(nk=2, mesh=(2, 1, 1), nao=16, NQ=12, ns=1),             # one 16-block, no padding; four 4-blocks
+    dict(nk=2, mesh=(2, 1, 1), nao=17, NQ=12, ns=1),             # two blocks, one row in the second; seven 4-blocks
+    dict(nk=2, mesh=(2, 1, 1), nao=30, NQ=10, ns=1),             # eight 4-blocks, ragged
+    dict(nk=2, mesh=(2, 1, 1), nao=32, NQ=8, ns=2),              # eight full 4-blocks, unrestricted
     dict(nk=2, mesh=(2, 1, 1), nao=33, NQ=10, ns=1),             # three blocks, ragged
     dict(nk=2, mesh=(2, 1, 1), nao=48, NQ=10, ns=1),             # three full blocks (largest register form)
     dict(nk=2, mesh=(2, 1, 1), nao=40, NQ=8, ns=2),              # three blocks, unrestricted
@@ -200,7 +204,8 @@ def test_orbital_gemm_form(case):
 ])
 def test_register_kernels_every_block_count(case):
     """The register-chained orbital kernels are instantiated for one, two and three 16-row blocks
-    of nao; each instantiation (Hermitian-G upper-block epilogue, fused P0 slicing for one spin
+    of nao, and the 4 x 4 x 4 Sigma epilogue (nao <= 32) for two, four, seven and eight 4-row
+    blocks; each instantiation (Hermitian-G upper-block epilogue, fused P0 slicing for one spin
     block, the separate slicing pass otherwise) against the oracle in both precisions."""
     nso = case.get("nso", case["nao"])
     G, V, T, d = synthetic(case["nk"], case["nao"], case["NQ"], ns=case["ns"], beta=20.0, lam=1e3,
