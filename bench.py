@@ -107,7 +107,7 @@ def roofline(stats, precision, w, T, slices, tuning, steps=1):
              "P0 GEMM": (f"{p_kernel}<{slices}> (P0 = Z conj(V)^T, int8 slices)"
                          if split else "zgemm_f64_kernel<64,64,true,true> (P0 = Z conj(V)^T, FP64 MFMA)"),
              "Sigma epilogue": "sigma_epi4_kernel<7> (Sigma += (V G) R, FP64 4x4x4 MFMA)",
-             "P0 prep": (f"z_prep_slice_kernel<2,{slices}>" if split else "z_prep_mfma_kernel<2,double>") +
+             "P0 prep": (f"z_prep_slice_kernel<2,{slices},1>" if split else "z_prep_mfma_kernel<2,double>") +
                         " (Z = Gb V Gq, FP64 MFMA)",
              "Dyson inverse": "block_inverse: Hermitian block Gauss-Jordan, gj_inverse_kernel pivots + "
                               "zgemm_f64_kernel upper-tile updates ((I - P0)^-1 per (q, Omega), FP64)"}
@@ -276,7 +276,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build measures one GPU only (asked for --gpus {args.gpus}); "
-              "sharding over q with one Sigma all-reduce is described in DESIGN.md, not run.", file=sys.stderr)
+              "sharding over q with one Sigma all-reduce is designed, not run.", file=sys.stderr)
         sys.exit(2)
 
     if args.copy_calibration:

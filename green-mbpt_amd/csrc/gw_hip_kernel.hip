@@ -437,21 +437,27 @@ void launch_z_prep(dim3 grid, size_t lds, hipStream_t st, const DevDims& dd, con
                    double2* Z, const int* kplus, int t0, int QPW) {
   launch_z_prep_t<NB, double>(grid, lds, st, dd, G, Vt, Z, kplus, t0, QPW);
 }
-template <int NB, int NS>
+template <int NB, int NS, int SB>
 void launch_z_prep_slice_t(dim3 grid, size_t lds, hipStream_t st, const DevDims& dd, const double2* G, const double2* Vt,
                            int8_t* dst, int* exps, int kc, const int* kplus, int t0, int QPW) {
-  allow_lds(z_prep_slice_kernel<NB, NS>, lds);
+  allow_lds(z_prep_slice_kernel<NB, NS, SB>, lds);
   const XcdGrid xg = make_xcd_grid(grid);
-  hipLaunchKernelGGL((z_prep_slice_kernel<NB, NS>), xcd_launch_grid(xg), dim3(256), lds, st, xg, dd, G, Vt, dst, exps,
-                     kc, kplus, t0, QPW);
+  hipLaunchKernelGGL((z_prep_slice_kernel<NB, NS, SB>), xcd_launch_grid(xg), dim3(256), lds, st, xg, dd, G, Vt, dst,
+                     exps, kc, kplus, t0, QPW);
   GWHIP_CHECK(hipGetLastError());
+}
+template <int NB, int SB>
+void launch_z_prep_slice_s(int ns, dim3 grid, size_t lds, hipStream_t st, const DevDims& dd, const double2* G,
+                           const double2* Vt, int8_t* dst, int* exps, int kc, const int* kplus, int t0, int QPW) {
+  if (ns == 5) launch_z_prep_slice_t<NB, 5, SB>(grid, lds, st, dd, G, Vt, dst, exps, kc, kplus, t0, QPW);
+  else if (ns == 4) launch_z_prep_slice_t<NB, 4, SB>(grid, lds, st, dd, G, Vt, dst, exps, kc, kplus, t0, QPW);
+  else launch_z_prep_slice_t<NB, 3, SB>(grid, lds, st, dd, G, Vt, dst, exps, kc, kplus, t0, QPW);
 }
 template <int NB>
 void launch_z_prep_slice(int ns, dim3 grid, size_t lds, hipStream_t st, const DevDims& dd, const double2* G,
                          const double2* Vt, int8_t* dst, int* exps, int kc, const int* kplus, int t0, int QPW) {
-  if (ns == 5) launch_z_prep_slice_t<NB, 5>(grid, lds, st, dd, G, Vt, dst, exps, kc, kplus, t0, QPW);
-  else if (ns == 4) launch_z_prep_slice_t<NB, 4>(grid, lds, st, dd, G, Vt, dst, exps, kc, kplus, t0, QPW);
-  else launch_z_prep_slice_t<NB, 3>(grid, lds, st, dd, G, Vt, dst, exps, kc, kplus, t0, QPW);
+  if (dd.pseudo_ns == 2) launch_z_prep_slice_s<NB, 2>(ns, grid, lds, st, dd, G, Vt, dst, exps, kc, kplus, t0, QPW);
+  else launch_z_prep_slice_s<NB, 1>(ns, grid, lds, st, dd, G, Vt, dst, exps, kc, kplus, t0, QPW);
 }
 template <int NB, typename T>
 void launch_sigma_epi_t(dim3 grid, size_t lds, hipStream_t st, const DevDims& dd, const double2* G, const double2* Vt,
@@ -495,12 +501,13 @@ void gw_hip_kernel::p0_prep(int64_t q, const double2* G, int64_t t0, int64_t tc,
   else launch_z_prep<3>(grid, lds, stream_, dd, G, Vtq, scratch_, kp, (int)t0, qpw);
 }
 
-// Split precision, one spin block: Z slices for q and tau in [t0, t0 + tc) straight into Zs_.
+// Split precision, one or two collinear spin blocks: Z slices for q and tau in [t0, t0 + tc)
+// straight into Zs_.
 void gw_hip_kernel::p0_prep_slice(int64_t q, const double2* G, int64_t t0, int64_t tc, int qpw) {
   const int64_t nk = d_.nk, NQ = d_.NQ, nao2 = d_.nao * d_.nao;
   DevDims dd{(int)d_.nts, (int)d_.ns, (int)nk, (int)d_.nso, (int)d_.nao, (int)NQ, (int)d_.pns, (int)d_.pseudo_ns};
   const int NB = (int)ceil_div(d_.nao, 16);
-  const size_t lds = (size_t)4 * (16 * NB) * (16 * NB + 1) * sizeof(double);
+  const size_t lds = (size_t)d_.pseudo_ns * 4 * (16 * NB) * (16 * NB + 1) * sizeof(double);
   const double2* Vtq = Vq_ + q * nk * NQ * nao2;
   const dim3 grid((unsigned)ceil_div(NQ, 4 * qpw), (unsigned)nk, (unsigned)tc);
   const int* kp = kplus_tab_ + q * nk;
@@ -621,7 +628,7 @@ void gw_hip_kernel::p0_one_q(int64_t q, const double2* G, bool split, double2* P
   const int64_t nk = d_.nk, NQ = d_.NQ, nao2 = d_.nao * d_.nao, NQ2 = NQ * NQ, half = d_.nts / 2;
   for (int64_t t0 = 0; t0 < half; t0 += tchunk_p0_) {
     const int64_t tc = std::min(tchunk_p0_, half - t0);
-    if (split && d_.pseudo_ns == 1 && tune_.p0_fused && !orb_gemm()) {
+    if (split && fused_p0() && tune_.p0_fused) {
       timed(K_P0_PREP, [&] { p0_prep_slice(q, G, t0, tc, tune_.qpw); });
     } else {
       timed(K_P0_PREP, [&] { p0_prep(q, G, t0, tc, tune_.qpw); });
@@ -882,23 +889,37 @@ void gw_hip_kernel::sigma_q(const double2* G, double2* Sigma, bool split, const 
       const dim3 grid((unsigned)nkb, (unsigned)qs, (unsigned)(2 * tc));
       const int up = sigma_upper() ? 1 : 0;
       const int n = (int)nkb, t = (int)t0;
-      if (d_.nao <= 32) {  // 4 x 4 x 4 MFMAs: orbital dimensions padded to 4 (i to 16)
-        const int nb4 = d_.nao <= 8 ? 2 : d_.nao <= 16 ? 4 : d_.nao <= 28 ? 7 : 8;
+      const int form = epi_form();
+      const int nb4 = d_.nao <= 8 ? 2 : d_.nao <= 16 ? 4 : d_.nao <= 28 ? 7 : d_.nao <= 32 ? 8 : d_.nao <= 40 ? 10 : 12;
+      const XcdGrid xg = make_xcd_grid(grid);
+      auto go = [&](auto kern, int threads, size_t l4) {
+        allow_lds(kern, l4);
+        hipLaunchKernelGGL(kern, xcd_launch_grid(xg), dim3(threads), l4, stream_, xg, dd, G, Vb, scratch_, Sigma, km, n,
+                           t, scale, up, 1, (int)grid.y, part);
+      };
+      if (form == 1) {  // 4 x 4 x 4 MFMAs, every i group in each wave (nao <= 32)
         const size_t l4 = sigma_epi4_lds_bytes(nb4);
-        const XcdGrid xg = make_xcd_grid(grid);
-        auto go = [&](auto kern) {
-          allow_lds(kern, l4);
-          hipLaunchKernelGGL(kern, xcd_launch_grid(xg), dim3(256), l4, stream_, xg, dd, G, Vb, scratch_, Sigma, km, n, t,
-                             scale, up, 1, (int)grid.y, part);
-        };
-        if (nb4 == 2) go(sigma_epi4_kernel<2>);
-        else if (nb4 == 4) go(sigma_epi4_kernel<4>);
-        else if (nb4 == 7) go(sigma_epi4_kernel<7>);
-        else go(sigma_epi4_kernel<8>);
-        GWHIP_CHECK(hipGetLastError());
-      } else {  // 32 < nao <= 48: 16 x 16 x 4 MFMAs
-        launch_sigma_epi<3>(grid, lds, stream_, dd, G, Vb, scratch_, Sigma, km, n, t, scale, up, 1, part);
+        if (nb4 == 2) go(sigma_epi4_kernel<2>, 256, l4);
+        else if (nb4 == 4) go(sigma_epi4_kernel<4>, 256, l4);
+        else if (nb4 == 7) go(sigma_epi4_kernel<7>, 256, l4);
+        else go(sigma_epi4_kernel<8>, 256, l4);
+      } else if (form == 2) {  // 4 x 4 x 4 MFMAs, one i group per wave
+        if (nb4 == 2) go(sigma_epi4w_kernel<2, 4, 4>, 256, sigma_epi4w_lds_bytes<2, 4>());
+        else if (nb4 == 4) go(sigma_epi4w_kernel<4, 4, 4>, 256, sigma_epi4w_lds_bytes<4, 4>());
+        else if (nb4 == 7) go(sigma_epi4w_kernel<7, 2, 4>, 256, sigma_epi4w_lds_bytes<7, 2>());
+        else if (nb4 == 8) go(sigma_epi4w_kernel<8, 2, 4>, 256, sigma_epi4w_lds_bytes<8, 2>());
+        else if (nb4 == 10) go(sigma_epi4w_kernel<10, 1, 3>, 192, sigma_epi4w_lds_bytes<10, 1>());
+        else go(sigma_epi4w_kernel<12, 1, 3>, 192, sigma_epi4w_lds_bytes<12, 1>());
+      } else {  // 16 x 16 x 4 MFMAs
+        const int nb16 = (int)ceil_div(d_.nao, 16);
+        if (nb16 == 1)
+          launch_sigma_epi<1>(grid, lds, stream_, dd, G, Vb, scratch_, Sigma, km, n, t, scale, up, 1, part);
+        else if (nb16 == 2)
+          launch_sigma_epi<2>(grid, lds, stream_, dd, G, Vb, scratch_, Sigma, km, n, t, scale, up, 1, part);
+        else
+          launch_sigma_epi<3>(grid, lds, stream_, dd, G, Vb, scratch_, Sigma, km, n, t, scale, up, 1, part);
       }
+      GWHIP_CHECK(hipGetLastError());
       if (qs > 1) {
         const int64_t per_tau = d_.ns * nkb * d_.nso * d_.nso;
         const int64_t nel = 2 * tc * per_tau;
@@ -913,6 +934,21 @@ void gw_hip_kernel::sigma_q(const double2* G, double2* Sigma, bool split, const 
 // Sigma of Hermitian G is Hermitian; with one spin block per (tau, s, k) only its upper 16 x 16
 // blocks were accumulated, so fill the rest from them.
 bool gw_hip_kernel::sigma_upper() const { return g_herm_ && d_.pns == 1 && !orb_gemm(); }
+
+// The fused P0 prep + slicing kernel takes one or two collinear spin blocks (their G planes share
+// the work-group's LDS: 150 KB at nao = 48 with two).
+bool gw_hip_kernel::fused_p0() const { return d_.pns == 1 && d_.pseudo_ns <= 2 && !orb_gemm(); }
+
+// The register Sigma epilogue for this nao: the tuned form where it applies, else by measurement
+// (one MI355X, split Sigma-evals): 4 x 4 x 4 MFMAs with all i groups per wave up to nao = 32
+// (nao = 26: 12.2 ms per launch against 13.6 for 16 x 16 x 4), one i group per wave up to 40
+// (nao = 40, ns = 2: 55.7 against 63.9 ms), and 16 x 16 x 4 above, where both pad to 48 (nao = 48:
+// 29.0 against 31.3 ms).
+int gw_hip_kernel::epi_form() const {
+  if (tune_.epi_form == 1 && d_.nao <= 32) return 1;
+  if (tune_.epi_form == 2 || tune_.epi_form == 3) return tune_.epi_form;
+  return d_.nao <= 32 ? 1 : d_.nao <= 40 ? 2 : 3;
+}
 
 int gw_hip_kernel::epi_qsplit(int64_t nks) const {
   // the full-mesh launch has nks x nts work-groups; under ~2048 (eight per CU) the Q loop is split
@@ -1147,7 +1183,7 @@ void gw_hip_kernel::autotune(const double2* G, int precision) {
     tune_.p0_shape = best_of(3, shapes, [&](int c) { p0_gemm(0, tcp, P_, c); });
     tune_.s_shape = best_of(3, shapes, [&](int c) { sigma_gemm(P_, Vq_neg(0), d_.nk, tcs, c); });
   } else {
-    if (d_.pseudo_ns == 1 && !orb_gemm()) {  // fused prep + slicing vs the two kernels
+    if (fused_p0()) {  // fused prep + slicing vs the two kernels
       const int fz[2] = {0, 1};
       tune_.p0_fused = best_of(2, fz, [&](int c) {
         if (c) {

@@ -44,23 +44,26 @@ struct Tuning {
   int p0_group = 1, s_group = 8;  // split precision: row tiles per L2 group of the int8 GEMMs
   int p0_oz_tile = 2, s_oz_tile = 2;  // int8 GEMM block tile: 2 (32 x 64, 16 x 16 x 64 MFMAs) or
                                       // 4 (64 x 64, 32 x 32 x 32 MFMAs, staged by LDS DMA)
-  int p0_fused = 1;  // split precision, one spin block: P0 orbital kernel writes int8 slices directly
+  int p0_fused = 1;  // split precision, one or two collinear spin blocks: P0 orbital kernel writes int8
+                     // slices directly
   int orb_gemm = 0;  // orbital contractions as batched FP64 GEMMs (always so for nao > 48)
   int p0_kchunks = 4;  // split precision: k chunks per launch of the int8 P0 GEMM (0: all in one), so
                        // that each launch's operand window stays in L2
-  static constexpr int count = 10;
+  int epi_form = 0;  // register Sigma epilogue: 0 by nao, 1 4x4x4 all i groups per wave (nao <= 32),
+                     // 2 4x4x4 one i group per wave, 3 16x16x4
+  static constexpr int count = 11;
   void to_array(int* v) const {
     const int a[count] = {p0_shape, s_shape, qpw, p0_group, s_group, p0_oz_tile, s_oz_tile, p0_fused, orb_gemm,
-                          p0_kchunks};
+                          p0_kchunks, epi_form};
     for (int i = 0; i < count; ++i) v[i] = a[i];
   }
   bool from_array(const int* v) {
     if (v[0] < 0 || v[0] > 2 || v[1] < 0 || v[1] > 2 || (v[2] != 1 && v[2] != 2 && v[2] != 4 && v[2] != 8) ||
         v[3] < 1 || v[4] < 1 || (v[5] != 2 && v[5] != 4) || (v[6] != 2 && v[6] != 4) || v[7] < 0 || v[7] > 1 || v[8] < 0 ||
-        v[8] > 1 || v[9] < 0)
+        v[8] > 1 || v[9] < 0 || v[10] < 0 || v[10] > 3)
       return false;
     p0_shape = v[0]; s_shape = v[1]; qpw = v[2]; p0_group = v[3]; s_group = v[4]; p0_oz_tile = v[5]; s_oz_tile = v[6];
-    p0_fused = v[7]; orb_gemm = v[8]; p0_kchunks = v[9];
+    p0_fused = v[7]; orb_gemm = v[8]; p0_kchunks = v[9]; epi_form = v[10];
     return true;
   }
 };
@@ -359,6 +362,8 @@ private:
   const int8_t* Vs_neg(int64_t q) const;
   const int* eVs_neg(int64_t q) const;
   bool sigma_upper() const;
+  int epi_form() const;
+  bool fused_p0() const;
   // Q ranges of the register-chained Sigma epilogue for nks k points: 1 when nks x nts work-groups
   // fill the chip, up to 4 otherwise; ensure_epi_part sizes epi_part_ for it
   int epi_qsplit(int64_t nks) const;

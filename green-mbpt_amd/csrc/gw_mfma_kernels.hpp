@@ -220,14 +220,15 @@ __global__ __launch_bounds__(256, NB < 3 ? 2 : 1) void z_prep_mfma_kernel(XcdGri
 }
 
 // ---------------------------------------------------------------------------------------
-// Split-precision P0 operand straight into int8 slices (one spin block, pseudo_ns == 1): the
-// wave that forms Z_Q(tau, k) = Gb V_Q Gq in registers also owns the whole slice row-chunk
+// Split-precision P0 operand straight into int8 slices (one or two spin blocks, pseudo_ns <= 2;
+// the G planes of every block sit in LDS and Z_Q = sum_s Gb_s V_Q Gq_s accumulates over s in the
+// same registers): the wave that forms Z_Q(tau, k) in registers also owns the whole slice row-chunk
 // (row (tau, Q), chunk k, nao^2 elements), so it finds the chunk exponent with a wave reduction
 // and writes the NS slices of both planes (Re, Im) itself -- the same arithmetic as
 // slice_rows_kernel, so the slices are bit-identical to slicing the FP64 Z, without Z ever reaching
 // HBM.  Slice layout: dst[((tz*NQ + Q)*nk + k)][plane][slice][kc], exps[(tz*NQ + Q)*nk + k].
 // Two waves per SIMD for NB <= 2 (three spill and run 15 % slower at nao = 26).
-template <int NB, int NS>
+template <int NB, int NS, int SB>
 __global__ __launch_bounds__(256, NB < 3 ? 2 : 1) void z_prep_slice_kernel(XcdGrid xg, DevDims d, const double2* __restrict__ G,
                                                            const double2* __restrict__ V, int8_t* __restrict__ dst,
                                                            int* __restrict__ exps, int kc,
@@ -246,9 +247,9 @@ __global__ __launch_bounds__(256, NB < 3 ? 2 : 1) void z_prep_slice_kernel(XcdGr
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int lrow = lane & 15, lk = lane >> 4;
   const int nslab = (nao + 3) / 4;
-  {
-    const double2* Gbar = G + g_index(d, tt, 0, k);
-    const double2* Gk = G + g_index(d, tau, 0, kq);
+  for (int s = 0; s < SB; ++s) {  // SB = pseudo_ns collinear spin blocks: is = s, a = b = 0
+    const double2* Gbar = G + g_index(d, tt, s, k);
+    const double2* Gk = G + g_index(d, tau, s, kq);
     for (int e = tid; e < N16 * N16; e += blockDim.x) {
       const int r = e / N16, c = e - (e / N16) * N16;
       double2 gb = make_double2(0.0, 0.0), gq = make_double2(0.0, 0.0);
@@ -256,10 +257,10 @@ __global__ __launch_bounds__(256, NB < 3 ? 2 : 1) void z_prep_slice_kernel(XcdGr
         gb = Gbar[(int64_t)r * d.nso + c];
         gq = Gk[(int64_t)r * d.nso + c];
       }
-      Gbr[r * LD + c] = gb.x;
-      Gbi[r * LD + c] = gb.y;
-      Gqr[r * LD + c] = gq.x;
-      Gqi[r * LD + c] = gq.y;
+      Gbr[s * 4 * PL + r * LD + c] = gb.x;
+      Gbi[s * 4 * PL + r * LD + c] = gb.y;
+      Gqr[s * 4 * PL + r * LD + c] = gq.x;
+      Gqi[s * 4 * PL + r * LD + c] = gq.y;
     }
   }
   __syncthreads();
@@ -276,41 +277,49 @@ __global__ __launch_bounds__(256, NB < 3 ? 2 : 1) void z_prep_slice_kernel(XcdGr
         const int m = 4 * sl + lk, p = 16 * bp + lrow;
         avs[sl][bp] = (p < nao && m < nao) ? vq[p * nao + m] : make_double2(0.0, 0.0);
       }
-    // W[:, bn] = V_Q Gq[:, bn] for every column block, then Z[:, bn] = Gb W[:, bn] for all of them
-    // at once: each Gb element read from LDS serves every column block and the second product has
-    // NB^2 independent accumulator pairs; the same sums in the same order as block by block
-    CAcc<double> w[NB][NB];  // [bn][bp]
-#pragma unroll
-    for (int bn = 0; bn < NB; ++bn) {
-#pragma unroll
-      for (int x = 0; x < NB; ++x) w[bn][x].zero();
-#pragma unroll
-      for (int sl = 0; sl < 4 * NB; ++sl) {
-        if (sl >= nslab) break;
-        const int m = 4 * sl + lk;
-        const double br = Gqr[m * LD + 16 * bn + lrow], bi = Gqi[m * LD + 16 * bn + lrow];
-#pragma unroll
-        for (int bp = 0; bp < NB; ++bp) w[bn][bp].mac(avs[sl][bp].x, avs[sl][bp].y, br, bi);
-      }
-    }
 #pragma unroll
     for (int bn = 0; bn < NB; ++bn)
 #pragma unroll
       for (int x = 0; x < NB; ++x) z[bn][x].zero();
 #pragma unroll
-    for (int bp = 0; bp < NB; ++bp)
+    for (int s = 0; s < SB; ++s) {
+      const double* gbr = Gbr + s * 4 * PL;
+      const double* gbi = Gbi + s * 4 * PL;
+      const double* gqr = Gqr + s * 4 * PL;
+      const double* gqi = Gqi + s * 4 * PL;
+      // W[:, bn] = V_Q Gq[:, bn] for every column block, then Z[:, bn] += Gb W[:, bn] for all of
+      // them at once: each Gb element read from LDS serves every column block and the second
+      // product has NB^2 independent accumulator pairs; the same sums in the same order as block
+      // by block
+      CAcc<double> w[NB][NB];  // [bn][bp]
 #pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        if (16 * bp + 4 * r >= nao) continue;  // K slab wholly in the padding
-        const int p = 16 * bp + lk + 4 * r;
+      for (int bn = 0; bn < NB; ++bn) {
 #pragma unroll
-        for (int bt = 0; bt < NB; ++bt) {
-          const int t = 16 * bt + lrow;
-          const double gr = Gbr[t * LD + p], gi = Gbi[t * LD + p];
+        for (int x = 0; x < NB; ++x) w[bn][x].zero();
 #pragma unroll
-          for (int bn = 0; bn < NB; ++bn) z[bn][bt].mac(gr, gi, w[bn][bp].re(r), w[bn][bp].im(r));
+        for (int sl = 0; sl < 4 * NB; ++sl) {
+          if (sl >= nslab) break;
+          const int m = 4 * sl + lk;
+          const double br = gqr[m * LD + 16 * bn + lrow], bi = gqi[m * LD + 16 * bn + lrow];
+#pragma unroll
+          for (int bp = 0; bp < NB; ++bp) w[bn][bp].mac(avs[sl][bp].x, avs[sl][bp].y, br, bi);
         }
       }
+#pragma unroll
+      for (int bp = 0; bp < NB; ++bp)
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+          if (16 * bp + 4 * r >= nao) continue;  // K slab wholly in the padding
+          const int p = 16 * bp + lk + 4 * r;
+#pragma unroll
+          for (int bt = 0; bt < NB; ++bt) {
+            const int t = 16 * bt + lrow;
+            const double gr = gbr[t * LD + p], gi = gbi[t * LD + p];
+#pragma unroll
+            for (int bn = 0; bn < NB; ++bn) z[bn][bt].mac(gr, gi, w[bn][bp].re(r), w[bn][bp].im(r));
+          }
+        }
+    }
     // Re and Im of the finished products
     v4d zr[NB][NB], zi[NB][NB];
 #pragma unroll
@@ -763,6 +772,149 @@ __global__ __launch_bounds__(256, NB4 <= 4 ? 3 : 2) void sigma_epi4_kernel(XcdGr
 inline size_t sigma_epi4_lds_bytes(int nb4) {
   const int n4 = 4 * nb4, ng = (nb4 + 3) / 4;
   return ((size_t)2 * n4 * (n4 + 1) + (size_t)nb4 * ng * 2 * 64) * sizeof(double);
+}
+
+// Buffer-resource word 3 on CDNA (gfx9 family): 32-bit data format, no swizzle; a raw (stride 0)
+// resource returns zero for a load at or past its byte size.
+constexpr int BUF_RSRC_WORD3 = 0x00020000;
+typedef double v2d __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ v2d buf_load_f64x2(__amdgpu_buffer_rsrc_t r, int byte_offset) {
+  return __builtin_bit_cast(v2d, __builtin_amdgcn_raw_buffer_load_b128(r, byte_offset, 0, 0));
+}
+
+// The same products with one group of 16 i per wave: a work-group of NG x QW waves, wave w owning
+// i group w % NG and every QW-th Q of the range from w / NG on.  The columns i of T^T and Sigma^T
+// are independent, so splitting them over waves costs no MFMA work; each wave holds V_Q, R_Q
+// and the Sigma^T tile of its own i group only (12 NB4 + 4 VGPRs of operands and accumulators),
+// which keeps nao up to 48 (NB4 = 12, three i groups) at three waves per SIMD where the all-groups
+// form above runs out of registers.  A wave loads only the R blocks its upper-triangle tiles
+// need.  The QW partial sums of one i group are added in the order of w / NG.
+template <int NB4, int QW, int WPE>
+__global__ __launch_bounds__(64 * ((NB4 + 3) / 4) * QW) __attribute__((amdgpu_waves_per_eu(WPE)))
+void sigma_epi4w_kernel(XcdGrid xg, DevDims d, const double2* __restrict__ G, const double2* __restrict__ V,
+                        const double2* __restrict__ R, double2* __restrict__ Sigma,
+                        const int* __restrict__ kminus, int nks, int t0, double scale, int upper, int mirror,
+                        int qsplit, double2* __restrict__ part) {
+  constexpr int NG = (NB4 + 3) / 4;
+  constexpr int N4 = 4 * NB4, LD = N4 + 1;
+  constexpr int NT = 64 * NG * QW;
+  extern __shared__ double lds_d[];
+  double* Gr = lds_d;
+  double* Gi = Gr + N4 * LD;
+  double* red = Gi + N4 * LD;  // [QW - 1][NG][NB4][2][64]
+  const int nao = d.nao, nao2 = nao * nao;
+  int bx, by, bz;
+  if (!xcd_block(xg, bx, by, bz)) return;
+  const int tz = mirror ? (bz >> 1) : bz;
+  const int k = bx, tau = mirror && (bz & 1) ? d.nts - 1 - (t0 + tz) : t0 + tz;
+  const int kmq = kminus[k];
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int ig = wave % NG, qw = wave / NG;
+  const int lk = lane >> 4, l16 = lane & 15, l4 = lane & 3;
+  const int i = 16 * ig + l16;
+  const int jmin = upper ? 4 * ig : 0;  // j blocks below it lie wholly below every i of the group
+  const int qspan = (d.NQ + qsplit - 1) / qsplit, qlo = by * qspan, qhi = min(d.NQ, qlo + qspan);
+
+  for (int s = 0; s < d.pseudo_ns; ++s) {
+    const int is = s / (d.pns * d.pns), a = (s / d.pns) % d.pns, b = s % d.pns;
+    const double2* Gk = G + g_index(d, tau, is, kmq);
+    __syncthreads();
+    for (int e = tid; e < N4 * N4; e += NT) {
+      const int r = e / N4, c = e - (e / N4) * N4;
+      double2 g = make_double2(0.0, 0.0);
+      if (r < nao && c < nao) g = Gk[(int64_t)(a * nao + r) * d.nso + b * nao + c];  // G[m = r][n = c]
+      Gr[r * LD + c] = g.x;
+      Gi[r * LD + c] = g.y;
+    }
+    __syncthreads();
+    double sr[NB4], si[NB4];  // Sigma^T[4 jb + L/16][16 ig + L%16]
+#pragma unroll
+    for (int x = 0; x < NB4; ++x) { sr[x] = 0.0; si[x] = 0.0; }
+
+    for (int Q = qlo + qw; Q < qhi; Q += QW) {
+      const double2* vq = V + ((int64_t)k * d.NQ + Q) * nao2;                // vq[i * nao + m] = V[i][m]
+      const double2* rq = R + (((int64_t)tz * nks + k) * d.NQ + Q) * nao2;  // rq[n * nao + j]
+      // V_Q and R_Q through buffer resources bounded to their nao^2 elements: a row past nao
+      // reads zero, a column past nao the next row's first elements (finite).  A padded i or j
+      // only reaches Sigma^T entries that are never stored, and a padded m or n meets the zero
+      // padding of G in the same product; the per-slab offsets are immediates.
+      const __amdgpu_buffer_rsrc_t vres = __builtin_amdgcn_make_buffer_rsrc(
+          const_cast<double2*>(vq), (short)0, nao2 * (int)sizeof(double2), BUF_RSRC_WORD3);
+      const __amdgpu_buffer_rsrc_t rres = __builtin_amdgcn_make_buffer_rsrc(
+          const_cast<double2*>(rq), (short)0, nao2 * (int)sizeof(double2), BUF_RSRC_WORD3);
+      double vr[NB4], vi[NB4];  // B of the first product: V[16 ig + L%16][4 ms + L/16]
+      const int voff = (i * nao + lk) * (int)sizeof(double2);
+#pragma unroll
+      for (int ms = 0; ms < NB4; ++ms) {
+        const v2d v = buf_load_f64x2(vres, voff + ms * 4 * (int)sizeof(double2));
+        vr[ms] = v.x;
+        vi[ms] = v.y;
+      }
+#pragma unroll 1
+      for (int nb = 0; nb < NB4; ++nb) {
+        if (4 * nb >= nao) break;  // K slab wholly in the padding
+        const int roff = ((4 * nb + lk) * nao + l4) * (int)sizeof(double2);
+        double rr[NB4], ri[NB4];  // A of the second product: R[4 nb + L/16][4 jb + L%4]
+#pragma unroll
+        for (int jb = 0; jb < NB4; ++jb) {
+          if (jb < jmin || 4 * jb >= nao) continue;
+          const v2d r = buf_load_f64x2(rres, roff + jb * 4 * (int)sizeof(double2));
+          rr[jb] = r.x;
+          ri[jb] = r.y;
+        }
+        double tr = 0.0, ti = 0.0;
+#pragma unroll
+        for (int ms = 0; ms < NB4; ++ms) {
+          if (4 * ms >= nao) break;
+          const int m = 4 * ms + lk, nn = 4 * nb + l4;  // A = G^T[n][m]: G[4 ms + L/16][4 nb + L%4]
+          cmfma4(tr, ti, Gr[m * LD + nn], Gi[m * LD + nn], vr[ms], vi[ms]);
+        }
+#pragma unroll
+        for (int jb = 0; jb < NB4; ++jb) {
+          if (jb < jmin || 4 * jb >= nao) continue;
+          cmfma4(sr[jb], si[jb], rr[jb], ri[jb], tr, ti);
+        }
+      }
+    }
+    if (QW > 1) {
+      if (qw > 0) {
+#pragma unroll
+        for (int x = 0; x < NB4; ++x) {
+          red[((((qw - 1) * NG + ig) * NB4 + x) * 2 + 0) * 64 + lane] = sr[x];
+          red[((((qw - 1) * NG + ig) * NB4 + x) * 2 + 1) * 64 + lane] = si[x];
+        }
+      }
+      __syncthreads();
+      if (qw == 0) {
+        for (int w = 1; w < QW; ++w)
+#pragma unroll
+          for (int x = 0; x < NB4; ++x) {
+            sr[x] += red[((((w - 1) * NG + ig) * NB4 + x) * 2 + 0) * 64 + lane];
+            si[x] += red[((((w - 1) * NG + ig) * NB4 + x) * 2 + 1) * 64 + lane];
+          }
+      }
+    }
+    if (qw == 0 && i < nao) {
+      const int64_t off = (((int64_t)tau * d.ns + is) * nks + k) * d.nso * d.nso;
+      double2* S = part ? part + (int64_t)by * d.nts * d.ns * nks * d.nso * d.nso + off : Sigma + off;
+#pragma unroll
+      for (int x = 0; x < NB4; ++x) {
+        const int j = 4 * x + lk;
+        if (x < jmin || j >= nao) continue;
+        double2* dst = S + (int64_t)(a * nao + i) * d.nso + b * nao + j;
+        double2 v = part ? make_double2(0.0, 0.0) : *dst;
+        v.x += scale * sr[x];
+        v.y += scale * si[x];
+        *dst = v;
+      }
+    }
+  }
+}
+
+template <int NB4, int QW>
+inline size_t sigma_epi4w_lds_bytes() {
+  constexpr int n4 = 4 * NB4, ng = (NB4 + 3) / 4;
+  return ((size_t)2 * n4 * (n4 + 1) + (size_t)(QW - 1) * ng * NB4 * 2 * 64) * sizeof(double);
 }
 
 // Sigma(tau) += sum over y < qsplit of part[y](tau), in order of y, for the 2 tc tau of one

@@ -27,7 +27,7 @@ HDF5_ROOT = os.environ.get("HDF5_ROOT", "/opt/conda")
 PREC_FP64, PREC_SPLIT, PREC_SPLIT_P0, PREC_SPLIT_SIGMA = 0, 1, 2, 3
 VJP_RECOMPUTE, VJP_STORED = 0, 1
 TUNING_KEYS = ("p0_tile", "sigma_tile", "aux_per_wave", "p0_group", "sigma_group", "p0_split_tile", "sigma_split_tile",
-               "p0_fused_slices", "orbital_gemm", "p0_kchunks")
+               "p0_fused_slices", "orbital_gemm", "p0_kchunks", "epi_form")
 REGIONS = ("P0_zgemm", "P0(t) -> P0(w)", "GW-BSE", "P(w) -> P(t)", "Selfenergy_zgemm", "total")
 KSTATS = ("P0 GEMM", "Sigma GEMM", "P0 prep", "Sigma epilogue", "Dyson inverse")
 

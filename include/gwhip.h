@@ -171,8 +171,11 @@ int gwhip_autotune(gwhip_handle* h, const void* G_dev, int precision);
  * [7] split P0 orbital kernel writes the int8 slices itself (1) or via a separate pass (0),
  * [8] orbital-size products (Z = Gb V Gq of P0, V G R of Sigma) as batched FP64 GEMMs (1) or the
  * register-chained MFMA kernels (0); always 1 in effect for nao > 48, [9] int8 P0 GEMM: k chunks
- * per launch (0: all in one launch), so that each launch's operands stay in L2. */
-#define GWHIP_TUNING_COUNT 10
+ * per launch (0: all in one launch), so that each launch's operands stay in L2, [10] register
+ * Sigma epilogue: 0 chosen by nao, 1 4x4x4 MFMAs with every 16-row group of i in each wave
+ * (nao <= 32; otherwise as 0), 2 4x4x4 MFMAs with one 16-row group per wave, 3 16x16x4 MFMAs
+ * (results agree to rounding). */
+#define GWHIP_TUNING_COUNT 11
 int gwhip_get_tuning(const gwhip_handle* h, int* values, int n);
 int gwhip_set_tuning(gwhip_handle* h, const int* values, int n);
 

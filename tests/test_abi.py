@@ -27,7 +27,7 @@ def test_debug_library_exports_every_header_symbol():
     """The probes live in their own library; libgwhip.so exports none of them."""
     dbg = gl.load_debug()
     names = gl.header_functions(gl.DEBUG_HEADER)
-    assert names == ["gwhip_debug_gemm_bench", "gwhip_debug_mfma_peak"]
+    assert names == ["gwhip_debug_gemm_bench", "gwhip_debug_mfma4_layout", "gwhip_debug_mfma_peak"]
     lib = gl.load()
     for n in names:
         assert hasattr(dbg, n), n

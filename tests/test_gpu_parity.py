@@ -225,3 +225,38 @@ def test_register_kernels_every_block_count(case):
     assert rel_err(P, P_ref) < TOL_FP64
     assert rel_err(S, S_ref) < TOL_FP64
     assert rel_err(S_split, S_ref) < 1e-8
+
+
+@pytest.mark.parametrize("case", [
+    dict(nk=2, mesh=(2, 1, 1), nao=6, NQ=12, ns=2),              # one i group, unrestricted
+    dict(nk=2, mesh=(2, 1, 1), nao=17, NQ=12, ns=1),             # two i groups, one row in the second
+    dict(nk=2, mesh=(2, 1, 1), nao=30, NQ=10, ns=1),             # two i groups, ragged 4-blocks
+    dict(nk=2, mesh=(2, 1, 1), nao=37, NQ=8, ns=2),              # three i groups (ten 4-blocks), unrestricted
+    dict(nk=2, mesh=(2, 1, 1), nao=45, NQ=8, ns=1),              # twelve 4-blocks, ragged
+    dict(nk=1, mesh=(1, 1, 1), nao=40, NQ=6, ns=1, nso=80),      # X2C spin blocks
+])
+def test_sigma_epilogue_forms(case):
+    """The three register forms of the Sigma epilogue (4x4x4 MFMAs with every 16-row group of i
+    per wave, 4x4x4 with one group per wave, 16x16x4) against each other and against the oracle;
+    the first form applies up to nao = 32 only and stands in for the default above it."""
+    nso = case.get("nso", case["nao"])
+    G, V, T, d = synthetic(case["nk"], case["nao"], case["NQ"], ns=case["ns"], beta=20.0, lam=1e3,
+                           mesh=case["mesh"], seed=23, nso=nso, sigma=0.3)
+    k = _kernel(d, T)
+    S = {}
+    try:
+        k.set_grid(T.T_bt, T.T_tb)
+        k.upload_V(V)
+        for form in (1, 2, 3):
+            k.set_tuning(epi_form=form)
+            S[form] = k.solve(G)
+        S_split = k.solve(G, gl.PREC_SPLIT)
+    finally:
+        k.close()
+    S_ref, _ = _oracle(G, V, T, d)
+    print(f"epilogue forms {case}: " + ", ".join(f"{f}: {rel_err(S[f], S_ref):.2e}" for f in S) +
+          f", 2 vs 3 {rel_err(S[2], S[3]):.2e}, split (form 3) {rel_err(S_split, S_ref):.2e}", flush=True)
+    for f in S:
+        assert rel_err(S[f], S_ref) < TOL_FP64
+    assert rel_err(S[2], S[3]) < 1e-12 and rel_err(S[1], S[2]) < 1e-12
+    assert rel_err(S_split, S_ref) < 1e-8
