@@ -856,10 +856,6 @@ const int* gw_hip_kernel::eVs_neg(int64_t q) const {
 void gw_hip_kernel::sigma_q(const double2* G, double2* Sigma, bool split, const double2* Pq, const double2* Vb,
                             const int8_t* Vsb, const int* eVsb, const int* km, int64_t nkb) {
   const int64_t nk = d_.nk, NQ = d_.NQ, NQ2 = NQ * NQ, nts = d_.nts;
-  DevDims dd{(int)nts, (int)d_.ns, (int)nk, (int)d_.nso, (int)d_.nao, (int)NQ, (int)d_.pns, (int)d_.pseudo_ns};
-  const int NB = (int)ceil_div(d_.nao, 16);
-  const size_t lds = ((size_t)2 * (16 * NB) * (16 * NB + 1) + (size_t)NB * NB * 8 * 64) * sizeof(double);
-  const double scale = -1.0 / (double)nk;
   if (nkb < 1 || nkb > nk) throw status_error(GWHIP_ERR_ARG, "Sigma stage k count out of range");
 
   // P0(beta - tau) = P0(tau) makes P-tilde(beta - tau) = P-tilde(tau) (to the rounding of the
@@ -874,60 +870,65 @@ void gw_hip_kernel::sigma_q(const double2* G, double2* Sigma, bool split, const 
     } else {
       timed(K_SIGMA_GEMM, [&] { sigma_gemm(Pq + t0 * NQ2, Vb, nkb, tc, tune_.s_shape); });
     }
-    timed(K_SIGMA_EPI, [&] {
-      if (orb_gemm()) {
-        sigma_epi_gemm(G, Sigma, Vb, km, nkb, t0, tc, scale);
-        return;
-      }
-      const int qs = epi_qsplit(nkb);
-      double2* part = nullptr;
-      if (qs > 1) {
-        ensure_epi_part(nkb);
-        part = epi_part_;
-        zero(part, (int64_t)qs * nts * d_.ns * nkb * d_.nso * d_.nso);
-      }
-      const dim3 grid((unsigned)nkb, (unsigned)qs, (unsigned)(2 * tc));
-      const int up = sigma_upper() ? 1 : 0;
-      const int n = (int)nkb, t = (int)t0;
-      const int form = epi_form();
-      const int nb4 = d_.nao <= 8 ? 2 : d_.nao <= 16 ? 4 : d_.nao <= 28 ? 7 : d_.nao <= 32 ? 8 : d_.nao <= 40 ? 10 : 12;
-      const XcdGrid xg = make_xcd_grid(grid);
-      auto go = [&](auto kern, int threads, size_t l4) {
-        allow_lds(kern, l4);
-        hipLaunchKernelGGL(kern, xcd_launch_grid(xg), dim3(threads), l4, stream_, xg, dd, G, Vb, scratch_, Sigma, km, n,
-                           t, scale, up, 1, (int)grid.y, part);
-      };
-      if (form == 1) {  // 4 x 4 x 4 MFMAs, every i group in each wave (nao <= 32)
-        const size_t l4 = sigma_epi4_lds_bytes(nb4);
-        if (nb4 == 2) go(sigma_epi4_kernel<2>, 256, l4);
-        else if (nb4 == 4) go(sigma_epi4_kernel<4>, 256, l4);
-        else if (nb4 == 7) go(sigma_epi4_kernel<7>, 256, l4);
-        else go(sigma_epi4_kernel<8>, 256, l4);
-      } else if (form == 2) {  // 4 x 4 x 4 MFMAs, one i group per wave
-        if (nb4 == 2) go(sigma_epi4w_kernel<2, 4, 4>, 256, sigma_epi4w_lds_bytes<2, 4>());
-        else if (nb4 == 4) go(sigma_epi4w_kernel<4, 4, 4>, 256, sigma_epi4w_lds_bytes<4, 4>());
-        else if (nb4 == 7) go(sigma_epi4w_kernel<7, 2, 4>, 256, sigma_epi4w_lds_bytes<7, 2>());
-        else if (nb4 == 8) go(sigma_epi4w_kernel<8, 2, 4>, 256, sigma_epi4w_lds_bytes<8, 2>());
-        else if (nb4 == 10) go(sigma_epi4w_kernel<10, 1, 3>, 192, sigma_epi4w_lds_bytes<10, 1>());
-        else go(sigma_epi4w_kernel<12, 1, 3>, 192, sigma_epi4w_lds_bytes<12, 1>());
-      } else {  // 16 x 16 x 4 MFMAs
-        const int nb16 = (int)ceil_div(d_.nao, 16);
-        if (nb16 == 1)
-          launch_sigma_epi<1>(grid, lds, stream_, dd, G, Vb, scratch_, Sigma, km, n, t, scale, up, 1, part);
-        else if (nb16 == 2)
-          launch_sigma_epi<2>(grid, lds, stream_, dd, G, Vb, scratch_, Sigma, km, n, t, scale, up, 1, part);
-        else
-          launch_sigma_epi<3>(grid, lds, stream_, dd, G, Vb, scratch_, Sigma, km, n, t, scale, up, 1, part);
-      }
-      GWHIP_CHECK(hipGetLastError());
-      if (qs > 1) {
-        const int64_t per_tau = d_.ns * nkb * d_.nso * d_.nso;
-        const int64_t nel = 2 * tc * per_tau;
-        hipLaunchKernelGGL(epi_part_reduce_kernel, dim3((unsigned)std::min<int64_t>(4096, ceil_div(nel, 256))), dim3(256),
-                           0, stream_, part, Sigma, qs, nts * per_tau, per_tau, t, (int)tc, (int)nts);
-        GWHIP_CHECK(hipGetLastError());
-      }
-    });
+    timed(K_SIGMA_EPI, [&] { sigma_epi(G, Sigma, Vb, km, nkb, t0, tc, epi_form()); });
+  }
+}
+
+// Sigma += -1/nk (V G) R for the 2 tc tau of the half-grid chunk at t0 (R in scratch_) with the
+// register epilogue `form` (epi_form()), or the GEMM form of the orbital products.
+void gw_hip_kernel::sigma_epi(const double2* G, double2* Sigma, const double2* Vb, const int* km, int64_t nkb,
+                              int64_t t0, int64_t tc, int form) {
+  const int64_t nk = d_.nk, NQ = d_.NQ, nts = d_.nts;
+  const double scale = -1.0 / (double)nk;
+  if (orb_gemm()) {
+    sigma_epi_gemm(G, Sigma, Vb, km, nkb, t0, tc, scale);
+    return;
+  }
+  DevDims dd{(int)nts, (int)d_.ns, (int)nk, (int)d_.nso, (int)d_.nao, (int)NQ, (int)d_.pns, (int)d_.pseudo_ns};
+  const int qs = epi_qsplit(nkb);
+  double2* part = nullptr;
+  if (qs > 1) {
+    ensure_epi_part(nkb);
+    part = epi_part_;
+    zero(part, (int64_t)qs * nts * d_.ns * nkb * d_.nso * d_.nso);
+  }
+  const dim3 grid((unsigned)nkb, (unsigned)qs, (unsigned)(2 * tc));
+  const int up = sigma_upper() ? 1 : 0;
+  const int n = (int)nkb, t = (int)t0;
+  const int nb4 = d_.nao <= 8 ? 2 : d_.nao <= 16 ? 4 : d_.nao <= 28 ? 7 : d_.nao <= 32 ? 8 : d_.nao <= 40 ? 10 : 12;
+  const XcdGrid xg = make_xcd_grid(grid);
+  auto go = [&](auto kern, int threads, size_t l4) {
+    allow_lds(kern, l4);
+    hipLaunchKernelGGL(kern, xcd_launch_grid(xg), dim3(threads), l4, stream_, xg, dd, G, Vb, scratch_, Sigma, km, n, t,
+                       scale, up, 1, (int)grid.y, part);
+  };
+  if (form == 1 && d_.nao <= 32) {  // 4 x 4 x 4 MFMAs, every i group in each wave
+    const size_t l4 = sigma_epi4_lds_bytes(nb4);
+    if (nb4 == 2) go(sigma_epi4_kernel<2>, 256, l4);
+    else if (nb4 == 4) go(sigma_epi4_kernel<4>, 256, l4);
+    else if (nb4 == 7) go(sigma_epi4_kernel<7>, 256, l4);
+    else go(sigma_epi4_kernel<8>, 256, l4);
+  } else if (form == 2) {  // 4 x 4 x 4 MFMAs, one i group per wave
+    if (nb4 == 2) go(sigma_epi4w_kernel<2, 4, 4>, 256, sigma_epi4w_lds_bytes<2, 4>());
+    else if (nb4 == 4) go(sigma_epi4w_kernel<4, 4, 4>, 256, sigma_epi4w_lds_bytes<4, 4>());
+    else if (nb4 == 7) go(sigma_epi4w_kernel<7, 2, 4>, 256, sigma_epi4w_lds_bytes<7, 2>());
+    else if (nb4 == 8) go(sigma_epi4w_kernel<8, 2, 4>, 256, sigma_epi4w_lds_bytes<8, 2>());
+    else if (nb4 == 10) go(sigma_epi4w_kernel<10, 1, 3>, 192, sigma_epi4w_lds_bytes<10, 1>());
+    else go(sigma_epi4w_kernel<12, 1, 3>, 192, sigma_epi4w_lds_bytes<12, 1>());
+  } else {  // 16 x 16 x 4 MFMAs
+    const int NB = (int)ceil_div(d_.nao, 16);
+    const size_t lds = ((size_t)2 * (16 * NB) * (16 * NB + 1) + (size_t)NB * NB * 8 * 64) * sizeof(double);
+    if (NB == 1) launch_sigma_epi<1>(grid, lds, stream_, dd, G, Vb, scratch_, Sigma, km, n, t, scale, up, 1, part);
+    else if (NB == 2) launch_sigma_epi<2>(grid, lds, stream_, dd, G, Vb, scratch_, Sigma, km, n, t, scale, up, 1, part);
+    else launch_sigma_epi<3>(grid, lds, stream_, dd, G, Vb, scratch_, Sigma, km, n, t, scale, up, 1, part);
+  }
+  GWHIP_CHECK(hipGetLastError());
+  if (qs > 1) {
+    const int64_t per_tau = d_.ns * nkb * d_.nso * d_.nso;
+    const int64_t nel = 2 * tc * per_tau;
+    hipLaunchKernelGGL(epi_part_reduce_kernel, dim3((unsigned)std::min<int64_t>(4096, ceil_div(nel, 256))), dim3(256), 0,
+                       stream_, part, Sigma, qs, nts * per_tau, per_tau, t, (int)tc, (int)nts);
+    GWHIP_CHECK(hipGetLastError());
   }
 }
 
@@ -1212,6 +1213,25 @@ void gw_hip_kernel::autotune(const double2* G, int precision) {
       sigma_gemm_split(Vs_neg(0), eVs_neg(0), d_.nk, tcs, tune_.s_group);
     });
     tune_.s_group = best_of(7, groups, [&](int c) { sigma_gemm_split(Vs_neg(0), eVs_neg(0), d_.nk, tcs, c); });
+  }
+  // register Sigma epilogue form, on the R of q = 0 the Sigma GEMMs above left in scratch_ (Sigma
+  // into a temporary buffer)
+  if (!orb_gemm()) {
+    const size_t nsig = (size_t)d_.nts * d_.ns * d_.nk * d_.nso * d_.nso;
+    double2* S = dalloc<double2>(nsig);
+    const int forms[3] = {1, 2, 3};
+    const int first = d_.nao <= 32 ? 0 : 1;  // the all-groups form covers nao <= 32
+    try {
+      tune_.epi_form = best_of(3 - first, forms + first,
+                               [&](int c) { sigma_epi(G, S, Vq_neg(0), kminus_tab_, d_.nk, 0, tcs, c); });
+    } catch (...) {
+      (void)hipFree(S);
+      bytes_ -= nsig * sizeof(double2);
+      throw;
+    }
+    GWHIP_CHECK(hipStreamSynchronize(stream_));
+    (void)hipFree(S);
+    bytes_ -= nsig * sizeof(double2);
   }
   stored_valid_ = false;
   ++generation_;

@@ -356,6 +356,8 @@ private:
   void check_dyson_flag();  // reads dyson_flag_ (synchronous) and throws when it is set
   void stage_dyson(int64_t nq, const int* q0_row);
   void stage_sigma(const double2* G, double2* Sigma, bool split);
+  void sigma_epi(const double2* G, double2* Sigma, const double2* Vb, const int* km, int64_t nkb, int64_t t0,
+                 int64_t tc, int form);
   void sigma_q(const double2* G, double2* Sigma, bool split, const double2* Pq, const double2* Vb, const int8_t* Vsb,
                const int* eVsb, const int* km, int64_t nkb);
   const double2* Vq_neg(int64_t q) const;  // V(k, k - q) for every k: the k + q' rows of q' = -q

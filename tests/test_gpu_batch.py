@@ -111,6 +111,7 @@ def test_recapture_rules_and_autotune():
         for s in b.systems:
             t = s.tuning()
             assert t["p0_tile"] in (0, 1, 2) and t["sigma_tile"] in (0, 1, 2) and t["aux_per_wave"] in (1, 2, 4, 8)
+            assert t["epi_form"] in (1, 2, 3)
         b.autotune(gp, gl.PREC_SPLIT)
         for s in b.systems:
             t = s.tuning()
