@@ -159,9 +159,10 @@ int gwhip_eval_sigma_device_noalloc(gwhip_handle* h, const void* G_dev, void* Si
  * and keep the fastest: the FP64 GEMM tiles of the P0 and Sigma contractions (0: 64x64,
  * 1: 128x64, 2: 64x128), the aux indices per wave of the P0 orbital kernel (1, 2, 4, 8) and, for
  * the split-precision modes, the block tile of each int8 GEMM, the number of row tiles that sweep
- * the column tiles together in the int8 GEMMs' L2-aware tile order (1 .. 64) and the k chunks per
- * launch of the int8 P0 GEMM.
- * Results do not change, only speed.
+ * the column tiles together in the int8 GEMMs' L2-aware tile order (1 .. 64), the k chunks per
+ * launch of the int8 P0 GEMM and the fused or separate slicing of the P0 operand; in both
+ * precisions the form of the register Sigma epilogue.
+ * Results change only by rounding (the epilogue forms and the k chunks add in another order).
  * No reference counterpart. */
 int gwhip_autotune(gwhip_handle* h, const void* G_dev, int precision);
 /* The tuning as an array of GWHIP_TUNING_COUNT ints: [0] FP64 P0 GEMM tile, [1] FP64 Sigma GEMM
