@@ -651,24 +651,6 @@ void gw_hip_kernel::p0_one_q(int64_t q, const double2* G, bool split, double2* P
   GWHIP_CHECK(hipGetLastError());
 }
 
-// In-place inverse of nmat Hermitian positive definite NQ x NQ matrices (I - P0(i Omega)).
-void gw_hip_kernel::gj_inverse(double2* X, int64_t nmat) {
-  const int64_t NQ = d_.NQ, NQ2 = NQ * NQ;
-  // the column panel [NQ][nb] sits in LDS: narrower panels for larger NQ (nb = 1 up to NQ ~ 10^4)
-  int nb = GJ_NB;
-  size_t gj_lds = (size_t)(GJ_NB * GJ_NB + NQ * nb) * sizeof(double2);
-  while ((int64_t)gj_lds > max_lds_ && nb > 1) {
-    nb /= 2;
-    gj_lds = (size_t)(GJ_NB * GJ_NB + NQ * nb) * sizeof(double2);
-  }
-  if ((int64_t)gj_lds > max_lds_) throw status_error(GWHIP_ERR_UNSUPPORTED, "NQ too large for the Dyson kernel");
-  allow_lds(gj_inverse_kernel, gj_lds);
-  const int gj_threads = (int)std::min<int64_t>(1024, std::max<int64_t>(256, ceil_div(NQ, 64) * 64));
-  hipLaunchKernelGGL(gj_inverse_kernel, dim3((unsigned)nmat), dim3(gj_threads), gj_lds, stream_, X, (int)NQ, NQ2, nb,
-                     dyson_flag_);
-  GWHIP_CHECK(hipGetLastError());
-}
-
 bool gw_hip_kernel::dyson_failed() const {
   int f = 0;
   GWHIP_CHECK(hipMemcpy(&f, dyson_flag_, sizeof(int), hipMemcpyDeviceToHost));

@@ -349,7 +349,6 @@ private:
   void sigma_gemm_split(const int8_t* Vsb, const int* eVsb, int64_t nkb, int64_t tc, int group);
   void stage_p0(const double2* G, bool split, const std::vector<int>* qs = nullptr);
   void p0_one_q(int64_t q, const double2* G, bool split, double2* Pq);
-  void gj_inverse(double2* X, int64_t nmat);
   // in-place inverse of nmat Hermitian positive-definite NQ x NQ matrices by block Gauss-Jordan steps
   // whose updates are batched FP64 MFMA GEMMs (panels in scratch_)
   void block_inverse(double2* X, int64_t nmat);

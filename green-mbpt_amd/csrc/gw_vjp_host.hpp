@@ -274,10 +274,13 @@ void gw_hip_kernel::vjp_device(const double2* G, const double2* Sbar, double2* G
     hipLaunchKernelGGL(identity_minus_kernel, dim3((unsigned)ceil_div(nwb * NQ2, 256)), dim3(256), 0, stream_, vBinv_,
                        vP0w_, (int)NQ, nwb);
     GWHIP_CHECK(hipGetLastError());
-    gj_inverse(vBinv_, nwb);
-    GWHIP_CHECK(hipMemcpyAsync(vAinv_, vBinv_, nwb * NQ2 * sizeof(double2), hipMemcpyDeviceToDevice, stream_));
-    hermitize(vAinv_, (int)NQ, NQ2, nwb, stream_);
+    // B = I - P0(i Omega) is Hermitian (P0(tau) is Hermitised and the folded transform is real on
+    // it), so B^-1 comes from the Hermitian block sweep of the forward's Dyson step, its upper
+    // triangle mirrored: the same B^-1 the forward used, Hermitian by construction (A^-1 = Herm(B^-1))
+    block_inverse(vBinv_, nwb);
+    mirror_upper(vBinv_, (int)NQ, NQ2, nwb, stream_);
     GWHIP_CHECK(hipGetLastError());
+    GWHIP_CHECK(hipMemcpyAsync(vAinv_, vBinv_, nwb * NQ2 * sizeof(double2), hipMemcpyDeviceToDevice, stream_));
     const double2* Pt = P_ + q * nts * NQ2;
     if (!stored) {
       mat_gemm(stream_, (int)NQ, nwb, vAinv_, false, false, vP0w_, false, false, vM1_, 1.0, false);
