@@ -44,6 +44,11 @@ PMC_CALIBRATION = (0.5, 1.0)
 PMC_SOURCE = "profiles/gw_sigma_nk64_nao26_naux300_hbm_traffic.md"
 # the int8 GEMM kernel of each block tile (tuning keys p0_split_tile / sigma_split_tile)
 OZAKI_KERNEL = {2: "zgemm_ozaki_pipe_kernel", 4: "zgemm_ozaki32_kernel"}
+# the register Sigma epilogue of each form (tuning key epi_form) at nao = 26; the profiled traffic
+# figures belong to form 1
+EPI_KERNEL = {1: "sigma_epi4_kernel<7> (Sigma += (V G) R, FP64 4x4x4 MFMA)",
+              2: "sigma_epi4w_kernel<7,2,4> (Sigma += (V G) R, FP64 4x4x4 MFMA, one i group per wave)",
+              3: "sigma_epi_mfma_kernel<2,double> (Sigma += (V G) R, FP64 16x16x4 MFMA)"}
 
 
 def algorithmic_flops(nk, nao, NQ, nts, nw_b, pseudo_ns=1):
@@ -106,7 +111,7 @@ def roofline(stats, precision, w, T, slices, tuning, steps=1):
                             if split else "zgemm_f64_kernel<64,64,true,false> (R = P-tilde conj(V), FP64 MFMA)"),
              "P0 GEMM": (f"{p_kernel}<{slices}> (P0 = Z conj(V)^T, int8 slices)"
                          if split else "zgemm_f64_kernel<64,64,true,true> (P0 = Z conj(V)^T, FP64 MFMA)"),
-             "Sigma epilogue": "sigma_epi4_kernel<7> (Sigma += (V G) R, FP64 4x4x4 MFMA)",
+             "Sigma epilogue": EPI_KERNEL.get(tuning.get("epi_form", 1) or 1, "?"),
              "P0 prep": (f"z_prep_slice_kernel<2,{slices},1>" if split else "z_prep_mfma_kernel<2,double>") +
                         " (Z = Gb V Gq, FP64 MFMA)",
              "Dyson inverse": "block_inverse: Hermitian block Gauss-Jordan, gj_inverse_kernel pivots + "
@@ -130,6 +135,8 @@ def roofline(stats, precision, w, T, slices, tuning, steps=1):
             r["issued_tops"] = wk["issued"] * per / t / 1e12
             r["issued_frac"] = r["issued_tops"] / peak
         pmc = PMC_PER_LAUNCH_KIB.get(precision, {}).get(name)
+        if name == "Sigma epilogue" and (tuning.get("epi_form", 1) or 1) != 1:
+            pmc = None  # profiled for the 4x4x4 all-groups form only
         if pmc is not None:
             tr = 1024.0 * (pmc[0] / PMC_CALIBRATION[0] + pmc[1] / PMC_CALIBRATION[1])
             r.update({"traffic": tr, "traffic_ratio": tr / alg_bytes, "traffic_source": PMC_SOURCE})

@@ -56,3 +56,12 @@ def test_roofline_fraction_is_algorithmic_work_over_peak():
     assert e["algorithmic_flops_per_launch"] == pytest.approx(16.0 * w["NQ"] * w["nao"] ** 3 * w["nk"] * T.nts)
     assert e["frac"] == pytest.approx(e["achieved"] / b.PEAK_FP64_TFLOPS)
     assert r["traffic"] == pytest.approx(e["traffic"]) and r["traffic"] > 0
+
+
+def test_epilogue_label_and_traffic_follow_the_tuned_form():
+    b = _bench()
+    r1 = b.roofline(_stats(1), "split", b.WORKLOAD, T, 4, dict(TUNING, epi_form=0), 1)["all_stages"]["Sigma epilogue"]
+    r3 = b.roofline(_stats(1), "split", b.WORKLOAD, T, 4, dict(TUNING, epi_form=3), 1)["all_stages"]["Sigma epilogue"]
+    assert r1["kernel"].startswith("sigma_epi4_kernel<7>") and r1["traffic"] > 0
+    assert r3["kernel"].startswith("sigma_epi_mfma_kernel<2,double>") and r3["traffic"] is None
+    assert r1["achieved"] == pytest.approx(r3["achieved"])
