@@ -207,7 +207,8 @@ __global__ void hgj_rows_kernel(double2* __restrict__ A, int N, int64_t sA, int 
 // Without pivoting the k-th pivot is the k-th leading Schur complement's corner, real and positive
 // exactly when the matrix is Hermitian positive definite; a pivot whose real part is not positive
 // (or not finite) sets *status to 1, so that the caller reports the matrix instead of returning a
-// silently degraded inverse.
+// silently degraded inverse.  The Dyson step (forward and VJP) runs it on the 64 x 64 pivot blocks
+// of the Hermitian block sweep above.
 constexpr int GJ_NB = 16;
 
 __global__ void gj_inverse_kernel(double2* A, int N, int64_t stride, int nb, int* status) {
