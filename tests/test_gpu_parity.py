@@ -260,3 +260,19 @@ def test_sigma_epilogue_forms(case):
         assert rel_err(S[f], S_ref) < TOL_FP64
     assert rel_err(S[2], S[3]) < 1e-12 and rel_err(S[1], S[2]) < 1e-12
     assert rel_err(S_split, S_ref) < 1e-8
+
+
+def test_out_of_range_tuning_is_refused():
+    """Tuning values outside each key's range (epilogue form 4, int8 tile 3, aux per wave 3) are
+    refused with GWHIP_ERR_ARG and leave the handle's tuning as it was."""
+    G, V, T, d = synthetic(1, 2, 3, beta=20.0, lam=1e3, mesh=(1, 1, 1), seed=1)
+    k = _kernel(d, T)
+    try:
+        before = k.tuning()
+        for bad in (dict(epi_form=4), dict(sigma_split_tile=3), dict(aux_per_wave=3)):
+            with pytest.raises(gl.GwhipError) as e:
+                k.set_tuning(**bad)
+            assert e.value.status == gl.ERR_ARG
+        assert k.tuning() == before
+    finally:
+        k.close()
