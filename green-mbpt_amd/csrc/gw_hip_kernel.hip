@@ -1197,10 +1197,18 @@ void gw_hip_kernel::autotune(const double2* G, int precision) {
     tune_.s_group = best_of(7, groups, [&](int c) { sigma_gemm_split(Vs_neg(0), eVs_neg(0), d_.nk, tcs, c); });
   }
   // register Sigma epilogue form, on the R of q = 0 the Sigma GEMMs above left in scratch_ (Sigma
-  // into a temporary buffer)
+  // into a temporary buffer; where device memory cannot hold one, the form stays chosen by nao)
+  double2* S = nullptr;
+  const size_t nsig = (size_t)d_.nts * d_.ns * d_.nk * d_.nso * d_.nso;
   if (!orb_gemm()) {
-    const size_t nsig = (size_t)d_.nts * d_.ns * d_.nk * d_.nso * d_.nso;
-    double2* S = dalloc<double2>(nsig);
+    try {
+      S = dalloc<double2>(nsig);
+    } catch (const status_error& e) {
+      if (e.status != GWHIP_ERR_OOM) throw;
+      tune_.epi_form = 0;
+    }
+  }
+  if (S) {
     const int forms[3] = {1, 2, 3};
     const int first = d_.nao <= 32 ? 0 : 1;  // the all-groups form covers nao <= 32
     try {
