@@ -298,11 +298,13 @@ __global__ __launch_bounds__(256, NB < 3 ? 2 : 1) void z_prep_slice_kernel(XcdGr
         for (int x = 0; x < NB; ++x) w[bn][x].zero();
 #pragma unroll
         for (int sl = 0; sl < 4 * NB; ++sl) {
-          if (sl >= nslab) break;
-          const int m = 4 * sl + lk;
-          const double br = gqr[m * LD + 16 * bn + lrow], bi = gqi[m * LD + 16 * bn + lrow];
+          if (NB < 3 && sl >= nslab) break;
+          if (sl < nslab) {  // NB = 3: a guard, not a break, so that the loop unrolls and avs stays in registers
+            const int m = 4 * sl + lk;
+            const double br = gqr[m * LD + 16 * bn + lrow], bi = gqi[m * LD + 16 * bn + lrow];
 #pragma unroll
-          for (int bp = 0; bp < NB; ++bp) w[bn][bp].mac(avs[sl][bp].x, avs[sl][bp].y, br, bi);
+            for (int bp = 0; bp < NB; ++bp) w[bn][bp].mac(avs[sl][bp].x, avs[sl][bp].y, br, bi);
+          }
         }
       }
 #pragma unroll
