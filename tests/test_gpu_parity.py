@@ -234,11 +234,15 @@ def test_register_kernels_every_block_count(case):
     dict(nk=2, mesh=(2, 1, 1), nao=37, NQ=8, ns=2),              # three i groups (ten 4-blocks), unrestricted
     dict(nk=2, mesh=(2, 1, 1), nao=45, NQ=8, ns=1),              # twelve 4-blocks, ragged
     dict(nk=1, mesh=(1, 1, 1), nao=40, NQ=6, ns=1, nso=80),      # X2C spin blocks
+    dict(nk=2, mesh=(2, 1, 1), nao=37, NQ=48, ns=1),             # few blocks: the Q range split in three
+    dict(nk=2, mesh=(2, 1, 1), nao=20, NQ=40, ns=2),             # ... in two, unrestricted
 ])
 def test_sigma_epilogue_forms(case):
     """The three register forms of the Sigma epilogue (4x4x4 MFMAs with every 16-row group of i
     per wave, 4x4x4 with one group per wave, 16x16x4) against each other and against the oracle;
-    the first form applies up to nao = 32 only and stands in for the default above it."""
+    the first form applies up to nao = 32 only and stands in for the default above it.  With few
+    (k, tau) blocks and NQ >= 32 the Q loop is split over work-groups whose partial sums are added
+    in order (the last two cases)."""
     nso = case.get("nso", case["nao"])
     G, V, T, d = synthetic(case["nk"], case["nao"], case["NQ"], ns=case["ns"], beta=20.0, lam=1e3,
                            mesh=case["mesh"], seed=23, nso=nso, sigma=0.3)
