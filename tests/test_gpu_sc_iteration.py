@@ -7,7 +7,8 @@ ns = 1, beta = 20, IR Lambda = 1e3, S = I, half filling).  Each device step is c
 oracle on the device's own input: Sigma against oracle.gw_oracle.gw_sigma within 1e-10, the chemical
 potential against oracle.dyson.find_mu on the same Sigma within 1e-8 (both bisect to a relative
 electron-count tolerance of 1e-9), G against oracle.dyson.compute_G at the device's mu within 1e-10.
-The first G is the Dyson G of H alone (Sigma1 = Sigma(tau) = 0) at its own mu.
+The first G is the Dyson G of H alone (Sigma1 = Sigma(tau) = 0) at its own mu.  The same loop with
+the GW step in split precision keeps Sigma within 1e-8 of the oracle's FP64 Sigma of the same G.
 """
 import numpy as np
 import pytest
@@ -34,7 +35,8 @@ def _system(T):
     return H, S, V
 
 
-def test_two_sc_iterations_on_device():
+@pytest.mark.parametrize("precision", [gl.PREC_FP64, gl.PREC_SPLIT])
+def test_two_sc_iterations_on_device(precision):
     T = transformer(LAM, BETA)
     wn = (2 * T.n_fermi + 1) * np.pi / BETA
     H, S, V = _system(T)
@@ -61,10 +63,10 @@ def test_two_sc_iterations_on_device():
 
         G, mu = step_mu_G(np.zeros((T.nts, NS, NK, NAO, NAO), np.complex128), 0.0)
         for it in range(2):
-            Sigma = k.solve(G)
+            Sigma = k.solve(G, precision)
             Sigma_or = go.gw_sigma(G, V, T.T_bt, T.T_tb, NK, NAO, NAO, NS, NQ, mesh=(2, 1, 1))
             print(f"iteration {it}: Sigma {rel_err(Sigma, Sigma_or):.2e} vs oracle", flush=True)
-            assert rel_err(Sigma, Sigma_or) < 1e-10
+            assert rel_err(Sigma, Sigma_or) < (1e-10 if precision == gl.PREC_FP64 else 1e-8)
             G_new, mu = step_mu_G(Sigma, mu)
             assert rel_err(G_new, G) > 1e-6  # the self-energy moved G
             G = G_new
